@@ -1,0 +1,313 @@
+"""MI355X-native HashGraph: hash tables stored as CSR graphs, built and
+probed by hand-written gfx950 kernels (libhashgraph_amd.so).
+
+This module is a thin torch front end over the C ABI (include/hashgraph.h)
+for tests and benchmarks: torch tensors provide device memory and the stream;
+all work happens in the library.  The C++ drop-in classes SingleHashGraph /
+MultiHashGraph (include/SingleHashGraph.hpp, include/MultiHashGraph.hpp) are
+the reference-compatible host API.
+
+Keys are uint32 or uint64 carried in int32 / int64 tensors (same bits).
+"""
+from __future__ import annotations
+
+import ctypes
+
+from . import _lib
+from ._lib import HG_HASH_MURMUR3, HG_HASH_RMI, HGError, check
+
+__all__ = ["Context", "build", "probe", "intersect", "intersect_rle", "exclusive_scan", "hash_bins", "find_splits",
+           "build_partitioned", "rmi_train", "rmi_model_bytes", "batch_build", "batch_probe", "HGError",
+           "HG_HASH_MURMUR3", "HG_HASH_RMI", "version"]
+
+_ctx = {}
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def version() -> str:
+    return _lib.load().hg_version().decode()
+
+
+class Context:
+    """One hg_ctx per device (scratch arena + stream binding)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = _lib.load()
+        self.device = device
+        h = ctypes.c_void_p()
+        check(self.lib.hg_ctx_create(device, ctypes.byref(h)), "hg_ctx_create")
+        self.h = h
+
+    def bind_stream(self):
+        torch = _torch()
+        s = torch.cuda.current_stream(self.device).cuda_stream
+        check(self.lib.hg_ctx_set_stream(self.h, ctypes.c_void_p(s)), "hg_ctx_set_stream")
+        return self.h
+
+    def reserve(self, nbytes: int):
+        check(self.lib.hg_ctx_reserve(self.h, nbytes), "hg_ctx_reserve")
+
+    def close(self):
+        if self.h:
+            self.lib.hg_ctx_destroy(self.h)
+            self.h = None
+
+
+def context(device=None) -> Context:
+    torch = _torch()
+    if device is None:
+        device = torch.cuda.current_device()
+    if device not in _ctx:
+        _ctx[device] = Context(device)
+    return _ctx[device]
+
+
+def _p(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else ctypes.c_void_p(0)
+
+
+def _key_bytes(t) -> int:
+    torch = _torch()
+    if t.dtype in (torch.int32, torch.uint32):
+        return 4
+    if t.dtype in (torch.int64, torch.uint64):
+        return 8
+    raise TypeError(f"keys must be a 32- or 64-bit integer tensor, got {t.dtype}")
+
+
+def _desc(n_keys, n_bins, key_bytes, offset_bytes=4, hash_kind=HG_HASH_MURMUR3, rmi_model=None):
+    d = _lib.TableDesc()
+    d.key_bytes = key_bytes
+    d.offset_bytes = offset_bytes
+    d.n_keys = n_keys
+    d.n_bins = n_bins
+    d.hash_kind = hash_kind
+    d.d_rmi_model = rmi_model.data_ptr() if rmi_model is not None else None
+    return d
+
+
+def _check_dev(*ts):
+    for t in ts:
+        if t is not None and (not t.is_cuda or not t.is_contiguous()):
+            raise ValueError("tensors must be contiguous device tensors")
+
+
+def build(keys, n_bins: int, offset_bytes: int = 4, hash_kind: int = HG_HASH_MURMUR3, rmi_model=None, out=None):
+    """Build the CSR of `keys` over n_bins bins.  Returns (offsets, csr_keys):
+    offsets int32/int64[n_bins+1], csr_keys like keys."""
+    torch = _torch()
+    _check_dev(keys)
+    kb = _key_bytes(keys)
+    c = context(keys.device.index)
+    if out is None:
+        off = torch.empty(n_bins + 1, dtype=torch.int32 if offset_bytes == 4 else torch.int64, device=keys.device)
+        csr = torch.empty_like(keys)
+    else:
+        off, csr = out
+    d = _desc(keys.numel(), n_bins, kb, offset_bytes, hash_kind, rmi_model)
+    check(c.lib.hg_build(c.bind_stream(), ctypes.byref(d), _p(keys), _p(off), _p(csr)), "hg_build")
+    return off, csr
+
+
+def probe(offsets, csr_keys, n_bins: int, probe_keys, emit: bool = True, capacity: int | None = None,
+          hash_kind: int = HG_HASH_MURMUR3, rmi_model=None, out=None):
+    """Probe raw keys against a built table.  Returns (count, matched keys or None)."""
+    torch = _torch()
+    _check_dev(offsets, csr_keys, probe_keys)
+    kb = _key_bytes(csr_keys)
+    ob = offsets.element_size()
+    c = context(csr_keys.device.index)
+    d = _desc(csr_keys.numel(), n_bins, kb, ob, hash_kind, rmi_model)
+    cnt = ctypes.c_uint64(0)
+    if not emit:
+        check(c.lib.hg_probe(c.bind_stream(), ctypes.byref(d), _p(offsets), _p(csr_keys), _p(probe_keys),
+                             probe_keys.numel(), None, 0, ctypes.byref(cnt)), "hg_probe")
+        return cnt.value, None
+    if out is None:
+        if capacity is None:
+            check(c.lib.hg_probe(c.bind_stream(), ctypes.byref(d), _p(offsets), _p(csr_keys), _p(probe_keys),
+                                 probe_keys.numel(), None, 0, ctypes.byref(cnt)), "hg_probe")
+            capacity = cnt.value
+        out = torch.empty(max(capacity, 1), dtype=csr_keys.dtype, device=csr_keys.device)
+    else:
+        capacity = out.numel()
+    check(c.lib.hg_probe(c.bind_stream(), ctypes.byref(d), _p(offsets), _p(csr_keys), _p(probe_keys),
+                         probe_keys.numel(), _p(out), capacity, ctypes.byref(cnt)), "hg_probe")
+    return cnt.value, out[: cnt.value]
+
+
+def intersect(offA, keysA, offB, keysB, n_bins: int, emit: bool = True, hash_kind: int = HG_HASH_MURMUR3,
+              rmi_model=None):
+    """Bag-product join of two built tables (MultiHashGraph::intersect).
+    Returns (count, matched keys or None)."""
+    torch = _torch()
+    _check_dev(offA, keysA, offB, keysB)
+    kb = _key_bytes(keysA)
+    ob = offA.element_size()
+    c = context(keysA.device.index)
+    dA = _desc(keysA.numel(), n_bins, kb, ob, hash_kind, rmi_model)
+    dB = _desc(keysB.numel(), n_bins, kb, ob, hash_kind, rmi_model)
+    cnt = ctypes.c_uint64(0)
+    check(c.lib.hg_intersect(c.bind_stream(), ctypes.byref(dA), _p(offA), _p(keysA), ctypes.byref(dB), _p(offB),
+                             _p(keysB), None, 0, ctypes.byref(cnt)), "hg_intersect")
+    if not emit:
+        return cnt.value, None
+    out = torch.empty(max(cnt.value, 1), dtype=keysA.dtype, device=keysA.device)
+    check(c.lib.hg_intersect(c.bind_stream(), ctypes.byref(dA), _p(offA), _p(keysA), ctypes.byref(dB), _p(offB),
+                             _p(keysB), _p(out), cnt.value, ctypes.byref(cnt)), "hg_intersect")
+    return cnt.value, out[: cnt.value]
+
+
+def intersect_rle(offA, keysA, offB, keysB, n_bins: int, hash_kind: int = HG_HASH_MURMUR3, rmi_model=None,
+                  capacity: int | None = None):
+    """Run-length join: returns (count, keys, countA, countB) of every value in both tables."""
+    torch = _torch()
+    _check_dev(offA, keysA, offB, keysB)
+    kb = _key_bytes(keysA)
+    ob = offA.element_size()
+    c = context(keysA.device.index)
+    dA = _desc(keysA.numel(), n_bins, kb, ob, hash_kind, rmi_model)
+    dB = _desc(keysB.numel(), n_bins, kb, ob, hash_kind, rmi_model)
+    cnt, nrec = ctypes.c_uint64(0), ctypes.c_uint64(0)
+    if capacity is None:
+        check(c.lib.hg_intersect_rle(c.bind_stream(), ctypes.byref(dA), _p(offA), _p(keysA), ctypes.byref(dB),
+                                     _p(offB), _p(keysB), None, None, None, 0, ctypes.byref(nrec), ctypes.byref(cnt)),
+              "hg_intersect_rle")
+        capacity = nrec.value
+    rk = torch.empty(max(capacity, 1), dtype=keysA.dtype, device=keysA.device)
+    ca = torch.empty(max(capacity, 1), dtype=torch.int64, device=keysA.device)
+    cb = torch.empty(max(capacity, 1), dtype=torch.int64, device=keysA.device)
+    check(c.lib.hg_intersect_rle(c.bind_stream(), ctypes.byref(dA), _p(offA), _p(keysA), ctypes.byref(dB), _p(offB),
+                                 _p(keysB), _p(rk), _p(ca), _p(cb), capacity, ctypes.byref(nrec), ctypes.byref(cnt)),
+          "hg_intersect_rle")
+    n = nrec.value
+    return cnt.value, rk[:n], ca[:n], cb[:n]
+
+
+def exclusive_scan(x):
+    """Exclusive prefix sum of a 32/64-bit integer tensor; returns n+1 values (total last)."""
+    torch = _torch()
+    _check_dev(x)
+    c = context(x.device.index)
+    out = torch.empty(x.numel() + 1, dtype=x.dtype, device=x.device)
+    check(c.lib.hg_exclusive_scan(c.bind_stream(), _p(x), _p(out), x.numel(), x.element_size()),
+          "hg_exclusive_scan")
+    return out
+
+
+def hash_bins(keys, n_bins: int):
+    """murmur3(low32(key)) mod n_bins for every key (int32 tensor holding uint32)."""
+    torch = _torch()
+    _check_dev(keys)
+    c = context(keys.device.index)
+    out = torch.empty(keys.numel(), dtype=torch.int32, device=keys.device)
+    check(c.lib.hg_hash_bins(c.bind_stream(), _p(keys), keys.numel(), _key_bytes(keys), n_bins, _p(out)),
+          "hg_hash_bins")
+    return out
+
+
+def find_splits(keys, n_bins: int, bin_count: int = 16000, n_parts: int = 16, hash_kind: int = HG_HASH_MURMUR3,
+                rmi_model=None):
+    """Key-balanced partition split points over the bin range (list of n_parts+1 ints)."""
+    _check_dev(keys)
+    c = context(keys.device.index)
+    d = _desc(keys.numel(), n_bins, _key_bytes(keys), 4, hash_kind, rmi_model)
+    sp = (ctypes.c_int64 * (n_parts + 1))()
+    check(c.lib.hg_find_splits(c.bind_stream(), ctypes.byref(d), _p(keys), bin_count, n_parts, sp),
+          "hg_find_splits")
+    return list(sp)
+
+
+def build_partitioned(keys, n_bins: int, splits, offset_bytes: int = 4, hash_kind: int = HG_HASH_MURMUR3,
+                      rmi_model=None):
+    """Per-partition CSRs.  Returns (offsets, csr_keys, part_offsets): partition p's
+    offsets are offsets[splits[p]+p : splits[p+1]+p+1], its keys
+    csr_keys[part_offsets[p]:part_offsets[p+1]]."""
+    torch = _torch()
+    _check_dev(keys)
+    n_parts = len(splits) - 1
+    c = context(keys.device.index)
+    d = _desc(keys.numel(), n_bins, _key_bytes(keys), offset_bytes, hash_kind, rmi_model)
+    off = torch.empty(int(splits[-1]) + n_parts + 1, dtype=torch.int32 if offset_bytes == 4 else torch.int64,
+                      device=keys.device)
+    csr = torch.empty_like(keys)
+    sp = (ctypes.c_int64 * (n_parts + 1))(*[int(x) for x in splits])
+    po = (ctypes.c_uint64 * (n_parts + 1))()
+    check(c.lib.hg_build_partitioned(c.bind_stream(), ctypes.byref(d), _p(keys), sp, n_parts, _p(off), _p(csr), po),
+          "hg_build_partitioned")
+    return off, csr, list(po)
+
+
+def rmi_model_bytes(n_leaves: int = 4096) -> int:
+    return int(_lib.load().hg_rmi_model_bytes(n_leaves))
+
+
+def rmi_train(sample, n_bins: int, n_leaves: int = 4096):
+    """Fit the learned CDF model on a key sample.  Returns (model tensor, use_rmi)."""
+    torch = _torch()
+    _check_dev(sample)
+    c = context(sample.device.index)
+    model = torch.empty(rmi_model_bytes(n_leaves), dtype=torch.uint8, device=sample.device)
+    use = ctypes.c_uint32(0)
+    check(c.lib.hg_rmi_train(c.bind_stream(), _p(sample), sample.numel(), _key_bytes(sample), n_bins, n_leaves,
+                             _p(model), ctypes.byref(use)), "hg_rmi_train")
+    return model, bool(use.value)
+
+
+def _batch_desc(key_off, n_bins, off_off, key_bytes, hash_kinds=None, rmi_models=None, rmi_leaves=0):
+    bd = _lib.BatchDesc()
+    T = len(n_bins)
+    bd.n_tables = T
+    bd.key_bytes = key_bytes
+    ko = (ctypes.c_uint64 * (T + 1))(*[int(x) for x in key_off])
+    nb = (ctypes.c_uint64 * T)(*[int(x) for x in n_bins])
+    oo = (ctypes.c_uint64 * T)(*[int(x) for x in off_off])
+    bd.h_key_off = ko
+    bd.h_n_bins = nb
+    bd.h_off_off = oo
+    hk = None
+    if hash_kinds is not None:
+        hk = (ctypes.c_uint32 * T)(*[int(x) for x in hash_kinds])
+        bd.h_hash_kind = hk
+    bd.d_rmi_models = rmi_models.data_ptr() if rmi_models is not None else None
+    bd.rmi_leaves = rmi_leaves
+    bd._keep = (ko, nb, oo, hk)
+    return bd
+
+
+def batch_build(keys, key_off, n_bins, hash_kinds=None, rmi_models=None, rmi_leaves=0):
+    """Build len(n_bins) tables at once.  Table t's keys are keys[key_off[t]:key_off[t+1]].
+    Returns (offsets int32, csr_keys, off_off) with table t's offsets at
+    offsets[off_off[t] : off_off[t]+n_bins[t]+1]."""
+    torch = _torch()
+    _check_dev(keys)
+    off_off, run = [], 0
+    for m in n_bins:
+        off_off.append(run)
+        run += int(m) + 1
+    bd = _batch_desc(key_off, n_bins, off_off, _key_bytes(keys), hash_kinds, rmi_models, rmi_leaves)
+    c = context(keys.device.index)
+    off = torch.empty(max(run, 1), dtype=torch.int32, device=keys.device)
+    csr = torch.empty_like(keys)
+    check(c.lib.hg_batch_build(c.bind_stream(), ctypes.byref(bd), _p(keys), _p(off), _p(csr)), "hg_batch_build")
+    return off, csr, off_off
+
+
+def batch_probe(offsets, csr_keys, key_off, n_bins, off_off, probe, probe_off, hash_kinds=None, rmi_models=None,
+                rmi_leaves=0):
+    """Per-table match counts of probe[probe_off[t]:probe_off[t+1]] against table t."""
+    torch = _torch()
+    _check_dev(offsets, csr_keys, probe)
+    bd = _batch_desc(key_off, n_bins, off_off, _key_bytes(csr_keys), hash_kinds, rmi_models, rmi_leaves)
+    c = context(csr_keys.device.index)
+    T = len(n_bins)
+    counts = torch.zeros(T, dtype=torch.int64, device=csr_keys.device)
+    po = (ctypes.c_uint64 * (T + 1))(*[int(x) for x in probe_off])
+    total = ctypes.c_uint64(0)
+    check(c.lib.hg_batch_probe(c.bind_stream(), ctypes.byref(bd), _p(offsets), _p(csr_keys), _p(probe), po,
+                               _p(counts), ctypes.byref(total)), "hg_batch_probe")
+    return total.value, counts

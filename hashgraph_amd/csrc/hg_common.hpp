@@ -1,0 +1,111 @@
+// hg_common.hpp -- types shared by the host engine and the gfx950 kernels.
+#pragma once
+#include <stdint.h>
+
+namespace hg {
+
+// Bin map of one table: bin = murmur3_x86_32(low32(key), seed 0) mod M
+// (include/SingleHashGraphOperators.cuh:71-79), or the learned CDF model.
+// The remainder uses Lemire, Kaser & Kurz, "Faster remainder by direct
+// computation" (2019): m64 = floor((2^64-1)/M) + 1, a mod M =
+// hi64((m64 * a mod 2^64) * M); exact for every 32-bit a and 1 <= M < 2^32.
+struct BinMap {
+  uint64_t m64;       // Lemire constant for M
+  uint32_t M;         // number of bins
+  uint32_t kind;      // 0 murmur3, 1 learned (RMI)
+  const void *rmi;    // device pointer to the RMI model (kind == 1)
+};
+
+// Division of a local bin by a bucket width W (same direct method):
+// q = hi64(d64 * a) with d64 = floor((2^64-1)/W) + 1; exact for 32-bit a.
+struct Div32 {
+  uint64_t d64;
+  uint32_t w;
+  uint32_t pad;
+};
+
+// One segment of a partition pass: a contiguous key range that is split
+// into buckets by one bucket function.
+//   kind 0 (range): bucket = (bin - bin_lo) / width, for bin in
+//                   [bin_lo, bin_lo + nbuckets*width)
+//   kind 1 (split): bucket = first p with bin < splits[p+1]; bucket
+//                   nbuckets collects keys past the last split (dropped)
+struct SegDesc {
+  uint64_t key_begin;   // first key of the segment (input and output)
+  uint64_t key_end;
+  uint64_t bin_lo;
+  BinMap map;
+  Div32 div;
+  uint32_t kind;
+  uint32_t nbuckets;    // F_s (plus one drop bucket for kind 1)
+  uint64_t bucket_base; // global id of bucket 0 of this segment
+  uint64_t row_base;    // first count-matrix element of this segment
+  uint32_t tile_begin;
+  uint32_t ntiles;
+  uint32_t ncols;       // row length of the count matrix (F_s, +1 for kind 1)
+  uint32_t nsplits;     // kind 1: number of partitions P
+  const int64_t *splits;// kind 1: device pointer, P+1 entries
+  // used when this segment feeds the final stage / the probe:
+  uint64_t seg_nbins;   // bins of the segment's range (kind 0)
+  uint64_t fin_off_base;// offsets index of local bin 0 (bin_lo)
+  uint64_t fin_key_rel; // key index subtracted from offset values
+  uint64_t fin_end_pos; // offsets index of the end entry, ~0 = none
+  uint32_t table;       // BinMap index
+  uint32_t pad2;
+};
+
+struct TileDesc {
+  uint64_t key_start;
+  uint32_t count;
+  uint32_t seg;
+};
+
+// Work item of the column scan: rows [row_lo, row_hi) of one segment,
+// columns [col_lo, col_lo + 256).
+struct ColItem {
+  uint32_t seg;
+  uint32_t chunk;       // chunk index inside the segment
+  uint32_t row_lo, row_hi;  // tile indices relative to the segment
+  uint32_t col_lo;
+  uint32_t pad;
+};
+
+// One bucket of the final stage: keys part[key_start .. +count) all have a
+// bin in [bin_lo, bin_lo + nbins) of table `map`; they are written as CSR
+// keys at csr[key_start ..] (same range) and the offsets of those bins at
+// off[off_pos .. off_pos + nbins) with value key_start - key_rel + prefix.
+// When end_pos != ~0 the item also writes off[end_pos] = key_start + count
+// - key_rel (the table's final offset).
+struct FinItem {
+  uint64_t key_start;
+  uint64_t key_rel;
+  uint64_t off_pos;
+  uint64_t end_pos;
+  uint32_t count;
+  uint32_t nbins;
+  uint32_t bin_lo;
+  uint32_t table;   // index into the BinMap array
+};
+
+// Learned CDF model (no counterpart in the reference).  Two stages:
+//   leaf  = clamp(floor(root_slope * (float)(key - key_min)), 0, L-1)
+//   bin   = clamp(base[leaf] + floor(slope[leaf] * (float)(key - x0[leaf])),
+//                 lo[leaf], hi[leaf])
+// The integer anchors (key_min, x0, base) keep FP32 precision where it is
+// needed: the float only ever holds a distance inside one leaf.  Leaves are
+// stored as arrays of RmiLeaf after the header.
+struct RmiHeader {
+  uint64_t key_min;
+  float root_slope;
+  uint32_t n_leaves;
+  uint32_t n_bins;
+  uint32_t pad;
+};
+struct RmiLeaf {
+  uint64_t x0;      // smallest sample key routed to the leaf
+  float slope;      // bins per key unit
+  uint32_t base;    // bin of x0
+  uint32_t lo, hi;  // bin range the leaf may emit (monotone, non-overlapping)
+};
+
+}  // namespace hg

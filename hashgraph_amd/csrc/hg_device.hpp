@@ -1,0 +1,159 @@
+// hg_device.hpp -- device helpers for the gfx950 kernels (wave64).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "hg_common.hpp"
+
+namespace hg {
+
+__device__ __forceinline__ uint32_t rotl32(uint32_t x, int r) { return (x << r) | (x >> (32 - r)); }
+
+// MurmurHash3_x86_32 of the four little-endian bytes of k, seed 0: the
+// reference's hash_murmur with len = sizeof(int)
+// (include/SingleHashGraphOperators.cuh:31-69): one body block, no tail,
+// h ^= 4, fmix32.
+__device__ __forceinline__ uint32_t murmur4(uint32_t k) {
+  k *= 0xcc9e2d51u;
+  k = rotl32(k, 15);
+  k *= 0x1b873593u;
+  uint32_t h = rotl32(k, 13) * 5u + 0xe6546b64u;
+  h ^= 4u;
+  h ^= h >> 16;
+  h *= 0x85ebca6bu;
+  h ^= h >> 13;
+  h *= 0xc2b2ae35u;
+  h ^= h >> 16;
+  return h;
+}
+
+// Raw buffer loads: one resource (base, size in bytes) per tile or bucket;
+// the hardware returns 0 past the end, so slot j of thread t needs neither a
+// clamp nor a branch, and the per-slot offset is a scalar.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *p, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes, 0x00020000);
+}
+template <typename K>
+__device__ __forceinline__ K buf_load(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff);
+template <>
+__device__ __forceinline__ uint32_t buf_load<uint32_t>(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+  return __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0);
+}
+template <>
+__device__ __forceinline__ uint64_t buf_load<uint64_t>(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+  auto v = __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0);
+  return (uint64_t)v[0] | ((uint64_t)v[1] << 32);
+}
+
+// a mod d and a / d by direct computation (Lemire, Kaser & Kurz 2019).
+__device__ __forceinline__ uint32_t fastmod(uint32_t a, uint64_t m64, uint32_t d) {
+  uint64_t low = m64 * (uint64_t)a;
+  return (uint32_t)__umul64hi(low, (uint64_t)d);
+}
+__device__ __forceinline__ uint32_t fastdiv(uint32_t a, const Div32 &dv) {
+  return dv.w == 1 ? a : (uint32_t)__umul64hi(dv.d64, (uint64_t)a);
+}
+
+// Bin of a key under the learned CDF model (layout in hg_common.hpp).
+__device__ __forceinline__ uint32_t rmi_bin(uint64_t key, const void *model, uint32_t M) {
+  const RmiHeader *h = (const RmiHeader *)model;
+  const RmiLeaf *lv = (const RmiLeaf *)(h + 1);
+  const uint32_t L = h->n_leaves;
+  float xr = key > h->key_min ? (float)(key - h->key_min) : 0.0f;
+  float lf = __fmul_rn(h->root_slope, xr);
+  uint32_t leaf = lf <= 0.0f ? 0u : (lf >= (float)(L - 1) ? L - 1 : (uint32_t)lf);
+  const RmiLeaf l = lv[leaf];
+  float d = key > l.x0 ? (float)(key - l.x0) : 0.0f;
+  float y = __fmul_rn(l.slope, d);
+  uint32_t off = y <= 0.0f ? 0u : (y >= 4294967040.0f ? 0xFFFFFF00u : (uint32_t)y);
+  uint32_t b = l.base + off;
+  if (b < l.base) b = l.hi;  // wrapped
+  b = b < l.lo ? l.lo : (b > l.hi ? l.hi : b);
+  return b < M ? b : M - 1;
+}
+
+template <typename K>
+__device__ __forceinline__ uint32_t bin_of(K key, const BinMap &m) {
+  if (m.kind == 0) {
+    uint32_t h = murmur4((uint32_t)key);
+    return (m.M & (m.M - 1)) == 0 ? (h & (m.M - 1)) : fastmod(h, m.m64, m.M);
+  }
+  return rmi_bin((uint64_t)key, m.rmi, m.M);
+}
+
+template <typename K>
+__device__ __forceinline__ uint32_t bucket_of(K key, const SegDesc &sd) {
+  uint32_t bin = bin_of<K>(key, sd.map);
+  if (sd.kind == 0) return fastdiv(bin - (uint32_t)sd.bin_lo, sd.div);
+  uint32_t p = 0;
+  while (p < sd.nsplits && (int64_t)bin >= sd.splits[p + 1]) p++;
+  return p;  // p == nsplits: past the last split
+}
+
+// Inclusive wave64 scan of a 32/64-bit value.
+template <typename T>
+__device__ __forceinline__ T wave_inclusive_sum(T x) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    T y = __shfl_up(x, d, 64);
+    if (lane >= d) x += y;
+  }
+  return x;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T x) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) x += __shfl_xor(x, d, 64);
+  return x;
+}
+
+// Exclusive block-wide sum of one value per thread.  s_tmp needs
+// BLOCK/64 + 1 elements of T.  Ends with a barrier so s_tmp can be reused.
+template <int BLOCK, typename T>
+__device__ __forceinline__ T block_exclusive_sum(T v, T *s_tmp, T &total) {
+  constexpr int NW = BLOCK / 64;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  T x = wave_inclusive_sum<T>(v);
+  if (lane == 63) s_tmp[wid] = x;
+  __syncthreads();
+  if (wid == 0) {
+    T w = lane < NW ? s_tmp[lane] : T(0);
+    T wx = wave_inclusive_sum<T>(w);
+    if (lane < NW) s_tmp[lane] = wx - w;
+    if (lane == NW - 1) s_tmp[NW] = wx;
+  }
+  __syncthreads();
+  total = s_tmp[NW];
+  T r = s_tmp[wid] + x - v;
+  __syncthreads();
+  return r;
+}
+
+template <int BLOCK, typename T>
+__device__ __forceinline__ T block_sum(T v, T *s_tmp) {
+  T tot;
+  (void)block_exclusive_sum<BLOCK, T>(v, s_tmp, tot);
+  return tot;
+}
+
+// Exclusive scan of a[0..n) in LDS (n <= a few times BLOCK*64).  Each thread
+// scans a contiguous chunk.  Returns the total; ends with a barrier.
+template <int BLOCK>
+__device__ __forceinline__ uint32_t lds_exclusive_scan(uint32_t *a, uint32_t n, uint32_t *s_tmp) {
+  const uint32_t per = (n + BLOCK - 1) / BLOCK;
+  const uint32_t lo = min(threadIdx.x * per, n), hi = min(lo + per, n);
+  uint32_t s = 0;
+  for (uint32_t i = lo; i < hi; i++) s += a[i];
+  uint32_t tot;
+  uint32_t ex = block_exclusive_sum<BLOCK, uint32_t>(s, s_tmp, tot);
+  for (uint32_t i = lo; i < hi; i++) {
+    uint32_t v = a[i];
+    a[i] = ex;
+    ex += v;
+  }
+  __syncthreads();
+  return tot;
+}
+
+}  // namespace hg

@@ -1,0 +1,842 @@
+// hg_engine.cpp -- host side of the C ABI (include/hashgraph.h): scratch
+// arena, descriptor upload and the multi-level build / probe drivers.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string.h>
+#include <algorithm>
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "../../include/hashgraph.h"
+#include "hg_common.hpp"
+#include "hg_kernels.hpp"
+
+using namespace hg;
+
+namespace {
+
+thread_local std::string g_err;
+
+hg_status fail(hg_status s, const std::string &m) {
+  g_err = m;
+  return s;
+}
+
+#define HG_HIP(call)                                                                                   \
+  do {                                                                                                 \
+    hipError_t e_ = (call);                                                                            \
+    if (e_ != hipSuccess) return fail(HG_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define HG_TRY(call)                 \
+  do {                               \
+    hg_status s_ = (call);           \
+    if (s_ != HG_OK) return s_;      \
+  } while (0)
+
+constexpr uint32_t kFMax = 16384;     // buckets per segment of one partition pass
+constexpr uint32_t kTileKpt = 32;     // keys per thread of a partition tile
+constexpr uint32_t kTile = 1024 * kTileKpt;
+constexpr uint32_t kChunk = 64;       // count-matrix rows per column-scan chunk
+
+uint64_t lemire_m64(uint64_t d) { return d > 1 ? (UINT64_C(0xFFFFFFFFFFFFFFFF) / d + 1) : 0; }
+
+Div32 make_div(uint32_t w) {
+  Div32 v;
+  v.w = w;
+  v.d64 = lemire_m64(w);
+  v.pad = 0;
+  return v;
+}
+
+uint32_t final_cap(int kb) { return std::min<uint32_t>(1024u * (uint32_t)final_kpt(kb), 65535u); }
+
+// Bucket width W (bins) of a table so that a bucket holds ~7/8 of what the
+// final stage keeps in VGPRs; W <= 65535 (u16 LDS counters).
+uint32_t choose_width(uint64_t n, uint64_t M, int kb) {
+  const double target = 0.875 * final_cap(kb);
+  double w = n ? target * (double)M / (double)n : 65535.0;
+  w = std::floor(w);
+  if (w < 1) w = 1;
+  if (w > 65535) w = 65535;
+  return (uint32_t)w;
+}
+
+}  // namespace
+
+struct hg_ctx_s {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  // arena 0 holds an operation's large buffers (partitioned copies), arena 1
+  // one partition level's descriptors and matrices; arena 1 may be regrown
+  // between levels without touching arena 0.
+  char *arena[2] = {nullptr, nullptr};
+  size_t arena_cap[2] = {0, 0};
+  static constexpr int kSlots = 4;
+  char *hslot[kSlots] = {nullptr, nullptr, nullptr, nullptr};
+  size_t hslot_cap[kSlots] = {0, 0, 0, 0};
+  hipEvent_t hev[kSlots] = {nullptr, nullptr, nullptr, nullptr};
+  bool hev_used[kSlots] = {false, false, false, false};
+  int next_slot = 0;
+  unsigned long long *h_small = nullptr;  // pinned read-back area
+};
+
+namespace {
+
+struct DeviceGuard {
+  int prev = -1;
+  explicit DeviceGuard(int dev) {
+    if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+    if (prev != dev) (void)hipSetDevice(dev);
+  }
+  ~DeviceGuard() {
+    if (prev >= 0) (void)hipSetDevice(prev);
+  }
+};
+
+hg_status arena_ensure(hg_ctx c, int a, size_t bytes) {
+  if (bytes <= c->arena_cap[a]) return HG_OK;
+  HG_HIP(hipStreamSynchronize(c->stream));
+  size_t want = std::max(bytes, c->arena_cap[a] * 5 / 4);
+  want = (want + (1 << 20) - 1) & ~(size_t)((1 << 20) - 1);
+  if (c->arena[a]) HG_HIP(hipFree(c->arena[a]));
+  c->arena[a] = nullptr;
+  c->arena_cap[a] = 0;
+  if (hipMalloc(&c->arena[a], want) != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(HG_ERR_OOM, "scratch arena: cannot allocate " + std::to_string(want) + " bytes");
+  }
+  c->arena_cap[a] = want;
+  return HG_OK;
+}
+
+// Sizes of the arena pieces of one call, carved in order.
+struct Plan {
+  std::vector<size_t> sizes;
+  size_t add(size_t bytes) {
+    sizes.push_back((bytes + 255) & ~(size_t)255);
+    return sizes.size() - 1;
+  }
+  size_t total() const {
+    size_t t = 0;
+    for (size_t s : sizes) t += s;
+    return t;
+  }
+};
+
+struct Carve {
+  std::vector<char *> ptr;
+  void init(hg_ctx c, int a, const Plan &p) {
+    ptr.clear();
+    size_t off = 0;
+    for (size_t s : p.sizes) {
+      ptr.push_back(c->arena[a] + off);
+      off += s;
+    }
+  }
+  template <typename T>
+  T *get(size_t i) const { return (T *)ptr[i]; }
+};
+
+// Copies the host blob to the device through a pinned slot; the slot is not
+// rewritten before the previous copy out of it has finished.
+hg_status upload(hg_ctx c, void *dst, const void *src, size_t bytes) {
+  if (!bytes) return HG_OK;
+  const int s = c->next_slot;
+  c->next_slot = (s + 1) % hg_ctx_s::kSlots;
+  if (c->hev_used[s]) HG_HIP(hipEventSynchronize(c->hev[s]));
+  if (c->hslot_cap[s] < bytes) {
+    if (c->hslot[s]) HG_HIP(hipHostFree(c->hslot[s]));
+    c->hslot[s] = nullptr;
+    size_t want = std::max<size_t>(bytes, 1 << 16);
+    HG_HIP(hipHostMalloc((void **)&c->hslot[s], want, hipHostMallocDefault));
+    c->hslot_cap[s] = want;
+  }
+  memcpy(c->hslot[s], src, bytes);
+  HG_HIP(hipMemcpyAsync(dst, c->hslot[s], bytes, hipMemcpyHostToDevice, c->stream));
+  HG_HIP(hipEventRecord(c->hev[s], c->stream));
+  c->hev_used[s] = true;
+  return HG_OK;
+}
+
+// Host blob builder: appends arrays with 256-byte alignment.
+struct Blob {
+  std::vector<char> data;
+  template <typename T>
+  size_t put(const std::vector<T> &v) {
+    size_t off = (data.size() + 255) & ~(size_t)255;
+    data.resize(off + std::max<size_t>(v.size() * sizeof(T), 1));
+    if (!v.empty()) memcpy(data.data() + off, v.data(), v.size() * sizeof(T));
+    return off;
+  }
+};
+
+BinMap make_map(uint64_t M, uint32_t kind, const void *rmi) {
+  BinMap m;
+  m.M = (uint32_t)M;
+  m.m64 = lemire_m64(M);
+  m.kind = kind;
+  m.rmi = rmi;
+  return m;
+}
+
+hg_status check_desc(const hg_table_desc *d, bool need_keys = true) {
+  if (!d) return fail(HG_ERR_INVALID_ARG, "null table description");
+  if (d->key_bytes != 4 && d->key_bytes != 8) return fail(HG_ERR_INVALID_ARG, "key_bytes must be 4 or 8");
+  if (d->offset_bytes != 4 && d->offset_bytes != 8) return fail(HG_ERR_INVALID_ARG, "offset_bytes must be 4 or 8");
+  if (d->n_bins == 0 || d->n_bins >= (1ull << 32)) return fail(HG_ERR_INVALID_ARG, "n_bins must be in [1, 2^32)");
+  if (d->hash_kind != HG_HASH_MURMUR3 && d->hash_kind != HG_HASH_RMI)
+    return fail(HG_ERR_INVALID_ARG, "unknown hash_kind");
+  if (d->hash_kind == HG_HASH_RMI && !d->d_rmi_model) return fail(HG_ERR_INVALID_ARG, "HG_HASH_RMI needs d_rmi_model");
+  if (need_keys && d->n_keys >= (1ull << 32))
+    return fail(HG_ERR_UNSUPPORTED, "n_keys >= 2^32 is not supported by this build");
+  if (d->offset_bytes == 4 && d->n_keys >= (1ull << 32))
+    return fail(HG_ERR_INVALID_ARG, "offset_bytes 4 needs n_keys < 2^32");
+  return HG_OK;
+}
+
+// One partition pass over host-described segments.  On the device it
+// produces positions in cmat, bucket starts (bstart) and sizes (coltot) for
+// every global column.  Its descriptors and matrices live in arena 1.
+struct Level {
+  std::vector<SegDesc> segs;   // tile_begin/ntiles/row_base/bucket_base filled by prepare()
+  std::vector<TileDesc> tiles;
+  std::vector<ColItem> items, items2;
+  std::vector<uint64_t> psum_base;
+  std::vector<uint32_t> nchunks;
+  std::vector<BinMap> maps;
+  std::vector<int64_t> splits;  // kind-1 segments point into this array
+  uint32_t ncols_total = 0, max_cols = 0;
+  uint64_t cmat_elems = 0, psum_elems = 0;
+
+  void prepare() {
+    tiles.clear();
+    items.clear();
+    items2.clear();
+    psum_base.clear();
+    nchunks.clear();
+    uint64_t row = 0, ps = 0;
+    uint32_t bb = 0;
+    max_cols = 1;
+    for (uint32_t s = 0; s < segs.size(); s++) {
+      SegDesc &sd = segs[s];
+      sd.tile_begin = (uint32_t)tiles.size();
+      for (uint64_t k = sd.key_begin; k < sd.key_end; k += kTile) {
+        TileDesc t;
+        t.key_start = k;
+        t.count = (uint32_t)std::min<uint64_t>(kTile, sd.key_end - k);
+        t.seg = s;
+        tiles.push_back(t);
+      }
+      sd.ntiles = (uint32_t)tiles.size() - sd.tile_begin;
+      sd.row_base = row;
+      sd.bucket_base = bb;
+      row += (uint64_t)sd.ntiles * sd.ncols;
+      bb += sd.ncols;
+      max_cols = std::max(max_cols, sd.ncols);
+      const uint32_t nch = (sd.ntiles + kChunk - 1) / kChunk;
+      psum_base.push_back(ps);
+      nchunks.push_back(nch);
+      ps += (uint64_t)nch * sd.ncols;
+      for (uint32_t cb = 0; cb < sd.ncols; cb += 256) {
+        ColItem ci{};
+        ci.seg = s;
+        ci.col_lo = cb;
+        items2.push_back(ci);
+        for (uint32_t ch = 0; ch < nch; ch++) {
+          ColItem x{};
+          x.seg = s;
+          x.chunk = ch;
+          x.row_lo = ch * kChunk;
+          x.row_hi = std::min(sd.ntiles, (ch + 1) * kChunk);
+          x.col_lo = cb;
+          items.push_back(x);
+        }
+      }
+    }
+    ncols_total = bb;
+    cmat_elems = row;
+    psum_elems = ps;
+  }
+};
+
+struct LevelDev {
+  TileDesc *tiles;
+  SegDesc *segs;
+  ColItem *items, *items2;
+  uint64_t *psum_base;
+  uint32_t *nchunks;
+  BinMap *maps;
+  int64_t *splits;
+  uint32_t *cmat, *psum, *coltot, *bstart;
+  FinItem *fin;
+};
+
+// Lays out a level in arena 1 (growing it if needed; the stream must be
+// idle of earlier users of arena 1, which holds because every level is
+// planned either at the start of an operation or after a read-back), patches
+// device pointers into the descriptors and uploads them.
+hg_status setup_level(hg_ctx c, Level &L, LevelDev &d) {
+  Blob b0;
+  const size_t o_tiles = b0.put(L.tiles), o_segs = b0.put(L.segs), o_items = b0.put(L.items), o_items2 = b0.put(L.items2),
+               o_psb = b0.put(L.psum_base), o_nch = b0.put(L.nchunks), o_maps = b0.put(L.maps), o_spl = b0.put(L.splits);
+  Plan P;
+  const size_t i_blob = P.add(b0.data.size());
+  const size_t i_cmat = P.add(std::max<uint64_t>(L.cmat_elems, 1) * 4);
+  const size_t i_psum = P.add(std::max<uint64_t>(L.psum_elems, 1) * 4);
+  const size_t i_coltot = P.add(std::max<uint32_t>(L.ncols_total, 1) * 4);
+  const size_t i_bstart = P.add(std::max<uint32_t>(L.ncols_total, 1) * 4);
+  const size_t i_fin = P.add((size_t)std::max<uint32_t>(L.ncols_total, 1) * sizeof(FinItem));
+  HG_TRY(arena_ensure(c, 1, P.total()));
+  Carve cv;
+  cv.init(c, 1, P);
+  char *blob = cv.get<char>(i_blob);
+  d.tiles = (TileDesc *)(blob + o_tiles);
+  d.segs = (SegDesc *)(blob + o_segs);
+  d.items = (ColItem *)(blob + o_items);
+  d.items2 = (ColItem *)(blob + o_items2);
+  d.psum_base = (uint64_t *)(blob + o_psb);
+  d.nchunks = (uint32_t *)(blob + o_nch);
+  d.maps = (BinMap *)(blob + o_maps);
+  d.splits = (int64_t *)(blob + o_spl);
+  d.cmat = cv.get<uint32_t>(i_cmat);
+  d.psum = cv.get<uint32_t>(i_psum);
+  d.coltot = cv.get<uint32_t>(i_coltot);
+  d.bstart = cv.get<uint32_t>(i_bstart);
+  d.fin = cv.get<FinItem>(i_fin);
+  for (SegDesc &sd : L.segs)
+    if (sd.kind == 1) sd.splits = d.splits;
+  Blob b1;
+  b1.put(L.tiles);
+  b1.put(L.segs);
+  b1.put(L.items);
+  b1.put(L.items2);
+  b1.put(L.psum_base);
+  b1.put(L.nchunks);
+  b1.put(L.maps);
+  b1.put(L.splits);
+  return upload(c, blob, b1.data.data(), b1.data.size());
+}
+
+hg_status run_level(hg_ctx c, const Level &L, const LevelDev &d, int kb, const void *in, void *out) {
+  LaunchPart p{};
+  p.key_bytes = kb;
+  p.kpt = (int)kTileKpt;
+  p.ntiles = (uint32_t)L.tiles.size();
+  p.max_cols = L.max_cols;
+  p.stage_cap = 8192;
+  p.keys = in;
+  p.out = out;
+  p.tiles = d.tiles;
+  p.segs = d.segs;
+  p.cmat = d.cmat;
+  if (p.ntiles) HG_HIP(launch_part_hist(p, c->stream));
+  LaunchCol lc{};
+  lc.nitems = (uint32_t)L.items.size();
+  lc.nitems2 = (uint32_t)L.items2.size();
+  lc.nsegs = (uint32_t)L.segs.size();
+  lc.cmat = d.cmat;
+  lc.items = d.items;
+  lc.items2 = d.items2;
+  lc.segs = d.segs;
+  lc.psum_base = d.psum_base;
+  lc.nchunks = d.nchunks;
+  lc.psum = d.psum;
+  lc.coltot = d.coltot;
+  lc.bstart = d.bstart;
+  if (lc.nsegs) HG_HIP(launch_colscan(lc, c->stream));
+  if (p.ntiles) HG_HIP(launch_part_scatter(p, c->stream));
+  return HG_OK;
+}
+
+SegDesc range_seg(uint64_t kbeg, uint64_t kend, uint64_t bin_lo, uint64_t nbins, uint32_t W, const BinMap &m) {
+  SegDesc s{};
+  s.key_begin = kbeg;
+  s.key_end = kend;
+  s.bin_lo = bin_lo;
+  s.map = m;
+  s.div = make_div(W);
+  s.kind = 0;
+  s.nbuckets = (uint32_t)((nbins + W - 1) / W);
+  s.ncols = s.nbuckets;
+  s.seg_nbins = nbins;
+  s.fin_end_pos = ~0ull;
+  return s;
+}
+
+// Final stage over the columns of a level.
+hg_status run_final(hg_ctx c, const Level &L, const LevelDev &d, int kb, int ob, const void *part, void *csr, void *off) {
+  HG_HIP(launch_make_items(d.segs, (uint32_t)L.segs.size(), d.bstart, d.coltot, d.fin, L.ncols_total, c->stream));
+  LaunchFinal f{};
+  f.key_bytes = kb;
+  f.off_bytes = ob;
+  f.nitems = L.ncols_total;
+  f.lds_bytes = 128 * 1024;
+  f.part = part;
+  f.csr = csr;
+  f.off = off;
+  f.items = d.fin;
+  f.maps = d.maps;
+  HG_HIP(launch_final(f, c->stream));
+  return HG_OK;
+}
+
+hg_status sync_read(hg_ctx c, void *dst, const void *src, size_t bytes) {
+  if (bytes > (1 << 20)) return fail(HG_ERR_INVALID_ARG, "read-back too large");
+  HG_HIP(hipMemcpyAsync(c->h_small, src, bytes, hipMemcpyDeviceToHost, c->stream));
+  HG_HIP(hipStreamSynchronize(c->stream));
+  memcpy(dst, c->h_small, bytes);
+  return HG_OK;
+}
+
+hg_status fill_offsets(hg_ctx c, void *off, int ob, uint64_t n, uint64_t v) {
+  if (!n) return HG_OK;
+  if (ob == 4) HG_HIP(launch_fill_u32((uint32_t *)off, (uint32_t)v, n, c->stream));
+  else HG_HIP(launch_fill_u64((unsigned long long *)off, (unsigned long long)v, n, c->stream));
+  return HG_OK;
+}
+
+// Second partition level over the buckets of a first level whose sizes were
+// read back: each first-level bucket becomes a range segment.
+struct Coarse {
+  uint64_t key_begin, count, bin_lo, nbins;
+  uint64_t off_base, key_rel, end_pos;
+};
+
+hg_status finish_two_level(hg_ctx c, const std::vector<Coarse> &cs, const BinMap &map, int kb, int ob, const void *part1,
+                           void *part2, void *csr, void *off) {
+  Level L2;
+  for (const Coarse &q : cs) {
+    if (q.nbins == 0) continue;
+    const uint32_t W = choose_width(q.count, q.nbins, kb);
+    SegDesc s = range_seg(q.key_begin, q.key_begin + q.count, q.bin_lo, q.nbins, W, map);
+    if (s.nbuckets > kFMax) return fail(HG_ERR_UNSUPPORTED, "table too large or too skewed for two partition levels");
+    s.fin_off_base = q.off_base;
+    s.fin_key_rel = q.key_rel;
+    s.fin_end_pos = q.end_pos;
+    s.table = 0;
+    L2.segs.push_back(s);
+  }
+  if (L2.segs.empty()) return HG_OK;
+  L2.maps.push_back(map);
+  L2.prepare();
+  LevelDev d2;
+  HG_TRY(setup_level(c, L2, d2));
+  HG_TRY(run_level(c, L2, d2, kb, part1, part2));
+  return run_final(c, L2, d2, kb, ob, part2, csr, off);
+}
+
+// Driver of hg_build.
+hg_status build_table(hg_ctx c, const hg_table_desc *desc, const void *d_keys, void *d_off, void *d_csr) {
+  const uint64_t n = desc->n_keys, M = desc->n_bins;
+  const int kb = (int)desc->key_bytes, ob = (int)desc->offset_bytes;
+  const BinMap map = make_map(M, desc->hash_kind, desc->d_rmi_model);
+  if (n == 0) return fill_offsets(c, d_off, ob, M + 1, 0);
+  const uint32_t W = choose_width(n, M, kb);
+  const uint64_t F = (M + W - 1) / W;
+  if (F <= kFMax) {
+    Level L;
+    SegDesc s = range_seg(0, n, 0, M, W, map);
+    s.fin_off_base = 0;
+    s.fin_key_rel = 0;
+    s.fin_end_pos = M;
+    s.table = 0;
+    L.segs.push_back(s);
+    L.maps.push_back(map);
+    L.prepare();
+    Plan P0;
+    const size_t i_part = P0.add(n * kb);
+    HG_TRY(arena_ensure(c, 0, P0.total()));
+    Carve cv0;
+    cv0.init(c, 0, P0);
+    LevelDev d;
+    HG_TRY(setup_level(c, L, d));
+    void *part = cv0.get<void>(i_part);
+    HG_TRY(run_level(c, L, d, kb, d_keys, part));
+    return run_final(c, L, d, kb, ob, part, d_csr, d_off);
+  }
+  // Two levels: coarse super-buckets of k*W bins, their sizes read back, then
+  // every super-bucket is a segment of the second pass.
+  const uint64_t k = (F + kFMax - 1) / kFMax;
+  const uint64_t W1 = (uint64_t)W * k;
+  if (W1 >= (1ull << 32)) return fail(HG_ERR_UNSUPPORTED, "table geometry too large");
+  Level L1;
+  L1.segs.push_back(range_seg(0, n, 0, M, (uint32_t)W1, map));
+  L1.maps.push_back(map);
+  L1.prepare();
+  Plan P0;
+  const size_t i_part1 = P0.add(n * kb), i_part2 = P0.add(n * kb);
+  HG_TRY(arena_ensure(c, 0, P0.total()));
+  Carve cv0;
+  cv0.init(c, 0, P0);
+  LevelDev d1;
+  HG_TRY(setup_level(c, L1, d1));
+  HG_TRY(run_level(c, L1, d1, kb, d_keys, cv0.get<void>(i_part1)));
+  std::vector<uint32_t> cnt(L1.ncols_total), st(L1.ncols_total);
+  HG_TRY(sync_read(c, cnt.data(), d1.coltot, cnt.size() * 4));
+  HG_TRY(sync_read(c, st.data(), d1.bstart, st.size() * 4));
+  std::vector<Coarse> cs;
+  for (uint32_t g = 0; g < L1.ncols_total; g++) {
+    Coarse q;
+    q.key_begin = st[g];
+    q.count = cnt[g];
+    q.bin_lo = (uint64_t)g * W1;
+    q.nbins = std::min<uint64_t>(W1, M - q.bin_lo);
+    q.off_base = q.bin_lo;
+    q.key_rel = 0;
+    q.end_pos = (g + 1 == L1.ncols_total) ? M : ~0ull;
+    cs.push_back(q);
+  }
+  return finish_two_level(c, cs, map, kb, ob, cv0.get<void>(i_part1), cv0.get<void>(i_part2), d_csr, d_off);
+}
+
+uint64_t ref_binary_search(const std::vector<int64_t> &bins, uint64_t l, uint64_t r, uint64_t x) {
+  // include/MultiHashGraphHostOperators.cuh:8-27 (recursive, size_t indices)
+  for (;;) {
+    if (r >= l && r != (uint64_t)-1) {
+      const uint64_t mid = l + (r - l) / 2;
+      if ((uint64_t)bins[mid] == x) return mid;
+      if ((uint64_t)bins[mid] > x) {
+        if (mid == 0) return l;
+        r = mid - 1;
+        continue;
+      }
+      l = mid + 1;
+      continue;
+    }
+    return l;
+  }
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+const char *hg_last_error(void) { return g_err.c_str(); }
+const char *hg_version(void) { return "hashgraph_amd 0.1 (gfx950)"; }
+
+hg_status hg_ctx_create(int device, hg_ctx *out) {
+  if (!out) return fail(HG_ERR_INVALID_ARG, "null out");
+  int n = 0;
+  HG_HIP(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n) return fail(HG_ERR_INVALID_ARG, "no such device");
+  DeviceGuard g(device);
+  hg_ctx c = new hg_ctx_s();
+  c->device = device;
+  for (int i = 0; i < hg_ctx_s::kSlots; i++) {
+    if (hipEventCreateWithFlags(&c->hev[i], hipEventDisableTiming) != hipSuccess) {
+      hg_ctx_destroy(c);
+      return fail(HG_ERR_DEVICE, "hipEventCreate failed");
+    }
+  }
+  if (hipHostMalloc((void **)&c->h_small, 1 << 20, hipHostMallocDefault) != hipSuccess) {
+    hg_ctx_destroy(c);
+    return fail(HG_ERR_DEVICE, "hipHostMalloc failed");
+  }
+  static bool attrs_done[64] = {false};
+  if (device < 64 && !attrs_done[device]) {
+    if (set_kernel_attributes() != hipSuccess) {
+      hg_ctx_destroy(c);
+      return fail(HG_ERR_DEVICE, "cannot raise the LDS limit of the kernels");
+    }
+    attrs_done[device] = true;
+  }
+  *out = c;
+  return HG_OK;
+}
+
+hg_status hg_ctx_destroy(hg_ctx c) {
+  if (!c) return HG_OK;
+  DeviceGuard g(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  for (int a = 0; a < 2; a++)
+    if (c->arena[a]) (void)hipFree(c->arena[a]);
+  for (int i = 0; i < hg_ctx_s::kSlots; i++) {
+    if (c->hslot[i]) (void)hipHostFree(c->hslot[i]);
+    if (c->hev[i]) (void)hipEventDestroy(c->hev[i]);
+  }
+  if (c->h_small) (void)hipHostFree(c->h_small);
+  delete c;
+  return HG_OK;
+}
+
+hg_status hg_ctx_set_stream(hg_ctx c, void *stream) {
+  if (!c) return fail(HG_ERR_INVALID_ARG, "null ctx");
+  c->stream = (hipStream_t)stream;
+  return HG_OK;
+}
+
+hg_status hg_ctx_reserve(hg_ctx c, size_t bytes) {
+  if (!c) return fail(HG_ERR_INVALID_ARG, "null ctx");
+  DeviceGuard g(c->device);
+  HG_TRY(arena_ensure(c, 0, bytes / 2));
+  return arena_ensure(c, 1, bytes - bytes / 2);
+}
+
+hg_status hg_scratch_bytes(const hg_table_desc *desc, int op, size_t *bytes) {
+  HG_TRY(check_desc(desc));
+  if (!bytes) return fail(HG_ERR_INVALID_ARG, "null bytes");
+  const uint64_t n = desc->n_keys, M = desc->n_bins;
+  const int kb = (int)desc->key_bytes;
+  const uint32_t W = choose_width(n, M, kb);
+  const uint64_t F = std::min<uint64_t>((M + W - 1) / W, kFMax);
+  const uint64_t tiles = (n + kTile - 1) / kTile + 1;
+  (void)op;
+  *bytes = (size_t)n * kb * 2 + (size_t)(tiles * (F + 1) * 4) * 2 + (size_t)F * 64 + (size_t)tiles * 64 + (1 << 20);
+  return HG_OK;
+}
+
+hg_status hg_build(hg_ctx c, const hg_table_desc *desc, const void *d_keys, void *d_offsets, void *d_csr_keys) {
+  if (!c) return fail(HG_ERR_INVALID_ARG, "null ctx");
+  HG_TRY(check_desc(desc));
+  if (!d_offsets || (desc->n_keys && (!d_keys || !d_csr_keys))) return fail(HG_ERR_INVALID_ARG, "null buffer");
+  DeviceGuard g(c->device);
+  return build_table(c, desc, d_keys, d_offsets, d_csr_keys);
+}
+
+hg_status hg_hash_bins(hg_ctx c, const void *d_keys, uint64_t n, uint32_t key_bytes, uint64_t n_bins, uint32_t *d_bins) {
+  if (!c) return fail(HG_ERR_INVALID_ARG, "null ctx");
+  if (key_bytes != 4 && key_bytes != 8) return fail(HG_ERR_INVALID_ARG, "key_bytes must be 4 or 8");
+  if (n_bins == 0 || n_bins >= (1ull << 32)) return fail(HG_ERR_INVALID_ARG, "n_bins must be in [1, 2^32)");
+  if (n && (!d_keys || !d_bins)) return fail(HG_ERR_INVALID_ARG, "null buffer");
+  DeviceGuard g(c->device);
+  HG_HIP(launch_bins(d_keys, n, (int)key_bytes, make_map(n_bins, 0, nullptr), d_bins, c->stream));
+  return HG_OK;
+}
+
+hg_status hg_exclusive_scan(hg_ctx c, const void *d_in, void *d_out, uint64_t n, uint32_t elem_bytes) {
+  if (!c) return fail(HG_ERR_INVALID_ARG, "null ctx");
+  if (elem_bytes != 4 && elem_bytes != 8) return fail(HG_ERR_INVALID_ARG, "elem_bytes must be 4 or 8");
+  if (!d_out || (n && !d_in)) return fail(HG_ERR_INVALID_ARG, "null buffer");
+  DeviceGuard g(c->device);
+  if (n == 0) return fill_offsets(c, d_out, (int)elem_bytes, 1, 0);
+  Plan P;
+  const size_t i_status = P.add(scan_tiles(n) * 8);
+  const size_t i_ticket = P.add(64);
+  HG_TRY(arena_ensure(c, 1, P.total()));
+  Carve cv;
+  cv.init(c, 1, P);
+  HG_HIP(launch_scan(d_in, d_out, n, (int)elem_bytes, cv.get<unsigned long long>(i_status), cv.get<unsigned int>(i_ticket),
+                     c->stream));
+  return HG_OK;
+}
+
+hg_status hg_probe(hg_ctx c, const hg_table_desc *descA, const void *d_offA, const void *d_keysA, const void *d_probe,
+                   uint64_t n_probe, void *d_out, uint64_t capacity, uint64_t *h_count) {
+  if (!c) return fail(HG_ERR_INVALID_ARG, "null ctx");
+  HG_TRY(check_desc(descA));
+  if (!h_count) return fail(HG_ERR_INVALID_ARG, "null h_count");
+  if (!d_offA || (descA->n_keys && !d_keysA) || (n_probe && !d_probe)) return fail(HG_ERR_INVALID_ARG, "null buffer");
+  if (n_probe >= (1ull << 32)) return fail(HG_ERR_UNSUPPORTED, "n_probe >= 2^32 is not supported by this build");
+  DeviceGuard g(c->device);
+  *h_count = 0;
+  if (n_probe == 0 || descA->n_keys == 0) return HG_OK;
+  const uint64_t M = descA->n_bins;
+  const int kb = (int)descA->key_bytes, ob = (int)descA->offset_bytes;
+  const BinMap map = make_map(M, descA->hash_kind, descA->d_rmi_model);
+  const uint32_t W = choose_width(std::max<uint64_t>(n_probe, descA->n_keys), M, kb);
+  const uint64_t F = (M + W - 1) / W;
+  if (F > kFMax) return fail(HG_ERR_UNSUPPORTED, "probe geometry needs more than one partition level");
+  Level L;
+  SegDesc s = range_seg(0, n_probe, 0, M, W, map);
+  s.fin_off_base = 0;
+  s.fin_key_rel = 0;
+  s.table = 0;
+  L.segs.push_back(s);
+  L.maps.push_back(map);
+  L.prepare();
+  Plan P0;
+  const size_t i_part = P0.add(n_probe * kb);
+  const size_t i_small = P0.add(256);
+  HG_TRY(arena_ensure(c, 0, P0.total()));
+  Carve cv0;
+  cv0.init(c, 0, P0);
+  LevelDev d;
+  HG_TRY(setup_level(c, L, d));
+  void *part = cv0.get<void>(i_part);
+  unsigned long long *small = cv0.get<unsigned long long>(i_small);
+  HG_HIP(launch_fill_u64(small, 0ull, 4, c->stream));
+  HG_TRY(run_level(c, L, d, kb, d_probe, part));
+  HG_HIP(launch_make_items(d.segs, (uint32_t)L.segs.size(), d.bstart, d.coltot, d.fin, L.ncols_total, c->stream));
+  LaunchProbe pr{};
+  pr.key_bytes = kb;
+  pr.off_bytes = ob;
+  pr.nitems = L.ncols_total;
+  pr.akey_cap = probe_akey_cap(kb, ob);
+  pr.bpart = part;
+  pr.items = d.fin;
+  pr.maps = d.maps;
+  pr.offA = d_offA;
+  pr.keysA = d_keysA;
+  pr.out = d_out;
+  pr.capacity = d_out ? capacity : 0;
+  pr.cursor = small;
+  pr.flag = (unsigned int *)(small + 1);
+  HG_HIP(launch_probe(pr, c->stream));
+  unsigned long long res[2];
+  HG_TRY(sync_read(c, res, small, 16));
+  *h_count = res[0];
+  if (res[1] & 0xFFFFFFFFull)
+    return fail(HG_ERR_UNSUPPORTED, "a bin of A holds more keys than the probe walks; use hg_intersect_rle");
+  if (d_out && res[0] > capacity) return fail(HG_ERR_CAPACITY, "output capacity smaller than the match count");
+  return HG_OK;
+}
+
+hg_status hg_intersect(hg_ctx c, const hg_table_desc *descA, const void *d_offA, const void *d_keysA,
+                       const hg_table_desc *descB, const void *d_offB, const void *d_keysB, void *d_out,
+                       uint64_t capacity, uint64_t *h_count) {
+  if (!c) return fail(HG_ERR_INVALID_ARG, "null ctx");
+  HG_TRY(check_desc(descA));
+  HG_TRY(check_desc(descB));
+  if (!h_count) return fail(HG_ERR_INVALID_ARG, "null h_count");
+  if (descA->n_bins != descB->n_bins || descA->key_bytes != descB->key_bytes || descA->hash_kind != descB->hash_kind ||
+      descA->offset_bytes != descB->offset_bytes)
+    return fail(HG_ERR_MISMATCH, "tables do not share a bin space");
+  if (!d_offA || !d_offB || (descA->n_keys && !d_keysA) || (descB->n_keys && !d_keysB))
+    return fail(HG_ERR_INVALID_ARG, "null buffer");
+  DeviceGuard g(c->device);
+  *h_count = 0;
+  Plan P;
+  const size_t i_small = P.add(256);
+  HG_TRY(arena_ensure(c, 1, P.total()));
+  Carve cv;
+  cv.init(c, 1, P);
+  unsigned long long *small = cv.get<unsigned long long>(i_small);
+  HG_HIP(launch_fill_u64(small, 0ull, 4, c->stream));
+  LaunchJoin j{};
+  j.key_bytes = (int)descA->key_bytes;
+  j.off_bytes = (int)descA->offset_bytes;
+  j.M = descA->n_bins;
+  j.key_cap = join_key_cap(j.key_bytes, j.off_bytes);
+  j.offA = d_offA;
+  j.keysA = d_keysA;
+  j.offB = d_offB;
+  j.keysB = d_keysB;
+  j.out = d_out;
+  j.capacity = d_out ? capacity : 0;
+  j.cursor = small;
+  j.flag = (unsigned int *)(small + 1);
+  HG_HIP(launch_join(j, c->stream));
+  unsigned long long res[2];
+  HG_TRY(sync_read(c, res, small, 16));
+  *h_count = res[0];
+  if (res[1] & 0xFFFFFFFFull)
+    return fail(HG_ERR_UNSUPPORTED, "a bin pair is too large for the nested join; use hg_intersect_rle");
+  if (d_out && res[0] > capacity) return fail(HG_ERR_CAPACITY, "output capacity smaller than the match count");
+  return HG_OK;
+}
+
+hg_status hg_find_splits(hg_ctx c, const hg_table_desc *desc, const void *d_keys, uint64_t bin_count, uint32_t n_parts,
+                         int64_t *h_splits) {
+  if (!c) return fail(HG_ERR_INVALID_ARG, "null ctx");
+  HG_TRY(check_desc(desc));
+  if (!h_splits || n_parts == 0 || bin_count == 0 || bin_count > 32768)
+    return fail(HG_ERR_INVALID_ARG, "bad split arguments (bin_count must be in [1, 32768])");
+  if (desc->n_keys && !d_keys) return fail(HG_ERR_INVALID_ARG, "null keys");
+  DeviceGuard g(c->device);
+  const uint64_t M = desc->n_bins, n = desc->n_keys;
+  // binRange = ceil(tableSize / (float)binCount)  (src/MultiHashGraph.cu:410)
+  const uint64_t range = (uint64_t)std::ceil((float)M / (float)bin_count);
+  if (range == 0) return fail(HG_ERR_INVALID_ARG, "bin range is zero");
+  Plan P;
+  const size_t i_hist = P.add(bin_count * 8 + 64);
+  HG_TRY(arena_ensure(c, 1, P.total()));
+  Carve cv;
+  cv.init(c, 1, P);
+  unsigned long long *hist = cv.get<unsigned long long>(i_hist);
+  unsigned int *bad = (unsigned int *)(hist + bin_count);
+  HG_HIP(launch_fill_u64(hist, 0ull, bin_count + 1, c->stream));
+  HG_HIP(launch_range_hist(d_keys, n, (int)desc->key_bytes, make_map(M, desc->hash_kind, desc->d_rmi_model),
+                           (uint32_t)range, (uint32_t)bin_count, hist, bad, c->stream));
+  std::vector<unsigned long long> h(bin_count + 1);
+  HG_TRY(sync_read(c, h.data(), hist, (bin_count + 1) * 8));
+  if (h[bin_count]) return fail(HG_ERR_INVALID_ARG, "bin range too small for bin_count (float rounding)");
+  // prefix as in include/MultiHashGraphHostOperators.cuh:55-59
+  std::vector<int64_t> ps(bin_count + 1);
+  ps[0] = 0;
+  for (uint64_t i = 1; i < bin_count; i++) ps[i] = ps[i - 1] + (int64_t)h[i - 1];
+  ps[bin_count] = (int64_t)n;
+  // avgKeyCount = ceil(countSize / (float)gpuCount)  (HostOperators.cuh:83)
+  const int64_t avg = (int64_t)std::ceil((float)n / (float)n_parts);
+  h_splits[0] = 0;
+  for (uint32_t t = 0; t < n_parts; t++) {
+    const uint64_t upper = (uint64_t)(avg * (int64_t)(t + 1));
+    const uint64_t idx = ref_binary_search(ps, 0, bin_count, upper);
+    const uint64_t minRange = idx * range;
+    h_splits[t + 1] = (int64_t)std::min<uint64_t>(minRange, M);
+  }
+  return HG_OK;
+}
+
+hg_status hg_build_partitioned(hg_ctx c, const hg_table_desc *desc, const void *d_keys, const int64_t *h_splits,
+                               uint32_t n_parts, void *d_offsets, void *d_csr_keys, uint64_t *h_part_offsets) {
+  if (!c) return fail(HG_ERR_INVALID_ARG, "null ctx");
+  HG_TRY(check_desc(desc));
+  if (!h_splits || n_parts == 0 || n_parts > 4096 || !d_offsets || !h_part_offsets)
+    return fail(HG_ERR_INVALID_ARG, "bad partition arguments");
+  const uint64_t M = desc->n_bins, n = desc->n_keys;
+  if (h_splits[0] != 0) return fail(HG_ERR_INVALID_ARG, "splits[0] must be 0");
+  for (uint32_t p = 0; p < n_parts; p++)
+    if (h_splits[p + 1] < h_splits[p] || (uint64_t)h_splits[p + 1] > M)
+      return fail(HG_ERR_INVALID_ARG, "splits must be non-decreasing within [0, M]");
+  if (n && (!d_keys || !d_csr_keys)) return fail(HG_ERR_INVALID_ARG, "null buffer");
+  DeviceGuard g(c->device);
+  const int kb = (int)desc->key_bytes, ob = (int)desc->offset_bytes;
+  const BinMap map = make_map(M, desc->hash_kind, desc->d_rmi_model);
+  Plan P0;
+  const size_t i_part1 = P0.add(std::max<uint64_t>(n, 1) * kb), i_part2 = P0.add(std::max<uint64_t>(n, 1) * kb);
+  HG_TRY(arena_ensure(c, 0, P0.total()));
+  Carve cv0;
+  cv0.init(c, 0, P0);
+  void *part1 = cv0.get<void>(i_part1), *part2 = cv0.get<void>(i_part2);
+  // level 1: the local partition by split ranges (a kind-1 segment)
+  Level L1;
+  SegDesc s1{};
+  s1.key_begin = 0;
+  s1.key_end = n;
+  s1.map = map;
+  s1.div = make_div(1);
+  s1.kind = 1;
+  s1.nbuckets = n_parts;
+  s1.ncols = n_parts + 1;
+  s1.nsplits = n_parts;
+  s1.fin_end_pos = ~0ull;
+  L1.segs.push_back(s1);
+  L1.maps.push_back(map);
+  L1.splits.assign(h_splits, h_splits + n_parts + 1);
+  L1.prepare();
+  LevelDev d1;
+  HG_TRY(setup_level(c, L1, d1));
+  HG_TRY(run_level(c, L1, d1, kb, d_keys, part1));
+  std::vector<uint32_t> cnt(L1.ncols_total), st(L1.ncols_total);
+  HG_TRY(sync_read(c, cnt.data(), d1.coltot, cnt.size() * 4));
+  HG_TRY(sync_read(c, st.data(), d1.bstart, st.size() * 4));
+  std::vector<Coarse> cs;
+  uint64_t kept = 0;
+  for (uint32_t p = 0; p < n_parts; p++) {
+    const uint64_t lo = (uint64_t)h_splits[p], R = (uint64_t)h_splits[p + 1] - lo;
+    h_part_offsets[p] = kept;
+    if (R == 0) {  // empty range: its single offset entry is 0
+      HG_TRY(fill_offsets(c, (char *)d_offsets + (lo + p) * ob, ob, 1, 0));
+      continue;
+    }
+    Coarse q;
+    q.key_begin = st[p];
+    q.count = cnt[p];
+    q.bin_lo = lo;
+    q.nbins = R;
+    q.off_base = lo + p;
+    q.key_rel = st[p];
+    q.end_pos = lo + R + p;
+    cs.push_back(q);
+    kept += cnt[p];
+  }
+  h_part_offsets[n_parts] = kept;
+  return finish_two_level(c, cs, map, kb, ob, part1, part2, d_csr_keys, d_offsets);
+}
+
+}  // extern "C"

@@ -1,0 +1,923 @@
+// hg_kernels.hip -- gfx950 kernels of the HashGraph build / probe / join.
+//
+// Build = partition pass (hash -> coarse bucket, per-tile LDS histogram,
+// exact count matrix, column scan, LDS-staged coalesced scatter) followed by
+// the final stage (one workgroup per bucket: keys held in VGPRs, packed u16
+// LDS counters give the bucket's CSR offsets and every key's slot, keys are
+// staged through LDS and written coalesced).  Replaces the reference's LRB
+// build (src/SingleHashGraph.cu:129-181, kernels
+// include/SingleHashGraphOperators.cuh:71-154), whose random global atomics
+// are replaced by LDS work and deterministic positions.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "hg_common.hpp"
+#include "hg_device.hpp"
+#include "hg_kernels.hpp"
+
+namespace hg {
+
+// A probe key walks at most this many keys of its bin, and a join bin at
+// most this many key pairs; larger bins (heavy duplicates) are reported to
+// the host, which answers them with the run-length join instead.
+constexpr uint64_t PROBE_BIN_LIMIT = 4096;
+constexpr uint64_t JOIN_PAIR_LIMIT = 1ull << 22;
+
+// Keys per thread that the final stage and the probe keep in VGPRs (u32
+// keys; u64 keys use 32).  1024 threads x KPT is the bucket capacity.
+#ifndef HG_KPT_U32
+#define HG_KPT_U32 32
+#endif
+constexpr int KPT_U32 = HG_KPT_U32;
+
+// ---------------------------------------------------------------------------
+// Partition pass: histogram.  One workgroup per tile; tile t of segment s
+// writes row t of the segment's count matrix (ncols u32).
+// ---------------------------------------------------------------------------
+template <typename K, int BLOCK, int KPT>
+__global__ __launch_bounds__(BLOCK) void k_part_hist(const K *__restrict__ keys, const TileDesc *__restrict__ tiles,
+                                                     const SegDesc *__restrict__ segs, uint32_t *__restrict__ cmat) {
+  extern __shared__ uint32_t s_cnt[];
+  const TileDesc td = tiles[blockIdx.x];
+  const SegDesc &sd = segs[td.seg];
+  const uint32_t ncols = sd.ncols;
+  for (uint32_t i = threadIdx.x; i < ncols; i += BLOCK) s_cnt[i] = 0;
+  __syncthreads();
+  const K *kp = keys + td.key_start;
+  // number of valid slots of this thread (slot j holds key j*BLOCK + tid)
+  const uint32_t nv = td.count > threadIdx.x ? (td.count - threadIdx.x + BLOCK - 1) / BLOCK : 0;
+  const __amdgpu_buffer_rsrc_t rs = make_rsrc(kp, td.count * (uint32_t)sizeof(K));
+  K kr[KPT];
+#pragma unroll
+  for (int j = 0; j < KPT; j++) kr[j] = buf_load<K>(rs, threadIdx.x * (uint32_t)sizeof(K), j * BLOCK * (uint32_t)sizeof(K));
+#pragma unroll
+  for (int j = 0; j < KPT; j++)
+    if (j < nv) atomicAdd(&s_cnt[bucket_of<K>(kr[j], sd)], 1u);
+  __syncthreads();
+  uint32_t *row = cmat + sd.row_base + (uint64_t)(blockIdx.x - sd.tile_begin) * ncols;
+  for (uint32_t i = threadIdx.x; i < ncols; i += BLOCK) row[i] = s_cnt[i];
+}
+
+// ---------------------------------------------------------------------------
+// Column scan of the count matrix (bucket-major order inside a segment):
+//   pos[t][b] = bucket_start[b] + sum_{t' < t} C[t'][b]
+// in three steps over chunks of CHUNK rows.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_col_chunksum(const uint32_t *__restrict__ cmat, const ColItem *__restrict__ items,
+                                                      const SegDesc *__restrict__ segs, const uint64_t *__restrict__ psum_base,
+                                                      uint32_t *__restrict__ psum) {
+  const ColItem it = items[blockIdx.x];
+  const SegDesc &sd = segs[it.seg];
+  const uint32_t b = it.col_lo + threadIdx.x;
+  if (b >= sd.ncols) return;
+  const uint32_t *p = cmat + sd.row_base + (uint64_t)it.row_lo * sd.ncols + b;
+  uint32_t s = 0;
+  for (uint32_t r = it.row_lo; r < it.row_hi; r++, p += sd.ncols) s += *p;
+  psum[psum_base[it.seg] + (uint64_t)it.chunk * sd.ncols + b] = s;
+}
+
+// Per column block of a segment: exclusive scan of the chunk sums down each
+// column; the column total goes to coltot[bucket_base + b].
+__global__ __launch_bounds__(256) void k_col_chunkscan(const ColItem *__restrict__ items, const SegDesc *__restrict__ segs,
+                                                       const uint64_t *__restrict__ psum_base,
+                                                       const uint32_t *__restrict__ nchunks, uint32_t *__restrict__ psum,
+                                                       uint32_t *__restrict__ coltot) {
+  const ColItem it = items[blockIdx.x];
+  const SegDesc &sd = segs[it.seg];
+  const uint32_t b = it.col_lo + threadIdx.x;
+  if (b >= sd.ncols) return;
+  const uint32_t nch = nchunks[it.seg];
+  uint32_t *p = psum + psum_base[it.seg] + b;
+  uint32_t run = 0;
+  for (uint32_t c = 0; c < nch; c++, p += sd.ncols) {
+    const uint32_t v = *p;
+    *p = run;
+    run += v;
+  }
+  coltot[sd.bucket_base + b] = run;
+}
+
+// One workgroup per segment: bucket starts = key_begin + exclusive scan of
+// the column totals (the drop column of a split segment gets no start).
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_seg_bucketscan(const SegDesc *__restrict__ segs, const uint32_t *__restrict__ coltot,
+                                                          uint32_t *__restrict__ bstart) {
+  __shared__ uint32_t s_tmp[BLOCK / 64 + 1];
+  const SegDesc &sd = segs[blockIdx.x];
+  const uint32_t ncols = sd.ncols, nb = sd.nbuckets;
+  const uint32_t per = (ncols + BLOCK - 1) / BLOCK;
+  const uint32_t c0 = min(threadIdx.x * per, ncols), c1 = min(c0 + per, ncols);
+  uint32_t local = 0;
+  for (uint32_t b = c0; b < c1; b++)
+    if (b < nb) local += coltot[sd.bucket_base + b];
+  uint32_t tot;
+  uint32_t base = (uint32_t)sd.key_begin + block_exclusive_sum<BLOCK, uint32_t>(local, s_tmp, tot);
+  for (uint32_t b = c0; b < c1; b++) {
+    if (b < nb) {
+      const uint32_t v = coltot[sd.bucket_base + b];
+      bstart[sd.bucket_base + b] = base;
+      base += v;
+    } else {
+      bstart[sd.bucket_base + b] = 0xFFFFFFFFu;
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_col_apply(uint32_t *__restrict__ cmat, const ColItem *__restrict__ items,
+                                                   const SegDesc *__restrict__ segs, const uint64_t *__restrict__ psum_base,
+                                                   const uint32_t *__restrict__ psum, const uint32_t *__restrict__ bstart) {
+  const ColItem it = items[blockIdx.x];
+  const SegDesc &sd = segs[it.seg];
+  const uint32_t b = it.col_lo + threadIdx.x;
+  if (b >= sd.ncols) return;
+  uint32_t base = bstart[sd.bucket_base + b] + psum[psum_base[it.seg] + (uint64_t)it.chunk * sd.ncols + b];
+  uint32_t *p = cmat + sd.row_base + (uint64_t)it.row_lo * sd.ncols + b;
+  for (uint32_t r = it.row_lo; r < it.row_hi; r++, p += sd.ncols) {
+    uint32_t v = *p;
+    *p = base;
+    base += v;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Partition pass: scatter.  Recounts the tile in LDS (rank of every key in
+// its bucket), sorts the tile by bucket through an LDS stage and writes each
+// (tile, bucket) run contiguously at its count-matrix position.
+// Requires tile count <= 65536 and ncols <= 65536.
+// ---------------------------------------------------------------------------
+template <typename K, int BLOCK, int KPT>
+__global__ __launch_bounds__(BLOCK) void k_part_scatter(const K *__restrict__ keys, const TileDesc *__restrict__ tiles,
+                                                        const SegDesc *__restrict__ segs, const uint32_t *__restrict__ cpos,
+                                                        K *__restrict__ out, uint32_t stage_cap) {
+  extern __shared__ __align__(16) uint8_t smem[];
+  __shared__ uint32_t s_tmp[BLOCK / 64 + 1];
+  const TileDesc td = tiles[blockIdx.x];
+  const SegDesc &sd = segs[td.seg];
+  const uint32_t ncols = sd.ncols;
+  uint32_t *s_cnt = (uint32_t *)smem;
+  const uint32_t cnt_words = (ncols + 3) & ~3u;
+  K *s_key = (K *)(smem + cnt_words * 4);
+  uint16_t *s_bkt = (uint16_t *)(s_key + stage_cap);
+  for (uint32_t i = threadIdx.x; i < ncols; i += BLOCK) s_cnt[i] = 0;
+  __syncthreads();
+  const K *kp = keys + td.key_start;
+  const uint32_t nv = td.count > threadIdx.x ? (td.count - threadIdx.x + BLOCK - 1) / BLOCK : 0;
+  const __amdgpu_buffer_rsrc_t rs = make_rsrc(kp, td.count * (uint32_t)sizeof(K));
+  K kr[KPT];
+  uint32_t pk[KPT];
+#pragma unroll
+  for (int j = 0; j < KPT; j++) kr[j] = buf_load<K>(rs, threadIdx.x * (uint32_t)sizeof(K), j * BLOCK * (uint32_t)sizeof(K));
+#pragma unroll
+  for (int j = 0; j < KPT; j++) {
+    if (j < nv) {
+      const uint32_t b = bucket_of<K>(kr[j], sd);
+      const uint32_t r = atomicAdd(&s_cnt[b], 1u);
+      pk[j] = (b << 16) | r;
+    }
+  }
+  __syncthreads();
+  lds_exclusive_scan<BLOCK>(s_cnt, ncols, s_tmp);
+#pragma unroll
+  for (int j = 0; j < KPT; j++) {
+    if (j < nv) {
+      const uint32_t b = pk[j] >> 16;
+      pk[j] = (b << 16) | (s_cnt[b] + (pk[j] & 0xFFFFu));
+    }
+  }
+  __syncthreads();
+  const uint32_t *row = cpos + sd.row_base + (uint64_t)(blockIdx.x - sd.tile_begin) * ncols;
+  for (uint32_t b = threadIdx.x; b < ncols; b += BLOCK) s_cnt[b] = row[b] - s_cnt[b];
+  __syncthreads();
+  const uint32_t nb = sd.nbuckets;
+  for (uint32_t r0 = 0; r0 < td.count; r0 += stage_cap) {
+#pragma unroll
+    for (int j = 0; j < KPT; j++) {
+      if (j < nv) {
+        const uint32_t q = (pk[j] & 0xFFFFu) - r0;
+        if (q < stage_cap) {
+          s_key[q] = kr[j];
+          s_bkt[q] = (uint16_t)(pk[j] >> 16);
+        }
+      }
+    }
+    __syncthreads();
+    const uint32_t n = min(stage_cap, td.count - r0);
+    for (uint32_t q = threadIdx.x; q < n; q += BLOCK) {
+      const uint32_t b = s_bkt[q];
+      if (b < nb) out[(uint32_t)(s_cnt[b] + r0 + q)] = s_key[q];
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Final stage: one workgroup per bucket (count <= BLOCK*KPT, count <= 65535,
+// nbins <= 65535).  Keys stay in VGPRs; the bucket-local bin (16 bit) is
+// computed once and kept packed.  LDS holds nbins u16 counters (two per
+// word); their exclusive scan gives the offsets, a second atomic pass gives
+// every key's slot, then LDS is reused to stage the bin-ordered keys.
+// ---------------------------------------------------------------------------
+template <typename K, typename OFF, int BLOCK, int KPT>
+__global__ __launch_bounds__(BLOCK) void k_final(const K *__restrict__ part, K *__restrict__ csr, OFF *__restrict__ off,
+                                                 const FinItem *__restrict__ items, const BinMap *__restrict__ maps,
+                                                 uint32_t lds_bytes) {
+  extern __shared__ __align__(16) uint32_t smem32[];
+  __shared__ uint32_t s_tmp[BLOCK / 64 + 1];
+  const FinItem it = items[blockIdx.x];
+  if (it.count > BLOCK * KPT || it.count > 65535u) return;  // k_final_big's item
+  const BinMap mp = maps[it.table];
+  const uint32_t W = it.nbins, cnt = it.count;
+  const uint32_t nw = (W + 1) >> 1;
+  for (uint32_t i = threadIdx.x; i < nw; i += BLOCK) smem32[i] = 0;
+  __syncthreads();
+  const K *kp = part + it.key_start;
+  const uint32_t nv = cnt > threadIdx.x ? (cnt - threadIdx.x + BLOCK - 1) / BLOCK : 0;
+  const __amdgpu_buffer_rsrc_t rs = make_rsrc(kp, cnt * (uint32_t)sizeof(K));
+  K kr[KPT];
+  uint32_t lb[(KPT + 1) / 2];
+#pragma unroll
+  for (int j = 0; j < KPT; j++) kr[j] = buf_load<K>(rs, threadIdx.x * (uint32_t)sizeof(K), j * BLOCK * (uint32_t)sizeof(K));
+#pragma unroll
+  for (int j = 0; j < (KPT + 1) / 2; j++) lb[j] = 0;
+#pragma unroll
+  for (int j = 0; j < KPT; j++) {
+    if (j < nv) {
+      const uint32_t l = bin_of<K>(kr[j], mp) - it.bin_lo;
+      lb[j >> 1] |= l << ((j & 1) * 16);
+      atomicAdd(&smem32[l >> 1], 1u << ((l & 1) * 16));
+    }
+  }
+  __syncthreads();
+  // exclusive scan of the u16 counts (bucket total <= 65535, so every
+  // prefix fits 16 bits and the packed halves never carry into each other)
+  {
+    const uint32_t per = (nw + BLOCK - 1) / BLOCK;
+    const uint32_t w0 = min(threadIdx.x * per, nw), w1 = min(w0 + per, nw);
+    uint32_t s = 0;
+    for (uint32_t w = w0; w < w1; w++) {
+      const uint32_t v = smem32[w];
+      s += (v & 0xFFFFu) + (v >> 16);
+    }
+    uint32_t tot;
+    uint32_t ex = block_exclusive_sum<BLOCK, uint32_t>(s, s_tmp, tot);
+    for (uint32_t w = w0; w < w1; w++) {
+      const uint32_t v = smem32[w];
+      const uint32_t lo = v & 0xFFFFu;
+      smem32[w] = ex | ((ex + lo) << 16);
+      ex += lo + (v >> 16);
+    }
+  }
+  __syncthreads();
+  const uint64_t obase = it.key_start - it.key_rel;
+  for (uint32_t l = threadIdx.x; l < W; l += BLOCK) {
+    const uint32_t pre = (smem32[l >> 1] >> ((l & 1) * 16)) & 0xFFFFu;
+    off[it.off_pos + l] = (OFF)(obase + pre);
+  }
+  if (it.end_pos != ~0ull && threadIdx.x == 0) off[it.end_pos] = (OFF)(obase + cnt);
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < KPT; j++) {
+    if (j < nv) {
+      const uint32_t sh = (j & 1) * 16;
+      const uint32_t l = (lb[j >> 1] >> sh) & 0xFFFFu;
+      const uint32_t old = atomicAdd(&smem32[l >> 1], 1u << ((l & 1) * 16));
+      const uint32_t pos = (old >> ((l & 1) * 16)) & 0xFFFFu;
+      lb[j >> 1] = (lb[j >> 1] & ~(0xFFFFu << sh)) | (pos << sh);
+    }
+  }
+  __syncthreads();
+  K *st = (K *)smem32;
+  const uint32_t cap = lds_bytes / sizeof(K);
+  K *dst = csr + it.key_start;
+  for (uint32_t r0 = 0; r0 < cnt; r0 += cap) {
+#pragma unroll
+    for (int j = 0; j < KPT; j++) {
+      if (j < nv) {
+        const uint32_t q = ((lb[j >> 1] >> ((j & 1) * 16)) & 0xFFFFu) - r0;
+        if (q < cap) st[q] = kr[j];
+      }
+    }
+    __syncthreads();
+    const uint32_t n = min(cap, cnt - r0);
+    for (uint32_t q = threadIdx.x; q < n; q += BLOCK) dst[r0 + q] = st[q];
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Final stage for buckets too large for k_final (skewed keys).  One
+// workgroup streams the bucket: per sub-range of SUB bins, count in LDS,
+// scan, write offsets, stream again placing keys with LDS atomics.
+// ---------------------------------------------------------------------------
+template <typename K, typename OFF, int BLOCK, int SUB>
+__global__ __launch_bounds__(BLOCK) void k_final_big(const K *__restrict__ part, K *__restrict__ csr, OFF *__restrict__ off,
+                                                     const FinItem *__restrict__ items, const BinMap *__restrict__ maps,
+                                                     uint32_t small_cap) {
+  __shared__ uint32_t s_cnt[SUB];
+  __shared__ uint32_t s_tmp[BLOCK / 64 + 1];
+  const FinItem it = items[blockIdx.x];
+  if (it.count <= small_cap) return;  // k_final's item
+  const BinMap mp = maps[it.table];
+  const uint64_t cnt = it.count;
+  const K *kp = part + it.key_start;
+  K *dst = csr + it.key_start;
+  const uint64_t obase = it.key_start - it.key_rel;
+  uint64_t done = 0;
+  for (uint32_t s0 = 0; s0 < it.nbins; s0 += SUB) {
+    const uint32_t n = min((uint32_t)SUB, it.nbins - s0);
+    for (uint32_t i = threadIdx.x; i < n; i += BLOCK) s_cnt[i] = 0;
+    __syncthreads();
+    for (uint64_t i = threadIdx.x; i < cnt; i += BLOCK) {
+      const uint32_t l = bin_of<K>(kp[i], mp) - it.bin_lo - s0;
+      if (l < n) atomicAdd(&s_cnt[l], 1u);
+    }
+    __syncthreads();
+    const uint32_t tot = lds_exclusive_scan<BLOCK>(s_cnt, n, s_tmp);
+    for (uint32_t l = threadIdx.x; l < n; l += BLOCK) off[it.off_pos + s0 + l] = (OFF)(obase + done + s_cnt[l]);
+    __syncthreads();
+    for (uint64_t i = threadIdx.x; i < cnt; i += BLOCK) {
+      const K k = kp[i];
+      const uint32_t l = bin_of<K>(k, mp) - it.bin_lo - s0;
+      if (l < n) dst[done + atomicAdd(&s_cnt[l], 1u)] = k;
+    }
+    __syncthreads();
+    done += tot;
+  }
+  if (it.end_pos != ~0ull && threadIdx.x == 0) off[it.end_pos] = (OFF)(obase + cnt);
+}
+
+// ---------------------------------------------------------------------------
+// Probe: B's keys of one bucket stay in VGPRs; A's offsets and keys of each
+// sub-range of SUB bins are staged in LDS; every B key walks its A bin.
+// Matches are emitted per sub-range through one atomic reservation per
+// workgroup (output grouped by bucket and sub-range).
+// ---------------------------------------------------------------------------
+template <typename K, typename OFF, int BLOCK, int KPT, int SUB>
+__global__ __launch_bounds__(BLOCK) void k_probe(const K *__restrict__ bpart, const FinItem *__restrict__ items,
+                                                 const BinMap *__restrict__ maps, const OFF *__restrict__ offA,
+                                                 const K *__restrict__ keysA, K *__restrict__ out, uint64_t capacity,
+                                                 unsigned long long *__restrict__ cursor, uint32_t akey_cap,
+                                                 unsigned int *__restrict__ flag) {
+  extern __shared__ __align__(16) uint8_t smem[];
+  __shared__ uint32_t s_tmp[BLOCK / 64 + 1];
+  __shared__ unsigned long long s_base;
+  OFF *s_off = (OFF *)smem;
+  K *s_ak = (K *)(smem + ((SUB + 2) * sizeof(OFF) + 15) / 16 * 16);
+  const FinItem it = items[blockIdx.x];
+  if (it.count > BLOCK * KPT || it.count > 65535u) return;
+  const BinMap mp = maps[it.table];
+  const uint32_t cnt = it.count, W = it.nbins;
+  const K *kp = bpart + it.key_start;
+  const uint32_t nv = cnt > threadIdx.x ? (cnt - threadIdx.x + BLOCK - 1) / BLOCK : 0;
+  const __amdgpu_buffer_rsrc_t rs = make_rsrc(kp, cnt * (uint32_t)sizeof(K));
+  K kr[KPT];
+  uint32_t lb[(KPT + 1) / 2];
+#pragma unroll
+  for (int j = 0; j < KPT; j++) kr[j] = buf_load<K>(rs, threadIdx.x * (uint32_t)sizeof(K), j * BLOCK * (uint32_t)sizeof(K));
+#pragma unroll
+  for (int j = 0; j < (KPT + 1) / 2; j++) lb[j] = 0;
+#pragma unroll
+  for (int j = 0; j < KPT; j++) {
+    if (j < nv) lb[j >> 1] |= (bin_of<K>(kr[j], mp) - it.bin_lo) << ((j & 1) * 16);
+    if ((j & 3) == 3) __builtin_amdgcn_sched_barrier(0);
+  }
+  unsigned long long my_total = 0;
+  for (uint32_t s0 = 0; s0 < W; s0 += SUB) {
+    const uint32_t n = min((uint32_t)SUB, W - s0);
+    for (uint32_t q = threadIdx.x; q <= n; q += BLOCK) s_off[q] = offA[it.off_pos + s0 + q];
+    __syncthreads();
+    const OFF a0 = s_off[0];
+    const uint64_t na = (uint64_t)(s_off[n] - a0);
+    const bool in_lds = na <= akey_cap;
+    if (in_lds)
+      for (uint32_t q = threadIdx.x; q < na; q += BLOCK) s_ak[q] = keysA[a0 + q];
+    __syncthreads();
+    uint32_t mine = 0;
+#pragma unroll
+    for (int j = 0; j < KPT; j++) {
+      if (j < nv) {
+        const uint32_t l = ((lb[j >> 1] >> ((j & 1) * 16)) & 0xFFFFu) - s0;
+        if (l < n) {
+          const uint64_t e = (uint64_t)(s_off[l + 1] - a0), b0 = (uint64_t)(s_off[l] - a0);
+          if (e - b0 > PROBE_BIN_LIMIT) {
+            atomicOr(flag, 1u);
+          } else {
+            for (uint64_t q = b0; q < e; q++) mine += (in_lds ? s_ak[q] : keysA[a0 + q]) == kr[j];
+          }
+        }
+      }
+    }
+    if (out != nullptr) {
+      uint32_t tot;
+      uint32_t ex = block_exclusive_sum<BLOCK, uint32_t>(mine, s_tmp, tot);
+      if (threadIdx.x == 0) s_base = tot ? atomicAdd(cursor, (unsigned long long)tot) : 0ull;
+      __syncthreads();
+      const unsigned long long base = s_base + ex;
+      if (mine) {
+        unsigned long long w = base;
+#pragma unroll
+        for (int j = 0; j < KPT; j++) {
+          if (j < nv) {
+            const uint32_t l = ((lb[j >> 1] >> ((j & 1) * 16)) & 0xFFFFu) - s0;
+            if (l < n) {
+              const uint64_t e = (uint64_t)(s_off[l + 1] - a0), b0 = (uint64_t)(s_off[l] - a0);
+              if (e - b0 > PROBE_BIN_LIMIT) continue;
+              for (uint64_t q = b0; q < e; q++) {
+                if ((in_lds ? s_ak[q] : keysA[a0 + q]) == kr[j]) {
+                  if (w < capacity) out[w] = kr[j];
+                  w++;
+                }
+              }
+            }
+          }
+        }
+      }
+    } else {
+      my_total += mine;
+    }
+    __syncthreads();
+  }
+  if (out == nullptr) {
+    __shared__ unsigned long long s_t64[BLOCK / 64 + 1];
+    unsigned long long t = block_sum<BLOCK, unsigned long long>(my_total, s_t64);
+    if (threadIdx.x == 0 && t) atomicAdd(cursor, t);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Join of two built tables over a range of bins (MultiHashGraph::intersect,
+// kernel simpleIntersect include/MultiHashGraphDeviceOperators.cuh:513-543):
+// every (a in A_b, x in B_b) with a == x counts and emits x.  One workgroup
+// per JB bins; A's and B's keys of the range are staged in LDS when they fit.
+// ---------------------------------------------------------------------------
+template <typename K, typename OFF, int BLOCK, int JB>
+__global__ __launch_bounds__(BLOCK) void k_join(const OFF *__restrict__ offA, const K *__restrict__ keysA,
+                                                const OFF *__restrict__ offB, const K *__restrict__ keysB, uint64_t M,
+                                                K *__restrict__ out, uint64_t capacity,
+                                                unsigned long long *__restrict__ cursor, uint32_t key_cap,
+                                                unsigned int *__restrict__ flag) {
+  extern __shared__ __align__(16) uint8_t smem[];
+  __shared__ OFF s_oa[JB + 1], s_ob[JB + 1];
+  __shared__ unsigned long long s_base;
+  K *s_a = (K *)smem;
+  K *s_b = s_a + key_cap;
+  const uint64_t b0 = (uint64_t)blockIdx.x * JB;
+  const uint32_t n = (uint32_t)min((uint64_t)JB, M - b0);
+  for (uint32_t q = threadIdx.x; q <= n; q += BLOCK) {
+    s_oa[q] = offA[b0 + q];
+    s_ob[q] = offB[b0 + q];
+  }
+  __syncthreads();
+  const OFF a0 = s_oa[0], x0 = s_ob[0];
+  const uint64_t na = (uint64_t)(s_oa[n] - a0), nbk = (uint64_t)(s_ob[n] - x0);
+  const bool a_lds = na <= key_cap, b_lds = nbk <= key_cap;
+  if (a_lds)
+    for (uint32_t q = threadIdx.x; q < na; q += BLOCK) s_a[q] = keysA[a0 + q];
+  if (b_lds)
+    for (uint32_t q = threadIdx.x; q < nbk; q += BLOCK) s_b[q] = keysB[x0 + q];
+  __syncthreads();
+  uint64_t mine = 0;
+  for (uint32_t l = threadIdx.x; l < n; l += BLOCK) {
+    const uint64_t ea = (uint64_t)(s_oa[l + 1] - a0), eb = (uint64_t)(s_ob[l + 1] - x0);
+    if ((ea - (uint64_t)(s_oa[l] - a0)) * (eb - (uint64_t)(s_ob[l] - x0)) > JOIN_PAIR_LIMIT) {
+      atomicOr(flag, 1u);
+      continue;
+    }
+    for (uint64_t ia = (uint64_t)(s_oa[l] - a0); ia < ea; ia++) {
+      const K a = a_lds ? s_a[ia] : keysA[a0 + ia];
+      for (uint64_t ib = (uint64_t)(s_ob[l] - x0); ib < eb; ib++) mine += (b_lds ? s_b[ib] : keysB[x0 + ib]) == a;
+    }
+  }
+  if (out == nullptr) {
+    __shared__ unsigned long long s_t64[BLOCK / 64 + 1];
+    unsigned long long t = block_sum<BLOCK, unsigned long long>(mine, s_t64);
+    if (threadIdx.x == 0 && t) atomicAdd(cursor, t);
+    return;
+  }
+  __shared__ unsigned long long s_t64[BLOCK / 64 + 1];
+  unsigned long long tot;
+  unsigned long long ex = block_exclusive_sum<BLOCK, unsigned long long>(mine, s_t64, tot);
+  if (threadIdx.x == 0) s_base = tot ? atomicAdd(cursor, tot) : 0ull;
+  __syncthreads();
+  unsigned long long w = s_base + ex;
+  if (!mine) return;
+  for (uint32_t l = threadIdx.x; l < n; l += BLOCK) {
+    const uint64_t ea = (uint64_t)(s_oa[l + 1] - a0), eb = (uint64_t)(s_ob[l + 1] - x0);
+    if ((ea - (uint64_t)(s_oa[l] - a0)) * (eb - (uint64_t)(s_ob[l] - x0)) > JOIN_PAIR_LIMIT) continue;
+    for (uint64_t ia = (uint64_t)(s_oa[l] - a0); ia < ea; ia++) {
+      const K a = a_lds ? s_a[ia] : keysA[a0 + ia];
+      for (uint64_t ib = (uint64_t)(s_ob[l] - x0); ib < eb; ib++) {
+        const K x = b_lds ? s_b[ib] : keysB[x0 + ib];
+        if (x == a) {
+          if (w < capacity) out[w] = x;
+          w++;
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Single-pass exclusive scan with decoupled look-back (Merrill & Garland
+// 2016).  Tiles take ids from an atomic ticket so a tile only waits on tiles
+// that started before it.  Status word per tile: bits 62-63 flag (1 =
+// aggregate, 2 = inclusive prefix), bits 0-61 value.
+// ---------------------------------------------------------------------------
+template <typename T, int BLOCK, int IPT>
+__global__ __launch_bounds__(BLOCK) void k_scan_lookback(const T *__restrict__ in, T *__restrict__ out, uint64_t n,
+                                                         unsigned long long *__restrict__ status,
+                                                         unsigned int *__restrict__ ticket) {
+  __shared__ unsigned int s_tile;
+  __shared__ unsigned long long s_excl;
+  __shared__ T s_tmp[BLOCK / 64 + 1];
+  if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
+  __syncthreads();
+  const uint64_t tile = s_tile;
+  const uint64_t base = tile * (uint64_t)(BLOCK * IPT) + (uint64_t)threadIdx.x * IPT;
+  T v[IPT];
+  T sum = 0;
+#pragma unroll
+  for (int j = 0; j < IPT; j++) {
+    v[j] = base + j < n ? in[base + j] : T(0);
+    sum += v[j];
+  }
+  T agg;
+  T ex = block_exclusive_sum<BLOCK, T>(sum, s_tmp, agg);
+  constexpr unsigned long long FA = 1ull << 62, FP = 2ull << 62, VM = (1ull << 62) - 1;
+  if (threadIdx.x < 64) {
+    unsigned long long prefix = 0;
+    if (tile == 0) {
+      if (threadIdx.x == 0)
+        __hip_atomic_store(&status[0], FP | ((unsigned long long)agg & VM), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      if (threadIdx.x == 0)
+        __hip_atomic_store(&status[tile], FA | ((unsigned long long)agg & VM), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      // wave-wide look-back: lane i inspects tile (p - i)
+      int64_t p = (int64_t)tile - 1;
+      const int lane = threadIdx.x;
+      while (true) {
+        const int64_t q = p - lane;
+        unsigned long long st = q >= 0 ? __hip_atomic_load(&status[q], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) : FP;
+        const unsigned long long flag = st >> 62;
+        // lanes up to and including the first inclusive prefix must all be ready
+        const unsigned long long inc_mask = __ballot(flag == 2);
+        const unsigned long long zero_mask = __ballot(flag == 0);
+        const int first_inc = inc_mask ? __ffsll((long long)inc_mask) - 1 : 64;
+        const unsigned long long need = first_inc >= 63 ? ~0ull : ((2ull << first_inc) - 1);
+        if (zero_mask & need) {
+          __builtin_amdgcn_s_sleep(1);
+          continue;
+        }
+        unsigned long long contrib = lane <= first_inc ? (st & VM) : 0ull;
+        contrib = wave_sum<unsigned long long>(contrib);
+        prefix += contrib;
+        if (first_inc < 64) break;
+        p -= 64;
+      }
+      if (threadIdx.x == 0) {
+        __hip_atomic_store(&status[tile], FP | ((prefix + (unsigned long long)agg) & VM), __ATOMIC_RELEASE,
+                           __HIP_MEMORY_SCOPE_AGENT);
+        s_excl = prefix;
+      }
+    }
+    if (tile == 0 && threadIdx.x == 0) s_excl = 0;
+  }
+  __syncthreads();
+  T run = (T)s_excl + ex;
+#pragma unroll
+  for (int j = 0; j < IPT; j++) {
+    if (base + j < n) {
+      T x = v[j];
+      out[base + j] = run;
+      run += x;
+    }
+  }
+  const uint64_t last = (uint64_t)(n - 1) / (BLOCK * IPT);
+  if (tile == last && threadIdx.x == BLOCK - 1) out[n] = (T)s_excl + agg;
+}
+
+template <typename T>
+__global__ void k_fill(T *__restrict__ p, T v, uint64_t n) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+template <typename K>
+__global__ void k_bins(const K *__restrict__ keys, uint64_t n, BinMap mp, uint32_t *__restrict__ bins) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    bins[i] = bin_of<K>(keys[i], mp);
+}
+
+// Histogram of bin / bin_range over bin_count ranges (countBinSizes,
+// include/MultiHashGraphDeviceOperators.cuh:153-164), LDS-privatised.
+template <typename K>
+__global__ __launch_bounds__(1024) void k_range_hist(const K *__restrict__ keys, uint64_t n, BinMap mp, uint32_t range,
+                                                     uint32_t nranges, unsigned long long *__restrict__ hist,
+                                                     unsigned int *__restrict__ bad) {
+  extern __shared__ uint32_t s_h[];
+  for (uint32_t i = threadIdx.x; i < nranges; i += blockDim.x) s_h[i] = 0;
+  __syncthreads();
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t c = bin_of<K>(keys[i], mp) / range;
+    if (c < nranges) atomicAdd(&s_h[c], 1u);
+    else atomicOr(bad, 1u);
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < nranges; i += blockDim.x)
+    if (s_h[i]) atomicAdd(&hist[i], (unsigned long long)s_h[i]);
+}
+
+// ---------------------------------------------------------------------------
+// Work items of the final stage / probe from the partition pass results: one
+// item per column (bucket) of every segment.
+// ---------------------------------------------------------------------------
+__global__ void k_make_items(const SegDesc *__restrict__ segs, uint32_t nsegs, const uint32_t *__restrict__ bstart,
+                             const uint32_t *__restrict__ coltot, FinItem *__restrict__ items, uint32_t ncols_total) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ncols_total) return;
+  uint32_t lo = 0, hi = nsegs;  // last segment with bucket_base <= g
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (segs[mid].bucket_base <= g) lo = mid; else hi = mid;
+  }
+  const SegDesc &sd = segs[lo];
+  const uint32_t b = g - (uint32_t)sd.bucket_base;
+  FinItem it;
+  it.table = sd.table;
+  it.key_rel = sd.fin_key_rel;
+  if (b >= sd.nbuckets || sd.kind != 0) {  // drop column
+    it.key_start = 0; it.count = 0; it.nbins = 0; it.bin_lo = 0; it.off_pos = 0; it.end_pos = ~0ull;
+  } else {
+    const uint64_t blo = (uint64_t)b * sd.div.w;
+    it.key_start = bstart[g];
+    it.count = coltot[g];
+    it.bin_lo = (uint32_t)(sd.bin_lo + blo);
+    it.nbins = (uint32_t)min((uint64_t)sd.div.w, sd.seg_nbins - blo);
+    it.off_pos = sd.fin_off_base + blo;
+    it.end_pos = (b + 1 == sd.nbuckets) ? sd.fin_end_pos : ~0ull;
+  }
+  items[g] = it;
+}
+
+// Probe of buckets too large for k_probe: one workgroup streams the
+// bucket's keys and walks A's bins in global memory.
+template <typename K, typename OFF, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_probe_big(const K *__restrict__ bpart, const FinItem *__restrict__ items,
+                                                     const BinMap *__restrict__ maps, const OFF *__restrict__ offA,
+                                                     const K *__restrict__ keysA, K *__restrict__ out, uint64_t capacity,
+                                                     unsigned long long *__restrict__ cursor, uint32_t small_cap,
+                                                     unsigned int *__restrict__ flag) {
+  __shared__ unsigned long long s_t64[BLOCK / 64 + 1];
+  __shared__ unsigned long long s_base;
+  const FinItem it = items[blockIdx.x];
+  if (it.count <= small_cap) return;
+  const BinMap mp = maps[it.table];
+  const K *kp = bpart + it.key_start;
+  for (uint64_t c0 = 0; c0 < it.count; c0 += BLOCK) {
+    const uint64_t i = c0 + threadIdx.x;
+    unsigned long long mine = 0;
+    K k = 0;
+    uint64_t s = 0, e = 0;
+    if (i < it.count) {
+      k = kp[i];
+      const uint64_t l = bin_of<K>(k, mp) - it.bin_lo;
+      s = (uint64_t)offA[it.off_pos + l];
+      e = (uint64_t)offA[it.off_pos + l + 1];
+      if (e - s > PROBE_BIN_LIMIT) {
+        atomicOr(flag, 1u);
+        e = s;
+      }
+      for (uint64_t q = s; q < e; q++) mine += keysA[q] == k;
+    }
+    if (out == nullptr) {
+      const unsigned long long t = block_sum<BLOCK, unsigned long long>(mine, s_t64);
+      if (threadIdx.x == 0 && t) atomicAdd(cursor, t);
+      continue;
+    }
+    unsigned long long tot;
+    const unsigned long long ex = block_exclusive_sum<BLOCK, unsigned long long>(mine, s_t64, tot);
+    if (threadIdx.x == 0) s_base = tot ? atomicAdd(cursor, tot) : 0ull;
+    __syncthreads();
+    unsigned long long w = s_base + ex;
+    for (uint64_t q = s; q < e && mine; q++)
+      if (keysA[q] == k) {
+        if (w < capacity) out[w] = k;
+        w++;
+      }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+template <typename K>
+static hipError_t launch_part_hist_t(const LaunchPart &lp, hipStream_t st) {
+  const size_t lds = (size_t)lp.max_cols * 4;
+  if (lp.kpt == 32)
+    hipLaunchKernelGGL((k_part_hist<K, 1024, 32>), dim3(lp.ntiles), dim3(1024), lds, st, (const K *)lp.keys, lp.tiles,
+                       lp.segs, lp.cmat);
+  else
+    hipLaunchKernelGGL((k_part_hist<K, 1024, 16>), dim3(lp.ntiles), dim3(1024), lds, st, (const K *)lp.keys, lp.tiles,
+                       lp.segs, lp.cmat);
+  return hipGetLastError();
+}
+
+template <typename K>
+static hipError_t launch_part_scatter_t(const LaunchPart &lp, hipStream_t st) {
+  const size_t cnt_bytes = (size_t)((lp.max_cols + 3) & ~3u) * 4;
+  const size_t lds = cnt_bytes + (size_t)lp.stage_cap * (sizeof(K) + 2);
+  if (lp.kpt == 32)
+    hipLaunchKernelGGL((k_part_scatter<K, 1024, 32>), dim3(lp.ntiles), dim3(1024), lds, st, (const K *)lp.keys, lp.tiles,
+                       lp.segs, lp.cmat, (K *)lp.out, lp.stage_cap);
+  else
+    hipLaunchKernelGGL((k_part_scatter<K, 1024, 16>), dim3(lp.ntiles), dim3(1024), lds, st, (const K *)lp.keys, lp.tiles,
+                       lp.segs, lp.cmat, (K *)lp.out, lp.stage_cap);
+  return hipGetLastError();
+}
+
+hipError_t launch_part_hist(const LaunchPart &lp, hipStream_t st) {
+  return lp.key_bytes == 4 ? launch_part_hist_t<uint32_t>(lp, st) : launch_part_hist_t<uint64_t>(lp, st);
+}
+hipError_t launch_part_scatter(const LaunchPart &lp, hipStream_t st) {
+  return lp.key_bytes == 4 ? launch_part_scatter_t<uint32_t>(lp, st) : launch_part_scatter_t<uint64_t>(lp, st);
+}
+
+hipError_t launch_colscan(const LaunchCol &lc, hipStream_t st) {
+  if (lc.nitems)
+    hipLaunchKernelGGL(k_col_chunksum, dim3(lc.nitems), dim3(256), 0, st, lc.cmat, lc.items, lc.segs, lc.psum_base, lc.psum);
+  if (lc.nitems2)
+    hipLaunchKernelGGL(k_col_chunkscan, dim3(lc.nitems2), dim3(256), 0, st, lc.items2, lc.segs, lc.psum_base, lc.nchunks,
+                       lc.psum, lc.coltot);
+  hipLaunchKernelGGL((k_seg_bucketscan<256>), dim3(lc.nsegs), dim3(256), 0, st, lc.segs, lc.coltot, lc.bstart);
+  if (lc.nitems)
+    hipLaunchKernelGGL(k_col_apply, dim3(lc.nitems), dim3(256), 0, st, lc.cmat, lc.items, lc.segs, lc.psum_base, lc.psum,
+                       lc.bstart);
+  return hipGetLastError();
+}
+
+template <typename K, typename OFF>
+static hipError_t launch_final_t(const LaunchFinal &lf, hipStream_t st) {
+  if (lf.nitems) {
+    constexpr int KPT = sizeof(K) == 4 ? KPT_U32 : 32;
+    hipLaunchKernelGGL((k_final<K, OFF, 1024, KPT>), dim3(lf.nitems), dim3(1024), lf.lds_bytes, st, (const K *)lf.part,
+                       (K *)lf.csr, (OFF *)lf.off, lf.items, lf.maps, lf.lds_bytes);
+  }
+  if (lf.nitems) {
+    constexpr int KPT = sizeof(K) == 4 ? KPT_U32 : 32;
+    const uint32_t small_cap = std::min(1024u * KPT, 65535u);
+    hipLaunchKernelGGL((k_final_big<K, OFF, 1024, 16384>), dim3(lf.nitems), dim3(1024), 0, st, (const K *)lf.part,
+                       (K *)lf.csr, (OFF *)lf.off, lf.items, lf.maps, small_cap);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_final(const LaunchFinal &lf, hipStream_t st) {
+  if (lf.key_bytes == 4)
+    return lf.off_bytes == 4 ? launch_final_t<uint32_t, uint32_t>(lf, st) : launch_final_t<uint32_t, uint64_t>(lf, st);
+  return lf.off_bytes == 4 ? launch_final_t<uint64_t, uint32_t>(lf, st) : launch_final_t<uint64_t, uint64_t>(lf, st);
+}
+
+int final_kpt(int key_bytes) { return key_bytes == 4 ? KPT_U32 : 32; }
+
+template <typename K, typename OFF>
+static hipError_t launch_probe_t(const LaunchProbe &p, hipStream_t st) {
+  constexpr int KPT = sizeof(K) == 4 ? KPT_U32 : 32;
+  constexpr int SUB = 8192;
+  const size_t lds = ((SUB + 2) * sizeof(OFF) + 15) / 16 * 16 + (size_t)p.akey_cap * sizeof(K);
+  hipLaunchKernelGGL((k_probe<K, OFF, 1024, KPT, SUB>), dim3(p.nitems), dim3(1024), lds, st, (const K *)p.bpart, p.items,
+                     p.maps, (const OFF *)p.offA, (const K *)p.keysA, (K *)p.out, p.capacity, p.cursor, p.akey_cap, p.flag);
+  const uint32_t small_cap = std::min(1024u * KPT, 65535u);
+  hipLaunchKernelGGL((k_probe_big<K, OFF, 1024>), dim3(p.nitems), dim3(1024), 0, st, (const K *)p.bpart, p.items, p.maps,
+                     (const OFF *)p.offA, (const K *)p.keysA, (K *)p.out, p.capacity, p.cursor, small_cap, p.flag);
+  return hipGetLastError();
+}
+
+hipError_t launch_probe(const LaunchProbe &p, hipStream_t st) {
+  if (!p.nitems) return hipSuccess;
+  if (p.key_bytes == 4)
+    return p.off_bytes == 4 ? launch_probe_t<uint32_t, uint32_t>(p, st) : launch_probe_t<uint32_t, uint64_t>(p, st);
+  return p.off_bytes == 4 ? launch_probe_t<uint64_t, uint32_t>(p, st) : launch_probe_t<uint64_t, uint64_t>(p, st);
+}
+
+uint32_t probe_akey_cap(int key_bytes, int off_bytes) {
+  const size_t budget = 150 * 1024;
+  const size_t used = ((8192 + 2) * (size_t)off_bytes + 15) / 16 * 16;
+  return (uint32_t)((budget - used) / key_bytes);
+}
+
+template <typename K, typename OFF>
+static hipError_t launch_join_t(const LaunchJoin &j, hipStream_t st) {
+  constexpr int JB = 4096;
+  const uint64_t nblk = (j.M + JB - 1) / JB;
+  const size_t lds = 2 * (size_t)j.key_cap * sizeof(K);
+  hipLaunchKernelGGL((k_join<K, OFF, 256, JB>), dim3((uint32_t)nblk), dim3(256), lds, st, (const OFF *)j.offA,
+                     (const K *)j.keysA, (const OFF *)j.offB, (const K *)j.keysB, j.M, (K *)j.out, j.capacity, j.cursor,
+                     j.key_cap, j.flag);
+  return hipGetLastError();
+}
+
+hipError_t launch_join(const LaunchJoin &j, hipStream_t st) {
+  if (j.M == 0) return hipSuccess;
+  if (j.key_bytes == 4)
+    return j.off_bytes == 4 ? launch_join_t<uint32_t, uint32_t>(j, st) : launch_join_t<uint32_t, uint64_t>(j, st);
+  return j.off_bytes == 4 ? launch_join_t<uint64_t, uint32_t>(j, st) : launch_join_t<uint64_t, uint64_t>(j, st);
+}
+
+uint32_t join_key_cap(int key_bytes, int off_bytes) {
+  // two key arrays next to the 2*(JB+1) offsets held in static LDS
+  const size_t budget = 150 * 1024 - 2 * (4096 + 1) * (size_t)off_bytes;
+  return (uint32_t)(budget / (2 * (size_t)key_bytes));
+}
+
+hipError_t launch_scan(const void *in, void *out, uint64_t n, int elem_bytes, unsigned long long *status,
+                       unsigned int *ticket, hipStream_t st) {
+  constexpr int B = 256, IPT = 16;
+  const uint64_t ntiles = (n + B * IPT - 1) / (B * IPT);
+  hipLaunchKernelGGL((k_fill<unsigned long long>), dim3(256), dim3(256), 0, st, status, 0ull, ntiles);
+  hipLaunchKernelGGL((k_fill<unsigned int>), dim3(1), dim3(64), 0, st, ticket, 0u, (uint64_t)1);
+  if (elem_bytes == 4)
+    hipLaunchKernelGGL((k_scan_lookback<uint32_t, B, IPT>), dim3((uint32_t)ntiles), dim3(B), 0, st, (const uint32_t *)in,
+                       (uint32_t *)out, n, status, ticket);
+  else
+    hipLaunchKernelGGL((k_scan_lookback<uint64_t, B, IPT>), dim3((uint32_t)ntiles), dim3(B), 0, st, (const uint64_t *)in,
+                       (uint64_t *)out, n, status, ticket);
+  return hipGetLastError();
+}
+uint64_t scan_tiles(uint64_t n) { return (n + 256 * 16 - 1) / (256 * 16); }
+
+hipError_t launch_make_items(const SegDesc *segs, uint32_t nsegs, const uint32_t *bstart, const uint32_t *coltot,
+                             FinItem *items, uint32_t ncols_total, hipStream_t st) {
+  if (!ncols_total) return hipSuccess;
+  hipLaunchKernelGGL(k_make_items, dim3((ncols_total + 255) / 256), dim3(256), 0, st, segs, nsegs, bstart, coltot, items,
+                     ncols_total);
+  return hipGetLastError();
+}
+
+template <typename F>
+static hipError_t set_lds(F *fn, size_t bytes) {
+  return hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
+hipError_t set_kernel_attributes() {
+  const size_t big = 160 * 1024 - 1024;
+  hipError_t e = hipSuccess;
+#define HG_SET(...) do { hipError_t e_ = set_lds(__VA_ARGS__, big); if (e_ != hipSuccess) e = e_; } while (0)
+  HG_SET(&k_part_hist<uint32_t, 1024, 32>);
+  HG_SET(&k_part_hist<uint64_t, 1024, 32>);
+  HG_SET(&k_part_hist<uint32_t, 1024, 16>);
+  HG_SET(&k_part_hist<uint64_t, 1024, 16>);
+  HG_SET(&k_part_scatter<uint32_t, 1024, 32>);
+  HG_SET(&k_part_scatter<uint64_t, 1024, 32>);
+  HG_SET(&k_part_scatter<uint32_t, 1024, 16>);
+  HG_SET(&k_part_scatter<uint64_t, 1024, 16>);
+  HG_SET(&k_final<uint32_t, uint32_t, 1024, KPT_U32>);
+  HG_SET(&k_final<uint32_t, uint64_t, 1024, KPT_U32>);
+  HG_SET(&k_final<uint64_t, uint32_t, 1024, 32>);
+  HG_SET(&k_final<uint64_t, uint64_t, 1024, 32>);
+  HG_SET(&k_probe<uint32_t, uint32_t, 1024, KPT_U32, 8192>);
+  HG_SET(&k_probe<uint32_t, uint64_t, 1024, KPT_U32, 8192>);
+  HG_SET(&k_probe<uint64_t, uint32_t, 1024, 32, 8192>);
+  HG_SET(&k_probe<uint64_t, uint64_t, 1024, 32, 8192>);
+  HG_SET(&k_join<uint32_t, uint32_t, 256, 4096>);
+  HG_SET(&k_join<uint32_t, uint64_t, 256, 4096>);
+  HG_SET(&k_join<uint64_t, uint32_t, 256, 4096>);
+  HG_SET(&k_join<uint64_t, uint64_t, 256, 4096>);
+  HG_SET(&k_range_hist<uint32_t>);
+  HG_SET(&k_range_hist<uint64_t>);
+#undef HG_SET
+  return e;
+}
+
+hipError_t launch_fill_u64(unsigned long long *p, unsigned long long v, uint64_t n, hipStream_t st) {
+  hipLaunchKernelGGL((k_fill<unsigned long long>), dim3(64), dim3(256), 0, st, p, v, n);
+  return hipGetLastError();
+}
+hipError_t launch_fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t st) {
+  hipLaunchKernelGGL((k_fill<uint32_t>), dim3(256), dim3(256), 0, st, p, v, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_bins(const void *keys, uint64_t n, int key_bytes, const BinMap &mp, uint32_t *bins, hipStream_t st) {
+  if (!n) return hipSuccess;
+  const uint32_t grid = (uint32_t)std::min<uint64_t>((n + 255) / 256, 8192);
+  if (key_bytes == 4)
+    hipLaunchKernelGGL((k_bins<uint32_t>), dim3(grid), dim3(256), 0, st, (const uint32_t *)keys, n, mp, bins);
+  else
+    hipLaunchKernelGGL((k_bins<uint64_t>), dim3(grid), dim3(256), 0, st, (const uint64_t *)keys, n, mp, bins);
+  return hipGetLastError();
+}
+
+hipError_t launch_range_hist(const void *keys, uint64_t n, int key_bytes, const BinMap &mp, uint32_t range,
+                             uint32_t nranges, unsigned long long *hist, unsigned int *bad, hipStream_t st) {
+  if (!n) return hipSuccess;
+  const uint32_t grid = (uint32_t)std::min<uint64_t>((n + 1023) / 1024, 1024);
+  const size_t lds = (size_t)nranges * 4;
+  if (key_bytes == 4)
+    hipLaunchKernelGGL((k_range_hist<uint32_t>), dim3(grid), dim3(1024), lds, st, (const uint32_t *)keys, n, mp, range,
+                       nranges, hist, bad);
+  else
+    hipLaunchKernelGGL((k_range_hist<uint64_t>), dim3(grid), dim3(1024), lds, st, (const uint64_t *)keys, n, mp, range,
+                       nranges, hist, bad);
+  return hipGetLastError();
+}
+
+}  // namespace hg

@@ -1,0 +1,97 @@
+// hg_kernels.hpp -- host-side launch interface of the kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <algorithm>
+#include "hg_common.hpp"
+
+namespace hg {
+
+struct LaunchPart {
+  int key_bytes;
+  int kpt;               // keys per thread of a tile (16 or 32), tile = 1024*kpt
+  uint32_t ntiles;
+  uint32_t max_cols;     // largest ncols over the segments
+  uint32_t stage_cap;    // keys staged per LDS round in the scatter
+  const void *keys;
+  void *out;
+  const TileDesc *tiles;
+  const SegDesc *segs;
+  uint32_t *cmat;        // counts after hist, positions after the column scan
+};
+
+struct LaunchCol {
+  uint32_t nitems;       // chunk items (seg, chunk, column block)
+  uint32_t nitems2;      // column-block items (seg, column block)
+  uint32_t nsegs;
+  uint32_t *cmat;
+  const ColItem *items;
+  const ColItem *items2;
+  const SegDesc *segs;
+  const uint64_t *psum_base;
+  const uint32_t *nchunks;
+  uint32_t *psum;
+  uint32_t *coltot;
+  uint32_t *bstart;
+};
+
+struct LaunchFinal {
+  int key_bytes, off_bytes;
+  uint32_t nitems;
+  uint32_t lds_bytes;
+  const void *part;
+  void *csr;
+  void *off;
+  const FinItem *items;
+  const BinMap *maps;
+};
+
+struct LaunchProbe {
+  int key_bytes, off_bytes;
+  uint32_t nitems;
+  uint32_t akey_cap;
+  const void *bpart;
+  const FinItem *items;
+  const BinMap *maps;
+  const void *offA;
+  const void *keysA;
+  void *out;
+  uint64_t capacity;
+  unsigned long long *cursor;
+  unsigned int *flag;
+};
+
+struct LaunchJoin {
+  int key_bytes, off_bytes;
+  uint64_t M;
+  uint32_t key_cap;
+  const void *offA, *keysA, *offB, *keysB;
+  void *out;
+  uint64_t capacity;
+  unsigned long long *cursor;
+  unsigned int *flag;
+};
+
+hipError_t launch_part_hist(const LaunchPart &lp, hipStream_t st);
+hipError_t launch_part_scatter(const LaunchPart &lp, hipStream_t st);
+hipError_t launch_colscan(const LaunchCol &lc, hipStream_t st);
+hipError_t launch_final(const LaunchFinal &lf, hipStream_t st);
+hipError_t launch_probe(const LaunchProbe &p, hipStream_t st);
+hipError_t launch_join(const LaunchJoin &j, hipStream_t st);
+hipError_t launch_scan(const void *in, void *out, uint64_t n, int elem_bytes, unsigned long long *status,
+                       unsigned int *ticket, hipStream_t st);
+hipError_t launch_fill_u64(unsigned long long *p, unsigned long long v, uint64_t n, hipStream_t st);
+hipError_t launch_fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t st);
+hipError_t launch_bins(const void *keys, uint64_t n, int key_bytes, const BinMap &mp, uint32_t *bins, hipStream_t st);
+hipError_t launch_range_hist(const void *keys, uint64_t n, int key_bytes, const BinMap &mp, uint32_t range,
+                             uint32_t nranges, unsigned long long *hist, unsigned int *bad, hipStream_t st);
+hipError_t set_kernel_attributes();
+hipError_t launch_make_items(const SegDesc *segs, uint32_t nsegs, const uint32_t *bstart, const uint32_t *coltot,
+                             FinItem *items, uint32_t ncols_total, hipStream_t st);
+
+int final_kpt(int key_bytes);
+uint32_t probe_akey_cap(int key_bytes, int off_bytes);
+uint32_t join_key_cap(int key_bytes, int off_bytes);
+uint64_t scan_tiles(uint64_t n);
+
+}  // namespace hg

@@ -1,0 +1,109 @@
+"""Seeded synthetic key sets of the benchmark configurations.
+
+The reference's inputs come from Thrust's minstd engine or a plain sequence
+(test/MultiHashGraphTest.cu:81-128); these configurations define their own:
+SplitMix64 (Steele, Lea & Flood, 2014) in counter form,
+    x_i = mix64(s0 + (i + 1) * 0x9E3779B97F4A7C15),  s0 = mix64(seed),
+uint32 keys = the top 32 bits of x_i, uint64 keys = x_i.  The same integer
+arithmetic runs in numpy (CPU) and in torch int64 (on the GPU, wrapping
+multiply, logical shifts by masking), so both produce identical bytes.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+GAMMA = 0x9E3779B97F4A7C15
+M1 = 0xBF58476D1CE4E5B9
+M2 = 0x94D049BB133111EB
+MASK64 = (1 << 64) - 1
+
+
+def mix64_int(z: int) -> int:
+    z &= MASK64
+    z = ((z ^ (z >> 30)) * M1) & MASK64
+    z = ((z ^ (z >> 27)) * M2) & MASK64
+    return z ^ (z >> 31)
+
+
+def _np_mix64(z):
+    z = (z ^ (z >> np.uint64(30))) * np.uint64(M1)
+    z = (z ^ (z >> np.uint64(27))) * np.uint64(M2)
+    return z ^ (z >> np.uint64(31))
+
+
+def stream_np(seed: int, n: int, start: int = 0) -> np.ndarray:
+    """x_start .. x_{start+n-1} of stream `seed` (uint64)."""
+    s0 = np.uint64(mix64_int(seed))
+    with np.errstate(over="ignore"):
+        i = np.arange(start + 1, start + n + 1, dtype=np.uint64)
+        return _np_mix64(s0 + i * np.uint64(GAMMA))
+
+
+def _t_shr(x, k):
+    import torch
+    return (x >> k) & ((1 << (64 - k)) - 1)
+
+
+def _t_mix64(z):
+    z = (z ^ _t_shr(z, 30)) * _s64(M1)
+    z = (z ^ _t_shr(z, 27)) * _s64(M2)
+    return z ^ _t_shr(z, 31)
+
+
+def _s64(v: int) -> int:
+    """uint64 constant as the int64 with the same bits."""
+    v &= MASK64
+    return v - (1 << 64) if v >= (1 << 63) else v
+
+
+def stream_torch(seed: int, n: int, device, start: int = 0, chunk: int = 1 << 26):
+    """Same values as stream_np, computed on `device` as int64 bit patterns."""
+    import torch
+    out = torch.empty(n, dtype=torch.int64, device=device)
+    s0 = _s64(mix64_int(seed))
+    for c0 in range(0, n, chunk):
+        c1 = min(n, c0 + chunk)
+        i = torch.arange(start + c0 + 1, start + c1 + 1, dtype=torch.int64, device=device)
+        out[c0:c1] = _t_mix64(i * _s64(GAMMA) + s0)
+    return out
+
+
+def uniform_keys(seed: int, n: int, key_bytes: int, device=None):
+    """n uniform keys: uint32 = top 32 bits, uint64 = all 64 bits.
+    numpy uint32/uint64 array when device is None, else an int32/int64 tensor."""
+    if device is None:
+        x = stream_np(seed, n)
+        return (x >> np.uint64(32)).astype(np.uint32) if key_bytes == 4 else x
+    import torch
+    x = stream_torch(seed, n, device)
+    if key_bytes == 8:
+        return x
+    return _t_shr(x, 32).to(torch.int32)  # wraps: same low 32 bits
+
+
+def probe_keys(A, seed: int, n: int, frac: float, key_bytes: int, device=None):
+    """n probe keys, each drawn from A with probability `frac` (position
+    A[(r >> 32) * len(A) >> 32]) and fresh uniform otherwise (stream seed+1).
+    r is element i of stream `seed`; the draw uses its low 16 bits."""
+    thr = int(round(frac * 65536))
+    nA = len(A) if device is None else A.numel()
+    if device is None:
+        r = stream_np(seed, n)
+        take = (r & np.uint64(0xFFFF)) < np.uint64(thr)
+        idx = (((r >> np.uint64(32)) * np.uint64(nA)) >> np.uint64(32)).astype(np.int64)
+        fresh = uniform_keys(seed + 1, n, key_bytes)
+        return np.where(take, A[idx], fresh)
+    import torch
+    r = stream_torch(seed, n, device)
+    take = (r & 0xFFFF) < thr
+    idx = (_t_shr(r, 32) * nA) >> 32  # < 2^32 * nA < 2^63 for nA < 2^31
+    fresh = uniform_keys(seed + 1, n, key_bytes, device)
+    return torch.where(take, A[idx], fresh)
+
+
+def sequential_keys(n: int, key_bytes: int, device=None):
+    """0 .. n-1 (the reference driver's default input, test/MultiHashGraphTest.cu:106-110)."""
+    if device is None:
+        return np.arange(n, dtype=np.uint32 if key_bytes == 4 else np.uint64)
+    import torch
+    return torch.arange(n, dtype=torch.int32 if key_bytes == 4 else torch.int64, device=device)

@@ -1,0 +1,194 @@
+/*
+ * hashgraph.h -- C ABI of the MI355X HashGraph library (libhashgraph_amd.so).
+ *
+ * A HashGraph is a hash table stored as a CSR graph: bin b's keys are
+ * keys[offsets[b] .. offsets[b+1]), offsets[M] = N.  bin(k) is
+ * murmur3_x86_32(low 32 bits of k, seed 0) mod M, as in the reference
+ * (include/SingleHashGraphOperators.cuh:71-79,
+ *  include/MultiHashGraphDeviceOperators.cuh:76-83), or the learned CDF model.
+ *
+ * Conventions
+ *  - Plain pointers and sizes only.  Pointers named d_* are device pointers
+ *    on the context's device; h_* are host pointers.
+ *  - Every call is ordered on the context's stream (hg_ctx_set_stream; the
+ *    null stream by default).  Calls that return a host value (h_count ...)
+ *    synchronise that stream before returning.
+ *  - The caller allocates inputs and outputs.  Temporary device memory lives
+ *    in a per-context arena that grows on demand (hg_ctx_reserve pre-sizes it).
+ *  - No call exits or throws; each returns hg_status, and hg_last_error()
+ *    describes the last failure on the calling thread.
+ *  - Bin order inside a bin is unspecified (the reference fills bins with
+ *    atomics); offsets are exact.
+ */
+#ifndef HASHGRAPH_H
+#define HASHGRAPH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  HG_OK = 0,
+  HG_ERR_INVALID_ARG = 1,  /* bad sizes, null pointers, mismatched tables   */
+  HG_ERR_OOM = 2,          /* scratch arena could not grow                  */
+  HG_ERR_DEVICE = 3,       /* HIP runtime error                             */
+  HG_ERR_CAPACITY = 4,     /* output buffer smaller than the result         */
+  HG_ERR_MISMATCH = 5,     /* two tables do not share a bin space           */
+  HG_ERR_UNSUPPORTED = 6   /* size or mode outside what this build handles */
+} hg_status;
+
+typedef enum { HG_HASH_MURMUR3 = 0, HG_HASH_RMI = 1 } hg_hash_kind;
+
+typedef struct hg_ctx_s *hg_ctx;
+
+/* Table description shared by every call.
+ *  key_bytes    4 (uint32 keys) or 8 (uint64 keys; bins use the low 32 bits
+ *               under murmur, as the reference does)
+ *  offset_bytes 4 (uint32 offsets) or 8 (uint64 offsets)
+ *  n_keys       N, number of keys
+ *  n_bins       M, number of bins, 1 <= M < 2^32
+ *  hash_kind    HG_HASH_MURMUR3 or HG_HASH_RMI
+ *  d_rmi_model  device pointer to an hg_rmi_model (HG_HASH_RMI only)      */
+typedef struct {
+  uint32_t key_bytes;
+  uint32_t offset_bytes;
+  uint64_t n_keys;
+  uint64_t n_bins;
+  uint32_t hash_kind;
+  uint32_t reserved;
+  const void *d_rmi_model;
+} hg_table_desc;
+
+/* ---- context ----------------------------------------------------------- */
+hg_status hg_ctx_create(int device, hg_ctx *out);
+hg_status hg_ctx_destroy(hg_ctx ctx);
+/* stream is a hipStream_t (NULL = the null stream). */
+hg_status hg_ctx_set_stream(hg_ctx ctx, void *stream);
+/* Grow the scratch arena to at least bytes now (no allocation later for
+ * calls whose scratch fits). */
+hg_status hg_ctx_reserve(hg_ctx ctx, size_t bytes);
+/* Device bytes of scratch a build / probe of this description needs. */
+hg_status hg_scratch_bytes(const hg_table_desc *desc, int op, size_t *bytes);
+const char *hg_last_error(void);
+const char *hg_version(void);
+
+/* ---- build --------------------------------------------------------------
+ * Replaces SingleHashGraph::build (src/SingleHashGraph.cu:216-232, the LRB
+ * build at :129-181) and the per-GPU CSR build of MultiHashGraph::build
+ * (src/MultiHashGraph.cu:303-375).
+ *  d_keys     N keys
+ *  d_offsets  M+1 offsets (offset_bytes each); offsets[M] = N
+ *  d_csr_keys N keys grouped by bin                                        */
+hg_status hg_build(hg_ctx ctx, const hg_table_desc *desc, const void *d_keys, void *d_offsets,
+                   void *d_csr_keys);
+
+/* ---- intersection of two built tables -----------------------------------
+ * Replaces MultiHashGraph::intersect (src/MultiHashGraph.cu:631-740, kernel
+ * simpleIntersect include/MultiHashGraphDeviceOperators.cuh:513-543): for
+ * every bin, every pair (a in A_bin, x in B_bin) with a == x counts once and
+ * emits x.  Count = sum over values v of cA(v)*cB(v).  Both tables must
+ * share key_bytes, hash_kind and n_bins.
+ *  h_count    total matches (host, exact 64-bit)
+ *  d_out      NULL for count only; else capacity keys, written in bin order
+ *  Returns HG_ERR_CAPACITY (with *h_count set) when capacity < count.     */
+hg_status hg_intersect(hg_ctx ctx, const hg_table_desc *descA, const void *d_offA, const void *d_keysA,
+                       const hg_table_desc *descB, const void *d_offB, const void *d_keysB, void *d_out,
+                       uint64_t capacity, uint64_t *h_count);
+
+/* ---- probe: raw keys against a built table -------------------------------
+ * Same result as building d_probe into a table B over A's bins and calling
+ * hg_intersect(A, B) (same count, same multiset of emitted keys: each probe
+ * key x is emitted cA(x) times), without materialising B's CSR.  Output order
+ * is grouped by A's bins.                                                   */
+hg_status hg_probe(hg_ctx ctx, const hg_table_desc *descA, const void *d_offA, const void *d_keysA,
+                   const void *d_probe, uint64_t n_probe, void *d_out, uint64_t capacity, uint64_t *h_count);
+
+/* ---- run-length intersection ----------------------------------------------
+ * Heavy duplicates make the emitted list of hg_intersect infeasible (a value
+ * with 10^8 copies in both tables yields 10^16 pairs).  This form writes one
+ * record per distinct value present in both tables: key (key_bytes),
+ * count in A and count in B (uint64 each), bins in order.
+ *  h_count    sum cA*cB (exact; saturates at 2^64-1)
+ *  h_records  number of records                                           */
+hg_status hg_intersect_rle(hg_ctx ctx, const hg_table_desc *descA, const void *d_offA, const void *d_keysA,
+                           const hg_table_desc *descB, const void *d_offB, const void *d_keysB, void *d_rec_keys,
+                           uint64_t *d_rec_ca, uint64_t *d_rec_cb, uint64_t capacity, uint64_t *h_records,
+                           uint64_t *h_count);
+
+/* ---- logical partitions (one GPU's share of MultiHashGraph) ---------------
+ * hg_find_splits replaces countBinSizes + binarySearch
+ * (include/MultiHashGraphHostOperators.cuh:8-108): a histogram of
+ * bin / binRange over bin_count ranges (binRange = ceil(M / (float)bin_count)),
+ * key-balanced split points h_splits[0..n_parts] (int64, host).            */
+hg_status hg_find_splits(hg_ctx ctx, const hg_table_desc *desc, const void *d_keys, uint64_t bin_count,
+                         uint32_t n_parts, int64_t *h_splits);
+
+/* hg_build_partitioned replaces the local partition + per-partition build of
+ * MultiHashGraph::build (src/MultiHashGraph.cu:408-629 with
+ * include/MultiHashGraphDeviceOperators.cuh:220-246,297-345,371-458):
+ * partition p holds the keys whose bin lies in [h_splits[p], h_splits[p+1])
+ * and gets its own CSR over local bins bin - h_splits[p].
+ *  d_offsets      partition p's R_p+1 offsets start at element
+ *                 h_splits[p] + p (R_p = h_splits[p+1] - h_splits[p]); each
+ *                 partition's offsets start at 0
+ *  d_csr_keys     all partitions' keys, partition after partition
+ *  h_part_offsets n_parts+1 host entries: partition p's keys are
+ *                 d_csr_keys[h_part_offsets[p] .. h_part_offsets[p+1])
+ * Keys whose bin is >= h_splits[n_parts] are dropped, as in the reference. */
+hg_status hg_build_partitioned(hg_ctx ctx, const hg_table_desc *desc, const void *d_keys, const int64_t *h_splits,
+                               uint32_t n_parts, void *d_offsets, void *d_csr_keys, uint64_t *h_part_offsets);
+
+/* ---- learned hash (RMI) ---------------------------------------------------
+ * No counterpart in the reference.  A two-stage piecewise-linear CDF model in
+ * FP32: root (slope, intercept) picks one of n_leaves leaves, each leaf
+ * (slope, intercept) predicts the CDF; bin = min(M-1, floor(M * clamp(cdf))).
+ * hg_rmi_train fits root and leaves by least squares on a key sample (device
+ * kernel), then compares max-bin and mean-probe length against murmur on
+ * that sample; *h_use_rmi = 1 when the model wins, 0 when murmur should be
+ * used (the fallback).  d_model must hold hg_rmi_model_bytes(n_leaves).   */
+size_t hg_rmi_model_bytes(uint32_t n_leaves);
+hg_status hg_rmi_train(hg_ctx ctx, const void *d_sample, uint64_t n_sample, uint32_t key_bytes, uint64_t n_bins,
+                       uint32_t n_leaves, void *d_model, uint32_t *h_use_rmi);
+
+/* ---- batched form -----------------------------------------------------------
+ * No counterpart in the reference.  Builds n_tables independent tables (and
+ * optionally probes one key set against each) in one set of launches.
+ * Table t: keys d_keys[key_off[t] .. key_off[t+1]), n_bins[t] bins, offsets
+ * at d_offsets[off_off[t] .. off_off[t] + n_bins[t]], CSR keys at
+ * d_csr_keys[key_off[t] ..].  Arrays h_* are host arrays of n_tables(+1).   */
+typedef struct {
+  uint32_t n_tables;
+  uint32_t key_bytes;
+  const uint64_t *h_key_off;   /* n_tables + 1 */
+  const uint64_t *h_n_bins;    /* n_tables     */
+  const uint64_t *h_off_off;   /* n_tables     */
+  const uint32_t *h_hash_kind; /* n_tables, NULL = all murmur */
+  const void *d_rmi_models;    /* n_tables models of rmi_leaves leaves each, or NULL */
+  uint32_t rmi_leaves;
+} hg_batch_desc;
+
+hg_status hg_batch_build(hg_ctx ctx, const hg_batch_desc *bd, const void *d_keys, uint32_t *d_offsets,
+                         void *d_csr_keys);
+/* Probe set of table t: d_probe[probe_off[t] .. probe_off[t+1]).  Writes the
+ * per-table match counts to d_counts (uint64, n_tables) and their sum to
+ * *h_total. */
+hg_status hg_batch_probe(hg_ctx ctx, const hg_batch_desc *bd, const uint32_t *d_offsets, const void *d_csr_keys,
+                         const void *d_probe, const uint64_t *h_probe_off, uint64_t *d_counts, uint64_t *h_total);
+
+/* ---- primitives ----------------------------------------------------------
+ * Exclusive prefix sum (single-pass decoupled look-back), replaces
+ * cub::DeviceScan::ExclusiveSum at src/SingleHashGraph.cu:147-152,166-172 and
+ * src/MultiHashGraph.cu:327-336,353-365,706-715.  elem_bytes 4 or 8;
+ * d_out may alias d_in; d_out has n + 1 elements (the total goes last).   */
+hg_status hg_exclusive_scan(hg_ctx ctx, const void *d_in, void *d_out, uint64_t n, uint32_t elem_bytes);
+/* bins of n keys (murmur3, seed 0, low 32 bits, mod M) -> uint32. */
+hg_status hg_hash_bins(hg_ctx ctx, const void *d_keys, uint64_t n, uint32_t key_bytes, uint64_t n_bins,
+                       uint32_t *d_bins);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HASHGRAPH_H */

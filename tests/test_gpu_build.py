@@ -1,0 +1,140 @@
+"""GPU build parity: hg_build (HIP kernels) against the CPU oracle.
+
+Offsets must be identical; bins must hold the same keys (compared sorted,
+since the reference fills bins with atomics)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+from hashgraph_amd import workloads as wl
+from oracle import oracle as o
+
+
+def _to_dev(a, dev):
+    import torch
+    t = torch.from_numpy(a.view(np.int32 if a.dtype == np.uint32 else np.int64).copy())
+    return t.to(dev)
+
+
+def _host(t, dtype):
+    return t.cpu().numpy().view(dtype)
+
+
+def _check_build(keys, M, dev, ob=4):
+    import hashgraph_amd as hg
+    print(f"build n={len(keys)} M={M} kb={keys.itemsize} ob={ob}", flush=True)
+    dk = _to_dev(keys, dev)
+    off, csr = hg.build(dk, M, offset_bytes=ob)
+    goff = _host(off, np.uint32 if ob == 4 else np.uint64).astype(np.uint64)
+    gcsr = _host(csr, keys.dtype).copy()
+    eoff, ecsr = o.build_csr(keys, M)
+    assert np.array_equal(goff, eoff), "offsets differ"
+    o.canon(goff, gcsr)
+    o.canon(eoff, ecsr)
+    assert np.array_equal(gcsr, ecsr), "bin contents differ"
+
+
+def test_c1_shape_u32(dev):
+    # 2^20 uint32 keys, seed 1, 2^20 bins
+    keys = wl.uniform_keys(1, 1 << 20, 4)
+    _check_build(keys, 1 << 20, dev)
+
+
+@pytest.mark.parametrize("n,M,kb,ob", [
+    (1, 1, 4, 4), (2, 1, 4, 4), (1000, 1, 4, 4), (100, 100000, 4, 4), (77777, 3, 4, 4), (5000, 97, 8, 4),
+    (300000, 299993, 4, 8), (1 << 18, 1 << 22, 4, 4), (1 << 18, 1 << 22, 8, 8), (123457, 65536, 8, 4),
+    (200000, 1000, 4, 4), (1 << 20, 1 << 16, 4, 4),
+])
+def test_shapes(n, M, kb, ob, dev):
+    keys = wl.uniform_keys(n + M, n, kb)
+    _check_build(keys, M, dev, ob)
+
+
+def test_empty_table(dev):
+    import torch
+    import hashgraph_amd as hg
+    keys = torch.empty(0, dtype=torch.int32, device=dev)
+    off, csr = hg.build(keys, 50)
+    assert off.cpu().numpy().tolist() == [0] * 51
+
+
+def test_oversized_buckets_single_bin(dev):
+    # M = 1: every key in one bin, far beyond one workgroup's VGPR capacity
+    keys = wl.uniform_keys(5, 150000, 4)
+    _check_build(keys, 1, dev)
+
+
+def test_heavy_duplicates(dev):
+    # 200k keys over 50 distinct values: bins with thousands of equal keys
+    base = wl.uniform_keys(6, 50, 8)
+    idx = (wl.stream_np(7, 200000) % np.uint64(50)).astype(np.int64)
+    keys = base[idx]
+    _check_build(keys, 1 << 16, dev)
+
+
+def test_sequential_keys(dev):
+    keys = wl.sequential_keys(1 << 20, 8)
+    _check_build(keys, 1 << 20, dev, ob=8)
+
+
+def test_u64_low_word_collisions(dev):
+    lo = wl.uniform_keys(8, 5000, 4).astype(np.uint64)
+    keys = np.concatenate([lo, lo | np.uint64(1 << 33), lo | np.uint64(0xFFFF << 40)])
+    _check_build(keys, 4096, dev)
+
+
+def test_repeat_build_is_stable(dev):
+    import hashgraph_amd as hg
+    keys = wl.uniform_keys(10, 1 << 19, 4)
+    dk = _to_dev(keys, dev)
+    a = hg.build(dk, 1 << 19)
+    b = hg.build(dk, 1 << 19)
+    assert np.array_equal(a[0].cpu().numpy(), b[0].cpu().numpy())
+
+
+def test_full_size_2e28_u32_properties(dev):
+    """2^28 uint32 keys, 2^28 bins: size-independent checks on the device."""
+    import torch
+    import hashgraph_amd as hg
+    n = M = 1 << 28
+    keys = wl.uniform_keys(1, n, 4, device=dev)
+    off, csr = hg.build(keys, M)
+    torch.cuda.synchronize()
+    offl = off.to(torch.int64)
+    deg = offl[1:] - offl[:-1]
+    assert int(offl[0]) == 0 and int(offl[-1]) == n
+    assert bool((deg >= 0).all())
+    bins = hg.hash_bins(csr, M).to(torch.int64)
+    # the bin of every CSR key is the bin whose range holds it
+    expect = torch.repeat_interleave(torch.arange(M, device=dev), deg, output_size=n)
+    assert bool((bins == expect).all())
+    del expect, bins
+    # same multiset of keys
+    assert torch.equal(torch.sort(csr.view(torch.int32))[0], torch.sort(keys)[0])
+    # offsets equal the bincount of the input's bins
+    cnt = torch.bincount(hg.hash_bins(keys, M).to(torch.int64), minlength=M)
+    assert torch.equal(cnt, deg)
+
+
+def test_hash_bins_matches_oracle(dev):
+    import hashgraph_amd as hg
+    for kb in (4, 8):
+        keys = wl.uniform_keys(12, 100000, kb)
+        for M in (1, 1000, 1 << 20, 4294967291):
+            g = hg.hash_bins(_to_dev(keys, dev), M).cpu().numpy().view(np.uint32)
+            assert np.array_equal(g, o.bins(keys, M))
+
+
+@pytest.mark.parametrize("n", [1, 4095, 4096, 4097, 1000003, 1 << 24])
+@pytest.mark.parametrize("width", [4, 8])
+def test_exclusive_scan(n, width, dev):
+    import torch
+    import hashgraph_amd as hg
+    dt = np.uint32 if width == 4 else np.uint64
+    x = (wl.stream_np(n, n) % np.uint64(1000)).astype(dt)
+    t = torch.from_numpy(x.view(np.int32 if width == 4 else np.int64).copy()).to(dev)
+    out = hg.exclusive_scan(t).cpu().numpy().view(dt)
+    exp = np.zeros(n + 1, dtype=dt)
+    exp[1:] = np.cumsum(x, dtype=dt)
+    assert np.array_equal(out, exp)

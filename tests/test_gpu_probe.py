@@ -1,0 +1,162 @@
+"""GPU probe / join parity: hg_probe and hg_intersect against the oracle's
+bin-wise bag-product join (src/MultiHashGraph.cu:631-740) and the reference
+driver's hash-free check (test/MultiHashGraphTest.cu:369-397)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+from hashgraph_amd import workloads as wl
+from oracle import oracle as o
+
+
+def _to_dev(a, dev):
+    import torch
+    return torch.from_numpy(a.view(np.int32 if a.dtype == np.uint32 else np.int64).copy()).to(dev)
+
+
+def _expected(A, B, M):
+    oa, ka = o.build_csr(A, M)
+    ob, kb = o.build_csr(B, M)
+    cnt, lst = o.intersect(oa, ka, ob, kb)
+    return cnt, np.sort(lst)
+
+
+def _check_probe(A, B, M, dev, ob=4):
+    import hashgraph_amd as hg
+    print(f"probe nA={len(A)} nB={len(B)} M={M} kb={A.itemsize} ob={ob}", flush=True)
+    dA = _to_dev(A, dev)
+    off, csr = hg.build(dA, M, offset_bytes=ob)
+    dB = _to_dev(B, dev)
+    cnt_only, _ = hg.probe(off, csr, M, dB, emit=False)
+    cnt, out = hg.probe(off, csr, M, dB)
+    ecnt, elst = _expected(A, B, M)
+    assert cnt_only == ecnt
+    assert cnt == ecnt
+    assert np.array_equal(np.sort(out.cpu().numpy().view(A.dtype)), elst)
+    assert cnt == o.bag_intersect_count(A, B)
+
+
+def _check_join(A, B, M, dev, ob=4):
+    import hashgraph_amd as hg
+    print(f"join nA={len(A)} nB={len(B)} M={M} kb={A.itemsize} ob={ob}", flush=True)
+    offA, csrA = hg.build(_to_dev(A, dev), M, offset_bytes=ob)
+    offB, csrB = hg.build(_to_dev(B, dev), M, offset_bytes=ob)
+    cnt, out = hg.intersect(offA, csrA, offB, csrB, M)
+    ecnt, elst = _expected(A, B, M)
+    assert cnt == ecnt
+    assert np.array_equal(np.sort(out.cpu().numpy().view(A.dtype)), elst)
+
+
+def test_c1_probe(dev):
+    # A = 2^20 uint32 (seed 1), B = 2^20 with 50% drawn from A (seed 2), 2^20 bins
+    A = wl.uniform_keys(1, 1 << 20, 4)
+    B = wl.probe_keys(A, 2, 1 << 20, 0.5, 4)
+    _check_probe(A, B, 1 << 20, dev)
+
+
+def test_c1_join(dev):
+    A = wl.uniform_keys(1, 1 << 20, 4)
+    B = wl.probe_keys(A, 2, 1 << 20, 0.5, 4)
+    _check_join(A, B, 1 << 20, dev)
+
+
+@pytest.mark.parametrize("nA,nB,M,kb,ob", [
+    (1, 1, 1, 4, 4), (1000, 5000, 97, 4, 4), (50000, 20000, 1 << 20, 8, 8), (300000, 100000, 299993, 4, 4),
+    (1 << 18, 1 << 19, 1 << 18, 8, 4), (20000, 200000, 5000, 4, 8),
+])
+def test_probe_shapes(nA, nB, M, kb, ob, dev):
+    A = wl.uniform_keys(nA + 1, nA, kb)
+    B = wl.probe_keys(A, nB + 2, nB, 0.3, kb)
+    _check_probe(A, B, M, dev, ob)
+    _check_join(A, B, M, dev, ob)
+
+
+def test_duplicates_on_both_sides(dev):
+    # bag product: a value with cA copies in A and cB in B contributes cA*cB
+    vals = wl.uniform_keys(3, 3000, 4)
+    A = vals[(wl.stream_np(4, 20000) % np.uint64(3000)).astype(np.int64)]
+    B = vals[(wl.stream_np(5, 15000) % np.uint64(6000)).astype(np.int64) % 3000]
+    _check_probe(A, B, 1 << 12, dev)
+    _check_join(A, B, 1 << 12, dev)
+
+
+def test_sequential_closed_form(dev):
+    import hashgraph_amd as hg
+    nA, nB = 1 << 20, 1 << 19
+    A = wl.sequential_keys(nA, 8)
+    B = wl.sequential_keys(nB, 8)
+    off, csr = hg.build(_to_dev(A, dev), nA, offset_bytes=8)
+    cnt, out = hg.probe(off, csr, nA, _to_dev(B, dev))
+    assert cnt == nB
+    assert np.array_equal(np.sort(out.cpu().numpy().view(np.uint64)), np.arange(nB, dtype=np.uint64))
+
+
+def test_empty_inputs(dev):
+    import torch
+    import hashgraph_amd as hg
+    A = wl.uniform_keys(1, 1000, 4)
+    off, csr = hg.build(_to_dev(A, dev), 512)
+    empty = torch.empty(0, dtype=torch.int32, device=dev)
+    assert hg.probe(off, csr, 512, empty)[0] == 0
+    eoff, ecsr = hg.build(empty, 512)
+    assert hg.probe(eoff, ecsr, 512, _to_dev(A, dev))[0] == 0
+    assert hg.intersect(off, csr, eoff, ecsr, 512)[0] == 0
+
+
+def test_heavy_bin_is_refused(dev):
+    # one value repeated 100k times: the walk per probe key would be 100k long;
+    # the library reports it instead of running it
+    import hashgraph_amd as hg
+    A = np.full(100000, 12345, dtype=np.uint32)
+    off, csr = hg.build(_to_dev(A, dev), 1024)
+    with pytest.raises(hg.HGError) as e:
+        hg.probe(off, csr, 1024, _to_dev(A[:1000], dev), emit=False)
+    assert e.value.status == 6
+
+
+def test_mismatched_tables_rejected(dev):
+    import hashgraph_amd as hg
+    A = wl.uniform_keys(1, 1000, 4)
+    offA, csrA = hg.build(_to_dev(A, dev), 512)
+    offB, csrB = hg.build(_to_dev(A, dev), 256)
+    with pytest.raises(hg.HGError) as e:
+        import ctypes
+        from hashgraph_amd import _lib
+        c = hg.context(0)
+        dA = hg._desc(1000, 512, 4)
+        dB = hg._desc(1000, 256, 4)
+        cnt = ctypes.c_uint64()
+        _lib.check(c.lib.hg_intersect(c.bind_stream(), ctypes.byref(dA), hg._p(offA), hg._p(csrA), ctypes.byref(dB),
+                                      hg._p(offB), hg._p(csrB), None, 0, ctypes.byref(cnt)), "hg_intersect")
+    assert e.value.status == 5
+
+
+def test_full_size_2e28_probe_properties(dev):
+    """2^28 uint32 A, B with 25% from A: count equals a device-side sort-merge
+    count, every emitted key is in A and in B."""
+    import torch
+    import hashgraph_amd as hg
+    n = M = 1 << 28
+    A = wl.uniform_keys(1, n, 4, device=dev)
+    B = wl.probe_keys(A, 2, n, 0.25, 4, device=dev)
+    off, csr = hg.build(A, M)
+    cnt, out = hg.probe(off, csr, M, B)
+    del off, csr
+    sa = torch.sort(A)[0]
+    del A
+    sb = torch.sort(B)[0]
+    del B
+    # hash-free count: for every value, count in A times count in B
+    ua, ca = torch.unique_consecutive(sa, return_counts=True)
+    ub, cb = torch.unique_consecutive(sb, return_counts=True)
+    pos = torch.searchsorted(ua, ub)
+    pos = pos.clamp(max=ua.numel() - 1)
+    hit = ua[pos] == ub
+    expect = int((ca[pos][hit].to(torch.int64) * cb[hit].to(torch.int64)).sum())
+    assert cnt == expect
+    so = torch.sort(out)[0]
+    # emitted multiset: value v appears cA(v)*cB(v) times
+    uo, co = torch.unique_consecutive(so, return_counts=True)
+    assert torch.equal(uo, ub[hit])
+    assert torch.equal(co.to(torch.int64), (ca[pos][hit].to(torch.int64) * cb[hit].to(torch.int64)))
