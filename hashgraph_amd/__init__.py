@@ -48,6 +48,17 @@ class Context:
         check(self.lib.hg_ctx_set_stream(self.h, ctypes.c_void_p(s)), "hg_ctx_set_stream")
         return self.h
 
+    def profile(self, enable: bool = True):
+        check(self.lib.hg_ctx_profile(self.h, 1 if enable else 0), "hg_ctx_profile")
+
+    def profile_read(self, reset: bool = True):
+        """{kernel name: (launches, total ms)} since the last reset (synchronises)."""
+        buf = (_lib.KernelTime * 64)()
+        n = ctypes.c_uint32(0)
+        check(self.lib.hg_ctx_profile_read(self.h, buf, 64, ctypes.byref(n), 1 if reset else 0),
+              "hg_ctx_profile_read")
+        return {buf[i].name.decode(): (int(buf[i].launches), float(buf[i].total_ms)) for i in range(n.value)}
+
     def reserve(self, nbytes: int):
         check(self.lib.hg_ctx_reserve(self.h, nbytes), "hg_ctx_reserve")
 

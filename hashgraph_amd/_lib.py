@@ -22,7 +22,7 @@ HG_HASH_RMI = 1
 # Every symbol include/hashgraph.h declares (checked by the CPU tests).
 EXPORTS = [
     "hg_ctx_create", "hg_ctx_destroy", "hg_ctx_set_stream", "hg_ctx_reserve", "hg_scratch_bytes", "hg_last_error",
-    "hg_version", "hg_build", "hg_intersect", "hg_probe", "hg_intersect_rle", "hg_find_splits",
+    "hg_version", "hg_ctx_profile", "hg_ctx_profile_read", "hg_build", "hg_intersect", "hg_probe", "hg_intersect_rle", "hg_find_splits",
     "hg_build_partitioned", "hg_rmi_model_bytes", "hg_rmi_train", "hg_batch_build", "hg_batch_probe",
     "hg_exclusive_scan", "hg_hash_bins",
 ]
@@ -38,6 +38,10 @@ class TableDesc(ctypes.Structure):
     _fields_ = [("key_bytes", ctypes.c_uint32), ("offset_bytes", ctypes.c_uint32), ("n_keys", ctypes.c_uint64),
                 ("n_bins", ctypes.c_uint64), ("hash_kind", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
                 ("d_rmi_model", ctypes.c_void_p)]
+
+
+class KernelTime(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char * 32), ("launches", ctypes.c_uint64), ("total_ms", ctypes.c_double)]
 
 
 class BatchDesc(ctypes.Structure):
@@ -70,6 +74,8 @@ def load(path: str = LIB_PATH):
             "hg_scratch_bytes": (i32, [P(TableDesc), i32, P(ctypes.c_size_t)]),
             "hg_last_error": (ctypes.c_char_p, []),
             "hg_version": (ctypes.c_char_p, []),
+            "hg_ctx_profile": (i32, [vp, i32]),
+            "hg_ctx_profile_read": (i32, [vp, P(KernelTime), u32, P(u32), i32]),
             "hg_build": (i32, [vp, P(TableDesc), vp, vp, vp]),
             "hg_intersect": (i32, [vp, P(TableDesc), vp, vp, P(TableDesc), vp, vp, vp, u64, P(u64)]),
             "hg_probe": (i32, [vp, P(TableDesc), vp, vp, vp, u64, vp, u64, P(u64)]),

@@ -80,6 +80,15 @@ struct hg_ctx_s {
   bool hev_used[kSlots] = {false, false, false, false};
   int next_slot = 0;
   unsigned long long *h_small = nullptr;  // pinned read-back area
+  // per-kernel timing (hg_ctx_profile)
+  bool prof = false;
+  std::vector<hipEvent_t> ev_pool;
+  size_t ev_used = 0;
+  struct Rec {
+    const char *name;
+    size_t a, b;
+  };
+  std::vector<Rec> recs;
 };
 
 namespace {
@@ -92,6 +101,39 @@ struct DeviceGuard {
   }
   ~DeviceGuard() {
     if (prev >= 0) (void)hipSetDevice(prev);
+  }
+};
+
+// Brackets the launches of one kernel (or kernel pair) with events when the
+// context profiles.
+struct Prof {
+  hg_ctx c;
+  const char *name;
+  size_t a = 0;
+  bool on;
+  hipEvent_t get(size_t &idx) {
+    if (c->ev_used == c->ev_pool.size()) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) return nullptr;
+      c->ev_pool.push_back(e);
+    }
+    idx = c->ev_used++;
+    return c->ev_pool[idx];
+  }
+  Prof(hg_ctx c_, const char *n) : c(c_), name(n), on(c_->prof) {
+    if (on) {
+      hipEvent_t e = get(a);
+      if (e) (void)hipEventRecord(e, c->stream);
+      else on = false;
+    }
+  }
+  ~Prof() {
+    if (!on) return;
+    size_t b;
+    hipEvent_t e = get(b);
+    if (!e) return;
+    (void)hipEventRecord(e, c->stream);
+    c->recs.push_back({name, a, b});
   }
 };
 
@@ -331,7 +373,10 @@ hg_status run_level(hg_ctx c, const Level &L, const LevelDev &d, int kb, const v
   p.tiles = d.tiles;
   p.segs = d.segs;
   p.cmat = d.cmat;
-  if (p.ntiles) HG_HIP(launch_part_hist(p, c->stream));
+  if (p.ntiles) {
+    Prof pf(c, "part_hist");
+    HG_HIP(launch_part_hist(p, c->stream));
+  }
   LaunchCol lc{};
   lc.nitems = (uint32_t)L.items.size();
   lc.nitems2 = (uint32_t)L.items2.size();
@@ -345,8 +390,14 @@ hg_status run_level(hg_ctx c, const Level &L, const LevelDev &d, int kb, const v
   lc.psum = d.psum;
   lc.coltot = d.coltot;
   lc.bstart = d.bstart;
-  if (lc.nsegs) HG_HIP(launch_colscan(lc, c->stream));
-  if (p.ntiles) HG_HIP(launch_part_scatter(p, c->stream));
+  if (lc.nsegs) {
+    Prof pf(c, "col_scan");
+    HG_HIP(launch_colscan(lc, c->stream));
+  }
+  if (p.ntiles) {
+    Prof pf(c, "part_scatter");
+    HG_HIP(launch_part_scatter(p, c->stream));
+  }
   return HG_OK;
 }
 
@@ -378,7 +429,10 @@ hg_status run_final(hg_ctx c, const Level &L, const LevelDev &d, int kb, int ob,
   f.off = off;
   f.items = d.fin;
   f.maps = d.maps;
-  HG_HIP(launch_final(f, c->stream));
+  {
+    Prof pf(c, "final");
+    HG_HIP(launch_final(f, c->stream));
+  }
   return HG_OK;
 }
 
@@ -539,10 +593,9 @@ hg_status hg_ctx_create(int device, hg_ctx *out) {
   }
   static bool attrs_done[64] = {false};
   if (device < 64 && !attrs_done[device]) {
-    if (set_kernel_attributes() != hipSuccess) {
-      hg_ctx_destroy(c);
-      return fail(HG_ERR_DEVICE, "cannot raise the LDS limit of the kernels");
-    }
+    // best effort: a kernel whose limit cannot be raised reports its own
+    // launch error if it ever needs more
+    if (set_kernel_attributes() != hipSuccess) (void)hipGetLastError();
     attrs_done[device] = true;
   }
   *out = c;
@@ -560,13 +613,52 @@ hg_status hg_ctx_destroy(hg_ctx c) {
     if (c->hev[i]) (void)hipEventDestroy(c->hev[i]);
   }
   if (c->h_small) (void)hipHostFree(c->h_small);
+  for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
   delete c;
   return HG_OK;
 }
 
 hg_status hg_ctx_set_stream(hg_ctx c, void *stream) {
   if (!c) return fail(HG_ERR_INVALID_ARG, "null ctx");
-  c->stream = (hipStream_t)stream;
+  if ((hipStream_t)stream != c->stream) {
+    // work queued on the old stream may still read the arena
+    DeviceGuard g(c->device);
+    HG_HIP(hipStreamSynchronize(c->stream));
+    c->stream = (hipStream_t)stream;
+  }
+  return HG_OK;
+}
+
+hg_status hg_ctx_profile(hg_ctx c, int enable) {
+  if (!c) return fail(HG_ERR_INVALID_ARG, "null ctx");
+  c->prof = enable != 0;
+  return HG_OK;
+}
+
+hg_status hg_ctx_profile_read(hg_ctx c, hg_kernel_time *out, uint32_t capacity, uint32_t *h_count, int reset) {
+  if (!c || !h_count) return fail(HG_ERR_INVALID_ARG, "null argument");
+  DeviceGuard g(c->device);
+  HG_HIP(hipStreamSynchronize(c->stream));
+  std::vector<hg_kernel_time> agg;
+  for (const auto &r : c->recs) {
+    float ms = 0;
+    HG_HIP(hipEventElapsedTime(&ms, c->ev_pool[r.a], c->ev_pool[r.b]));
+    size_t i = 0;
+    while (i < agg.size() && strcmp(agg[i].name, r.name) != 0) i++;
+    if (i == agg.size()) {
+      hg_kernel_time t{};
+      strncpy(t.name, r.name, sizeof(t.name) - 1);
+      agg.push_back(t);
+    }
+    agg[i].launches++;
+    agg[i].total_ms += ms;
+  }
+  *h_count = (uint32_t)agg.size();
+  for (uint32_t i = 0; i < agg.size() && i < capacity && out; i++) out[i] = agg[i];
+  if (reset) {
+    c->recs.clear();
+    c->ev_used = 0;
+  }
   return HG_OK;
 }
 
@@ -676,7 +768,10 @@ hg_status hg_probe(hg_ctx c, const hg_table_desc *descA, const void *d_offA, con
   pr.capacity = d_out ? capacity : 0;
   pr.cursor = small;
   pr.flag = (unsigned int *)(small + 1);
-  HG_HIP(launch_probe(pr, c->stream));
+  {
+    Prof pf(c, "probe");
+    HG_HIP(launch_probe(pr, c->stream));
+  }
   unsigned long long res[2];
   HG_TRY(sync_read(c, res, small, 16));
   *h_count = res[0];
@@ -720,7 +815,10 @@ hg_status hg_intersect(hg_ctx c, const hg_table_desc *descA, const void *d_offA,
   j.capacity = d_out ? capacity : 0;
   j.cursor = small;
   j.flag = (unsigned int *)(small + 1);
-  HG_HIP(launch_join(j, c->stream));
+  {
+    Prof pf(c, "join");
+    HG_HIP(launch_join(j, c->stream));
+  }
   unsigned long long res[2];
   HG_TRY(sync_read(c, res, small, 16));
   *h_count = res[0];

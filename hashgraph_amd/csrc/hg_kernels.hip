@@ -600,6 +600,24 @@ __global__ void k_fill(T *__restrict__ p, T v, uint64_t n) {
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) p[i] = v;
 }
 
+// SplitMix64 counter form x_i = mix64(s0 + (i+1)*gamma); key = x_i reduced
+// to [0, range) by the high half of x_i * range (range 0: no reduction;
+// 32-bit keys take the top 32 bits).
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+template <typename K>
+__global__ void k_gen_uniform(K *__restrict__ out, uint64_t n, uint64_t s0, uint64_t range) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t x = mix64(s0 + (i + 1) * 0x9E3779B97F4A7C15ull);
+    uint64_t k = range ? __umul64hi(x, range) : x;
+    if (sizeof(K) == 4 && !range) k >>= 32;
+    out[i] = (K)k;
+  }
+}
+
 template <typename K>
 __global__ void k_bins(const K *__restrict__ keys, uint64_t n, BinMap mp, uint32_t *__restrict__ bins) {
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
@@ -852,15 +870,25 @@ hipError_t launch_make_items(const SegDesc *segs, uint32_t nsegs, const uint32_t
   return hipGetLastError();
 }
 
+// Raise a kernel's dynamic-LDS limit to what the device allows beside the
+// kernel's static LDS.
 template <typename F>
-static hipError_t set_lds(F *fn, size_t bytes) {
-  return hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+static hipError_t set_lds(F *fn) {
+  hipFuncAttributes fa;
+  hipError_t e = hipFuncGetAttributes(&fa, (const void *)fn);
+  if (e != hipSuccess) return e;
+  int dev = 0, lim = 0;
+  (void)hipGetDevice(&dev);
+  e = hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+  if (e != hipSuccess) return e;
+  const int room = lim - (int)fa.sharedSizeBytes;
+  if (room <= 0) return hipSuccess;
+  return hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, room);
 }
 
 hipError_t set_kernel_attributes() {
-  const size_t big = 160 * 1024 - 1024;
   hipError_t e = hipSuccess;
-#define HG_SET(...) do { hipError_t e_ = set_lds(__VA_ARGS__, big); if (e_ != hipSuccess) e = e_; } while (0)
+#define HG_SET(...) do { hipError_t e_ = set_lds(__VA_ARGS__); if (e_ != hipSuccess) e = e_; } while (0)
   HG_SET(&k_part_hist<uint32_t, 1024, 32>);
   HG_SET(&k_part_hist<uint64_t, 1024, 32>);
   HG_SET(&k_part_hist<uint32_t, 1024, 16>);
@@ -893,6 +921,16 @@ hipError_t launch_fill_u64(unsigned long long *p, unsigned long long v, uint64_t
 }
 hipError_t launch_fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t st) {
   hipLaunchKernelGGL((k_fill<uint32_t>), dim3(256), dim3(256), 0, st, p, v, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_gen_uniform(void *out, uint64_t n, int key_bytes, uint64_t s0, uint64_t range, hipStream_t st) {
+  if (!n) return hipSuccess;
+  const uint32_t grid = (uint32_t)std::min<uint64_t>((n + 255) / 256, 16384);
+  if (key_bytes == 4)
+    hipLaunchKernelGGL((k_gen_uniform<uint32_t>), dim3(grid), dim3(256), 0, st, (uint32_t *)out, n, s0, range);
+  else
+    hipLaunchKernelGGL((k_gen_uniform<uint64_t>), dim3(grid), dim3(256), 0, st, (uint64_t *)out, n, s0, range);
   return hipGetLastError();
 }
 

@@ -86,6 +86,7 @@ hipError_t launch_bins(const void *keys, uint64_t n, int key_bytes, const BinMap
 hipError_t launch_range_hist(const void *keys, uint64_t n, int key_bytes, const BinMap &mp, uint32_t range,
                              uint32_t nranges, unsigned long long *hist, unsigned int *bad, hipStream_t st);
 hipError_t set_kernel_attributes();
+hipError_t launch_gen_uniform(void *out, uint64_t n, int key_bytes, uint64_t s0, uint64_t range, hipStream_t st);
 hipError_t launch_make_items(const SegDesc *segs, uint32_t nsegs, const uint32_t *bstart, const uint32_t *coltot,
                              FinItem *items, uint32_t ncols_total, hipStream_t st);
 

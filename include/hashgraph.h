@@ -75,6 +75,19 @@ hg_status hg_scratch_bytes(const hg_table_desc *desc, int op, size_t *bytes);
 const char *hg_last_error(void);
 const char *hg_version(void);
 
+/* Per-kernel timing (no counterpart in the reference, whose drivers time
+ * whole calls with cudaEvents).  When enabled, every kernel launch of the
+ * context is bracketed by hipEvents on the context's stream; read returns,
+ * per kernel name, the number of launches and the summed event time since
+ * the last reset (it synchronises the stream).                            */
+typedef struct {
+  char name[32];
+  uint64_t launches;
+  double total_ms;
+} hg_kernel_time;
+hg_status hg_ctx_profile(hg_ctx ctx, int enable);
+hg_status hg_ctx_profile_read(hg_ctx ctx, hg_kernel_time *out, uint32_t capacity, uint32_t *h_count, int reset);
+
 /* ---- build --------------------------------------------------------------
  * Replaces SingleHashGraph::build (src/SingleHashGraph.cu:216-232, the LRB
  * build at :129-181) and the per-GPU CSR build of MultiHashGraph::build

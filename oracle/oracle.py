@@ -48,6 +48,9 @@ def lib():
         L.hgo_sort_keys.argtypes = [_vp, ctypes.c_uint64, ctypes.c_int]
         L.hgo_intersect.argtypes = [_vp, _vp, _vp, _vp, ctypes.c_uint64, ctypes.c_int, _vp, ctypes.c_uint64]
         L.hgo_intersect.restype = ctypes.c_uint64
+        L.hgo_intersect_par.argtypes = [_vp, _vp, _vp, _vp, ctypes.c_uint64, ctypes.c_int, _vp, ctypes.c_uint64,
+                                        ctypes.c_int]
+        L.hgo_intersect_par.restype = ctypes.c_uint64
         L.hgo_intersect_rle.argtypes = [_vp, _vp, _vp, _vp, ctypes.c_uint64, ctypes.c_int, _vp, _vp, _vp,
                                         ctypes.c_uint64, _vp]
         L.hgo_intersect_rle.restype = ctypes.c_uint64
@@ -141,6 +144,16 @@ def intersect(offA, keysA, offB, keysB, emit: bool = True):
     out = np.empty(cnt, dtype=keysA.dtype)
     lib().hgo_intersect(_ptr(offA), _ptr(keysA), _ptr(offB), _ptr(keysB), M, kb, _ptr(out), cnt)
     return cnt, out
+
+
+def intersect_par(offA, keysA, offB, keysB, threads: int, out=None):
+    """Threaded bin join (CPU baseline); same count and list as intersect()."""
+    offA = np.ascontiguousarray(offA, dtype=np.uint64)
+    offB = np.ascontiguousarray(offB, dtype=np.uint64)
+    M = len(offA) - 1
+    return int(lib().hgo_intersect_par(_ptr(offA), _ptr(keysA), _ptr(offB), _ptr(keysB), M, _kb(keysA),
+                                       _ptr(out) if out is not None else None, len(out) if out is not None else 0,
+                                       threads))
 
 
 def intersect_rle(offA, keysA, offB, keysB):
