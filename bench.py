@@ -106,6 +106,8 @@ def main():
     ap.add_argument("--config", default="c2-u32", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run device copies of MIB MiB (calibrates the HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); the path would shard "
@@ -118,6 +120,15 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a ROCm GPU"
     dev = torch.device("cuda", 0)
+    if args.copy_calibration:
+        n = args.copy_calibration << 20
+        a = torch.ones(n // 4, dtype=torch.int32, device=dev)
+        b = torch.empty_like(a)
+        for _ in range(max(args.steps, 1)):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": n, "copy_bytes_written": n, "launches": max(args.steps, 1)}))
+        return
     nA, nB, M, kb, frac = CONFIGS[args.config]
     s = kb
     log(f"config {args.config}: nA={nA} nB={nB} M={M} key_bytes={kb}")

@@ -828,6 +828,88 @@ hg_status hg_intersect(hg_ctx c, const hg_table_desc *descA, const void *d_offA,
   return HG_OK;
 }
 
+hg_status hg_intersect_rle(hg_ctx c, const hg_table_desc *descA, const void *d_offA, const void *d_keysA,
+                           const hg_table_desc *descB, const void *d_offB, const void *d_keysB, void *d_rec_keys,
+                           uint64_t *d_rec_ca, uint64_t *d_rec_cb, uint64_t capacity, uint64_t *h_records,
+                           uint64_t *h_count) {
+  if (!c) return fail(HG_ERR_INVALID_ARG, "null ctx");
+  HG_TRY(check_desc(descA));
+  HG_TRY(check_desc(descB));
+  if (!h_records || !h_count) return fail(HG_ERR_INVALID_ARG, "null h_records / h_count");
+  if (descA->n_bins != descB->n_bins || descA->key_bytes != descB->key_bytes || descA->hash_kind != descB->hash_kind ||
+      descA->offset_bytes != descB->offset_bytes)
+    return fail(HG_ERR_MISMATCH, "tables do not share a bin space");
+  if (!d_offA || !d_offB || (descA->n_keys && !d_keysA) || (descB->n_keys && !d_keysB))
+    return fail(HG_ERR_INVALID_ARG, "null buffer");
+  if (capacity && (!d_rec_keys || !d_rec_ca || !d_rec_cb)) return fail(HG_ERR_INVALID_ARG, "null record buffers");
+  DeviceGuard g(c->device);
+  *h_records = 0;
+  *h_count = 0;
+  const uint64_t M = descA->n_bins, nch = rle_chunks(M);
+  const uint64_t heavy_cap = std::min<uint64_t>(M, 1u << 22);
+  Plan P;
+  const size_t i_cnt = P.add((nch + 1) * 8), i_base = P.add((nch + 1) * 8), i_small = P.add(256),
+               i_heavy = P.add(heavy_cap * 4), i_status = P.add(scan_tiles(nch) * 8 + 64);
+  HG_TRY(arena_ensure(c, 1, P.total()));
+  Carve cv;
+  cv.init(c, 1, P);
+  unsigned long long *small = cv.get<unsigned long long>(i_small);  // [heavy_n, pairs, cursor, overflow]
+  HG_HIP(launch_fill_u64(small, 0ull, 8, c->stream));
+  LaunchRle r{};
+  r.key_bytes = (int)descA->key_bytes;
+  r.off_bytes = (int)descA->offset_bytes;
+  r.M = M;
+  r.offA = d_offA;
+  r.keysA = d_keysA;
+  r.offB = d_offB;
+  r.keysB = d_keysB;
+  r.chunk_cnt = cv.get<unsigned long long>(i_cnt);
+  r.chunk_base = cv.get<unsigned long long>(i_base);
+  r.rk = d_rec_keys;
+  r.rca = (unsigned long long *)d_rec_ca;
+  r.rcb = (unsigned long long *)d_rec_cb;
+  r.capacity = capacity;
+  r.heavy_n = small;
+  r.heavy = cv.get<uint32_t>(i_heavy);
+  r.heavy_cap = heavy_cap;
+  r.pairs = small + 1;
+  r.cursor = small + 2;
+  r.overflow = (unsigned int *)(small + 3);
+  r.pass = 0;
+  {
+    Prof pf(c, "rle_small");
+    HG_HIP(launch_rle(r, c->stream));
+  }
+  // record positions of the small bins: exclusive scan of the chunk counts
+  HG_HIP(launch_scan(r.chunk_cnt, (void *)r.chunk_base, nch, 8, cv.get<unsigned long long>(i_status),
+                     (unsigned int *)(cv.get<unsigned long long>(i_status) + scan_tiles(nch)), c->stream));
+  unsigned long long hv[2];
+  HG_TRY(sync_read(c, hv, small, 16));
+  unsigned long long nsmall = 0;
+  HG_TRY(sync_read(c, &nsmall, r.chunk_base + nch, 8));
+  if (hv[0] > heavy_cap) return fail(HG_ERR_UNSUPPORTED, "too many heavy bins");
+  r.pass = 1;
+  if (capacity) {
+    Prof pf(c, "rle_small");
+    HG_HIP(launch_rle(r, c->stream));
+  }
+  // heavy bins append after the small bins' records
+  HG_HIP(launch_fill_u64(small + 2, nsmall, 1, c->stream));
+  r.pass = 2;
+  r.nheavy = hv[0];
+  {
+    Prof pf(c, "rle_heavy");
+    HG_HIP(launch_rle(r, c->stream));
+  }
+  unsigned long long res[4];
+  HG_TRY(sync_read(c, res, small, 32));
+  if (res[3] & 0xFFFFFFFFull) return fail(HG_ERR_UNSUPPORTED, "a heavy bin holds too many distinct values");
+  *h_records = res[2];
+  *h_count = res[1];
+  if (res[2] > capacity && capacity) return fail(HG_ERR_CAPACITY, "record capacity smaller than the record count");
+  return HG_OK;
+}
+
 hg_status hg_find_splits(hg_ctx c, const hg_table_desc *desc, const void *d_keys, uint64_t bin_count, uint32_t n_parts,
                          int64_t *h_splits) {
   if (!c) return fail(HG_ERR_INVALID_ARG, "null ctx");

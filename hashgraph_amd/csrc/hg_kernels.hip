@@ -231,6 +231,9 @@ __global__ __launch_bounds__(BLOCK) void k_final(const K *__restrict__ part, K *
   __syncthreads();
   const K *kp = part + it.key_start;
   const uint32_t nv = cnt > threadIdx.x ? (cnt - threadIdx.x + BLOCK - 1) / BLOCK : 0;
+  // slots below nfull are valid in every lane: a uniform (scalar) branch, so
+  // no per-slot exec mask has to stay live across the unrolled loops
+  const uint32_t nfull = cnt / BLOCK;
   const __amdgpu_buffer_rsrc_t rs = make_rsrc(kp, cnt * (uint32_t)sizeof(K));
   K kr[KPT];
   uint32_t lb[(KPT + 1) / 2];
@@ -238,13 +241,15 @@ __global__ __launch_bounds__(BLOCK) void k_final(const K *__restrict__ part, K *
   for (int j = 0; j < KPT; j++) kr[j] = buf_load<K>(rs, threadIdx.x * (uint32_t)sizeof(K), j * BLOCK * (uint32_t)sizeof(K));
 #pragma unroll
   for (int j = 0; j < (KPT + 1) / 2; j++) lb[j] = 0;
+  auto count_slot = [&](int j) {
+    const uint32_t l = bin_of<K>(kr[j], mp) - it.bin_lo;
+    lb[j >> 1] |= l << ((j & 1) * 16);
+    atomicAdd(&smem32[l >> 1], 1u << ((l & 1) * 16));
+  };
 #pragma unroll
   for (int j = 0; j < KPT; j++) {
-    if (j < nv) {
-      const uint32_t l = bin_of<K>(kr[j], mp) - it.bin_lo;
-      lb[j >> 1] |= l << ((j & 1) * 16);
-      atomicAdd(&smem32[l >> 1], 1u << ((l & 1) * 16));
-    }
+    if (__builtin_amdgcn_readfirstlane(j < (int)nfull)) count_slot(j);
+    else if (j < nv) count_slot(j);
   }
   __syncthreads();
   // exclusive scan of the u16 counts (bucket total <= 65535, so every
@@ -274,27 +279,31 @@ __global__ __launch_bounds__(BLOCK) void k_final(const K *__restrict__ part, K *
   }
   if (it.end_pos != ~0ull && threadIdx.x == 0) off[it.end_pos] = (OFF)(obase + cnt);
   __syncthreads();
+  auto place_slot = [&](int j) {
+    const uint32_t sh = (j & 1) * 16;
+    const uint32_t l = (lb[j >> 1] >> sh) & 0xFFFFu;
+    const uint32_t old = atomicAdd(&smem32[l >> 1], 1u << ((l & 1) * 16));
+    const uint32_t pos = (old >> ((l & 1) * 16)) & 0xFFFFu;
+    lb[j >> 1] = (lb[j >> 1] & ~(0xFFFFu << sh)) | (pos << sh);
+  };
 #pragma unroll
   for (int j = 0; j < KPT; j++) {
-    if (j < nv) {
-      const uint32_t sh = (j & 1) * 16;
-      const uint32_t l = (lb[j >> 1] >> sh) & 0xFFFFu;
-      const uint32_t old = atomicAdd(&smem32[l >> 1], 1u << ((l & 1) * 16));
-      const uint32_t pos = (old >> ((l & 1) * 16)) & 0xFFFFu;
-      lb[j >> 1] = (lb[j >> 1] & ~(0xFFFFu << sh)) | (pos << sh);
-    }
+    if (__builtin_amdgcn_readfirstlane(j < (int)nfull)) place_slot(j);
+    else if (j < nv) place_slot(j);
   }
   __syncthreads();
   K *st = (K *)smem32;
   const uint32_t cap = lds_bytes / sizeof(K);
   K *dst = csr + it.key_start;
   for (uint32_t r0 = 0; r0 < cnt; r0 += cap) {
+    auto stage_slot = [&](int j) {
+      const uint32_t q = ((lb[j >> 1] >> ((j & 1) * 16)) & 0xFFFFu) - r0;
+      if (q < cap) st[q] = kr[j];
+    };
 #pragma unroll
     for (int j = 0; j < KPT; j++) {
-      if (j < nv) {
-        const uint32_t q = ((lb[j >> 1] >> ((j & 1) * 16)) & 0xFFFFu) - r0;
-        if (q < cap) st[q] = kr[j];
-      }
+      if (__builtin_amdgcn_readfirstlane(j < (int)nfull)) stage_slot(j);
+      else if (j < nv) stage_slot(j);
     }
     __syncthreads();
     const uint32_t n = min(cap, cnt - r0);
@@ -644,6 +653,217 @@ __global__ __launch_bounds__(1024) void k_range_hist(const K *__restrict__ keys,
 }
 
 // ---------------------------------------------------------------------------
+// Run-length join of two built tables: one record (key, cA, cB) per value
+// present in both tables.  Small bins (|A_b|, |B_b| <= RLE_SMALL) are done by
+// one thread each (quadratic in the bin, in bin order); a bin above that is
+// appended to a heavy list and handled by k_rle_heavy.
+// Pass 0 counts records per workgroup chunk of RLE_CHUNK bins, pass 1 writes
+// them at the chunk's exclusive prefix.
+// ---------------------------------------------------------------------------
+constexpr uint32_t RLE_SMALL = 64;
+constexpr uint32_t RLE_CHUNK = 1024;
+
+template <typename K, typename OFF, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_rle_small(const OFF *__restrict__ offA, const K *__restrict__ keysA,
+                                                     const OFF *__restrict__ offB, const K *__restrict__ keysB, uint64_t M,
+                                                     int pass, unsigned long long *__restrict__ chunk_cnt,
+                                                     const unsigned long long *__restrict__ chunk_base,
+                                                     K *__restrict__ rk, unsigned long long *__restrict__ rca,
+                                                     unsigned long long *__restrict__ rcb, uint64_t capacity,
+                                                     unsigned long long *__restrict__ heavy_n, uint32_t *__restrict__ heavy,
+                                                     uint64_t heavy_cap, unsigned long long *__restrict__ pairs) {
+  __shared__ unsigned long long s_t64[BLOCK / 64 + 1];
+  const uint64_t b0 = (uint64_t)blockIdx.x * RLE_CHUNK;
+  const uint32_t nb = (uint32_t)min((uint64_t)RLE_CHUNK, M - b0);
+  constexpr int PER = RLE_CHUNK / BLOCK;
+  unsigned long long nrec = 0, npair = 0;
+  // thread t owns bins b0 + t*PER .. +PER (contiguous: records stay in bin order)
+  for (int q = 0; q < PER; q++) {
+    const uint32_t l = threadIdx.x * PER + q;
+    if (l >= nb) break;
+    const uint64_t b = b0 + l;
+    const uint64_t a0 = offA[b], a1 = offA[b + 1], x0 = offB[b], x1 = offB[b + 1];
+    if (a1 == a0 || x1 == x0) continue;
+    if (a1 - a0 > RLE_SMALL || x1 - x0 > RLE_SMALL) {
+      if (pass == 0) {
+        const unsigned long long h = atomicAdd(heavy_n, 1ull);
+        if (h < heavy_cap) heavy[h] = (uint32_t)b;
+      }
+      continue;
+    }
+    for (uint64_t i = a0; i < a1; i++) {
+      const K v = keysA[i];
+      bool first = true;
+      for (uint64_t j = a0; j < i; j++) first &= keysA[j] != v;
+      if (!first) continue;
+      unsigned long long ca = 0, cb = 0;
+      for (uint64_t j = i; j < a1; j++) ca += keysA[j] == v;
+      for (uint64_t j = x0; j < x1; j++) cb += keysB[j] == v;
+      if (cb) {
+        nrec++;
+        npair += ca * cb;
+      }
+    }
+  }
+  unsigned long long tot;
+  unsigned long long ex = block_exclusive_sum<BLOCK, unsigned long long>(nrec, s_t64, tot);
+  if (pass == 0) {
+    const unsigned long long pt = block_sum<BLOCK, unsigned long long>(npair, s_t64);
+    if (threadIdx.x == 0) {
+      chunk_cnt[blockIdx.x] = tot;
+      if (pt) atomicAdd(pairs, pt);
+    }
+    return;
+  }
+  unsigned long long w = chunk_base[blockIdx.x] + ex;
+  for (int q = 0; q < PER; q++) {
+    const uint32_t l = threadIdx.x * PER + q;
+    if (l >= nb) break;
+    const uint64_t b = b0 + l;
+    const uint64_t a0 = offA[b], a1 = offA[b + 1], x0 = offB[b], x1 = offB[b + 1];
+    if (a1 == a0 || x1 == x0 || a1 - a0 > RLE_SMALL || x1 - x0 > RLE_SMALL) continue;
+    for (uint64_t i = a0; i < a1; i++) {
+      const K v = keysA[i];
+      bool first = true;
+      for (uint64_t j = a0; j < i; j++) first &= keysA[j] != v;
+      if (!first) continue;
+      unsigned long long ca = 0, cb = 0;
+      for (uint64_t j = i; j < a1; j++) ca += keysA[j] == v;
+      for (uint64_t j = x0; j < x1; j++) cb += keysB[j] == v;
+      if (cb) {
+        if (w < capacity) {
+          rk[w] = v;
+          rca[w] = ca;
+          rcb[w] = cb;
+        }
+        w++;
+      }
+    }
+  }
+}
+
+// Heavy bins: one workgroup per listed bin builds an LDS table value ->
+// (count in A, count in B) by streaming both sides, then emits the values
+// found on both sides.  Slots are claimed by compare-and-swap of the key
+// itself against the all-ones sentinel (no lane ever waits on another);
+// keys equal to the sentinel are counted apart.  A bin whose A side holds
+// more than HT_SLOTS/2 distinct values sets the overflow flag.
+constexpr uint32_t HT_SLOTS = 4096;
+
+__device__ __forceinline__ uint32_t lds_cas(uint32_t *p, uint32_t cmp, uint32_t v) { return atomicCAS(p, cmp, v); }
+__device__ __forceinline__ uint64_t lds_cas(uint64_t *p, uint64_t cmp, uint64_t v) {
+  return atomicCAS((unsigned long long *)p, (unsigned long long)cmp, (unsigned long long)v);
+}
+
+template <typename K, typename OFF, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_rle_heavy(const OFF *__restrict__ offA, const K *__restrict__ keysA,
+                                                     const OFF *__restrict__ offB, const K *__restrict__ keysB,
+                                                     const uint32_t *__restrict__ heavy, uint64_t nheavy,
+                                                     K *__restrict__ rk, unsigned long long *__restrict__ rca,
+                                                     unsigned long long *__restrict__ rcb, uint64_t capacity,
+                                                     unsigned long long *__restrict__ cursor,
+                                                     unsigned long long *__restrict__ pairs,
+                                                     unsigned int *__restrict__ overflow) {
+  constexpr K EMPTY = (K)~(K)0;
+  __shared__ K s_key[HT_SLOTS];
+  __shared__ unsigned long long s_ca[HT_SLOTS], s_cb[HT_SLOTS];
+  __shared__ unsigned int s_fill;
+  __shared__ unsigned long long s_eca, s_ecb, s_base, s_t64[BLOCK / 64 + 1];
+  for (uint64_t h = blockIdx.x; h < nheavy; h += gridDim.x) {
+    const uint64_t b = heavy[h];
+    for (uint32_t i = threadIdx.x; i < HT_SLOTS; i += BLOCK) {
+      s_key[i] = EMPTY;
+      s_ca[i] = 0;
+      s_cb[i] = 0;
+    }
+    if (threadIdx.x == 0) {
+      s_fill = 0;
+      s_eca = 0;
+      s_ecb = 0;
+    }
+    __syncthreads();
+    const uint64_t a0 = offA[b], a1 = offA[b + 1], x0 = offB[b], x1 = offB[b + 1];
+    for (uint64_t i = a0 + threadIdx.x; i < a1; i += BLOCK) {
+      const K v = keysA[i];
+      if (v == EMPTY) {
+        atomicAdd(&s_eca, 1ull);
+        continue;
+      }
+      uint32_t slot = murmur4((uint32_t)v ^ (uint32_t)((uint64_t)v >> 32)) & (HT_SLOTS - 1);
+      for (uint32_t probe = 0; probe < HT_SLOTS; probe++, slot = (slot + 1) & (HT_SLOTS - 1)) {
+        const K old = lds_cas(&s_key[slot], EMPTY, v);
+        if (old == EMPTY || old == v) {
+          atomicAdd(&s_ca[slot], 1ull);
+          if (old == EMPTY) atomicAdd(&s_fill, 1u);
+          break;
+        }
+      }
+    }
+    __syncthreads();
+    if (s_fill > HT_SLOTS / 2) {  // too many distinct values for the table
+      if (threadIdx.x == 0) atomicOr(overflow, 1u);
+      __syncthreads();
+      continue;
+    }
+    for (uint64_t i = x0 + threadIdx.x; i < x1; i += BLOCK) {
+      const K v = keysB[i];
+      if (v == EMPTY) {
+        atomicAdd(&s_ecb, 1ull);
+        continue;
+      }
+      uint32_t slot = murmur4((uint32_t)v ^ (uint32_t)((uint64_t)v >> 32)) & (HT_SLOTS - 1);
+      for (uint32_t probe = 0; probe < HT_SLOTS; probe++, slot = (slot + 1) & (HT_SLOTS - 1)) {
+        const K cur = s_key[slot];
+        if (cur == EMPTY) break;
+        if (cur == v) {
+          atomicAdd(&s_cb[slot], 1ull);
+          break;
+        }
+      }
+    }
+    __syncthreads();
+    unsigned long long mine = 0, mp = 0;
+    for (uint32_t i = threadIdx.x; i < HT_SLOTS; i += BLOCK)
+      if (s_cb[i]) {
+        mine++;
+        mp += s_ca[i] * s_cb[i];
+      }
+    const bool ehit = threadIdx.x == 0 && s_eca && s_ecb;
+    if (ehit) {
+      mine++;
+      mp += s_eca * s_ecb;
+    }
+    unsigned long long tot;
+    unsigned long long ex = block_exclusive_sum<BLOCK, unsigned long long>(mine, s_t64, tot);
+    const unsigned long long pt = block_sum<BLOCK, unsigned long long>(mp, s_t64);
+    if (threadIdx.x == 0) {
+      s_base = tot ? atomicAdd(cursor, tot) : 0ull;
+      if (pt) atomicAdd(pairs, pt);
+    }
+    __syncthreads();
+    unsigned long long w = s_base + ex;
+    for (uint32_t i = threadIdx.x; i < HT_SLOTS; i += BLOCK)
+      if (s_cb[i]) {
+        if (w < capacity) {
+          rk[w] = s_key[i];
+          rca[w] = s_ca[i];
+          rcb[w] = s_cb[i];
+        }
+        w++;
+      }
+    if (ehit) {
+      if (w < capacity) {
+        rk[w] = EMPTY;
+        rca[w] = s_eca;
+        rcb[w] = s_ecb;
+      }
+      w++;
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Work items of the final stage / probe from the partition pass results: one
 // item per column (bucket) of every segment.
 // ---------------------------------------------------------------------------
@@ -861,6 +1081,27 @@ hipError_t launch_scan(const void *in, void *out, uint64_t n, int elem_bytes, un
   return hipGetLastError();
 }
 uint64_t scan_tiles(uint64_t n) { return (n + 256 * 16 - 1) / (256 * 16); }
+
+template <typename K, typename OFF>
+static hipError_t launch_rle_t(const LaunchRle &r, hipStream_t st) {
+  const uint32_t nchunks = (uint32_t)((r.M + RLE_CHUNK - 1) / RLE_CHUNK);
+  if (r.pass <= 1 && nchunks)
+    hipLaunchKernelGGL((k_rle_small<K, OFF, 256>), dim3(nchunks), dim3(256), 0, st, (const OFF *)r.offA,
+                       (const K *)r.keysA, (const OFF *)r.offB, (const K *)r.keysB, r.M, r.pass, r.chunk_cnt,
+                       r.chunk_base, (K *)r.rk, r.rca, r.rcb, r.capacity, r.heavy_n, r.heavy, r.heavy_cap, r.pairs);
+  if (r.pass == 2 && r.nheavy)
+    hipLaunchKernelGGL((k_rle_heavy<K, OFF, 1024>), dim3((uint32_t)std::min<uint64_t>(r.nheavy, 1024)), dim3(1024), 0,
+                       st, (const OFF *)r.offA, (const K *)r.keysA, (const OFF *)r.offB, (const K *)r.keysB, r.heavy,
+                       r.nheavy, (K *)r.rk, r.rca, r.rcb, r.capacity, r.cursor, r.pairs, r.overflow);
+  return hipGetLastError();
+}
+
+hipError_t launch_rle(const LaunchRle &r, hipStream_t st) {
+  if (r.key_bytes == 4)
+    return r.off_bytes == 4 ? launch_rle_t<uint32_t, uint32_t>(r, st) : launch_rle_t<uint32_t, uint64_t>(r, st);
+  return r.off_bytes == 4 ? launch_rle_t<uint64_t, uint32_t>(r, st) : launch_rle_t<uint64_t, uint64_t>(r, st);
+}
+uint64_t rle_chunks(uint64_t M) { return (M + RLE_CHUNK - 1) / RLE_CHUNK; }
 
 hipError_t launch_make_items(const SegDesc *segs, uint32_t nsegs, const uint32_t *bstart, const uint32_t *coltot,
                              FinItem *items, uint32_t ncols_total, hipStream_t st) {

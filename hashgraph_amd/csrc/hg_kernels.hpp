@@ -72,6 +72,25 @@ struct LaunchJoin {
   unsigned int *flag;
 };
 
+struct LaunchRle {
+  int key_bytes, off_bytes;
+  int pass;  // 0 count small bins, 1 emit small bins, 2 heavy bins
+  uint64_t M;
+  const void *offA, *keysA, *offB, *keysB;
+  unsigned long long *chunk_cnt;
+  const unsigned long long *chunk_base;
+  void *rk;
+  unsigned long long *rca, *rcb;
+  uint64_t capacity;
+  unsigned long long *heavy_n;
+  uint32_t *heavy;
+  uint64_t heavy_cap, nheavy;
+  unsigned long long *cursor, *pairs;
+  unsigned int *overflow;
+};
+
+hipError_t launch_rle(const LaunchRle &r, hipStream_t st);
+uint64_t rle_chunks(uint64_t M);
 hipError_t launch_part_hist(const LaunchPart &lp, hipStream_t st);
 hipError_t launch_part_scatter(const LaunchPart &lp, hipStream_t st);
 hipError_t launch_colscan(const LaunchCol &lc, hipStream_t st);

@@ -3,11 +3,6 @@
 #include "../../include/hashgraph.h"
 
 extern "C" {
-hg_status hg_intersect_rle(hg_ctx, const hg_table_desc *, const void *, const void *, const hg_table_desc *,
-                           const void *, const void *, void *, uint64_t *, uint64_t *, uint64_t, uint64_t *,
-                           uint64_t *) {
-  return HG_ERR_UNSUPPORTED;
-}
 size_t hg_rmi_model_bytes(uint32_t) { return 0; }
 hg_status hg_rmi_train(hg_ctx, const void *, uint64_t, uint32_t, uint64_t, uint32_t, void *, uint32_t *) {
   return HG_ERR_UNSUPPORTED;

@@ -107,3 +107,99 @@ def sequential_keys(n: int, key_bytes: int, device=None):
         return np.arange(n, dtype=np.uint32 if key_bytes == 4 else np.uint64)
     import torch
     return torch.arange(n, dtype=torch.int32 if key_bytes == 4 else torch.int64, device=device)
+
+
+# ---------------------------------------------------------------------------
+# Zipf(s) over ranks 1..n by rejection-inversion (Hormann & Derflinger,
+# "Rejection-inversion to generate variates from monotone discrete
+# distributions", 1996).  Attempt t of element i draws its uniform from
+# element i*64 + t of the stream (53-bit doubles); an element that needs
+# more than 64 attempts (probability far below 1e-12) keeps rank 1.
+# Keys are rank - 1.
+# ---------------------------------------------------------------------------
+def _zipf_consts(s: float, n: int):
+    import math
+    def h_int(x):
+        lx = math.log(x)
+        t = (1.0 - s) * lx
+        return (math.expm1(t) / t if abs(t) > 1e-8 else 1.0 + t / 2.0) * lx
+    def h_inv(x):
+        t = x * (1.0 - s)
+        if t < -1.0:
+            t = -1.0
+        return math.exp((math.log1p(t) / t if abs(t) > 1e-8 else 1.0 - t / 2.0) * x)
+    hx1 = h_int(1.5) - 1.0
+    hn = h_int(n + 0.5)
+    sp = 2.0 - h_inv(h_int(2.5) - math.exp(-s * math.log(2.0)))
+    return hx1, hn, sp
+
+
+def _zipf_np(u, s, n, hx1, hn, sp):
+    """One rejection-inversion attempt for a vector of uniforms; returns (k, accepted)."""
+    x_u = hn + u * (hx1 - hn)
+    t = x_u * (1.0 - s)
+    t = np.maximum(t, -1.0)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        f1 = np.where(np.abs(t) > 1e-8, np.log1p(t) / t, 1.0 - t / 2.0)
+    x = np.exp(f1 * x_u)
+    k = np.floor(x + 0.5)
+    k = np.clip(k, 1.0, float(n))
+    lk = np.log(k + 0.5)
+    t2 = (1.0 - s) * lk
+    with np.errstate(divide="ignore", invalid="ignore"):
+        f2 = np.where(np.abs(t2) > 1e-8, np.expm1(t2) / t2, 1.0 + t2 / 2.0)
+    h_int_k = f2 * lk
+    hk = np.exp(-s * np.log(k))
+    acc = (k - x <= sp) | (x_u >= h_int_k - hk)
+    return k, acc
+
+
+def zipf_keys(seed: int, count: int, s: float = 1.1, universe: int = 1 << 32, device=None, chunk: int = 1 << 24):
+    """count keys = rank - 1 with rank ~ Zipf(s) over 1..universe (uint64 numpy, or int64 on device)."""
+    hx1, hn, sp = _zipf_consts(s, universe)
+    if device is None:
+        out = np.ones(count, dtype=np.uint64)
+        for c0 in range(0, count, chunk):
+            c1 = min(count, c0 + chunk)
+            idx = np.arange(c0, c1, dtype=np.uint64)
+            k_res = np.ones(c1 - c0)
+            todo = np.ones(c1 - c0, dtype=bool)
+            for t in range(64):
+                if not todo.any():
+                    break
+                ii = np.nonzero(todo)[0]
+                r = _np_mix64(np.uint64(mix64_int(seed)) + (idx[ii] * np.uint64(64) + np.uint64(t + 1)) *
+                              np.uint64(GAMMA))
+                u = (r >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+                k, acc = _zipf_np(u, s, universe, hx1, hn, sp)
+                k_res[ii[acc]] = k[acc]
+                todo[ii[acc]] = False
+            out[c0:c1] = (k_res - 1.0).astype(np.uint64)
+        return out
+    import torch
+    out = torch.empty(count, dtype=torch.int64, device=device)
+    s0 = _s64(mix64_int(seed))
+    for c0 in range(0, count, chunk):
+        c1 = min(count, c0 + chunk)
+        idx = torch.arange(c0, c1, dtype=torch.int64, device=device)
+        k_res = torch.ones(c1 - c0, dtype=torch.float64, device=device)
+        todo = torch.ones(c1 - c0, dtype=torch.bool, device=device)
+        for t in range(64):
+            ii = torch.nonzero(todo).flatten()
+            if ii.numel() == 0:
+                break
+            r = _t_mix64((idx[ii] * 64 + (t + 1)) * _s64(GAMMA) + s0)
+            u = _t_shr(r, 11).to(torch.float64) * (1.0 / 9007199254740992.0)
+            x_u = hn + u * (hx1 - hn)
+            tt = torch.clamp(x_u * (1.0 - s), min=-1.0)
+            f1 = torch.where(tt.abs() > 1e-8, torch.log1p(tt) / tt, 1.0 - tt / 2.0)
+            x = torch.exp(f1 * x_u)
+            k = torch.clamp(torch.floor(x + 0.5), 1.0, float(universe))
+            lk = torch.log(k + 0.5)
+            t2 = (1.0 - s) * lk
+            f2 = torch.where(t2.abs() > 1e-8, torch.expm1(t2) / t2, 1.0 + t2 / 2.0)
+            acc = (k - x <= sp) | (x_u >= f2 * lk - torch.exp(-s * torch.log(k)))
+            k_res[ii[acc]] = k[acc]
+            todo[ii[acc]] = False
+        out[c0:c1] = (k_res - 1.0).to(torch.int64)
+    return out

@@ -1,0 +1,118 @@
+"""GPU parity of the one-GPU MultiHashGraph path: split finding, partitioned
+build and the run-length join, against the oracle (the reference's
+buildSingle / CPU-intersect checks, src/MultiHashGraph.cu:742-862,
+test/MultiHashGraphTest.cu:340-425)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+from hashgraph_amd import workloads as wl
+from oracle import oracle as o
+
+
+def _to_dev(a, dev):
+    import torch
+    return torch.from_numpy(a.view(np.int32 if a.dtype == np.uint32 else np.int64).copy()).to(dev)
+
+
+@pytest.mark.parametrize("n,M,P,kb", [(100000, 80000, 16, 8), (1 << 20, 1 << 20, 16, 4), (5000, 5000, 3, 4),
+                                      (200000, 1 << 18, 1, 8)])
+def test_find_splits(n, M, P, kb, dev):
+    import hashgraph_amd as hg
+    keys = wl.uniform_keys(n, n, kb)
+    got = hg.find_splits(_to_dev(keys, dev), M, 16000, P)
+    exp, _ = o.find_splits(o.bins(keys, M), M, 16000, P)
+    assert got == exp.tolist()
+
+
+def _check_partitioned(keys, M, P, dev, ob=4):
+    import hashgraph_amd as hg
+    print(f"partitioned n={len(keys)} M={M} P={P} kb={keys.itemsize} ob={ob}", flush=True)
+    dk = _to_dev(keys, dev)
+    splits = hg.find_splits(dk, M, 16000, P)
+    off, csr, po = hg.build_partitioned(dk, M, splits, offset_bytes=ob)
+    eoff, ecsr = o.build_multi(keys, M, np.array(splits, dtype=np.int64))
+    goff = off.cpu().numpy().view(np.uint32 if ob == 4 else np.uint64).astype(np.uint64)
+    assert np.array_equal(goff, eoff), "per-partition offsets differ"
+    gcsr = csr.cpu().numpy().view(keys.dtype).copy()
+    # canonicalise each partition's bins and compare
+    for p in range(P):
+        lo, hi = splits[p], splits[p + 1]
+        loc = eoff[lo + p: hi + p + 1]
+        a = gcsr[po[p]: po[p + 1]].copy()
+        b = ecsr[po[p]: po[p + 1]].copy()
+        o.canon(loc, a)
+        o.canon(loc, b)
+        assert np.array_equal(a, b), f"partition {p} bins differ"
+    return splits
+
+
+def test_partitioned_uniform(dev):
+    keys = wl.uniform_keys(21, 1 << 20, 8)
+    _check_partitioned(keys, 1 << 20, 16, dev)
+
+
+def test_partitioned_zipf(dev):
+    keys = wl.zipf_keys(3, 1 << 20, 1.1)
+    _check_partitioned(keys, 1 << 20, 16, dev, ob=8)
+
+
+def test_partitioned_u32_few_parts(dev):
+    keys = wl.uniform_keys(22, 300000, 4)
+    _check_partitioned(keys, 250000, 3, dev)
+
+
+def _rle_expected(A, B, M):
+    oa, ka = o.build_csr(A, M)
+    ob, kb = o.build_csr(B, M)
+    o.canon(oa, ka)
+    o.canon(ob, kb)
+    return o.intersect_rle(oa, ka, ob, kb)
+
+
+def _check_rle(A, B, M, dev):
+    import hashgraph_amd as hg
+    print(f"rle nA={len(A)} nB={len(B)} M={M} kb={A.itemsize}", flush=True)
+    offA, csrA = hg.build(_to_dev(A, dev), M)
+    offB, csrB = hg.build(_to_dev(B, dev), M)
+    cnt, rk, ca, cb = hg.intersect_rle(offA, csrA, offB, csrB, M)
+    ecnt, ek, eca, ecb = _rle_expected(A, B, M)
+    assert cnt == ecnt
+    rk = rk.cpu().numpy().view(A.dtype).astype(np.uint64)
+    order = np.argsort(rk, kind="stable")
+    eorder = np.argsort(ek, kind="stable")
+    assert np.array_equal(rk[order], ek[eorder])
+    assert np.array_equal(ca.cpu().numpy()[order].astype(np.uint64), eca[eorder])
+    assert np.array_equal(cb.cpu().numpy()[order].astype(np.uint64), ecb[eorder])
+
+
+def test_rle_uniform(dev):
+    A = wl.uniform_keys(1, 1 << 18, 4)
+    B = wl.probe_keys(A, 2, 1 << 18, 0.5, 4)
+    _check_rle(A, B, 1 << 18, dev)
+
+
+def test_rle_zipf_heavy(dev):
+    # C3's shape at 2^20: the top key holds ~10% of the keys (heavy bins)
+    A = wl.zipf_keys(3, 1 << 20, 1.1)
+    B = wl.zipf_keys(4, 1 << 19, 1.1)
+    _check_rle(A, B, 1 << 20, dev)
+
+
+def test_rle_all_ones_key(dev):
+    # the all-ones key is the heavy table's sentinel: counted apart
+    A = np.concatenate([np.full(5000, 2**64 - 1, dtype=np.uint64), wl.uniform_keys(5, 3000, 8)])
+    B = np.concatenate([np.full(300, 2**64 - 1, dtype=np.uint64), A[5000:5100]])
+    _check_rle(A, B, 1024, dev)
+
+
+def test_rle_count_matches_bag_count(dev):
+    A = wl.zipf_keys(7, 300000, 1.2)
+    B = wl.zipf_keys(8, 200000, 1.2)
+    import hashgraph_amd as hg
+    M = 1 << 17
+    offA, csrA = hg.build(_to_dev(A, dev), M)
+    offB, csrB = hg.build(_to_dev(B, dev), M)
+    cnt, *_ = hg.intersect_rle(offA, csrA, offB, csrB, M)
+    assert cnt == o.bag_intersect_count(A, B)

@@ -39,3 +39,19 @@ def test_probe_fraction():
     b = wl.probe_keys(a, 2, 1 << 16, 0.5, 4)
     frac = np.isin(b, a).mean()
     assert 0.49 < frac < 0.52
+
+
+def test_zipf_matches_pmf_and_torch():
+    import torch
+    from scipy.special import zeta
+    s, n = 1.1, 1 << 32
+    z = wl.zipf_keys(3, 1 << 18, s)
+    Z = zeta(s) - n ** (1 - s) / (s - 1)  # sum_{k<=n} k^-s (Euler-Maclaurin tail)
+    for r in range(1, 8):
+        p = r ** -s / Z
+        f = float((z == r - 1).mean())
+        sd = (p * (1 - p) / len(z)) ** 0.5
+        assert abs(f - p) < 5 * sd, (r, f, p)
+    assert int(z.max()) < n
+    zt = wl.zipf_keys(3, 1 << 14, s, device=torch.device("cpu"))
+    assert np.array_equal(zt.numpy().view(np.uint64), wl.zipf_keys(3, 1 << 14, s))
