@@ -257,16 +257,38 @@ def rmi_model_bytes(n_leaves: int = 4096) -> int:
     return int(_lib.load().hg_rmi_model_bytes(n_leaves))
 
 
-def rmi_train(sample, n_bins: int, n_leaves: int = 4096):
-    """Fit the learned CDF model on a key sample.  Returns (model tensor, use_rmi)."""
+def rmi_train(keys, n_bins: int, n_sample: int | None = None, seed: int = 7, n_leaves: int = 4096,
+              report: bool = False):
+    """Fit the learned CDF model on a seeded sample of `keys` (default 1%).
+    Returns (model tensor, use_rmi) or, with report=True, (model, use_rmi, dict)
+    where the dict holds the sampled max-bin / mean-probe figures of both hashes."""
     torch = _torch()
-    _check_dev(sample)
-    c = context(sample.device.index)
-    model = torch.empty(rmi_model_bytes(n_leaves), dtype=torch.uint8, device=sample.device)
+    _check_dev(keys)
+    c = context(keys.device.index)
+    n = keys.numel()
+    if n_sample is None:
+        n_sample = max(1, n // 100)
+    model = torch.empty(rmi_model_bytes(n_leaves), dtype=torch.uint8, device=keys.device)
     use = ctypes.c_uint32(0)
-    check(c.lib.hg_rmi_train(c.bind_stream(), _p(sample), sample.numel(), _key_bytes(sample), n_bins, n_leaves,
-                             _p(model), ctypes.byref(use)), "hg_rmi_train")
-    return model, bool(use.value)
+    rep = _lib.RmiReport()
+    check(c.lib.hg_rmi_train(c.bind_stream(), _p(keys), n, _key_bytes(keys), n_sample, seed, n_bins, n_leaves,
+                             _p(model), ctypes.byref(use), ctypes.byref(rep)), "hg_rmi_train")
+    if not report:
+        return model, bool(use.value)
+    return model, bool(use.value), {
+        "sample_bins": rep.sample_bins, "max_bin_rmi": rep.max_bin_rmi, "max_bin_murmur": rep.max_bin_murmur,
+        "mean_probe_rmi": rep.mean_probe_rmi, "mean_probe_murmur": rep.mean_probe_murmur}
+
+
+def table_bins(keys, n_bins: int, hash_kind: int = HG_HASH_MURMUR3, rmi_model=None):
+    """Bins of `keys` under a table's hash (murmur3 or the learned model), int32 tensor of uint32."""
+    torch = _torch()
+    _check_dev(keys)
+    c = context(keys.device.index)
+    out = torch.empty(keys.numel(), dtype=torch.int32, device=keys.device)
+    d = _desc(keys.numel(), n_bins, _key_bytes(keys), 4, hash_kind, rmi_model)
+    check(c.lib.hg_table_bins(c.bind_stream(), ctypes.byref(d), _p(keys), keys.numel(), _p(out)), "hg_table_bins")
+    return out
 
 
 def _batch_desc(key_off, n_bins, off_off, key_bytes, hash_kinds=None, rmi_models=None, rmi_leaves=0):

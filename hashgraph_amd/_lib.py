@@ -24,7 +24,7 @@ EXPORTS = [
     "hg_ctx_create", "hg_ctx_destroy", "hg_ctx_set_stream", "hg_ctx_reserve", "hg_scratch_bytes", "hg_last_error",
     "hg_version", "hg_ctx_profile", "hg_ctx_profile_read", "hg_build", "hg_intersect", "hg_probe", "hg_intersect_rle", "hg_find_splits",
     "hg_build_partitioned", "hg_rmi_model_bytes", "hg_rmi_train", "hg_batch_build", "hg_batch_probe",
-    "hg_exclusive_scan", "hg_hash_bins",
+    "hg_exclusive_scan", "hg_hash_bins", "hg_table_bins",
 ]
 
 
@@ -38,6 +38,12 @@ class TableDesc(ctypes.Structure):
     _fields_ = [("key_bytes", ctypes.c_uint32), ("offset_bytes", ctypes.c_uint32), ("n_keys", ctypes.c_uint64),
                 ("n_bins", ctypes.c_uint64), ("hash_kind", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
                 ("d_rmi_model", ctypes.c_void_p)]
+
+
+class RmiReport(ctypes.Structure):
+    _fields_ = [("sample_bins", ctypes.c_uint64), ("max_bin_rmi", ctypes.c_uint64),
+                ("max_bin_murmur", ctypes.c_uint64), ("mean_probe_rmi", ctypes.c_double),
+                ("mean_probe_murmur", ctypes.c_double)]
 
 
 class KernelTime(ctypes.Structure):
@@ -84,11 +90,12 @@ def load(path: str = LIB_PATH):
             "hg_find_splits": (i32, [vp, P(TableDesc), vp, u64, u32, P(ctypes.c_int64)]),
             "hg_build_partitioned": (i32, [vp, P(TableDesc), vp, P(ctypes.c_int64), u32, vp, vp, P(u64)]),
             "hg_rmi_model_bytes": (ctypes.c_size_t, [u32]),
-            "hg_rmi_train": (i32, [vp, vp, u64, u32, u64, u32, vp, P(u32)]),
+            "hg_rmi_train": (i32, [vp, vp, u64, u32, u64, u64, u64, u32, vp, P(u32), P(RmiReport)]),
             "hg_batch_build": (i32, [vp, P(BatchDesc), vp, vp, vp]),
             "hg_batch_probe": (i32, [vp, P(BatchDesc), vp, vp, vp, P(u64), vp, P(u64)]),
             "hg_exclusive_scan": (i32, [vp, vp, vp, u64, u32]),
             "hg_hash_bins": (i32, [vp, vp, u64, u32, u64, vp]),
+            "hg_table_bins": (i32, [vp, P(TableDesc), vp, u64, vp]),
         }
         for name, (res, args) in sig.items():
             if hasattr(L, name):

@@ -87,25 +87,31 @@ struct FinItem {
   uint32_t table;   // index into the BinMap array
 };
 
-// Learned CDF model (no counterpart in the reference).  Two stages:
-//   leaf  = clamp(floor(root_slope * (float)(key - key_min)), 0, L-1)
-//   bin   = clamp(base[leaf] + floor(slope[leaf] * (float)(key - x0[leaf])),
-//                 lo[leaf], hi[leaf])
-// The integer anchors (key_min, x0, base) keep FP32 precision where it is
-// needed: the float only ever holds a distance inside one leaf.  Leaves are
-// stored as arrays of RmiLeaf after the header.
+// Learned CDF model (no counterpart in the reference).  Two FP32 linear
+// stages over integer anchors:
+//   leaf = clamp(floor(root_slope * f(key - key_min) + root_icpt), 0, L-1)
+//   bin  = min(base[leaf] + floor(slope[leaf] * f(key - x0[leaf])),
+//              base[leaf+1], M-1)
+// f() is the round-to-nearest u64 -> f32 conversion, every product and sum is
+// rounded separately (no fused multiply-add), differences below an anchor
+// count as 0, and floor saturates at 2^32-256.  The anchors keep FP32
+// precision where it is needed: the float only holds a distance from the
+// smallest key or from the first training key of one leaf.  The header is
+// followed by L+1 leaves; leaf L is a sentinel whose base is M-1.  Bases are
+// non-decreasing, so the map is monotone in the key.
 struct RmiHeader {
   uint64_t key_min;
   float root_slope;
+  float root_icpt;
   uint32_t n_leaves;
   uint32_t n_bins;
-  uint32_t pad;
+  uint32_t pad[2];
 };
 struct RmiLeaf {
-  uint64_t x0;      // smallest sample key routed to the leaf
-  float slope;      // bins per key unit
-  uint32_t base;    // bin of x0
-  uint32_t lo, hi;  // bin range the leaf may emit (monotone, non-overlapping)
+  uint64_t x0;     // first training key routed to the leaf
+  float slope;     // bins per key unit
+  uint32_t base;   // bin of x0
 };
+static_assert(sizeof(RmiHeader) == 32 && sizeof(RmiLeaf) == 16, "RMI layout");
 
 }  // namespace hg

@@ -53,37 +53,49 @@ __device__ __forceinline__ uint32_t fastdiv(uint32_t a, const Div32 &dv) {
   return dv.w == 1 ? a : (uint32_t)__umul64hi(dv.d64, (uint64_t)a);
 }
 
-// Bin of a key under the learned CDF model (layout in hg_common.hpp).
+// Learned CDF model (layout and rounding rules in hg_common.hpp).
+__device__ __forceinline__ uint32_t rmi_leaf(uint64_t key, const RmiHeader &h) {
+  const float xr = key > h.key_min ? (float)(key - h.key_min) : 0.0f;
+  const float t = __fadd_rn(__fmul_rn(h.root_slope, xr), h.root_icpt);
+  const uint32_t L = h.n_leaves;
+  return !(t > 0.0f) ? 0u : (t >= (float)(L - 1) ? L - 1 : (uint32_t)t);
+}
+
 __device__ __forceinline__ uint32_t rmi_bin(uint64_t key, const void *model, uint32_t M) {
   const RmiHeader *h = (const RmiHeader *)model;
   const RmiLeaf *lv = (const RmiLeaf *)(h + 1);
-  const uint32_t L = h->n_leaves;
-  float xr = key > h->key_min ? (float)(key - h->key_min) : 0.0f;
-  float lf = __fmul_rn(h->root_slope, xr);
-  uint32_t leaf = lf <= 0.0f ? 0u : (lf >= (float)(L - 1) ? L - 1 : (uint32_t)lf);
+  const uint32_t leaf = rmi_leaf(key, *h);
   const RmiLeaf l = lv[leaf];
-  float d = key > l.x0 ? (float)(key - l.x0) : 0.0f;
-  float y = __fmul_rn(l.slope, d);
-  uint32_t off = y <= 0.0f ? 0u : (y >= 4294967040.0f ? 0xFFFFFF00u : (uint32_t)y);
+  const uint32_t hi = lv[leaf + 1].base;
+  const float d = key > l.x0 ? (float)(key - l.x0) : 0.0f;
+  const float y = __fmul_rn(l.slope, d);
+  const uint32_t off = !(y > 0.0f) ? 0u : (y >= 4294967040.0f ? 0xFFFFFF00u : (uint32_t)y);
   uint32_t b = l.base + off;
-  if (b < l.base) b = l.hi;  // wrapped
-  b = b < l.lo ? l.lo : (b > l.hi ? l.hi : b);
+  if (b < l.base || b > hi) b = hi;
   return b < M ? b : M - 1;
 }
 
-template <typename K>
+// Hash kinds a kernel is compiled for: only the selected path is inlined
+// into every unrolled slot (keeps the kernels' code inside the I-cache).
+enum : int { HK_MURMUR = 0, HK_RMI = 1, HK_ANY = 2 };
+
+template <typename K, int HK = HK_ANY>
 __device__ __forceinline__ uint32_t bin_of(K key, const BinMap &m) {
-  if (m.kind == 0) {
+  if (HK == HK_MURMUR || (HK == HK_ANY && m.kind == 0)) {
     uint32_t h = murmur4((uint32_t)key);
     return (m.M & (m.M - 1)) == 0 ? (h & (m.M - 1)) : fastmod(h, m.m64, m.M);
   }
   return rmi_bin((uint64_t)key, m.rmi, m.M);
 }
 
-template <typename K>
+// Bucket kinds: SK_RANGE = (bin - bin_lo) / width, SK_SPLIT = partition by
+// split points, SK_ANY = chosen per segment at run time.
+enum : int { SK_RANGE = 0, SK_SPLIT = 1, SK_ANY = 2 };
+
+template <typename K, int HK = HK_ANY, int SK = SK_ANY>
 __device__ __forceinline__ uint32_t bucket_of(K key, const SegDesc &sd) {
-  uint32_t bin = bin_of<K>(key, sd.map);
-  if (sd.kind == 0) return fastdiv(bin - (uint32_t)sd.bin_lo, sd.div);
+  uint32_t bin = bin_of<K, HK>(key, sd.map);
+  if (SK == SK_RANGE || (SK == SK_ANY && sd.kind == 0)) return fastdiv(bin - (uint32_t)sd.bin_lo, sd.div);
   uint32_t p = 0;
   while (p < sd.nsplits && (int64_t)bin >= sd.splits[p + 1]) p++;
   return p;  // p == nsplits: past the last split

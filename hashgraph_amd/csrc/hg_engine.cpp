@@ -40,6 +40,12 @@ constexpr uint32_t kTileKpt = 32;     // keys per thread of a partition tile
 constexpr uint32_t kTile = 1024 * kTileKpt;
 constexpr uint32_t kChunk = 64;       // count-matrix rows per column-scan chunk
 
+uint64_t splitmix_mix(uint64_t z) {
+  z = (z ^ (z >> 30)) * UINT64_C(0xBF58476D1CE4E5B9);
+  z = (z ^ (z >> 27)) * UINT64_C(0x94D049BB133111EB);
+  return z ^ (z >> 31);
+}
+
 uint64_t lemire_m64(uint64_t d) { return d > 1 ? (UINT64_C(0xFFFFFFFFFFFFFFFF) / d + 1) : 0; }
 
 Div32 make_div(uint32_t w) {
@@ -252,6 +258,19 @@ struct Level {
   uint32_t ncols_total = 0, max_cols = 0;
   uint64_t cmat_elems = 0, psum_elems = 0;
 
+  // One kind shared by all segments selects a specialised kernel.
+  int hash_kind() const {
+    int k = -1;
+    for (const BinMap &m : maps) k = (k < 0 || k == (int)m.kind) ? (int)m.kind : KIND_MIXED;
+    for (const SegDesc &sd : segs) k = (k < 0 || k == (int)sd.map.kind) ? (int)sd.map.kind : KIND_MIXED;
+    return k < 0 ? 0 : k;
+  }
+  int bucket_kind() const {
+    int k = -1;
+    for (const SegDesc &sd : segs) k = (k < 0 || k == (int)sd.kind) ? (int)sd.kind : KIND_MIXED;
+    return k < 0 ? 0 : k;
+  }
+
   void prepare() {
     tiles.clear();
     items.clear();
@@ -364,7 +383,8 @@ hg_status setup_level(hg_ctx c, Level &L, LevelDev &d) {
 hg_status run_level(hg_ctx c, const Level &L, const LevelDev &d, int kb, const void *in, void *out) {
   LaunchPart p{};
   p.key_bytes = kb;
-  p.kpt = (int)kTileKpt;
+  p.hk = L.hash_kind();
+  p.sk = L.bucket_kind();
   p.ntiles = (uint32_t)L.tiles.size();
   p.max_cols = L.max_cols;
   p.stage_cap = 8192;
@@ -422,6 +442,7 @@ hg_status run_final(hg_ctx c, const Level &L, const LevelDev &d, int kb, int ob,
   LaunchFinal f{};
   f.key_bytes = kb;
   f.off_bytes = ob;
+  f.hk = L.hash_kind();
   f.nitems = L.ncols_total;
   f.lds_bytes = 128 * 1024;
   f.part = part;
@@ -688,6 +709,86 @@ hg_status hg_build(hg_ctx c, const hg_table_desc *desc, const void *d_keys, void
   if (!d_offsets || (desc->n_keys && (!d_keys || !d_csr_keys))) return fail(HG_ERR_INVALID_ARG, "null buffer");
   DeviceGuard g(c->device);
   return build_table(c, desc, d_keys, d_offsets, d_csr_keys);
+}
+
+hg_status hg_table_bins(hg_ctx c, const hg_table_desc *desc, const void *d_keys, uint64_t n, uint32_t *d_bins) {
+  if (!c) return fail(HG_ERR_INVALID_ARG, "null ctx");
+  HG_TRY(check_desc(desc));
+  if (n && (!d_keys || !d_bins)) return fail(HG_ERR_INVALID_ARG, "null buffer");
+  DeviceGuard g(c->device);
+  HG_HIP(launch_bins(d_keys, n, (int)desc->key_bytes, make_map(desc->n_bins, desc->hash_kind, desc->d_rmi_model), d_bins,
+                     c->stream));
+  return HG_OK;
+}
+
+size_t hg_rmi_model_bytes(uint32_t n_leaves) {
+  if (n_leaves == 0 || n_leaves > 65536) return 0;
+  return sizeof(RmiHeader) + ((size_t)n_leaves + 1) * sizeof(RmiLeaf);
+}
+
+hg_status hg_rmi_train(hg_ctx c, const void *d_keys, uint64_t n_keys, uint32_t key_bytes, uint64_t n_sample,
+                       uint64_t seed, uint64_t n_bins, uint32_t n_leaves, void *d_model, uint32_t *h_use_rmi,
+                       hg_rmi_report *h_report) {
+  if (!c) return fail(HG_ERR_INVALID_ARG, "null ctx");
+  if (key_bytes != 4 && key_bytes != 8) return fail(HG_ERR_INVALID_ARG, "key_bytes must be 4 or 8");
+  if (n_bins == 0 || n_bins >= (1ull << 32)) return fail(HG_ERR_INVALID_ARG, "n_bins must be in [1, 2^32)");
+  if (n_leaves == 0 || n_leaves > 65536) return fail(HG_ERR_INVALID_ARG, "n_leaves must be in [1, 65536]");
+  if (n_keys == 0 || n_sample == 0 || n_sample >= (1ull << 31))
+    return fail(HG_ERR_INVALID_ARG, "need n_keys >= 1 and n_sample in [1, 2^31)");
+  if (!d_keys || !d_model || !h_use_rmi) return fail(HG_ERR_INVALID_ARG, "null argument");
+  DeviceGuard g(c->device);
+  const uint32_t ntiles = rmi_sort_tiles(n_sample);
+  const uint32_t nblk = rmi_moment_blocks(n_sample);
+  // sample bins at the table's load factor
+  long double mp = (long double)n_bins * (long double)n_sample / (long double)n_keys;
+  uint64_t Mp = (uint64_t)(mp + 0.5L);
+  Mp = std::max<uint64_t>(1, std::min<uint64_t>(Mp, n_bins));
+  Plan P;
+  const size_t i_b0 = P.add(n_sample * key_bytes), i_b1 = P.add(n_sample * key_bytes);
+  const size_t i_cnt = P.add(((size_t)256 * ntiles + 1) * 4);
+  const size_t i_st = P.add(scan_tiles((uint64_t)256 * ntiles) * 8), i_tk = P.add(64);
+  const size_t i_part = P.add((size_t)nblk * 4 * sizeof(double));
+  const size_t i_start = P.add(((size_t)n_leaves + 1) * 4);
+  const size_t i_cr = P.add(Mp * 4), i_cm = P.add(Mp * 4), i_out = P.add(64);
+  HG_TRY(arena_ensure(c, 1, P.total()));
+  Carve cv;
+  cv.init(c, 1, P);
+  LaunchRmi r{};
+  r.key_bytes = (int)key_bytes;
+  r.keys = d_keys;
+  r.n_keys = n_keys;
+  r.n_sample = n_sample;
+  r.s0 = splitmix_mix(seed);
+  r.buf0 = cv.get<void>(i_b0);
+  r.buf1 = cv.get<void>(i_b1);
+  r.counts = cv.get<uint32_t>(i_cnt);
+  r.scan_status = cv.get<unsigned long long>(i_st);
+  r.scan_ticket = cv.get<unsigned int>(i_tk);
+  r.partials = cv.get<double>(i_part);
+  r.nblk = nblk;
+  r.n_leaves = n_leaves;
+  r.map = make_map(n_bins, 0, nullptr);
+  r.model = d_model;
+  r.start = cv.get<uint32_t>(i_start);
+  r.Mp = (uint32_t)Mp;
+  r.cnt_r = cv.get<uint32_t>(i_cr);
+  r.cnt_m = cv.get<uint32_t>(i_cm);
+  r.out = cv.get<unsigned long long>(i_out);
+  {
+    Prof pf(c, "rmi_train");
+    HG_HIP(launch_rmi_train(r, c->stream));
+  }
+  unsigned long long res[4];
+  HG_TRY(sync_read(c, res, r.out, sizeof(res)));
+  *h_use_rmi = (res[0] <= res[2] && res[1] <= res[3]) ? 1u : 0u;
+  if (h_report) {
+    h_report->sample_bins = Mp;
+    h_report->max_bin_rmi = res[0];
+    h_report->max_bin_murmur = res[2];
+    h_report->mean_probe_rmi = (double)res[1] / (double)n_sample;
+    h_report->mean_probe_murmur = (double)res[3] / (double)n_sample;
+  }
+  return HG_OK;
 }
 
 hg_status hg_hash_bins(hg_ctx c, const void *d_keys, uint64_t n, uint32_t key_bytes, uint64_t n_bins, uint32_t *d_bins) {

@@ -10,6 +10,7 @@
 // are replaced by LDS work and deterministic positions.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <type_traits>
 #include "hg_common.hpp"
 #include "hg_device.hpp"
 #include "hg_kernels.hpp"
@@ -29,15 +30,34 @@ constexpr uint64_t JOIN_PAIR_LIMIT = 1ull << 22;
 #endif
 constexpr int KPT_U32 = HG_KPT_U32;
 
+// Probe staging: bins per LDS sub-range (by offset width) and the LDS bytes
+// of one probe workgroup (two workgroups fit a CU's 160 KB).
+#ifndef HG_PROBE_LDS
+#define HG_PROBE_LDS (78 * 1024)
+#endif
+template <typename OFF> constexpr int probe_sub() { return sizeof(OFF) == 4 ? 8192 : 4096; }
+constexpr size_t PROBE_LDS = HG_PROBE_LDS;
+
+// Workgroup b of a partition pass runs on XCD b % 8 (round-robin dispatch).
+// Tiles are handed out so that each XCD walks a contiguous range of tiles:
+// the runs that concurrently running tiles write for one bucket are then
+// adjacent and written from one XCD, so they merge in that XCD's L2 instead
+// of reaching HBM as partial lines.
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t nt) {
+  const uint32_t q = nt >> 3, r = nt & 7, x = b & 7, i = b >> 3;
+  return x * q + min(x, r) + i;
+}
+
 // ---------------------------------------------------------------------------
 // Partition pass: histogram.  One workgroup per tile; tile t of segment s
 // writes row t of the segment's count matrix (ncols u32).
 // ---------------------------------------------------------------------------
-template <typename K, int BLOCK, int KPT>
+template <typename K, int BLOCK, int KPT, int HK, int SK>
 __global__ __launch_bounds__(BLOCK) void k_part_hist(const K *__restrict__ keys, const TileDesc *__restrict__ tiles,
                                                      const SegDesc *__restrict__ segs, uint32_t *__restrict__ cmat) {
   extern __shared__ uint32_t s_cnt[];
-  const TileDesc td = tiles[blockIdx.x];
+  const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
+  const TileDesc td = tiles[tile];
   const SegDesc &sd = segs[td.seg];
   const uint32_t ncols = sd.ncols;
   for (uint32_t i = threadIdx.x; i < ncols; i += BLOCK) s_cnt[i] = 0;
@@ -51,9 +71,9 @@ __global__ __launch_bounds__(BLOCK) void k_part_hist(const K *__restrict__ keys,
   for (int j = 0; j < KPT; j++) kr[j] = buf_load<K>(rs, threadIdx.x * (uint32_t)sizeof(K), j * BLOCK * (uint32_t)sizeof(K));
 #pragma unroll
   for (int j = 0; j < KPT; j++)
-    if (j < nv) atomicAdd(&s_cnt[bucket_of<K>(kr[j], sd)], 1u);
+    if (j < nv) atomicAdd(&s_cnt[bucket_of<K, HK, SK>(kr[j], sd)], 1u);
   __syncthreads();
-  uint32_t *row = cmat + sd.row_base + (uint64_t)(blockIdx.x - sd.tile_begin) * ncols;
+  uint32_t *row = cmat + sd.row_base + (uint64_t)(tile - sd.tile_begin) * ncols;
   for (uint32_t i = threadIdx.x; i < ncols; i += BLOCK) row[i] = s_cnt[i];
 }
 
@@ -144,13 +164,14 @@ __global__ __launch_bounds__(256) void k_col_apply(uint32_t *__restrict__ cmat, 
 // (tile, bucket) run contiguously at its count-matrix position.
 // Requires tile count <= 65536 and ncols <= 65536.
 // ---------------------------------------------------------------------------
-template <typename K, int BLOCK, int KPT>
+template <typename K, int BLOCK, int KPT, int HK, int SK>
 __global__ __launch_bounds__(BLOCK) void k_part_scatter(const K *__restrict__ keys, const TileDesc *__restrict__ tiles,
                                                         const SegDesc *__restrict__ segs, const uint32_t *__restrict__ cpos,
                                                         K *__restrict__ out, uint32_t stage_cap) {
   extern __shared__ __align__(16) uint8_t smem[];
   __shared__ uint32_t s_tmp[BLOCK / 64 + 1];
-  const TileDesc td = tiles[blockIdx.x];
+  const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
+  const TileDesc td = tiles[tile];
   const SegDesc &sd = segs[td.seg];
   const uint32_t ncols = sd.ncols;
   uint32_t *s_cnt = (uint32_t *)smem;
@@ -169,7 +190,7 @@ __global__ __launch_bounds__(BLOCK) void k_part_scatter(const K *__restrict__ ke
 #pragma unroll
   for (int j = 0; j < KPT; j++) {
     if (j < nv) {
-      const uint32_t b = bucket_of<K>(kr[j], sd);
+      const uint32_t b = bucket_of<K, HK, SK>(kr[j], sd);
       const uint32_t r = atomicAdd(&s_cnt[b], 1u);
       pk[j] = (b << 16) | r;
     }
@@ -184,7 +205,7 @@ __global__ __launch_bounds__(BLOCK) void k_part_scatter(const K *__restrict__ ke
     }
   }
   __syncthreads();
-  const uint32_t *row = cpos + sd.row_base + (uint64_t)(blockIdx.x - sd.tile_begin) * ncols;
+  const uint32_t *row = cpos + sd.row_base + (uint64_t)(tile - sd.tile_begin) * ncols;
   for (uint32_t b = threadIdx.x; b < ncols; b += BLOCK) s_cnt[b] = row[b] - s_cnt[b];
   __syncthreads();
   const uint32_t nb = sd.nbuckets;
@@ -216,7 +237,7 @@ __global__ __launch_bounds__(BLOCK) void k_part_scatter(const K *__restrict__ ke
 // word); their exclusive scan gives the offsets, a second atomic pass gives
 // every key's slot, then LDS is reused to stage the bin-ordered keys.
 // ---------------------------------------------------------------------------
-template <typename K, typename OFF, int BLOCK, int KPT>
+template <typename K, typename OFF, int BLOCK, int KPT, int HK>
 __global__ __launch_bounds__(BLOCK) void k_final(const K *__restrict__ part, K *__restrict__ csr, OFF *__restrict__ off,
                                                  const FinItem *__restrict__ items, const BinMap *__restrict__ maps,
                                                  uint32_t lds_bytes) {
@@ -242,7 +263,7 @@ __global__ __launch_bounds__(BLOCK) void k_final(const K *__restrict__ part, K *
 #pragma unroll
   for (int j = 0; j < (KPT + 1) / 2; j++) lb[j] = 0;
   auto count_slot = [&](int j) {
-    const uint32_t l = bin_of<K>(kr[j], mp) - it.bin_lo;
+    const uint32_t l = bin_of<K, HK>(kr[j], mp) - it.bin_lo;
     lb[j >> 1] |= l << ((j & 1) * 16);
     atomicAdd(&smem32[l >> 1], 1u << ((l & 1) * 16));
   };
@@ -355,12 +376,17 @@ __global__ __launch_bounds__(BLOCK) void k_final_big(const K *__restrict__ part,
 }
 
 // ---------------------------------------------------------------------------
-// Probe: B's keys of one bucket stay in VGPRs; A's offsets and keys of each
-// sub-range of SUB bins are staged in LDS; every B key walks its A bin.
-// Matches are emitted per sub-range through one atomic reservation per
-// workgroup (output grouped by bucket and sub-range).
+// Probe: B's keys of one bucket stay in VGPRs with their 16-bit local bins.
+// The bucket's bins of A are visited in sub-ranges of at most SUB bins and at
+// most akey_cap keys; each sub-range's offsets and keys are staged in LDS
+// and every B key in it walks its A bin there, accumulating a 16-bit match
+// count.  A match is always the probe key itself, so the emission needs no
+// second walk: after the last sub-range one block scan and one atomic
+// reservation per workgroup place each key's copies (output grouped by
+// bucket).  A bin longer than PROBE_BIN_LIMIT is not walked; it raises
+// *flag and the host reports the probe as unsupported.
 // ---------------------------------------------------------------------------
-template <typename K, typename OFF, int BLOCK, int KPT, int SUB>
+template <typename K, typename OFF, int BLOCK, int KPT, int SUB, int HK>
 __global__ __launch_bounds__(BLOCK) void k_probe(const K *__restrict__ bpart, const FinItem *__restrict__ items,
                                                  const BinMap *__restrict__ maps, const OFF *__restrict__ offA,
                                                  const K *__restrict__ keysA, K *__restrict__ out, uint64_t capacity,
@@ -368,87 +394,82 @@ __global__ __launch_bounds__(BLOCK) void k_probe(const K *__restrict__ bpart, co
                                                  unsigned int *__restrict__ flag) {
   extern __shared__ __align__(16) uint8_t smem[];
   __shared__ uint32_t s_tmp[BLOCK / 64 + 1];
+  __shared__ uint32_t s_n;
   __shared__ unsigned long long s_base;
   OFF *s_off = (OFF *)smem;
   K *s_ak = (K *)(smem + ((SUB + 2) * sizeof(OFF) + 15) / 16 * 16);
   const FinItem it = items[blockIdx.x];
-  if (it.count > BLOCK * KPT || it.count > 65535u) return;
+  if (it.count > BLOCK * KPT || it.count > 65535u) return;  // k_probe_big's item
   const BinMap mp = maps[it.table];
   const uint32_t cnt = it.count, W = it.nbins;
   const K *kp = bpart + it.key_start;
+  const K *ka = keysA + it.key_rel;  // A's keys of this table (batched tables)
   const uint32_t nv = cnt > threadIdx.x ? (cnt - threadIdx.x + BLOCK - 1) / BLOCK : 0;
   const __amdgpu_buffer_rsrc_t rs = make_rsrc(kp, cnt * (uint32_t)sizeof(K));
   K kr[KPT];
-  uint32_t lb[(KPT + 1) / 2];
+  uint32_t lb[(KPT + 1) / 2], mc[(KPT + 1) / 2];
 #pragma unroll
   for (int j = 0; j < KPT; j++) kr[j] = buf_load<K>(rs, threadIdx.x * (uint32_t)sizeof(K), j * BLOCK * (uint32_t)sizeof(K));
 #pragma unroll
-  for (int j = 0; j < (KPT + 1) / 2; j++) lb[j] = 0;
-#pragma unroll
-  for (int j = 0; j < KPT; j++) {
-    if (j < nv) lb[j >> 1] |= (bin_of<K>(kr[j], mp) - it.bin_lo) << ((j & 1) * 16);
-    if ((j & 3) == 3) __builtin_amdgcn_sched_barrier(0);
+  for (int j = 0; j < (KPT + 1) / 2; j++) {
+    lb[j] = 0;
+    mc[j] = 0;
   }
-  unsigned long long my_total = 0;
-  for (uint32_t s0 = 0; s0 < W; s0 += SUB) {
-    const uint32_t n = min((uint32_t)SUB, W - s0);
+#pragma unroll
+  for (int j = 0; j < KPT; j++)
+    if (j < nv) lb[j >> 1] |= (bin_of<K, HK>(kr[j], mp) - it.bin_lo) << ((j & 1) * 16);
+  for (uint32_t s0 = 0; s0 < W;) {
+    uint32_t n = min((uint32_t)SUB, W - s0);
     for (uint32_t q = threadIdx.x; q <= n; q += BLOCK) s_off[q] = offA[it.off_pos + s0 + q];
     __syncthreads();
     const OFF a0 = s_off[0];
-    const uint64_t na = (uint64_t)(s_off[n] - a0);
-    const bool in_lds = na <= akey_cap;
-    if (in_lds)
-      for (uint32_t q = threadIdx.x; q < na; q += BLOCK) s_ak[q] = keysA[a0 + q];
+    if ((uint64_t)(s_off[n] - a0) > akey_cap) {
+      // shrink to the longest prefix of bins whose keys fit (at least one bin)
+      if (threadIdx.x == 0) {
+        uint32_t lo = 1, hi = n;  // s_off[lo] fits or lo == 1
+        while (lo < hi) {
+          const uint32_t mid = (lo + hi + 1) >> 1;
+          if ((uint64_t)(s_off[mid] - a0) <= akey_cap) lo = mid;
+          else hi = mid - 1;
+        }
+        s_n = lo;
+      }
+      __syncthreads();
+      n = s_n;
+    }
+    const uint32_t na = (uint32_t)min((uint64_t)(s_off[n] - a0), (uint64_t)akey_cap);
+    for (uint32_t q = threadIdx.x; q < na; q += BLOCK) s_ak[q] = ka[a0 + q];
     __syncthreads();
-    uint32_t mine = 0;
 #pragma unroll
     for (int j = 0; j < KPT; j++) {
-      if (j < nv) {
-        const uint32_t l = ((lb[j >> 1] >> ((j & 1) * 16)) & 0xFFFFu) - s0;
-        if (l < n) {
-          const uint64_t e = (uint64_t)(s_off[l + 1] - a0), b0 = (uint64_t)(s_off[l] - a0);
-          if (e - b0 > PROBE_BIN_LIMIT) {
-            atomicOr(flag, 1u);
-          } else {
-            for (uint64_t q = b0; q < e; q++) mine += (in_lds ? s_ak[q] : keysA[a0 + q]) == kr[j];
-          }
-        }
+      const uint32_t sh = (j & 1) * 16;
+      const uint32_t l = ((lb[j >> 1] >> sh) & 0xFFFFu) - s0;
+      if (j < nv && l < n) {
+        const uint32_t b0 = (uint32_t)(s_off[l] - a0), e = (uint32_t)(s_off[l + 1] - a0);
+        uint32_t m = 0;
+        if (e - b0 > PROBE_BIN_LIMIT) atomicOr(flag, 1u);
+        else
+          for (uint32_t q = b0; q < e; q++) m += s_ak[q] == kr[j];
+        mc[j >> 1] += m << sh;  // m <= PROBE_BIN_LIMIT: no carry into the other half
       }
     }
-    if (out != nullptr) {
-      uint32_t tot;
-      uint32_t ex = block_exclusive_sum<BLOCK, uint32_t>(mine, s_tmp, tot);
-      if (threadIdx.x == 0) s_base = tot ? atomicAdd(cursor, (unsigned long long)tot) : 0ull;
-      __syncthreads();
-      const unsigned long long base = s_base + ex;
-      if (mine) {
-        unsigned long long w = base;
-#pragma unroll
-        for (int j = 0; j < KPT; j++) {
-          if (j < nv) {
-            const uint32_t l = ((lb[j >> 1] >> ((j & 1) * 16)) & 0xFFFFu) - s0;
-            if (l < n) {
-              const uint64_t e = (uint64_t)(s_off[l + 1] - a0), b0 = (uint64_t)(s_off[l] - a0);
-              if (e - b0 > PROBE_BIN_LIMIT) continue;
-              for (uint64_t q = b0; q < e; q++) {
-                if ((in_lds ? s_ak[q] : keysA[a0 + q]) == kr[j]) {
-                  if (w < capacity) out[w] = kr[j];
-                  w++;
-                }
-              }
-            }
-          }
-        }
-      }
-    } else {
-      my_total += mine;
-    }
+    s0 += n;
     __syncthreads();
   }
-  if (out == nullptr) {
-    __shared__ unsigned long long s_t64[BLOCK / 64 + 1];
-    unsigned long long t = block_sum<BLOCK, unsigned long long>(my_total, s_t64);
-    if (threadIdx.x == 0 && t) atomicAdd(cursor, t);
+  uint32_t mine = 0;
+#pragma unroll
+  for (int j = 0; j < (KPT + 1) / 2; j++) mine += (mc[j] & 0xFFFFu) + (mc[j] >> 16);
+  uint32_t tot;
+  const uint32_t ex = block_exclusive_sum<BLOCK, uint32_t>(mine, s_tmp, tot);
+  if (threadIdx.x == 0) s_base = tot ? atomicAdd(cursor, (unsigned long long)tot) : 0ull;
+  __syncthreads();
+  if (out == nullptr || !mine) return;
+  unsigned long long w = s_base + ex;
+#pragma unroll
+  for (int j = 0; j < KPT; j++) {
+    const uint32_t m = (mc[j >> 1] >> ((j & 1) * 16)) & 0xFFFFu;
+    for (uint32_t r = 0; r < m; r++, w++)
+      if (w < capacity) out[w] = kr[j];
   }
 }
 
@@ -909,6 +930,7 @@ __global__ __launch_bounds__(BLOCK) void k_probe_big(const K *__restrict__ bpart
   if (it.count <= small_cap) return;
   const BinMap mp = maps[it.table];
   const K *kp = bpart + it.key_start;
+  const K *ka = keysA + it.key_rel;
   for (uint64_t c0 = 0; c0 < it.count; c0 += BLOCK) {
     const uint64_t i = c0 + threadIdx.x;
     unsigned long long mine = 0;
@@ -923,7 +945,7 @@ __global__ __launch_bounds__(BLOCK) void k_probe_big(const K *__restrict__ bpart
         atomicOr(flag, 1u);
         e = s;
       }
-      for (uint64_t q = s; q < e; q++) mine += keysA[q] == k;
+      for (uint64_t q = s; q < e; q++) mine += ka[q] == k;
     }
     if (out == nullptr) {
       const unsigned long long t = block_sum<BLOCK, unsigned long long>(mine, s_t64);
@@ -936,7 +958,7 @@ __global__ __launch_bounds__(BLOCK) void k_probe_big(const K *__restrict__ bpart
     __syncthreads();
     unsigned long long w = s_base + ex;
     for (uint64_t q = s; q < e && mine; q++)
-      if (keysA[q] == k) {
+      if (ka[q] == k) {
         if (w < capacity) out[w] = k;
         w++;
       }
@@ -947,29 +969,42 @@ __global__ __launch_bounds__(BLOCK) void k_probe_big(const K *__restrict__ bpart
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
+template <int V> using IC = std::integral_constant<int, V>;
+
+// Calls f(IC<hash kind>) for the specialisation of hash kind hk.
+template <typename F> static hipError_t dispatch_hk(int hk, F &&f) {
+  if (hk == HK_MURMUR) return f(IC<HK_MURMUR>{});
+  if (hk == HK_RMI) return f(IC<HK_RMI>{});
+  return f(IC<HK_ANY>{});
+}
+// Calls f(IC<hash kind>, IC<bucket kind>); a mixed bucket kind takes the
+// fully general kernel.
+template <typename F> static hipError_t dispatch_part(int hk, int sk, F &&f) {
+  if (sk != SK_RANGE && sk != SK_SPLIT) return f(IC<HK_ANY>{}, IC<SK_ANY>{});
+  return dispatch_hk(hk, [&](auto h) { return sk == SK_SPLIT ? f(h, IC<SK_SPLIT>{}) : f(h, IC<SK_RANGE>{}); });
+}
+
+constexpr int PART_KPT = 32;  // tile = 1024 * 32 keys (the engine's tiling)
+
 template <typename K>
 static hipError_t launch_part_hist_t(const LaunchPart &lp, hipStream_t st) {
   const size_t lds = (size_t)lp.max_cols * 4;
-  if (lp.kpt == 32)
-    hipLaunchKernelGGL((k_part_hist<K, 1024, 32>), dim3(lp.ntiles), dim3(1024), lds, st, (const K *)lp.keys, lp.tiles,
-                       lp.segs, lp.cmat);
-  else
-    hipLaunchKernelGGL((k_part_hist<K, 1024, 16>), dim3(lp.ntiles), dim3(1024), lds, st, (const K *)lp.keys, lp.tiles,
-                       lp.segs, lp.cmat);
-  return hipGetLastError();
+  return dispatch_part(lp.hk, lp.sk, [&](auto h, auto b) {
+    hipLaunchKernelGGL((k_part_hist<K, 1024, PART_KPT, decltype(h)::value, decltype(b)::value>), dim3(lp.ntiles),
+                       dim3(1024), lds, st, (const K *)lp.keys, lp.tiles, lp.segs, lp.cmat);
+    return hipGetLastError();
+  });
 }
 
 template <typename K>
 static hipError_t launch_part_scatter_t(const LaunchPart &lp, hipStream_t st) {
   const size_t cnt_bytes = (size_t)((lp.max_cols + 3) & ~3u) * 4;
   const size_t lds = cnt_bytes + (size_t)lp.stage_cap * (sizeof(K) + 2);
-  if (lp.kpt == 32)
-    hipLaunchKernelGGL((k_part_scatter<K, 1024, 32>), dim3(lp.ntiles), dim3(1024), lds, st, (const K *)lp.keys, lp.tiles,
-                       lp.segs, lp.cmat, (K *)lp.out, lp.stage_cap);
-  else
-    hipLaunchKernelGGL((k_part_scatter<K, 1024, 16>), dim3(lp.ntiles), dim3(1024), lds, st, (const K *)lp.keys, lp.tiles,
-                       lp.segs, lp.cmat, (K *)lp.out, lp.stage_cap);
-  return hipGetLastError();
+  return dispatch_part(lp.hk, lp.sk, [&](auto h, auto b) {
+    hipLaunchKernelGGL((k_part_scatter<K, 1024, PART_KPT, decltype(h)::value, decltype(b)::value>), dim3(lp.ntiles),
+                       dim3(1024), lds, st, (const K *)lp.keys, lp.tiles, lp.segs, lp.cmat, (K *)lp.out, lp.stage_cap);
+    return hipGetLastError();
+  });
 }
 
 hipError_t launch_part_hist(const LaunchPart &lp, hipStream_t st) {
@@ -996,8 +1031,12 @@ template <typename K, typename OFF>
 static hipError_t launch_final_t(const LaunchFinal &lf, hipStream_t st) {
   if (lf.nitems) {
     constexpr int KPT = sizeof(K) == 4 ? KPT_U32 : 32;
-    hipLaunchKernelGGL((k_final<K, OFF, 1024, KPT>), dim3(lf.nitems), dim3(1024), lf.lds_bytes, st, (const K *)lf.part,
-                       (K *)lf.csr, (OFF *)lf.off, lf.items, lf.maps, lf.lds_bytes);
+    const hipError_t e = dispatch_hk(lf.hk, [&](auto h) {
+      hipLaunchKernelGGL((k_final<K, OFF, 1024, KPT, decltype(h)::value>), dim3(lf.nitems), dim3(1024), lf.lds_bytes, st,
+                         (const K *)lf.part, (K *)lf.csr, (OFF *)lf.off, lf.items, lf.maps, lf.lds_bytes);
+      return hipGetLastError();
+    });
+    if (e != hipSuccess) return e;
   }
   if (lf.nitems) {
     constexpr int KPT = sizeof(K) == 4 ? KPT_U32 : 32;
@@ -1019,10 +1058,16 @@ int final_kpt(int key_bytes) { return key_bytes == 4 ? KPT_U32 : 32; }
 template <typename K, typename OFF>
 static hipError_t launch_probe_t(const LaunchProbe &p, hipStream_t st) {
   constexpr int KPT = sizeof(K) == 4 ? KPT_U32 : 32;
-  constexpr int SUB = 8192;
+  constexpr int SUB = probe_sub<OFF>();
+  if (p.akey_cap < PROBE_BIN_LIMIT) return hipErrorInvalidValue;  // a walkable bin must fit in LDS
   const size_t lds = ((SUB + 2) * sizeof(OFF) + 15) / 16 * 16 + (size_t)p.akey_cap * sizeof(K);
-  hipLaunchKernelGGL((k_probe<K, OFF, 1024, KPT, SUB>), dim3(p.nitems), dim3(1024), lds, st, (const K *)p.bpart, p.items,
-                     p.maps, (const OFF *)p.offA, (const K *)p.keysA, (K *)p.out, p.capacity, p.cursor, p.akey_cap, p.flag);
+  const hipError_t e = dispatch_hk(p.hk, [&](auto h) {
+    hipLaunchKernelGGL((k_probe<K, OFF, 1024, KPT, SUB, decltype(h)::value>), dim3(p.nitems), dim3(1024), lds, st,
+                       (const K *)p.bpart, p.items, p.maps, (const OFF *)p.offA, (const K *)p.keysA, (K *)p.out,
+                       p.capacity, p.cursor, p.akey_cap, p.flag);
+    return hipGetLastError();
+  });
+  if (e != hipSuccess) return e;
   const uint32_t small_cap = std::min(1024u * KPT, 65535u);
   hipLaunchKernelGGL((k_probe_big<K, OFF, 1024>), dim3(p.nitems), dim3(1024), 0, st, (const K *)p.bpart, p.items, p.maps,
                      (const OFF *)p.offA, (const K *)p.keysA, (K *)p.out, p.capacity, p.cursor, small_cap, p.flag);
@@ -1037,9 +1082,9 @@ hipError_t launch_probe(const LaunchProbe &p, hipStream_t st) {
 }
 
 uint32_t probe_akey_cap(int key_bytes, int off_bytes) {
-  const size_t budget = 150 * 1024;
-  const size_t used = ((8192 + 2) * (size_t)off_bytes + 15) / 16 * 16;
-  return (uint32_t)((budget - used) / key_bytes);
+  const size_t sub = off_bytes == 4 ? probe_sub<uint32_t>() : probe_sub<uint64_t>();
+  const size_t used = ((sub + 2) * (size_t)off_bytes + 15) / 16 * 16;
+  return (uint32_t)((PROBE_LDS - used) / key_bytes);
 }
 
 template <typename K, typename OFF>
@@ -1130,22 +1175,31 @@ static hipError_t set_lds(F *fn) {
 hipError_t set_kernel_attributes() {
   hipError_t e = hipSuccess;
 #define HG_SET(...) do { hipError_t e_ = set_lds(__VA_ARGS__); if (e_ != hipSuccess) e = e_; } while (0)
-  HG_SET(&k_part_hist<uint32_t, 1024, 32>);
-  HG_SET(&k_part_hist<uint64_t, 1024, 32>);
-  HG_SET(&k_part_hist<uint32_t, 1024, 16>);
-  HG_SET(&k_part_hist<uint64_t, 1024, 16>);
-  HG_SET(&k_part_scatter<uint32_t, 1024, 32>);
-  HG_SET(&k_part_scatter<uint64_t, 1024, 32>);
-  HG_SET(&k_part_scatter<uint32_t, 1024, 16>);
-  HG_SET(&k_part_scatter<uint64_t, 1024, 16>);
-  HG_SET(&k_final<uint32_t, uint32_t, 1024, KPT_U32>);
-  HG_SET(&k_final<uint32_t, uint64_t, 1024, KPT_U32>);
-  HG_SET(&k_final<uint64_t, uint32_t, 1024, 32>);
-  HG_SET(&k_final<uint64_t, uint64_t, 1024, 32>);
-  HG_SET(&k_probe<uint32_t, uint32_t, 1024, KPT_U32, 8192>);
-  HG_SET(&k_probe<uint32_t, uint64_t, 1024, KPT_U32, 8192>);
-  HG_SET(&k_probe<uint64_t, uint32_t, 1024, 32, 8192>);
-  HG_SET(&k_probe<uint64_t, uint64_t, 1024, 32, 8192>);
+#define HG_SET_D(call) do { hipError_t e_ = (call); if (e_ != hipSuccess) e = e_; } while (0)
+  for (int hk = 0; hk < 3; hk++)
+    for (int sk = 0; sk < 3; sk++) {
+      HG_SET_D(dispatch_part(hk, sk, [&](auto h, auto b) {
+        constexpr int H = decltype(h)::value, S = decltype(b)::value;
+        hipError_t r = set_lds(&k_part_hist<uint32_t, 1024, PART_KPT, H, S>);
+        if (r == hipSuccess) r = set_lds(&k_part_hist<uint64_t, 1024, PART_KPT, H, S>);
+        if (r == hipSuccess) r = set_lds(&k_part_scatter<uint32_t, 1024, PART_KPT, H, S>);
+        if (r == hipSuccess) r = set_lds(&k_part_scatter<uint64_t, 1024, PART_KPT, H, S>);
+        return r;
+      }));
+    }
+  for (int hk = 0; hk < 3; hk++)
+    HG_SET_D(dispatch_hk(hk, [&](auto h) {
+      constexpr int H = decltype(h)::value;
+      hipError_t r = set_lds(&k_final<uint32_t, uint32_t, 1024, KPT_U32, H>);
+      if (r == hipSuccess) r = set_lds(&k_final<uint32_t, uint64_t, 1024, KPT_U32, H>);
+      if (r == hipSuccess) r = set_lds(&k_final<uint64_t, uint32_t, 1024, 32, H>);
+      if (r == hipSuccess) r = set_lds(&k_final<uint64_t, uint64_t, 1024, 32, H>);
+      if (r == hipSuccess) r = set_lds(&k_probe<uint32_t, uint32_t, 1024, KPT_U32, probe_sub<uint32_t>(), H>);
+      if (r == hipSuccess) r = set_lds(&k_probe<uint32_t, uint64_t, 1024, KPT_U32, probe_sub<uint64_t>(), H>);
+      if (r == hipSuccess) r = set_lds(&k_probe<uint64_t, uint32_t, 1024, 32, probe_sub<uint32_t>(), H>);
+      if (r == hipSuccess) r = set_lds(&k_probe<uint64_t, uint64_t, 1024, 32, probe_sub<uint64_t>(), H>);
+      return r;
+    }));
   HG_SET(&k_join<uint32_t, uint32_t, 256, 4096>);
   HG_SET(&k_join<uint32_t, uint64_t, 256, 4096>);
   HG_SET(&k_join<uint64_t, uint32_t, 256, 4096>);
@@ -1153,6 +1207,7 @@ hipError_t set_kernel_attributes() {
   HG_SET(&k_range_hist<uint32_t>);
   HG_SET(&k_range_hist<uint64_t>);
 #undef HG_SET
+#undef HG_SET_D
   return e;
 }
 

@@ -7,9 +7,13 @@
 
 namespace hg {
 
+// Kernel specialisations: hash kind (0 murmur3, 1 learned, 2 mixed) and
+// bucket kind (0 range, 1 split, 2 mixed) of every segment of a launch.
+enum : int { KIND_MIXED = 2 };
+
 struct LaunchPart {
   int key_bytes;
-  int kpt;               // keys per thread of a tile (16 or 32), tile = 1024*kpt
+  int hk, sk;            // hash kind, bucket kind
   uint32_t ntiles;
   uint32_t max_cols;     // largest ncols over the segments
   uint32_t stage_cap;    // keys staged per LDS round in the scatter
@@ -37,6 +41,7 @@ struct LaunchCol {
 
 struct LaunchFinal {
   int key_bytes, off_bytes;
+  int hk;
   uint32_t nitems;
   uint32_t lds_bytes;
   const void *part;
@@ -48,6 +53,7 @@ struct LaunchFinal {
 
 struct LaunchProbe {
   int key_bytes, off_bytes;
+  int hk;
   uint32_t nitems;
   uint32_t akey_cap;
   const void *bpart;
@@ -89,6 +95,29 @@ struct LaunchRle {
   unsigned int *overflow;
 };
 
+struct LaunchRmi {
+  int key_bytes;
+  const void *keys;
+  uint64_t n_keys, n_sample;
+  uint64_t s0;               // mix64(seed)
+  void *buf0, *buf1;         // n_sample keys each
+  uint32_t *counts;          // 256 * sort tiles + 1
+  unsigned long long *scan_status;
+  unsigned int *scan_ticket;
+  double *partials;          // 4 * nblk
+  uint32_t nblk;
+  uint32_t n_leaves;
+  BinMap map;                // murmur3 map of the table's bins
+  void *model;
+  uint32_t *start;           // n_leaves + 1
+  uint32_t Mp;               // bins of the sampled check
+  uint32_t *cnt_r, *cnt_m;   // Mp each
+  unsigned long long *out;   // 4
+};
+
+hipError_t launch_rmi_train(const LaunchRmi &r, hipStream_t st);
+uint32_t rmi_moment_blocks(uint64_t n_sample);
+uint32_t rmi_sort_tiles(uint64_t n_sample);
 hipError_t launch_rle(const LaunchRle &r, hipStream_t st);
 uint64_t rle_chunks(uint64_t M);
 hipError_t launch_part_hist(const LaunchPart &lp, hipStream_t st);

@@ -3,10 +3,6 @@
 #include "../../include/hashgraph.h"
 
 extern "C" {
-size_t hg_rmi_model_bytes(uint32_t) { return 0; }
-hg_status hg_rmi_train(hg_ctx, const void *, uint64_t, uint32_t, uint64_t, uint32_t, void *, uint32_t *) {
-  return HG_ERR_UNSUPPORTED;
-}
 hg_status hg_batch_build(hg_ctx, const hg_batch_desc *, const void *, uint32_t *, void *) { return HG_ERR_UNSUPPORTED; }
 hg_status hg_batch_probe(hg_ctx, const hg_batch_desc *, const uint32_t *, const void *, const void *, const uint64_t *,
                          uint64_t *, uint64_t *) {

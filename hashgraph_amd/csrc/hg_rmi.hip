@@ -1,0 +1,359 @@
+// hg_rmi.hip -- training of the learned CDF model (RMI) on gfx950.
+//
+// No counterpart in the reference.  Steps, all on the device:
+//   1. seeded sample of the table's keys (with replacement),
+//   2. LSD radix sort of the sample (8-bit digits, stable wave-ballot ranks),
+//   3. least-squares fit of the root (key -> leaf) in FP64, stored as FP32,
+//   4. leaf ranges of the sorted sample under the stored FP32 root,
+//   5. one least-squares fit per leaf (key -> global rank * M / n),
+//   6. monotone bases and the sentinel leaf,
+//   7. the fallback check: both hashes bin the sample at the table's load
+//      factor; max bin and sum of squared bin sizes are compared.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "hg_common.hpp"
+#include "hg_device.hpp"
+#include "hg_kernels.hpp"
+
+namespace hg {
+
+constexpr int SORT_BLOCK = 256;
+constexpr int SORT_IPT = 16;
+constexpr uint32_t SORT_TILE = SORT_BLOCK * SORT_IPT;
+constexpr uint64_t SPLITMIX_GAMMA = 0x9E3779B97F4A7C15ull;
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+// sample[i] = keys[hi64(x_i * n)], x_i = mix64(s0 + (i + 1) * gamma).
+template <typename K>
+__global__ __launch_bounds__(256) void k_sample(const K *__restrict__ keys, uint64_t n, uint64_t s0, uint64_t ns,
+                                                K *__restrict__ out) {
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < ns; i += (uint64_t)gridDim.x * 256) {
+    const uint64_t x = mix64(s0 + (i + 1) * SPLITMIX_GAMMA);
+    out[i] = keys[__umul64hi(x, n)];
+  }
+}
+
+// Digit histogram of one tile, digit-major: counts[d * ntiles + tile].
+template <typename K>
+__global__ __launch_bounds__(SORT_BLOCK) void k_sort_hist(const K *__restrict__ in, uint64_t n, int shift,
+                                                          uint32_t *__restrict__ counts, uint32_t ntiles) {
+  __shared__ uint32_t h[256];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  const uint64_t base = (uint64_t)blockIdx.x * SORT_TILE;
+#pragma unroll
+  for (int r = 0; r < SORT_IPT; r++) {
+    const uint64_t i = base + r * SORT_BLOCK + threadIdx.x;
+    if (i < n) atomicAdd(&h[(uint32_t)(in[i] >> shift) & 255u], 1u);
+  }
+  __syncthreads();
+  counts[(uint64_t)threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
+}
+
+// Stable scatter of one tile by digit.  offs = exclusive scan of counts.
+// Round r takes tile elements [256 r, 256 r + 256); within a wave the rank
+// among equal digits comes from 8 ballots, across waves from LDS counts.
+template <typename K>
+__global__ __launch_bounds__(SORT_BLOCK) void k_sort_scatter(const K *__restrict__ in, K *__restrict__ out, uint64_t n,
+                                                             int shift, const uint32_t *__restrict__ offs,
+                                                             uint32_t ntiles) {
+  __shared__ uint32_t s_base[256];
+  __shared__ uint32_t s_w[SORT_BLOCK / 64][256];
+  const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const uint64_t below = (1ull << lane) - 1ull;
+  s_base[threadIdx.x] = offs[(uint64_t)threadIdx.x * ntiles + blockIdx.x];
+  const uint64_t base = (uint64_t)blockIdx.x * SORT_TILE;
+  for (int r = 0; r < SORT_IPT; r++) {
+#pragma unroll
+    for (int q = 0; q < SORT_BLOCK / 64; q++) s_w[q][threadIdx.x] = 0;
+    __syncthreads();
+    const uint64_t i = base + r * SORT_BLOCK + threadIdx.x;
+    const bool valid = i < n;
+    const K k = valid ? in[i] : (K)0;
+    const uint32_t d = (uint32_t)(k >> shift) & 255u;
+    uint64_t m = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; b++) {
+      const bool bit = (d >> b) & 1u;
+      const uint64_t bb = __ballot(bit);
+      m &= bit ? bb : ~bb;
+    }
+    const uint32_t rank = (uint32_t)__popcll(m & below);
+    if (valid && rank == 0) s_w[w][d] = (uint32_t)__popcll(m);
+    __syncthreads();
+    if (valid) {
+      uint32_t pos = s_base[d] + rank;
+      for (uint32_t q = 0; q < w; q++) pos += s_w[q][d];
+      out[pos] = k;
+    }
+    __syncthreads();
+    uint32_t add = 0;
+#pragma unroll
+    for (int q = 0; q < SORT_BLOCK / 64; q++) add += s_w[q][threadIdx.x];
+    s_base[threadIdx.x] += add;
+    __syncthreads();
+  }
+}
+
+// Deterministic block sum of NV doubles (fixed tree order).
+template <int NV>
+__device__ __forceinline__ void block_sum_d(double (&v)[NV], double *s_red /* 256 * NV */) {
+#pragma unroll
+  for (int j = 0; j < NV; j++) s_red[j * 256 + threadIdx.x] = v[j];
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < (uint32_t)st)
+#pragma unroll
+      for (int j = 0; j < NV; j++) s_red[j * 256 + threadIdx.x] += s_red[j * 256 + threadIdx.x + st];
+    __syncthreads();
+  }
+#pragma unroll
+  for (int j = 0; j < NV; j++) v[j] = s_red[j * 256];
+  __syncthreads();
+}
+
+// Root moments over the sorted sample: x = key - key_min, y = i * yscale.
+// One row of 4 partial sums per block.
+template <typename K>
+__global__ __launch_bounds__(256) void k_rmi_root_moments(const K *__restrict__ sorted, uint64_t n, double yscale,
+                                                          double *__restrict__ part) {
+  __shared__ double s_red[4 * 256];
+  const uint64_t kmin = (uint64_t)sorted[0];
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) {
+    const double x = (double)((uint64_t)sorted[i] - kmin);
+    const double y = (double)i * yscale;
+    v[0] += x;
+    v[1] += y;
+    v[2] += x * x;
+    v[3] += x * y;
+  }
+  block_sum_d<4>(v, s_red);
+  if (threadIdx.x < 4) part[(uint64_t)blockIdx.x * 4 + threadIdx.x] = v[threadIdx.x];
+}
+
+// Closed-form least squares y = alpha + beta x from the partial sums; a
+// non-positive or undefined slope becomes 0 (the root stays monotone).
+__device__ __forceinline__ void lsq_line(double n, double sx, double sy, double sxx, double sxy, double &alpha,
+                                         double &beta) {
+  const double mx = sx / n, my = sy / n;
+  const double vxx = sxx - sx * mx, vxy = sxy - sx * my;
+  beta = vxx > 0.0 ? vxy / vxx : 0.0;
+  if (!(beta > 0.0) || !(beta < 3.0e38)) beta = 0.0;
+  alpha = my - beta * mx;
+}
+
+template <typename K>
+__global__ __launch_bounds__(256) void k_rmi_root_fit(const K *__restrict__ sorted, uint64_t n, const double *part,
+                                                      uint32_t nblk, uint32_t L, uint32_t M, RmiHeader *__restrict__ h) {
+  __shared__ double s_red[4 * 256];
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
+  for (uint32_t b = threadIdx.x; b < nblk; b += 256)
+#pragma unroll
+    for (int j = 0; j < 4; j++) v[j] += part[(uint64_t)b * 4 + j];
+  block_sum_d<4>(v, s_red);
+  if (threadIdx.x == 0) {
+    double alpha, beta;
+    lsq_line((double)n, v[0], v[1], v[2], v[3], alpha, beta);
+    RmiHeader hd{};
+    hd.key_min = (uint64_t)sorted[0];
+    hd.root_slope = (float)beta;
+    hd.root_icpt = (float)alpha;
+    hd.n_leaves = L;
+    hd.n_bins = M;
+    *h = hd;
+  }
+}
+
+// start[l] = first sample position routed to leaf >= l (start[L] = n).
+template <typename K>
+__global__ __launch_bounds__(256) void k_rmi_bounds(const K *__restrict__ sorted, uint64_t n,
+                                                    const RmiHeader *__restrict__ hp, uint32_t *__restrict__ start) {
+  const RmiHeader h = *hp;
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i <= n; i += (uint64_t)gridDim.x * 256) {
+    const int64_t cur = i < n ? (int64_t)rmi_leaf((uint64_t)sorted[i], h) : (int64_t)h.n_leaves;
+    const int64_t prev = i > 0 ? (int64_t)rmi_leaf((uint64_t)sorted[i - 1], h) : -1;
+    for (int64_t l = prev + 1; l <= cur; l++) start[l] = (uint32_t)i;
+  }
+}
+
+// One workgroup per leaf: least squares of the global rank position
+// (i * M / n) on the distance from the leaf's first key.
+template <typename K>
+__global__ __launch_bounds__(256) void k_rmi_leaf_fit(const K *__restrict__ sorted, uint64_t n,
+                                                      const uint32_t *__restrict__ start, uint32_t M,
+                                                      RmiLeaf *__restrict__ leaves) {
+  __shared__ double s_red[4 * 256];
+  const uint32_t l = blockIdx.x;
+  const uint32_t a = start[l], b = start[l + 1];
+  const double yscale = (double)M / (double)n;
+  const uint64_t x0 = (uint64_t)sorted[a < n ? a : n - 1];
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
+  for (uint32_t i = a + threadIdx.x; i < b; i += 256) {
+    const double x = (double)((uint64_t)sorted[i] - x0);
+    const double y = (double)i * yscale;
+    v[0] += x;
+    v[1] += y;
+    v[2] += x * x;
+    v[3] += x * y;
+  }
+  block_sum_d<4>(v, s_red);
+  if (threadIdx.x == 0) {
+    double alpha, beta;
+    if (b > a) {
+      lsq_line((double)(b - a), v[0], v[1], v[2], v[3], alpha, beta);
+    } else {
+      alpha = (double)a * yscale;
+      beta = 0.0;
+    }
+    double bs = floor(alpha + 0.5);
+    bs = bs < 0.0 ? 0.0 : (bs > (double)(M - 1) ? (double)(M - 1) : bs);
+    RmiLeaf lf;
+    lf.x0 = x0;
+    lf.slope = (float)beta;
+    lf.base = (uint32_t)bs;
+    leaves[l] = lf;
+  }
+}
+
+// Running maximum of the bases (monotone map) and the sentinel leaf L.
+__global__ __launch_bounds__(1024) void k_rmi_finish(RmiLeaf *__restrict__ leaves, uint32_t L, uint32_t M) {
+  __shared__ uint32_t s_w[16];
+  __shared__ uint32_t s_carry;
+  if (threadIdx.x == 0) s_carry = 0;
+  __syncthreads();
+  for (uint32_t c0 = 0; c0 < L; c0 += 1024) {
+    const uint32_t l = c0 + threadIdx.x;
+    uint32_t v = l < L ? leaves[l].base : 0u;
+    // inclusive max-scan of the block
+    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t y = __shfl_up(v, d, 64);
+      if (lane >= (uint32_t)d) v = v > y ? v : y;
+    }
+    if (lane == 63) s_w[w] = v;
+    __syncthreads();
+    uint32_t pre = s_carry;
+    for (uint32_t q = 0; q < w; q++) pre = pre > s_w[q] ? pre : s_w[q];
+    v = v > pre ? v : pre;
+    if (l < L) leaves[l].base = v;
+    __syncthreads();
+    if (threadIdx.x == 1023) s_carry = v;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    RmiLeaf s;
+    s.x0 = ~0ull;
+    s.slope = 0.0f;
+    s.base = M - 1;
+    leaves[L] = s;
+  }
+}
+
+// Both hashes bin the sample; bins are scaled to Mp = M * n_sample / n_keys
+// so that the sample sees the table's load factor.
+template <typename K>
+__global__ __launch_bounds__(256) void k_rmi_check(const K *__restrict__ keys, uint64_t n, const void *model,
+                                                   BinMap mm, uint32_t Mp, uint32_t *__restrict__ cnt_r,
+                                                   uint32_t *__restrict__ cnt_m) {
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) {
+    const K k = keys[i];
+    const uint64_t br = rmi_bin((uint64_t)k, model, mm.M);
+    const uint64_t bm = bin_of<K, HK_MURMUR>(k, mm);
+    atomicAdd(&cnt_r[br * Mp / mm.M], 1u);
+    atomicAdd(&cnt_m[bm * Mp / mm.M], 1u);
+  }
+}
+
+// out[0] = max count, out[1] = sum of squares (learned); out[2], out[3] murmur.
+__global__ __launch_bounds__(256) void k_rmi_check_reduce(const uint32_t *__restrict__ cnt_r,
+                                                          const uint32_t *__restrict__ cnt_m, uint32_t Mp,
+                                                          unsigned long long *__restrict__ out) {
+  __shared__ unsigned long long s_t[8];
+  unsigned long long mr = 0, sr = 0, mm = 0, sm = 0;
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < Mp; i += (uint64_t)gridDim.x * 256) {
+    const unsigned long long a = cnt_r[i], b = cnt_m[i];
+    mr = a > mr ? a : mr;
+    mm = b > mm ? b : mm;
+    sr += a * a;
+    sm += b * b;
+  }
+  sr = wave_sum(sr);
+  sm = wave_sum(sm);
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) {
+    const unsigned long long x = __shfl_xor(mr, d, 64), y = __shfl_xor(mm, d, 64);
+    mr = x > mr ? x : mr;
+    mm = y > mm ? y : mm;
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicMax(&out[0], mr);
+    atomicAdd(&out[1], sr);
+    atomicMax(&out[2], mm);
+    atomicAdd(&out[3], sm);
+  }
+  (void)s_t;
+}
+
+static uint32_t grid_for(uint64_t n, uint32_t cap) {
+  const uint64_t g = (n + 255) / 256;
+  return (uint32_t)(g == 0 ? 1 : (g < cap ? g : cap));
+}
+
+template <typename K>
+static hipError_t launch_rmi_train_t(const LaunchRmi &r, hipStream_t st) {
+  const uint64_t ns = r.n_sample;
+  K *a = (K *)r.buf0, *b = (K *)r.buf1;
+  hipLaunchKernelGGL((k_sample<K>), dim3(grid_for(ns, 4096)), dim3(256), 0, st, (const K *)r.keys, r.n_keys, r.s0, ns,
+                     a);
+  const uint32_t ntiles = (uint32_t)((ns + SORT_TILE - 1) / SORT_TILE);
+  for (int pass = 0; pass < (int)sizeof(K); pass++) {
+    const int shift = pass * 8;
+    hipLaunchKernelGGL((k_sort_hist<K>), dim3(ntiles), dim3(SORT_BLOCK), 0, st, (const K *)a, ns, shift, r.counts,
+                       ntiles);
+    hipError_t e = launch_scan(r.counts, r.counts, (uint64_t)256 * ntiles, 4, r.scan_status, r.scan_ticket, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_sort_scatter<K>), dim3(ntiles), dim3(SORT_BLOCK), 0, st, (const K *)a, b, ns, shift,
+                       (const uint32_t *)r.counts, ntiles);
+    K *t = a;
+    a = b;
+    b = t;
+  }
+  // sizeof(K) passes (even): the sorted sample is back in buf0
+  RmiHeader *h = (RmiHeader *)r.model;
+  RmiLeaf *leaves = (RmiLeaf *)(h + 1);
+  hipLaunchKernelGGL((k_rmi_root_moments<K>), dim3(r.nblk), dim3(256), 0, st, (const K *)a, ns,
+                     (double)r.n_leaves / (double)ns, r.partials);
+  hipLaunchKernelGGL((k_rmi_root_fit<K>), dim3(1), dim3(256), 0, st, (const K *)a, ns, (const double *)r.partials, r.nblk,
+                     r.n_leaves, r.map.M, h);
+  hipLaunchKernelGGL((k_rmi_bounds<K>), dim3(grid_for(ns + 1, 4096)), dim3(256), 0, st, (const K *)a, ns,
+                     (const RmiHeader *)h, r.start);
+  hipLaunchKernelGGL((k_rmi_leaf_fit<K>), dim3(r.n_leaves), dim3(256), 0, st, (const K *)a, ns,
+                     (const uint32_t *)r.start, r.map.M, leaves);
+  hipLaunchKernelGGL(k_rmi_finish, dim3(1), dim3(1024), 0, st, leaves, r.n_leaves, r.map.M);
+  // fallback check on the sample
+  hipError_t e = launch_fill_u32(r.cnt_r, 0u, r.Mp, st);
+  if (e == hipSuccess) e = launch_fill_u32(r.cnt_m, 0u, r.Mp, st);
+  if (e == hipSuccess) e = launch_fill_u64(r.out, 0ull, 4, st);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((k_rmi_check<K>), dim3(grid_for(ns, 4096)), dim3(256), 0, st, (const K *)a, ns,
+                     (const void *)r.model, r.map, r.Mp, r.cnt_r, r.cnt_m);
+  hipLaunchKernelGGL(k_rmi_check_reduce, dim3(grid_for(r.Mp, 2048)), dim3(256), 0, st, (const uint32_t *)r.cnt_r,
+                     (const uint32_t *)r.cnt_m, r.Mp, r.out);
+  return hipGetLastError();
+}
+
+hipError_t launch_rmi_train(const LaunchRmi &r, hipStream_t st) {
+  return r.key_bytes == 4 ? launch_rmi_train_t<uint32_t>(r, st) : launch_rmi_train_t<uint64_t>(r, st);
+}
+
+uint32_t rmi_moment_blocks(uint64_t n_sample) { return grid_for(n_sample, 1024); }
+uint32_t rmi_sort_tiles(uint64_t n_sample) { return (uint32_t)((n_sample + SORT_TILE - 1) / SORT_TILE); }
+
+}  // namespace hg

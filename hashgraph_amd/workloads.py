@@ -203,3 +203,129 @@ def zipf_keys(seed: int, count: int, s: float = 1.1, universe: int = 1 << 32, de
             todo[ii[acc]] = False
         out[c0:c1] = (k_res - 1.0).to(torch.int64)
     return out
+
+
+# ---------------------------------------------------------------------------
+# Skewed u64 sets of the learned-hash comparison.
+#   lognormal: key = floor(exp(mu + sigma * z)), z standard normal by
+#     Box-Muller (cos branch) from the 53-bit doubles u1 = (r_a >> 11 + 1)/2^53,
+#     u2 = (r_b >> 11)/2^53 with r_a, r_b = elements 2j, 2j+1 of the stream,
+#     j = i*64 + t for attempt t; a value >= 2^64 is redrawn (attempt t+1),
+#     and after 64 attempts the key is 2^64 - 1.  numpy and torch agree to
+#     the last bit of libm's exp/log/cos only approximately, so each side's
+#     keys are used on their own.
+#   clusters: 64 centres c_j = hi64(x_j * (2^64 - 2^24)), x_j = element j of
+#     stream `seed`; key i = c[r >> 58] + (r & (2^24 - 1)), r = element i of
+#     stream seed + 1.  Integer-only: numpy and torch give identical keys.
+# ---------------------------------------------------------------------------
+def _np_hi64(x, m: int):
+    """hi64(x * m) for a uint64 array x and 0 < m < 2^64 (exact, by 32-bit limbs)."""
+    x = x.astype(np.uint64)
+    xl, xh = x & np.uint64(0xFFFFFFFF), x >> np.uint64(32)
+    ml, mh = np.uint64(m & 0xFFFFFFFF), np.uint64(m >> 32)
+    with np.errstate(over="ignore"):
+        ll = xl * ml
+        lh = xl * mh
+        hl = xh * ml
+        hh = xh * mh
+        mid = (ll >> np.uint64(32)) + (lh & np.uint64(0xFFFFFFFF)) + (hl & np.uint64(0xFFFFFFFF))
+        return hh + (lh >> np.uint64(32)) + (hl >> np.uint64(32)) + (mid >> np.uint64(32))
+
+
+def cluster_keys(seed: int, n: int, n_clusters: int = 64, width_bits: int = 24, device=None):
+    span = (1 << 64) - (1 << width_bits)
+    cx = stream_np(seed, n_clusters)
+    centres = _np_hi64(cx, span)
+    cbits = n_clusters.bit_length() - 1
+    assert 1 << cbits == n_clusters
+    if device is None:
+        r = stream_np(seed + 1, n)
+        idx = (r >> np.uint64(64 - cbits)).astype(np.int64)
+        with np.errstate(over="ignore"):
+            return centres[idx] + (r & np.uint64((1 << width_bits) - 1))
+    import torch
+    ct = torch.tensor(centres.view(np.int64), device=device)
+    r = stream_torch(seed + 1, n, device)
+    idx = _t_shr(r, 64 - cbits)
+    return ct[idx] + (r & ((1 << width_bits) - 1))
+
+
+def lognormal_keys(seed: int, n: int, mu: float = 40.0, sigma: float = 2.0, device=None, chunk: int = 1 << 24):
+    two64 = 18446744073709551616.0
+    if device is None:
+        out = np.empty(n, dtype=np.uint64)
+        for c0 in range(0, n, chunk):
+            c1 = min(n, c0 + chunk)
+            idx = np.arange(c0, c1, dtype=np.uint64)
+            res = np.full(c1 - c0, two64)
+            todo = np.ones(c1 - c0, dtype=bool)
+            s0 = np.uint64(mix64_int(seed))
+            for t in range(64):
+                ii = np.nonzero(todo)[0]
+                if len(ii) == 0:
+                    break
+                j = idx[ii] * np.uint64(64) + np.uint64(t)
+                with np.errstate(over="ignore"):
+                    ra = _np_mix64(s0 + (j * np.uint64(2) + np.uint64(1)) * np.uint64(GAMMA))
+                    rb = _np_mix64(s0 + (j * np.uint64(2) + np.uint64(2)) * np.uint64(GAMMA))
+                u1 = ((ra >> np.uint64(11)).astype(np.float64) + 1.0) * (1.0 / 9007199254740992.0)
+                u2 = (rb >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+                z = np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * np.pi * u2)
+                v = np.floor(np.exp(mu + sigma * z))
+                ok = v < two64
+                res[ii[ok]] = v[ok]
+                todo[ii[ok]] = False
+            o = np.empty(c1 - c0, dtype=np.uint64)
+            big = res >= two64
+            o[~big] = res[~big].astype(np.uint64)
+            o[big] = np.uint64(0xFFFFFFFFFFFFFFFF)
+            out[c0:c1] = o
+        return out
+    import torch
+    out = torch.empty(n, dtype=torch.int64, device=device)
+    s0 = _s64(mix64_int(seed))
+    for c0 in range(0, n, chunk):
+        c1 = min(n, c0 + chunk)
+        idx = torch.arange(c0, c1, dtype=torch.int64, device=device)
+        res = torch.full((c1 - c0,), two64, dtype=torch.float64, device=device)
+        todo = torch.ones(c1 - c0, dtype=torch.bool, device=device)
+        for t in range(64):
+            ii = torch.nonzero(todo).flatten()
+            if ii.numel() == 0:
+                break
+            j = idx[ii] * 64 + t
+            ra = _t_mix64((j * 2 + 1) * _s64(GAMMA) + s0)
+            rb = _t_mix64((j * 2 + 2) * _s64(GAMMA) + s0)
+            u1 = (_t_shr(ra, 11).to(torch.float64) + 1.0) * (1.0 / 9007199254740992.0)
+            u2 = _t_shr(rb, 11).to(torch.float64) * (1.0 / 9007199254740992.0)
+            z = torch.sqrt(-2.0 * torch.log(u1)) * torch.cos(2.0 * 3.141592653589793 * u2)
+            v = torch.floor(torch.exp(mu + sigma * z))
+            ok = v < two64
+            res[ii[ok]] = v[ok]
+            todo[ii[ok]] = False
+        # float64 in [2^63, 2^64) -> int64 bit pattern of the uint64 value
+        hi = res >= 9223372036854775808.0
+        lo_part = torch.where(hi, res - 9223372036854775808.0, res)
+        o = lo_part.to(torch.int64)
+        o = torch.where(hi, o + (-9223372036854775807 - 1), o)
+        o = torch.where(res >= two64, torch.full_like(o, -1), o)
+        out[c0:c1] = o
+    return out
+
+
+def mixed_probe(A, fresh, seed: int, frac: float):
+    """Probe set: element i is A[(r >> 32) * len(A) >> 32] when the low 16 bits
+    of r (element i of stream `seed`) fall below frac * 65536, else fresh[i]."""
+    thr = int(round(frac * 65536))
+    if isinstance(A, np.ndarray):
+        n = len(fresh)
+        r = stream_np(seed, n)
+        take = (r & np.uint64(0xFFFF)) < np.uint64(thr)
+        idx = (((r >> np.uint64(32)) * np.uint64(len(A))) >> np.uint64(32)).astype(np.int64)
+        return np.where(take, A[idx], fresh)
+    import torch
+    n = fresh.numel()
+    r = stream_torch(seed, n, A.device)
+    take = (r & 0xFFFF) < thr
+    idx = (_t_shr(r, 32) * A.numel()) >> 32
+    return torch.where(take, A[idx], fresh)

@@ -156,15 +156,30 @@ hg_status hg_build_partitioned(hg_ctx ctx, const hg_table_desc *desc, const void
 
 /* ---- learned hash (RMI) ---------------------------------------------------
  * No counterpart in the reference.  A two-stage piecewise-linear CDF model in
- * FP32: root (slope, intercept) picks one of n_leaves leaves, each leaf
- * (slope, intercept) predicts the CDF; bin = min(M-1, floor(M * clamp(cdf))).
- * hg_rmi_train fits root and leaves by least squares on a key sample (device
- * kernel), then compares max-bin and mean-probe length against murmur on
- * that sample; *h_use_rmi = 1 when the model wins, 0 when murmur should be
- * used (the fallback).  d_model must hold hg_rmi_model_bytes(n_leaves).   */
+ * FP32: a linear root picks one of n_leaves leaves, each leaf's line maps the
+ * key's distance from the leaf's first key to a bin; bases are monotone, so
+ * the bin order follows the key order.  Layout: 32-byte header, then
+ * n_leaves + 1 leaves of 16 bytes (the last is a sentinel).
+ *
+ * hg_rmi_train draws n_sample keys of d_keys (with replacement, SplitMix64
+ * stream `seed`), sorts them, fits root and leaves by least squares in FP64
+ * on the device, stores the model in FP32 at d_model, then bins the sample
+ * with the model and with murmur3 at the table's load factor (n_bins *
+ * n_sample / n_keys bins).  *h_use_rmi = 1 when the model's max bin and its
+ * sum of squared bin sizes (the mean probe length times n_sample) are both
+ * no larger than murmur's; 0 means murmur is the better hash (the fallback).
+ * h_report (optional) receives the four figures.  n_leaves in [1, 65536],
+ * n_sample in [1, 2^31).  d_model needs hg_rmi_model_bytes(n_leaves).     */
+typedef struct {
+  uint64_t sample_bins;     /* bins the sample was binned into */
+  uint64_t max_bin_rmi, max_bin_murmur;
+  double mean_probe_rmi, mean_probe_murmur; /* sum of squared bin sizes / n_sample */
+} hg_rmi_report;
+
 size_t hg_rmi_model_bytes(uint32_t n_leaves);
-hg_status hg_rmi_train(hg_ctx ctx, const void *d_sample, uint64_t n_sample, uint32_t key_bytes, uint64_t n_bins,
-                       uint32_t n_leaves, void *d_model, uint32_t *h_use_rmi);
+hg_status hg_rmi_train(hg_ctx ctx, const void *d_keys, uint64_t n_keys, uint32_t key_bytes, uint64_t n_sample,
+                       uint64_t seed, uint64_t n_bins, uint32_t n_leaves, void *d_model, uint32_t *h_use_rmi,
+                       hg_rmi_report *h_report);
 
 /* ---- batched form -----------------------------------------------------------
  * No counterpart in the reference.  Builds n_tables independent tables (and
@@ -200,6 +215,8 @@ hg_status hg_exclusive_scan(hg_ctx ctx, const void *d_in, void *d_out, uint64_t 
 /* bins of n keys (murmur3, seed 0, low 32 bits, mod M) -> uint32. */
 hg_status hg_hash_bins(hg_ctx ctx, const void *d_keys, uint64_t n, uint32_t key_bytes, uint64_t n_bins,
                        uint32_t *d_bins);
+/* bins of n keys under a table's hash (murmur3 or its learned model). */
+hg_status hg_table_bins(hg_ctx ctx, const hg_table_desc *desc, const void *d_keys, uint64_t n, uint32_t *d_bins);
 
 #ifdef __cplusplus
 }

@@ -105,20 +105,76 @@ void hgo_bins(const void *keys, uint64_t n, int kb, uint64_t M, uint32_t *bins) 
 /* scan, offsets[M] = N, scatter) with the nondeterministic atomic order      */
 /* inside a bin replaced by input order.                                       */
 /* ------------------------------------------------------------------------ */
-int hgo_build_csr(const void *keys, uint64_t n, int kb, uint64_t M, uint64_t *off, void *out) {
+int hgo_build_csr_bins(const void *keys, const uint32_t *bins, uint64_t n, int kb, uint64_t M, uint64_t *off,
+                       void *out) {
   if (M == 0) return -1;
-  uint32_t *bins = (uint32_t *)malloc(n ? n * 4 : 4);
   uint64_t *cur = (uint64_t *)calloc(M + 1, 8);
-  if (!bins || !cur) { free(bins); free(cur); return -2; }
-  hgo_bins(keys, n, kb, M, bins);
+  if (!cur) return -2;
   memset(off, 0, (M + 1) * 8);
-  for (uint64_t i = 0; i < n; i++) off[bins[i]]++;
+  for (uint64_t i = 0; i < n; i++) {
+    if (bins[i] >= M) { free(cur); return -3; }
+    off[bins[i]]++;
+  }
   uint64_t run = 0;
   for (uint64_t b = 0; b < M; b++) { uint64_t c = off[b]; off[b] = run; cur[b] = run; run += c; }
   off[M] = n;  /* src/SingleHashGraph.cu:173 writes offsets[M] = N */
   for (uint64_t i = 0; i < n; i++) store_key(out, kb, cur[bins[i]]++, load_key(keys, kb, i));
-  free(bins); free(cur);
+  free(cur);
   return 0;
+}
+
+int hgo_build_csr(const void *keys, uint64_t n, int kb, uint64_t M, uint64_t *off, void *out) {
+  if (M == 0) return -1;
+  uint32_t *bins = (uint32_t *)malloc(n ? n * 4 : 4);
+  if (!bins) return -2;
+  hgo_bins(keys, n, kb, M, bins);
+  int rc = hgo_build_csr_bins(keys, bins, n, kb, M, off, out);
+  free(bins);
+  return rc;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Learned CDF model (no counterpart in the reference): evaluation of the    */
+/* FP32 two-stage model, byte layout as the product's (32-byte header:       */
+/* u64 key_min, f32 root_slope, f32 root_icpt, u32 n_leaves, u32 n_bins, 8    */
+/* pad bytes; then n_leaves + 1 leaves of {u64 x0, f32 slope, u32 base}).     */
+/* Every product and sum is rounded on its own (-ffp-contract=off), u64 ->    */
+/* f32 conversions round to nearest even, float -> u32 truncates.             */
+/* Pinned by closed forms on hand-built models, an exact integer restatement  */
+/* of the u64 -> f32 rounding in the tests, monotonicity, and the intersection */
+/* being independent of the hash.                                             */
+/* ------------------------------------------------------------------------ */
+typedef struct { uint64_t key_min; float root_slope, root_icpt; uint32_t n_leaves, n_bins, pad[2]; } hgo_rmi_header;
+typedef struct { uint64_t x0; float slope; uint32_t base; } hgo_rmi_leaf;
+
+uint32_t hgo_rmi_leaf_of(const void *model, uint64_t key) {
+  const hgo_rmi_header *h = (const hgo_rmi_header *)model;
+  float xr = key > h->key_min ? (float)(key - h->key_min) : 0.0f;
+  float p = h->root_slope * xr;
+  float t = p + h->root_icpt;
+  uint32_t L = h->n_leaves;
+  if (!(t > 0.0f)) return 0;
+  if (t >= (float)(L - 1)) return L - 1;
+  return (uint32_t)t;
+}
+
+uint32_t hgo_rmi_bin(const void *model, uint64_t key, uint64_t M) {
+  const hgo_rmi_header *h = (const hgo_rmi_header *)model;
+  const hgo_rmi_leaf *lv = (const hgo_rmi_leaf *)(h + 1);
+  uint32_t leaf = hgo_rmi_leaf_of(model, key);
+  hgo_rmi_leaf l = lv[leaf];
+  uint32_t hi = lv[leaf + 1].base;
+  float d = key > l.x0 ? (float)(key - l.x0) : 0.0f;
+  float y = l.slope * d;
+  uint32_t off = !(y > 0.0f) ? 0u : (y >= 4294967040.0f ? 0xFFFFFF00u : (uint32_t)y);
+  uint32_t b = l.base + off;
+  if (b < l.base || b > hi) b = hi;
+  return b < M ? b : (uint32_t)(M - 1);
+}
+
+void hgo_bins_rmi(const void *keys, uint64_t n, int kb, const void *model, uint64_t M, uint32_t *bins) {
+  #pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < (int64_t)n; i++) bins[i] = hgo_rmi_bin(model, load_key(keys, kb, (uint64_t)i), M);
 }
 
 /* ------------------------------------------------------------------------ */

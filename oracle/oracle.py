@@ -62,6 +62,12 @@ def lib():
                                     _vp, _vp]
         L.hgo_build_multi.argtypes = [_vp, ctypes.c_uint64, ctypes.c_int, ctypes.c_uint64, _vp, ctypes.c_uint64,
                                       _vp, _vp]
+        L.hgo_build_csr_bins.argtypes = [_vp, _vp, ctypes.c_uint64, ctypes.c_int, ctypes.c_uint64, _vp, _vp]
+        L.hgo_rmi_leaf_of.argtypes = [_vp, ctypes.c_uint64]
+        L.hgo_rmi_leaf_of.restype = ctypes.c_uint32
+        L.hgo_rmi_bin.argtypes = [_vp, ctypes.c_uint64, ctypes.c_uint64]
+        L.hgo_rmi_bin.restype = ctypes.c_uint32
+        L.hgo_bins_rmi.argtypes = [_vp, ctypes.c_uint64, ctypes.c_int, _vp, ctypes.c_uint64, _vp]
         L.hgo_splitmix.argtypes = [ctypes.c_uint64, ctypes.c_uint64]
         L.hgo_splitmix.restype = ctypes.c_uint64
         _lib = L
@@ -120,6 +126,56 @@ def build_lrb(keys: np.ndarray, M: int, lrb_bins: int = 16000, threads: int = 0)
     if rc:
         raise ValueError(f"hgo_build_lrb failed ({rc})")
     return off, out
+
+
+def build_csr_bins(keys: np.ndarray, bins_arr: np.ndarray, M: int):
+    """Canonical CSR from given bins (any hash); offsets uint64[M+1]."""
+    keys = np.ascontiguousarray(keys)
+    bins_arr = np.ascontiguousarray(bins_arr, dtype=np.uint32)
+    off = np.empty(M + 1, dtype=np.uint64)
+    out = np.empty_like(keys)
+    rc = lib().hgo_build_csr_bins(_ptr(keys), _ptr(bins_arr), len(keys), _kb(keys), M, _ptr(off), _ptr(out))
+    if rc:
+        raise ValueError(f"hgo_build_csr_bins failed ({rc})")
+    return off, out
+
+
+def rmi_leaf(model: np.ndarray, key: int) -> int:
+    """Root stage of the learned model (model = its bytes as uint8)."""
+    model = np.ascontiguousarray(model, dtype=np.uint8)
+    return int(lib().hgo_rmi_leaf_of(_ptr(model), key & 0xFFFFFFFFFFFFFFFF))
+
+
+def rmi_bin(model: np.ndarray, key: int, M: int) -> int:
+    model = np.ascontiguousarray(model, dtype=np.uint8)
+    return int(lib().hgo_rmi_bin(_ptr(model), key & 0xFFFFFFFFFFFFFFFF, M))
+
+
+def bins_rmi(keys: np.ndarray, model: np.ndarray, M: int) -> np.ndarray:
+    """Bins of keys under the learned model (model = its bytes as uint8)."""
+    keys = np.ascontiguousarray(keys)
+    model = np.ascontiguousarray(model, dtype=np.uint8)
+    out = np.empty(len(keys), dtype=np.uint32)
+    lib().hgo_bins_rmi(_ptr(keys), len(keys), _kb(keys), _ptr(model), M, _ptr(out))
+    return out
+
+
+def rmi_model(key_min: int, root_slope: float, root_icpt: float, leaves, M: int) -> np.ndarray:
+    """Bytes of a learned model: leaves = [(x0, slope, base), ...]; the
+    sentinel leaf (base M-1) is appended."""
+    L = len(leaves)
+    hdr = np.zeros(1, dtype=[("key_min", "<u8"), ("rs", "<f4"), ("ri", "<f4"), ("L", "<u4"), ("M", "<u4"),
+                             ("pad", "<u4", 2)])
+    hdr["key_min"] = key_min
+    hdr["rs"] = root_slope
+    hdr["ri"] = root_icpt
+    hdr["L"] = L
+    hdr["M"] = M
+    lv = np.zeros(L + 1, dtype=[("x0", "<u8"), ("slope", "<f4"), ("base", "<u4")])
+    for i, (x0, sl, b) in enumerate(leaves):
+        lv[i] = (x0, sl, b)
+    lv[L] = (0xFFFFFFFFFFFFFFFF, 0.0, M - 1)
+    return np.concatenate([hdr.view(np.uint8), lv.view(np.uint8)])
 
 
 def canon(off: np.ndarray, keys: np.ndarray) -> np.ndarray:

@@ -1,0 +1,168 @@
+"""Learned hash (RMI) on the GPU: model evaluation parity with the oracle,
+training against a CPU restatement of its sampling and np.polyfit (a library
+least-squares routine), the sampled fallback figures against exact CPU
+counts, and the CSR / intersection of learned-hash tables (the intersection
+must equal murmur's: the join does not depend on the hash)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+from hashgraph_amd import workloads as wl
+from oracle import oracle as o
+
+HDR = np.dtype([("key_min", "<u8"), ("rs", "<f4"), ("ri", "<f4"), ("L", "<u4"), ("M", "<u4"), ("pad", "<u4", 2)])
+LEAF = np.dtype([("x0", "<u8"), ("slope", "<f4"), ("base", "<u4")])
+
+
+def _to_dev(a, dev):
+    import torch
+    return torch.from_numpy(a.view(np.int32 if a.dtype == np.uint32 else np.int64).copy()).to(dev)
+
+
+def _model_dev(m, dev):
+    import torch
+    return torch.from_numpy(m.copy()).to(dev)
+
+
+def _parse(model_bytes):
+    h = model_bytes[:32].view(HDR)[0]
+    lv = model_bytes[32:].view(LEAF)
+    return h, lv
+
+
+def _sample_cpu(keys, n_sample, seed):
+    pos = wl._np_hi64(wl.stream_np(seed, n_sample), len(keys)).astype(np.int64)
+    return keys[pos]
+
+
+def test_table_bins_match_oracle_u64(dev):
+    import hashgraph_amd as hg
+    rng = np.random.default_rng(5)
+    L, M = 64, 1 << 20
+    x0 = np.sort(rng.integers(0, 1 << 63, L, dtype=np.uint64))
+    base = np.sort(rng.integers(0, M, L))
+    leaves = [(int(x0[i]), float(rng.random() * 1e-12), int(base[i])) for i in range(L)]
+    m = o.rmi_model(int(x0[0]), float(L / 2.0 ** 63), 0.25, leaves, M)
+    keys = rng.integers(0, 1 << 64, 1 << 18, dtype=np.uint64)
+    keys[:8] = [0, 1, int(x0[0]) - 1, int(x0[0]), (1 << 64) - 1, (1 << 63), int(x0[5]), int(x0[5]) + 1]
+    print("table_bins rmi u64", flush=True)
+    got = hg.table_bins(_to_dev(keys, dev), M, hg.HG_HASH_RMI, _model_dev(m, dev)).cpu().numpy().view(np.uint32)
+    assert np.array_equal(got, o.bins_rmi(keys, m, M))
+
+
+def test_table_bins_match_oracle_u32(dev):
+    import hashgraph_amd as hg
+    M = 1000
+    leaves = [(1024 * l, 0.25, 250 * l) for l in range(4)]
+    m = o.rmi_model(0, 1.0 / 1024, 0.0, leaves, M)
+    keys = np.arange(0, 1 << 16, dtype=np.uint32)
+    got = hg.table_bins(_to_dev(keys, dev), M, hg.HG_HASH_RMI, _model_dev(m, dev)).cpu().numpy().view(np.uint32)
+    assert np.array_equal(got, o.bins_rmi(keys, m, M))
+
+
+@pytest.mark.parametrize("kind", ["uniform64", "lognormal", "clusters", "uniform32"])
+def test_train_against_cpu_fit(dev, kind):
+    import hashgraph_amd as hg
+    n, ns, M, L, seed = 1 << 20, 1 << 15, 1 << 20, 1024, 9
+    if kind == "uniform64":
+        A = wl.uniform_keys(3, n, 8)
+    elif kind == "uniform32":
+        A = wl.uniform_keys(3, n, 4)
+    elif kind == "lognormal":
+        A = wl.lognormal_keys(3, n)
+    else:
+        A = wl.cluster_keys(3, n)
+    print(f"rmi train {kind}", flush=True)
+    model, use, rep = hg.rmi_train(_to_dev(A, dev), M, n_sample=ns, seed=seed, n_leaves=L, report=True)
+    mb = model.cpu().numpy()
+    h, lv = _parse(mb)
+    S = np.sort(_sample_cpu(A, ns, seed)).astype(np.uint64)
+    assert h["key_min"] == S[0] and h["L"] == L and h["M"] == M
+    # root: least squares of i*L/ns on (key - key_min), against np.polyfit
+    x = (S - S[0]).astype(np.float64)
+    y = np.arange(ns, dtype=np.float64) * (L / ns)
+    b1, b0 = np.polyfit(x, y, 1)
+    assert np.isclose(h["rs"], max(b1, 0.0), rtol=1e-5, atol=0)
+    assert abs(h["ri"] - b0) <= 1e-4 * L
+    # routing of the sorted sample under the stored FP32 root
+    leaf = np.array([o.rmi_leaf(mb, int(k)) for k in S])
+    assert (np.diff(leaf) >= 0).all()
+    start = np.searchsorted(leaf, np.arange(L + 1), side="left")
+    yb = np.arange(ns, dtype=np.float64) * (M / ns)
+    want_base = np.zeros(L)
+    checked = 0
+    for l in range(L):
+        a, b = start[l], start[l + 1]
+        if b == a:
+            alpha = a * (M / ns)
+        else:
+            assert lv[l]["x0"] == S[a]
+            if S[b - 1] > S[a]:
+                d = (S[a:b] - S[a]).astype(np.float64)
+                s1, alpha = np.polyfit(d, yb[a:b], 1)
+                if s1 <= 0:
+                    alpha = yb[a:b].mean()
+                elif b - a >= 8:
+                    assert np.isclose(lv[l]["slope"], s1, rtol=1e-4, atol=0)
+                    checked += 1
+            else:
+                alpha = yb[a:b].mean()
+        want_base[l] = min(max(np.floor(alpha + 0.5), 0), M - 1)
+    want_base = np.maximum.accumulate(want_base)
+    assert np.abs(lv["base"][:L].astype(np.int64) - want_base.astype(np.int64)).max() <= 1
+    assert checked > 0
+    assert (np.diff(lv["base"][:L].astype(np.int64)) >= 0).all()
+    assert lv[L]["base"] == M - 1
+    # the fallback figures, recomputed exactly on the CPU
+    Sm = _sample_cpu(A, ns, seed)
+    Mp = max(1, min(M, int(M * ns / n + 0.5)))
+    br = o.bins_rmi(Sm, mb, M).astype(np.uint64) * np.uint64(Mp) // np.uint64(M)
+    bm = o.bins(Sm, M).astype(np.uint64) * np.uint64(Mp) // np.uint64(M)
+    cr = np.bincount(br.astype(np.int64), minlength=Mp)
+    cm = np.bincount(bm.astype(np.int64), minlength=Mp)
+    assert rep["sample_bins"] == Mp
+    assert rep["max_bin_rmi"] == cr.max() and rep["max_bin_murmur"] == cm.max()
+    assert np.isclose(rep["mean_probe_rmi"], (cr.astype(np.float64) ** 2).sum() / ns)
+    assert np.isclose(rep["mean_probe_murmur"], (cm.astype(np.float64) ** 2).sum() / ns)
+    assert use == (cr.max() <= cm.max() and (cr ** 2).sum() <= (cm ** 2).sum())
+
+
+@pytest.mark.parametrize("kind,kb", [("clusters", 8), ("lognormal", 8), ("uniform", 4), ("uniform", 8)])
+def test_build_and_probe_learned(dev, kind, kb):
+    """CSR under the learned hash equals the oracle's CSR from the same bins,
+    and the probe result equals murmur's and the oracle's bag join."""
+    import hashgraph_amd as hg
+    n, M = 1 << 20, 1 << 20
+    if kind == "clusters":
+        A = wl.cluster_keys(4, n)
+        B = wl.mixed_probe(A, wl.cluster_keys(40, n), 5, 0.5)
+    elif kind == "lognormal":
+        A = wl.lognormal_keys(4, n)
+        B = wl.mixed_probe(A, wl.lognormal_keys(40, n), 5, 0.5)
+    else:
+        A = wl.uniform_keys(4, n, kb)
+        B = wl.probe_keys(A, 5, n, 0.5, kb)
+    print(f"rmi build/probe {kind} kb={kb}", flush=True)
+    dA, dB = _to_dev(A, dev), _to_dev(B, dev)
+    model, use = hg.rmi_train(dA, M, n_sample=n // 100, seed=1)
+    mb = model.cpu().numpy()
+    off, csr = hg.build(dA, M, hash_kind=hg.HG_HASH_RMI, rmi_model=model)
+    eoff, ekeys = o.build_csr_bins(A, o.bins_rmi(A, mb, M), M)
+    offn = off.cpu().numpy().astype(np.uint64)
+    assert np.array_equal(offn, eoff)
+    got = csr.cpu().numpy().view(A.dtype).copy()
+    o.canon(eoff, got)
+    o.canon(eoff, ekeys)
+    assert np.array_equal(got, ekeys)
+    try:
+        cnt, out = hg.probe(off, csr, M, dB, hash_kind=hg.HG_HASH_RMI, rmi_model=model)
+    except hg.HGError as e:
+        # a bin beyond the probe's walk limit: only acceptable for a model that
+        # lost the sampled check
+        assert not use and e.status == 6, e
+        return
+    offm, csrm = hg.build(dA, M)
+    cntm, outm = hg.probe(offm, csrm, M, dB)
+    assert cnt == cntm == o.bag_intersect_count(A, B)
+    assert np.array_equal(np.sort(out.cpu().numpy().view(A.dtype)), np.sort(outm.cpu().numpy().view(A.dtype)))
