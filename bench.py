@@ -98,14 +98,161 @@ def cpu_baseline(cfg, threads):
     }
 
 
+def bin_stats(off):
+    """max bin, mean length of non-empty bins, mean probe length (sum c^2 / n)."""
+    import torch
+    c = (off[1:] - off[:-1]).to(torch.int64)
+    n = int(c.sum().item())
+    nz = int((c > 0).sum().item())
+    return {"max_bin": int(c.max().item()), "mean_nonempty_bin": round(n / max(nz, 1), 4),
+            "mean_probe_len": round(float((c.double() ** 2).sum().item()) / max(n, 1), 4)}
+
+
+def _time_table(A, B, M, hk, mdl, steps, dev):
+    import torch
+    import hashgraph_amd as hg
+    n = A.numel()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    off = torch.empty(M + 1, dtype=torch.int32, device=dev)
+    csr = torch.empty_like(A)
+    hg.build(A, M, out=(off, csr), hash_kind=hk, rmi_model=mdl)
+    K, _ = hg.probe(off, csr, M, B, emit=False, hash_kind=hk, rmi_model=mdl)
+    out = torch.empty(max(K, 1), dtype=A.dtype, device=dev)
+    for _ in range(2):
+        hg.probe(off, csr, M, B, out=out, hash_kind=hk, rmi_model=mdl)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(steps):
+        hg.build(A, M, out=(off, csr), hash_kind=hk, rmi_model=mdl)
+    e1.record()
+    torch.cuda.synchronize()
+    ms_b = e0.elapsed_time(e1) / steps
+    e0.record()
+    for _ in range(steps):
+        hg.probe(off, csr, M, B, out=out, hash_kind=hk, rmi_model=mdl)
+    e1.record()
+    torch.cuda.synchronize()
+    ms_p = e0.elapsed_time(e1) / steps
+    st = bin_stats(off)
+    st.update({"build_gkeys_s": round(n / ms_b / 1e6, 3), "probe_gkeys_s": round(B.numel() / ms_p / 1e6, 3),
+               "ms_build": round(ms_b, 4), "ms_probe": round(ms_p, 4), "matches": K})
+    return st, torch.sort(out[:K])[0]
+
+
+def learned_block(dev, steps, n=1 << 27, frac=0.5):
+    """Learned hash beside murmur on the skewed u64 sets (lognormal(40, 2) and
+    64 clusters of width 2^24): probe throughput of both, bin statistics, the
+    fallback decision and equality of the intersections."""
+    import torch
+    import hashgraph_amd as hg
+    from hashgraph_amd import workloads as wl
+    M = n
+    res = {}
+    for name in ("lognormal", "clusters"):
+        gen = wl.lognormal_keys if name == "lognormal" else wl.cluster_keys
+        A = gen(21, n, device=dev)
+        B = wl.mixed_probe(A, gen(22, n, device=dev), 23, frac)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        model, use, rep = hg.rmi_train(A, M, n_sample=n // 100, seed=24, report=True)
+        torch.cuda.synchronize()
+        t_train = time.perf_counter() - t0
+        row = {"fallback_to_murmur": not use, "train_ms": round(t_train * 1e3, 3), "sample_check": rep}
+        lists = {}
+        for hk, tag in ((hg.HG_HASH_MURMUR3, "murmur"), (hg.HG_HASH_RMI, "learned")):
+            mdl = model if hk == hg.HG_HASH_RMI else None
+            try:
+                row[tag], lists[tag] = _time_table(A, B, M, hk, mdl, steps, dev)
+            except hg.HGError as e:
+                row[tag] = {"error": str(e)}
+        if "learned" not in lists or "murmur" not in lists:
+            res[name] = row
+            continue
+        row["identical_intersection"] = bool(lists["murmur"].numel() == lists["learned"].numel() and
+                                             torch.equal(lists["murmur"], lists["learned"]))
+        row["probe_ratio_learned_over_murmur"] = round(row["learned"]["probe_gkeys_s"] /
+                                                       row["murmur"]["probe_gkeys_s"], 4)
+        res[name] = row
+        del A, B, model, lists
+        torch.cuda.empty_cache()
+    return {"workload": f"A = {n} uint64 keys, {M} bins; B = {n} keys, {int(frac * 100)}% drawn from A, the rest "
+                        f"fresh from the same distribution; RMI with 4096 leaves trained on a 1% sample",
+            **res}
+
+
+def c3_run(dev, steps, warmup):
+    """One GPU's share of MultiHashGraph: 2^30 uint64 Zipf(1.1) keys over 2^32
+    ranks, 2^30 bins, 16 logical partitions (key-balanced splits over a
+    16000-range histogram), per-partition CSR; B = 2^29 Zipf keys built with
+    A's splits and joined partition by partition with the run-length join
+    (the count, ~10^16, needs 64 bits and a run-length output)."""
+    import torch
+    import hashgraph_amd as hg
+    from hashgraph_amd import workloads as wl
+    nA, nB, M, P = 1 << 30, 1 << 29, 1 << 30, 16
+    log("c3: generating keys")
+    A = wl.zipf_keys(3, nA, 1.1, device=dev)
+    B = wl.zipf_keys(4, nB, 1.1, device=dev)
+    torch.cuda.synchronize()
+
+    def build_side(keys, splits=None):
+        sp = hg.find_splits(keys, M, 16000, P) if splits is None else splits
+        off, csr, po = hg.build_partitioned(keys, M, sp)
+        return sp, off, csr, po
+
+    def join(sa, ta, tb):
+        _, offA, csrA, poA = ta
+        _, offB, csrB, poB = tb
+        total, recs = 0, 0
+        for p in range(P):
+            lo, hi = sa[p], sa[p + 1]
+            oa = offA[lo + p: hi + p + 1]
+            ob = offB[lo + p: hi + p + 1]
+            ka = csrA[poA[p]: poA[p + 1]]
+            kb = csrB[poB[p]: poB[p + 1]]
+            cnt, rk, ca, cb = hg.intersect_rle(oa, ka, ob, kb, hi - lo)
+            total += cnt
+            recs += rk.numel()
+        return total, recs
+
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    for _ in range(warmup):
+        ta = build_side(A)
+        tb = build_side(B, ta[0])
+        cnt, recs = join(ta[0], ta, tb)
+        del ta, tb
+    torch.cuda.synchronize()
+    tb_ms, ti_ms = 0.0, 0.0
+    for _ in range(steps):
+        e0.record()
+        ta = build_side(A)
+        e1.record()
+        torch.cuda.synchronize()
+        tb_ms += e0.elapsed_time(e1)
+        e0.record()
+        tb = build_side(B, ta[0])
+        cnt, recs = join(ta[0], ta, tb)
+        e1.record()
+        torch.cuda.synchronize()
+        ti_ms += e0.elapsed_time(e1)
+        del ta, tb
+    tb_ms /= steps
+    ti_ms /= steps
+    return {"nA": nA, "nB": nB, "M": M, "P": P, "ms_build": tb_ms, "ms_intersect": ti_ms, "count": cnt,
+            "records": recs}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="c2-u32", choices=sorted(CONFIGS))
+    ap.add_argument("--config", default="c2-u32", choices=sorted(CONFIGS) + ["c3"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-hash comparison")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run device copies of MIB MiB (calibrates the HBM byte counters)")
     args = ap.parse_args()
@@ -128,6 +275,28 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_read": n, "copy_bytes_written": n, "launches": max(args.steps, 1)}))
+        return
+    if args.config == "c3":
+        r = c3_run(dev, args.steps, args.warmup)
+        ms = r["ms_build"] + r["ms_intersect"]
+        build_bytes = 36 * r["nA"]  # 4 s N + 4 (M + 1) with s = 8, N = M
+        inter_bytes = 24 * r["nB"] + 8 * r["nA"] + 4 * r["M"]
+        out = {
+            "metric": "Gkeys/s build & intersect at 2^28 u32 keys, %HBM peak; learned/murmur ratio",
+            "value": round((r["nA"] + r["nB"]) / (ms / 1e3) / 1e9, 3), "unit": "Gkeys/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "u64", "data": "synthetic (Zipf(1.1) on the device)",
+            "config": {"workload": "one GPU's share of MultiHashGraph: A = 2^30 uint64 Zipf(1.1) keys over 2^32 "
+                                   "ranks, 2^30 bins, 16 logical partitions; B = 2^29 Zipf keys with A's splits; "
+                                   "run-length join per partition", "parallelism": "single GPU"},
+            "build_gkeys_s": round(r["nA"] / r["ms_build"] / 1e6, 3),
+            "intersect_gkeys_s": round(r["nB"] / r["ms_intersect"] / 1e6, 3),
+            "ms_build": round(r["ms_build"], 3), "ms_intersect": round(r["ms_intersect"], 3),
+            "build_hbm_frac": round(build_bytes / (r["ms_build"] / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+            "intersect_hbm_frac": round(inter_bytes / (r["ms_intersect"] / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+            "count": r["count"], "rle_records": r["records"], "roofline": None, "cpu_baseline": None,
+        }
+        print(json.dumps(out))
         return
     nA, nB, M, kb, frac = CONFIGS[args.config]
     s = kb
@@ -233,6 +402,11 @@ def main():
     }
     del A, B, off, csr, matches
     torch.cuda.empty_cache()
+    if not args.no_learned and args.config.startswith("c2"):
+        log("learned-hash comparison")
+        res["learned"] = learned_block(dev, max(2, min(args.steps, 5)))
+        ratios = [res["learned"][k].get("probe_ratio_learned_over_murmur") for k in ("lognormal", "clusters")]
+        res["learned_murmur_probe_ratio"] = {k: r for k, r in zip(("lognormal", "clusters"), ratios)}
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         log(f"cpu baseline ({threads} threads)")

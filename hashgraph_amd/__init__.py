@@ -18,7 +18,7 @@ from ._lib import HG_HASH_MURMUR3, HG_HASH_RMI, HGError, check
 
 __all__ = ["Context", "build", "probe", "intersect", "intersect_rle", "exclusive_scan", "hash_bins", "find_splits",
            "build_partitioned", "rmi_train", "rmi_model_bytes", "batch_build", "batch_probe", "HGError",
-           "HG_HASH_MURMUR3", "HG_HASH_RMI", "version"]
+           "HG_HASH_MURMUR3", "HG_HASH_RMI", "version", "table_bins", "BatchPlan"]
 
 _ctx = {}
 
@@ -243,7 +243,7 @@ def build_partitioned(keys, n_bins: int, splits, offset_bytes: int = 4, hash_kin
     n_parts = len(splits) - 1
     c = context(keys.device.index)
     d = _desc(keys.numel(), n_bins, _key_bytes(keys), offset_bytes, hash_kind, rmi_model)
-    off = torch.empty(int(splits[-1]) + n_parts + 1, dtype=torch.int32 if offset_bytes == 4 else torch.int64,
+    off = torch.empty(int(splits[-1]) + n_parts, dtype=torch.int32 if offset_bytes == 4 else torch.int64,
                       device=keys.device)
     csr = torch.empty_like(keys)
     sp = (ctypes.c_int64 * (n_parts + 1))(*[int(x) for x in splits])
@@ -344,3 +344,53 @@ def batch_probe(offsets, csr_keys, key_off, n_bins, off_off, probe, probe_off, h
     check(c.lib.hg_batch_probe(c.bind_stream(), ctypes.byref(bd), _p(offsets), _p(csr_keys), _p(probe), po,
                                _p(counts), ctypes.byref(total)), "hg_batch_probe")
     return total.value, counts
+
+
+class BatchPlan:
+    """A captured batch (hg_batch_plan): build of every table, then the
+    per-table probe counts when probe keys are given, replayed by run()."""
+
+    def __init__(self, keys, key_off, n_bins, probe=None, probe_off=None, hash_kinds=None, rmi_models=None,
+                 rmi_leaves=0, autotune=True):
+        torch = _torch()
+        _check_dev(keys)
+        self.off_off, run = [], 0
+        for m in n_bins:
+            self.off_off.append(run)
+            run += int(m) + 1
+        T = len(n_bins)
+        self._bd = _batch_desc(key_off, n_bins, self.off_off, _key_bytes(keys), hash_kinds, rmi_models, rmi_leaves)
+        self._c = context(keys.device.index)
+        self.offsets = torch.empty(max(run, 1), dtype=torch.int32, device=keys.device)
+        self.csr = torch.empty_like(keys)
+        self.counts = torch.zeros(T, dtype=torch.int64, device=keys.device)
+        self._keep = (keys, probe, rmi_models)
+        po = None
+        if probe is not None:
+            po = (ctypes.c_uint64 * (T + 1))(*[int(x) for x in probe_off])
+        self._plan = ctypes.c_void_p()
+        check(self._c.lib.hg_batch_plan_create(self._c.bind_stream(), ctypes.byref(self._bd), _p(keys),
+                                               _p(self.offsets), _p(self.csr),
+                                               _p(probe) if probe is not None else None, po,
+                                               _p(self.counts) if probe is not None else None, 1 if autotune else 0,
+                                               ctypes.byref(self._plan)), "hg_batch_plan_create")
+
+    def run(self):
+        self._c.bind_stream()
+        check(self._c.lib.hg_batch_plan_run(self._plan), "hg_batch_plan_run")
+
+    def info(self):
+        i = _lib.BatchPlanInfo()
+        check(self._c.lib.hg_batch_plan_query(self._plan, ctypes.byref(i)), "hg_batch_plan_query")
+        return {"tile_keys": i.tile_keys, "fill": i.fill, "candidates": i.candidates, "tuned_ms": i.tuned_ms}
+
+    def close(self):
+        if self._plan:
+            self._c.lib.hg_batch_plan_destroy(self._plan)
+            self._plan = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -24,7 +24,8 @@ EXPORTS = [
     "hg_ctx_create", "hg_ctx_destroy", "hg_ctx_set_stream", "hg_ctx_reserve", "hg_scratch_bytes", "hg_last_error",
     "hg_version", "hg_ctx_profile", "hg_ctx_profile_read", "hg_build", "hg_intersect", "hg_probe", "hg_intersect_rle", "hg_find_splits",
     "hg_build_partitioned", "hg_rmi_model_bytes", "hg_rmi_train", "hg_batch_build", "hg_batch_probe",
-    "hg_exclusive_scan", "hg_hash_bins", "hg_table_bins",
+    "hg_exclusive_scan", "hg_hash_bins", "hg_table_bins", "hg_batch_plan_create", "hg_batch_plan_run",
+    "hg_batch_plan_query", "hg_batch_plan_destroy",
 ]
 
 
@@ -44,6 +45,11 @@ class RmiReport(ctypes.Structure):
     _fields_ = [("sample_bins", ctypes.c_uint64), ("max_bin_rmi", ctypes.c_uint64),
                 ("max_bin_murmur", ctypes.c_uint64), ("mean_probe_rmi", ctypes.c_double),
                 ("mean_probe_murmur", ctypes.c_double)]
+
+
+class BatchPlanInfo(ctypes.Structure):
+    _fields_ = [("tile_keys", ctypes.c_uint32), ("fill", ctypes.c_float), ("candidates", ctypes.c_uint32),
+                ("tuned_ms", ctypes.c_float)]
 
 
 class KernelTime(ctypes.Structure):
@@ -93,6 +99,10 @@ def load(path: str = LIB_PATH):
             "hg_rmi_train": (i32, [vp, vp, u64, u32, u64, u64, u64, u32, vp, P(u32), P(RmiReport)]),
             "hg_batch_build": (i32, [vp, P(BatchDesc), vp, vp, vp]),
             "hg_batch_probe": (i32, [vp, P(BatchDesc), vp, vp, vp, P(u64), vp, P(u64)]),
+            "hg_batch_plan_create": (i32, [vp, P(BatchDesc), vp, vp, vp, vp, P(u64), vp, u32, P(vp)]),
+            "hg_batch_plan_run": (i32, [vp]),
+            "hg_batch_plan_query": (i32, [vp, P(BatchPlanInfo)]),
+            "hg_batch_plan_destroy": (i32, [vp]),
             "hg_exclusive_scan": (i32, [vp, vp, vp, u64, u32]),
             "hg_hash_bins": (i32, [vp, vp, u64, u32, u64, vp]),
             "hg_table_bins": (i32, [vp, P(TableDesc), vp, u64, vp]),

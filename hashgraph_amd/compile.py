@@ -27,7 +27,7 @@ DEVFLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 CFLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", f"--offload-arch={ARCH}",
           "-I", os.path.join(ROOT, "include"), "-I", CSRC]
 
-SOURCES = ["hg_kernels.hip", "hg_rmi.hip", "hg_engine.cpp", "hg_pending.cpp", "SingleHashGraph.cpp", "MultiHashGraph.cpp"]
+SOURCES = ["hg_kernels.hip", "hg_rmi.hip", "hg_engine.cpp", "SingleHashGraph.cpp", "MultiHashGraph.cpp"]
 DRIVERS = {"sing-hash": "drivers/sing_hash.cpp", "multi-hash": "drivers/multi_hash.cpp"}
 
 

@@ -60,8 +60,8 @@ uint32_t final_cap(int kb) { return std::min<uint32_t>(1024u * (uint32_t)final_k
 
 // Bucket width W (bins) of a table so that a bucket holds ~7/8 of what the
 // final stage keeps in VGPRs; W <= 65535 (u16 LDS counters).
-uint32_t choose_width(uint64_t n, uint64_t M, int kb) {
-  const double target = 0.875 * final_cap(kb);
+uint32_t choose_width(uint64_t n, uint64_t M, int kb, double fill = 0.875) {
+  const double target = fill * final_cap(kb);
   double w = n ? target * (double)M / (double)n : 65535.0;
   w = std::floor(w);
   if (w < 1) w = 1;
@@ -257,6 +257,7 @@ struct Level {
   std::vector<int64_t> splits;  // kind-1 segments point into this array
   uint32_t ncols_total = 0, max_cols = 0;
   uint64_t cmat_elems = 0, psum_elems = 0;
+  uint32_t tile_keys = kTile;   // keys per partition tile (<= kTile)
 
   // One kind shared by all segments selects a specialised kernel.
   int hash_kind() const {
@@ -283,10 +284,10 @@ struct Level {
     for (uint32_t s = 0; s < segs.size(); s++) {
       SegDesc &sd = segs[s];
       sd.tile_begin = (uint32_t)tiles.size();
-      for (uint64_t k = sd.key_begin; k < sd.key_end; k += kTile) {
+      for (uint64_t k = sd.key_begin; k < sd.key_end; k += tile_keys) {
         TileDesc t;
         t.key_start = k;
-        t.count = (uint32_t)std::min<uint64_t>(kTile, sd.key_end - k);
+        t.count = (uint32_t)std::min<uint64_t>(tile_keys, sd.key_end - k);
         t.seg = s;
         tiles.push_back(t);
       }
@@ -1118,6 +1119,348 @@ hg_status hg_build_partitioned(hg_ctx c, const hg_table_desc *desc, const void *
   }
   h_part_offsets[n_parts] = kept;
   return finish_two_level(c, cs, map, kb, ob, part1, part2, d_csr_keys, d_offsets);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batched tables (no counterpart in the reference).  Every table is a range
+// segment of one partition level, so one set of launches builds (or probes)
+// all of them; a plan keeps the descriptors resident and replays the launches
+// as one captured hipGraph.
+// ---------------------------------------------------------------------------
+namespace {
+
+hg_status batch_validate(const hg_batch_desc *bd) {
+  if (!bd || bd->n_tables == 0) return fail(HG_ERR_INVALID_ARG, "empty batch");
+  if (bd->key_bytes != 4 && bd->key_bytes != 8) return fail(HG_ERR_INVALID_ARG, "key_bytes must be 4 or 8");
+  if (!bd->h_key_off || !bd->h_n_bins || !bd->h_off_off) return fail(HG_ERR_INVALID_ARG, "null batch array");
+  for (uint32_t t = 0; t < bd->n_tables; t++) {
+    if (bd->h_key_off[t + 1] < bd->h_key_off[t]) return fail(HG_ERR_INVALID_ARG, "key offsets must not decrease");
+    if (bd->h_key_off[t + 1] - bd->h_key_off[t] >= (1ull << 32))
+      return fail(HG_ERR_UNSUPPORTED, "a batched table needs fewer than 2^32 keys (uint32 offsets)");
+    if (bd->h_n_bins[t] == 0 || bd->h_n_bins[t] >= (1ull << 32))
+      return fail(HG_ERR_INVALID_ARG, "n_bins must be in [1, 2^32)");
+    const uint32_t k = bd->h_hash_kind ? bd->h_hash_kind[t] : HG_HASH_MURMUR3;
+    if (k != HG_HASH_MURMUR3 && k != HG_HASH_RMI) return fail(HG_ERR_INVALID_ARG, "unknown hash kind");
+    if (k == HG_HASH_RMI && (!bd->d_rmi_models || hg_rmi_model_bytes(bd->rmi_leaves) == 0))
+      return fail(HG_ERR_INVALID_ARG, "learned tables need d_rmi_models and rmi_leaves");
+  }
+  return HG_OK;
+}
+
+BinMap batch_map(const hg_batch_desc *bd, uint32_t t) {
+  const uint32_t k = bd->h_hash_kind ? bd->h_hash_kind[t] : HG_HASH_MURMUR3;
+  const void *m = k == HG_HASH_RMI ? (const char *)bd->d_rmi_models + (size_t)t * hg_rmi_model_bytes(bd->rmi_leaves)
+                                   : nullptr;
+  return make_map(bd->h_n_bins[t], k, m);
+}
+
+struct BatchCfg {
+  uint32_t tile_keys = kTile;
+  double fill = 0.875;
+};
+
+// One side of a batch: the level over all tables, its device descriptors and
+// the partitioned copy (in arena 0 of the context that owns the side).
+struct BatchSide {
+  Level L;
+  LevelDev d;
+  void *part = nullptr;
+  unsigned int *flag = nullptr;
+  uint64_t nkeys = 0;
+};
+
+// Lays out the level of a batch side.  probe_off == nullptr: the build side
+// over the tables' keys; else the probe side over the probe sets.
+hg_status batch_prepare(hg_ctx c, const hg_batch_desc *bd, const uint64_t *probe_off, const BatchCfg &cfg,
+                        BatchSide &S) {
+  const int kb = (int)bd->key_bytes;
+  S.L = Level();
+  const uint64_t *ko = bd->h_key_off;
+  const uint64_t *src = probe_off ? probe_off : ko;
+  for (uint32_t t = 0; t < bd->n_tables; t++) {
+    const uint64_t n = ko[t + 1] - ko[t], np = src[t + 1] - src[t], M = bd->h_n_bins[t];
+    const BinMap map = batch_map(bd, t);
+    const uint32_t W = choose_width(probe_off ? std::max(n, np) : n, M, kb, cfg.fill);
+    SegDesc s = range_seg(src[t], src[t + 1], 0, M, W, map);
+    if (s.nbuckets > kFMax) return fail(HG_ERR_UNSUPPORTED, "a batched table is too large for one partition level");
+    s.fin_off_base = bd->h_off_off[t];
+    s.fin_key_rel = ko[t];
+    s.fin_end_pos = probe_off ? ~0ull : bd->h_off_off[t] + M;
+    s.table = t;
+    S.L.segs.push_back(s);
+    S.L.maps.push_back(map);
+  }
+  S.L.tile_keys = cfg.tile_keys;
+  S.L.prepare();
+  S.nkeys = src[bd->n_tables] - src[0];
+  Plan P0;
+  const size_t i_part = P0.add(std::max<uint64_t>(src[bd->n_tables], 1) * kb);
+  const size_t i_flag = P0.add(64);
+  HG_TRY(arena_ensure(c, 0, P0.total()));
+  Carve cv;
+  cv.init(c, 0, P0);
+  S.part = cv.get<void>(i_part);
+  S.flag = cv.get<unsigned int>(i_flag);
+  return setup_level(c, S.L, S.d);
+}
+
+// Kernel launches only (no allocation, upload or host sync): capturable.
+hg_status batch_launch_build(hg_ctx c, const hg_batch_desc *bd, BatchSide &S, const void *d_keys, uint32_t *d_offsets,
+                             void *d_csr) {
+  HG_TRY(run_level(c, S.L, S.d, (int)bd->key_bytes, d_keys, S.part));
+  return run_final(c, S.L, S.d, (int)bd->key_bytes, 4, S.part, d_csr, d_offsets);
+}
+
+hg_status batch_launch_probe(hg_ctx c, const hg_batch_desc *bd, BatchSide &S, const void *d_probe,
+                             const uint32_t *d_offsets, const void *d_csr, uint64_t *d_counts) {
+  HG_HIP(launch_fill_u64((unsigned long long *)d_counts, 0ull, bd->n_tables, c->stream));
+  HG_HIP(launch_fill_u32(S.flag, 0u, 1, c->stream));
+  HG_TRY(run_level(c, S.L, S.d, (int)bd->key_bytes, d_probe, S.part));
+  HG_HIP(launch_make_items(S.d.segs, (uint32_t)S.L.segs.size(), S.d.bstart, S.d.coltot, S.d.fin, S.L.ncols_total,
+                           c->stream));
+  LaunchProbe pr{};
+  pr.key_bytes = (int)bd->key_bytes;
+  pr.off_bytes = 4;
+  pr.hk = S.L.hash_kind();
+  pr.nitems = S.L.ncols_total;
+  pr.akey_cap = probe_akey_cap(pr.key_bytes, 4);
+  pr.bpart = S.part;
+  pr.items = S.d.fin;
+  pr.maps = S.d.maps;
+  pr.offA = d_offsets;
+  pr.keysA = d_csr;
+  pr.out = nullptr;
+  pr.capacity = 0;
+  pr.cursor = (unsigned long long *)d_counts;
+  pr.flag = S.flag;
+  Prof pf(c, "probe");
+  HG_HIP(launch_probe(pr, c->stream));
+  return HG_OK;
+}
+
+}  // namespace
+
+struct hg_batch_plan_s {
+  int device = 0;
+  hg_ctx owner = nullptr;             // caller's context (stream to order against)
+  hg_ctx cb = nullptr, cp = nullptr;  // private contexts of the build and probe sides
+  hipStream_t stream = nullptr;       // private stream the graph runs on
+  hipEvent_t ev_in = nullptr, ev_out = nullptr, t0 = nullptr, t1 = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hg_batch_desc bd{};
+  std::vector<uint64_t> key_off, n_bins, off_off, probe_off;
+  std::vector<uint32_t> kinds;
+  const void *d_keys = nullptr, *d_probe = nullptr;
+  uint32_t *d_offsets = nullptr;
+  void *d_csr = nullptr;
+  uint64_t *d_counts = nullptr;
+  BatchSide sb, sp;
+  BatchCfg cfg;
+  hg_batch_plan_info info{};
+};
+
+namespace {
+
+void plan_drop_graph(hg_batch_plan p) {
+  if (p->exec) (void)hipGraphExecDestroy(p->exec);
+  if (p->graph) (void)hipGraphDestroy(p->graph);
+  p->exec = nullptr;
+  p->graph = nullptr;
+}
+
+// Lays out both sides for cfg, then captures their launches into a graph.
+hg_status plan_capture(hg_batch_plan p, const BatchCfg &cfg) {
+  plan_drop_graph(p);
+  HG_TRY(batch_prepare(p->cb, &p->bd, nullptr, cfg, p->sb));
+  if (p->d_probe) HG_TRY(batch_prepare(p->cp, &p->bd, p->probe_off.data(), cfg, p->sp));
+  HG_HIP(hipStreamSynchronize(p->stream));  // uploads done before capture
+  HG_HIP(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
+  hg_status st = batch_launch_build(p->cb, &p->bd, p->sb, p->d_keys, p->d_offsets, p->d_csr);
+  if (st == HG_OK && p->d_probe)
+    st = batch_launch_probe(p->cp, &p->bd, p->sp, p->d_probe, p->d_offsets, p->d_csr, p->d_counts);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(p->stream, &g);
+  if (st != HG_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return st;
+  }
+  if (e != hipSuccess) return fail(HG_ERR_DEVICE, std::string("graph capture: ") + hipGetErrorString(e));
+  p->graph = g;
+  HG_HIP(hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0));
+  p->cfg = cfg;
+  return HG_OK;
+}
+
+hg_status plan_replay(hg_batch_plan p) {
+  HG_HIP(hipGraphLaunch(p->exec, p->stream));
+  return HG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+hg_status hg_batch_build(hg_ctx c, const hg_batch_desc *bd, const void *d_keys, uint32_t *d_offsets, void *d_csr_keys) {
+  if (!c) return fail(HG_ERR_INVALID_ARG, "null ctx");
+  HG_TRY(batch_validate(bd));
+  if (!d_offsets || (bd->h_key_off[bd->n_tables] > bd->h_key_off[0] && (!d_keys || !d_csr_keys)))
+    return fail(HG_ERR_INVALID_ARG, "null buffer");
+  DeviceGuard g(c->device);
+  BatchSide S;
+  HG_TRY(batch_prepare(c, bd, nullptr, BatchCfg(), S));
+  return batch_launch_build(c, bd, S, d_keys, d_offsets, d_csr_keys);
+}
+
+hg_status hg_batch_probe(hg_ctx c, const hg_batch_desc *bd, const uint32_t *d_offsets, const void *d_csr_keys,
+                         const void *d_probe, const uint64_t *h_probe_off, uint64_t *d_counts, uint64_t *h_total) {
+  if (!c) return fail(HG_ERR_INVALID_ARG, "null ctx");
+  HG_TRY(batch_validate(bd));
+  if (!h_probe_off || !d_counts || !d_offsets) return fail(HG_ERR_INVALID_ARG, "null argument");
+  for (uint32_t t = 0; t < bd->n_tables; t++)
+    if (h_probe_off[t + 1] < h_probe_off[t]) return fail(HG_ERR_INVALID_ARG, "probe offsets must not decrease");
+  DeviceGuard g(c->device);
+  BatchSide S;
+  HG_TRY(batch_prepare(c, bd, h_probe_off, BatchCfg(), S));
+  HG_TRY(batch_launch_probe(c, bd, S, d_probe, d_offsets, d_csr_keys, d_counts));
+  unsigned int flag = 0;
+  HG_TRY(sync_read(c, &flag, S.flag, 4));
+  if (flag) return fail(HG_ERR_UNSUPPORTED, "a bin of a batched table holds more keys than the probe walks");
+  if (h_total) {
+    std::vector<uint64_t> cnt(bd->n_tables);
+    for (uint32_t t0 = 0; t0 < bd->n_tables; t0 += 65536) {
+      const uint32_t k = std::min<uint32_t>(65536, bd->n_tables - t0);
+      HG_TRY(sync_read(c, cnt.data() + t0, d_counts + t0, (size_t)k * 8));
+    }
+    uint64_t s = 0;
+    for (uint64_t v : cnt) s += v;
+    *h_total = s;
+  }
+  return HG_OK;
+}
+
+hg_status hg_batch_plan_create(hg_ctx c, const hg_batch_desc *bd, const void *d_keys, uint32_t *d_offsets,
+                               void *d_csr_keys, const void *d_probe, const uint64_t *h_probe_off, uint64_t *d_counts,
+                               uint32_t autotune, hg_batch_plan *out) {
+  if (!c || !out) return fail(HG_ERR_INVALID_ARG, "null argument");
+  HG_TRY(batch_validate(bd));
+  if (!d_offsets || (d_probe && (!h_probe_off || !d_counts))) return fail(HG_ERR_INVALID_ARG, "null buffer");
+  DeviceGuard g(c->device);
+  hg_batch_plan p = new hg_batch_plan_s();
+  auto bail = [&](hg_status st) {
+    hg_batch_plan_destroy(p);
+    return st;
+  };
+  p->device = c->device;
+  p->owner = c;
+  const uint32_t T = bd->n_tables;
+  p->key_off.assign(bd->h_key_off, bd->h_key_off + T + 1);
+  p->n_bins.assign(bd->h_n_bins, bd->h_n_bins + T);
+  p->off_off.assign(bd->h_off_off, bd->h_off_off + T);
+  if (bd->h_hash_kind) p->kinds.assign(bd->h_hash_kind, bd->h_hash_kind + T);
+  if (d_probe) p->probe_off.assign(h_probe_off, h_probe_off + T + 1);
+  p->bd = *bd;
+  p->bd.h_key_off = p->key_off.data();
+  p->bd.h_n_bins = p->n_bins.data();
+  p->bd.h_off_off = p->off_off.data();
+  p->bd.h_hash_kind = bd->h_hash_kind ? p->kinds.data() : nullptr;
+  p->d_keys = d_keys;
+  p->d_offsets = d_offsets;
+  p->d_csr = d_csr_keys;
+  p->d_probe = d_probe;
+  p->d_counts = d_counts;
+  if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&p->ev_in, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&p->ev_out, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreate(&p->t0) != hipSuccess || hipEventCreate(&p->t1) != hipSuccess)
+    return bail(fail(HG_ERR_DEVICE, "plan stream/event creation failed"));
+  hg_status st = hg_ctx_create(c->device, &p->cb);
+  if (st == HG_OK) st = hg_ctx_create(c->device, &p->cp);
+  if (st != HG_OK) return bail(st);
+  p->cb->stream = p->stream;
+  p->cp->stream = p->stream;
+  // inputs are produced on the caller's stream
+  if (hipEventRecord(p->ev_in, c->stream) != hipSuccess || hipStreamWaitEvent(p->stream, p->ev_in, 0) != hipSuccess)
+    return bail(fail(HG_ERR_DEVICE, "stream ordering failed"));
+  std::vector<BatchCfg> cands;
+  if (autotune) {
+    for (double f : {0.875, 0.5})
+      for (uint32_t tk : {kTile, kTile / 2, kTile / 4}) {
+        BatchCfg x;
+        x.fill = f;
+        x.tile_keys = tk;
+        cands.push_back(x);
+      }
+  } else {
+    cands.push_back(BatchCfg());
+  }
+  BatchCfg best = cands[0];
+  float best_ms = -1.0f;
+  if (cands.size() > 1) {
+    for (const BatchCfg &x : cands) {
+      st = plan_capture(p, x);
+      if (st != HG_OK) return bail(st);
+      float ms = 0.0f;
+      if (plan_replay(p) != HG_OK || hipEventRecord(p->t0, p->stream) != hipSuccess) return bail(HG_ERR_DEVICE);
+      for (int r = 0; r < 3; r++)
+        if (plan_replay(p) != HG_OK) return bail(HG_ERR_DEVICE);
+      if (hipEventRecord(p->t1, p->stream) != hipSuccess || hipEventSynchronize(p->t1) != hipSuccess ||
+          hipEventElapsedTime(&ms, p->t0, p->t1) != hipSuccess)
+        return bail(fail(HG_ERR_DEVICE, "timing a candidate failed"));
+      ms /= 3.0f;
+      if (best_ms < 0.0f || ms < best_ms) {
+        best_ms = ms;
+        best = x;
+      }
+    }
+  }
+  st = plan_capture(p, best);
+  if (st != HG_OK) return bail(st);
+  p->info.tile_keys = best.tile_keys;
+  p->info.fill = (float)best.fill;
+  p->info.candidates = (uint32_t)cands.size();
+  p->info.tuned_ms = best_ms;
+  *out = p;
+  return HG_OK;
+}
+
+hg_status hg_batch_plan_run(hg_batch_plan p) {
+  if (!p) return fail(HG_ERR_INVALID_ARG, "null plan");
+  DeviceGuard g(p->device);
+  HG_HIP(hipEventRecord(p->ev_in, p->owner->stream));
+  HG_HIP(hipStreamWaitEvent(p->stream, p->ev_in, 0));
+  HG_TRY(plan_replay(p));
+  HG_HIP(hipEventRecord(p->ev_out, p->stream));
+  HG_HIP(hipStreamWaitEvent(p->owner->stream, p->ev_out, 0));
+  return HG_OK;
+}
+
+hg_status hg_batch_plan_query(hg_batch_plan p, hg_batch_plan_info *info) {
+  if (!p || !info) return fail(HG_ERR_INVALID_ARG, "null argument");
+  *info = p->info;
+  return HG_OK;
+}
+
+hg_status hg_batch_plan_destroy(hg_batch_plan p) {
+  if (!p) return HG_OK;
+  DeviceGuard g(p->device);
+  if (p->stream) (void)hipStreamSynchronize(p->stream);
+  plan_drop_graph(p);
+  if (p->cb) {
+    p->cb->stream = nullptr;
+    hg_ctx_destroy(p->cb);
+  }
+  if (p->cp) {
+    p->cp->stream = nullptr;
+    hg_ctx_destroy(p->cp);
+  }
+  for (hipEvent_t e : {p->ev_in, p->ev_out, p->t0, p->t1})
+    if (e) (void)hipEventDestroy(e);
+  if (p->stream) (void)hipStreamDestroy(p->stream);
+  delete p;
+  return HG_OK;
 }
 
 }  // extern "C"

@@ -461,7 +461,7 @@ __global__ __launch_bounds__(BLOCK) void k_probe(const K *__restrict__ bpart, co
   for (int j = 0; j < (KPT + 1) / 2; j++) mine += (mc[j] & 0xFFFFu) + (mc[j] >> 16);
   uint32_t tot;
   const uint32_t ex = block_exclusive_sum<BLOCK, uint32_t>(mine, s_tmp, tot);
-  if (threadIdx.x == 0) s_base = tot ? atomicAdd(cursor, (unsigned long long)tot) : 0ull;
+  if (threadIdx.x == 0) s_base = tot ? atomicAdd(cursor + it.table, (unsigned long long)tot) : 0ull;
   __syncthreads();
   if (out == nullptr || !mine) return;
   unsigned long long w = s_base + ex;
@@ -949,12 +949,12 @@ __global__ __launch_bounds__(BLOCK) void k_probe_big(const K *__restrict__ bpart
     }
     if (out == nullptr) {
       const unsigned long long t = block_sum<BLOCK, unsigned long long>(mine, s_t64);
-      if (threadIdx.x == 0 && t) atomicAdd(cursor, t);
+      if (threadIdx.x == 0 && t) atomicAdd(cursor + it.table, t);
       continue;
     }
     unsigned long long tot;
     const unsigned long long ex = block_exclusive_sum<BLOCK, unsigned long long>(mine, s_t64, tot);
-    if (threadIdx.x == 0) s_base = tot ? atomicAdd(cursor, tot) : 0ull;
+    if (threadIdx.x == 0) s_base = tot ? atomicAdd(cursor + it.table, tot) : 0ull;
     __syncthreads();
     unsigned long long w = s_base + ex;
     for (uint64_t q = s; q < e && mine; q++)

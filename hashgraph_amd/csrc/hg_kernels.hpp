@@ -63,7 +63,7 @@ struct LaunchProbe {
   const void *keysA;
   void *out;
   uint64_t capacity;
-  unsigned long long *cursor;
+  unsigned long long *cursor;  // match counters, one per table (FinItem.table)
   unsigned int *flag;
 };
 

@@ -329,3 +329,40 @@ def mixed_probe(A, fresh, seed: int, frac: float):
     take = (r & 0xFFFF) < thr
     idx = (_t_shr(r, 32) * A.numel()) >> 32
     return torch.where(take, A[idx], fresh)
+
+
+# ---------------------------------------------------------------------------
+# Batched tables: size of table t = round(2^u), u uniform on [lo_bits, hi_bits]
+# from element t of stream `seed` (53-bit double); even tables hold uniform
+# uint32 keys (stream seed*4096 + 2t+1), odd tables Zipf(s) ranks over 2^32
+# (stream seed*4096 + 2t+2).  Probe set t has the same size, `frac` of it
+# drawn from table t (probe_keys rule, stream seed*4096 + 2t+1 + 8192*4096).
+# ---------------------------------------------------------------------------
+def batch_sizes(seed: int, T: int, lo_bits: float = 8.0, hi_bits: float = 20.0):
+    r = stream_np(seed, T)
+    u = (r >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+    return np.rint(np.exp2(lo_bits + u * (hi_bits - lo_bits))).astype(np.int64)
+
+
+def batch_tables(seed: int, T: int, lo_bits: float = 8.0, hi_bits: float = 20.0, zipf_s: float = 1.2,
+                 frac: float = 0.3, device=None):
+    """Keys and probe sets of a batch: (keys, key_off, probe, probe_off); numpy
+    uint32 arrays, or int32 tensors on `device` (same values)."""
+    sizes = batch_sizes(seed, T, lo_bits, hi_bits)
+    keys, probes = [], []
+    for t, n in enumerate(sizes.tolist()):
+        base = seed * 4096 + 2 * t + 1
+        if t % 2 == 0:
+            a = uniform_keys(base, n, 4, device)
+        elif device is None:
+            a = (zipf_keys(base + 1, n, zipf_s) & np.uint64(0xFFFFFFFF)).astype(np.uint32)
+        else:
+            z = zipf_keys(base + 1, n, zipf_s, device=device)
+            a = (z - ((z >> 31) << 32)).to(__import__("torch").int32)  # low 32 bits
+        keys.append(a)
+        probes.append(probe_keys(a, base + 8192 * 4096, n, frac, 4, device) if n else a)
+    ko = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
+    if device is None:
+        return np.concatenate(keys), ko, np.concatenate(probes), ko.copy()
+    import torch
+    return torch.cat(keys), ko, torch.cat(probes), ko.copy()

@@ -206,6 +206,30 @@ hg_status hg_batch_build(hg_ctx ctx, const hg_batch_desc *bd, const void *d_keys
 hg_status hg_batch_probe(hg_ctx ctx, const hg_batch_desc *bd, const uint32_t *d_offsets, const void *d_csr_keys,
                          const void *d_probe, const uint64_t *h_probe_off, uint64_t *d_counts, uint64_t *h_total);
 
+/* Plans: the batch's descriptors stay resident on the device and its launches
+ * (build, then the probe counts when d_probe is given) are captured once into
+ * a hipGraph; hg_batch_plan_run replays it, ordered after the work already
+ * queued on the context's stream and before work queued later.  With
+ * autotune = 1 the plan times candidate tile sizes (keys per partition
+ * workgroup) and bucket fills (bins per final-stage workgroup) by replaying
+ * each candidate's graph, and keeps the fastest.  Buffers passed here must
+ * stay valid for the plan's life.  A plan has its own stream and scratch;
+ * destroy it before its context. */
+typedef struct hg_batch_plan_s *hg_batch_plan;
+typedef struct {
+  uint32_t tile_keys;   /* keys per partition tile */
+  float fill;           /* bucket fill target of the final stage */
+  uint32_t candidates;  /* configurations timed */
+  float tuned_ms;       /* replay time of the chosen one (-1 without tuning) */
+} hg_batch_plan_info;
+
+hg_status hg_batch_plan_create(hg_ctx ctx, const hg_batch_desc *bd, const void *d_keys, uint32_t *d_offsets,
+                               void *d_csr_keys, const void *d_probe, const uint64_t *h_probe_off, uint64_t *d_counts,
+                               uint32_t autotune, hg_batch_plan *out);
+hg_status hg_batch_plan_run(hg_batch_plan plan);
+hg_status hg_batch_plan_query(hg_batch_plan plan, hg_batch_plan_info *info);
+hg_status hg_batch_plan_destroy(hg_batch_plan plan);
+
 /* ---- primitives ----------------------------------------------------------
  * Exclusive prefix sum (single-pass decoupled look-back), replaces
  * cub::DeviceScan::ExclusiveSum at src/SingleHashGraph.cu:147-152,166-172 and

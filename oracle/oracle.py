@@ -267,7 +267,7 @@ def build_multi(keys: np.ndarray, M: int, splits: np.ndarray):
     keys = np.ascontiguousarray(keys)
     splits = np.ascontiguousarray(splits, dtype=np.int64)
     P = len(splits) - 1
-    off = np.zeros(int(splits[P]) + P + 1, dtype=np.uint64)
+    off = np.zeros(int(splits[P]) + P, dtype=np.uint64)
     out = np.empty_like(keys)
     rc = lib().hgo_build_multi(_ptr(keys), len(keys), _kb(keys), M, _ptr(splits), P, _ptr(off), _ptr(out))
     if rc:

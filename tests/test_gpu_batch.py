@@ -1,0 +1,135 @@
+"""Batched tables on the GPU: every table of a batch (eager, and replayed
+from a captured plan) equals the oracle's single-table CSR, and the per-table
+probe counts equal the oracle's bag joins.  The batch has no counterpart in
+the reference; each table is pinned by the same oracle as a single build."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+from hashgraph_amd import workloads as wl
+from oracle import oracle as o
+
+
+def _to_dev(a, dev):
+    import torch
+    return torch.from_numpy(a.view(np.int32 if a.dtype == np.uint32 else np.int64).copy()).to(dev)
+
+
+def _batch(seed, T, lo, hi):
+    keys, ko, probe, po = wl.batch_tables(seed, T, lo, hi)
+    # a few edge tables: empty, single key, one bin
+    return keys, ko, probe, po
+
+
+def _check_tables(keys, ko, n_bins, off_off, off, csr, bins_fn=None):
+    offn = off.cpu().numpy().view(np.uint32).astype(np.uint64)
+    csrn = csr.cpu().numpy().view(np.uint32)
+    for t in range(len(n_bins)):
+        a = keys[ko[t]:ko[t + 1]]
+        M = int(n_bins[t])
+        if bins_fn is None:
+            eoff, ekeys = o.build_csr(a, M)
+        else:
+            eoff, ekeys = o.build_csr_bins(a, bins_fn(t, a, M), M)
+        goff = offn[off_off[t]: off_off[t] + M + 1]
+        assert np.array_equal(goff, eoff), f"table {t} offsets"
+        g = csrn[ko[t]:ko[t + 1]].copy()
+        o.canon(eoff, g)
+        o.canon(eoff, ekeys)
+        assert np.array_equal(g, ekeys), f"table {t} keys"
+
+
+def _expected_counts(keys, ko, probe, po):
+    return [o.bag_intersect_count(keys[ko[t]:ko[t + 1]], probe[po[t]:po[t + 1]]) if ko[t + 1] > ko[t] and
+            po[t + 1] > po[t] else 0 for t in range(len(ko) - 1)]
+
+
+@pytest.mark.parametrize("T,lo,hi", [(64, 4, 14), (300, 0, 10)])
+def test_batch_build_and_probe(dev, T, lo, hi):
+    import hashgraph_amd as hg
+    keys, ko, probe, po = _batch(11, T, lo, hi)
+    n_bins = np.maximum(np.diff(ko), 1)
+    print(f"batch T={T} keys={len(keys)}", flush=True)
+    dk, dp = _to_dev(keys, dev), _to_dev(probe, dev)
+    off, csr, off_off = hg.batch_build(dk, ko, n_bins)
+    _check_tables(keys, ko, n_bins, off_off, off, csr)
+    total, counts = hg.batch_probe(off, csr, ko, n_bins, off_off, dp, po)
+    exp = _expected_counts(keys, ko, probe, po)
+    assert counts.cpu().numpy().tolist() == exp
+    assert total == sum(exp)
+
+
+def test_batch_with_empty_and_tiny_tables(dev):
+    import hashgraph_amd as hg
+    rng = np.random.default_rng(2)
+    sizes = [0, 1, 2, 0, 5000, 1, 70000, 0]
+    parts = [wl.uniform_keys(100 + i, s, 4) for i, s in enumerate(sizes)]
+    keys = np.concatenate(parts)
+    ko = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
+    n_bins = np.array([1, 1, 7, 3, 4096, 1, 65536, 2])
+    probe = keys[rng.integers(0, len(keys), len(keys))]
+    po = ko.copy()
+    dk, dp = _to_dev(keys, dev), _to_dev(probe, dev)
+    off, csr, off_off = hg.batch_build(dk, ko, n_bins)
+    _check_tables(keys, ko, n_bins, off_off, off, csr)
+    total, counts = hg.batch_probe(off, csr, ko, n_bins, off_off, dp, po)
+    assert counts.cpu().numpy().tolist() == _expected_counts(keys, ko, probe, po)
+
+
+def test_plan_replays_match_eager(dev):
+    """Captured plan (autotuned): several replays, with eager library work in
+    between, each equal to the oracle."""
+    import torch
+    import hashgraph_amd as hg
+    keys, ko, probe, po = _batch(12, 200, 6, 14)
+    n_bins = np.maximum(np.diff(ko), 1)
+    dk, dp = _to_dev(keys, dev), _to_dev(probe, dev)
+    exp = _expected_counts(keys, ko, probe, po)
+    plan = hg.BatchPlan(dk, ko, n_bins, probe=dp, probe_off=po, autotune=True)
+    info = plan.info()
+    assert info["candidates"] == 6 and info["tile_keys"] in (8192, 16384, 32768)
+    for r in range(3):
+        plan.run()
+        torch.cuda.synchronize()
+        _check_tables(keys, ko, n_bins, plan.off_off, plan.offsets, plan.csr)
+        assert plan.counts.cpu().numpy().tolist() == exp, f"replay {r}"
+        # unrelated eager work between replays
+        hg.build(dk[:1000], 512)
+        plan.offsets.fill_(-1)
+        plan.counts.fill_(-7)
+        torch.cuda.synchronize()
+    plan.close()
+
+
+def test_batch_learned_tables(dev):
+    """Tables with the learned hash (trained per table) beside murmur tables:
+    CSRs equal the oracle's from the same bins, counts equal murmur's."""
+    import torch
+    import hashgraph_amd as hg
+    keys, ko, probe, po = _batch(13, 32, 8, 14)
+    n_bins = np.maximum(np.diff(ko), 1)
+    T = len(n_bins)
+    dk, dp = _to_dev(keys, dev), _to_dev(probe, dev)
+    L = 256
+    mb = hg.rmi_model_bytes(L)
+    models = torch.zeros(T * mb, dtype=torch.uint8, device=dev)
+    kinds = []
+    for t in range(T):
+        n = int(ko[t + 1] - ko[t])
+        m, use = hg.rmi_train(dk[int(ko[t]):int(ko[t + 1])], int(n_bins[t]), n_sample=max(n // 100, min(n, 1024)),
+                              seed=t, n_leaves=L)
+        models[t * mb:(t + 1) * mb] = m
+        kinds.append(hg.HG_HASH_RMI if (use or t % 3 == 0) else hg.HG_HASH_MURMUR3)
+    mh = models.cpu().numpy()
+
+    def bins_fn(t, a, M):
+        if kinds[t] == hg.HG_HASH_RMI:
+            return o.bins_rmi(a, mh[t * mb:(t + 1) * mb], M)
+        return o.bins(a, M)
+
+    off, csr, off_off = hg.batch_build(dk, ko, n_bins, hash_kinds=kinds, rmi_models=models, rmi_leaves=L)
+    _check_tables(keys, ko, n_bins, off_off, off, csr, bins_fn)
+    total, counts = hg.batch_probe(off, csr, ko, n_bins, off_off, dp, po, hash_kinds=kinds, rmi_models=models,
+                                   rmi_leaves=L)
+    assert counts.cpu().numpy().tolist() == _expected_counts(keys, ko, probe, po)
