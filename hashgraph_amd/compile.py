@@ -22,8 +22,10 @@ LIB = os.path.join(LIBDIR, "libhashgraph_amd.so")
 
 ARCH = os.environ.get("HG_OFFLOAD_ARCH", "gfx950")
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
-# -packed-fp32-ops: no packed FP32 instructions are generated
-DEVFLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+# -packed-fp32-ops: no packed FP32 instructions are generated; -ffp-contract=off:
+# float products and sums stay separately rounded (the learned model's
+# evaluation must equal its CPU restatement bit for bit)
+DEVFLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-ffp-contract=off"]
 CFLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", f"--offload-arch={ARCH}",
           "-I", os.path.join(ROOT, "include"), "-I", CSRC]
 

@@ -89,28 +89,30 @@ struct FinItem {
 
 // Learned CDF model (no counterpart in the reference).  Two FP32 linear
 // stages over integer anchors:
-//   leaf = clamp(floor(root_slope * f(key - key_min) + root_icpt), 0, L-1)
-//   bin  = min(base[leaf] + floor(slope[leaf] * f(key - x0[leaf])),
-//              base[leaf+1], M-1)
+//   feature = f(key - key_min)                     (root_kind 0), or
+//             the bits of f(key - key_min) + 1 read as an unsigned integer
+//             and converted to f32 (root_kind 1: a piecewise-linear log)
+//   leaf    = clamp(floor(root_slope * feature + root_icpt), 0, L-1)
+//   bin     = clamp(anchor[leaf] +- floor(slope[leaf] * f(|key - x0[leaf]|)),
+//                   0, M-1)        (+ for key >= x0, - below it)
 // f() is the round-to-nearest u64 -> f32 conversion, every product and sum is
-// rounded separately (no fused multiply-add), differences below an anchor
-// count as 0, and floor saturates at 2^32-256.  The anchors keep FP32
-// precision where it is needed: the float only holds a distance from the
-// smallest key or from the first training key of one leaf.  The header is
-// followed by L+1 leaves; leaf L is a sentinel whose base is M-1.  Bases are
-// non-decreasing, so the map is monotone in the key.
+// rounded separately (no fused multiply-add), floor saturates at 2^32-256.
+// x0 (the leaf's first training key) keeps FP32 precision where it is
+// needed: the float only holds a distance from the smallest key or from x0.
+// The header is followed by L leaves.
 struct RmiHeader {
   uint64_t key_min;
   float root_slope;
   float root_icpt;
   uint32_t n_leaves;
   uint32_t n_bins;
-  uint32_t pad[2];
+  uint32_t root_kind;
+  uint32_t pad;
 };
 struct RmiLeaf {
-  uint64_t x0;     // first training key routed to the leaf
-  float slope;     // bins per key unit
-  uint32_t base;   // bin of x0
+  uint64_t x0;      // first training key routed to the leaf
+  float slope;      // bins per key unit
+  uint32_t anchor;  // bin of x0
 };
 static_assert(sizeof(RmiHeader) == 32 && sizeof(RmiLeaf) == 16, "RMI layout");
 

@@ -53,26 +53,37 @@ __device__ __forceinline__ uint32_t fastdiv(uint32_t a, const Div32 &dv) {
   return dv.w == 1 ? a : (uint32_t)__umul64hi(dv.d64, (uint64_t)a);
 }
 
-// Learned CDF model (layout and rounding rules in hg_common.hpp).
-__device__ __forceinline__ uint32_t rmi_leaf(uint64_t key, const RmiHeader &h) {
-  const float xr = key > h.key_min ? (float)(key - h.key_min) : 0.0f;
-  const float t = __fadd_rn(__fmul_rn(h.root_slope, xr), h.root_icpt);
-  const uint32_t L = h.n_leaves;
+// Learned CDF model (layout and rounding rules in hg_common.hpp).  The
+// products and sums must not fuse into FMAs (HIP contracts by default).
+__device__ __forceinline__ float rmi_feature(uint64_t key, uint64_t key_min, uint32_t kind) {
+#pragma clang fp contract(off)
+  const float xr = key > key_min ? (float)(key - key_min) : 0.0f;
+  if (kind == 0) return xr;
+  return (float)__float_as_uint(__fadd_rn(xr, 1.0f));
+}
+
+__device__ __forceinline__ uint32_t rmi_leaf_at(float feature, float slope, float icpt, uint32_t L) {
+#pragma clang fp contract(off)
+  const float t = __fadd_rn(__fmul_rn(slope, feature), icpt);
   return !(t > 0.0f) ? 0u : (t >= (float)(L - 1) ? L - 1 : (uint32_t)t);
 }
 
+__device__ __forceinline__ uint32_t rmi_leaf(uint64_t key, const RmiHeader &h) {
+  return rmi_leaf_at(rmi_feature(key, h.key_min, h.root_kind), h.root_slope, h.root_icpt, h.n_leaves);
+}
+
 __device__ __forceinline__ uint32_t rmi_bin(uint64_t key, const void *model, uint32_t M) {
+#pragma clang fp contract(off)
   const RmiHeader *h = (const RmiHeader *)model;
   const RmiLeaf *lv = (const RmiLeaf *)(h + 1);
-  const uint32_t leaf = rmi_leaf(key, *h);
-  const RmiLeaf l = lv[leaf];
-  const uint32_t hi = lv[leaf + 1].base;
-  const float d = key > l.x0 ? (float)(key - l.x0) : 0.0f;
+  const RmiLeaf l = lv[rmi_leaf(key, *h)];
+  const bool up = key >= l.x0;
+  const float d = up ? (float)(key - l.x0) : (float)(l.x0 - key);
   const float y = __fmul_rn(l.slope, d);
-  const uint32_t off = !(y > 0.0f) ? 0u : (y >= 4294967040.0f ? 0xFFFFFF00u : (uint32_t)y);
-  uint32_t b = l.base + off;
-  if (b < l.base || b > hi) b = hi;
-  return b < M ? b : M - 1;
+  const int64_t off = !(y > 0.0f) ? 0 : (y >= 4294967040.0f ? 0xFFFFFF00ll : (int64_t)(uint32_t)y);
+  int64_t b = (int64_t)l.anchor + (up ? off : -off);
+  b = b < 0 ? 0 : (b > (int64_t)M - 1 ? (int64_t)M - 1 : b);
+  return (uint32_t)b;
 }
 
 // Hash kinds a kernel is compiled for: only the selected path is inlined

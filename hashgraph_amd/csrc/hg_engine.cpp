@@ -724,7 +724,7 @@ hg_status hg_table_bins(hg_ctx c, const hg_table_desc *desc, const void *d_keys,
 
 size_t hg_rmi_model_bytes(uint32_t n_leaves) {
   if (n_leaves == 0 || n_leaves > 65536) return 0;
-  return sizeof(RmiHeader) + ((size_t)n_leaves + 1) * sizeof(RmiLeaf);
+  return sizeof(RmiHeader) + (size_t)n_leaves * sizeof(RmiLeaf);
 }
 
 hg_status hg_rmi_train(hg_ctx c, const void *d_keys, uint64_t n_keys, uint32_t key_bytes, uint64_t n_sample,
@@ -748,7 +748,8 @@ hg_status hg_rmi_train(hg_ctx c, const void *d_keys, uint64_t n_keys, uint32_t k
   const size_t i_b0 = P.add(n_sample * key_bytes), i_b1 = P.add(n_sample * key_bytes);
   const size_t i_cnt = P.add(((size_t)256 * ntiles + 1) * 4);
   const size_t i_st = P.add(scan_tiles((uint64_t)256 * ntiles) * 8), i_tk = P.add(64);
-  const size_t i_part = P.add((size_t)nblk * 4 * sizeof(double));
+  const size_t i_part = P.add((size_t)nblk * 8 * sizeof(double));
+  const size_t i_cand = P.add(64), i_lcnt = P.add((size_t)4 * n_leaves * 4);
   const size_t i_start = P.add(((size_t)n_leaves + 1) * 4);
   const size_t i_cr = P.add(Mp * 4), i_cm = P.add(Mp * 4), i_out = P.add(64);
   HG_TRY(arena_ensure(c, 1, P.total()));
@@ -766,6 +767,8 @@ hg_status hg_rmi_train(hg_ctx c, const void *d_keys, uint64_t n_keys, uint32_t k
   r.scan_status = cv.get<unsigned long long>(i_st);
   r.scan_ticket = cv.get<unsigned int>(i_tk);
   r.partials = cv.get<double>(i_part);
+  r.cand = cv.get<void>(i_cand);
+  r.leafcnt = cv.get<uint32_t>(i_lcnt);
   r.nblk = nblk;
   r.n_leaves = n_leaves;
   r.map = make_map(n_bins, 0, nullptr);
@@ -846,6 +849,7 @@ hg_status hg_probe(hg_ctx c, const hg_table_desc *descA, const void *d_offA, con
   Plan P0;
   const size_t i_part = P0.add(n_probe * kb);
   const size_t i_small = P0.add(256);
+  const size_t i_mark = P0.add((size_t)L.ncols_total * 4);
   HG_TRY(arena_ensure(c, 0, P0.total()));
   Carve cv0;
   cv0.init(c, 0, P0);
@@ -853,12 +857,15 @@ hg_status hg_probe(hg_ctx c, const hg_table_desc *descA, const void *d_offA, con
   HG_TRY(setup_level(c, L, d));
   void *part = cv0.get<void>(i_part);
   unsigned long long *small = cv0.get<unsigned long long>(i_small);
+  unsigned int *hmark = cv0.get<unsigned int>(i_mark);
   HG_HIP(launch_fill_u64(small, 0ull, 4, c->stream));
+  HG_HIP(launch_fill_u32(hmark, 0u, L.ncols_total, c->stream));
   HG_TRY(run_level(c, L, d, kb, d_probe, part));
   HG_HIP(launch_make_items(d.segs, (uint32_t)L.segs.size(), d.bstart, d.coltot, d.fin, L.ncols_total, c->stream));
   LaunchProbe pr{};
   pr.key_bytes = kb;
   pr.off_bytes = ob;
+  pr.hk = L.hash_kind();
   pr.nitems = L.ncols_total;
   pr.akey_cap = probe_akey_cap(kb, ob);
   pr.bpart = part;
@@ -870,6 +877,7 @@ hg_status hg_probe(hg_ctx c, const hg_table_desc *descA, const void *d_offA, con
   pr.capacity = d_out ? capacity : 0;
   pr.cursor = small;
   pr.flag = (unsigned int *)(small + 1);
+  pr.hmark = hmark;
   {
     Prof pf(c, "probe");
     HG_HIP(launch_probe(pr, c->stream));
@@ -878,7 +886,7 @@ hg_status hg_probe(hg_ctx c, const hg_table_desc *descA, const void *d_offA, con
   HG_TRY(sync_read(c, res, small, 16));
   *h_count = res[0];
   if (res[1] & 0xFFFFFFFFull)
-    return fail(HG_ERR_UNSUPPORTED, "a bin of A holds more keys than the probe walks; use hg_intersect_rle");
+    return fail(HG_ERR_UNSUPPORTED, "a heavy bin of A holds too many distinct values; use hg_intersect_rle");
   if (d_out && res[0] > capacity) return fail(HG_ERR_CAPACITY, "output capacity smaller than the match count");
   return HG_OK;
 }
@@ -1168,6 +1176,7 @@ struct BatchSide {
   LevelDev d;
   void *part = nullptr;
   unsigned int *flag = nullptr;
+  unsigned int *hmark = nullptr;
   uint64_t nkeys = 0;
 };
 
@@ -1198,11 +1207,13 @@ hg_status batch_prepare(hg_ctx c, const hg_batch_desc *bd, const uint64_t *probe
   Plan P0;
   const size_t i_part = P0.add(std::max<uint64_t>(src[bd->n_tables], 1) * kb);
   const size_t i_flag = P0.add(64);
+  const size_t i_mark = P0.add((size_t)std::max<uint32_t>(S.L.ncols_total, 1) * 4);
   HG_TRY(arena_ensure(c, 0, P0.total()));
   Carve cv;
   cv.init(c, 0, P0);
   S.part = cv.get<void>(i_part);
   S.flag = cv.get<unsigned int>(i_flag);
+  S.hmark = cv.get<unsigned int>(i_mark);
   return setup_level(c, S.L, S.d);
 }
 
@@ -1217,6 +1228,7 @@ hg_status batch_launch_probe(hg_ctx c, const hg_batch_desc *bd, BatchSide &S, co
                              const uint32_t *d_offsets, const void *d_csr, uint64_t *d_counts) {
   HG_HIP(launch_fill_u64((unsigned long long *)d_counts, 0ull, bd->n_tables, c->stream));
   HG_HIP(launch_fill_u32(S.flag, 0u, 1, c->stream));
+  HG_HIP(launch_fill_u32(S.hmark, 0u, S.L.ncols_total, c->stream));
   HG_TRY(run_level(c, S.L, S.d, (int)bd->key_bytes, d_probe, S.part));
   HG_HIP(launch_make_items(S.d.segs, (uint32_t)S.L.segs.size(), S.d.bstart, S.d.coltot, S.d.fin, S.L.ncols_total,
                            c->stream));
@@ -1235,6 +1247,7 @@ hg_status batch_launch_probe(hg_ctx c, const hg_batch_desc *bd, BatchSide &S, co
   pr.capacity = 0;
   pr.cursor = (unsigned long long *)d_counts;
   pr.flag = S.flag;
+  pr.hmark = S.hmark;
   Prof pf(c, "probe");
   HG_HIP(launch_probe(pr, c->stream));
   return HG_OK;
@@ -1327,7 +1340,7 @@ hg_status hg_batch_probe(hg_ctx c, const hg_batch_desc *bd, const uint32_t *d_of
   HG_TRY(batch_launch_probe(c, bd, S, d_probe, d_offsets, d_csr_keys, d_counts));
   unsigned int flag = 0;
   HG_TRY(sync_read(c, &flag, S.flag, 4));
-  if (flag) return fail(HG_ERR_UNSUPPORTED, "a bin of a batched table holds more keys than the probe walks");
+  if (flag) return fail(HG_ERR_UNSUPPORTED, "a heavy bin of a batched table holds too many distinct values");
   if (h_total) {
     std::vector<uint64_t> cnt(bd->n_tables);
     for (uint32_t t0 = 0; t0 < bd->n_tables; t0 += 65536) {

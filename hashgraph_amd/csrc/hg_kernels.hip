@@ -383,15 +383,15 @@ __global__ __launch_bounds__(BLOCK) void k_final_big(const K *__restrict__ part,
 // count.  A match is always the probe key itself, so the emission needs no
 // second walk: after the last sub-range one block scan and one atomic
 // reservation per workgroup place each key's copies (output grouped by
-// bucket).  A bin longer than PROBE_BIN_LIMIT is not walked; it raises
-// *flag and the host reports the probe as unsupported.
+// bucket).  A bin longer than PROBE_BIN_LIMIT is not walked: the item is
+// marked in hmark and k_probe_heavy answers the keys of such bins.
 // ---------------------------------------------------------------------------
 template <typename K, typename OFF, int BLOCK, int KPT, int SUB, int HK>
 __global__ __launch_bounds__(BLOCK) void k_probe(const K *__restrict__ bpart, const FinItem *__restrict__ items,
                                                  const BinMap *__restrict__ maps, const OFF *__restrict__ offA,
                                                  const K *__restrict__ keysA, K *__restrict__ out, uint64_t capacity,
                                                  unsigned long long *__restrict__ cursor, uint32_t akey_cap,
-                                                 unsigned int *__restrict__ flag) {
+                                                 unsigned int *__restrict__ hmark) {
   extern __shared__ __align__(16) uint8_t smem[];
   __shared__ uint32_t s_tmp[BLOCK / 64 + 1];
   __shared__ uint32_t s_n;
@@ -417,7 +417,7 @@ __global__ __launch_bounds__(BLOCK) void k_probe(const K *__restrict__ bpart, co
   }
 #pragma unroll
   for (int j = 0; j < KPT; j++)
-    if (j < nv) lb[j >> 1] |= (bin_of<K, HK>(kr[j], mp) - it.bin_lo) << ((j & 1) * 16);
+    if (j < nv) lb[j >> 1] |= ((bin_of<K, HK>(kr[j], mp) - it.bin_lo) & 0xFFFFu) << ((j & 1) * 16);
   for (uint32_t s0 = 0; s0 < W;) {
     uint32_t n = min((uint32_t)SUB, W - s0);
     for (uint32_t q = threadIdx.x; q <= n; q += BLOCK) s_off[q] = offA[it.off_pos + s0 + q];
@@ -447,7 +447,7 @@ __global__ __launch_bounds__(BLOCK) void k_probe(const K *__restrict__ bpart, co
       if (j < nv && l < n) {
         const uint32_t b0 = (uint32_t)(s_off[l] - a0), e = (uint32_t)(s_off[l + 1] - a0);
         uint32_t m = 0;
-        if (e - b0 > PROBE_BIN_LIMIT) atomicOr(flag, 1u);
+        if (e - b0 > PROBE_BIN_LIMIT) hmark[blockIdx.x] = 1u;  // k_probe_heavy answers this bin
         else
           for (uint32_t q = b0; q < e; q++) m += s_ak[q] == kr[j];
         mc[j >> 1] += m << sh;  // m <= PROBE_BIN_LIMIT: no carry into the other half
@@ -923,7 +923,7 @@ __global__ __launch_bounds__(BLOCK) void k_probe_big(const K *__restrict__ bpart
                                                      const BinMap *__restrict__ maps, const OFF *__restrict__ offA,
                                                      const K *__restrict__ keysA, K *__restrict__ out, uint64_t capacity,
                                                      unsigned long long *__restrict__ cursor, uint32_t small_cap,
-                                                     unsigned int *__restrict__ flag) {
+                                                     unsigned int *__restrict__ hmark) {
   __shared__ unsigned long long s_t64[BLOCK / 64 + 1];
   __shared__ unsigned long long s_base;
   const FinItem it = items[blockIdx.x];
@@ -942,7 +942,7 @@ __global__ __launch_bounds__(BLOCK) void k_probe_big(const K *__restrict__ bpart
       s = (uint64_t)offA[it.off_pos + l];
       e = (uint64_t)offA[it.off_pos + l + 1];
       if (e - s > PROBE_BIN_LIMIT) {
-        atomicOr(flag, 1u);
+        hmark[blockIdx.x] = 1u;  // k_probe_heavy answers this bin
         e = s;
       }
       for (uint64_t q = s; q < e; q++) mine += ka[q] == k;
@@ -963,6 +963,119 @@ __global__ __launch_bounds__(BLOCK) void k_probe_big(const K *__restrict__ bpart
         w++;
       }
     __syncthreads();
+  }
+}
+
+// Bins longer than PROBE_BIN_LIMIT (heavy duplicates) of the items k_probe and
+// k_probe_big marked: one workgroup per marked item finds the bucket's heavy
+// bins; each is folded into an LDS table value -> copies in A (slots claimed
+// by compare-and-swap of the key against the all-ones sentinel, sentinel keys
+// counted apart), then the bucket's probe keys of that bin read their match
+// counts from the table.  A heavy bin with more than HT_SLOTS/2 distinct
+// values sets *flag (the host reports the probe as unsupported).
+template <typename K, typename OFF, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_probe_heavy(const K *__restrict__ bpart, const FinItem *__restrict__ items,
+                                                       const BinMap *__restrict__ maps, const OFF *__restrict__ offA,
+                                                       const K *__restrict__ keysA, K *__restrict__ out,
+                                                       uint64_t capacity, unsigned long long *__restrict__ cursor,
+                                                       const unsigned int *__restrict__ hmark,
+                                                       unsigned int *__restrict__ flag) {
+  constexpr K EMPTY = (K)~(K)0;
+  __shared__ K s_key[HT_SLOTS];
+  __shared__ uint32_t s_cnt[HT_SLOTS];
+  __shared__ uint32_t s_hl[BLOCK];
+  __shared__ uint32_t s_nh, s_fill, s_ecnt;
+  __shared__ unsigned long long s_t64[BLOCK / 64 + 1], s_base;
+  if (!hmark[blockIdx.x]) return;
+  const FinItem it = items[blockIdx.x];
+  const BinMap mp = maps[it.table];
+  const K *kp = bpart + it.key_start;
+  const K *ka = keysA + it.key_rel;
+  for (uint32_t h0 = 0; h0 < it.nbins; h0 += BLOCK) {
+    if (threadIdx.x == 0) s_nh = 0;
+    __syncthreads();
+    const uint32_t l = h0 + threadIdx.x;
+    if (l < it.nbins) {
+      const uint64_t s = (uint64_t)offA[it.off_pos + l], e = (uint64_t)offA[it.off_pos + l + 1];
+      if (e - s > PROBE_BIN_LIMIT) s_hl[atomicAdd(&s_nh, 1u)] = l;
+    }
+    __syncthreads();
+    const uint32_t nh = s_nh;
+    __syncthreads();
+    for (uint32_t hi = 0; hi < nh; hi++) {
+      const uint32_t hb = s_hl[hi];
+      for (uint32_t i = threadIdx.x; i < HT_SLOTS; i += BLOCK) {
+        s_key[i] = EMPTY;
+        s_cnt[i] = 0;
+      }
+      if (threadIdx.x == 0) {
+        s_fill = 0;
+        s_ecnt = 0;
+      }
+      __syncthreads();
+      const uint64_t a0 = (uint64_t)offA[it.off_pos + hb], a1 = (uint64_t)offA[it.off_pos + hb + 1];
+      for (uint64_t q = a0 + threadIdx.x; q < a1; q += BLOCK) {
+        const K v = ka[q];
+        if (v == EMPTY) {
+          atomicAdd(&s_ecnt, 1u);
+          continue;
+        }
+        uint32_t slot = murmur4((uint32_t)v ^ (uint32_t)((uint64_t)v >> 32)) & (HT_SLOTS - 1);
+        for (uint32_t pr = 0; pr < HT_SLOTS; pr++, slot = (slot + 1) & (HT_SLOTS - 1)) {
+          const K old = lds_cas(&s_key[slot], EMPTY, v);
+          if (old == EMPTY || old == v) {
+            atomicAdd(&s_cnt[slot], 1u);
+            if (old == EMPTY) atomicAdd(&s_fill, 1u);
+            break;
+          }
+        }
+      }
+      __syncthreads();
+      const bool ovf = s_fill > HT_SLOTS / 2;
+      __syncthreads();
+      if (ovf) {
+        if (threadIdx.x == 0) atomicOr(flag, 1u);
+        continue;
+      }
+      for (uint64_t c0 = 0; c0 < it.count; c0 += BLOCK) {
+        const uint64_t i = c0 + threadIdx.x;
+        unsigned long long m = 0;
+        K k = 0;
+        if (i < it.count) {
+          k = kp[i];
+          if (bin_of<K>(k, mp) - it.bin_lo == hb) {
+            if (k == EMPTY) {
+              m = s_ecnt;
+            } else {
+              uint32_t slot = murmur4((uint32_t)k ^ (uint32_t)((uint64_t)k >> 32)) & (HT_SLOTS - 1);
+              for (uint32_t pr = 0; pr < HT_SLOTS; pr++, slot = (slot + 1) & (HT_SLOTS - 1)) {
+                const K cur = s_key[slot];
+                if (cur == EMPTY) break;
+                if (cur == k) {
+                  m = s_cnt[slot];
+                  break;
+                }
+              }
+            }
+          }
+        }
+        if (out == nullptr) {
+          const unsigned long long t = block_sum<BLOCK, unsigned long long>(m, s_t64);
+          if (threadIdx.x == 0 && t) atomicAdd(cursor + it.table, t);
+          continue;
+        }
+        unsigned long long tot;
+        const unsigned long long ex = block_exclusive_sum<BLOCK, unsigned long long>(m, s_t64, tot);
+        if (threadIdx.x == 0) s_base = tot ? atomicAdd(cursor + it.table, tot) : 0ull;
+        __syncthreads();
+        const unsigned long long w = s_base + ex;
+        if (m && w < capacity) {
+          const unsigned long long lim = min(m, (unsigned long long)capacity - w);
+          for (unsigned long long r = 0; r < lim; r++) out[w + r] = k;
+        }
+        __syncthreads();
+      }
+    }
   }
 }
 
@@ -1064,13 +1177,16 @@ static hipError_t launch_probe_t(const LaunchProbe &p, hipStream_t st) {
   const hipError_t e = dispatch_hk(p.hk, [&](auto h) {
     hipLaunchKernelGGL((k_probe<K, OFF, 1024, KPT, SUB, decltype(h)::value>), dim3(p.nitems), dim3(1024), lds, st,
                        (const K *)p.bpart, p.items, p.maps, (const OFF *)p.offA, (const K *)p.keysA, (K *)p.out,
-                       p.capacity, p.cursor, p.akey_cap, p.flag);
+                       p.capacity, p.cursor, p.akey_cap, p.hmark);
     return hipGetLastError();
   });
   if (e != hipSuccess) return e;
   const uint32_t small_cap = std::min(1024u * KPT, 65535u);
   hipLaunchKernelGGL((k_probe_big<K, OFF, 1024>), dim3(p.nitems), dim3(1024), 0, st, (const K *)p.bpart, p.items, p.maps,
-                     (const OFF *)p.offA, (const K *)p.keysA, (K *)p.out, p.capacity, p.cursor, small_cap, p.flag);
+                     (const OFF *)p.offA, (const K *)p.keysA, (K *)p.out, p.capacity, p.cursor, small_cap, p.hmark);
+  hipLaunchKernelGGL((k_probe_heavy<K, OFF, 1024>), dim3(p.nitems), dim3(1024), 0, st, (const K *)p.bpart, p.items,
+                     p.maps, (const OFF *)p.offA, (const K *)p.keysA, (K *)p.out, p.capacity, p.cursor,
+                     (const unsigned int *)p.hmark, p.flag);
   return hipGetLastError();
 }
 

@@ -64,7 +64,8 @@ struct LaunchProbe {
   void *out;
   uint64_t capacity;
   unsigned long long *cursor;  // match counters, one per table (FinItem.table)
-  unsigned int *flag;
+  unsigned int *hmark;         // nitems, zeroed: items with a bin beyond the walk limit
+  unsigned int *flag;          // a heavy bin with too many distinct values
 };
 
 struct LaunchJoin {
@@ -104,7 +105,9 @@ struct LaunchRmi {
   uint32_t *counts;          // 256 * sort tiles + 1
   unsigned long long *scan_status;
   unsigned int *scan_ticket;
-  double *partials;          // 4 * nblk
+  double *partials;          // 8 * nblk
+  void *cand;                // 4 root candidates (16 bytes each)
+  uint32_t *leafcnt;         // 4 * n_leaves
   uint32_t nblk;
   uint32_t n_leaves;
   BinMap map;                // murmur3 map of the table's bins

@@ -3,11 +3,12 @@
 // No counterpart in the reference.  Steps, all on the device:
 //   1. seeded sample of the table's keys (with replacement),
 //   2. LSD radix sort of the sample (8-bit digits, stable wave-ballot ranks),
-//   3. least-squares fit of the root (key -> leaf) in FP64, stored as FP32,
+//   3. four root candidates (raw or log feature; least squares or range
+//      fit, in FP64, stored as FP32), the one with the most even leaf loads
+//      on the sample is kept,
 //   4. leaf ranges of the sorted sample under the stored FP32 root,
-//   5. one least-squares fit per leaf (key -> global rank * M / n),
-//   6. monotone bases and the sentinel leaf,
-//   7. the fallback check: both hashes bin the sample at the table's load
+//   5. one least-squares fit per leaf (key -> global rank position * M / n),
+//   6. the fallback check: both hashes bin the sample at the table's load
 //      factor; max bin and sum of squared bin sizes are compared.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -117,24 +118,38 @@ __device__ __forceinline__ void block_sum_d(double (&v)[NV], double *s_red /* 25
   __syncthreads();
 }
 
-// Root moments over the sorted sample: x = key - key_min, y = i * yscale.
-// One row of 4 partial sums per block.
+// Root candidates.  The root is linear in one of two features of the key
+// (raw distance from the smallest key, or the bits of its float: a
+// piecewise-linear log); for each feature it is fitted either by least
+// squares of (i + 1/2) L / n on the feature, or through the feature's range
+// (min -> 0, max -> L).  The candidate whose leaf loads on the sample have
+// the smallest sum of squares is kept.
+struct RootCand {
+  uint32_t kind;
+  float slope, icpt;
+  uint32_t pad;
+};
+
+// Per block: sums of x, y, x^2, x*y for both features (8 doubles).
 template <typename K>
 __global__ __launch_bounds__(256) void k_rmi_root_moments(const K *__restrict__ sorted, uint64_t n, double yscale,
                                                           double *__restrict__ part) {
-  __shared__ double s_red[4 * 256];
+  __shared__ double s_red[8 * 256];
   const uint64_t kmin = (uint64_t)sorted[0];
-  double v[4] = {0.0, 0.0, 0.0, 0.0};
+  double v[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) {
-    const double x = (double)((uint64_t)sorted[i] - kmin);
-    const double y = (double)i * yscale;
-    v[0] += x;
-    v[1] += y;
-    v[2] += x * x;
-    v[3] += x * y;
+    const double y = ((double)i + 0.5) * yscale;
+#pragma unroll
+    for (uint32_t f = 0; f < 2; f++) {
+      const double x = (double)rmi_feature((uint64_t)sorted[i], kmin, f);
+      v[4 * f + 0] += x;
+      v[4 * f + 1] += y;
+      v[4 * f + 2] += x * x;
+      v[4 * f + 3] += x * y;
+    }
   }
-  block_sum_d<4>(v, s_red);
-  if (threadIdx.x < 4) part[(uint64_t)blockIdx.x * 4 + threadIdx.x] = v[threadIdx.x];
+  block_sum_d<8>(v, s_red);
+  if (threadIdx.x < 8) part[(uint64_t)blockIdx.x * 8 + threadIdx.x] = v[threadIdx.x];
 }
 
 // Closed-form least squares y = alpha + beta x from the partial sums; a
@@ -150,23 +165,74 @@ __device__ __forceinline__ void lsq_line(double n, double sx, double sy, double 
 
 template <typename K>
 __global__ __launch_bounds__(256) void k_rmi_root_fit(const K *__restrict__ sorted, uint64_t n, const double *part,
-                                                      uint32_t nblk, uint32_t L, uint32_t M, RmiHeader *__restrict__ h) {
-  __shared__ double s_red[4 * 256];
-  double v[4] = {0.0, 0.0, 0.0, 0.0};
+                                                      uint32_t nblk, uint32_t L, uint32_t M, RmiHeader *__restrict__ h,
+                                                      RootCand *__restrict__ cand) {
+  __shared__ double s_red[8 * 256];
+  double v[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   for (uint32_t b = threadIdx.x; b < nblk; b += 256)
 #pragma unroll
-    for (int j = 0; j < 4; j++) v[j] += part[(uint64_t)b * 4 + j];
-  block_sum_d<4>(v, s_red);
+    for (int j = 0; j < 8; j++) v[j] += part[(uint64_t)b * 8 + j];
+  block_sum_d<8>(v, s_red);
   if (threadIdx.x == 0) {
-    double alpha, beta;
-    lsq_line((double)n, v[0], v[1], v[2], v[3], alpha, beta);
+    const uint64_t kmin = (uint64_t)sorted[0], kmax = (uint64_t)sorted[n - 1];
+    for (uint32_t f = 0; f < 2; f++) {
+      double alpha, beta;
+      lsq_line((double)n, v[4 * f], v[4 * f + 1], v[4 * f + 2], v[4 * f + 3], alpha, beta);
+      RootCand c{f, (float)beta, (float)alpha, 0};
+      cand[2 * f] = c;
+      const double lo = (double)rmi_feature(kmin, kmin, f), hi = (double)rmi_feature(kmax, kmin, f);
+      const double s = hi > lo ? (double)L * (1.0 - 1e-6) / (hi - lo) : 0.0;
+      RootCand r{f, (float)s, (float)(-s * lo), 0};
+      cand[2 * f + 1] = r;
+    }
     RmiHeader hd{};
-    hd.key_min = (uint64_t)sorted[0];
-    hd.root_slope = (float)beta;
-    hd.root_icpt = (float)alpha;
+    hd.key_min = kmin;
     hd.n_leaves = L;
     hd.n_bins = M;
     *h = hd;
+  }
+}
+
+// Leaf loads of the sample under every candidate: cnt[c * L + leaf].
+template <typename K>
+__global__ __launch_bounds__(256) void k_rmi_root_score(const K *__restrict__ sorted, uint64_t n,
+                                                        const RootCand *__restrict__ cand, uint32_t L,
+                                                        uint32_t *__restrict__ cnt) {
+  const uint64_t kmin = (uint64_t)sorted[0];
+  RootCand c[4];
+#pragma unroll
+  for (int j = 0; j < 4; j++) c[j] = cand[j];
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) {
+    const uint64_t k = (uint64_t)sorted[i];
+    const float fr = rmi_feature(k, kmin, 0), fl = rmi_feature(k, kmin, 1);
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+      atomicAdd(&cnt[(uint32_t)j * L + rmi_leaf_at(c[j].kind ? fl : fr, c[j].slope, c[j].icpt, L)], 1u);
+  }
+}
+
+// Keeps the candidate with the smallest sum of squared leaf loads.
+__global__ __launch_bounds__(1024) void k_rmi_root_pick(const uint32_t *__restrict__ cnt, uint32_t L,
+                                                        const RootCand *__restrict__ cand, RmiHeader *__restrict__ h) {
+  __shared__ unsigned long long s_t[1024 / 64 + 1];
+  __shared__ unsigned long long s_sc[4];
+  for (int j = 0; j < 4; j++) {
+    unsigned long long q = 0;
+    for (uint32_t l = threadIdx.x; l < L; l += 1024) {
+      const unsigned long long c = cnt[(uint32_t)j * L + l];
+      q += c * c;
+    }
+    q = block_sum<1024, unsigned long long>(q, s_t);
+    if (threadIdx.x == 0) s_sc[j] = q;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int best = 0;
+    for (int j = 1; j < 4; j++)
+      if (s_sc[j] < s_sc[best]) best = j;
+    h->root_kind = cand[best].kind;
+    h->root_slope = cand[best].slope;
+    h->root_icpt = cand[best].icpt;
   }
 }
 
@@ -182,8 +248,10 @@ __global__ __launch_bounds__(256) void k_rmi_bounds(const K *__restrict__ sorted
   }
 }
 
-// One workgroup per leaf: least squares of the global rank position
-// (i * M / n) on the distance from the leaf's first key.
+// One workgroup per leaf: least squares of the rank position (i + 1/2) M / n
+// on the signed distance from the leaf's first key.  A sparse leaf (fewer
+// than 8 keys or a single value) also takes the sample keys on either side,
+// so that its line follows its neighbourhood.
 template <typename K>
 __global__ __launch_bounds__(256) void k_rmi_leaf_fit(const K *__restrict__ sorted, uint64_t n,
                                                       const uint32_t *__restrict__ start, uint32_t M,
@@ -193,10 +261,14 @@ __global__ __launch_bounds__(256) void k_rmi_leaf_fit(const K *__restrict__ sort
   const uint32_t a = start[l], b = start[l + 1];
   const double yscale = (double)M / (double)n;
   const uint64_t x0 = (uint64_t)sorted[a < n ? a : n - 1];
+  const bool sparse = b - a < 8 || (uint64_t)sorted[b - 1] == x0;
+  const uint32_t lo = sparse && a > 0 ? a - 1 : a;
+  const uint32_t hi = sparse && b < n ? b + 1 : b;
   double v[4] = {0.0, 0.0, 0.0, 0.0};
-  for (uint32_t i = a + threadIdx.x; i < b; i += 256) {
-    const double x = (double)((uint64_t)sorted[i] - x0);
-    const double y = (double)i * yscale;
+  for (uint32_t i = lo + threadIdx.x; i < hi; i += 256) {
+    const uint64_t k = (uint64_t)sorted[i];
+    const double x = k >= x0 ? (double)(k - x0) : -(double)(x0 - k);
+    const double y = ((double)i + 0.5) * yscale;
     v[0] += x;
     v[1] += y;
     v[2] += x * x;
@@ -205,54 +277,19 @@ __global__ __launch_bounds__(256) void k_rmi_leaf_fit(const K *__restrict__ sort
   block_sum_d<4>(v, s_red);
   if (threadIdx.x == 0) {
     double alpha, beta;
-    if (b > a) {
-      lsq_line((double)(b - a), v[0], v[1], v[2], v[3], alpha, beta);
+    if (hi > lo) {
+      lsq_line((double)(hi - lo), v[0], v[1], v[2], v[3], alpha, beta);
     } else {
-      alpha = (double)a * yscale;
+      alpha = ((double)a + 0.5) * yscale;
       beta = 0.0;
     }
-    double bs = floor(alpha + 0.5);
-    bs = bs < 0.0 ? 0.0 : (bs > (double)(M - 1) ? (double)(M - 1) : bs);
+    double an = floor(alpha + 0.5);
+    an = an < 0.0 ? 0.0 : (an > (double)(M - 1) ? (double)(M - 1) : an);
     RmiLeaf lf;
     lf.x0 = x0;
     lf.slope = (float)beta;
-    lf.base = (uint32_t)bs;
+    lf.anchor = (uint32_t)an;
     leaves[l] = lf;
-  }
-}
-
-// Running maximum of the bases (monotone map) and the sentinel leaf L.
-__global__ __launch_bounds__(1024) void k_rmi_finish(RmiLeaf *__restrict__ leaves, uint32_t L, uint32_t M) {
-  __shared__ uint32_t s_w[16];
-  __shared__ uint32_t s_carry;
-  if (threadIdx.x == 0) s_carry = 0;
-  __syncthreads();
-  for (uint32_t c0 = 0; c0 < L; c0 += 1024) {
-    const uint32_t l = c0 + threadIdx.x;
-    uint32_t v = l < L ? leaves[l].base : 0u;
-    // inclusive max-scan of the block
-    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-      const uint32_t y = __shfl_up(v, d, 64);
-      if (lane >= (uint32_t)d) v = v > y ? v : y;
-    }
-    if (lane == 63) s_w[w] = v;
-    __syncthreads();
-    uint32_t pre = s_carry;
-    for (uint32_t q = 0; q < w; q++) pre = pre > s_w[q] ? pre : s_w[q];
-    v = v > pre ? v : pre;
-    if (l < L) leaves[l].base = v;
-    __syncthreads();
-    if (threadIdx.x == 1023) s_carry = v;
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    RmiLeaf s;
-    s.x0 = ~0ull;
-    s.slope = 0.0f;
-    s.base = M - 1;
-    leaves[L] = s;
   }
 }
 
@@ -328,15 +365,21 @@ static hipError_t launch_rmi_train_t(const LaunchRmi &r, hipStream_t st) {
   // sizeof(K) passes (even): the sorted sample is back in buf0
   RmiHeader *h = (RmiHeader *)r.model;
   RmiLeaf *leaves = (RmiLeaf *)(h + 1);
+  RootCand *cand = (RootCand *)r.cand;
   hipLaunchKernelGGL((k_rmi_root_moments<K>), dim3(r.nblk), dim3(256), 0, st, (const K *)a, ns,
                      (double)r.n_leaves / (double)ns, r.partials);
   hipLaunchKernelGGL((k_rmi_root_fit<K>), dim3(1), dim3(256), 0, st, (const K *)a, ns, (const double *)r.partials, r.nblk,
-                     r.n_leaves, r.map.M, h);
+                     r.n_leaves, r.map.M, h, cand);
+  hipError_t e0 = launch_fill_u32(r.leafcnt, 0u, (uint64_t)4 * r.n_leaves, st);
+  if (e0 != hipSuccess) return e0;
+  hipLaunchKernelGGL((k_rmi_root_score<K>), dim3(grid_for(ns, 2048)), dim3(256), 0, st, (const K *)a, ns,
+                     (const RootCand *)cand, r.n_leaves, r.leafcnt);
+  hipLaunchKernelGGL(k_rmi_root_pick, dim3(1), dim3(1024), 0, st, (const uint32_t *)r.leafcnt, r.n_leaves,
+                     (const RootCand *)cand, h);
   hipLaunchKernelGGL((k_rmi_bounds<K>), dim3(grid_for(ns + 1, 4096)), dim3(256), 0, st, (const K *)a, ns,
                      (const RmiHeader *)h, r.start);
   hipLaunchKernelGGL((k_rmi_leaf_fit<K>), dim3(r.n_leaves), dim3(256), 0, st, (const K *)a, ns,
                      (const uint32_t *)r.start, r.map.M, leaves);
-  hipLaunchKernelGGL(k_rmi_finish, dim3(1), dim3(1024), 0, st, leaves, r.n_leaves, r.map.M);
   // fallback check on the sample
   hipError_t e = launch_fill_u32(r.cnt_r, 0u, r.Mp, st);
   if (e == hipSuccess) e = launch_fill_u32(r.cnt_m, 0u, r.Mp, st);

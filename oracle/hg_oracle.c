@@ -135,22 +135,32 @@ int hgo_build_csr(const void *keys, uint64_t n, int kb, uint64_t M, uint64_t *of
 
 /* ------------------------------------------------------------------------ */
 /* Learned CDF model (no counterpart in the reference): evaluation of the    */
-/* FP32 two-stage model, byte layout as the product's (32-byte header:       */
-/* u64 key_min, f32 root_slope, f32 root_icpt, u32 n_leaves, u32 n_bins, 8    */
-/* pad bytes; then n_leaves + 1 leaves of {u64 x0, f32 slope, u32 base}).     */
-/* Every product and sum is rounded on its own (-ffp-contract=off), u64 ->    */
-/* f32 conversions round to nearest even, float -> u32 truncates.             */
-/* Pinned by closed forms on hand-built models, an exact integer restatement  */
-/* of the u64 -> f32 rounding in the tests, monotonicity, and the intersection */
-/* being independent of the hash.                                             */
+/* FP32 two-stage model, byte layout as the product's (32-byte header: u64   */
+/* key_min, f32 root_slope, f32 root_icpt, u32 n_leaves, u32 n_bins, u32      */
+/* root_kind, 4 pad bytes; then n_leaves leaves of {u64 x0, f32 slope, u32    */
+/* anchor}).  Every product and sum is rounded on its own                     */
+/* (-ffp-contract=off), u64 -> f32 conversions round to nearest even, float   */
+/* -> u32 truncates.  Pinned by closed forms on hand-built models, an exact   */
+/* integer restatement of the u64 -> f32 rounding in the tests, and the       */
+/* intersection being independent of the hash.                               */
 /* ------------------------------------------------------------------------ */
-typedef struct { uint64_t key_min; float root_slope, root_icpt; uint32_t n_leaves, n_bins, pad[2]; } hgo_rmi_header;
-typedef struct { uint64_t x0; float slope; uint32_t base; } hgo_rmi_leaf;
+typedef struct { uint64_t key_min; float root_slope, root_icpt; uint32_t n_leaves, n_bins, root_kind, pad; }
+    hgo_rmi_header;
+typedef struct { uint64_t x0; float slope; uint32_t anchor; } hgo_rmi_leaf;
+
+static float rmi_feature_(uint64_t key, uint64_t key_min, uint32_t kind) {
+  float xr = key > key_min ? (float)(key - key_min) : 0.0f;
+  if (kind == 0) return xr;
+  float x1 = xr + 1.0f;
+  uint32_t bits;
+  memcpy(&bits, &x1, 4);
+  return (float)bits;
+}
 
 uint32_t hgo_rmi_leaf_of(const void *model, uint64_t key) {
   const hgo_rmi_header *h = (const hgo_rmi_header *)model;
-  float xr = key > h->key_min ? (float)(key - h->key_min) : 0.0f;
-  float p = h->root_slope * xr;
+  float f = rmi_feature_(key, h->key_min, h->root_kind);
+  float p = h->root_slope * f;
   float t = p + h->root_icpt;
   uint32_t L = h->n_leaves;
   if (!(t > 0.0f)) return 0;
@@ -161,15 +171,15 @@ uint32_t hgo_rmi_leaf_of(const void *model, uint64_t key) {
 uint32_t hgo_rmi_bin(const void *model, uint64_t key, uint64_t M) {
   const hgo_rmi_header *h = (const hgo_rmi_header *)model;
   const hgo_rmi_leaf *lv = (const hgo_rmi_leaf *)(h + 1);
-  uint32_t leaf = hgo_rmi_leaf_of(model, key);
-  hgo_rmi_leaf l = lv[leaf];
-  uint32_t hi = lv[leaf + 1].base;
-  float d = key > l.x0 ? (float)(key - l.x0) : 0.0f;
+  hgo_rmi_leaf l = lv[hgo_rmi_leaf_of(model, key)];
+  int up = key >= l.x0;
+  float d = up ? (float)(key - l.x0) : (float)(l.x0 - key);
   float y = l.slope * d;
-  uint32_t off = !(y > 0.0f) ? 0u : (y >= 4294967040.0f ? 0xFFFFFF00u : (uint32_t)y);
-  uint32_t b = l.base + off;
-  if (b < l.base || b > hi) b = hi;
-  return b < M ? b : (uint32_t)(M - 1);
+  int64_t off = !(y > 0.0f) ? 0 : (y >= 4294967040.0f ? (int64_t)0xFFFFFF00u : (int64_t)(uint32_t)y);
+  int64_t b = (int64_t)l.anchor + (up ? off : -off);
+  if (b < 0) b = 0;
+  if (b > (int64_t)M - 1) b = (int64_t)M - 1;
+  return (uint32_t)b;
 }
 
 void hgo_bins_rmi(const void *keys, uint64_t n, int kb, const void *model, uint64_t M, uint32_t *bins) {

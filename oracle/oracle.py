@@ -160,21 +160,20 @@ def bins_rmi(keys: np.ndarray, model: np.ndarray, M: int) -> np.ndarray:
     return out
 
 
-def rmi_model(key_min: int, root_slope: float, root_icpt: float, leaves, M: int) -> np.ndarray:
-    """Bytes of a learned model: leaves = [(x0, slope, base), ...]; the
-    sentinel leaf (base M-1) is appended."""
+def rmi_model(key_min: int, root_slope: float, root_icpt: float, leaves, M: int, root_kind: int = 0) -> np.ndarray:
+    """Bytes of a learned model: leaves = [(x0, slope, anchor), ...]."""
     L = len(leaves)
     hdr = np.zeros(1, dtype=[("key_min", "<u8"), ("rs", "<f4"), ("ri", "<f4"), ("L", "<u4"), ("M", "<u4"),
-                             ("pad", "<u4", 2)])
+                             ("kind", "<u4"), ("pad", "<u4")])
     hdr["key_min"] = key_min
     hdr["rs"] = root_slope
     hdr["ri"] = root_icpt
     hdr["L"] = L
     hdr["M"] = M
-    lv = np.zeros(L + 1, dtype=[("x0", "<u8"), ("slope", "<f4"), ("base", "<u4")])
+    hdr["kind"] = root_kind
+    lv = np.zeros(L, dtype=[("x0", "<u8"), ("slope", "<f4"), ("anchor", "<u4")])
     for i, (x0, sl, b) in enumerate(leaves):
         lv[i] = (x0, sl, b)
-    lv[L] = (0xFFFFFFFFFFFFFFFF, 0.0, M - 1)
     return np.concatenate([hdr.view(np.uint8), lv.view(np.uint8)])
 
 

@@ -104,14 +104,30 @@ def test_empty_inputs(dev):
     assert hg.intersect(off, csr, eoff, ecsr, 512)[0] == 0
 
 
-def test_heavy_bin_is_refused(dev):
-    # one value repeated 100k times: the walk per probe key would be 100k long;
-    # the library reports it instead of running it
+def test_heavy_bins_are_answered(dev):
+    """Bins beyond the walk limit (one value repeated 100k times) are answered
+    from a per-bin value table: count and list equal the bag join."""
     import hashgraph_amd as hg
-    A = np.full(100000, 12345, dtype=np.uint32)
+    rest = wl.uniform_keys(5, 20000, 4)
+    A = np.concatenate([np.full(100000, 12345, dtype=np.uint32), rest])
+    B = np.concatenate([np.full(1000, 12345, dtype=np.uint32), rest[:3000], wl.uniform_keys(6, 3000, 4)])
     off, csr = hg.build(_to_dev(A, dev), 1024)
+    cnt, _ = hg.probe(off, csr, 1024, _to_dev(B, dev), emit=False)
+    assert cnt == o.bag_intersect_count(A, B)
+    # emission, small enough to list
+    A2 = np.concatenate([np.full(5000, 7, dtype=np.uint32), np.full(4500, 0xFFFFFFFF, dtype=np.uint32), rest[:100]])
+    B2 = np.concatenate([np.full(3, 7, dtype=np.uint32), np.full(2, 0xFFFFFFFF, dtype=np.uint32), rest[:50]])
+    _check_probe(A2, B2, 64, dev)
+
+
+def test_heavy_bin_with_many_values_is_refused(dev):
+    """One bin of 10000 distinct values: beyond both the walk limit and the
+    per-bin value table; the library reports it instead of running it."""
+    import hashgraph_amd as hg
+    A = wl.uniform_keys(1, 10000, 4)
+    off, csr = hg.build(_to_dev(A, dev), 1)
     with pytest.raises(hg.HGError) as e:
-        hg.probe(off, csr, 1024, _to_dev(A[:1000], dev), emit=False)
+        hg.probe(off, csr, 1, _to_dev(A[:1000], dev), emit=False)
     assert e.value.status == 6
 
 

@@ -11,8 +11,9 @@ pytestmark = pytest.mark.gpu
 from hashgraph_amd import workloads as wl
 from oracle import oracle as o
 
-HDR = np.dtype([("key_min", "<u8"), ("rs", "<f4"), ("ri", "<f4"), ("L", "<u4"), ("M", "<u4"), ("pad", "<u4", 2)])
-LEAF = np.dtype([("x0", "<u8"), ("slope", "<f4"), ("base", "<u4")])
+HDR = np.dtype([("key_min", "<u8"), ("rs", "<f4"), ("ri", "<f4"), ("L", "<u4"), ("M", "<u4"), ("kind", "<u4"),
+                ("pad", "<u4")])
+LEAF = np.dtype([("x0", "<u8"), ("slope", "<f4"), ("anchor", "<u4")])
 
 
 def _to_dev(a, dev):
@@ -44,11 +45,14 @@ def test_table_bins_match_oracle_u64(dev):
     base = np.sort(rng.integers(0, M, L))
     leaves = [(int(x0[i]), float(rng.random() * 1e-12), int(base[i])) for i in range(L)]
     m = o.rmi_model(int(x0[0]), float(L / 2.0 ** 63), 0.25, leaves, M)
+    mlog = o.rmi_model(int(x0[0]), 2.0 ** -23 * 3.1, -400.3, leaves, M, root_kind=1)
     keys = rng.integers(0, 1 << 64, 1 << 18, dtype=np.uint64)
     keys[:8] = [0, 1, int(x0[0]) - 1, int(x0[0]), (1 << 64) - 1, (1 << 63), int(x0[5]), int(x0[5]) + 1]
     print("table_bins rmi u64", flush=True)
     got = hg.table_bins(_to_dev(keys, dev), M, hg.HG_HASH_RMI, _model_dev(m, dev)).cpu().numpy().view(np.uint32)
     assert np.array_equal(got, o.bins_rmi(keys, m, M))
+    got = hg.table_bins(_to_dev(keys, dev), M, hg.HG_HASH_RMI, _model_dev(mlog, dev)).cpu().numpy().view(np.uint32)
+    assert np.array_equal(got, o.bins_rmi(keys, mlog, M))
 
 
 def test_table_bins_match_oracle_u32(dev):
@@ -79,41 +83,54 @@ def test_train_against_cpu_fit(dev, kind):
     h, lv = _parse(mb)
     S = np.sort(_sample_cpu(A, ns, seed)).astype(np.uint64)
     assert h["key_min"] == S[0] and h["L"] == L and h["M"] == M
-    # root: least squares of i*L/ns on (key - key_min), against np.polyfit
-    x = (S - S[0]).astype(np.float64)
-    y = np.arange(ns, dtype=np.float64) * (L / ns)
-    b1, b0 = np.polyfit(x, y, 1)
-    assert np.isclose(h["rs"], max(b1, 0.0), rtol=1e-5, atol=0)
-    assert abs(h["ri"] - b0) <= 1e-4 * L
+    # root: four candidates (raw / log feature, least squares / range fit),
+    # the one with the smallest sum of squared leaf loads is kept
+    y = (np.arange(ns, dtype=np.float64) + 0.5) * (L / ns)
+    raw = (S - S[0]).astype(np.float32)
+    feats = [raw, (raw + np.float32(1.0)).view(np.uint32).astype(np.float32)]
+    cands = []
+    for f, F in enumerate(feats):
+        Fd = F.astype(np.float64)
+        b1, b0 = np.polyfit(Fd, y, 1)
+        cands.append((f, max(b1, 0.0), b0 if b1 > 0 else y.mean()))
+        sr = L * (1 - 1e-6) / (Fd.max() - Fd.min())
+        cands.append((f, sr, -sr * Fd.min()))
+    scores = []
+    for f, a_, b_ in cands:
+        t = np.float32(np.float32(a_) * feats[f]) + np.float32(b_)
+        lf = np.clip(np.floor(t), 0, L - 1).astype(np.int64)
+        scores.append((np.bincount(lf, minlength=L).astype(np.float64) ** 2).sum())
+    ok = [c for c, sc in zip(cands, scores) if sc <= min(scores) * (1 + 1e-3)]
+    assert any(h["kind"] == f and np.isclose(h["rs"], a_, rtol=1e-4) and abs(h["ri"] - b_) <= 1e-3 * L
+               for f, a_, b_ in ok), (h, cands, scores)
     # routing of the sorted sample under the stored FP32 root
     leaf = np.array([o.rmi_leaf(mb, int(k)) for k in S])
     assert (np.diff(leaf) >= 0).all()
     start = np.searchsorted(leaf, np.arange(L + 1), side="left")
-    yb = np.arange(ns, dtype=np.float64) * (M / ns)
-    want_base = np.zeros(L)
+    yb = (np.arange(ns, dtype=np.float64) + 0.5) * (M / ns)
     checked = 0
     for l in range(L):
         a, b = start[l], start[l + 1]
-        if b == a:
-            alpha = a * (M / ns)
+        x0 = S[min(a, ns - 1)]
+        assert lv[l]["x0"] == x0
+        sparse = b - a < 8 or S[b - 1] == x0
+        lo, hi = (max(a - 1, 0) if sparse else a), (min(b + 1, ns) if sparse else b)
+        xs = S[lo:hi]
+        d = np.where(xs >= x0, (xs - x0).astype(np.float64), -(x0 - xs).astype(np.float64))
+        if hi - lo >= 2 and d.max() > d.min():
+            s1, alpha = np.polyfit(d, yb[lo:hi], 1)
+            if s1 <= 0:
+                s1, alpha = 0.0, yb[lo:hi].mean()
         else:
-            assert lv[l]["x0"] == S[a]
-            if S[b - 1] > S[a]:
-                d = (S[a:b] - S[a]).astype(np.float64)
-                s1, alpha = np.polyfit(d, yb[a:b], 1)
-                if s1 <= 0:
-                    alpha = yb[a:b].mean()
-                elif b - a >= 8:
-                    assert np.isclose(lv[l]["slope"], s1, rtol=1e-4, atol=0)
-                    checked += 1
-            else:
-                alpha = yb[a:b].mean()
-        want_base[l] = min(max(np.floor(alpha + 0.5), 0), M - 1)
-    want_base = np.maximum.accumulate(want_base)
-    assert np.abs(lv["base"][:L].astype(np.int64) - want_base.astype(np.int64)).max() <= 1
+            s1, alpha = 0.0, (yb[lo:hi].mean() if hi > lo else (a + 0.5) * (M / ns))
+        if s1 > 0:
+            assert np.isclose(lv[l]["slope"], s1, rtol=1e-4, atol=0), l
+            checked += 1
+        else:
+            assert lv[l]["slope"] == 0
+        want = min(max(np.floor(alpha + 0.5), 0), M - 1)
+        assert abs(int(lv[l]["anchor"]) - int(want)) <= 1, l
     assert checked > 0
-    assert (np.diff(lv["base"][:L].astype(np.int64)) >= 0).all()
-    assert lv[L]["base"] == M - 1
     # the fallback figures, recomputed exactly on the CPU
     Sm = _sample_cpu(A, ns, seed)
     Mp = max(1, min(M, int(M * ns / n + 0.5)))

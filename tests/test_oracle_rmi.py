@@ -37,7 +37,7 @@ def test_exact_f32_known_values():
 
 
 def test_identity_leaf_is_truncation():
-    """One leaf, slope 1, base 0: bin = min(f32(key - x0), M-1) exactly."""
+    """One leaf, slope 1, anchor 0: bin = min(f32(key - x0), M-1) exactly."""
     M = 1 << 20
     m = o.rmi_model(0, 0.0, 0.0, [(0, 1.0, 0)], M)
     for k in [0, 1, 7, 1000, M - 2, M - 1, M, 1 << 22, (1 << 24) - 1]:
@@ -62,17 +62,36 @@ def test_piecewise_model_closed_form():
         assert o.rmi_bin(m, k, 1024) == k >> 2
 
 
-def test_key_min_and_x0_anchors():
-    """Keys below key_min go to leaf 0; keys below x0 take the leaf's base."""
+def test_signed_extrapolation_below_x0():
+    """Keys below a leaf's first key go down from its anchor (truncated
+    distance), clamped at bin 0."""
+    m = o.rmi_model(0, 0.0, 0.0, [(5000, 0.5, 100)], 1 << 16)
+    assert o.rmi_bin(m, 5000, 1 << 16) == 100
+    assert o.rmi_bin(m, 5011, 1 << 16) == 105
+    assert o.rmi_bin(m, 4989, 1 << 16) == 95      # 100 - trunc(5.5)
+    assert o.rmi_bin(m, 4800, 1 << 16) == 0       # 100 - 100
+    assert o.rmi_bin(m, 0, 1 << 16) == 0          # clamped
+    assert o.rmi_bin(m, 5000 + 10 ** 6, 1 << 16) == (1 << 16) - 1
+
+
+def test_key_min_anchor_of_root():
     leaves = [(5000, 1.0, 10), (9000, 1.0, 5000)]
     m = o.rmi_model(4000, 1.0 / 4000, 0.0, leaves, 1 << 16)
-    assert o.rmi_leaf(m, 0) == 0 and o.rmi_bin(m, 0, 1 << 16) == 10
-    assert o.rmi_bin(m, 4999, 1 << 16) == 10
-    assert o.rmi_bin(m, 5003, 1 << 16) == 13
-    # leaf 0 is clamped by leaf 1's base
-    assert o.rmi_bin(m, 7999, 1 << 16) == min(10 + 2999, 5000)
-    assert o.rmi_leaf(m, 8000) == 1 and o.rmi_bin(m, 8000, 1 << 16) == 5000  # 8000 < x0 of leaf 1
-    assert o.rmi_bin(m, 9100, 1 << 16) == 5100
+    assert o.rmi_leaf(m, 0) == 0 and o.rmi_leaf(m, 4000) == 0 and o.rmi_leaf(m, 7999) == 0
+    assert o.rmi_leaf(m, 8000) == 1
+    assert o.rmi_bin(m, 8000, 1 << 16) == 5000 - 1000
+
+
+def test_log_feature_root():
+    """root_kind 1: the feature is the bits of f32(key - key_min) + 1 as an
+    integer; with slope 2^-23 and icpt -127 the leaf is floor(log2(d + 1))."""
+    L = 64
+    m = o.rmi_model(0, 2.0 ** -23, -127.0, [(0, 0.0, l) for l in range(L)], 1 << 10, root_kind=1)
+    for e in range(0, 60):
+        k = 1 << e
+        want = min(int(np.floor(np.log2(exact_f32(k) + 1.0))) if e < 24 else e, L - 1)
+        assert o.rmi_leaf(m, k) == want, e
+        assert o.rmi_bin(m, k, 1 << 10) == want
 
 
 def test_nan_and_negative_root_go_to_leaf_0():
@@ -92,6 +111,9 @@ def test_u64_to_f32_rounding(d):
     m = o.rmi_model(0, 0.0, 0.0, [(x0, 1.0, 0)], M)
     want = min(int(exact_f32(d)), 0xFFFFFF00, M - 1)
     assert o.rmi_bin(m, x0 + d, M) == want
+    # and below x0, from an anchor at the top
+    m2 = o.rmi_model(0, 0.0, 0.0, [(x0, 1.0, M - 1)], M)
+    assert o.rmi_bin(m2, x0 - d, M) == max(M - 1 - min(int(exact_f32(d)), 0xFFFFFF00), 0)
 
 
 def test_root_rounding_of_large_keys():
@@ -104,20 +126,18 @@ def test_root_rounding_of_large_keys():
         assert o.rmi_leaf(m, k) == min(int(t), L - 1)
 
 
-def test_monotone_on_random_models():
+def test_monotone_when_leaves_are_monotone():
+    """Leaves whose lines do not cross (anchors and slopes chosen so each
+    leaf ends below the next one's start) give a monotone map."""
     rng = np.random.default_rng(11)
-    for trial in range(5):
-        L = 64
-        M = 1 << 16
-        x0 = np.sort(rng.integers(0, 1 << 62, L, dtype=np.uint64))
-        base = np.sort(rng.integers(0, M, L))
-        slopes = rng.random(L) * 1e-12
-        m = o.rmi_model(int(x0[0]), float(L / 2.0 ** 62), 0.0,
-                        [(int(x0[i]), float(slopes[i]), int(base[i])) for i in range(L)], M)
-        keys = np.sort(rng.integers(0, 1 << 62, 20000, dtype=np.uint64))
-        b = o.bins_rmi(keys, m, M)
-        assert (np.diff(b.astype(np.int64)) >= 0).all()
-        assert b.max() < M
+    L, M = 64, 1 << 16
+    span = 1 << 56
+    leaves = [(l * span, (M / L) / span * 0.9, l * (M // L)) for l in range(L)]
+    m = o.rmi_model(0, float(L / 2.0 ** 62), 0.0, leaves, M)
+    keys = np.sort(rng.integers(0, 1 << 62, 20000, dtype=np.uint64))
+    b = o.bins_rmi(keys, m, M)
+    assert (np.diff(b.astype(np.int64)) >= 0).all()
+    assert b.max() < M
 
 
 def test_build_csr_bins_matches_murmur_build():
