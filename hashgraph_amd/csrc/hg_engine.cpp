@@ -2,6 +2,7 @@
 // arena, descriptor upload and the multi-level build / probe drivers.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <stdlib.h>
 #include <string.h>
 #include <algorithm>
 #include <cmath>
@@ -272,6 +273,22 @@ struct Level {
     return k < 0 ? 0 : k;
   }
 
+  // Keys per tile: rounds of 32 K keys until a (tile, bucket) run averages
+  // >= 32 keys (full-line writes), keeping >= 512 tiles for the chip and at
+  // most 16 rounds.  HG_TILE_ROUNDS overrides the round count.
+  void auto_tile(uint64_t n) {
+    uint32_t F = 1;
+    for (const SegDesc &sd : segs) F = std::max(F, sd.ncols);
+    uint64_t r = (F + 1023) / 1024;
+    r = std::min<uint64_t>(r, std::max<uint64_t>(1, n / ((uint64_t)kTile * 512)));
+    r = std::max<uint64_t>(1, std::min<uint64_t>(r, 16));
+    if (const char *e = getenv("HG_TILE_ROUNDS")) {
+      const long v = atol(e);
+      if (v >= 1 && v <= 64) r = (uint64_t)v;
+    }
+    tile_keys = (uint32_t)(kTile * r);
+  }
+
   void prepare() {
     tiles.clear();
     items.clear();
@@ -388,7 +405,8 @@ hg_status run_level(hg_ctx c, const Level &L, const LevelDev &d, int kb, const v
   p.sk = L.bucket_kind();
   p.ntiles = (uint32_t)L.tiles.size();
   p.max_cols = L.max_cols;
-  p.stage_cap = 8192;
+  p.stage_cap = part_stage_cap(L.max_cols, kb);
+  if (L.tiles.size() && p.stage_cap < 1024) return fail(HG_ERR_UNSUPPORTED, "partition level too wide for LDS");
   p.keys = in;
   p.out = out;
   p.tiles = d.tiles;
@@ -496,6 +514,9 @@ hg_status finish_two_level(hg_ctx c, const std::vector<Coarse> &cs, const BinMap
   }
   if (L2.segs.empty()) return HG_OK;
   L2.maps.push_back(map);
+  uint64_t n2 = 0;
+  for (const SegDesc &sd : L2.segs) n2 += sd.key_end - sd.key_begin;
+  L2.auto_tile(n2);
   L2.prepare();
   LevelDev d2;
   HG_TRY(setup_level(c, L2, d2));
@@ -520,6 +541,7 @@ hg_status build_table(hg_ctx c, const hg_table_desc *desc, const void *d_keys, v
     s.table = 0;
     L.segs.push_back(s);
     L.maps.push_back(map);
+    L.auto_tile(n);
     L.prepare();
     Plan P0;
     const size_t i_part = P0.add(n * kb);
@@ -540,6 +562,7 @@ hg_status build_table(hg_ctx c, const hg_table_desc *desc, const void *d_keys, v
   Level L1;
   L1.segs.push_back(range_seg(0, n, 0, M, (uint32_t)W1, map));
   L1.maps.push_back(map);
+  L1.auto_tile(n);
   L1.prepare();
   Plan P0;
   const size_t i_part1 = P0.add(n * kb), i_part2 = P0.add(n * kb);
@@ -845,6 +868,7 @@ hg_status hg_probe(hg_ctx c, const hg_table_desc *descA, const void *d_offA, con
   s.table = 0;
   L.segs.push_back(s);
   L.maps.push_back(map);
+  L.auto_tile(n_probe);
   L.prepare();
   Plan P0;
   const size_t i_part = P0.add(n_probe * kb);
@@ -1098,6 +1122,7 @@ hg_status hg_build_partitioned(hg_ctx c, const hg_table_desc *desc, const void *
   L1.segs.push_back(s1);
   L1.maps.push_back(map);
   L1.splits.assign(h_splits, h_splits + n_parts + 1);
+  L1.auto_tile(n);
   L1.prepare();
   LevelDev d1;
   HG_TRY(setup_level(c, L1, d1));
@@ -1400,7 +1425,7 @@ hg_status hg_batch_plan_create(hg_ctx c, const hg_batch_desc *bd, const void *d_
   std::vector<BatchCfg> cands;
   if (autotune) {
     for (double f : {0.875, 0.5})
-      for (uint32_t tk : {kTile, kTile / 2, kTile / 4}) {
+      for (uint32_t tk : {kTile / 4, kTile, kTile * 4}) {
         BatchCfg x;
         x.fill = f;
         x.tile_keys = tk;

@@ -62,16 +62,21 @@ __global__ __launch_bounds__(BLOCK) void k_part_hist(const K *__restrict__ keys,
   const uint32_t ncols = sd.ncols;
   for (uint32_t i = threadIdx.x; i < ncols; i += BLOCK) s_cnt[i] = 0;
   __syncthreads();
-  const K *kp = keys + td.key_start;
-  // number of valid slots of this thread (slot j holds key j*BLOCK + tid)
-  const uint32_t nv = td.count > threadIdx.x ? (td.count - threadIdx.x + BLOCK - 1) / BLOCK : 0;
-  const __amdgpu_buffer_rsrc_t rs = make_rsrc(kp, td.count * (uint32_t)sizeof(K));
-  K kr[KPT];
+  // a tile is processed in rounds of BLOCK*KPT keys (slot j of a round holds
+  // key j*BLOCK + tid of the round)
+  for (uint32_t base = 0; base < td.count; base += BLOCK * KPT) {
+    const uint32_t cnt = min((uint32_t)(BLOCK * KPT), td.count - base);
+    const K *kp = keys + td.key_start + base;
+    const uint32_t nv = cnt > threadIdx.x ? (cnt - threadIdx.x + BLOCK - 1) / BLOCK : 0;
+    const __amdgpu_buffer_rsrc_t rs = make_rsrc(kp, cnt * (uint32_t)sizeof(K));
+    K kr[KPT];
 #pragma unroll
-  for (int j = 0; j < KPT; j++) kr[j] = buf_load<K>(rs, threadIdx.x * (uint32_t)sizeof(K), j * BLOCK * (uint32_t)sizeof(K));
+    for (int j = 0; j < KPT; j++)
+      kr[j] = buf_load<K>(rs, threadIdx.x * (uint32_t)sizeof(K), j * BLOCK * (uint32_t)sizeof(K));
 #pragma unroll
-  for (int j = 0; j < KPT; j++)
-    if (j < nv) atomicAdd(&s_cnt[bucket_of<K, HK, SK>(kr[j], sd)], 1u);
+    for (int j = 0; j < KPT; j++)
+      if (j < nv) atomicAdd(&s_cnt[bucket_of<K, HK, SK>(kr[j], sd)], 1u);
+  }
   __syncthreads();
   uint32_t *row = cmat + sd.row_base + (uint64_t)(tile - sd.tile_begin) * ncols;
   for (uint32_t i = threadIdx.x; i < ncols; i += BLOCK) row[i] = s_cnt[i];
@@ -174,61 +179,73 @@ __global__ __launch_bounds__(BLOCK) void k_part_scatter(const K *__restrict__ ke
   const TileDesc td = tiles[tile];
   const SegDesc &sd = segs[td.seg];
   const uint32_t ncols = sd.ncols;
-  uint32_t *s_cnt = (uint32_t *)smem;
+  // LDS: s_cnt (round counts, then round offsets), s_adj (global position of
+  // the tile's next key of each bucket, minus the round offset while a round
+  // is written), then the staging area
   const uint32_t cnt_words = (ncols + 3) & ~3u;
-  K *s_key = (K *)(smem + cnt_words * 4);
+  uint32_t *s_cnt = (uint32_t *)smem;
+  uint32_t *s_adj = s_cnt + cnt_words;
+  K *s_key = (K *)(s_adj + cnt_words);
   uint16_t *s_bkt = (uint16_t *)(s_key + stage_cap);
-  for (uint32_t i = threadIdx.x; i < ncols; i += BLOCK) s_cnt[i] = 0;
-  __syncthreads();
-  const K *kp = keys + td.key_start;
-  const uint32_t nv = td.count > threadIdx.x ? (td.count - threadIdx.x + BLOCK - 1) / BLOCK : 0;
-  const __amdgpu_buffer_rsrc_t rs = make_rsrc(kp, td.count * (uint32_t)sizeof(K));
-  K kr[KPT];
-  uint32_t pk[KPT];
-#pragma unroll
-  for (int j = 0; j < KPT; j++) kr[j] = buf_load<K>(rs, threadIdx.x * (uint32_t)sizeof(K), j * BLOCK * (uint32_t)sizeof(K));
-#pragma unroll
-  for (int j = 0; j < KPT; j++) {
-    if (j < nv) {
-      const uint32_t b = bucket_of<K, HK, SK>(kr[j], sd);
-      const uint32_t r = atomicAdd(&s_cnt[b], 1u);
-      pk[j] = (b << 16) | r;
-    }
-  }
-  __syncthreads();
-  lds_exclusive_scan<BLOCK>(s_cnt, ncols, s_tmp);
-#pragma unroll
-  for (int j = 0; j < KPT; j++) {
-    if (j < nv) {
-      const uint32_t b = pk[j] >> 16;
-      pk[j] = (b << 16) | (s_cnt[b] + (pk[j] & 0xFFFFu));
-    }
-  }
-  __syncthreads();
   const uint32_t *row = cpos + sd.row_base + (uint64_t)(tile - sd.tile_begin) * ncols;
-  for (uint32_t b = threadIdx.x; b < ncols; b += BLOCK) s_cnt[b] = row[b] - s_cnt[b];
-  __syncthreads();
+  for (uint32_t b = threadIdx.x; b < ncols; b += BLOCK) s_adj[b] = row[b];
   const uint32_t nb = sd.nbuckets;
-  for (uint32_t r0 = 0; r0 < td.count; r0 += stage_cap) {
+  for (uint32_t base = 0; base < td.count; base += BLOCK * KPT) {
+    const uint32_t cnt = min((uint32_t)(BLOCK * KPT), td.count - base);
+    for (uint32_t i = threadIdx.x; i < ncols; i += BLOCK) s_cnt[i] = 0;
+    __syncthreads();
+    const K *kp = keys + td.key_start + base;
+    const uint32_t nv = cnt > threadIdx.x ? (cnt - threadIdx.x + BLOCK - 1) / BLOCK : 0;
+    const __amdgpu_buffer_rsrc_t rs = make_rsrc(kp, cnt * (uint32_t)sizeof(K));
+    K kr[KPT];
+    uint32_t pk[KPT];
+#pragma unroll
+    for (int j = 0; j < KPT; j++)
+      kr[j] = buf_load<K>(rs, threadIdx.x * (uint32_t)sizeof(K), j * BLOCK * (uint32_t)sizeof(K));
 #pragma unroll
     for (int j = 0; j < KPT; j++) {
       if (j < nv) {
-        const uint32_t q = (pk[j] & 0xFFFFu) - r0;
-        if (q < stage_cap) {
-          s_key[q] = kr[j];
-          s_bkt[q] = (uint16_t)(pk[j] >> 16);
-        }
+        const uint32_t b = bucket_of<K, HK, SK>(kr[j], sd);
+        const uint32_t r = atomicAdd(&s_cnt[b], 1u);
+        pk[j] = (b << 16) | r;
       }
     }
     __syncthreads();
-    const uint32_t n = min(stage_cap, td.count - r0);
-    for (uint32_t q = threadIdx.x; q < n; q += BLOCK) {
-      const uint32_t b = s_bkt[q];
-      if (b < nb) out[(uint32_t)(s_cnt[b] + r0 + q)] = s_key[q];
+    const uint32_t tot = lds_exclusive_scan<BLOCK>(s_cnt, ncols, s_tmp);
+#pragma unroll
+    for (int j = 0; j < KPT; j++) {
+      if (j < nv) {
+        const uint32_t b = pk[j] >> 16;
+        pk[j] = (b << 16) | (s_cnt[b] + (pk[j] & 0xFFFFu));
+      }
     }
+    for (uint32_t b = threadIdx.x; b < ncols; b += BLOCK) s_adj[b] -= s_cnt[b];
+    __syncthreads();
+    for (uint32_t r0 = 0; r0 < cnt; r0 += stage_cap) {
+#pragma unroll
+      for (int j = 0; j < KPT; j++) {
+        if (j < nv) {
+          const uint32_t q = (pk[j] & 0xFFFFu) - r0;
+          if (q < stage_cap) {
+            s_key[q] = kr[j];
+            s_bkt[q] = (uint16_t)(pk[j] >> 16);
+          }
+        }
+      }
+      __syncthreads();
+      const uint32_t n = min(stage_cap, cnt - r0);
+      for (uint32_t q = threadIdx.x; q < n; q += BLOCK) {
+        const uint32_t b = s_bkt[q];
+        if (b < nb) out[(uint32_t)(s_adj[b] + r0 + q)] = s_key[q];
+      }
+      __syncthreads();
+    }
+    // next round: s_adj[b] = global position after this round's keys of b
+    for (uint32_t b = threadIdx.x; b < ncols; b += BLOCK) s_adj[b] += b + 1 < ncols ? s_cnt[b + 1] : tot;
     __syncthreads();
   }
 }
+
 
 // ---------------------------------------------------------------------------
 // Final stage: one workgroup per bucket (count <= BLOCK*KPT, count <= 65535,
@@ -1112,12 +1129,20 @@ static hipError_t launch_part_hist_t(const LaunchPart &lp, hipStream_t st) {
 template <typename K>
 static hipError_t launch_part_scatter_t(const LaunchPart &lp, hipStream_t st) {
   const size_t cnt_bytes = (size_t)((lp.max_cols + 3) & ~3u) * 4;
-  const size_t lds = cnt_bytes + (size_t)lp.stage_cap * (sizeof(K) + 2);
+  const size_t lds = 2 * cnt_bytes + (size_t)lp.stage_cap * (sizeof(K) + 2);
   return dispatch_part(lp.hk, lp.sk, [&](auto h, auto b) {
     hipLaunchKernelGGL((k_part_scatter<K, 1024, PART_KPT, decltype(h)::value, decltype(b)::value>), dim3(lp.ntiles),
                        dim3(1024), lds, st, (const K *)lp.keys, lp.tiles, lp.segs, lp.cmat, (K *)lp.out, lp.stage_cap);
     return hipGetLastError();
   });
+}
+
+uint32_t part_stage_cap(uint32_t max_cols, int key_bytes) {
+  const size_t used = 2 * (size_t)((max_cols + 3) & ~3u) * 4 + 256;
+  const size_t avail = used < 160 * 1024 ? 160 * 1024 - used : 0;
+  size_t cap = avail / (size_t)(key_bytes + 2) / 1024 * 1024;
+  cap = cap > 8192 ? 8192 : cap;
+  return (uint32_t)cap;
 }
 
 hipError_t launch_part_hist(const LaunchPart &lp, hipStream_t st) {

@@ -16,7 +16,7 @@ struct LaunchPart {
   int hk, sk;            // hash kind, bucket kind
   uint32_t ntiles;
   uint32_t max_cols;     // largest ncols over the segments
-  uint32_t stage_cap;    // keys staged per LDS round in the scatter
+  uint32_t stage_cap;    // keys staged per LDS round in the scatter (part_stage_cap)
   const void *keys;
   void *out;
   const TileDesc *tiles;
@@ -142,6 +142,9 @@ hipError_t launch_make_items(const SegDesc *segs, uint32_t nsegs, const uint32_t
                              FinItem *items, uint32_t ncols_total, hipStream_t st);
 
 int final_kpt(int key_bytes);
+// keys staged per LDS round of the scatter for a level whose widest segment
+// has max_cols columns (0: the columns do not leave room for staging)
+uint32_t part_stage_cap(uint32_t max_cols, int key_bytes);
 uint32_t probe_akey_cap(int key_bytes, int off_bytes);
 uint32_t join_key_cap(int key_bytes, int off_bytes);
 uint64_t scan_tiles(uint64_t n);

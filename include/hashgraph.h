@@ -210,8 +210,8 @@ hg_status hg_batch_probe(hg_ctx ctx, const hg_batch_desc *bd, const uint32_t *d_
  * (build, then the probe counts when d_probe is given) are captured once into
  * a hipGraph; hg_batch_plan_run replays it, ordered after the work already
  * queued on the context's stream and before work queued later.  With
- * autotune = 1 the plan times candidate tile sizes (keys per partition
- * workgroup) and bucket fills (bins per final-stage workgroup) by replaying
+ * autotune = 1 the plan times candidate tile sizes (8 K, 32 K, 128 K keys per
+ * partition workgroup) and bucket fills (bins per final-stage workgroup) by replaying
  * each candidate's graph, and keeps the fastest.  Buffers passed here must
  * stay valid for the plan's life.  A plan has its own stream and scratch;
  * destroy it before its context. */

@@ -88,7 +88,7 @@ def test_plan_replays_match_eager(dev):
     exp = _expected_counts(keys, ko, probe, po)
     plan = hg.BatchPlan(dk, ko, n_bins, probe=dp, probe_off=po, autotune=True)
     info = plan.info()
-    assert info["candidates"] == 6 and info["tile_keys"] in (8192, 16384, 32768)
+    assert info["candidates"] == 6 and info["tile_keys"] in (8192, 32768, 131072)
     for r in range(3):
         plan.run()
         torch.cuda.synchronize()
