@@ -28,19 +28,21 @@ __device__ __forceinline__ uint32_t murmur4(uint32_t k) {
 
 // Raw buffer loads: one resource (base, size in bytes) per tile or bucket;
 // the hardware returns 0 past the end, so slot j of thread t needs neither a
-// clamp nor a branch, and the per-slot offset is a scalar.
+// clamp nor a branch.  The range check covers the VGPR offset (plus the
+// instruction's immediate) but not the SGPR offset, so both parts of the
+// offset go into the VGPR operand.
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *p, uint32_t bytes) {
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes, 0x00020000);
 }
 template <typename K>
-__device__ __forceinline__ K buf_load(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff);
+__device__ __forceinline__ K buf_load(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t off2);
 template <>
-__device__ __forceinline__ uint32_t buf_load<uint32_t>(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
-  return __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0);
+__device__ __forceinline__ uint32_t buf_load<uint32_t>(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t off2) {
+  return __builtin_amdgcn_raw_buffer_load_b32(r, (int)(voff + off2), 0, 0);
 }
 template <>
-__device__ __forceinline__ uint64_t buf_load<uint64_t>(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
-  auto v = __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0);
+__device__ __forceinline__ uint64_t buf_load<uint64_t>(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t off2) {
+  auto v = __builtin_amdgcn_raw_buffer_load_b64(r, (int)(voff + off2), 0, 0);
   return (uint64_t)v[0] | ((uint64_t)v[1] << 32);
 }
 
@@ -51,6 +53,27 @@ __device__ __forceinline__ uint32_t fastmod(uint32_t a, uint64_t m64, uint32_t d
 }
 __device__ __forceinline__ uint32_t fastdiv(uint32_t a, const Div32 &dv) {
   return dv.w == 1 ? a : (uint32_t)__umul64hi(dv.d64, (uint64_t)a);
+}
+
+// dst[0..n) = src[0..n) through the workgroup: every thread issues PER
+// bounds-checked buffer loads before it stores any of them, so a chunk of
+// BLOCK*PER elements costs one memory latency (a plain strided loop waits
+// for each load before the next).
+template <typename T, int BLOCK, int PER>
+__device__ __forceinline__ void lds_fill(T *dst, const T *src, uint32_t n) {
+  for (uint32_t c0 = 0; c0 < n; c0 += BLOCK * PER) {
+    const uint32_t m = min(n - c0, (uint32_t)(BLOCK * PER));
+    const __amdgpu_buffer_rsrc_t rs = make_rsrc(src + c0, m * (uint32_t)sizeof(T));
+    T v[PER];
+#pragma unroll
+    for (int j = 0; j < PER; j++)
+      v[j] = buf_load<T>(rs, threadIdx.x * (uint32_t)sizeof(T), (uint32_t)(j * BLOCK * sizeof(T)));
+#pragma unroll
+    for (int j = 0; j < PER; j++) {
+      const uint32_t q = j * BLOCK + threadIdx.x;
+      if (q < m) dst[c0 + q] = v[j];
+    }
+  }
 }
 
 // Learned CDF model (layout and rounding rules in hg_common.hpp).  The

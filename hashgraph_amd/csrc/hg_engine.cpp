@@ -59,6 +59,11 @@ Div32 make_div(uint32_t w) {
 
 uint32_t final_cap(int kb) { return std::min<uint32_t>(1024u * (uint32_t)final_kpt(kb), 65535u); }
 
+// Bucket fill of the probe: a bucket's keys of A must fit the one-pass
+// probe's LDS (39 K uint32 or 19 K uint64 keys), and its keys of B its VGPRs
+// (32 K uint32 or 16 K uint64 keys).
+double probe_fill(int kb) { return kb == 4 ? 0.875 : 0.45; }
+
 // Bucket width W (bins) of a table so that a bucket holds ~7/8 of what the
 // final stage keeps in VGPRs; W <= 65535 (u16 LDS counters).
 uint32_t choose_width(uint64_t n, uint64_t M, int kb, double fill = 0.875) {
@@ -273,15 +278,14 @@ struct Level {
     return k < 0 ? 0 : k;
   }
 
-  // Keys per tile: rounds of 32 K keys until a (tile, bucket) run averages
-  // >= 32 keys (full-line writes), keeping >= 512 tiles for the chip and at
-  // most 16 rounds.  HG_TILE_ROUNDS overrides the round count.
+  // Keys per tile: rounds of 32 K keys.  More rounds give longer (tile,
+  // bucket) runs and a smaller count matrix but fewer, longer workgroups; on
+  // 2^28 keys (F = 9363) one round took 1.24 + 0.31 ms (scatter + column
+  // scan), two 1.31 + 0.16 ms, four 1.42 + 0.09 ms, ten 1.91 + 0.08 ms.  Two
+  // rounds are used from 2^25 keys on (>= 512 tiles remain).  HG_TILE_ROUNDS
+  // sets the round count (1..64) for tuning.
   void auto_tile(uint64_t n) {
-    uint32_t F = 1;
-    for (const SegDesc &sd : segs) F = std::max(F, sd.ncols);
-    uint64_t r = (F + 1023) / 1024;
-    r = std::min<uint64_t>(r, std::max<uint64_t>(1, n / ((uint64_t)kTile * 512)));
-    r = std::max<uint64_t>(1, std::min<uint64_t>(r, 16));
+    uint64_t r = n >= (1ull << 25) ? 2 : 1;
     if (const char *e = getenv("HG_TILE_ROUNDS")) {
       const long v = atol(e);
       if (v >= 1 && v <= 64) r = (uint64_t)v;
@@ -858,7 +862,7 @@ hg_status hg_probe(hg_ctx c, const hg_table_desc *descA, const void *d_offA, con
   const uint64_t M = descA->n_bins;
   const int kb = (int)descA->key_bytes, ob = (int)descA->offset_bytes;
   const BinMap map = make_map(M, descA->hash_kind, descA->d_rmi_model);
-  const uint32_t W = choose_width(std::max<uint64_t>(n_probe, descA->n_keys), M, kb);
+  const uint32_t W = choose_width(std::max<uint64_t>(n_probe, descA->n_keys), M, kb, probe_fill(kb));
   const uint64_t F = (M + W - 1) / W;
   if (F > kFMax) return fail(HG_ERR_UNSUPPORTED, "probe geometry needs more than one partition level");
   Level L;
@@ -1216,7 +1220,8 @@ hg_status batch_prepare(hg_ctx c, const hg_batch_desc *bd, const uint64_t *probe
   for (uint32_t t = 0; t < bd->n_tables; t++) {
     const uint64_t n = ko[t + 1] - ko[t], np = src[t + 1] - src[t], M = bd->h_n_bins[t];
     const BinMap map = batch_map(bd, t);
-    const uint32_t W = choose_width(probe_off ? std::max(n, np) : n, M, kb, cfg.fill);
+    const uint32_t W = choose_width(probe_off ? std::max(n, np) : n, M, kb,
+                                    probe_off ? std::min(cfg.fill, probe_fill(kb)) : cfg.fill);
     SegDesc s = range_seg(src[t], src[t + 1], 0, M, W, map);
     if (s.nbuckets > kFMax) return fail(HG_ERR_UNSUPPORTED, "a batched table is too large for one partition level");
     s.fin_off_base = bd->h_off_off[t];

@@ -188,7 +188,7 @@ __global__ __launch_bounds__(BLOCK) void k_part_scatter(const K *__restrict__ ke
   K *s_key = (K *)(s_adj + cnt_words);
   uint16_t *s_bkt = (uint16_t *)(s_key + stage_cap);
   const uint32_t *row = cpos + sd.row_base + (uint64_t)(tile - sd.tile_begin) * ncols;
-  for (uint32_t b = threadIdx.x; b < ncols; b += BLOCK) s_adj[b] = row[b];
+  lds_fill<uint32_t, BLOCK, 4>(s_adj, row, ncols);
   const uint32_t nb = sd.nbuckets;
   for (uint32_t base = 0; base < td.count; base += BLOCK * KPT) {
     const uint32_t cnt = min((uint32_t)(BLOCK * KPT), td.count - base);
@@ -393,6 +393,102 @@ __global__ __launch_bounds__(BLOCK) void k_final_big(const K *__restrict__ part,
 }
 
 // ---------------------------------------------------------------------------
+// One-pass probe, for buckets whose keys of A fit in LDS (all of them on
+// uniform keys).  B's keys stay in VGPRs with their 16-bit local bins; the
+// bucket's keys of A are staged in LDS once; each B key reads its bin's two
+// offsets from global memory (a window of W+1 entries that stays in the L2
+// while the workgroup runs) and walks the bin in LDS, counting matches in
+// 16-bit halves.  Slots go in groups of 8 so that their offset loads overlap,
+// and the first 4 keys of a bin are compared without a branch.  Emission as
+// in k_probe.  Heavy bins are left to k_probe_heavy.
+// ---------------------------------------------------------------------------
+template <typename K, typename OFF, int BLOCK, int KPT, int HK>
+__global__ __launch_bounds__(BLOCK) void k_probe1(const K *__restrict__ bpart, const FinItem *__restrict__ items,
+                                                  const BinMap *__restrict__ maps, const OFF *__restrict__ offA,
+                                                  const K *__restrict__ keysA, K *__restrict__ out, uint64_t capacity,
+                                                  unsigned long long *__restrict__ cursor, uint32_t akey_cap,
+                                                  unsigned int *__restrict__ hmark) {
+  extern __shared__ __align__(16) uint8_t smem[];
+  __shared__ uint32_t s_tmp[BLOCK / 64 + 1];
+  __shared__ unsigned long long s_base;
+  K *s_ak = (K *)smem;
+  const FinItem it = items[blockIdx.x];
+  if (it.count > BLOCK * KPT || it.count > 65535u) return;
+  const uint32_t W = it.nbins;
+  const OFF *ob = offA + it.off_pos;
+  const uint64_t a_lo = (uint64_t)ob[0], a_hi = (uint64_t)ob[W];
+  if (a_hi - a_lo > akey_cap) return;  // k_probe's item (A staged in sub-ranges)
+  const uint32_t na = (uint32_t)(a_hi - a_lo);
+  const BinMap mp = maps[it.table];
+  const uint32_t cnt = it.count;
+  const K *kp = bpart + it.key_start;
+  const K *ka = keysA + it.key_rel;
+  const uint32_t nv = cnt > threadIdx.x ? (cnt - threadIdx.x + BLOCK - 1) / BLOCK : 0;
+  const __amdgpu_buffer_rsrc_t rs = make_rsrc(kp, cnt * (uint32_t)sizeof(K));
+  K kr[KPT];
+  uint32_t lb[(KPT + 1) / 2], mc[(KPT + 1) / 2];
+#pragma unroll
+  for (int j = 0; j < KPT; j++) kr[j] = buf_load<K>(rs, threadIdx.x * (uint32_t)sizeof(K), j * BLOCK * (uint32_t)sizeof(K));
+  lds_fill<K, BLOCK, sizeof(K) == 4 ? 8 : 4>(s_ak, ka + a_lo, na);
+#pragma unroll
+  for (int j = 0; j < (KPT + 1) / 2; j++) {
+    lb[j] = 0;
+    mc[j] = 0;
+  }
+#pragma unroll
+  for (int j = 0; j < KPT; j++)
+    if (j < nv) lb[j >> 1] |= ((bin_of<K, HK>(kr[j], mp) - it.bin_lo) & 0xFFFFu) << ((j & 1) * 16);
+  __syncthreads();
+  const __amdgpu_buffer_rsrc_t ro = make_rsrc(ob, (W + 1) * (uint32_t)sizeof(OFF));
+  constexpr int G = 8;
+#pragma unroll
+  for (int g = 0; g < KPT; g += G) {
+    OFF o0[G], o1[G];
+#pragma unroll
+    for (int u = 0; u < G; u++) {
+      const int j = g + u;
+      const uint32_t l = (lb[j >> 1] >> ((j & 1) * 16)) & 0xFFFFu;
+      const uint32_t vo = j < nv ? l * (uint32_t)sizeof(OFF) : 0xFFFFFFF0u;  // past the window: loads 0
+      o0[u] = buf_load<OFF>(ro, vo, 0);
+      o1[u] = buf_load<OFF>(ro, vo, (uint32_t)sizeof(OFF));
+    }
+#pragma unroll
+    for (int u = 0; u < G; u++) {
+      const int j = g + u;
+      uint32_t len = (uint32_t)(o1[u] - o0[u]);
+      const uint32_t b0 = (uint32_t)((uint64_t)o0[u] - a_lo);
+      if (len > PROBE_BIN_LIMIT) {
+        hmark[blockIdx.x] = 1u;  // k_probe_heavy answers this bin
+        len = 0;
+      }
+      uint32_t m = 0;
+#pragma unroll
+      for (uint32_t q = 0; q < 4; q++) {
+        const K v = s_ak[q < len ? b0 + q : 0u];
+        m += (q < len && v == kr[j]) ? 1u : 0u;
+      }
+      for (uint32_t q = 4; q < len; q++) m += s_ak[b0 + q] == kr[j];
+      mc[j >> 1] += m << ((j & 1) * 16);  // m <= PROBE_BIN_LIMIT: no carry into the other half
+    }
+  }
+  uint32_t mine = 0;
+#pragma unroll
+  for (int j = 0; j < (KPT + 1) / 2; j++) mine += (mc[j] & 0xFFFFu) + (mc[j] >> 16);
+  uint32_t tot;
+  const uint32_t ex = block_exclusive_sum<BLOCK, uint32_t>(mine, s_tmp, tot);
+  if (threadIdx.x == 0) s_base = tot ? atomicAdd(cursor + it.table, (unsigned long long)tot) : 0ull;
+  __syncthreads();
+  if (out == nullptr || !mine) return;
+  unsigned long long w = s_base + ex;
+#pragma unroll
+  for (int j = 0; j < KPT; j++) {
+    const uint32_t m = (mc[j >> 1] >> ((j & 1) * 16)) & 0xFFFFu;
+    for (uint32_t r = 0; r < m; r++, w++)
+      if (w < capacity) out[w] = kr[j];
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Probe: B's keys of one bucket stay in VGPRs with their 16-bit local bins.
 // The bucket's bins of A are visited in sub-ranges of at most SUB bins and at
 // most akey_cap keys; each sub-range's offsets and keys are staged in LDS
@@ -408,7 +504,8 @@ __global__ __launch_bounds__(BLOCK) void k_probe(const K *__restrict__ bpart, co
                                                  const BinMap *__restrict__ maps, const OFF *__restrict__ offA,
                                                  const K *__restrict__ keysA, K *__restrict__ out, uint64_t capacity,
                                                  unsigned long long *__restrict__ cursor, uint32_t akey_cap,
-                                                 unsigned int *__restrict__ hmark) {
+                                                 unsigned int *__restrict__ hmark, uint32_t one_cnt,
+                                                 uint32_t one_cap) {
   extern __shared__ __align__(16) uint8_t smem[];
   __shared__ uint32_t s_tmp[BLOCK / 64 + 1];
   __shared__ uint32_t s_n;
@@ -417,6 +514,8 @@ __global__ __launch_bounds__(BLOCK) void k_probe(const K *__restrict__ bpart, co
   K *s_ak = (K *)(smem + ((SUB + 2) * sizeof(OFF) + 15) / 16 * 16);
   const FinItem it = items[blockIdx.x];
   if (it.count > BLOCK * KPT || it.count > 65535u) return;  // k_probe_big's item
+  if (it.count <= one_cnt && (uint64_t)(offA[it.off_pos + it.nbins] - offA[it.off_pos]) <= one_cap)
+    return;  // k_probe1's item
   const BinMap mp = maps[it.table];
   const uint32_t cnt = it.count, W = it.nbins;
   const K *kp = bpart + it.key_start;
@@ -437,7 +536,7 @@ __global__ __launch_bounds__(BLOCK) void k_probe(const K *__restrict__ bpart, co
     if (j < nv) lb[j >> 1] |= ((bin_of<K, HK>(kr[j], mp) - it.bin_lo) & 0xFFFFu) << ((j & 1) * 16);
   for (uint32_t s0 = 0; s0 < W;) {
     uint32_t n = min((uint32_t)SUB, W - s0);
-    for (uint32_t q = threadIdx.x; q <= n; q += BLOCK) s_off[q] = offA[it.off_pos + s0 + q];
+    lds_fill<OFF, BLOCK, (sizeof(K) == 4 && sizeof(OFF) == 4) ? 8 : 4>(s_off, offA + it.off_pos + s0, n + 1);
     __syncthreads();
     const OFF a0 = s_off[0];
     if ((uint64_t)(s_off[n] - a0) > akey_cap) {
@@ -455,7 +554,7 @@ __global__ __launch_bounds__(BLOCK) void k_probe(const K *__restrict__ bpart, co
       n = s_n;
     }
     const uint32_t na = (uint32_t)min((uint64_t)(s_off[n] - a0), (uint64_t)akey_cap);
-    for (uint32_t q = threadIdx.x; q < na; q += BLOCK) s_ak[q] = ka[a0 + q];
+    lds_fill<K, BLOCK, sizeof(K) == 4 ? 8 : 4>(s_ak, ka + a0, na);
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < KPT; j++) {
@@ -1193,16 +1292,25 @@ hipError_t launch_final(const LaunchFinal &lf, hipStream_t st) {
 
 int final_kpt(int key_bytes) { return key_bytes == 4 ? KPT_U32 : 32; }
 
+// One-pass probe's bucket limits: keys of B per workgroup, keys of A in LDS.
+template <typename K> constexpr int probe1_kpt() { return sizeof(K) == 4 ? 32 : 16; }
+template <typename K> constexpr uint32_t probe1_cap() { return (uint32_t)(156 * 1024 / sizeof(K)); }
+
 template <typename K, typename OFF>
 static hipError_t launch_probe_t(const LaunchProbe &p, hipStream_t st) {
   constexpr int KPT = sizeof(K) == 4 ? KPT_U32 : 32;
+  constexpr int KPT1 = probe1_kpt<K>();
+  constexpr uint32_t CAP1 = probe1_cap<K>();
   constexpr int SUB = probe_sub<OFF>();
   if (p.akey_cap < PROBE_BIN_LIMIT) return hipErrorInvalidValue;  // a walkable bin must fit in LDS
   const size_t lds = ((SUB + 2) * sizeof(OFF) + 15) / 16 * 16 + (size_t)p.akey_cap * sizeof(K);
   const hipError_t e = dispatch_hk(p.hk, [&](auto h) {
+    hipLaunchKernelGGL((k_probe1<K, OFF, 1024, KPT1, decltype(h)::value>), dim3(p.nitems), dim3(1024),
+                       (size_t)CAP1 * sizeof(K), st, (const K *)p.bpart, p.items, p.maps, (const OFF *)p.offA,
+                       (const K *)p.keysA, (K *)p.out, p.capacity, p.cursor, CAP1, p.hmark);
     hipLaunchKernelGGL((k_probe<K, OFF, 1024, KPT, SUB, decltype(h)::value>), dim3(p.nitems), dim3(1024), lds, st,
                        (const K *)p.bpart, p.items, p.maps, (const OFF *)p.offA, (const K *)p.keysA, (K *)p.out,
-                       p.capacity, p.cursor, p.akey_cap, p.hmark);
+                       p.capacity, p.cursor, p.akey_cap, p.hmark, (uint32_t)(1024 * KPT1), CAP1);
     return hipGetLastError();
   });
   if (e != hipSuccess) return e;
@@ -1335,6 +1443,10 @@ hipError_t set_kernel_attributes() {
       if (r == hipSuccess) r = set_lds(&k_final<uint32_t, uint64_t, 1024, KPT_U32, H>);
       if (r == hipSuccess) r = set_lds(&k_final<uint64_t, uint32_t, 1024, 32, H>);
       if (r == hipSuccess) r = set_lds(&k_final<uint64_t, uint64_t, 1024, 32, H>);
+      if (r == hipSuccess) r = set_lds(&k_probe1<uint32_t, uint32_t, 1024, probe1_kpt<uint32_t>(), H>);
+      if (r == hipSuccess) r = set_lds(&k_probe1<uint32_t, uint64_t, 1024, probe1_kpt<uint32_t>(), H>);
+      if (r == hipSuccess) r = set_lds(&k_probe1<uint64_t, uint32_t, 1024, probe1_kpt<uint64_t>(), H>);
+      if (r == hipSuccess) r = set_lds(&k_probe1<uint64_t, uint64_t, 1024, probe1_kpt<uint64_t>(), H>);
       if (r == hipSuccess) r = set_lds(&k_probe<uint32_t, uint32_t, 1024, KPT_U32, probe_sub<uint32_t>(), H>);
       if (r == hipSuccess) r = set_lds(&k_probe<uint32_t, uint64_t, 1024, KPT_U32, probe_sub<uint64_t>(), H>);
       if (r == hipSuccess) r = set_lds(&k_probe<uint64_t, uint32_t, 1024, 32, probe_sub<uint32_t>(), H>);
