@@ -4,24 +4,28 @@
 
 namespace hg {
 
+// Division of a 32-bit value by an invariant d (1 <= d < 2^32) with one
+// 32-bit multiply-high (Granlund & Montgomery, "Division by invariant
+// integers using multiplication", PLDI 1994), l = floor(log2 d):
+//   mode 0, d = 2^s:  q = a >> s
+//   mode 1:           q = mulhi(a, m) >> s,  m = ceil(2^(32+l) / d), s = l,
+//                     used when m*d - 2^(32+l) <= 2^l (then exact for all a)
+//   mode 2:           t = mulhi(a, m); q = (((a - t) >> 1) + t) >> s,
+//                     m = ceil(2^(33+l) / d) - 2^32, s = l (always exact)
+struct Div32 {
+  uint32_t m;
+  uint32_t s;
+  uint32_t d;
+  uint32_t mode;
+};
+
 // Bin map of one table: bin = murmur3_x86_32(low32(key), seed 0) mod M
 // (include/SingleHashGraphOperators.cuh:71-79), or the learned CDF model.
-// The remainder uses Lemire, Kaser & Kurz, "Faster remainder by direct
-// computation" (2019): m64 = floor((2^64-1)/M) + 1, a mod M =
-// hi64((m64 * a mod 2^64) * M); exact for every 32-bit a and 1 <= M < 2^32.
 struct BinMap {
-  uint64_t m64;       // Lemire constant for M
+  Div32 mod;          // division by M (the remainder is h - q*M)
   uint32_t M;         // number of bins
   uint32_t kind;      // 0 murmur3, 1 learned (RMI)
   const void *rmi;    // device pointer to the RMI model (kind == 1)
-};
-
-// Division of a local bin by a bucket width W (same direct method):
-// q = hi64(d64 * a) with d64 = floor((2^64-1)/W) + 1; exact for 32-bit a.
-struct Div32 {
-  uint64_t d64;
-  uint32_t w;
-  uint32_t pad;
 };
 
 // One segment of a partition pass: a contiguous key range that is split

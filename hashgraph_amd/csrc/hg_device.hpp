@@ -46,13 +46,17 @@ __device__ __forceinline__ uint64_t buf_load<uint64_t>(__amdgpu_buffer_rsrc_t r,
   return (uint64_t)v[0] | ((uint64_t)v[1] << 32);
 }
 
-// a mod d and a / d by direct computation (Lemire, Kaser & Kurz 2019).
-__device__ __forceinline__ uint32_t fastmod(uint32_t a, uint64_t m64, uint32_t d) {
-  uint64_t low = m64 * (uint64_t)a;
-  return (uint32_t)__umul64hi(low, (uint64_t)d);
+// a / d and a mod d with the invariant divider (layout in hg_common.hpp).
+// The mode is uniform, so the branches are scalar and only one path runs.
+__device__ __forceinline__ uint32_t divq(uint32_t a, const Div32 &v) {
+  if (v.mode == 0) return a >> v.s;
+  const uint32_t t = __umulhi(a, v.m);
+  if (v.mode == 1) return t >> v.s;
+  return (((a - t) >> 1) + t) >> v.s;
 }
-__device__ __forceinline__ uint32_t fastdiv(uint32_t a, const Div32 &dv) {
-  return dv.w == 1 ? a : (uint32_t)__umul64hi(dv.d64, (uint64_t)a);
+__device__ __forceinline__ uint32_t modq(uint32_t a, const Div32 &v) {
+  if (v.mode == 0) return a & (v.d - 1);
+  return a - divq(a, v) * v.d;
 }
 
 // dst[0..n) = src[0..n) through the workgroup: every thread issues PER
@@ -116,8 +120,7 @@ enum : int { HK_MURMUR = 0, HK_RMI = 1, HK_ANY = 2 };
 template <typename K, int HK = HK_ANY>
 __device__ __forceinline__ uint32_t bin_of(K key, const BinMap &m) {
   if (HK == HK_MURMUR || (HK == HK_ANY && m.kind == 0)) {
-    uint32_t h = murmur4((uint32_t)key);
-    return (m.M & (m.M - 1)) == 0 ? (h & (m.M - 1)) : fastmod(h, m.m64, m.M);
+    return modq(murmur4((uint32_t)key), m.mod);
   }
   return rmi_bin((uint64_t)key, m.rmi, m.M);
 }
@@ -129,7 +132,7 @@ enum : int { SK_RANGE = 0, SK_SPLIT = 1, SK_ANY = 2 };
 template <typename K, int HK = HK_ANY, int SK = SK_ANY>
 __device__ __forceinline__ uint32_t bucket_of(K key, const SegDesc &sd) {
   uint32_t bin = bin_of<K, HK>(key, sd.map);
-  if (SK == SK_RANGE || (SK == SK_ANY && sd.kind == 0)) return fastdiv(bin - (uint32_t)sd.bin_lo, sd.div);
+  if (SK == SK_RANGE || (SK == SK_ANY && sd.kind == 0)) return divq(bin - (uint32_t)sd.bin_lo, sd.div);
   uint32_t p = 0;
   while (p < sd.nsplits && (int64_t)bin >= sd.splits[p + 1]) p++;
   return p;  // p == nsplits: past the last split

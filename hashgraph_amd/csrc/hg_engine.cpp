@@ -47,17 +47,43 @@ uint64_t splitmix_mix(uint64_t z) {
   return z ^ (z >> 31);
 }
 
-uint64_t lemire_m64(uint64_t d) { return d > 1 ? (UINT64_C(0xFFFFFFFFFFFFFFFF) / d + 1) : 0; }
-
-Div32 make_div(uint32_t w) {
-  Div32 v;
-  v.w = w;
-  v.d64 = lemire_m64(w);
-  v.pad = 0;
+// Invariant-divider constants (the derivation is in hg_common.hpp).
+Div32 make_div(uint32_t d) {
+  Div32 v{0, 0, d, 0};
+  if (d == 0) return v;
+  const uint32_t l = 31 - (uint32_t)__builtin_clz(d);
+  if ((d & (d - 1)) == 0) {
+    v.s = l;
+    return v;
+  }
+  const unsigned __int128 p = (unsigned __int128)1 << (32 + l);
+  const uint64_t m1 = (uint64_t)(p / d) + 1;  // ceil(p / d): p % d != 0 for d not a power of two
+  const uint64_t e = (uint64_t)((unsigned __int128)m1 * d - p);
+  if (e <= (1ull << l)) {
+    v.mode = 1;
+    v.m = (uint32_t)m1;
+    v.s = l;
+    return v;
+  }
+  const unsigned __int128 p2 = (unsigned __int128)1 << (33 + l);
+  const unsigned __int128 m2 = p2 / d + 1;  // in (2^32, 2^33)
+  v.mode = 2;
+  v.m = (uint32_t)(m2 - ((unsigned __int128)1 << 32));
+  v.s = l;
   return v;
 }
 
 uint32_t final_cap(int kb) { return std::min<uint32_t>(1024u * (uint32_t)final_kpt(kb), 65535u); }
+
+// Bucket fill of the build (HG_BUILD_FILL overrides it for tuning: above 1
+// buckets exceed the final stage's VGPR capacity and take k_final_big).
+double build_fill() {
+  if (const char *e = getenv("HG_BUILD_FILL")) {
+    const double v = atof(e);
+    if (v > 0.01 && v < 64.0) return v;
+  }
+  return 0.875;
+}
 
 // Bucket fill of the probe: a bucket's keys of A must fit the one-pass
 // probe's LDS (39 K uint32 or 19 K uint64 keys), and its keys of B its VGPRs
@@ -229,7 +255,7 @@ struct Blob {
 BinMap make_map(uint64_t M, uint32_t kind, const void *rmi) {
   BinMap m;
   m.M = (uint32_t)M;
-  m.m64 = lemire_m64(M);
+  m.mod = make_div((uint32_t)M);
   m.kind = kind;
   m.rmi = rmi;
   return m;
@@ -534,7 +560,7 @@ hg_status build_table(hg_ctx c, const hg_table_desc *desc, const void *d_keys, v
   const int kb = (int)desc->key_bytes, ob = (int)desc->offset_bytes;
   const BinMap map = make_map(M, desc->hash_kind, desc->d_rmi_model);
   if (n == 0) return fill_offsets(c, d_off, ob, M + 1, 0);
-  const uint32_t W = choose_width(n, M, kb);
+  const uint32_t W = choose_width(n, M, kb, build_fill());
   const uint64_t F = (M + W - 1) / W;
   if (F <= kFMax) {
     Level L;

@@ -58,7 +58,7 @@ __global__ __launch_bounds__(BLOCK) void k_part_hist(const K *__restrict__ keys,
   extern __shared__ uint32_t s_cnt[];
   const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
   const TileDesc td = tiles[tile];
-  const SegDesc &sd = segs[td.seg];
+  const SegDesc sd = segs[td.seg];  // a copy: uniform fields stay in SGPRs
   const uint32_t ncols = sd.ncols;
   for (uint32_t i = threadIdx.x; i < ncols; i += BLOCK) s_cnt[i] = 0;
   __syncthreads();
@@ -177,7 +177,7 @@ __global__ __launch_bounds__(BLOCK) void k_part_scatter(const K *__restrict__ ke
   __shared__ uint32_t s_tmp[BLOCK / 64 + 1];
   const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
   const TileDesc td = tiles[tile];
-  const SegDesc &sd = segs[td.seg];
+  const SegDesc sd = segs[td.seg];  // a copy: uniform fields stay in SGPRs
   const uint32_t ncols = sd.ncols;
   // LDS: s_cnt (round counts, then round offsets), s_adj (global position of
   // the tile's next key of each bucket, minus the round offset while a round
@@ -1021,11 +1021,11 @@ __global__ void k_make_items(const SegDesc *__restrict__ segs, uint32_t nsegs, c
   if (b >= sd.nbuckets || sd.kind != 0) {  // drop column
     it.key_start = 0; it.count = 0; it.nbins = 0; it.bin_lo = 0; it.off_pos = 0; it.end_pos = ~0ull;
   } else {
-    const uint64_t blo = (uint64_t)b * sd.div.w;
+    const uint64_t blo = (uint64_t)b * sd.div.d;
     it.key_start = bstart[g];
     it.count = coltot[g];
     it.bin_lo = (uint32_t)(sd.bin_lo + blo);
-    it.nbins = (uint32_t)min((uint64_t)sd.div.w, sd.seg_nbins - blo);
+    it.nbins = (uint32_t)min((uint64_t)sd.div.d, sd.seg_nbins - blo);
     it.off_pos = sd.fin_off_base + blo;
     it.end_pos = (b + 1 == sd.nbuckets) ? sd.fin_end_pos : ~0ull;
   }
