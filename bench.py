@@ -207,6 +207,8 @@ def c3_run(dev, steps, warmup):
         total, recs = 0, 0
         for p in range(P):
             lo, hi = sa[p], sa[p + 1]
+            if hi == lo:  # an empty bin range (repeated split point)
+                continue
             oa = offA[lo + p: hi + p + 1]
             ob = offB[lo + p: hi + p + 1]
             ka = csrA[poA[p]: poA[p + 1]]
