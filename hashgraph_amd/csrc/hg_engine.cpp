@@ -289,7 +289,8 @@ struct Level {
   std::vector<int64_t> splits;  // kind-1 segments point into this array
   uint32_t ncols_total = 0, max_cols = 0;
   uint64_t cmat_elems = 0, psum_elems = 0;
-  uint32_t tile_keys = kTile;   // keys per partition tile (<= kTile)
+  uint32_t tile_keys = kTile;   // keys per partition tile
+  uint32_t big_cap = 1;         // per-bin counters of the final stage's oversized buckets
 
   // One kind shared by all segments selects a specialised kernel.
   int hash_kind() const {
@@ -380,6 +381,8 @@ struct LevelDev {
   int64_t *splits;
   uint32_t *cmat, *psum, *coltot, *bstart;
   FinItem *fin;
+  uint32_t *big_slot, *big_cnt;
+  void *big_ctl;
 };
 
 // Lays out a level in arena 1 (growing it if needed; the stream must be
@@ -397,6 +400,14 @@ hg_status setup_level(hg_ctx c, Level &L, LevelDev &d) {
   const size_t i_coltot = P.add(std::max<uint32_t>(L.ncols_total, 1) * 4);
   const size_t i_bstart = P.add(std::max<uint32_t>(L.ncols_total, 1) * 4);
   const size_t i_fin = P.add((size_t)std::max<uint32_t>(L.ncols_total, 1) * sizeof(FinItem));
+  // per-bin counters of the oversized buckets: 256 M counters at most (1 GB),
+  // fewer when the level has fewer bins
+  uint64_t nbins_total = 0;
+  for (const SegDesc &sd : L.segs) nbins_total += sd.seg_nbins;
+  L.big_cap = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(nbins_total, 1ull << 28));
+  const size_t i_bslot = P.add((size_t)std::max<uint32_t>(L.ncols_total, 1) * 4);
+  const size_t i_bctl = P.add(final_big_ctl_bytes());
+  const size_t i_bcnt = P.add((size_t)L.big_cap * 4);
   HG_TRY(arena_ensure(c, 1, P.total()));
   Carve cv;
   cv.init(c, 1, P);
@@ -414,6 +425,9 @@ hg_status setup_level(hg_ctx c, Level &L, LevelDev &d) {
   d.coltot = cv.get<uint32_t>(i_coltot);
   d.bstart = cv.get<uint32_t>(i_bstart);
   d.fin = cv.get<FinItem>(i_fin);
+  d.big_slot = cv.get<uint32_t>(i_bslot);
+  d.big_ctl = cv.get<void>(i_bctl);
+  d.big_cnt = cv.get<uint32_t>(i_bcnt);
   for (SegDesc &sd : L.segs)
     if (sd.kind == 1) sd.splits = d.splits;
   Blob b1;
@@ -499,6 +513,10 @@ hg_status run_final(hg_ctx c, const Level &L, const LevelDev &d, int kb, int ob,
   f.off = off;
   f.items = d.fin;
   f.maps = d.maps;
+  f.big_slot = d.big_slot;
+  f.big_ctl = d.big_ctl;
+  f.big_cnt = d.big_cnt;
+  f.big_cap = L.big_cap;
   {
     Prof pf(c, "final");
     HG_HIP(launch_final(f, c->stream));
@@ -1013,7 +1031,8 @@ hg_status hg_intersect_rle(hg_ctx c, const hg_table_desc *descA, const void *d_o
   const uint64_t heavy_cap = std::min<uint64_t>(M, 1u << 22);
   Plan P;
   const size_t i_cnt = P.add((nch + 1) * 8), i_base = P.add((nch + 1) * 8), i_small = P.add(256),
-               i_heavy = P.add(heavy_cap * 4), i_status = P.add(scan_tiles(nch) * 8 + 64);
+               i_heavy = P.add(heavy_cap * 4), i_status = P.add(scan_tiles(nch) * 8 + 64),
+               i_gmark = P.add(heavy_cap * 4), i_giant = P.add(rle_giant_bytes((int)descA->key_bytes));
   HG_TRY(arena_ensure(c, 1, P.total()));
   Carve cv;
   cv.init(c, 1, P);
@@ -1039,6 +1058,16 @@ hg_status hg_intersect_rle(hg_ctx c, const hg_table_desc *descA, const void *d_o
   r.pairs = small + 1;
   r.cursor = small + 2;
   r.overflow = (unsigned int *)(small + 3);
+  {
+    // giant-bin scratch: control block, then keys, counts of A, counts of B
+    char *gb = cv.get<char>(i_giant);
+    const size_t ctl_bytes = (rle_giant_ctl_bytes() + 255) & ~(size_t)255;
+    r.gmark = cv.get<uint32_t>(i_gmark);
+    r.gctl = gb;
+    r.gkey = gb + ctl_bytes;
+    r.gca = (unsigned long long *)(gb + ctl_bytes + (((size_t)rle_giant_slots() * descA->key_bytes + 255) & ~(size_t)255));
+    r.gcb = r.gca + rle_giant_slots();
+  }
   r.pass = 0;
   {
     Prof pf(c, "rle_small");

@@ -38,6 +38,12 @@ constexpr int KPT_U32 = HG_KPT_U32;
 template <typename OFF> constexpr int probe_sub() { return sizeof(OFF) == 4 ? 8192 : 4096; }
 constexpr size_t PROBE_LDS = HG_PROBE_LDS;
 
+// Oversized buckets handled by the multi-workgroup path (per launch), their
+// chunk size in keys and the bins per LDS pass.
+constexpr uint32_t BIG_CAP = 16384;
+constexpr uint32_t BIG_CHUNK = 65536;
+constexpr uint32_t BIG_BINS = 32768;
+
 // Workgroup b of a partition pass runs on XCD b % 8 (round-robin dispatch).
 // Tiles are handed out so that each XCD walks a contiguous range of tiles:
 // the runs that concurrently running tiles write for one bucket are then
@@ -358,11 +364,12 @@ __global__ __launch_bounds__(BLOCK) void k_final(const K *__restrict__ part, K *
 template <typename K, typename OFF, int BLOCK, int SUB>
 __global__ __launch_bounds__(BLOCK) void k_final_big(const K *__restrict__ part, K *__restrict__ csr, OFF *__restrict__ off,
                                                      const FinItem *__restrict__ items, const BinMap *__restrict__ maps,
-                                                     uint32_t small_cap) {
+                                                     uint32_t small_cap, const uint32_t *__restrict__ big_slot) {
   __shared__ uint32_t s_cnt[SUB];
   __shared__ uint32_t s_tmp[BLOCK / 64 + 1];
   const FinItem it = items[blockIdx.x];
-  if (it.count <= small_cap) return;  // k_final's item
+  if (it.count <= small_cap) return;               // k_final's item
+  if (big_slot[blockIdx.x] != ~0u) return;         // the multi-workgroup path's item
   const BinMap mp = maps[it.table];
   const uint64_t cnt = it.count;
   const K *kp = part + it.key_start;
@@ -892,6 +899,256 @@ __device__ __forceinline__ uint64_t lds_cas(uint64_t *p, uint64_t cmp, uint64_t 
   return atomicCAS((unsigned long long *)p, (unsigned long long)cmp, (unsigned long long)v);
 }
 
+// Wave aggregation for the value tables: when every active lane of the wave
+// holds the same value (a hot key's bin), one lane acts for all of them with
+// the lane count; otherwise every lane acts for itself.  Returns the number
+// of values this lane stands for (0: another lane does).
+template <typename K>
+__device__ __forceinline__ uint32_t wave_same_value(K v) {
+  const uint64_t act = __ballot(1);
+  const uint32_t first = (uint32_t)__ffsll((unsigned long long)act) - 1;
+  const K v0 = __shfl(v, first, 64);
+  const uint64_t same = __ballot(v == v0);
+  if (same != act) return 1u;
+  return (threadIdx.x & 63) == first ? (uint32_t)__popcll(act) : 0u;
+}
+
+// Adds c copies of v to an LDS value -> count table (compare-and-swap of the
+// key against the EMPTY sentinel; no lane waits on another).
+template <typename K, typename C>
+__device__ __forceinline__ void vt_insert(K *s_key, C *s_cnt, uint32_t *s_fill, K v, uint32_t c) {
+  constexpr K EMPTY = (K)~(K)0;
+  uint32_t slot = murmur4((uint32_t)v ^ (uint32_t)((uint64_t)v >> 32)) & (HT_SLOTS - 1);
+  for (uint32_t probe = 0; probe < HT_SLOTS; probe++, slot = (slot + 1) & (HT_SLOTS - 1)) {
+    const K old = lds_cas(&s_key[slot], EMPTY, v);
+    if (old == EMPTY || old == v) {
+      atomicAdd(&s_cnt[slot], (C)c);
+      if (old == EMPTY) atomicAdd(s_fill, 1u);
+      return;
+    }
+  }
+}
+
+template <typename K, typename C>
+__device__ __forceinline__ void vt_add_existing(const K *s_key, C *s_cnt, K v, uint32_t c) {
+  constexpr K EMPTY = (K)~(K)0;
+  uint32_t slot = murmur4((uint32_t)v ^ (uint32_t)((uint64_t)v >> 32)) & (HT_SLOTS - 1);
+  for (uint32_t probe = 0; probe < HT_SLOTS; probe++, slot = (slot + 1) & (HT_SLOTS - 1)) {
+    const K cur = s_key[slot];
+    if (cur == EMPTY) return;
+    if (cur == v) {
+      atomicAdd(&s_cnt[slot], (C)c);
+      return;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Giant heavy bins of the run-length join (a hot key's bin: up to 10^8 keys
+// per side).  Up to GIANT_CAP of them are split into chunks of GIANT_CHUNK
+// keys over the whole grid: each chunk folds its keys into an LDS value table
+// (wave-aggregated), then merges the table into the bin's global table of
+// GT_SLOTS values (compare-and-swap on the key, atomic counts).  A's chunks
+// insert, B's chunks only add to values present in A, and one workgroup per
+// giant bin emits the records.  k_rle_heavy skips the bins taken here.
+// ---------------------------------------------------------------------------
+constexpr uint32_t GIANT_MIN = 1u << 18;
+constexpr uint32_t GIANT_CAP = 256;
+constexpr uint32_t GIANT_CHUNK = 1u << 16;
+constexpr uint32_t GT_SLOTS = 4096;
+
+struct GiantCtl {
+  uint32_t n, totalA, totalB, pad;
+  uint32_t hidx[GIANT_CAP];
+  uint32_t chA[GIANT_CAP + 1], chB[GIANT_CAP + 1];
+  unsigned long long eA[GIANT_CAP], eB[GIANT_CAP];
+};
+
+template <typename OFF>
+__global__ __launch_bounds__(256) void k_giant_mark(const OFF *__restrict__ offA, const OFF *__restrict__ offB,
+                                                    const uint32_t *__restrict__ heavy, uint64_t nheavy,
+                                                    uint32_t *__restrict__ gmark, GiantCtl *__restrict__ ctl) {
+  for (uint64_t h = (uint64_t)blockIdx.x * 256 + threadIdx.x; h < nheavy; h += (uint64_t)gridDim.x * 256) {
+    const uint64_t b = heavy[h];
+    const uint64_t sz = (uint64_t)(offA[b + 1] - offA[b]) + (uint64_t)(offB[b + 1] - offB[b]);
+    uint32_t slot = ~0u;
+    if (sz > GIANT_MIN) {
+      const uint32_t s = atomicAdd(&ctl->n, 1u);
+      if (s < GIANT_CAP) {
+        slot = s;
+        ctl->hidx[s] = (uint32_t)h;
+      }
+    }
+    gmark[h] = slot;
+  }
+}
+
+template <typename OFF>
+__global__ __launch_bounds__(64) void k_giant_prefix(const OFF *__restrict__ offA, const OFF *__restrict__ offB,
+                                                     const uint32_t *__restrict__ heavy, GiantCtl *__restrict__ ctl) {
+  if (threadIdx.x != 0) return;
+  const uint32_t n = min(ctl->n, GIANT_CAP);
+  uint32_t a = 0, b = 0;
+  for (uint32_t s = 0; s < n; s++) {
+    const uint64_t bin = heavy[ctl->hidx[s]];
+    ctl->chA[s] = a;
+    ctl->chB[s] = b;
+    a += (uint32_t)(((uint64_t)(offA[bin + 1] - offA[bin]) + GIANT_CHUNK - 1) / GIANT_CHUNK);
+    b += (uint32_t)(((uint64_t)(offB[bin + 1] - offB[bin]) + GIANT_CHUNK - 1) / GIANT_CHUNK);
+    ctl->eA[s] = 0;
+    ctl->eB[s] = 0;
+  }
+  ctl->chA[n] = a;
+  ctl->chB[n] = b;
+  ctl->totalA = a;
+  ctl->totalB = b;
+  ctl->n = n;
+}
+
+template <typename K>
+__global__ __launch_bounds__(256) void k_giant_clear(const GiantCtl *__restrict__ ctl, K *__restrict__ gkey,
+                                                     unsigned long long *__restrict__ gca,
+                                                     unsigned long long *__restrict__ gcb) {
+  const uint64_t n = (uint64_t)ctl->n * GT_SLOTS;
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) {
+    gkey[i] = (K)~(K)0;
+    gca[i] = 0;
+    gcb[i] = 0;
+  }
+}
+
+__device__ __forceinline__ uint32_t giant_slot_of(const uint32_t *ch, uint32_t n, uint32_t g) {
+  uint32_t lo = 0, hi = n;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (ch[mid] <= g) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// SIDE 0: A's chunks insert into the global tables; SIDE 1: B's chunks add
+// to the values already there.
+template <typename K, typename OFF, int SIDE>
+__global__ __launch_bounds__(1024) void k_giant_fold(const OFF *__restrict__ off, const K *__restrict__ keys,
+                                                     const uint32_t *__restrict__ heavy, GiantCtl *__restrict__ ctl,
+                                                     K *__restrict__ gkey, unsigned long long *__restrict__ gcnt,
+                                                     unsigned int *__restrict__ overflow) {
+  constexpr K EMPTY = (K)~(K)0;
+  __shared__ K s_key[HT_SLOTS];
+  __shared__ uint32_t s_cnt[HT_SLOTS];
+  __shared__ uint32_t s_fill, s_e;
+  const uint32_t n = ctl->n;
+  const uint32_t *ch = SIDE == 0 ? ctl->chA : ctl->chB;
+  const uint32_t total = SIDE == 0 ? ctl->totalA : ctl->totalB;
+  for (uint32_t g = blockIdx.x; g < total; g += gridDim.x) {
+    const uint32_t slot = giant_slot_of(ch, n, g);
+    const uint64_t bin = heavy[ctl->hidx[slot]];
+    const uint64_t b0 = (uint64_t)off[bin], b1 = (uint64_t)off[bin + 1];
+    const uint64_t c0 = b0 + (uint64_t)(g - ch[slot]) * GIANT_CHUNK;
+    const uint64_t c1 = min(b1, c0 + GIANT_CHUNK);
+    for (uint32_t i = threadIdx.x; i < HT_SLOTS; i += 1024) {
+      s_key[i] = EMPTY;
+      s_cnt[i] = 0;
+    }
+    if (threadIdx.x == 0) {
+      s_fill = 0;
+      s_e = 0;
+    }
+    __syncthreads();
+    for (uint64_t i = c0 + threadIdx.x; i < c1; i += 1024) {
+      const K v = keys[i];
+      const uint32_t c = wave_same_value(v);
+      if (c == 0) continue;
+      if (v == EMPTY) atomicAdd(&s_e, c);
+      else vt_insert(s_key, s_cnt, &s_fill, v, c);
+    }
+    __syncthreads();
+    const bool ovf = s_fill > HT_SLOTS / 2;
+    K *tk = gkey + (uint64_t)slot * GT_SLOTS;
+    unsigned long long *tc = gcnt + (uint64_t)slot * GT_SLOTS;
+    if (ovf) {
+      if (threadIdx.x == 0) atomicOr(overflow, 1u);
+    } else {
+      for (uint32_t i = threadIdx.x; i < HT_SLOTS; i += 1024) {
+        const K v = s_key[i];
+        if (v == EMPTY) continue;
+        uint32_t q = murmur4((uint32_t)v ^ (uint32_t)((uint64_t)v >> 32)) & (GT_SLOTS - 1);
+        bool done = false;
+        for (uint32_t p = 0; p < GT_SLOTS && !done; p++, q = (q + 1) & (GT_SLOTS - 1)) {
+          if (SIDE == 0) {
+            const K old = atomicCAS((typename std::conditional<sizeof(K) == 4, unsigned int, unsigned long long>::type *)&tk[q],
+                                    (typename std::conditional<sizeof(K) == 4, unsigned int, unsigned long long>::type)EMPTY,
+                                    (typename std::conditional<sizeof(K) == 4, unsigned int, unsigned long long>::type)v);
+            if (old == EMPTY || old == v) {
+              atomicAdd(&tc[q], (unsigned long long)s_cnt[i]);
+              done = true;
+            }
+          } else {
+            const K cur = tk[q];
+            if (cur == EMPTY) break;
+            if (cur == v) {
+              atomicAdd(&tc[q], (unsigned long long)s_cnt[i]);
+              done = true;
+            }
+          }
+        }
+        if (SIDE == 0 && !done) atomicOr(overflow, 1u);
+      }
+      if (threadIdx.x == 0 && s_e) atomicAdd(SIDE == 0 ? &ctl->eA[slot] : &ctl->eB[slot], (unsigned long long)s_e);
+    }
+    __syncthreads();
+  }
+}
+
+template <typename K>
+__global__ __launch_bounds__(1024) void k_giant_emit(const GiantCtl *__restrict__ ctl, const K *__restrict__ gkey,
+                                                     const unsigned long long *__restrict__ gca,
+                                                     const unsigned long long *__restrict__ gcb, K *__restrict__ rk,
+                                                     unsigned long long *__restrict__ rca,
+                                                     unsigned long long *__restrict__ rcb, uint64_t capacity,
+                                                     unsigned long long *__restrict__ cursor,
+                                                     unsigned long long *__restrict__ pairs) {
+  constexpr K EMPTY = (K)~(K)0;
+  __shared__ unsigned long long s_t64[1024 / 64 + 1], s_base;
+  for (uint32_t slot = blockIdx.x; slot < ctl->n; slot += gridDim.x) {
+    const K *tk = gkey + (uint64_t)slot * GT_SLOTS;
+    const unsigned long long *ta = gca + (uint64_t)slot * GT_SLOTS, *tb = gcb + (uint64_t)slot * GT_SLOTS;
+    for (uint32_t i0 = 0; i0 <= GT_SLOTS; i0 += 1024) {
+      const uint32_t i = i0 + threadIdx.x;
+      K v = EMPTY;
+      unsigned long long a = 0, b = 0;
+      if (i < GT_SLOTS) {
+        v = tk[i];
+        a = ta[i];
+        b = tb[i];
+        if (v == EMPTY) a = b = 0;
+      } else if (i == GT_SLOTS) {  // keys equal to the sentinel, counted apart
+        a = ctl->eA[slot];
+        b = ctl->eB[slot];
+      }
+      const bool hit = a && b;
+      unsigned long long tot;
+      const unsigned long long ex = block_exclusive_sum<1024, unsigned long long>(hit ? 1ull : 0ull, s_t64, tot);
+      const unsigned long long pt = block_sum<1024, unsigned long long>(hit ? a * b : 0ull, s_t64);
+      if (threadIdx.x == 0) {
+        s_base = tot ? atomicAdd(cursor, tot) : 0ull;
+        if (pt) atomicAdd(pairs, pt);
+      }
+      __syncthreads();
+      if (hit) {
+        const unsigned long long w = s_base + ex;
+        if (w < capacity) {
+          rk[w] = v;
+          rca[w] = a;
+          rcb[w] = b;
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
 template <typename K, typename OFF, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_rle_heavy(const OFF *__restrict__ offA, const K *__restrict__ keysA,
                                                      const OFF *__restrict__ offB, const K *__restrict__ keysB,
@@ -900,13 +1157,15 @@ __global__ __launch_bounds__(BLOCK) void k_rle_heavy(const OFF *__restrict__ off
                                                      unsigned long long *__restrict__ rcb, uint64_t capacity,
                                                      unsigned long long *__restrict__ cursor,
                                                      unsigned long long *__restrict__ pairs,
-                                                     unsigned int *__restrict__ overflow) {
+                                                     unsigned int *__restrict__ overflow,
+                                                     const uint32_t *__restrict__ gmark) {
   constexpr K EMPTY = (K)~(K)0;
   __shared__ K s_key[HT_SLOTS];
   __shared__ unsigned long long s_ca[HT_SLOTS], s_cb[HT_SLOTS];
   __shared__ unsigned int s_fill;
   __shared__ unsigned long long s_eca, s_ecb, s_base, s_t64[BLOCK / 64 + 1];
   for (uint64_t h = blockIdx.x; h < nheavy; h += gridDim.x) {
+    if (gmark[h] != ~0u) continue;  // a giant bin: k_giant_fold's
     const uint64_t b = heavy[h];
     for (uint32_t i = threadIdx.x; i < HT_SLOTS; i += BLOCK) {
       s_key[i] = EMPTY;
@@ -922,19 +1181,10 @@ __global__ __launch_bounds__(BLOCK) void k_rle_heavy(const OFF *__restrict__ off
     const uint64_t a0 = offA[b], a1 = offA[b + 1], x0 = offB[b], x1 = offB[b + 1];
     for (uint64_t i = a0 + threadIdx.x; i < a1; i += BLOCK) {
       const K v = keysA[i];
-      if (v == EMPTY) {
-        atomicAdd(&s_eca, 1ull);
-        continue;
-      }
-      uint32_t slot = murmur4((uint32_t)v ^ (uint32_t)((uint64_t)v >> 32)) & (HT_SLOTS - 1);
-      for (uint32_t probe = 0; probe < HT_SLOTS; probe++, slot = (slot + 1) & (HT_SLOTS - 1)) {
-        const K old = lds_cas(&s_key[slot], EMPTY, v);
-        if (old == EMPTY || old == v) {
-          atomicAdd(&s_ca[slot], 1ull);
-          if (old == EMPTY) atomicAdd(&s_fill, 1u);
-          break;
-        }
-      }
+      const uint32_t c = wave_same_value(v);
+      if (c == 0) continue;
+      if (v == EMPTY) atomicAdd(&s_eca, (unsigned long long)c);
+      else vt_insert(s_key, s_ca, &s_fill, v, c);
     }
     __syncthreads();
     if (s_fill > HT_SLOTS / 2) {  // too many distinct values for the table
@@ -944,19 +1194,10 @@ __global__ __launch_bounds__(BLOCK) void k_rle_heavy(const OFF *__restrict__ off
     }
     for (uint64_t i = x0 + threadIdx.x; i < x1; i += BLOCK) {
       const K v = keysB[i];
-      if (v == EMPTY) {
-        atomicAdd(&s_ecb, 1ull);
-        continue;
-      }
-      uint32_t slot = murmur4((uint32_t)v ^ (uint32_t)((uint64_t)v >> 32)) & (HT_SLOTS - 1);
-      for (uint32_t probe = 0; probe < HT_SLOTS; probe++, slot = (slot + 1) & (HT_SLOTS - 1)) {
-        const K cur = s_key[slot];
-        if (cur == EMPTY) break;
-        if (cur == v) {
-          atomicAdd(&s_cb[slot], 1ull);
-          break;
-        }
-      }
+      const uint32_t c = wave_same_value(v);
+      if (c == 0) continue;
+      if (v == EMPTY) atomicAdd(&s_ecb, (unsigned long long)c);
+      else vt_add_existing(s_key, s_cb, v, c);
     }
     __syncthreads();
     unsigned long long mine = 0, mp = 0;
@@ -1030,6 +1271,218 @@ __global__ void k_make_items(const SegDesc *__restrict__ segs, uint32_t nsegs, c
     it.end_pos = (b + 1 == sd.nbuckets) ? sd.fin_end_pos : ~0ull;
   }
   items[g] = it;
+}
+
+// ---------------------------------------------------------------------------
+// Oversized buckets (skewed keys: a hot key's bin can hold 10^8 keys).  Up to
+// BIG_CAP of them are split into chunks of BIG_CHUNK keys and spread over the
+// whole grid: per-chunk LDS bin counts are added into global per-bin
+// counters, one workgroup per bucket scans them into the offsets, and each
+// chunk then reserves its places bin by bin (one global atomic per chunk and
+// non-empty bin) and writes its keys.  A wave whose keys all fall into one
+// bin adds them with one LDS atomic.  Bins are handled in passes of
+// BIG_BINS.  Buckets beyond BIG_CAP take k_final_big.
+// ---------------------------------------------------------------------------
+struct BigCtl {
+  uint32_t nbig, total_chunks, total_bins, pad;
+  uint32_t item[BIG_CAP];
+  uint32_t chunk_start[BIG_CAP + 1];
+  uint32_t bin_base[BIG_CAP + 1];
+};
+
+// Takes the oversized items in order while their bins fit big_cap counters
+// (and at most BIG_CAP of them); prefix sums give chunk and bin bases.
+__global__ __launch_bounds__(1024) void k_big_list(const FinItem *__restrict__ items, uint32_t nitems, uint32_t small_cap,
+                                                   uint32_t big_cap, uint32_t *__restrict__ big_slot,
+                                                   BigCtl *__restrict__ ctl) {
+  __shared__ uint32_t s_tmp[1024 / 64 + 1];
+  __shared__ uint32_t s_slot, s_ch, s_bin, s_nacc, s_acc_ch, s_acc_bin;
+  if (threadIdx.x == 0) s_slot = s_ch = s_bin = s_nacc = s_acc_ch = s_acc_bin = 0;
+  __syncthreads();
+  for (uint32_t i0 = 0; i0 < nitems; i0 += 1024) {
+    const uint32_t i = i0 + threadIdx.x;
+    FinItem it{};
+    if (i < nitems) it = items[i];
+    const bool big = i < nitems && it.count > small_cap;
+    const uint32_t nch = big ? (uint32_t)((it.count + BIG_CHUNK - 1) / BIG_CHUNK) : 0u;
+    const uint32_t nbn = big ? it.nbins : 0u;
+    uint32_t t_slot, t_ch, t_bin;
+    const uint32_t e_slot = block_exclusive_sum<1024, uint32_t>(big ? 1u : 0u, s_tmp, t_slot);
+    const uint32_t e_ch = block_exclusive_sum<1024, uint32_t>(nch, s_tmp, t_ch);
+    const uint32_t e_bin = block_exclusive_sum<1024, uint32_t>(nbn, s_tmp, t_bin);
+    const uint32_t slot = s_slot + e_slot, cb = s_ch + e_ch, bb = s_bin + e_bin;
+    const bool acc = big && slot < BIG_CAP && (uint64_t)bb + nbn <= big_cap;
+    if (i < nitems) big_slot[i] = acc ? slot : ~0u;
+    if (acc) {
+      ctl->item[slot] = i;
+      ctl->chunk_start[slot] = cb;
+      ctl->bin_base[slot] = bb;
+    }
+    uint32_t a_tot, c_tot, b_tot;
+    (void)block_exclusive_sum<1024, uint32_t>(acc ? 1u : 0u, s_tmp, a_tot);
+    (void)block_exclusive_sum<1024, uint32_t>(acc ? nch : 0u, s_tmp, c_tot);
+    (void)block_exclusive_sum<1024, uint32_t>(acc ? nbn : 0u, s_tmp, b_tot);
+    if (threadIdx.x == 0) {
+      s_slot += t_slot;
+      s_ch += t_ch;
+      s_bin += t_bin;
+      s_nacc += a_tot;
+      s_acc_ch += c_tot;
+      s_acc_bin += b_tot;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const uint32_t nbig = s_nacc;
+    ctl->chunk_start[nbig] = s_acc_ch;
+    ctl->bin_base[nbig] = s_acc_bin;
+    ctl->nbig = nbig;
+    ctl->total_chunks = s_acc_ch;
+    ctl->total_bins = s_acc_bin;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_big_zero(const BigCtl *__restrict__ ctl, uint32_t *__restrict__ gcnt) {
+  const uint32_t n = ctl->total_bins;
+  for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) gcnt[i] = 0;
+}
+
+// slot of chunk g: last s with chunk_start[s] <= g
+__device__ __forceinline__ uint32_t big_slot_of(const BigCtl *ctl, uint32_t g) {
+  uint32_t lo = 0, hi = ctl->nbig;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (ctl->chunk_start[mid] <= g) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// LDS add of 1 at l for every valid lane; a wave whose valid lanes share one
+// l adds them with one atomic.  Returns each lane's old value + its rank
+// among the lanes with that l when RET (positions), else 0.
+template <bool RET>
+__device__ __forceinline__ uint32_t lds_add_agg(uint32_t *cnt, uint32_t l, bool valid) {
+  const uint64_t act = __ballot(valid);
+  if (act == 0) return 0;
+  const uint32_t first = (uint32_t)__ffsll((unsigned long long)act) - 1;
+  const uint32_t l0 = __shfl(l, first, 64);
+  const uint64_t same = __ballot(valid && l == l0);
+  const uint32_t lane = threadIdx.x & 63;
+  if (same == act) {
+    uint32_t base = 0;
+    if (lane == first) base = atomicAdd(&cnt[l0], (uint32_t)__popcll(act));
+    if (!RET) return 0;
+    base = __shfl(base, first, 64);
+    return base + (uint32_t)__popcll(act & ((1ull << lane) - 1ull));
+  }
+  uint32_t r = 0;
+  if (valid) r = atomicAdd(&cnt[l], 1u);
+  return r;
+}
+
+template <typename K, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_big_count(const K *__restrict__ part, const FinItem *__restrict__ items,
+                                                     const BinMap *__restrict__ maps, const BigCtl *__restrict__ ctl,
+                                                     uint32_t *__restrict__ gcnt) {
+  __shared__ uint32_t s_cnt[BIG_BINS];
+  const uint32_t total = ctl->total_chunks;
+  for (uint32_t g = blockIdx.x; g < total; g += gridDim.x) {
+    const uint32_t slot = big_slot_of(ctl, g);
+    const FinItem it = items[ctl->item[slot]];
+    const BinMap mp = maps[it.table];
+    const uint64_t c0 = (uint64_t)(g - ctl->chunk_start[slot]) * BIG_CHUNK;
+    const uint32_t n = (uint32_t)min((uint64_t)BIG_CHUNK, it.count - c0);
+    const K *kp = part + it.key_start + c0;
+    for (uint32_t b0 = 0; b0 < it.nbins; b0 += BIG_BINS) {
+      const uint32_t nb = min((uint32_t)BIG_BINS, it.nbins - b0);
+      for (uint32_t i = threadIdx.x; i < nb; i += BLOCK) s_cnt[i] = 0;
+      __syncthreads();
+      for (uint32_t i0 = 0; i0 < n; i0 += BLOCK) {
+        const uint32_t i = i0 + threadIdx.x;
+        uint32_t l = 0;
+        if (i < n) l = bin_of<K>(kp[i], mp) - it.bin_lo - b0;
+        lds_add_agg<false>(s_cnt, l, i < n && l < nb);
+      }
+      __syncthreads();
+      for (uint32_t i = threadIdx.x; i < nb; i += BLOCK)
+        if (s_cnt[i]) atomicAdd(&gcnt[ctl->bin_base[slot] + b0 + i], s_cnt[i]);
+      __syncthreads();
+    }
+  }
+}
+
+// One workgroup per oversized bucket: offsets from the global bin counts;
+// gcnt becomes each bin's next free place (relative to the bucket).
+template <typename OFF>
+__global__ __launch_bounds__(1024) void k_big_scan(const FinItem *__restrict__ items, const BigCtl *__restrict__ ctl,
+                                                   uint32_t *__restrict__ gcnt, OFF *__restrict__ off) {
+  __shared__ uint32_t s_tmp[1024 / 64 + 1];
+  for (uint32_t slot = blockIdx.x; slot < ctl->nbig; slot += gridDim.x) {
+    const FinItem it = items[ctl->item[slot]];
+    uint32_t *g = gcnt + ctl->bin_base[slot];
+    const uint64_t obase = it.key_start - it.key_rel;
+    uint32_t run = 0;
+    for (uint32_t l0 = 0; l0 < it.nbins; l0 += 1024) {
+      const uint32_t l = l0 + threadIdx.x;
+      const uint32_t v = l < it.nbins ? g[l] : 0u;
+      uint32_t tot;
+      const uint32_t ex = block_exclusive_sum<1024, uint32_t>(v, s_tmp, tot);
+      if (l < it.nbins) {
+        g[l] = run + ex;
+        off[it.off_pos + l] = (OFF)(obase + run + ex);
+      }
+      run += tot;
+    }
+    if (it.end_pos != ~0ull && threadIdx.x == 0) off[it.end_pos] = (OFF)(obase + it.count);
+    __syncthreads();
+  }
+}
+
+template <typename K, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_big_place(const K *__restrict__ part, K *__restrict__ csr,
+                                                     const FinItem *__restrict__ items, const BinMap *__restrict__ maps,
+                                                     const BigCtl *__restrict__ ctl, uint32_t *__restrict__ gcur) {
+  __shared__ uint32_t s_cnt[BIG_BINS];
+  const uint32_t total = ctl->total_chunks;
+  for (uint32_t g = blockIdx.x; g < total; g += gridDim.x) {
+    const uint32_t slot = big_slot_of(ctl, g);
+    const FinItem it = items[ctl->item[slot]];
+    const BinMap mp = maps[it.table];
+    const uint64_t c0 = (uint64_t)(g - ctl->chunk_start[slot]) * BIG_CHUNK;
+    const uint32_t n = (uint32_t)min((uint64_t)BIG_CHUNK, it.count - c0);
+    const K *kp = part + it.key_start + c0;
+    K *dst = csr + it.key_start;
+    for (uint32_t b0 = 0; b0 < it.nbins; b0 += BIG_BINS) {
+      const uint32_t nb = min((uint32_t)BIG_BINS, it.nbins - b0);
+      for (uint32_t i = threadIdx.x; i < nb; i += BLOCK) s_cnt[i] = 0;
+      __syncthreads();
+      for (uint32_t i0 = 0; i0 < n; i0 += BLOCK) {
+        const uint32_t i = i0 + threadIdx.x;
+        uint32_t l = 0;
+        if (i < n) l = bin_of<K>(kp[i], mp) - it.bin_lo - b0;
+        lds_add_agg<false>(s_cnt, l, i < n && l < nb);
+      }
+      __syncthreads();
+      // reserve this chunk's places, bin by bin
+      for (uint32_t i = threadIdx.x; i < nb; i += BLOCK)
+        if (s_cnt[i]) s_cnt[i] = atomicAdd(&gcur[ctl->bin_base[slot] + b0 + i], s_cnt[i]);
+      __syncthreads();
+      for (uint32_t i0 = 0; i0 < n; i0 += BLOCK) {
+        const uint32_t i = i0 + threadIdx.x;
+        K k = 0;
+        uint32_t l = 0;
+        if (i < n) {
+          k = kp[i];
+          l = bin_of<K>(k, mp) - it.bin_lo - b0;
+        }
+        const bool v = i < n && l < nb;
+        const uint32_t pos = lds_add_agg<true>(s_cnt, l, v);
+        if (v) dst[pos] = k;
+      }
+      __syncthreads();
+    }
+  }
 }
 
 // Probe of buckets too large for k_probe: one workgroup streams the
@@ -1132,19 +1585,10 @@ __global__ __launch_bounds__(BLOCK) void k_probe_heavy(const K *__restrict__ bpa
       const uint64_t a0 = (uint64_t)offA[it.off_pos + hb], a1 = (uint64_t)offA[it.off_pos + hb + 1];
       for (uint64_t q = a0 + threadIdx.x; q < a1; q += BLOCK) {
         const K v = ka[q];
-        if (v == EMPTY) {
-          atomicAdd(&s_ecnt, 1u);
-          continue;
-        }
-        uint32_t slot = murmur4((uint32_t)v ^ (uint32_t)((uint64_t)v >> 32)) & (HT_SLOTS - 1);
-        for (uint32_t pr = 0; pr < HT_SLOTS; pr++, slot = (slot + 1) & (HT_SLOTS - 1)) {
-          const K old = lds_cas(&s_key[slot], EMPTY, v);
-          if (old == EMPTY || old == v) {
-            atomicAdd(&s_cnt[slot], 1u);
-            if (old == EMPTY) atomicAdd(&s_fill, 1u);
-            break;
-          }
-        }
+        const uint32_t c = wave_same_value(v);
+        if (c == 0) continue;
+        if (v == EMPTY) atomicAdd(&s_ecnt, c);
+        else vt_insert(s_key, s_cnt, &s_fill, v, c);
       }
       __syncthreads();
       const bool ovf = s_fill > HT_SLOTS / 2;
@@ -1278,8 +1722,18 @@ static hipError_t launch_final_t(const LaunchFinal &lf, hipStream_t st) {
   if (lf.nitems) {
     constexpr int KPT = sizeof(K) == 4 ? KPT_U32 : 32;
     const uint32_t small_cap = std::min(1024u * KPT, 65535u);
+    BigCtl *ctl = (BigCtl *)lf.big_ctl;
+    hipLaunchKernelGGL(k_big_list, dim3(1), dim3(1024), 0, st, lf.items, lf.nitems, small_cap, lf.big_cap, lf.big_slot,
+                       ctl);
+    hipLaunchKernelGGL(k_big_zero, dim3(256), dim3(256), 0, st, (const BigCtl *)ctl, lf.big_cnt);
+    hipLaunchKernelGGL((k_big_count<K, 1024>), dim3(2048), dim3(1024), 0, st, (const K *)lf.part, lf.items, lf.maps,
+                       (const BigCtl *)ctl, lf.big_cnt);
+    hipLaunchKernelGGL((k_big_scan<OFF>), dim3(256), dim3(1024), 0, st, lf.items, (const BigCtl *)ctl, lf.big_cnt,
+                       (OFF *)lf.off);
+    hipLaunchKernelGGL((k_big_place<K, 1024>), dim3(2048), dim3(1024), 0, st, (const K *)lf.part, (K *)lf.csr, lf.items,
+                       lf.maps, (const BigCtl *)ctl, lf.big_cnt);
     hipLaunchKernelGGL((k_final_big<K, OFF, 1024, 16384>), dim3(lf.nitems), dim3(1024), 0, st, (const K *)lf.part,
-                       (K *)lf.csr, (OFF *)lf.off, lf.items, lf.maps, small_cap);
+                       (K *)lf.csr, (OFF *)lf.off, lf.items, lf.maps, small_cap, (const uint32_t *)lf.big_slot);
   }
   return hipGetLastError();
 }
@@ -1291,6 +1745,8 @@ hipError_t launch_final(const LaunchFinal &lf, hipStream_t st) {
 }
 
 int final_kpt(int key_bytes) { return key_bytes == 4 ? KPT_U32 : 32; }
+uint32_t final_big_items() { return BIG_CAP; }
+size_t final_big_ctl_bytes() { return sizeof(BigCtl); }
 
 // One-pass probe's bucket limits: keys of B per workgroup, keys of A in LDS.
 template <typename K> constexpr int probe1_kpt() { return sizeof(K) == 4 ? 32 : 16; }
@@ -1383,10 +1839,26 @@ static hipError_t launch_rle_t(const LaunchRle &r, hipStream_t st) {
     hipLaunchKernelGGL((k_rle_small<K, OFF, 256>), dim3(nchunks), dim3(256), 0, st, (const OFF *)r.offA,
                        (const K *)r.keysA, (const OFF *)r.offB, (const K *)r.keysB, r.M, r.pass, r.chunk_cnt,
                        r.chunk_base, (K *)r.rk, r.rca, r.rcb, r.capacity, r.heavy_n, r.heavy, r.heavy_cap, r.pairs);
-  if (r.pass == 2 && r.nheavy)
+  if (r.pass == 2 && r.nheavy) {
+    GiantCtl *ctl = (GiantCtl *)r.gctl;
+    hipLaunchKernelGGL(k_fill<uint32_t>, dim3(1), dim3(64), 0, st, (uint32_t *)&ctl->n, 0u, (uint64_t)1);
+    hipLaunchKernelGGL((k_giant_mark<OFF>), dim3((uint32_t)std::min<uint64_t>((r.nheavy + 255) / 256, 4096)), dim3(256), 0,
+                       st, (const OFF *)r.offA, (const OFF *)r.offB, (const uint32_t *)r.heavy, r.nheavy, r.gmark, ctl);
+    hipLaunchKernelGGL((k_giant_prefix<OFF>), dim3(1), dim3(64), 0, st, (const OFF *)r.offA, (const OFF *)r.offB,
+                       (const uint32_t *)r.heavy, ctl);
+    hipLaunchKernelGGL((k_giant_clear<K>), dim3(1024), dim3(256), 0, st, (const GiantCtl *)ctl, (K *)r.gkey, r.gca, r.gcb);
+    hipLaunchKernelGGL((k_giant_fold<K, OFF, 0>), dim3(1024), dim3(1024), 0, st, (const OFF *)r.offA, (const K *)r.keysA,
+                       (const uint32_t *)r.heavy, ctl, (K *)r.gkey, r.gca, r.overflow);
+    hipLaunchKernelGGL((k_giant_fold<K, OFF, 1>), dim3(1024), dim3(1024), 0, st, (const OFF *)r.offB, (const K *)r.keysB,
+                       (const uint32_t *)r.heavy, ctl, (K *)r.gkey, r.gcb, r.overflow);
+    hipLaunchKernelGGL((k_giant_emit<K>), dim3(GIANT_CAP), dim3(1024), 0, st, (const GiantCtl *)ctl, (const K *)r.gkey,
+                       (const unsigned long long *)r.gca, (const unsigned long long *)r.gcb, (K *)r.rk, r.rca, r.rcb,
+                       r.capacity, r.cursor, r.pairs);
     hipLaunchKernelGGL((k_rle_heavy<K, OFF, 1024>), dim3((uint32_t)std::min<uint64_t>(r.nheavy, 1024)), dim3(1024), 0,
                        st, (const OFF *)r.offA, (const K *)r.keysA, (const OFF *)r.offB, (const K *)r.keysB, r.heavy,
-                       r.nheavy, (K *)r.rk, r.rca, r.rcb, r.capacity, r.cursor, r.pairs, r.overflow);
+                       r.nheavy, (K *)r.rk, r.rca, r.rcb, r.capacity, r.cursor, r.pairs, r.overflow,
+                       (const uint32_t *)r.gmark);
+  }
   return hipGetLastError();
 }
 
@@ -1395,6 +1867,10 @@ hipError_t launch_rle(const LaunchRle &r, hipStream_t st) {
     return r.off_bytes == 4 ? launch_rle_t<uint32_t, uint32_t>(r, st) : launch_rle_t<uint32_t, uint64_t>(r, st);
   return r.off_bytes == 4 ? launch_rle_t<uint64_t, uint32_t>(r, st) : launch_rle_t<uint64_t, uint64_t>(r, st);
 }
+size_t rle_giant_slots() { return (size_t)GIANT_CAP * GT_SLOTS; }
+size_t rle_giant_ctl_bytes() { return sizeof(GiantCtl); }
+size_t rle_giant_bytes(int key_bytes) { return sizeof(GiantCtl) + rle_giant_slots() * (key_bytes + 16) + 1024; }
+
 uint64_t rle_chunks(uint64_t M) { return (M + RLE_CHUNK - 1) / RLE_CHUNK; }
 
 hipError_t launch_make_items(const SegDesc *segs, uint32_t nsegs, const uint32_t *bstart, const uint32_t *coltot,

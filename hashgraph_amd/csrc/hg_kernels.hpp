@@ -49,6 +49,12 @@ struct LaunchFinal {
   void *off;
   const FinItem *items;
   const BinMap *maps;
+  // oversized buckets: per-item slot (nitems), control block, big_cap per-bin
+  // counters (buckets whose bins do not fit take the single-workgroup path)
+  uint32_t *big_slot;
+  void *big_ctl;
+  uint32_t *big_cnt;
+  uint32_t big_cap;
 };
 
 struct LaunchProbe {
@@ -94,6 +100,11 @@ struct LaunchRle {
   uint64_t heavy_cap, nheavy;
   unsigned long long *cursor, *pairs;
   unsigned int *overflow;
+  // giant heavy bins: per-heavy-bin slot (heavy_cap), control block, tables
+  uint32_t *gmark;
+  void *gctl;
+  void *gkey;
+  unsigned long long *gca, *gcb;
 };
 
 struct LaunchRmi {
@@ -123,6 +134,10 @@ uint32_t rmi_moment_blocks(uint64_t n_sample);
 uint32_t rmi_sort_tiles(uint64_t n_sample);
 hipError_t launch_rle(const LaunchRle &r, hipStream_t st);
 uint64_t rle_chunks(uint64_t M);
+// scratch bytes of the giant-bin path of the run-length join
+size_t rle_giant_bytes(int key_bytes);
+size_t rle_giant_slots();
+size_t rle_giant_ctl_bytes();
 hipError_t launch_part_hist(const LaunchPart &lp, hipStream_t st);
 hipError_t launch_part_scatter(const LaunchPart &lp, hipStream_t st);
 hipError_t launch_colscan(const LaunchCol &lc, hipStream_t st);
@@ -142,6 +157,8 @@ hipError_t launch_make_items(const SegDesc *segs, uint32_t nsegs, const uint32_t
                              FinItem *items, uint32_t ncols_total, hipStream_t st);
 
 int final_kpt(int key_bytes);
+uint32_t final_big_items();
+size_t final_big_ctl_bytes();
 // keys staged per LDS round of the scatter for a level whose widest segment
 // has max_cols columns (0: the columns do not leave room for staging)
 uint32_t part_stage_cap(uint32_t max_cols, int key_bytes);

@@ -73,6 +73,23 @@ def test_heavy_duplicates(dev):
     _check_build(keys, 1 << 16, dev)
 
 
+def test_many_oversized_buckets(dev):
+    # 300 hot values x 60k copies plus uniform keys: about 300 buckets beyond a
+    # workgroup's capacity, more than the multi-workgroup path takes in one
+    # launch (256); the rest take the single-workgroup fallback
+    hot = wl.uniform_keys(8, 300, 4)
+    keys = np.concatenate([np.repeat(hot, 60000), wl.uniform_keys(9, 2000000, 4)])
+    keys = keys[np.argsort(wl.stream_np(10, len(keys)), kind="stable")]
+    _check_build(keys, 1 << 22, dev)
+
+
+def test_giant_bin_u64(dev):
+    # one value with 3M copies (a Zipf head) among 1M uniform uint64 keys
+    keys = np.concatenate([np.full(3000000, 0x123456789ABCDEF, dtype=np.uint64), wl.uniform_keys(11, 1000000, 8)])
+    keys = keys[np.argsort(wl.stream_np(12, len(keys)), kind="stable")]
+    _check_build(keys, 1 << 20, dev, ob=8)
+
+
 def test_sequential_keys(dev):
     keys = wl.sequential_keys(1 << 20, 8)
     _check_build(keys, 1 << 20, dev, ob=8)

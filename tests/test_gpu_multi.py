@@ -116,3 +116,34 @@ def test_rle_count_matches_bag_count(dev):
     offB, csrB = hg.build(_to_dev(B, dev), M)
     cnt, *_ = hg.intersect_rle(offA, csrA, offB, csrB, M)
     assert cnt == o.bag_intersect_count(A, B)
+
+
+def test_rle_giant_single_value_bin(dev):
+    # one key holds 300000 of A's and 280000 of B's keys: its bin exceeds
+    # 2^18 keys and is folded by the multi-workgroup path
+    rng = np.random.default_rng(31)
+    hot = np.uint64(0x1234_5678_9ABC_DEF0)
+    rest = wl.uniform_keys(32, 100000, 8)
+    A = np.concatenate([np.full(300000, hot, dtype=np.uint64), rest])
+    B = np.concatenate([np.full(280000, hot, dtype=np.uint64), rest[rng.integers(0, len(rest), 50000)]])
+    rng.shuffle(A)
+    rng.shuffle(B)
+    _check_rle(A, B, 1 << 16, dev)
+
+
+def test_rle_giant_bins_many_values(dev):
+    # two bins of ~400000 keys each, ~600 distinct values (with the all-ones
+    # key) spread over many chunks: per-chunk tables merged across chunks
+    rng = np.random.default_rng(33)
+    vals = np.concatenate([wl.uniform_keys(34, 600, 8), np.array([2**64 - 1], dtype=np.uint64)])
+    A = vals[rng.integers(0, len(vals), 800000)]
+    B = vals[rng.integers(0, len(vals) - 100, 500000)]
+    _check_rle(A, B, 2, dev)
+
+
+def test_rle_giant_u32(dev):
+    rng = np.random.default_rng(35)
+    vals = wl.uniform_keys(36, 300, 4)
+    A = vals[rng.integers(0, len(vals), 600000)]
+    B = np.concatenate([vals[rng.integers(0, len(vals), 200000)], wl.uniform_keys(37, 1000, 4)])
+    _check_rle(A, B, 3, dev)
