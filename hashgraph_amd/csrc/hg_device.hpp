@@ -80,6 +80,35 @@ __device__ __forceinline__ void lds_fill(T *dst, const T *src, uint32_t n) {
   }
 }
 
+// Fills dk[0..nk) from sk and dof[0..no) with so[i] - base as 16-bit values
+// (the caller guarantees they fit), with one batch of PER loads of each array
+// per thread in flight before any store.
+template <typename T, typename OFF, int BLOCK, int PER>
+__device__ __forceinline__ void lds_fill_keys_offs(T *dk, const T *sk, uint32_t nk, uint16_t *dof, const OFF *so,
+                                                   uint32_t no, OFF base) {
+  const uint32_t n = max(nk, no);
+  for (uint32_t c0 = 0; c0 < n; c0 += BLOCK * PER) {
+    const uint32_t mk = c0 < nk ? min(nk - c0, (uint32_t)(BLOCK * PER)) : 0u;
+    const uint32_t mo = c0 < no ? min(no - c0, (uint32_t)(BLOCK * PER)) : 0u;
+    const __amdgpu_buffer_rsrc_t rk = make_rsrc(sk + c0, mk * (uint32_t)sizeof(T));
+    const __amdgpu_buffer_rsrc_t ro = make_rsrc(so + c0, mo * (uint32_t)sizeof(OFF));
+    T v[PER];
+    OFF w[PER];
+#pragma unroll
+    for (int j = 0; j < PER; j++)
+      v[j] = buf_load<T>(rk, threadIdx.x * (uint32_t)sizeof(T), (uint32_t)(j * BLOCK * sizeof(T)));
+#pragma unroll
+    for (int j = 0; j < PER; j++)
+      w[j] = buf_load<OFF>(ro, threadIdx.x * (uint32_t)sizeof(OFF), (uint32_t)(j * BLOCK * sizeof(OFF)));
+#pragma unroll
+    for (int j = 0; j < PER; j++) {
+      const uint32_t q = j * BLOCK + threadIdx.x;
+      if (q < mk) dk[c0 + q] = v[j];
+      if (q < mo) dof[c0 + q] = (uint16_t)(w[j] - base);
+    }
+  }
+}
+
 // Learned CDF model (layout and rounding rules in hg_common.hpp).  The
 // products and sums must not fuse into FMAs (HIP contracts by default).
 __device__ __forceinline__ float rmi_feature(uint64_t key, uint64_t key_min, uint32_t kind) {

@@ -85,10 +85,11 @@ double build_fill() {
   return 0.875;
 }
 
-// Bucket fill of the probe: a bucket's keys of A must fit the one-pass
-// probe's LDS (39 K uint32 or 19 K uint64 keys), and its keys of B its VGPRs
-// (32 K uint32 or 16 K uint64 keys).
-double probe_fill(int kb) { return kb == 4 ? 0.875 : 0.45; }
+// Bucket fill of the probe: a bucket's keys of A and its 16-bit offsets must
+// fit the one-pass probe's 156 KB of LDS (W bins at load factor 1 take 6 W
+// bytes for uint32 keys, 10 W for uint64: 0.72 and 0.43 of 32 K bins leave
+// ~10% headroom), and its keys of B its VGPRs (32 K uint32 or 16 K uint64).
+double probe_fill(int kb) { return kb == 4 ? 0.72 : 0.43; }
 
 // Bucket width W (bins) of a table so that a bucket holds ~7/8 of what the
 // final stage keeps in VGPRs; W <= 65535 (u16 LDS counters).

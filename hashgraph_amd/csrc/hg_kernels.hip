@@ -400,32 +400,42 @@ __global__ __launch_bounds__(BLOCK) void k_final_big(const K *__restrict__ part,
 }
 
 // ---------------------------------------------------------------------------
-// One-pass probe, for buckets whose keys of A fit in LDS (all of them on
-// uniform keys).  B's keys stay in VGPRs with their 16-bit local bins; the
-// bucket's keys of A are staged in LDS once; each B key reads its bin's two
-// offsets from global memory (a window of W+1 entries that stays in the L2
-// while the workgroup runs) and walks the bin in LDS, counting matches in
-// 16-bit halves.  Slots go in groups of 8 so that their offset loads overlap,
-// and the first 4 keys of a bin are compared without a branch.  Emission as
-// in k_probe.  Heavy bins are left to k_probe_heavy.
+// One-pass probe, for buckets whose keys of A and offsets fit in LDS (all of
+// them on uniform keys).  B's keys stay in VGPRs with their 16-bit local
+// bins; the bucket's keys of A and its W+1 offsets (relative to the bucket's
+// first key, so 16 bits each) are staged in LDS together, with both arrays'
+// loads in flight at once; each B key reads its bin's two offsets and walks
+// the bin in LDS, counting matches in 16-bit halves.  The first 4 keys of a
+// bin are compared without a branch.  Emission as in k_probe.  Heavy bins
+// are left to k_probe_heavy.
 // ---------------------------------------------------------------------------
+__host__ __device__ inline uint32_t probe1_off_bytes(uint32_t W) { return ((W + 1) * 2u + 15u) & ~15u; }
+
+// Whether k_probe1 (LDS budget lds_bytes, kpb keys per workgroup) takes an
+// item; k_probe takes the others of its size class.
+__device__ __forceinline__ bool probe1_takes(uint32_t count, uint64_t na, uint32_t W, uint32_t key_bytes, uint32_t kpb,
+                                             uint32_t lds_bytes) {
+  return count <= kpb && count <= 65535u && na * key_bytes + probe1_off_bytes(W) <= lds_bytes;
+}
+
 template <typename K, typename OFF, int BLOCK, int KPT, int HK>
 __global__ __launch_bounds__(BLOCK) void k_probe1(const K *__restrict__ bpart, const FinItem *__restrict__ items,
                                                   const BinMap *__restrict__ maps, const OFF *__restrict__ offA,
                                                   const K *__restrict__ keysA, K *__restrict__ out, uint64_t capacity,
-                                                  unsigned long long *__restrict__ cursor, uint32_t akey_cap,
+                                                  unsigned long long *__restrict__ cursor, uint32_t lds_bytes,
                                                   unsigned int *__restrict__ hmark) {
   extern __shared__ __align__(16) uint8_t smem[];
   __shared__ uint32_t s_tmp[BLOCK / 64 + 1];
   __shared__ unsigned long long s_base;
-  K *s_ak = (K *)smem;
   const FinItem it = items[blockIdx.x];
-  if (it.count > BLOCK * KPT || it.count > 65535u) return;
   const uint32_t W = it.nbins;
   const OFF *ob = offA + it.off_pos;
   const uint64_t a_lo = (uint64_t)ob[0], a_hi = (uint64_t)ob[W];
-  if (a_hi - a_lo > akey_cap) return;  // k_probe's item (A staged in sub-ranges)
+  if (!probe1_takes(it.count, a_hi - a_lo, W, (uint32_t)sizeof(K), BLOCK * KPT, lds_bytes))
+    return;  // k_probe's or k_probe_big's item
   const uint32_t na = (uint32_t)(a_hi - a_lo);
+  uint16_t *s_off = (uint16_t *)smem;
+  K *s_ak = (K *)(smem + probe1_off_bytes(W));
   const BinMap mp = maps[it.table];
   const uint32_t cnt = it.count;
   const K *kp = bpart + it.key_start;
@@ -436,7 +446,7 @@ __global__ __launch_bounds__(BLOCK) void k_probe1(const K *__restrict__ bpart, c
   uint32_t lb[(KPT + 1) / 2], mc[(KPT + 1) / 2];
 #pragma unroll
   for (int j = 0; j < KPT; j++) kr[j] = buf_load<K>(rs, threadIdx.x * (uint32_t)sizeof(K), j * BLOCK * (uint32_t)sizeof(K));
-  lds_fill<K, BLOCK, sizeof(K) == 4 ? 8 : 4>(s_ak, ka + a_lo, na);
+  lds_fill_keys_offs<K, OFF, BLOCK, sizeof(K) == 4 ? 8 : 4>(s_ak, ka + a_lo, na, s_off, ob, W + 1, (OFF)a_lo);
 #pragma unroll
   for (int j = 0; j < (KPT + 1) / 2; j++) {
     lb[j] = 0;
@@ -446,37 +456,24 @@ __global__ __launch_bounds__(BLOCK) void k_probe1(const K *__restrict__ bpart, c
   for (int j = 0; j < KPT; j++)
     if (j < nv) lb[j >> 1] |= ((bin_of<K, HK>(kr[j], mp) - it.bin_lo) & 0xFFFFu) << ((j & 1) * 16);
   __syncthreads();
-  const __amdgpu_buffer_rsrc_t ro = make_rsrc(ob, (W + 1) * (uint32_t)sizeof(OFF));
-  constexpr int G = 8;
 #pragma unroll
-  for (int g = 0; g < KPT; g += G) {
-    OFF o0[G], o1[G];
-#pragma unroll
-    for (int u = 0; u < G; u++) {
-      const int j = g + u;
-      const uint32_t l = (lb[j >> 1] >> ((j & 1) * 16)) & 0xFFFFu;
-      const uint32_t vo = j < nv ? l * (uint32_t)sizeof(OFF) : 0xFFFFFFF0u;  // past the window: loads 0
-      o0[u] = buf_load<OFF>(ro, vo, 0);
-      o1[u] = buf_load<OFF>(ro, vo, (uint32_t)sizeof(OFF));
+  for (int j = 0; j < KPT; j++) {
+    const uint32_t l = (lb[j >> 1] >> ((j & 1) * 16)) & 0xFFFFu;  // < W: the key's bin is in this bucket
+    if (j >= nv || l >= W) continue;
+    const uint32_t b0 = s_off[l];
+    uint32_t len = s_off[l + 1] - b0;
+    if (len > PROBE_BIN_LIMIT) {
+      hmark[blockIdx.x] = 1u;  // k_probe_heavy answers this bin
+      len = 0;
     }
+    uint32_t m = 0;
 #pragma unroll
-    for (int u = 0; u < G; u++) {
-      const int j = g + u;
-      uint32_t len = (uint32_t)(o1[u] - o0[u]);
-      const uint32_t b0 = (uint32_t)((uint64_t)o0[u] - a_lo);
-      if (len > PROBE_BIN_LIMIT) {
-        hmark[blockIdx.x] = 1u;  // k_probe_heavy answers this bin
-        len = 0;
-      }
-      uint32_t m = 0;
-#pragma unroll
-      for (uint32_t q = 0; q < 4; q++) {
-        const K v = s_ak[q < len ? b0 + q : 0u];
-        m += (q < len && v == kr[j]) ? 1u : 0u;
-      }
-      for (uint32_t q = 4; q < len; q++) m += s_ak[b0 + q] == kr[j];
-      mc[j >> 1] += m << ((j & 1) * 16);  // m <= PROBE_BIN_LIMIT: no carry into the other half
+    for (uint32_t q = 0; q < 4; q++) {
+      const K v = s_ak[q < len ? b0 + q : 0u];
+      m += (q < len && v == kr[j]) ? 1u : 0u;
     }
+    for (uint32_t q = 4; q < len; q++) m += s_ak[b0 + q] == kr[j];
+    mc[j >> 1] += m << ((j & 1) * 16);  // m <= PROBE_BIN_LIMIT: no carry into the other half
   }
   uint32_t mine = 0;
 #pragma unroll
@@ -512,7 +509,7 @@ __global__ __launch_bounds__(BLOCK) void k_probe(const K *__restrict__ bpart, co
                                                  const K *__restrict__ keysA, K *__restrict__ out, uint64_t capacity,
                                                  unsigned long long *__restrict__ cursor, uint32_t akey_cap,
                                                  unsigned int *__restrict__ hmark, uint32_t one_cnt,
-                                                 uint32_t one_cap) {
+                                                 uint32_t one_lds) {
   extern __shared__ __align__(16) uint8_t smem[];
   __shared__ uint32_t s_tmp[BLOCK / 64 + 1];
   __shared__ uint32_t s_n;
@@ -521,7 +518,8 @@ __global__ __launch_bounds__(BLOCK) void k_probe(const K *__restrict__ bpart, co
   K *s_ak = (K *)(smem + ((SUB + 2) * sizeof(OFF) + 15) / 16 * 16);
   const FinItem it = items[blockIdx.x];
   if (it.count > BLOCK * KPT || it.count > 65535u) return;  // k_probe_big's item
-  if (it.count <= one_cnt && (uint64_t)(offA[it.off_pos + it.nbins] - offA[it.off_pos]) <= one_cap)
+  if (probe1_takes(it.count, (uint64_t)(offA[it.off_pos + it.nbins] - offA[it.off_pos]), it.nbins,
+                   (uint32_t)sizeof(K), one_cnt, one_lds))
     return;  // k_probe1's item
   const BinMap mp = maps[it.table];
   const uint32_t cnt = it.count, W = it.nbins;
@@ -1750,23 +1748,22 @@ size_t final_big_ctl_bytes() { return sizeof(BigCtl); }
 
 // One-pass probe's bucket limits: keys of B per workgroup, keys of A in LDS.
 template <typename K> constexpr int probe1_kpt() { return sizeof(K) == 4 ? 32 : 16; }
-template <typename K> constexpr uint32_t probe1_cap() { return (uint32_t)(156 * 1024 / sizeof(K)); }
+constexpr uint32_t PROBE1_LDS = 156 * 1024;  // keys of A + 16-bit offsets
 
 template <typename K, typename OFF>
 static hipError_t launch_probe_t(const LaunchProbe &p, hipStream_t st) {
   constexpr int KPT = sizeof(K) == 4 ? KPT_U32 : 32;
   constexpr int KPT1 = probe1_kpt<K>();
-  constexpr uint32_t CAP1 = probe1_cap<K>();
   constexpr int SUB = probe_sub<OFF>();
   if (p.akey_cap < PROBE_BIN_LIMIT) return hipErrorInvalidValue;  // a walkable bin must fit in LDS
   const size_t lds = ((SUB + 2) * sizeof(OFF) + 15) / 16 * 16 + (size_t)p.akey_cap * sizeof(K);
   const hipError_t e = dispatch_hk(p.hk, [&](auto h) {
     hipLaunchKernelGGL((k_probe1<K, OFF, 1024, KPT1, decltype(h)::value>), dim3(p.nitems), dim3(1024),
-                       (size_t)CAP1 * sizeof(K), st, (const K *)p.bpart, p.items, p.maps, (const OFF *)p.offA,
-                       (const K *)p.keysA, (K *)p.out, p.capacity, p.cursor, CAP1, p.hmark);
+                       (size_t)PROBE1_LDS, st, (const K *)p.bpart, p.items, p.maps, (const OFF *)p.offA,
+                       (const K *)p.keysA, (K *)p.out, p.capacity, p.cursor, PROBE1_LDS, p.hmark);
     hipLaunchKernelGGL((k_probe<K, OFF, 1024, KPT, SUB, decltype(h)::value>), dim3(p.nitems), dim3(1024), lds, st,
                        (const K *)p.bpart, p.items, p.maps, (const OFF *)p.offA, (const K *)p.keysA, (K *)p.out,
-                       p.capacity, p.cursor, p.akey_cap, p.hmark, (uint32_t)(1024 * KPT1), CAP1);
+                       p.capacity, p.cursor, p.akey_cap, p.hmark, (uint32_t)(1024 * KPT1), PROBE1_LDS);
     return hipGetLastError();
   });
   if (e != hipSuccess) return e;
