@@ -118,28 +118,136 @@ __device__ __forceinline__ float rmi_feature(uint64_t key, uint64_t key_min, uin
   return (float)__float_as_uint(__fadd_rn(xr, 1.0f));
 }
 
+// v_cvt_u32_f32 saturates: NaN and values <= 0 give 0, values >= 2^32 give
+// 0xFFFFFFFF, the rest truncate.
+__device__ __forceinline__ uint32_t cvt_u32_sat(float x) {
+  uint32_t r;
+  asm("v_cvt_u32_f32 %0, %1" : "=v"(r) : "v"(x));
+  return r;
+}
+
+// Leaf of a root value t: 0 for NaN and t <= 0, L-1 from L-1 on, else
+// trunc(t) (the same as clamping in float while L-1 < 2^24).
 __device__ __forceinline__ uint32_t rmi_leaf_at(float feature, float slope, float icpt, uint32_t L) {
 #pragma clang fp contract(off)
   const float t = __fadd_rn(__fmul_rn(slope, feature), icpt);
-  return !(t > 0.0f) ? 0u : (t >= (float)(L - 1) ? L - 1 : (uint32_t)t);
+  return min(cvt_u32_sat(t), L - 1);
 }
 
 __device__ __forceinline__ uint32_t rmi_leaf(uint64_t key, const RmiHeader &h) {
   return rmi_leaf_at(rmi_feature(key, h.key_min, h.root_kind), h.root_slope, h.root_icpt, h.n_leaves);
 }
 
-__device__ __forceinline__ uint32_t rmi_bin(uint64_t key, const void *model, uint32_t M) {
+// Bin of a key routed to leaf l (M bins).
+__device__ __forceinline__ uint32_t rmi_leaf_bin(uint64_t key, const RmiLeaf &l, uint32_t M) {
 #pragma clang fp contract(off)
-  const RmiHeader *h = (const RmiHeader *)model;
-  const RmiLeaf *lv = (const RmiLeaf *)(h + 1);
-  const RmiLeaf l = lv[rmi_leaf(key, *h)];
   const bool up = key >= l.x0;
   const float d = up ? (float)(key - l.x0) : (float)(l.x0 - key);
   const float y = __fmul_rn(l.slope, d);
-  const int64_t off = !(y > 0.0f) ? 0 : (y >= 4294967040.0f ? 0xFFFFFF00ll : (int64_t)(uint32_t)y);
-  int64_t b = (int64_t)l.anchor + (up ? off : -off);
-  b = b < 0 ? 0 : (b > (int64_t)M - 1 ? (int64_t)M - 1 : b);
-  return (uint32_t)b;
+  // offset: 0 for NaN and y <= 0, trunc(y) up to 0xFFFFFF00 (the largest
+  // float below 2^32); the bin anchor +- offset clamped to [0, M-1] with
+  // saturating 32-bit arithmetic
+  const uint32_t off = min(cvt_u32_sat(y), 0xFFFFFF00u);
+  const uint32_t b = up ? __builtin_elementwise_add_sat(l.anchor, off) : __builtin_elementwise_sub_sat(l.anchor, off);
+  return min(b, M - 1);
+}
+
+__device__ __forceinline__ uint32_t rmi_bin(uint64_t key, const void *model, uint32_t M) {
+  const RmiHeader *h = (const RmiHeader *)model;
+  return rmi_leaf_bin(key, ((const RmiLeaf *)(h + 1))[rmi_leaf(key, *h)], M);
+}
+
+// Learned-model bins from LDS windows of leaves, for kernels whose keys sit
+// in registers (kr[0..KPT), slots below nv valid).  A window holds leaves
+// [w0, w0 + n); rmi_window_bins calls f(j, bin) for the slots whose leaf is
+// in it.  One code path per slot whatever the model's size: a workgroup walks
+// as many windows as its keys' leaf range needs (one on most workgroups).
+template <typename K, int KPT, typename F>
+__device__ __forceinline__ void rmi_window_bins(const K *kr, uint32_t nv, const RmiHeader &h, const RmiLeaf *s_leaf,
+                                                uint32_t w0, uint32_t n, uint32_t M, F &&f) {
+#pragma unroll
+  for (int j = 0; j < KPT; j++)
+    if (j < (int)nv) {
+      uint64_t k = (uint64_t)kr[j];
+      asm volatile("" : "+v"(k));  // recomputed per window: nothing is hoisted into registers across windows
+      const uint32_t leaf = rmi_leaf(k, h);
+      if (leaf - w0 < n) f(j, rmi_leaf_bin(k, s_leaf[leaf - w0], M));
+    }
+}
+
+// Copies leaves [w0, w0 + n) of a model to s_leaf (no barrier).
+template <int BLOCK>
+__device__ __forceinline__ void rmi_copy_leaves(const void *model, RmiLeaf *s_leaf, uint32_t w0, uint32_t n) {
+  const uint4 *src = (const uint4 *)((const RmiHeader *)model + 1) + w0;
+  for (uint32_t i = threadIdx.x; i < n; i += BLOCK) ((uint4 *)s_leaf)[i] = src[i];
+}
+
+// Leaf range of a workgroup's keys (kr[0..KPT), slots below nv valid): the
+// root is monotone in the key, so every key's leaf lies between the leaves of
+// the smallest and the largest key.  Loads the header into h; [l0, l1] is
+// empty (l0 > l1) when no slot is valid.  Ends with a barrier.
+template <typename K, int BLOCK, int KPT>
+__device__ __forceinline__ void rmi_leaf_range(const K *kr, uint32_t nv, const void *model, RmiHeader &h,
+                                               uint32_t &l0, uint32_t &l1) {
+  __shared__ uint64_t s_mn[BLOCK / 64], s_mx[BLOCK / 64];
+  __shared__ uint32_t s_rng[2];
+  h = *(const RmiHeader *)model;
+  uint64_t mn = ~0ull, mx = 0;
+#pragma unroll
+  for (int j = 0; j < KPT; j++)
+    if (j < (int)nv) {
+      const uint64_t k = (uint64_t)kr[j];
+      mn = k < mn ? k : mn;
+      mx = k > mx ? k : mx;
+    }
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) {
+    const uint64_t a = __shfl_xor(mn, d, 64), b = __shfl_xor(mx, d, 64);
+    mn = a < mn ? a : mn;
+    mx = b > mx ? b : mx;
+  }
+  if ((threadIdx.x & 63) == 0) {
+    s_mn[threadIdx.x >> 6] = mn;
+    s_mx[threadIdx.x >> 6] = mx;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < BLOCK / 64; w++) {
+      mn = s_mn[w] < mn ? s_mn[w] : mn;
+      mx = s_mx[w] > mx ? s_mx[w] : mx;
+    }
+    uint32_t a = 1, b = 0;  // empty
+    if (mn <= mx) {
+      const uint32_t x = rmi_leaf(mn, h), y = rmi_leaf(mx, h);
+      a = x < y ? x : y;
+      b = x < y ? y : x;
+    }
+    s_rng[0] = a;
+    s_rng[1] = b;
+  }
+  __syncthreads();
+  l0 = s_rng[0];
+  l1 = s_rng[1];
+}
+
+// Local bins (bin - bin_lo, 16 bits, OR-ed into the packed lb) of a
+// workgroup's keys under a learned model, through LDS windows of cap leaves.
+template <typename K, int BLOCK, int KPT>
+__device__ __forceinline__ void rmi_local_bins(const K *kr, uint32_t nv, uint32_t *lb, const void *model,
+                                               RmiLeaf *s_leaf, uint32_t cap, uint32_t bin_lo, uint32_t M) {
+  RmiHeader h;
+  uint32_t l0, l1;
+  rmi_leaf_range<K, BLOCK, KPT>(kr, nv, model, h, l0, l1);
+  for (uint32_t w0 = l0; w0 <= l1; w0 += cap) {
+    const uint32_t n = min(cap, l1 - w0 + 1);
+    rmi_copy_leaves<BLOCK>(model, s_leaf, w0, n);
+    __syncthreads();
+    rmi_window_bins<K, KPT>(kr, nv, h, s_leaf, w0, n, M, [&](int j, uint32_t bin) {
+      lb[j >> 1] |= ((bin - bin_lo) & 0xFFFFu) << ((j & 1) * 16);
+    });
+    __syncthreads();
+    if (l1 - w0 < cap) break;  // last window (w0 + cap may wrap)
+  }
 }
 
 // Hash kinds a kernel is compiled for: only the selected path is inlined
@@ -158,13 +266,17 @@ __device__ __forceinline__ uint32_t bin_of(K key, const BinMap &m) {
 // split points, SK_ANY = chosen per segment at run time.
 enum : int { SK_RANGE = 0, SK_SPLIT = 1, SK_ANY = 2 };
 
-template <typename K, int HK = HK_ANY, int SK = SK_ANY>
-__device__ __forceinline__ uint32_t bucket_of(K key, const SegDesc &sd) {
-  uint32_t bin = bin_of<K, HK>(key, sd.map);
+template <int SK = SK_ANY>
+__device__ __forceinline__ uint32_t bucket_of_bin(uint32_t bin, const SegDesc &sd) {
   if (SK == SK_RANGE || (SK == SK_ANY && sd.kind == 0)) return divq(bin - (uint32_t)sd.bin_lo, sd.div);
   uint32_t p = 0;
   while (p < sd.nsplits && (int64_t)bin >= sd.splits[p + 1]) p++;
   return p;  // p == nsplits: past the last split
+}
+
+template <typename K, int HK = HK_ANY, int SK = SK_ANY>
+__device__ __forceinline__ uint32_t bucket_of(K key, const SegDesc &sd) {
+  return bucket_of_bin<SK>(bin_of<K, HK>(key, sd.map), sd);
 }
 
 // Inclusive wave64 scan of a 32/64-bit value.

@@ -21,6 +21,10 @@ namespace hg {
 // most this many key pairs; larger bins (heavy duplicates) are reported to
 // the host, which answers them with the run-length join instead.
 constexpr uint64_t PROBE_BIN_LIMIT = 4096;
+// Leaves of a learned model a bucket-local kernel stages in LDS (the range
+// between its smallest and largest key's leaves; wider ranges are read from
+// global memory).
+constexpr uint32_t RANGE_LEAVES = 512;
 constexpr uint64_t JOIN_PAIR_LIMIT = 1ull << 22;
 
 // Keys per thread that the final stage and the probe keep in VGPRs (u32
@@ -60,8 +64,9 @@ __device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t nt) {
 // ---------------------------------------------------------------------------
 template <typename K, int BLOCK, int KPT, int HK, int SK>
 __global__ __launch_bounds__(BLOCK) void k_part_hist(const K *__restrict__ keys, const TileDesc *__restrict__ tiles,
-                                                     const SegDesc *__restrict__ segs, uint32_t *__restrict__ cmat) {
-  extern __shared__ uint32_t s_cnt[];
+                                                     const SegDesc *__restrict__ segs, uint32_t *__restrict__ cmat,
+                                                     uint32_t leaf_cap) {
+  extern __shared__ __align__(16) uint32_t s_cnt[];
   const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
   const TileDesc td = tiles[tile];
   const SegDesc sd = segs[td.seg];  // a copy: uniform fields stay in SGPRs
@@ -70,18 +75,45 @@ __global__ __launch_bounds__(BLOCK) void k_part_hist(const K *__restrict__ keys,
   __syncthreads();
   // a tile is processed in rounds of BLOCK*KPT keys (slot j of a round holds
   // key j*BLOCK + tid of the round)
-  for (uint32_t base = 0; base < td.count; base += BLOCK * KPT) {
-    const uint32_t cnt = min((uint32_t)(BLOCK * KPT), td.count - base);
-    const K *kp = keys + td.key_start + base;
-    const uint32_t nv = cnt > threadIdx.x ? (cnt - threadIdx.x + BLOCK - 1) / BLOCK : 0;
-    const __amdgpu_buffer_rsrc_t rs = make_rsrc(kp, cnt * (uint32_t)sizeof(K));
-    K kr[KPT];
+  if constexpr (HK == HK_RMI) {
+    // learned model: its leaves are read from LDS, in windows of leaf_cap
+    // (one window for models up to leaf_cap leaves; each further window
+    // reads the tile again)
+    RmiLeaf *s_leaf = (RmiLeaf *)(s_cnt + ((ncols + 3) & ~3u));
+    const RmiHeader h = *(const RmiHeader *)sd.map.rmi;
+    for (uint32_t w0 = 0; w0 < h.n_leaves; w0 += leaf_cap) {
+      const uint32_t n = min(leaf_cap, h.n_leaves - w0);
+      if (w0) __syncthreads();  // the previous window's readers are done
+      rmi_copy_leaves<BLOCK>(sd.map.rmi, s_leaf, w0, n);
+      __syncthreads();
+      for (uint32_t base = 0; base < td.count; base += BLOCK * KPT) {
+        const uint32_t cnt = min((uint32_t)(BLOCK * KPT), td.count - base);
+        const K *kp = keys + td.key_start + base;
+        const uint32_t nv = cnt > threadIdx.x ? (cnt - threadIdx.x + BLOCK - 1) / BLOCK : 0;
+        const __amdgpu_buffer_rsrc_t rs = make_rsrc(kp, cnt * (uint32_t)sizeof(K));
+        K kr[KPT];
 #pragma unroll
-    for (int j = 0; j < KPT; j++)
-      kr[j] = buf_load<K>(rs, threadIdx.x * (uint32_t)sizeof(K), j * BLOCK * (uint32_t)sizeof(K));
+        for (int j = 0; j < KPT; j++)
+          kr[j] = buf_load<K>(rs, threadIdx.x * (uint32_t)sizeof(K), j * BLOCK * (uint32_t)sizeof(K));
+        rmi_window_bins<K, KPT>(kr, nv, h, s_leaf, w0, n, sd.map.M, [&](int, uint32_t bin) {
+          atomicAdd(&s_cnt[bucket_of_bin<SK>(bin, sd)], 1u);
+        });
+      }
+    }
+  } else {
+    for (uint32_t base = 0; base < td.count; base += BLOCK * KPT) {
+      const uint32_t cnt = min((uint32_t)(BLOCK * KPT), td.count - base);
+      const K *kp = keys + td.key_start + base;
+      const uint32_t nv = cnt > threadIdx.x ? (cnt - threadIdx.x + BLOCK - 1) / BLOCK : 0;
+      const __amdgpu_buffer_rsrc_t rs = make_rsrc(kp, cnt * (uint32_t)sizeof(K));
+      K kr[KPT];
 #pragma unroll
-    for (int j = 0; j < KPT; j++)
-      if (j < nv) atomicAdd(&s_cnt[bucket_of<K, HK, SK>(kr[j], sd)], 1u);
+      for (int j = 0; j < KPT; j++)
+        kr[j] = buf_load<K>(rs, threadIdx.x * (uint32_t)sizeof(K), j * BLOCK * (uint32_t)sizeof(K));
+#pragma unroll
+      for (int j = 0; j < KPT; j++)
+        if (j < nv) atomicAdd(&s_cnt[bucket_of<K, HK, SK>(kr[j], sd)], 1u);
+    }
   }
   __syncthreads();
   uint32_t *row = cmat + sd.row_base + (uint64_t)(tile - sd.tile_begin) * ncols;
@@ -285,9 +317,19 @@ __global__ __launch_bounds__(BLOCK) void k_final(const K *__restrict__ part, K *
   for (int j = 0; j < KPT; j++) kr[j] = buf_load<K>(rs, threadIdx.x * (uint32_t)sizeof(K), j * BLOCK * (uint32_t)sizeof(K));
 #pragma unroll
   for (int j = 0; j < (KPT + 1) / 2; j++) lb[j] = 0;
+  // a learned model: bins from the leaves this bucket's keys reach, copied
+  // to LDS (one or a few leaves on most buckets)
+  __shared__ RmiLeaf s_leaf[HK == HK_RMI ? RANGE_LEAVES : 1];
+  if (HK == HK_RMI)
+    rmi_local_bins<K, BLOCK, KPT>(kr, nv, lb, mp.rmi, s_leaf, RANGE_LEAVES, (uint32_t)it.bin_lo, mp.M);
   auto count_slot = [&](int j) {
-    const uint32_t l = bin_of<K, HK>(kr[j], mp) - it.bin_lo;
-    lb[j >> 1] |= l << ((j & 1) * 16);
+    uint32_t l;
+    if (HK == HK_RMI) {
+      l = (lb[j >> 1] >> ((j & 1) * 16)) & 0xFFFFu;
+    } else {
+      l = bin_of<K, HK>(kr[j], mp) - (uint32_t)it.bin_lo;
+      lb[j >> 1] |= l << ((j & 1) * 16);
+    }
     atomicAdd(&smem32[l >> 1], 1u << ((l & 1) * 16));
   };
 #pragma unroll
@@ -410,12 +452,14 @@ __global__ __launch_bounds__(BLOCK) void k_final_big(const K *__restrict__ part,
 // are left to k_probe_heavy.
 // ---------------------------------------------------------------------------
 __host__ __device__ inline uint32_t probe1_off_bytes(uint32_t W) { return ((W + 1) * 2u + 15u) & ~15u; }
+constexpr uint32_t PROBE1_LEAF_BYTES = 1024;  // room for 64 leaves of a learned model
 
 // Whether k_probe1 (LDS budget lds_bytes, kpb keys per workgroup) takes an
 // item; k_probe takes the others of its size class.
 __device__ __forceinline__ bool probe1_takes(uint32_t count, uint64_t na, uint32_t W, uint32_t key_bytes, uint32_t kpb,
                                              uint32_t lds_bytes) {
-  return count <= kpb && count <= 65535u && na * key_bytes + probe1_off_bytes(W) <= lds_bytes;
+  return count <= kpb && count <= 65535u &&
+         na * key_bytes + 16 + probe1_off_bytes(W) + PROBE1_LEAF_BYTES <= lds_bytes;
 }
 
 template <typename K, typename OFF, int BLOCK, int KPT, int HK>
@@ -452,9 +496,18 @@ __global__ __launch_bounds__(BLOCK) void k_probe1(const K *__restrict__ bpart, c
     lb[j] = 0;
     mc[j] = 0;
   }
+  if (HK == HK_RMI) {
+    // a learned model: bins from the leaves B's keys reach, copied to the
+    // LDS after A's keys (probe1_takes leaves at least PROBE1_LEAF_BYTES)
+    const uint32_t used = probe1_off_bytes(W) + ((na * (uint32_t)sizeof(K) + 15u) & ~15u);
+    RmiLeaf *s_leaf = (RmiLeaf *)(smem + used);
+    const uint32_t cap = min((lds_bytes - used) / (uint32_t)sizeof(RmiLeaf), RANGE_LEAVES);
+    rmi_local_bins<K, BLOCK, KPT>(kr, nv, lb, mp.rmi, s_leaf, cap, (uint32_t)it.bin_lo, mp.M);
+  } else {
 #pragma unroll
-  for (int j = 0; j < KPT; j++)
-    if (j < nv) lb[j >> 1] |= ((bin_of<K, HK>(kr[j], mp) - it.bin_lo) & 0xFFFFu) << ((j & 1) * 16);
+    for (int j = 0; j < KPT; j++)
+      if (j < nv) lb[j >> 1] |= ((bin_of<K, HK>(kr[j], mp) - it.bin_lo) & 0xFFFFu) << ((j & 1) * 16);
+  }
   __syncthreads();
 #pragma unroll
   for (int j = 0; j < KPT; j++) {
@@ -1657,12 +1710,17 @@ template <typename F> static hipError_t dispatch_part(int hk, int sk, F &&f) {
 
 constexpr int PART_KPT = 32;  // tile = 1024 * 32 keys (the engine's tiling)
 
+// Leaves of a learned model the histogram stages in LDS (64 KB; larger
+// models are read from global memory).
+constexpr uint32_t HIST_LEAF_CAP = 4096;
+
 template <typename K>
 static hipError_t launch_part_hist_t(const LaunchPart &lp, hipStream_t st) {
-  const size_t lds = (size_t)lp.max_cols * 4;
+  const uint32_t leaf_cap = lp.hk == HK_RMI ? HIST_LEAF_CAP : 0u;
+  const size_t lds = (size_t)((lp.max_cols + 3) & ~3u) * 4 + (size_t)leaf_cap * sizeof(RmiLeaf);
   return dispatch_part(lp.hk, lp.sk, [&](auto h, auto b) {
     hipLaunchKernelGGL((k_part_hist<K, 1024, PART_KPT, decltype(h)::value, decltype(b)::value>), dim3(lp.ntiles),
-                       dim3(1024), lds, st, (const K *)lp.keys, lp.tiles, lp.segs, lp.cmat);
+                       dim3(1024), lds, st, (const K *)lp.keys, lp.tiles, lp.segs, lp.cmat, leaf_cap);
     return hipGetLastError();
   });
 }

@@ -55,6 +55,29 @@ def test_table_bins_match_oracle_u64(dev):
     assert np.array_equal(got, o.bins_rmi(keys, mlog, M))
 
 
+def test_table_bins_extremes_match_oracle(dev):
+    """Model values far outside their ranges: huge and negative slopes,
+    offsets beyond 2^32, anchors at M-1, a negative root slope and a root that
+    overflows to infinity; the evaluation's clamps must equal the oracle's."""
+    import hashgraph_amd as hg
+    rng = np.random.default_rng(8)
+    L, M = 16, (1 << 32) - 5
+    x0 = np.sort(rng.integers(0, 1 << 64, L, dtype=np.uint64))
+    slopes = [1e30, -3.0, 1e-3, 2.5e9, 0.0, 1e-19, 7.0, -1e30]
+    leaves = [(int(x0[i]), slopes[i % 8], int(rng.integers(0, M))) for i in range(L)]
+    leaves[3] = (int(x0[3]), 1e30, M - 1)
+    leaves[5] = (int(x0[5]), 1e30, 0)
+    models = [o.rmi_model(int(x0[0]), -L / 2.0 ** 64, L + 0.5, leaves, M),
+              o.rmi_model(0, 3e38, -1e38, leaves, M, root_kind=1),
+              o.rmi_model(int(x0[2]), 2.0 ** -60, -1.5, leaves, M)]
+    keys = np.concatenate([rng.integers(0, 1 << 64, 1 << 16, dtype=np.uint64), x0, x0 - np.uint64(1),
+                           x0 + np.uint64(1), np.array([0, (1 << 64) - 1], dtype=np.uint64)])
+    print("table_bins rmi extremes", flush=True)
+    for m in models:
+        got = hg.table_bins(_to_dev(keys, dev), M, hg.HG_HASH_RMI, _model_dev(m, dev)).cpu().numpy().view(np.uint32)
+        assert np.array_equal(got, o.bins_rmi(keys, m, M))
+
+
 def test_table_bins_match_oracle_u32(dev):
     import hashgraph_amd as hg
     M = 1000
