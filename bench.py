@@ -213,7 +213,10 @@ def c3_run(dev, steps, warmup):
             ob = offB[lo + p: hi + p + 1]
             ka = csrA[poA[p]: poA[p + 1]]
             kb = csrB[poB[p]: poB[p + 1]]
-            cnt, rk, ca, cb = hg.intersect_rle(oa, ka, ob, kb, hi - lo)
+            try:  # one pass when the records fit the guess, else count first
+                cnt, rk, ca, cb = hg.intersect_rle(oa, ka, ob, kb, hi - lo, capacity=1 << 23)
+            except hg.HGError:
+                cnt, rk, ca, cb = hg.intersect_rle(oa, ka, ob, kb, hi - lo)
             total += cnt
             recs += rk.numel()
         return total, recs

@@ -852,11 +852,86 @@ __global__ __launch_bounds__(1024) void k_range_hist(const K *__restrict__ keys,
 // present in both tables.  Small bins (|A_b|, |B_b| <= RLE_SMALL) are done by
 // one thread each (quadratic in the bin, in bin order); a bin above that is
 // appended to a heavy list and handled by k_rle_heavy.
-// Pass 0 counts records per workgroup chunk of RLE_CHUNK bins, pass 1 writes
-// them at the chunk's exclusive prefix.
+// A workgroup takes a chunk of RLE_CHUNK bins: both sides' offsets, and their
+// keys when the chunk holds at most RLE_KCAP keys per side (no heavy bin),
+// are read coalesced into LDS first; thread t then walks bins t*PER.. there.
+// Pass 0 counts records per chunk, pass 1 writes them at the chunk's
+// exclusive prefix.
 // ---------------------------------------------------------------------------
 constexpr uint32_t RLE_SMALL = 64;
 constexpr uint32_t RLE_CHUNK = 1024;
+constexpr uint32_t RLE_KCAP = 2048;
+constexpr uint32_t RLE_TINY = 8;  // bins above this on a side (up to RLE_SMALL) are walked by a wave
+
+// A medium bin (A's keys a0.., a of them; B's x0.., b of them; a, b <= 64)
+// walked by one wave: lane i takes A's i-th key and counts its copies on both
+// sides by broadcast reads; it holds a record if the key occurs first at i
+// and B has it.
+template <typename K, typename GA, typename GB>
+__device__ __forceinline__ bool rle_medium(uint64_t a0, uint32_t a, uint64_t x0, uint32_t b, GA getA, GB getB,
+                                           uint32_t lane, K &v, uint32_t &ca, uint32_t &cb) {
+  const bool mine = lane < a;
+  v = getA(a0 + (mine ? lane : 0u));
+  ca = 0;
+  cb = 0;
+  bool first = true;
+  for (uint32_t j = 0; j < a; j++) {
+    const bool eq = getA(a0 + j) == v;
+    ca += eq ? 1u : 0u;
+    first &= !(eq && j < lane);
+  }
+  for (uint32_t j = 0; j < b; j++) cb += getB(x0 + j) == v ? 1u : 0u;
+  return mine && first && cb > 0;
+}
+
+// pass 0 counts (and lists heavy bins when heavy_n is non-null), pass 1
+// writes from w on
+template <typename K, int PER, typename OFF, typename GA, typename GB>
+__device__ __forceinline__ void rle_walk(const OFF *s_oa, const OFF *s_ob, uint32_t nb, int pass, GA getA, GB getB,
+                                         uint64_t b0, unsigned long long &nrec, unsigned long long &npair,
+                                         unsigned long long *heavy_n, uint32_t *heavy, uint64_t heavy_cap,
+                                         unsigned long long w, K *rk, unsigned long long *rca, unsigned long long *rcb,
+                                         uint64_t capacity, uint32_t &medmask) {
+  medmask = 0;
+  for (int q = 0; q < PER; q++) {
+    const uint32_t l = threadIdx.x * PER + q;
+    if (l >= nb) break;
+    const uint64_t a0 = s_oa[l], a1 = s_oa[l + 1], x0 = s_ob[l], x1 = s_ob[l + 1];
+    if (a1 == a0 || x1 == x0) continue;
+    if (a1 - a0 > RLE_SMALL || x1 - x0 > RLE_SMALL) {
+      if (heavy_n) {
+        const unsigned long long h = atomicAdd(heavy_n, 1ull);
+        if (h < heavy_cap) heavy[h] = (uint32_t)(b0 + l);
+      }
+      continue;
+    }
+    if (a1 - a0 > RLE_TINY || x1 - x0 > RLE_TINY) {  // a wave takes it (rle_medium)
+      medmask |= 1u << q;
+      continue;
+    }
+    for (uint64_t i = a0; i < a1; i++) {
+      const K v = getA(i);
+      bool first = true;
+      for (uint64_t j = a0; j < i; j++) first &= getA(j) != v;
+      if (!first) continue;
+      unsigned long long ca = 0, cb = 0;
+      for (uint64_t j = i; j < a1; j++) ca += getA(j) == v;
+      for (uint64_t j = x0; j < x1; j++) cb += getB(j) == v;
+      if (!cb) continue;
+      if (pass == 0) {
+        nrec++;
+        npair += ca * cb;
+      } else {
+        if (w < capacity) {
+          rk[w] = v;
+          rca[w] = ca;
+          rcb[w] = cb;
+        }
+        w++;
+      }
+    }
+  }
+}
 
 template <typename K, typename OFF, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_rle_small(const OFF *__restrict__ offA, const K *__restrict__ keysA,
@@ -867,71 +942,116 @@ __global__ __launch_bounds__(BLOCK) void k_rle_small(const OFF *__restrict__ off
                                                      unsigned long long *__restrict__ rcb, uint64_t capacity,
                                                      unsigned long long *__restrict__ heavy_n, uint32_t *__restrict__ heavy,
                                                      uint64_t heavy_cap, unsigned long long *__restrict__ pairs) {
+  constexpr int PER = RLE_CHUNK / BLOCK;
   __shared__ unsigned long long s_t64[BLOCK / 64 + 1];
+  __shared__ OFF s_oa[RLE_CHUNK + 1], s_ob[RLE_CHUNK + 1];
+  __shared__ K s_ka[RLE_KCAP], s_kb[RLE_KCAP];
   const uint64_t b0 = (uint64_t)blockIdx.x * RLE_CHUNK;
   const uint32_t nb = (uint32_t)min((uint64_t)RLE_CHUNK, M - b0);
-  constexpr int PER = RLE_CHUNK / BLOCK;
-  unsigned long long nrec = 0, npair = 0;
-  // thread t owns bins b0 + t*PER .. +PER (contiguous: records stay in bin order)
-  for (int q = 0; q < PER; q++) {
-    const uint32_t l = threadIdx.x * PER + q;
-    if (l >= nb) break;
-    const uint64_t b = b0 + l;
-    const uint64_t a0 = offA[b], a1 = offA[b + 1], x0 = offB[b], x1 = offB[b + 1];
-    if (a1 == a0 || x1 == x0) continue;
-    if (a1 - a0 > RLE_SMALL || x1 - x0 > RLE_SMALL) {
-      if (pass == 0) {
-        const unsigned long long h = atomicAdd(heavy_n, 1ull);
-        if (h < heavy_cap) heavy[h] = (uint32_t)b;
-      }
-      continue;
-    }
-    for (uint64_t i = a0; i < a1; i++) {
-      const K v = keysA[i];
-      bool first = true;
-      for (uint64_t j = a0; j < i; j++) first &= keysA[j] != v;
-      if (!first) continue;
-      unsigned long long ca = 0, cb = 0;
-      for (uint64_t j = i; j < a1; j++) ca += keysA[j] == v;
-      for (uint64_t j = x0; j < x1; j++) cb += keysB[j] == v;
-      if (cb) {
-        nrec++;
-        npair += ca * cb;
-      }
-    }
+  for (uint32_t i = threadIdx.x; i <= nb; i += BLOCK) {
+    s_oa[i] = offA[b0 + i];
+    s_ob[i] = offB[b0 + i];
   }
-  unsigned long long tot;
-  unsigned long long ex = block_exclusive_sum<BLOCK, unsigned long long>(nrec, s_t64, tot);
+  __syncthreads();
+  const uint64_t ka0 = s_oa[0], kb0 = s_ob[0];
+  const uint64_t na = (uint64_t)s_oa[nb] - ka0, nx = (uint64_t)s_ob[nb] - kb0;
+  const bool staged = na <= RLE_KCAP && nx <= RLE_KCAP;  // uniform
+  if (staged) {
+    for (uint32_t i = threadIdx.x; i < (uint32_t)na; i += BLOCK) s_ka[i] = keysA[ka0 + i];
+    for (uint32_t i = threadIdx.x; i < (uint32_t)nx; i += BLOCK) s_kb[i] = keysB[kb0 + i];
+    __syncthreads();
+  }
+  auto lA = [&](uint64_t i) { return s_ka[i - ka0]; };
+  auto lB = [&](uint64_t i) { return s_kb[i - kb0]; };
+  auto gA = [&](uint64_t i) { return keysA[i]; };
+  auto gB = [&](uint64_t i) { return keysB[i]; };
+  unsigned long long nrec = 0, npair = 0;
+  uint32_t medmask = 0;
+  // per-thread walk of the tiny bins (counting; pass 0 also lists heavy bins)
+  if (staged)
+    rle_walk<K, PER>(s_oa, s_ob, nb, 0, lA, lB, b0, nrec, npair, pass == 0 ? heavy_n : nullptr, heavy, heavy_cap, 0ull,
+                     rk, rca, rcb, 0, medmask);
+  else
+    rle_walk<K, PER>(s_oa, s_ob, nb, 0, gA, gB, b0, nrec, npair, pass == 0 ? heavy_n : nullptr, heavy, heavy_cap, 0ull,
+                     rk, rca, rcb, 0, medmask);
+  // list of the chunk's medium bins (thread order: deterministic)
+  __shared__ uint16_t s_med[RLE_CHUNK];
+  __shared__ uint32_t s_mcnt[RLE_CHUNK];
+  __shared__ unsigned long long s_mbase;
+  unsigned long long nmed_tot;
+  const unsigned long long mex = block_exclusive_sum<BLOCK, unsigned long long>(
+      (unsigned long long)__popc(medmask), s_t64, nmed_tot);
+  {
+    uint32_t k = (uint32_t)mex;
+    for (int q = 0; q < PER; q++)
+      if (medmask & (1u << q)) s_med[k++] = (uint16_t)(threadIdx.x * PER + q);
+  }
+  __syncthreads();
+  const uint32_t nmed = (uint32_t)nmed_tot;
+  const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  constexpr uint32_t NW = BLOCK / 64;
+  // medium bins: one wave each; per-bin record counts to s_mcnt
+  unsigned long long mrec = 0, mpair = 0;
+  for (uint32_t k = wv; k < nmed; k += NW) {
+    const uint32_t l = s_med[k];
+    const uint64_t a0 = s_oa[l], x0 = s_ob[l];
+    const uint32_t a = (uint32_t)((uint64_t)s_oa[l + 1] - a0), b = (uint32_t)((uint64_t)s_ob[l + 1] - x0);
+    K v;
+    uint32_t ca, cb;
+    const bool rec = staged ? rle_medium<K>(a0, a, x0, b, lA, lB, lane, v, ca, cb)
+                            : rle_medium<K>(a0, a, x0, b, gA, gB, lane, v, ca, cb);
+    const uint64_t bal = __ballot(rec);
+    if (lane == 0) s_mcnt[k] = (uint32_t)__popcll(bal);
+    mrec += rec ? 1ull : 0ull;
+    mpair += rec ? (unsigned long long)ca * cb : 0ull;
+  }
   if (pass == 0) {
-    const unsigned long long pt = block_sum<BLOCK, unsigned long long>(npair, s_t64);
+    unsigned long long tot;
+    block_exclusive_sum<BLOCK, unsigned long long>(nrec + mrec, s_t64, tot);
+    const unsigned long long pt = block_sum<BLOCK, unsigned long long>(npair + mpair, s_t64);
     if (threadIdx.x == 0) {
       chunk_cnt[blockIdx.x] = tot;
       if (pt) atomicAdd(pairs, pt);
     }
     return;
   }
-  unsigned long long w = chunk_base[blockIdx.x] + ex;
-  for (int q = 0; q < PER; q++) {
-    const uint32_t l = threadIdx.x * PER + q;
-    if (l >= nb) break;
-    const uint64_t b = b0 + l;
-    const uint64_t a0 = offA[b], a1 = offA[b + 1], x0 = offB[b], x1 = offB[b + 1];
-    if (a1 == a0 || x1 == x0 || a1 - a0 > RLE_SMALL || x1 - x0 > RLE_SMALL) continue;
-    for (uint64_t i = a0; i < a1; i++) {
-      const K v = keysA[i];
-      bool first = true;
-      for (uint64_t j = a0; j < i; j++) first &= keysA[j] != v;
-      if (!first) continue;
-      unsigned long long ca = 0, cb = 0;
-      for (uint64_t j = i; j < a1; j++) ca += keysA[j] == v;
-      for (uint64_t j = x0; j < x1; j++) cb += keysB[j] == v;
-      if (cb) {
-        if (w < capacity) {
-          rk[w] = v;
-          rca[w] = ca;
-          rcb[w] = cb;
-        }
-        w++;
+  // pass 1: the tiny bins' records at the chunk's prefix plus the threads
+  // before, then the medium bins' records after all of them
+  unsigned long long tot_small;
+  const unsigned long long ex = block_exclusive_sum<BLOCK, unsigned long long>(nrec, s_t64, tot_small);
+  const unsigned long long w = chunk_base[blockIdx.x] + ex;
+  if (staged)
+    rle_walk<K, PER>(s_oa, s_ob, nb, 1, lA, lB, b0, nrec, npair, nullptr, nullptr, 0, w, rk, rca, rcb, capacity,
+                     medmask);
+  else
+    rle_walk<K, PER>(s_oa, s_ob, nb, 1, gA, gB, b0, nrec, npair, nullptr, nullptr, 0, w, rk, rca, rcb, capacity,
+                     medmask);
+  if (nmed == 0) return;  // uniform
+  if (threadIdx.x == 0) {
+    unsigned long long acc = chunk_base[blockIdx.x] + tot_small;
+    for (uint32_t k = 0; k < nmed; k++) {  // record base of each medium bin
+      const uint32_t c = s_mcnt[k];
+      s_mcnt[k] = (uint32_t)(acc - chunk_base[blockIdx.x]);
+      acc += c;
+    }
+    s_mbase = chunk_base[blockIdx.x];
+  }
+  __syncthreads();
+  for (uint32_t k = wv; k < nmed; k += NW) {
+    const uint32_t l = s_med[k];
+    const uint64_t a0 = s_oa[l], x0 = s_ob[l];
+    const uint32_t a = (uint32_t)((uint64_t)s_oa[l + 1] - a0), b = (uint32_t)((uint64_t)s_ob[l + 1] - x0);
+    K v;
+    uint32_t ca, cb;
+    const bool rec = staged ? rle_medium<K>(a0, a, x0, b, lA, lB, lane, v, ca, cb)
+                            : rle_medium<K>(a0, a, x0, b, gA, gB, lane, v, ca, cb);
+    const uint64_t bal = __ballot(rec);
+    if (rec) {
+      const unsigned long long p = s_mbase + s_mcnt[k] + (unsigned long long)__popcll(bal & ((1ull << lane) - 1ull));
+      if (p < capacity) {
+        rk[p] = v;
+        rca[p] = ca;
+        rcb[p] = cb;
       }
     }
   }
