@@ -266,17 +266,56 @@ __device__ __forceinline__ uint32_t bin_of(K key, const BinMap &m) {
 // split points, SK_ANY = chosen per segment at run time.
 enum : int { SK_RANGE = 0, SK_SPLIT = 1, SK_ANY = 2 };
 
+// Split-point bucket: the number of splits spl[1..n] (sorted) that are <= bin,
+// i.e. the first p with bin < spl[p+1] (n: past the last split).  A
+// branch-free binary search (uniform step, clamped reads).
+__device__ __forceinline__ uint32_t split_bucket(uint32_t bin, const int64_t *spl, uint32_t n) {
+  uint32_t p = 0;
+  for (uint32_t step = n ? 1u << (31 - __clz(n)) : 0u; step; step >>= 1) {
+    const uint32_t q = p + step;
+    const int64_t v = spl[q <= n ? q : n];
+    p = (q <= n && v <= (int64_t)bin) ? q : p;
+  }
+  return p;
+}
+
+// Most partitions a split-point segment may have (its splits are staged in
+// LDS by the partition kernels).
+constexpr uint32_t MAX_SPLITS = 4096;
+
+// Bucket of a bin; spl: the segment's splits (an LDS copy, or sd.splits).
+template <int SK = SK_ANY>
+__device__ __forceinline__ uint32_t bucket_of_bin(uint32_t bin, const SegDesc &sd, const int64_t *spl) {
+  if (SK == SK_RANGE || (SK == SK_ANY && sd.kind == 0)) return divq(bin - (uint32_t)sd.bin_lo, sd.div);
+  return split_bucket(bin, spl, sd.nsplits);
+}
+
 template <int SK = SK_ANY>
 __device__ __forceinline__ uint32_t bucket_of_bin(uint32_t bin, const SegDesc &sd) {
-  if (SK == SK_RANGE || (SK == SK_ANY && sd.kind == 0)) return divq(bin - (uint32_t)sd.bin_lo, sd.div);
-  uint32_t p = 0;
-  while (p < sd.nsplits && (int64_t)bin >= sd.splits[p + 1]) p++;
-  return p;  // p == nsplits: past the last split
+  return bucket_of_bin<SK>(bin, sd, sd.splits);
+}
+
+// Copies a split-point segment's splits to LDS (s_spl holds MAX_SPLITS + 1)
+// and returns the pointer to use: s_spl, or sd.splits for a range segment.
+// Ends with a barrier.
+template <int SK, int BLOCK>
+__device__ __forceinline__ const int64_t *stage_splits(const SegDesc &sd, int64_t *s_spl) {
+  if (SK == SK_RANGE) return sd.splits;
+  const bool split = SK == SK_SPLIT || sd.kind == 1;
+  if (split)
+    for (uint32_t i = threadIdx.x; i <= sd.nsplits; i += BLOCK) s_spl[i] = sd.splits[i];
+  __syncthreads();
+  return split ? s_spl : sd.splits;
 }
 
 template <typename K, int HK = HK_ANY, int SK = SK_ANY>
 __device__ __forceinline__ uint32_t bucket_of(K key, const SegDesc &sd) {
   return bucket_of_bin<SK>(bin_of<K, HK>(key, sd.map), sd);
+}
+
+template <typename K, int HK, int SK>
+__device__ __forceinline__ uint32_t bucket_of(K key, const SegDesc &sd, const int64_t *spl) {
+  return bucket_of_bin<SK>(bin_of<K, HK>(key, sd.map), sd, spl);
 }
 
 // Inclusive wave64 scan of a 32/64-bit value.

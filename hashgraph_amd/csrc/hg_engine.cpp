@@ -450,7 +450,7 @@ hg_status run_level(hg_ctx c, const Level &L, const LevelDev &d, int kb, const v
   p.sk = L.bucket_kind();
   p.ntiles = (uint32_t)L.tiles.size();
   p.max_cols = L.max_cols;
-  p.stage_cap = part_stage_cap(L.max_cols, kb);
+  p.stage_cap = part_stage_cap(L.max_cols, kb, p.sk);
   if (L.tiles.size() && p.stage_cap < 1024) return fail(HG_ERR_UNSUPPORTED, "partition level too wide for LDS");
   p.keys = in;
   p.out = out;

@@ -71,8 +71,9 @@ __global__ __launch_bounds__(BLOCK) void k_part_hist(const K *__restrict__ keys,
   const TileDesc td = tiles[tile];
   const SegDesc sd = segs[td.seg];  // a copy: uniform fields stay in SGPRs
   const uint32_t ncols = sd.ncols;
+  __shared__ int64_t s_spl[SK == SK_RANGE ? 1 : MAX_SPLITS + 1];
   for (uint32_t i = threadIdx.x; i < ncols; i += BLOCK) s_cnt[i] = 0;
-  __syncthreads();
+  const int64_t *spl = stage_splits<SK, BLOCK>(sd, s_spl);  // ends with a barrier
   // a tile is processed in rounds of BLOCK*KPT keys (slot j of a round holds
   // key j*BLOCK + tid of the round)
   if constexpr (HK == HK_RMI) {
@@ -96,7 +97,7 @@ __global__ __launch_bounds__(BLOCK) void k_part_hist(const K *__restrict__ keys,
         for (int j = 0; j < KPT; j++)
           kr[j] = buf_load<K>(rs, threadIdx.x * (uint32_t)sizeof(K), j * BLOCK * (uint32_t)sizeof(K));
         rmi_window_bins<K, KPT>(kr, nv, h, s_leaf, w0, n, sd.map.M, [&](int, uint32_t bin) {
-          atomicAdd(&s_cnt[bucket_of_bin<SK>(bin, sd)], 1u);
+          atomicAdd(&s_cnt[bucket_of_bin<SK>(bin, sd, spl)], 1u);
         });
       }
     }
@@ -112,7 +113,7 @@ __global__ __launch_bounds__(BLOCK) void k_part_hist(const K *__restrict__ keys,
         kr[j] = buf_load<K>(rs, threadIdx.x * (uint32_t)sizeof(K), j * BLOCK * (uint32_t)sizeof(K));
 #pragma unroll
       for (int j = 0; j < KPT; j++)
-        if (j < nv) atomicAdd(&s_cnt[bucket_of<K, HK, SK>(kr[j], sd)], 1u);
+        if (j < nv) atomicAdd(&s_cnt[bucket_of<K, HK, SK>(kr[j], sd, spl)], 1u);
     }
   }
   __syncthreads();
@@ -226,6 +227,8 @@ __global__ __launch_bounds__(BLOCK) void k_part_scatter(const K *__restrict__ ke
   K *s_key = (K *)(s_adj + cnt_words);
   uint16_t *s_bkt = (uint16_t *)(s_key + stage_cap);
   const uint32_t *row = cpos + sd.row_base + (uint64_t)(tile - sd.tile_begin) * ncols;
+  __shared__ int64_t s_spl[SK == SK_RANGE ? 1 : MAX_SPLITS + 1];
+  const int64_t *spl = stage_splits<SK, BLOCK>(sd, s_spl);
   lds_fill<uint32_t, BLOCK, 4>(s_adj, row, ncols);
   const uint32_t nb = sd.nbuckets;
   for (uint32_t base = 0; base < td.count; base += BLOCK * KPT) {
@@ -243,7 +246,7 @@ __global__ __launch_bounds__(BLOCK) void k_part_scatter(const K *__restrict__ ke
 #pragma unroll
     for (int j = 0; j < KPT; j++) {
       if (j < nv) {
-        const uint32_t b = bucket_of<K, HK, SK>(kr[j], sd);
+        const uint32_t b = bucket_of<K, HK, SK>(kr[j], sd, spl);
         const uint32_t r = atomicAdd(&s_cnt[b], 1u);
         pk[j] = (b << 16) | r;
       }
@@ -1856,8 +1859,9 @@ static hipError_t launch_part_scatter_t(const LaunchPart &lp, hipStream_t st) {
   });
 }
 
-uint32_t part_stage_cap(uint32_t max_cols, int key_bytes) {
-  const size_t used = 2 * (size_t)((max_cols + 3) & ~3u) * 4 + 256;
+uint32_t part_stage_cap(uint32_t max_cols, int key_bytes, int sk) {
+  // counters, positions, static scan scratch and (split kinds) the splits
+  const size_t used = 2 * (size_t)((max_cols + 3) & ~3u) * 4 + 256 + (sk != SK_RANGE ? (MAX_SPLITS + 1) * 8 : 0);
   const size_t avail = used < 160 * 1024 ? 160 * 1024 - used : 0;
   size_t cap = avail / (size_t)(key_bytes + 2) / 1024 * 1024;
   cap = cap > 8192 ? 8192 : cap;

@@ -161,7 +161,7 @@ uint32_t final_big_items();
 size_t final_big_ctl_bytes();
 // keys staged per LDS round of the scatter for a level whose widest segment
 // has max_cols columns (0: the columns do not leave room for staging)
-uint32_t part_stage_cap(uint32_t max_cols, int key_bytes);
+uint32_t part_stage_cap(uint32_t max_cols, int key_bytes, int sk);
 uint32_t probe_akey_cap(int key_bytes, int off_bytes);
 uint32_t join_key_cap(int key_bytes, int off_bytes);
 uint64_t scan_tiles(uint64_t n);
