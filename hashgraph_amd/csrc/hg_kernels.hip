@@ -20,7 +20,10 @@ namespace hg {
 // A probe key walks at most this many keys of its bin, and a join bin at
 // most this many key pairs; larger bins (heavy duplicates) are reported to
 // the host, which answers them with the run-length join instead.
-constexpr uint64_t PROBE_BIN_LIMIT = 4096;
+// Bins longer than this are not walked by the probe kernels but answered from
+// a value table by k_probe_heavy (duplicates: a walk would cost
+// |A_bin| x |B_bin| comparisons).
+constexpr uint64_t PROBE_BIN_LIMIT = 32;
 // Leaves of a learned model a bucket-local kernel stages in LDS (the range
 // between its smallest and largest key's leaves; wider ranges are read from
 // global memory).
@@ -1709,13 +1712,16 @@ __global__ __launch_bounds__(BLOCK) void k_probe_big(const K *__restrict__ bpart
   }
 }
 
-// Bins longer than PROBE_BIN_LIMIT (heavy duplicates) of the items k_probe and
-// k_probe_big marked: one workgroup per marked item finds the bucket's heavy
-// bins; each is folded into an LDS table value -> copies in A (slots claimed
-// by compare-and-swap of the key against the all-ones sentinel, sentinel keys
-// counted apart), then the bucket's probe keys of that bin read their match
-// counts from the table.  A heavy bin with more than HT_SLOTS/2 distinct
-// values sets *flag (the host reports the probe as unsupported).
+// Bins longer than PROBE_BIN_LIMIT (duplicates) of the items k_probe1, k_probe
+// and k_probe_big marked: one workgroup per marked item folds the bucket's
+// long bins of A, bin after bin, into one LDS table value -> copies in A
+// (slots claimed by compare-and-swap of the key against the all-ones
+// sentinel, sentinel keys counted apart; a value lives in one bin, so the
+// bins share the table).  When the table is half full it is answered: every
+// probe key of the bucket looks its value up (a value found is in a long
+// bin; the walking kernels skipped those) and the table is cleared.  A single
+// bin with more than HT_SLOTS/2 distinct values sets *flag (the host reports
+// the probe as unsupported).
 template <typename K, typename OFF, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_probe_heavy(const K *__restrict__ bpart, const FinItem *__restrict__ items,
                                                        const BinMap *__restrict__ maps, const OFF *__restrict__ offA,
@@ -1731,9 +1737,60 @@ __global__ __launch_bounds__(BLOCK) void k_probe_heavy(const K *__restrict__ bpa
   __shared__ unsigned long long s_t64[BLOCK / 64 + 1], s_base;
   if (!hmark[blockIdx.x]) return;
   const FinItem it = items[blockIdx.x];
-  const BinMap mp = maps[it.table];
   const K *kp = bpart + it.key_start;
   const K *ka = keysA + it.key_rel;
+  auto clear = [&]() {
+    for (uint32_t i = threadIdx.x; i < HT_SLOTS; i += BLOCK) {
+      s_key[i] = EMPTY;
+      s_cnt[i] = 0;
+    }
+    if (threadIdx.x == 0) {
+      s_fill = 0;
+      s_ecnt = 0;
+    }
+    __syncthreads();
+  };
+  // every probe key of the bucket looks its value up in the table
+  auto answer = [&]() {
+    for (uint64_t c0 = 0; c0 < it.count; c0 += BLOCK) {
+      const uint64_t i = c0 + threadIdx.x;
+      unsigned long long m = 0;
+      K k = 0;
+      if (i < it.count) {
+        k = kp[i];
+        if (k == EMPTY) {
+          m = s_ecnt;
+        } else {
+          uint32_t slot = murmur4((uint32_t)k ^ (uint32_t)((uint64_t)k >> 32)) & (HT_SLOTS - 1);
+          for (uint32_t pr = 0; pr < HT_SLOTS; pr++, slot = (slot + 1) & (HT_SLOTS - 1)) {
+            const K cur = s_key[slot];
+            if (cur == EMPTY) break;
+            if (cur == k) {
+              m = s_cnt[slot];
+              break;
+            }
+          }
+        }
+      }
+      if (out == nullptr) {
+        const unsigned long long t = block_sum<BLOCK, unsigned long long>(m, s_t64);
+        if (threadIdx.x == 0 && t) atomicAdd(cursor + it.table, t);
+        continue;
+      }
+      unsigned long long tot;
+      const unsigned long long ex = block_exclusive_sum<BLOCK, unsigned long long>(m, s_t64, tot);
+      if (threadIdx.x == 0) s_base = tot ? atomicAdd(cursor + it.table, tot) : 0ull;
+      __syncthreads();
+      const unsigned long long w = s_base + ex;
+      if (m && w < capacity) {
+        const unsigned long long lim = min(m, (unsigned long long)capacity - w);
+        for (unsigned long long r = 0; r < lim; r++) out[w + r] = k;
+      }
+      __syncthreads();
+    }
+  };
+  clear();
+  bool pending = false;  // bins folded since the last answer (uniform)
   for (uint32_t h0 = 0; h0 < it.nbins; h0 += BLOCK) {
     if (threadIdx.x == 0) s_nh = 0;
     __syncthreads();
@@ -1746,16 +1803,14 @@ __global__ __launch_bounds__(BLOCK) void k_probe_heavy(const K *__restrict__ bpa
     const uint32_t nh = s_nh;
     __syncthreads();
     for (uint32_t hi = 0; hi < nh; hi++) {
+      if (s_fill > HT_SLOTS / 2) {  // uniform (read after a barrier)
+        answer();
+        clear();
+        pending = false;
+      }
+      const uint32_t f0 = s_fill;
+      __syncthreads();  // every thread has read f0 before the fold adds to s_fill
       const uint32_t hb = s_hl[hi];
-      for (uint32_t i = threadIdx.x; i < HT_SLOTS; i += BLOCK) {
-        s_key[i] = EMPTY;
-        s_cnt[i] = 0;
-      }
-      if (threadIdx.x == 0) {
-        s_fill = 0;
-        s_ecnt = 0;
-      }
-      __syncthreads();
       const uint64_t a0 = (uint64_t)offA[it.off_pos + hb], a1 = (uint64_t)offA[it.off_pos + hb + 1];
       for (uint64_t q = a0 + threadIdx.x; q < a1; q += BLOCK) {
         const K v = ka[q];
@@ -1765,52 +1820,11 @@ __global__ __launch_bounds__(BLOCK) void k_probe_heavy(const K *__restrict__ bpa
         else vt_insert(s_key, s_cnt, &s_fill, v, c);
       }
       __syncthreads();
-      const bool ovf = s_fill > HT_SLOTS / 2;
-      __syncthreads();
-      if (ovf) {
-        if (threadIdx.x == 0) atomicOr(flag, 1u);
-        continue;
-      }
-      for (uint64_t c0 = 0; c0 < it.count; c0 += BLOCK) {
-        const uint64_t i = c0 + threadIdx.x;
-        unsigned long long m = 0;
-        K k = 0;
-        if (i < it.count) {
-          k = kp[i];
-          if (bin_of<K>(k, mp) - it.bin_lo == hb) {
-            if (k == EMPTY) {
-              m = s_ecnt;
-            } else {
-              uint32_t slot = murmur4((uint32_t)k ^ (uint32_t)((uint64_t)k >> 32)) & (HT_SLOTS - 1);
-              for (uint32_t pr = 0; pr < HT_SLOTS; pr++, slot = (slot + 1) & (HT_SLOTS - 1)) {
-                const K cur = s_key[slot];
-                if (cur == EMPTY) break;
-                if (cur == k) {
-                  m = s_cnt[slot];
-                  break;
-                }
-              }
-            }
-          }
-        }
-        if (out == nullptr) {
-          const unsigned long long t = block_sum<BLOCK, unsigned long long>(m, s_t64);
-          if (threadIdx.x == 0 && t) atomicAdd(cursor + it.table, t);
-          continue;
-        }
-        unsigned long long tot;
-        const unsigned long long ex = block_exclusive_sum<BLOCK, unsigned long long>(m, s_t64, tot);
-        if (threadIdx.x == 0) s_base = tot ? atomicAdd(cursor + it.table, tot) : 0ull;
-        __syncthreads();
-        const unsigned long long w = s_base + ex;
-        if (m && w < capacity) {
-          const unsigned long long lim = min(m, (unsigned long long)capacity - w);
-          for (unsigned long long r = 0; r < lim; r++) out[w + r] = k;
-        }
-        __syncthreads();
-      }
+      pending = true;
+      if (s_fill - f0 > HT_SLOTS / 2 && threadIdx.x == 0) atomicOr(flag, 1u);  // too many values in one bin
     }
   }
+  if (pending) answer();
 }
 
 // ---------------------------------------------------------------------------

@@ -120,6 +120,18 @@ def test_heavy_bins_are_answered(dev):
     _check_probe(A2, B2, 64, dev)
 
 
+def test_many_long_bins_answered_in_groups(dev):
+    """12000 values with 3 copies each in 512 bins: every bin is beyond the
+    walk limit, and the bucket's long bins hold more distinct values than one
+    value table, so they are answered in several groups."""
+    rng = np.random.default_rng(41)
+    vals = wl.uniform_keys(42, 12000, 4)
+    A = np.repeat(vals, 3)
+    rng.shuffle(A)
+    B = np.concatenate([vals[rng.integers(0, len(vals), 5000)], wl.uniform_keys(43, 2000, 4)])
+    _check_probe(A, B, 512, dev)
+
+
 def test_heavy_bin_with_many_values_is_refused(dev):
     """One bin of 10000 distinct values: beyond both the walk limit and the
     per-bin value table; the library reports it instead of running it."""
