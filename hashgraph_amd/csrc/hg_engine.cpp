@@ -950,6 +950,7 @@ hg_status hg_probe(hg_ctx c, const hg_table_desc *descA, const void *d_offA, con
   pr.capacity = d_out ? capacity : 0;
   pr.cursor = small;
   pr.flag = (unsigned int *)(small + 1);
+  pr.gate = (unsigned int *)(small + 2);
   pr.hmark = hmark;
   {
     Prof pf(c, "probe");
@@ -1313,7 +1314,7 @@ hg_status batch_launch_build(hg_ctx c, const hg_batch_desc *bd, BatchSide &S, co
 hg_status batch_launch_probe(hg_ctx c, const hg_batch_desc *bd, BatchSide &S, const void *d_probe,
                              const uint32_t *d_offsets, const void *d_csr, uint64_t *d_counts) {
   HG_HIP(launch_fill_u64((unsigned long long *)d_counts, 0ull, bd->n_tables, c->stream));
-  HG_HIP(launch_fill_u32(S.flag, 0u, 1, c->stream));
+  HG_HIP(launch_fill_u32(S.flag, 0u, 3, c->stream));  // overflow flag, probe gates
   HG_HIP(launch_fill_u32(S.hmark, 0u, S.L.ncols_total, c->stream));
   HG_TRY(run_level(c, S.L, S.d, (int)bd->key_bytes, d_probe, S.part));
   HG_HIP(launch_make_items(S.d.segs, (uint32_t)S.L.segs.size(), S.d.bstart, S.d.coltot, S.d.fin, S.L.ncols_total,
@@ -1333,6 +1334,7 @@ hg_status batch_launch_probe(hg_ctx c, const hg_batch_desc *bd, BatchSide &S, co
   pr.capacity = 0;
   pr.cursor = (unsigned long long *)d_counts;
   pr.flag = S.flag;
+  pr.gate = S.flag + 1;
   pr.hmark = S.hmark;
   Prof pf(c, "probe");
   HG_HIP(launch_probe(pr, c->stream));

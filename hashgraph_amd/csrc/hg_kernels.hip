@@ -301,11 +301,14 @@ __global__ __launch_bounds__(BLOCK) void k_part_scatter(const K *__restrict__ ke
 template <typename K, typename OFF, int BLOCK, int KPT, int HK>
 __global__ __launch_bounds__(BLOCK) void k_final(const K *__restrict__ part, K *__restrict__ csr, OFF *__restrict__ off,
                                                  const FinItem *__restrict__ items, const BinMap *__restrict__ maps,
-                                                 uint32_t lds_bytes) {
+                                                 uint32_t lds_bytes, unsigned int *__restrict__ big_seen) {
   extern __shared__ __align__(16) uint32_t smem32[];
   __shared__ uint32_t s_tmp[BLOCK / 64 + 1];
   const FinItem it = items[blockIdx.x];
-  if (it.count > BLOCK * KPT || it.count > 65535u) return;  // k_final_big's item
+  if (it.count > BLOCK * KPT || it.count > 65535u) {  // the oversized-bucket kernels' item: they run
+    if (threadIdx.x == 0) atomicOr(big_seen, 1u);
+    return;
+  }
   const BinMap mp = maps[it.table];
   const uint32_t W = it.nbins, cnt = it.count;
   const uint32_t nw = (W + 1) >> 1;
@@ -412,9 +415,11 @@ __global__ __launch_bounds__(BLOCK) void k_final(const K *__restrict__ part, K *
 template <typename K, typename OFF, int BLOCK, int SUB>
 __global__ __launch_bounds__(BLOCK) void k_final_big(const K *__restrict__ part, K *__restrict__ csr, OFF *__restrict__ off,
                                                      const FinItem *__restrict__ items, const BinMap *__restrict__ maps,
-                                                     uint32_t small_cap, const uint32_t *__restrict__ big_slot) {
+                                                     uint32_t small_cap, const uint32_t *__restrict__ big_slot,
+                                                     const unsigned int *__restrict__ big_seen) {
   __shared__ uint32_t s_cnt[SUB];
   __shared__ uint32_t s_tmp[BLOCK / 64 + 1];
+  if (*big_seen == 0) return;  // no oversized item
   const FinItem it = items[blockIdx.x];
   if (it.count <= small_cap) return;               // k_final's item
   if (big_slot[blockIdx.x] != ~0u) return;         // the multi-workgroup path's item
@@ -1461,7 +1466,8 @@ __global__ void k_make_items(const SegDesc *__restrict__ segs, uint32_t nsegs, c
 // BIG_BINS.  Buckets beyond BIG_CAP take k_final_big.
 // ---------------------------------------------------------------------------
 struct BigCtl {
-  uint32_t nbig, total_chunks, total_bins, pad;
+  uint32_t nbig, total_chunks, total_bins;
+  uint32_t seen;  // k_final left an oversized item (zeroed before k_final)
   uint32_t item[BIG_CAP];
   uint32_t chunk_start[BIG_CAP + 1];
   uint32_t bin_base[BIG_CAP + 1];
@@ -1474,6 +1480,16 @@ __global__ __launch_bounds__(1024) void k_big_list(const FinItem *__restrict__ i
                                                    BigCtl *__restrict__ ctl) {
   __shared__ uint32_t s_tmp[1024 / 64 + 1];
   __shared__ uint32_t s_slot, s_ch, s_bin, s_nacc, s_acc_ch, s_acc_bin;
+  if (ctl->seen == 0) {  // no oversized item: empty lists (k_final_big reads no big_slot)
+    if (threadIdx.x == 0) {
+      ctl->chunk_start[0] = 0;
+      ctl->bin_base[0] = 0;
+      ctl->nbig = 0;
+      ctl->total_chunks = 0;
+      ctl->total_bins = 0;
+    }
+    return;
+  }
   if (threadIdx.x == 0) s_slot = s_ch = s_bin = s_nacc = s_acc_ch = s_acc_bin = 0;
   __syncthreads();
   for (uint32_t i0 = 0; i0 < nitems; i0 += 1024) {
@@ -1904,11 +1920,13 @@ hipError_t launch_colscan(const LaunchCol &lc, hipStream_t st) {
 
 template <typename K, typename OFF>
 static hipError_t launch_final_t(const LaunchFinal &lf, hipStream_t st) {
+  if (lf.nitems) hipLaunchKernelGGL(k_fill<uint32_t>, dim3(1), dim3(64), 0, st, &((BigCtl *)lf.big_ctl)->seen, 0u, (uint64_t)1);
   if (lf.nitems) {
     constexpr int KPT = sizeof(K) == 4 ? KPT_U32 : 32;
     const hipError_t e = dispatch_hk(lf.hk, [&](auto h) {
       hipLaunchKernelGGL((k_final<K, OFF, 1024, KPT, decltype(h)::value>), dim3(lf.nitems), dim3(1024), lf.lds_bytes, st,
-                         (const K *)lf.part, (K *)lf.csr, (OFF *)lf.off, lf.items, lf.maps, lf.lds_bytes);
+                         (const K *)lf.part, (K *)lf.csr, (OFF *)lf.off, lf.items, lf.maps, lf.lds_bytes,
+                         &((BigCtl *)lf.big_ctl)->seen);
       return hipGetLastError();
     });
     if (e != hipSuccess) return e;
@@ -1927,7 +1945,8 @@ static hipError_t launch_final_t(const LaunchFinal &lf, hipStream_t st) {
     hipLaunchKernelGGL((k_big_place<K, 1024>), dim3(2048), dim3(1024), 0, st, (const K *)lf.part, (K *)lf.csr, lf.items,
                        lf.maps, (const BigCtl *)ctl, lf.big_cnt);
     hipLaunchKernelGGL((k_final_big<K, OFF, 1024, 16384>), dim3(lf.nitems), dim3(1024), 0, st, (const K *)lf.part,
-                       (K *)lf.csr, (OFF *)lf.off, lf.items, lf.maps, small_cap, (const uint32_t *)lf.big_slot);
+                       (K *)lf.csr, (OFF *)lf.off, lf.items, lf.maps, small_cap, (const uint32_t *)lf.big_slot,
+                       (const unsigned int *)&ctl->seen);
   }
   return hipGetLastError();
 }

@@ -72,6 +72,7 @@ struct LaunchProbe {
   unsigned long long *cursor;  // match counters, one per table (FinItem.table)
   unsigned int *hmark;         // nitems, zeroed: items with a bin beyond the walk limit
   unsigned int *flag;          // a heavy bin with too many distinct values
+  unsigned int *gate;          // 2 zeroed words: [0] k_probe1 left items, [1] a bin was marked heavy
 };
 
 struct LaunchJoin {
