@@ -1894,7 +1894,7 @@ uint32_t part_stage_cap(uint32_t max_cols, int key_bytes, int sk) {
   const size_t used = 2 * (size_t)((max_cols + 3) & ~3u) * 4 + 256 + (sk != SK_RANGE ? (MAX_SPLITS + 1) * 8 : 0);
   const size_t avail = used < 160 * 1024 ? 160 * 1024 - used : 0;
   size_t cap = avail / (size_t)(key_bytes + 2) / 1024 * 1024;
-  cap = cap > 8192 ? 8192 : cap;
+  cap = cap > 32768 ? 32768 : cap;
   return (uint32_t)cap;
 }
 
