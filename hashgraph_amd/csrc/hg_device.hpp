@@ -300,7 +300,10 @@ __device__ __forceinline__ uint32_t bucket_of_bin(uint32_t bin, const SegDesc &s
 // Ends with a barrier.
 template <int SK, int BLOCK>
 __device__ __forceinline__ const int64_t *stage_splits(const SegDesc &sd, int64_t *s_spl) {
-  if (SK == SK_RANGE) return sd.splits;
+  if (SK == SK_RANGE) {
+    __syncthreads();  // the barrier callers rely on (e.g. after zeroing counters)
+    return sd.splits;
+  }
   const bool split = SK == SK_SPLIT || sd.kind == 1;
   if (split)
     for (uint32_t i = threadIdx.x; i <= sd.nsplits; i += BLOCK) s_spl[i] = sd.splits[i];

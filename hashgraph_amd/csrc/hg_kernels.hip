@@ -501,7 +501,7 @@ __global__ __launch_bounds__(BLOCK) void k_probe1(const K *__restrict__ bpart, c
   uint32_t lb[(KPT + 1) / 2], mc[(KPT + 1) / 2];
 #pragma unroll
   for (int j = 0; j < KPT; j++) kr[j] = buf_load<K>(rs, threadIdx.x * (uint32_t)sizeof(K), j * BLOCK * (uint32_t)sizeof(K));
-  lds_fill_keys_offs<K, OFF, BLOCK, sizeof(K) == 4 ? 8 : 4>(s_ak, ka + a_lo, na, s_off, ob, W + 1, (OFF)a_lo);
+  lds_fill_keys_offs<K, OFF, BLOCK, sizeof(K) == 4 ? 12 : 4>(s_ak, ka + a_lo, na, s_off, ob, W + 1, (OFF)a_lo);
 #pragma unroll
   for (int j = 0; j < (KPT + 1) / 2; j++) {
     lb[j] = 0;
@@ -1862,6 +1862,9 @@ template <typename F> static hipError_t dispatch_part(int hk, int sk, F &&f) {
 }
 
 constexpr int PART_KPT = 32;  // tile = 1024 * 32 keys (the engine's tiling)
+// The histogram walks a tile in rounds of 1024 * 16 keys: fewer registers,
+// so two workgroups share a CU.
+constexpr int HIST_KPT = 16;
 
 // Leaves of a learned model the histogram stages in LDS (64 KB; larger
 // models are read from global memory).
@@ -1872,7 +1875,7 @@ static hipError_t launch_part_hist_t(const LaunchPart &lp, hipStream_t st) {
   const uint32_t leaf_cap = lp.hk == HK_RMI ? HIST_LEAF_CAP : 0u;
   const size_t lds = (size_t)((lp.max_cols + 3) & ~3u) * 4 + (size_t)leaf_cap * sizeof(RmiLeaf);
   return dispatch_part(lp.hk, lp.sk, [&](auto h, auto b) {
-    hipLaunchKernelGGL((k_part_hist<K, 1024, PART_KPT, decltype(h)::value, decltype(b)::value>), dim3(lp.ntiles),
+    hipLaunchKernelGGL((k_part_hist<K, 1024, HIST_KPT, decltype(h)::value, decltype(b)::value>), dim3(lp.ntiles),
                        dim3(1024), lds, st, (const K *)lp.keys, lp.tiles, lp.segs, lp.cmat, leaf_cap);
     return hipGetLastError();
   });
@@ -2117,8 +2120,8 @@ hipError_t set_kernel_attributes() {
     for (int sk = 0; sk < 3; sk++) {
       HG_SET_D(dispatch_part(hk, sk, [&](auto h, auto b) {
         constexpr int H = decltype(h)::value, S = decltype(b)::value;
-        hipError_t r = set_lds(&k_part_hist<uint32_t, 1024, PART_KPT, H, S>);
-        if (r == hipSuccess) r = set_lds(&k_part_hist<uint64_t, 1024, PART_KPT, H, S>);
+        hipError_t r = set_lds(&k_part_hist<uint32_t, 1024, HIST_KPT, H, S>);
+        if (r == hipSuccess) r = set_lds(&k_part_hist<uint64_t, 1024, HIST_KPT, H, S>);
         if (r == hipSuccess) r = set_lds(&k_part_scatter<uint32_t, 1024, PART_KPT, H, S>);
         if (r == hipSuccess) r = set_lds(&k_part_scatter<uint64_t, 1024, PART_KPT, H, S>);
         return r;
