@@ -138,7 +138,15 @@ __global__ __launch_bounds__(256) void k_col_chunksum(const uint32_t *__restrict
   if (b >= sd.ncols) return;
   const uint32_t *p = cmat + sd.row_base + (uint64_t)it.row_lo * sd.ncols + b;
   uint32_t s = 0;
-  for (uint32_t r = it.row_lo; r < it.row_hi; r++, p += sd.ncols) s += *p;
+  // rows in groups of 16: the group's loads are all in flight before the sums
+  for (uint32_t r = it.row_lo; r < it.row_hi; r += 16, p += 16 * (uint64_t)sd.ncols) {
+    const uint32_t n = min(16u, it.row_hi - r);
+    uint32_t v[16];
+#pragma unroll
+    for (uint32_t u = 0; u < 16; u++) v[u] = u < n ? p[(uint64_t)u * sd.ncols] : 0u;
+#pragma unroll
+    for (uint32_t u = 0; u < 16; u++) s += v[u];
+  }
   psum[psum_base[it.seg] + (uint64_t)it.chunk * sd.ncols + b] = s;
 }
 
@@ -198,10 +206,18 @@ __global__ __launch_bounds__(256) void k_col_apply(uint32_t *__restrict__ cmat, 
   if (b >= sd.ncols) return;
   uint32_t base = bstart[sd.bucket_base + b] + psum[psum_base[it.seg] + (uint64_t)it.chunk * sd.ncols + b];
   uint32_t *p = cmat + sd.row_base + (uint64_t)it.row_lo * sd.ncols + b;
-  for (uint32_t r = it.row_lo; r < it.row_hi; r++, p += sd.ncols) {
-    uint32_t v = *p;
-    *p = base;
-    base += v;
+  // rows in groups of 16: loads first, then the stores of the positions
+  for (uint32_t r = it.row_lo; r < it.row_hi; r += 16, p += 16 * (uint64_t)sd.ncols) {
+    const uint32_t n = min(16u, it.row_hi - r);
+    uint32_t v[16];
+#pragma unroll
+    for (uint32_t u = 0; u < 16; u++) v[u] = u < n ? p[(uint64_t)u * sd.ncols] : 0u;
+#pragma unroll
+    for (uint32_t u = 0; u < 16; u++)
+      if (u < n) {
+        p[(uint64_t)u * sd.ncols] = base;
+        base += v[u];
+      }
   }
 }
 
