@@ -163,10 +163,18 @@ __global__ __launch_bounds__(256) void k_col_chunkscan(const ColItem *__restrict
   const uint32_t nch = nchunks[it.seg];
   uint32_t *p = psum + psum_base[it.seg] + b;
   uint32_t run = 0;
-  for (uint32_t c = 0; c < nch; c++, p += sd.ncols) {
-    const uint32_t v = *p;
-    *p = run;
-    run += v;
+  // chunks in groups of 16: loads first, then the stores
+  for (uint32_t c = 0; c < nch; c += 16, p += 16 * (uint64_t)sd.ncols) {
+    const uint32_t n = min(16u, nch - c);
+    uint32_t v[16];
+#pragma unroll
+    for (uint32_t u = 0; u < 16; u++) v[u] = u < n ? p[(uint64_t)u * sd.ncols] : 0u;
+#pragma unroll
+    for (uint32_t u = 0; u < 16; u++)
+      if (u < n) {
+        p[(uint64_t)u * sd.ncols] = run;
+        run += v[u];
+      }
   }
   coltot[sd.bucket_base + b] = run;
 }
@@ -1930,7 +1938,10 @@ hipError_t launch_colscan(const LaunchCol &lc, hipStream_t st) {
   if (lc.nitems2)
     hipLaunchKernelGGL(k_col_chunkscan, dim3(lc.nitems2), dim3(256), 0, st, lc.items2, lc.segs, lc.psum_base, lc.nchunks,
                        lc.psum, lc.coltot);
-  hipLaunchKernelGGL((k_seg_bucketscan<256>), dim3(lc.nsegs), dim3(256), 0, st, lc.segs, lc.coltot, lc.bstart);
+  if (lc.nsegs < 64)  // few wide segments: more threads per segment
+    hipLaunchKernelGGL((k_seg_bucketscan<1024>), dim3(lc.nsegs), dim3(1024), 0, st, lc.segs, lc.coltot, lc.bstart);
+  else
+    hipLaunchKernelGGL((k_seg_bucketscan<256>), dim3(lc.nsegs), dim3(256), 0, st, lc.segs, lc.coltot, lc.bstart);
   if (lc.nitems)
     hipLaunchKernelGGL(k_col_apply, dim3(lc.nitems), dim3(256), 0, st, lc.cmat, lc.items, lc.segs, lc.psum_base, lc.psum,
                        lc.bstart);
