@@ -502,7 +502,7 @@ __global__ __launch_bounds__(BLOCK) void k_probe1(const K *__restrict__ bpart, c
                                                   const BinMap *__restrict__ maps, const OFF *__restrict__ offA,
                                                   const K *__restrict__ keysA, K *__restrict__ out, uint64_t capacity,
                                                   unsigned long long *__restrict__ cursor, uint32_t lds_bytes,
-                                                  unsigned int *__restrict__ hmark) {
+                                                  unsigned int *__restrict__ hmark, unsigned int *__restrict__ gate) {
   extern __shared__ __align__(16) uint8_t smem[];
   __shared__ uint32_t s_tmp[BLOCK / 64 + 1];
   __shared__ unsigned long long s_base;
@@ -510,8 +510,10 @@ __global__ __launch_bounds__(BLOCK) void k_probe1(const K *__restrict__ bpart, c
   const uint32_t W = it.nbins;
   const OFF *ob = offA + it.off_pos;
   const uint64_t a_lo = (uint64_t)ob[0], a_hi = (uint64_t)ob[W];
-  if (!probe1_takes(it.count, a_hi - a_lo, W, (uint32_t)sizeof(K), BLOCK * KPT, lds_bytes))
-    return;  // k_probe's or k_probe_big's item
+  if (!probe1_takes(it.count, a_hi - a_lo, W, (uint32_t)sizeof(K), BLOCK * KPT, lds_bytes)) {
+    if (threadIdx.x == 0) atomicOr(&gate[0], 1u);  // k_probe's or k_probe_big's item: they run
+    return;
+  }
   const uint32_t na = (uint32_t)(a_hi - a_lo);
   uint16_t *s_off = (uint16_t *)smem;
   K *s_ak = (K *)(smem + probe1_off_bytes(W));
@@ -552,6 +554,7 @@ __global__ __launch_bounds__(BLOCK) void k_probe1(const K *__restrict__ bpart, c
     uint32_t len = s_off[l + 1] - b0;
     if (len > PROBE_BIN_LIMIT) {
       hmark[blockIdx.x] = 1u;  // k_probe_heavy answers this bin
+      gate[1] = 1u;
       len = 0;
     }
     uint32_t m = 0;
@@ -597,13 +600,14 @@ __global__ __launch_bounds__(BLOCK) void k_probe(const K *__restrict__ bpart, co
                                                  const K *__restrict__ keysA, K *__restrict__ out, uint64_t capacity,
                                                  unsigned long long *__restrict__ cursor, uint32_t akey_cap,
                                                  unsigned int *__restrict__ hmark, uint32_t one_cnt,
-                                                 uint32_t one_lds) {
+                                                 uint32_t one_lds, unsigned int *__restrict__ gate) {
   extern __shared__ __align__(16) uint8_t smem[];
   __shared__ uint32_t s_tmp[BLOCK / 64 + 1];
   __shared__ uint32_t s_n;
   __shared__ unsigned long long s_base;
   OFF *s_off = (OFF *)smem;
   K *s_ak = (K *)(smem + ((SUB + 2) * sizeof(OFF) + 15) / 16 * 16);
+  if (gate[0] == 0) return;  // k_probe1 took every item
   const FinItem it = items[blockIdx.x];
   if (it.count > BLOCK * KPT || it.count > 65535u) return;  // k_probe_big's item
   if (probe1_takes(it.count, (uint64_t)(offA[it.off_pos + it.nbins] - offA[it.off_pos]), it.nbins,
@@ -656,7 +660,10 @@ __global__ __launch_bounds__(BLOCK) void k_probe(const K *__restrict__ bpart, co
       if (j < nv && l < n) {
         const uint32_t b0 = (uint32_t)(s_off[l] - a0), e = (uint32_t)(s_off[l + 1] - a0);
         uint32_t m = 0;
-        if (e - b0 > PROBE_BIN_LIMIT) hmark[blockIdx.x] = 1u;  // k_probe_heavy answers this bin
+        if (e - b0 > PROBE_BIN_LIMIT) {  // k_probe_heavy answers this bin
+          hmark[blockIdx.x] = 1u;
+          gate[1] = 1u;
+        }
         else
           for (uint32_t q = b0; q < e; q++) m += s_ak[q] == kr[j];
         mc[j >> 1] += m << sh;  // m <= PROBE_BIN_LIMIT: no carry into the other half
@@ -1709,29 +1716,49 @@ __global__ __launch_bounds__(BLOCK) void k_probe_big(const K *__restrict__ bpart
                                                      const BinMap *__restrict__ maps, const OFF *__restrict__ offA,
                                                      const K *__restrict__ keysA, K *__restrict__ out, uint64_t capacity,
                                                      unsigned long long *__restrict__ cursor, uint32_t small_cap,
-                                                     unsigned int *__restrict__ hmark) {
+                                                     unsigned int *__restrict__ hmark,
+                                                     unsigned int *__restrict__ gate) {
   __shared__ unsigned long long s_t64[BLOCK / 64 + 1];
   __shared__ unsigned long long s_base;
+  if (gate[0] == 0) return;  // k_probe1 took every item
   const FinItem it = items[blockIdx.x];
   if (it.count <= small_cap) return;
   const BinMap mp = maps[it.table];
   const K *kp = bpart + it.key_start;
   const K *ka = keysA + it.key_rel;
-  for (uint64_t c0 = 0; c0 < it.count; c0 += BLOCK) {
-    const uint64_t i = c0 + threadIdx.x;
-    unsigned long long mine = 0;
-    K k = 0;
-    uint64_t s = 0, e = 0;
-    if (i < it.count) {
-      k = kp[i];
-      const uint64_t l = bin_of<K>(k, mp) - it.bin_lo;
-      s = (uint64_t)offA[it.off_pos + l];
-      e = (uint64_t)offA[it.off_pos + l + 1];
-      if (e - s > PROBE_BIN_LIMIT) {
-        hmark[blockIdx.x] = 1u;  // k_probe_heavy answers this bin
-        e = s;
+  // U keys per thread per pass: every load of a pass is in flight at once
+  constexpr int U = 8;
+  for (uint64_t c0 = 0; c0 < it.count; c0 += (uint64_t)BLOCK * U) {
+    K kk[U];
+    uint64_t ss[U], ee[U];
+    uint32_t mm[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const uint64_t i = c0 + (uint64_t)u * BLOCK + threadIdx.x;
+      kk[u] = i < it.count ? kp[i] : (K)0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const uint64_t i = c0 + (uint64_t)u * BLOCK + threadIdx.x;
+      ss[u] = ee[u] = 0;
+      if (i < it.count) {
+        const uint64_t l = bin_of<K>(kk[u], mp) - it.bin_lo;
+        ss[u] = (uint64_t)offA[it.off_pos + l];
+        ee[u] = (uint64_t)offA[it.off_pos + l + 1];
       }
-      for (uint64_t q = s; q < e; q++) mine += ka[q] == k;
+    }
+    unsigned long long mine = 0;
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      if (ee[u] - ss[u] > PROBE_BIN_LIMIT) {
+        hmark[blockIdx.x] = 1u;  // k_probe_heavy answers this bin
+        gate[1] = 1u;
+        ee[u] = ss[u];
+      }
+      uint32_t m = 0;
+      for (uint64_t q = ss[u]; q < ee[u]; q++) m += ka[q] == kk[u];
+      mm[u] = m;
+      mine += m;
     }
     if (out == nullptr) {
       const unsigned long long t = block_sum<BLOCK, unsigned long long>(mine, s_t64);
@@ -1743,11 +1770,13 @@ __global__ __launch_bounds__(BLOCK) void k_probe_big(const K *__restrict__ bpart
     if (threadIdx.x == 0) s_base = tot ? atomicAdd(cursor + it.table, tot) : 0ull;
     __syncthreads();
     unsigned long long w = s_base + ex;
-    for (uint64_t q = s; q < e && mine; q++)
-      if (ka[q] == k) {
-        if (w < capacity) out[w] = k;
-        w++;
-      }
+#pragma unroll
+    for (int u = 0; u < U; u++)
+      for (uint64_t q = ss[u]; q < ee[u] && mm[u]; q++)
+        if (ka[q] == kk[u]) {
+          if (w < capacity) out[w] = kk[u];
+          w++;
+        }
     __syncthreads();
   }
 }
@@ -1768,14 +1797,15 @@ __global__ __launch_bounds__(BLOCK) void k_probe_heavy(const K *__restrict__ bpa
                                                        const K *__restrict__ keysA, K *__restrict__ out,
                                                        uint64_t capacity, unsigned long long *__restrict__ cursor,
                                                        const unsigned int *__restrict__ hmark,
-                                                       unsigned int *__restrict__ flag) {
+                                                       unsigned int *__restrict__ flag,
+                                                       const unsigned int *__restrict__ gate) {
   constexpr K EMPTY = (K)~(K)0;
   __shared__ K s_key[HT_SLOTS];
   __shared__ uint32_t s_cnt[HT_SLOTS];
   __shared__ uint32_t s_hl[BLOCK];
   __shared__ uint32_t s_nh, s_fill, s_ecnt;
   __shared__ unsigned long long s_t64[BLOCK / 64 + 1], s_base;
-  if (!hmark[blockIdx.x]) return;
+  if (gate[1] == 0 || !hmark[blockIdx.x]) return;
   const FinItem it = items[blockIdx.x];
   const K *kp = bpart + it.key_start;
   const K *ka = keysA + it.key_rel;
@@ -1792,39 +1822,57 @@ __global__ __launch_bounds__(BLOCK) void k_probe_heavy(const K *__restrict__ bpa
   };
   // every probe key of the bucket looks its value up in the table
   auto answer = [&]() {
-    for (uint64_t c0 = 0; c0 < it.count; c0 += BLOCK) {
-      const uint64_t i = c0 + threadIdx.x;
-      unsigned long long m = 0;
-      K k = 0;
-      if (i < it.count) {
-        k = kp[i];
-        if (k == EMPTY) {
-          m = s_ecnt;
-        } else {
-          uint32_t slot = murmur4((uint32_t)k ^ (uint32_t)((uint64_t)k >> 32)) & (HT_SLOTS - 1);
-          for (uint32_t pr = 0; pr < HT_SLOTS; pr++, slot = (slot + 1) & (HT_SLOTS - 1)) {
-            const K cur = s_key[slot];
-            if (cur == EMPTY) break;
-            if (cur == k) {
-              m = s_cnt[slot];
-              break;
+    constexpr int U = 8;  // keys per thread per pass (loads in flight together)
+    for (uint64_t c0 = 0; c0 < it.count; c0 += (uint64_t)BLOCK * U) {
+      K kk[U];
+      uint32_t mm[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const uint64_t i = c0 + (uint64_t)u * BLOCK + threadIdx.x;
+        kk[u] = i < it.count ? kp[i] : (K)0;
+      }
+      unsigned long long mine = 0;
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const uint64_t i = c0 + (uint64_t)u * BLOCK + threadIdx.x;
+        const K k = kk[u];
+        uint32_t m = 0;
+        if (i < it.count) {
+          if (k == EMPTY) {
+            m = s_ecnt;
+          } else {
+            uint32_t slot = murmur4((uint32_t)k ^ (uint32_t)((uint64_t)k >> 32)) & (HT_SLOTS - 1);
+            for (uint32_t pr = 0; pr < HT_SLOTS; pr++, slot = (slot + 1) & (HT_SLOTS - 1)) {
+              const K cur = s_key[slot];
+              if (cur == EMPTY) break;
+              if (cur == k) {
+                m = s_cnt[slot];
+                break;
+              }
             }
           }
         }
+        mm[u] = m;
+        mine += m;
       }
       if (out == nullptr) {
-        const unsigned long long t = block_sum<BLOCK, unsigned long long>(m, s_t64);
+        const unsigned long long t = block_sum<BLOCK, unsigned long long>(mine, s_t64);
         if (threadIdx.x == 0 && t) atomicAdd(cursor + it.table, t);
         continue;
       }
       unsigned long long tot;
-      const unsigned long long ex = block_exclusive_sum<BLOCK, unsigned long long>(m, s_t64, tot);
+      const unsigned long long ex = block_exclusive_sum<BLOCK, unsigned long long>(mine, s_t64, tot);
       if (threadIdx.x == 0) s_base = tot ? atomicAdd(cursor + it.table, tot) : 0ull;
       __syncthreads();
-      const unsigned long long w = s_base + ex;
-      if (m && w < capacity) {
-        const unsigned long long lim = min(m, (unsigned long long)capacity - w);
-        for (unsigned long long r = 0; r < lim; r++) out[w + r] = k;
+      unsigned long long w = s_base + ex;
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const unsigned long long m = mm[u];
+        if (m && w < capacity) {
+          const unsigned long long lim = min(m, (unsigned long long)capacity - w);
+          for (unsigned long long r = 0; r < lim; r++) out[w + r] = kk[u];
+        }
+        w += m;
       }
       __syncthreads();
     }
@@ -2005,19 +2053,20 @@ static hipError_t launch_probe_t(const LaunchProbe &p, hipStream_t st) {
   const hipError_t e = dispatch_hk(p.hk, [&](auto h) {
     hipLaunchKernelGGL((k_probe1<K, OFF, 1024, KPT1, decltype(h)::value>), dim3(p.nitems), dim3(1024),
                        (size_t)PROBE1_LDS, st, (const K *)p.bpart, p.items, p.maps, (const OFF *)p.offA,
-                       (const K *)p.keysA, (K *)p.out, p.capacity, p.cursor, PROBE1_LDS, p.hmark);
+                       (const K *)p.keysA, (K *)p.out, p.capacity, p.cursor, PROBE1_LDS, p.hmark, p.gate);
     hipLaunchKernelGGL((k_probe<K, OFF, 1024, KPT, SUB, decltype(h)::value>), dim3(p.nitems), dim3(1024), lds, st,
                        (const K *)p.bpart, p.items, p.maps, (const OFF *)p.offA, (const K *)p.keysA, (K *)p.out,
-                       p.capacity, p.cursor, p.akey_cap, p.hmark, (uint32_t)(1024 * KPT1), PROBE1_LDS);
+                       p.capacity, p.cursor, p.akey_cap, p.hmark, (uint32_t)(1024 * KPT1), PROBE1_LDS, p.gate);
     return hipGetLastError();
   });
   if (e != hipSuccess) return e;
   const uint32_t small_cap = std::min(1024u * KPT, 65535u);
   hipLaunchKernelGGL((k_probe_big<K, OFF, 1024>), dim3(p.nitems), dim3(1024), 0, st, (const K *)p.bpart, p.items, p.maps,
-                     (const OFF *)p.offA, (const K *)p.keysA, (K *)p.out, p.capacity, p.cursor, small_cap, p.hmark);
+                     (const OFF *)p.offA, (const K *)p.keysA, (K *)p.out, p.capacity, p.cursor, small_cap, p.hmark,
+                     p.gate);
   hipLaunchKernelGGL((k_probe_heavy<K, OFF, 1024>), dim3(p.nitems), dim3(1024), 0, st, (const K *)p.bpart, p.items,
                      p.maps, (const OFF *)p.offA, (const K *)p.keysA, (K *)p.out, p.capacity, p.cursor,
-                     (const unsigned int *)p.hmark, p.flag);
+                     (const unsigned int *)p.hmark, p.flag, (const unsigned int *)p.gate);
   return hipGetLastError();
 }
 

@@ -133,3 +133,43 @@ def test_batch_learned_tables(dev):
     total, counts = hg.batch_probe(off, csr, ko, n_bins, off_off, dp, po, hash_kinds=kinds, rmi_models=models,
                                    rmi_leaves=L)
     assert counts.cpu().numpy().tolist() == _expected_counts(keys, ko, probe, po)
+
+
+def test_batch_u64_tables(dev):
+    """uint64 tables (uniform and Zipf(1.1) keys, sizes 0..20000): offsets and
+    canonical keys equal the oracle's, per-table counts equal the bag joins."""
+    import hashgraph_amd as hg
+    rng = np.random.default_rng(17)
+    T = 40
+    sizes = [int(x) for x in rng.integers(0, 20000, T)]
+    sizes[3] = 0
+    parts = [wl.uniform_keys(200 + t, s, 8) if t % 2 == 0 else wl.zipf_keys(300 + t, s, 1.1) for t, s in enumerate(sizes)]
+    keys = np.concatenate(parts) if sum(sizes) else np.zeros(0, dtype=np.uint64)
+    ko = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
+    probe_parts = []
+    for t, s in enumerate(sizes):
+        a = parts[t]
+        half = a[rng.integers(0, max(len(a), 1), s // 2)] if len(a) else np.zeros(0, dtype=np.uint64)
+        probe_parts.append(np.concatenate([half, wl.uniform_keys(400 + t, s - s // 2, 8)]))
+    probe = np.concatenate(probe_parts)
+    po = ko.copy()
+    n_bins = np.maximum(np.array(sizes), 1)
+    print(f"batch u64 T={T} keys={len(keys)}", flush=True)
+    dk, dp = _to_dev(keys, dev), _to_dev(probe, dev)
+    off, csr, off_off = hg.batch_build(dk, ko, n_bins)
+    offn = off.cpu().numpy().view(np.uint32).astype(np.uint64)
+    csrn = csr.cpu().numpy().view(np.uint64)
+    for t in range(T):
+        a = keys[ko[t]:ko[t + 1]]
+        M = int(n_bins[t])
+        eoff, ekeys = o.build_csr(a, M)
+        goff = offn[off_off[t]: off_off[t] + M + 1]
+        assert np.array_equal(goff, eoff), f"table {t} offsets"
+        g = csrn[ko[t]:ko[t + 1]].copy()
+        o.canon(eoff, g)
+        o.canon(eoff, ekeys)
+        assert np.array_equal(g, ekeys), f"table {t} keys"
+    total, counts = hg.batch_probe(off, csr, ko, n_bins, off_off, dp, po)
+    exp = _expected_counts(keys, ko, probe, po)
+    assert counts.cpu().numpy().tolist() == exp
+    assert total == sum(exp)
