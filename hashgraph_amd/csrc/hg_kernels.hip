@@ -1129,10 +1129,11 @@ __device__ __forceinline__ uint32_t wave_same_value(K v) {
 // Adds c copies of v to an LDS value -> count table (compare-and-swap of the
 // key against the EMPTY sentinel; no lane waits on another).
 template <typename K, typename C>
-__device__ __forceinline__ void vt_insert(K *s_key, C *s_cnt, uint32_t *s_fill, K v, uint32_t c) {
+__device__ __forceinline__ void vt_insert(K *s_key, C *s_cnt, uint32_t *s_fill, K v, uint32_t c,
+                                          uint32_t mask = HT_SLOTS - 1) {
   constexpr K EMPTY = (K)~(K)0;
-  uint32_t slot = murmur4((uint32_t)v ^ (uint32_t)((uint64_t)v >> 32)) & (HT_SLOTS - 1);
-  for (uint32_t probe = 0; probe < HT_SLOTS; probe++, slot = (slot + 1) & (HT_SLOTS - 1)) {
+  uint32_t slot = murmur4((uint32_t)v ^ (uint32_t)((uint64_t)v >> 32)) & mask;
+  for (uint32_t probe = 0; probe <= mask; probe++, slot = (slot + 1) & mask) {
     const K old = lds_cas(&s_key[slot], EMPTY, v);
     if (old == EMPTY || old == v) {
       atomicAdd(&s_cnt[slot], (C)c);
@@ -1143,10 +1144,11 @@ __device__ __forceinline__ void vt_insert(K *s_key, C *s_cnt, uint32_t *s_fill, 
 }
 
 template <typename K, typename C>
-__device__ __forceinline__ void vt_add_existing(const K *s_key, C *s_cnt, K v, uint32_t c) {
+__device__ __forceinline__ void vt_add_existing(const K *s_key, C *s_cnt, K v, uint32_t c,
+                                                uint32_t mask = HT_SLOTS - 1) {
   constexpr K EMPTY = (K)~(K)0;
-  uint32_t slot = murmur4((uint32_t)v ^ (uint32_t)((uint64_t)v >> 32)) & (HT_SLOTS - 1);
-  for (uint32_t probe = 0; probe < HT_SLOTS; probe++, slot = (slot + 1) & (HT_SLOTS - 1)) {
+  uint32_t slot = murmur4((uint32_t)v ^ (uint32_t)((uint64_t)v >> 32)) & mask;
+  for (uint32_t probe = 0; probe <= mask; probe++, slot = (slot + 1) & mask) {
     const K cur = s_key[slot];
     if (cur == EMPTY) return;
     if (cur == v) {
@@ -1380,7 +1382,13 @@ __global__ __launch_bounds__(BLOCK) void k_rle_heavy(const OFF *__restrict__ off
   for (uint64_t h = blockIdx.x; h < nheavy; h += gridDim.x) {
     if (gmark[h] != ~0u) continue;  // a giant bin: k_giant_fold's
     const uint64_t b = heavy[h];
-    for (uint32_t i = threadIdx.x; i < HT_SLOTS; i += BLOCK) {
+    const uint64_t a0 = offA[b], a1 = offA[b + 1], x0 = offB[b], x1 = offB[b + 1];
+    // a table of at least twice A's keys in the bin (its distinct values),
+    // 64 to HT_SLOTS slots: small bins clear and scan a small table
+    uint32_t slots = 64;
+    while (slots < HT_SLOTS && (uint64_t)slots < 2 * (a1 - a0)) slots <<= 1;
+    const uint32_t mask = slots - 1;
+    for (uint32_t i = threadIdx.x; i < slots; i += BLOCK) {
       s_key[i] = EMPTY;
       s_ca[i] = 0;
       s_cb[i] = 0;
@@ -1391,16 +1399,15 @@ __global__ __launch_bounds__(BLOCK) void k_rle_heavy(const OFF *__restrict__ off
       s_ecb = 0;
     }
     __syncthreads();
-    const uint64_t a0 = offA[b], a1 = offA[b + 1], x0 = offB[b], x1 = offB[b + 1];
     for (uint64_t i = a0 + threadIdx.x; i < a1; i += BLOCK) {
       const K v = keysA[i];
       const uint32_t c = wave_same_value(v);
       if (c == 0) continue;
       if (v == EMPTY) atomicAdd(&s_eca, (unsigned long long)c);
-      else vt_insert(s_key, s_ca, &s_fill, v, c);
+      else vt_insert(s_key, s_ca, &s_fill, v, c, mask);
     }
     __syncthreads();
-    if (s_fill > HT_SLOTS / 2) {  // too many distinct values for the table
+    if (s_fill > slots / 2) {  // too many distinct values for the table (only when slots == HT_SLOTS)
       if (threadIdx.x == 0) atomicOr(overflow, 1u);
       __syncthreads();
       continue;
@@ -1410,11 +1417,11 @@ __global__ __launch_bounds__(BLOCK) void k_rle_heavy(const OFF *__restrict__ off
       const uint32_t c = wave_same_value(v);
       if (c == 0) continue;
       if (v == EMPTY) atomicAdd(&s_ecb, (unsigned long long)c);
-      else vt_add_existing(s_key, s_cb, v, c);
+      else vt_add_existing(s_key, s_cb, v, c, mask);
     }
     __syncthreads();
     unsigned long long mine = 0, mp = 0;
-    for (uint32_t i = threadIdx.x; i < HT_SLOTS; i += BLOCK)
+    for (uint32_t i = threadIdx.x; i < slots; i += BLOCK)
       if (s_cb[i]) {
         mine++;
         mp += s_ca[i] * s_cb[i];
@@ -1433,7 +1440,7 @@ __global__ __launch_bounds__(BLOCK) void k_rle_heavy(const OFF *__restrict__ off
     }
     __syncthreads();
     unsigned long long w = s_base + ex;
-    for (uint32_t i = threadIdx.x; i < HT_SLOTS; i += BLOCK)
+    for (uint32_t i = threadIdx.x; i < slots; i += BLOCK)
       if (s_cb[i]) {
         if (w < capacity) {
           rk[w] = s_key[i];
