@@ -340,6 +340,13 @@ __device__ __forceinline__ T wave_sum(T x) {
   return x;
 }
 
+// A value the same in every lane, moved to scalar registers (frees VGPRs
+// in kernels whose block-uniform values come from LDS).
+__device__ __forceinline__ uint32_t uni32(uint32_t x) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)x); }
+__device__ __forceinline__ uint64_t uni64(uint64_t x) {
+  return ((uint64_t)uni32((uint32_t)(x >> 32)) << 32) | uni32((uint32_t)x);
+}
+
 // Exclusive block-wide sum of one value per thread.  s_tmp needs
 // BLOCK/64 + 1 elements of T.  Ends with a barrier so s_tmp can be reused.
 template <int BLOCK, typename T>

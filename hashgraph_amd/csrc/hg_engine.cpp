@@ -1032,13 +1032,16 @@ hg_status hg_intersect_rle(hg_ctx c, const hg_table_desc *descA, const void *d_o
   const uint64_t M = descA->n_bins, nch = rle_chunks(M);
   const uint64_t heavy_cap = std::min<uint64_t>(M, 1u << 22);
   Plan P;
-  const size_t i_cnt = P.add((nch + 1) * 8), i_base = P.add((nch + 1) * 8), i_small = P.add(256),
+  const size_t i_cnt = P.add((nch + 1) * 8), i_base = P.add((nch + 1) * 8), i_hcnt = P.add((nch + 1) * 4),
+               i_hbase = P.add((nch + 1) * 4), i_pairs = P.add((nch + 1) * 8), i_small = P.add(256),
                i_heavy = P.add(heavy_cap * 4), i_status = P.add(scan_tiles(nch) * 8 + 64),
-               i_gmark = P.add(heavy_cap * 4), i_giant = P.add(rle_giant_bytes((int)descA->key_bytes));
+               i_gmark = P.add(heavy_cap * 4), i_giant = P.add(rle_giant_bytes((int)descA->key_bytes)),
+               i_pkey = P.add(rle_priv_records() * descA->key_bytes), i_pca = P.add(rle_priv_records() * 8),
+               i_pcb = P.add(rle_priv_records() * 8), i_pn = P.add((size_t)rle_heavy_grid() * 4);
   HG_TRY(arena_ensure(c, 1, P.total()));
   Carve cv;
   cv.init(c, 1, P);
-  unsigned long long *small = cv.get<unsigned long long>(i_small);  // [heavy_n, pairs, cursor, overflow]
+  unsigned long long *small = cv.get<unsigned long long>(i_small);  // [-, pairs, cursor, overflow]
   HG_HIP(launch_fill_u64(small, 0ull, 8, c->stream));
   LaunchRle r{};
   r.key_bytes = (int)descA->key_bytes;
@@ -1050,11 +1053,17 @@ hg_status hg_intersect_rle(hg_ctx c, const hg_table_desc *descA, const void *d_o
   r.keysB = d_keysB;
   r.chunk_cnt = cv.get<unsigned long long>(i_cnt);
   r.chunk_base = cv.get<unsigned long long>(i_base);
+  r.chunk_heavy = cv.get<uint32_t>(i_hcnt);
+  r.heavy_base = cv.get<uint32_t>(i_hbase);
+  r.chunk_pairs = cv.get<unsigned long long>(i_pairs);
+  r.pkey = cv.get<char>(i_pkey);
+  r.pca = cv.get<unsigned long long>(i_pca);
+  r.pcb = cv.get<unsigned long long>(i_pcb);
+  r.priv_n = cv.get<uint32_t>(i_pn);
   r.rk = d_rec_keys;
   r.rca = (unsigned long long *)d_rec_ca;
   r.rcb = (unsigned long long *)d_rec_cb;
   r.capacity = capacity;
-  r.heavy_n = small;
   r.heavy = cv.get<uint32_t>(i_heavy);
   r.heavy_cap = heavy_cap;
   r.pairs = small + 1;
@@ -1075,23 +1084,27 @@ hg_status hg_intersect_rle(hg_ctx c, const hg_table_desc *descA, const void *d_o
     Prof pf(c, "rle_small");
     HG_HIP(launch_rle(r, c->stream));
   }
-  // record positions of the small bins: exclusive scan of the chunk counts
-  HG_HIP(launch_scan(r.chunk_cnt, (void *)r.chunk_base, nch, 8, cv.get<unsigned long long>(i_status),
-                     (unsigned int *)(cv.get<unsigned long long>(i_status) + scan_tiles(nch)), c->stream));
-  unsigned long long hv[2];
-  HG_TRY(sync_read(c, hv, small, 16));
+  // record positions of the small bins and heavy-list positions: exclusive
+  // scans of the chunk counts
+  unsigned long long *status = cv.get<unsigned long long>(i_status);
+  unsigned int *ticket = (unsigned int *)(status + scan_tiles(nch));
+  HG_HIP(launch_scan(r.chunk_cnt, (void *)r.chunk_base, nch, 8, status, ticket, c->stream));
+  HG_HIP(launch_scan(r.chunk_heavy, (void *)r.heavy_base, nch, 4, status, ticket, c->stream));
   unsigned long long nsmall = 0;
+  uint32_t nheavy = 0;
   HG_TRY(sync_read(c, &nsmall, r.chunk_base + nch, 8));
-  if (hv[0] > heavy_cap) return fail(HG_ERR_UNSUPPORTED, "too many heavy bins");
+  HG_TRY(sync_read(c, &nheavy, r.heavy_base + nch, 4));
+  if (nheavy > heavy_cap) return fail(HG_ERR_UNSUPPORTED, "too many heavy bins");
+  // pass 1 lists the heavy bins and, with a capacity, writes the records
   r.pass = 1;
-  if (capacity) {
+  {
     Prof pf(c, "rle_small");
     HG_HIP(launch_rle(r, c->stream));
   }
   // heavy bins append after the small bins' records
   HG_HIP(launch_fill_u64(small + 2, nsmall, 1, c->stream));
   r.pass = 2;
-  r.nheavy = hv[0];
+  r.nheavy = nheavy;
   {
     Prof pf(c, "rle_heavy");
     HG_HIP(launch_rle(r, c->stream));

@@ -902,7 +902,7 @@ __global__ __launch_bounds__(1024) void k_range_hist(const K *__restrict__ keys,
 // ---------------------------------------------------------------------------
 constexpr uint32_t RLE_SMALL = 64;
 constexpr uint32_t RLE_CHUNK = 1024;
-constexpr uint32_t RLE_KCAP = 2048;
+constexpr uint32_t RLE_KCAP = 1024;
 constexpr uint32_t RLE_TINY = 8;  // bins above this on a side (up to RLE_SMALL) are walked by a wave
 
 // A medium bin (A's keys a0.., a of them; B's x0.., b of them; a, b <= 64)
@@ -926,29 +926,35 @@ __device__ __forceinline__ bool rle_medium(uint64_t a0, uint32_t a, uint64_t x0,
   return mine && first && cb > 0;
 }
 
-// pass 0 counts (and lists heavy bins when heavy_n is non-null), pass 1
-// writes from w on
-template <typename K, int PER, typename OFF, typename GA, typename GB>
-__device__ __forceinline__ void rle_walk(const OFF *s_oa, const OFF *s_ob, uint32_t nb, int pass, GA getA, GB getB,
-                                         uint64_t b0, unsigned long long &nrec, unsigned long long &npair,
-                                         unsigned long long *heavy_n, uint32_t *heavy, uint64_t heavy_cap,
-                                         unsigned long long w, K *rk, unsigned long long *rca, unsigned long long *rcb,
-                                         uint64_t capacity, uint32_t &medmask) {
-  medmask = 0;
+// A thread's walk over its PER bins of the chunk.  WRITE == false counts the
+// tiny bins' records and pairs, the heavy bins, and marks the medium bins in
+// medmask; WRITE == true lists the heavy bins from hw on and writes the tiny
+// bins' records from w on.  Without emit only the heavy bins are looked at.
+template <typename K, int PER, bool WRITE, typename OFF, typename GA, typename GB>
+__device__ __forceinline__ void rle_walk(const OFF *s_oa, const OFF *s_ob, uint32_t lo, uint32_t hi, bool emit,
+                                         GA getA, GB getB,
+                                         uint64_t b0, uint32_t &nrec, unsigned long long &npair, uint32_t &nheavy,
+                                         uint32_t &medmask, unsigned long long w, uint64_t hw, uint32_t *heavy,
+                                         uint64_t heavy_cap, K *rk, unsigned long long *rca, unsigned long long *rcb,
+                                         uint64_t capacity) {
   for (int q = 0; q < PER; q++) {
     const uint32_t l = threadIdx.x * PER + q;
-    if (l >= nb) break;
+    if (l >= hi) break;
+    if (l < lo) continue;
     const uint64_t a0 = s_oa[l], a1 = s_oa[l + 1], x0 = s_ob[l], x1 = s_ob[l + 1];
     if (a1 == a0 || x1 == x0) continue;
     if (a1 - a0 > RLE_SMALL || x1 - x0 > RLE_SMALL) {
-      if (heavy_n) {
-        const unsigned long long h = atomicAdd(heavy_n, 1ull);
-        if (h < heavy_cap) heavy[h] = (uint32_t)(b0 + l);
+      if (WRITE) {
+        if (hw < heavy_cap) heavy[hw] = (uint32_t)(b0 + l);
+        hw++;
+      } else {
+        nheavy++;
       }
       continue;
     }
+    if (!emit) continue;
     if (a1 - a0 > RLE_TINY || x1 - x0 > RLE_TINY) {  // a wave takes it (rle_medium)
-      medmask |= 1u << q;
+      if (!WRITE) medmask |= 1u << q;
       continue;
     }
     for (uint64_t i = a0; i < a1; i++) {
@@ -960,7 +966,7 @@ __device__ __forceinline__ void rle_walk(const OFF *s_oa, const OFF *s_ob, uint3
       for (uint64_t j = i; j < a1; j++) ca += getA(j) == v;
       for (uint64_t j = x0; j < x1; j++) cb += getB(j) == v;
       if (!cb) continue;
-      if (pass == 0) {
+      if (!WRITE) {
         nrec++;
         npair += ca * cb;
       } else {
@@ -975,127 +981,213 @@ __device__ __forceinline__ void rle_walk(const OFF *s_oa, const OFF *s_ob, uint3
   }
 }
 
+// Pass 0 stores per chunk its record count, heavy-bin count and pair count
+// (plain stores: one atomic per chunk on a shared counter would serialise the
+// whole grid); pass 1, after the host scanned the first two, lists the heavy
+// bins at the chunk's heavy prefix and, when emit, writes the records at the
+// chunk's record prefix.  Pass 1 without emit only lists the heavy bins.
 template <typename K, typename OFF, int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_rle_small(const OFF *__restrict__ offA, const K *__restrict__ keysA,
+__global__ __launch_bounds__(BLOCK, 5) void k_rle_small(const OFF *__restrict__ offA, const K *__restrict__ keysA,
                                                      const OFF *__restrict__ offB, const K *__restrict__ keysB, uint64_t M,
                                                      int pass, unsigned long long *__restrict__ chunk_cnt,
                                                      const unsigned long long *__restrict__ chunk_base,
-                                                     K *__restrict__ rk, unsigned long long *__restrict__ rca,
+                                                     uint32_t *__restrict__ chunk_heavy,
+                                                     const uint32_t *__restrict__ heavy_base,
+                                                     unsigned long long *__restrict__ chunk_pairs, K *__restrict__ rk,
+                                                     unsigned long long *__restrict__ rca,
                                                      unsigned long long *__restrict__ rcb, uint64_t capacity,
-                                                     unsigned long long *__restrict__ heavy_n, uint32_t *__restrict__ heavy,
-                                                     uint64_t heavy_cap, unsigned long long *__restrict__ pairs) {
+                                                     uint32_t *__restrict__ heavy, uint64_t heavy_cap) {
   constexpr int PER = RLE_CHUNK / BLOCK;
+  constexpr int KPT = RLE_KCAP / BLOCK;
   __shared__ unsigned long long s_t64[BLOCK / 64 + 1];
   __shared__ OFF s_oa[RLE_CHUNK + 1], s_ob[RLE_CHUNK + 1];
   __shared__ K s_ka[RLE_KCAP], s_kb[RLE_KCAP];
   const uint64_t b0 = (uint64_t)blockIdx.x * RLE_CHUNK;
   const uint32_t nb = (uint32_t)min((uint64_t)RLE_CHUNK, M - b0);
-  for (uint32_t i = threadIdx.x; i <= nb; i += BLOCK) {
-    s_oa[i] = offA[b0 + i];
-    s_ob[i] = offB[b0 + i];
-  }
-  __syncthreads();
-  const uint64_t ka0 = s_oa[0], kb0 = s_ob[0];
-  const uint64_t na = (uint64_t)s_oa[nb] - ka0, nx = (uint64_t)s_ob[nb] - kb0;
-  const bool staged = na <= RLE_KCAP && nx <= RLE_KCAP;  // uniform
-  if (staged) {
-    for (uint32_t i = threadIdx.x; i < (uint32_t)na; i += BLOCK) s_ka[i] = keysA[ka0 + i];
-    for (uint32_t i = threadIdx.x; i < (uint32_t)nx; i += BLOCK) s_kb[i] = keysB[kb0 + i];
-    __syncthreads();
-  }
-  auto lA = [&](uint64_t i) { return s_ka[i - ka0]; };
-  auto lB = [&](uint64_t i) { return s_kb[i - kb0]; };
-  auto gA = [&](uint64_t i) { return keysA[i]; };
-  auto gB = [&](uint64_t i) { return keysB[i]; };
-  unsigned long long nrec = 0, npair = 0;
-  uint32_t medmask = 0;
-  // per-thread walk of the tiny bins (counting; pass 0 also lists heavy bins)
-  if (staged)
-    rle_walk<K, PER>(s_oa, s_ob, nb, 0, lA, lB, b0, nrec, npair, pass == 0 ? heavy_n : nullptr, heavy, heavy_cap, 0ull,
-                     rk, rca, rcb, 0, medmask);
-  else
-    rle_walk<K, PER>(s_oa, s_ob, nb, 0, gA, gB, b0, nrec, npair, pass == 0 ? heavy_n : nullptr, heavy, heavy_cap, 0ull,
-                     rk, rca, rcb, 0, medmask);
-  // list of the chunk's medium bins (thread order: deterministic)
-  __shared__ uint16_t s_med[RLE_CHUNK];
-  __shared__ uint32_t s_mcnt[RLE_CHUNK];
-  __shared__ unsigned long long s_mbase;
-  unsigned long long nmed_tot;
-  const unsigned long long mex = block_exclusive_sum<BLOCK, unsigned long long>(
-      (unsigned long long)__popc(medmask), s_t64, nmed_tot);
+  const bool emit = pass == 0 || capacity != 0;
   {
-    uint32_t k = (uint32_t)mex;
-    for (int q = 0; q < PER; q++)
-      if (medmask & (1u << q)) s_med[k++] = (uint16_t)(threadIdx.x * PER + q);
-  }
-  __syncthreads();
-  const uint32_t nmed = (uint32_t)nmed_tot;
-  const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  constexpr uint32_t NW = BLOCK / 64;
-  // medium bins: one wave each; per-bin record counts to s_mcnt
-  unsigned long long mrec = 0, mpair = 0;
-  for (uint32_t k = wv; k < nmed; k += NW) {
-    const uint32_t l = s_med[k];
-    const uint64_t a0 = s_oa[l], x0 = s_ob[l];
-    const uint32_t a = (uint32_t)((uint64_t)s_oa[l + 1] - a0), b = (uint32_t)((uint64_t)s_ob[l + 1] - x0);
-    K v;
-    uint32_t ca, cb;
-    const bool rec = staged ? rle_medium<K>(a0, a, x0, b, lA, lB, lane, v, ca, cb)
-                            : rle_medium<K>(a0, a, x0, b, gA, gB, lane, v, ca, cb);
-    const uint64_t bal = __ballot(rec);
-    if (lane == 0) s_mcnt[k] = (uint32_t)__popcll(bal);
-    mrec += rec ? 1ull : 0ull;
-    mpair += rec ? (unsigned long long)ca * cb : 0ull;
-  }
-  if (pass == 0) {
-    unsigned long long tot;
-    block_exclusive_sum<BLOCK, unsigned long long>(nrec + mrec, s_t64, tot);
-    const unsigned long long pt = block_sum<BLOCK, unsigned long long>(npair + mpair, s_t64);
-    if (threadIdx.x == 0) {
-      chunk_cnt[blockIdx.x] = tot;
-      if (pt) atomicAdd(pairs, pt);
-    }
-    return;
-  }
-  // pass 1: the tiny bins' records at the chunk's prefix plus the threads
-  // before, then the medium bins' records after all of them
-  unsigned long long tot_small;
-  const unsigned long long ex = block_exclusive_sum<BLOCK, unsigned long long>(nrec, s_t64, tot_small);
-  const unsigned long long w = chunk_base[blockIdx.x] + ex;
-  if (staged)
-    rle_walk<K, PER>(s_oa, s_ob, nb, 1, lA, lB, b0, nrec, npair, nullptr, nullptr, 0, w, rk, rca, rcb, capacity,
-                     medmask);
-  else
-    rle_walk<K, PER>(s_oa, s_ob, nb, 1, gA, gB, b0, nrec, npair, nullptr, nullptr, 0, w, rk, rca, rcb, capacity,
-                     medmask);
-  if (nmed == 0) return;  // uniform
-  if (threadIdx.x == 0) {
-    unsigned long long acc = chunk_base[blockIdx.x] + tot_small;
-    for (uint32_t k = 0; k < nmed; k++) {  // record base of each medium bin
-      const uint32_t c = s_mcnt[k];
-      s_mcnt[k] = (uint32_t)(acc - chunk_base[blockIdx.x]);
-      acc += c;
-    }
-    s_mbase = chunk_base[blockIdx.x];
-  }
-  __syncthreads();
-  for (uint32_t k = wv; k < nmed; k += NW) {
-    const uint32_t l = s_med[k];
-    const uint64_t a0 = s_oa[l], x0 = s_ob[l];
-    const uint32_t a = (uint32_t)((uint64_t)s_oa[l + 1] - a0), b = (uint32_t)((uint64_t)s_ob[l + 1] - x0);
-    K v;
-    uint32_t ca, cb;
-    const bool rec = staged ? rle_medium<K>(a0, a, x0, b, lA, lB, lane, v, ca, cb)
-                            : rle_medium<K>(a0, a, x0, b, gA, gB, lane, v, ca, cb);
-    const uint64_t bal = __ballot(rec);
-    if (rec) {
-      const unsigned long long p = s_mbase + s_mcnt[k] + (unsigned long long)__popcll(bal & ((1ull << lane) - 1ull));
-      if (p < capacity) {
-        rk[p] = v;
-        rca[p] = ca;
-        rcb[p] = cb;
+    // all offset loads of the thread in flight together (one round trip)
+    OFF va[PER + 1], vb[PER + 1];
+#pragma unroll
+    for (int j = 0; j <= PER; j++) {
+      const uint32_t i = threadIdx.x + j * BLOCK;
+      if (i <= nb) {
+        va[j] = offA[b0 + i];
+        vb[j] = offB[b0 + i];
       }
     }
+#pragma unroll
+    for (int j = 0; j <= PER; j++) {
+      const uint32_t i = threadIdx.x + j * BLOCK;
+      if (i <= nb) {
+        s_oa[i] = va[j];
+        s_ob[i] = vb[j];
+      }
+    }
+  }
+  __syncthreads();
+  // Sub-ranges: a chunk whose keys exceed RLE_KCAP on a side is walked in
+  // 2 or 4 sub-ranges of bins, each staged on its own (dense tables); when
+  // none fits, in one range reading the keys from global memory.
+  uint32_t nsub = 1;
+  bool staged = false;
+  if (emit) {
+    for (uint32_t ns = 1; ns <= 4 && !staged; ns <<= 1) {
+      const uint32_t span = RLE_CHUNK / ns;
+      bool fits = true;
+      for (uint32_t r = 0; r < ns && r * span < nb; r++) {
+        const uint32_t lo = r * span, hi = min(lo + span, nb);
+        fits &= (uint64_t)s_oa[hi] - (uint64_t)s_oa[lo] <= RLE_KCAP && (uint64_t)s_ob[hi] - (uint64_t)s_ob[lo] <= RLE_KCAP;
+      }
+      if (fits) {
+        nsub = ns;
+        staged = true;
+      }
+    }
+  }
+  // list of the medium bins (thread order: deterministic)
+  __shared__ uint16_t s_med[RLE_CHUNK];
+  __shared__ uint16_t s_mcnt[RLE_CHUNK];  // records of a medium bin, then its base in the sub-range (< 2^16)
+  __shared__ unsigned long long s_mbase;
+  __shared__ uint32_t s_mtot;
+  const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  constexpr uint32_t NW = BLOCK / 64;
+  // pass 0: the chunk's totals (records | heavy << 32, pairs); pass 1: the
+  // records and heavy bins written so far
+  unsigned long long acc = 0, accp = 0;
+  const unsigned long long cbase = pass == 0 ? 0ull : chunk_base[blockIdx.x];
+  const uint64_t hbase = pass == 0 ? 0ull : (uint64_t)heavy_base[blockIdx.x];
+  const uint32_t span = RLE_CHUNK / nsub;
+  for (uint32_t r = 0; r < nsub; r++) {
+    const uint32_t lo = r * span, hi = min(lo + span, nb);
+    if (lo >= nb) break;  // uniform
+    const uint64_t ka0 = uni64(s_oa[lo]), kb0 = uni64(s_ob[lo]);
+    const uint64_t na = uni64((uint64_t)s_oa[hi] - ka0), nx = uni64((uint64_t)s_ob[hi] - kb0);
+    if (staged) {
+      if (r) __syncthreads();  // the previous sub-range is done with s_ka / s_kb
+      K ta[KPT], tb[KPT];
+#pragma unroll
+      for (int j = 0; j < KPT; j++) {
+        const uint32_t i = threadIdx.x + j * BLOCK;
+        if (i < na) ta[j] = keysA[ka0 + i];
+        if (i < nx) tb[j] = keysB[kb0 + i];
+      }
+#pragma unroll
+      for (int j = 0; j < KPT; j++) {
+        const uint32_t i = threadIdx.x + j * BLOCK;
+        if (i < na) s_ka[i] = ta[j];
+        if (i < nx) s_kb[i] = tb[j];
+      }
+      __syncthreads();
+    }
+    auto lA = [&](uint64_t i) { return s_ka[i - ka0]; };
+    auto lB = [&](uint64_t i) { return s_kb[i - kb0]; };
+    auto gA = [&](uint64_t i) { return keysA[i]; };
+    auto gB = [&](uint64_t i) { return keysB[i]; };
+    uint32_t nrec = 0, nheavy = 0, medmask = 0;
+    unsigned long long npair = 0;
+    if (staged)
+      rle_walk<K, PER, false>(s_oa, s_ob, lo, hi, emit, lA, lB, b0, nrec, npair, nheavy, medmask, 0ull, 0ull, heavy, 0,
+                              rk, rca, rcb, 0);
+    else
+      rle_walk<K, PER, false>(s_oa, s_ob, lo, hi, emit, gA, gB, b0, nrec, npair, nheavy, medmask, 0ull, 0ull, heavy, 0,
+                              rk, rca, rcb, 0);
+    unsigned long long nmed_tot;
+    const unsigned long long mex = block_exclusive_sum<BLOCK, unsigned long long>(
+        (unsigned long long)__popc(medmask), s_t64, nmed_tot);
+    {
+      uint32_t k = (uint32_t)mex;
+      for (int q = 0; q < PER; q++)
+        if (medmask & (1u << q)) s_med[k++] = (uint16_t)(threadIdx.x * PER + q);
+    }
+    __syncthreads();
+    const uint32_t nmed = uni32((uint32_t)nmed_tot);
+    // medium bins: one wave each
+    uint32_t mrec = 0;
+    unsigned long long mpair = 0;
+    if (pass == 0)
+      for (uint32_t k = wv; k < nmed; k += NW) {
+        const uint32_t l = s_med[k];
+        const uint64_t a0 = s_oa[l], x0 = s_ob[l];
+        const uint32_t a = (uint32_t)((uint64_t)s_oa[l + 1] - a0), b = (uint32_t)((uint64_t)s_ob[l + 1] - x0);
+        K v;
+        uint32_t ca, cb;
+        const bool rec = staged ? rle_medium<K>(a0, a, x0, b, lA, lB, lane, v, ca, cb)
+                                : rle_medium<K>(a0, a, x0, b, gA, gB, lane, v, ca, cb);
+        mrec += rec ? 1u : 0u;
+        mpair += rec ? (unsigned long long)ca * cb : 0ull;
+      }
+    // records in the low word, heavy bins in the high word (a chunk holds at
+    // most 64 K records and 1024 heavy bins)
+    unsigned long long tot;
+    const unsigned long long ex = block_exclusive_sum<BLOCK, unsigned long long>(
+        (unsigned long long)(nrec + mrec) | ((unsigned long long)nheavy << 32), s_t64, tot);
+    if (pass == 0) {
+      acc += uni64(tot);
+      accp += uni64(block_sum<BLOCK, unsigned long long>(npair + mpair, s_t64));
+      continue;
+    }
+    // pass 1: the tiny bins' records at the sub-range's base plus the threads
+    // before, then the medium bins' records after all of them
+    const unsigned long long rbase = cbase + (acc & 0xFFFFFFFFull);
+    const uint64_t hw = hbase + (acc >> 32) + (ex >> 32);
+    if (staged)
+      rle_walk<K, PER, true>(s_oa, s_ob, lo, hi, emit, lA, lB, b0, nrec, npair, nheavy, medmask,
+                             rbase + (ex & 0xFFFFFFFFull), hw, heavy, heavy_cap, rk, rca, rcb, capacity);
+    else
+      rle_walk<K, PER, true>(s_oa, s_ob, lo, hi, emit, gA, gB, b0, nrec, npair, nheavy, medmask,
+                             rbase + (ex & 0xFFFFFFFFull), hw, heavy, heavy_cap, rk, rca, rcb, capacity);
+    acc += uni64(tot);  // tiny records and heavy bins of the sub-range
+    if (nmed == 0) continue;  // uniform
+    // medium bins: count each bin's records, then the bins' bases in bin order
+    for (uint32_t k = wv; k < nmed; k += NW) {
+      const uint32_t l = s_med[k];
+      const uint64_t a0 = s_oa[l], x0 = s_ob[l];
+      const uint32_t a = (uint32_t)((uint64_t)s_oa[l + 1] - a0), b = (uint32_t)((uint64_t)s_ob[l + 1] - x0);
+      K v;
+      uint32_t ca, cb;
+      const bool rec = staged ? rle_medium<K>(a0, a, x0, b, lA, lB, lane, v, ca, cb)
+                              : rle_medium<K>(a0, a, x0, b, gA, gB, lane, v, ca, cb);
+      const uint64_t bal = __ballot(rec);
+      if (lane == 0) s_mcnt[k] = (uint16_t)__popcll(bal);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t m = 0;
+      for (uint32_t k = 0; k < nmed; k++) {  // record base of each medium bin past the tiny bins' records
+        const uint32_t c = s_mcnt[k];
+        s_mcnt[k] = (uint16_t)m;
+        m += c;
+      }
+      s_mbase = cbase + (acc & 0xFFFFFFFFull);
+      s_mtot = m;
+    }
+    __syncthreads();
+    const unsigned long long mb = uni64(s_mbase);
+    acc += uni32(s_mtot);  // the medium bins' records
+    for (uint32_t k = wv; k < nmed; k += NW) {
+      const uint32_t l = s_med[k];
+      const uint64_t a0 = s_oa[l], x0 = s_ob[l];
+      const uint32_t a = (uint32_t)((uint64_t)s_oa[l + 1] - a0), b = (uint32_t)((uint64_t)s_ob[l + 1] - x0);
+      K v;
+      uint32_t ca, cb;
+      const bool rec = staged ? rle_medium<K>(a0, a, x0, b, lA, lB, lane, v, ca, cb)
+                              : rle_medium<K>(a0, a, x0, b, gA, gB, lane, v, ca, cb);
+      const uint64_t bal = __ballot(rec);
+      if (rec) {
+        const unsigned long long p = mb + s_mcnt[k] + (unsigned long long)__popcll(bal & ((1ull << lane) - 1ull));
+        if (p < capacity) {
+          rk[p] = v;
+          rca[p] = ca;
+          rcb[p] = cb;
+        }
+      }
+    }
+  }
+  if (pass == 0 && threadIdx.x == 0) {
+    chunk_cnt[blockIdx.x] = acc & 0xFFFFFFFFull;
+    chunk_heavy[blockIdx.x] = (uint32_t)(acc >> 32);
+    chunk_pairs[blockIdx.x] = accp;
   }
 }
 
@@ -1106,6 +1198,9 @@ __global__ __launch_bounds__(BLOCK) void k_rle_small(const OFF *__restrict__ off
 // keys equal to the sentinel are counted apart.  A bin whose A side holds
 // more than HT_SLOTS/2 distinct values sets the overflow flag.
 constexpr uint32_t HT_SLOTS = 4096;
+// records a k_rle_heavy workgroup keeps in its private region
+constexpr uint32_t RLE_PRIV_CAP = 1024;
+constexpr uint32_t RLE_HEAVY_GRID = 1024;
 
 __device__ __forceinline__ uint32_t lds_cas(uint32_t *p, uint32_t cmp, uint32_t v) { return atomicCAS(p, cmp, v); }
 __device__ __forceinline__ uint64_t lds_cas(uint64_t *p, uint64_t cmp, uint64_t v) {
@@ -1373,12 +1468,18 @@ __global__ __launch_bounds__(BLOCK) void k_rle_heavy(const OFF *__restrict__ off
                                                      unsigned long long *__restrict__ cursor,
                                                      unsigned long long *__restrict__ pairs,
                                                      unsigned int *__restrict__ overflow,
-                                                     const uint32_t *__restrict__ gmark) {
+                                                     const uint32_t *__restrict__ gmark, K *__restrict__ pk,
+                                                     unsigned long long *__restrict__ pca,
+                                                     unsigned long long *__restrict__ pcb,
+                                                     uint32_t *__restrict__ priv_n) {
   constexpr K EMPTY = (K)~(K)0;
   __shared__ K s_key[HT_SLOTS];
   __shared__ unsigned long long s_ca[HT_SLOTS], s_cb[HT_SLOTS];
-  __shared__ unsigned int s_fill;
+  __shared__ unsigned int s_fill, s_pn, s_priv;
   __shared__ unsigned long long s_eca, s_ecb, s_base, s_t64[BLOCK / 64 + 1];
+  unsigned long long wpairs = 0;  // thread 0: the workgroup's pairs
+  if (threadIdx.x == 0) s_pn = 0;
+  const uint64_t pbase = (uint64_t)blockIdx.x * RLE_PRIV_CAP;
   for (uint64_t h = blockIdx.x; h < nheavy; h += gridDim.x) {
     if (gmark[h] != ~0u) continue;  // a giant bin: k_giant_fold's
     const uint64_t b = heavy[h];
@@ -1435,30 +1536,92 @@ __global__ __launch_bounds__(BLOCK) void k_rle_heavy(const OFF *__restrict__ off
     unsigned long long ex = block_exclusive_sum<BLOCK, unsigned long long>(mine, s_t64, tot);
     const unsigned long long pt = block_sum<BLOCK, unsigned long long>(mp, s_t64);
     if (threadIdx.x == 0) {
-      s_base = tot ? atomicAdd(cursor, tot) : 0ull;
-      if (pt) atomicAdd(pairs, pt);
+      wpairs += pt;
+      // the records go to the workgroup's private region while it has room
+      // (k_rle_compact appends them), else straight to the output
+      if ((uint64_t)s_pn + tot <= RLE_PRIV_CAP) {
+        s_priv = 1;
+        s_base = s_pn;
+        s_pn += (uint32_t)tot;
+      } else {
+        s_priv = 0;
+        s_base = tot ? atomicAdd(cursor, tot) : 0ull;
+      }
     }
     __syncthreads();
+    const bool priv = s_priv != 0;
+    K *ok = priv ? pk + pbase : rk;
+    unsigned long long *oca = priv ? pca + pbase : rca, *ocb = priv ? pcb + pbase : rcb;
+    const uint64_t ocap = priv ? RLE_PRIV_CAP : capacity;
     unsigned long long w = s_base + ex;
     for (uint32_t i = threadIdx.x; i < slots; i += BLOCK)
       if (s_cb[i]) {
-        if (w < capacity) {
-          rk[w] = s_key[i];
-          rca[w] = s_ca[i];
-          rcb[w] = s_cb[i];
+        if (w < ocap) {
+          ok[w] = s_key[i];
+          oca[w] = s_ca[i];
+          ocb[w] = s_cb[i];
         }
         w++;
       }
     if (ehit) {
-      if (w < capacity) {
-        rk[w] = EMPTY;
-        rca[w] = s_eca;
-        rcb[w] = s_ecb;
+      if (w < ocap) {
+        ok[w] = EMPTY;
+        oca[w] = s_eca;
+        ocb[w] = s_ecb;
       }
       w++;
     }
     __syncthreads();
   }
+  if (threadIdx.x == 0) {
+    priv_n[blockIdx.x] = s_pn;
+    if (wpairs) atomicAdd(pairs, wpairs);
+  }
+}
+
+// Appends the private regions of k_rle_heavy's workgroups after the records
+// written so far (workgroup order), then k_rle_finish moves the cursor past
+// them.
+template <typename K>
+__global__ __launch_bounds__(256) void k_rle_compact(const uint32_t *__restrict__ priv_n, const K *__restrict__ pk,
+                                                     const unsigned long long *__restrict__ pca,
+                                                     const unsigned long long *__restrict__ pcb, K *__restrict__ rk,
+                                                     unsigned long long *__restrict__ rca,
+                                                     unsigned long long *__restrict__ rcb, uint64_t capacity,
+                                                     const unsigned long long *__restrict__ cursor) {
+  __shared__ unsigned long long s_t64[256 / 64 + 1];
+  unsigned long long before = 0;
+  for (uint32_t i = threadIdx.x; i < blockIdx.x; i += 256) before += priv_n[i];
+  before = block_sum<256, unsigned long long>(before, s_t64);
+  const uint32_t n = priv_n[blockIdx.x];
+  const unsigned long long base = *cursor + before;
+  const uint64_t pbase = (uint64_t)blockIdx.x * RLE_PRIV_CAP;
+  for (uint32_t i = threadIdx.x; i < n; i += 256) {
+    const unsigned long long w = base + i;
+    if (w < capacity) {
+      rk[w] = pk[pbase + i];
+      rca[w] = pca[pbase + i];
+      rcb[w] = pcb[pbase + i];
+    }
+  }
+}
+
+__global__ __launch_bounds__(1024) void k_rle_finish(const uint32_t *__restrict__ priv_n, uint32_t nblk,
+                                                     unsigned long long *__restrict__ cursor) {
+  __shared__ unsigned long long s_t64[1024 / 64 + 1];
+  const unsigned long long tot =
+      block_sum<1024, unsigned long long>(threadIdx.x < nblk ? (unsigned long long)priv_n[threadIdx.x] : 0ull, s_t64);
+  if (threadIdx.x == 0) *cursor += tot;
+}
+
+// acc += sum of in[0..n) (one workgroup)
+__global__ __launch_bounds__(1024) void k_sum_add(const unsigned long long *__restrict__ in, uint64_t n,
+                                                  unsigned long long *__restrict__ acc) {
+  __shared__ unsigned long long s_t64[1024 / 64 + 1];
+  unsigned long long s = 0;
+  for (uint64_t i = threadIdx.x; i < n; i += 1024) s += in[i];
+  s = block_sum<1024, unsigned long long>(s, s_t64);
+  if (threadIdx.x == 0 && s) atomicAdd(acc, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -2133,10 +2296,15 @@ uint64_t scan_tiles(uint64_t n) { return (n + 256 * 16 - 1) / (256 * 16); }
 template <typename K, typename OFF>
 static hipError_t launch_rle_t(const LaunchRle &r, hipStream_t st) {
   const uint32_t nchunks = (uint32_t)((r.M + RLE_CHUNK - 1) / RLE_CHUNK);
-  if (r.pass <= 1 && nchunks)
+  if (r.pass <= 1 && nchunks) {
     hipLaunchKernelGGL((k_rle_small<K, OFF, 256>), dim3(nchunks), dim3(256), 0, st, (const OFF *)r.offA,
                        (const K *)r.keysA, (const OFF *)r.offB, (const K *)r.keysB, r.M, r.pass, r.chunk_cnt,
-                       r.chunk_base, (K *)r.rk, r.rca, r.rcb, r.capacity, r.heavy_n, r.heavy, r.heavy_cap, r.pairs);
+                       r.chunk_base, r.chunk_heavy, r.heavy_base, r.chunk_pairs, (K *)r.rk, r.rca, r.rcb, r.capacity,
+                       r.heavy, r.heavy_cap);
+    if (r.pass == 0)
+      hipLaunchKernelGGL(k_sum_add, dim3(1), dim3(1024), 0, st, (const unsigned long long *)r.chunk_pairs,
+                         (uint64_t)nchunks, r.pairs);
+  }
   if (r.pass == 2 && r.nheavy) {
     GiantCtl *ctl = (GiantCtl *)r.gctl;
     hipLaunchKernelGGL(k_fill<uint32_t>, dim3(1), dim3(64), 0, st, (uint32_t *)&ctl->n, 0u, (uint64_t)1);
@@ -2152,10 +2320,14 @@ static hipError_t launch_rle_t(const LaunchRle &r, hipStream_t st) {
     hipLaunchKernelGGL((k_giant_emit<K>), dim3(GIANT_CAP), dim3(1024), 0, st, (const GiantCtl *)ctl, (const K *)r.gkey,
                        (const unsigned long long *)r.gca, (const unsigned long long *)r.gcb, (K *)r.rk, r.rca, r.rcb,
                        r.capacity, r.cursor, r.pairs);
-    hipLaunchKernelGGL((k_rle_heavy<K, OFF, 1024>), dim3((uint32_t)std::min<uint64_t>(r.nheavy, 1024)), dim3(1024), 0,
-                       st, (const OFF *)r.offA, (const K *)r.keysA, (const OFF *)r.offB, (const K *)r.keysB, r.heavy,
-                       r.nheavy, (K *)r.rk, r.rca, r.rcb, r.capacity, r.cursor, r.pairs, r.overflow,
-                       (const uint32_t *)r.gmark);
+    const uint32_t g = (uint32_t)std::min<uint64_t>(r.nheavy, RLE_HEAVY_GRID);
+    hipLaunchKernelGGL((k_rle_heavy<K, OFF, 1024>), dim3(g), dim3(1024), 0, st, (const OFF *)r.offA, (const K *)r.keysA,
+                       (const OFF *)r.offB, (const K *)r.keysB, r.heavy, r.nheavy, (K *)r.rk, r.rca, r.rcb, r.capacity,
+                       r.cursor, r.pairs, r.overflow, (const uint32_t *)r.gmark, (K *)r.pkey, r.pca, r.pcb, r.priv_n);
+    hipLaunchKernelGGL((k_rle_compact<K>), dim3(g), dim3(256), 0, st, (const uint32_t *)r.priv_n, (const K *)r.pkey,
+                       (const unsigned long long *)r.pca, (const unsigned long long *)r.pcb, (K *)r.rk, r.rca, r.rcb,
+                       r.capacity, (const unsigned long long *)r.cursor);
+    hipLaunchKernelGGL(k_rle_finish, dim3(1), dim3(1024), 0, st, (const uint32_t *)r.priv_n, g, r.cursor);
   }
   return hipGetLastError();
 }
@@ -2166,6 +2338,8 @@ hipError_t launch_rle(const LaunchRle &r, hipStream_t st) {
   return r.off_bytes == 4 ? launch_rle_t<uint64_t, uint32_t>(r, st) : launch_rle_t<uint64_t, uint64_t>(r, st);
 }
 size_t rle_giant_slots() { return (size_t)GIANT_CAP * GT_SLOTS; }
+size_t rle_priv_records() { return (size_t)RLE_HEAVY_GRID * RLE_PRIV_CAP; }
+uint32_t rle_heavy_grid() { return RLE_HEAVY_GRID; }
 size_t rle_giant_ctl_bytes() { return sizeof(GiantCtl); }
 size_t rle_giant_bytes(int key_bytes) { return sizeof(GiantCtl) + rle_giant_slots() * (key_bytes + 16) + 1024; }
 

@@ -91,12 +91,14 @@ struct LaunchRle {
   int pass;  // 0 count small bins, 1 emit small bins, 2 heavy bins
   uint64_t M;
   const void *offA, *keysA, *offB, *keysB;
-  unsigned long long *chunk_cnt;
-  const unsigned long long *chunk_base;
+  unsigned long long *chunk_cnt;         // per chunk: records (pass 0)
+  const unsigned long long *chunk_base;  // its exclusive scan
+  uint32_t *chunk_heavy;                 // per chunk: heavy bins (pass 0)
+  const uint32_t *heavy_base;            // its exclusive scan
+  unsigned long long *chunk_pairs;       // per chunk: pairs (pass 0)
   void *rk;
   unsigned long long *rca, *rcb;
   uint64_t capacity;
-  unsigned long long *heavy_n;
   uint32_t *heavy;
   uint64_t heavy_cap, nheavy;
   unsigned long long *cursor, *pairs;
@@ -106,6 +108,10 @@ struct LaunchRle {
   void *gctl;
   void *gkey;
   unsigned long long *gca, *gcb;
+  // k_rle_heavy's private record regions (rle_priv_records each) and counts (rle_heavy_grid)
+  void *pkey;
+  unsigned long long *pca, *pcb;
+  uint32_t *priv_n;
 };
 
 struct LaunchRmi {
@@ -138,6 +144,8 @@ uint64_t rle_chunks(uint64_t M);
 // scratch bytes of the giant-bin path of the run-length join
 size_t rle_giant_bytes(int key_bytes);
 size_t rle_giant_slots();
+size_t rle_priv_records();
+uint32_t rle_heavy_grid();
 size_t rle_giant_ctl_bytes();
 hipError_t launch_part_hist(const LaunchPart &lp, hipStream_t st);
 hipError_t launch_part_scatter(const LaunchPart &lp, hipStream_t st);

@@ -93,6 +93,22 @@ def test_rle_uniform(dev):
     _check_rle(A, B, 1 << 18, dev)
 
 
+def test_rle_dense_bins_in_sub_ranges_u64(dev):
+    # ~2.3 keys per bin: a chunk's keys exceed the staging buffer and are
+    # walked in staged sub-ranges of its bins
+    A = wl.uniform_keys(41, 300000, 8)
+    B = wl.probe_keys(A, 42, 250000, 0.5, 8)
+    _check_rle(A, B, 1 << 17, dev)
+
+
+def test_rle_very_dense_bins_from_global_u64(dev):
+    # ~6 keys per bin: no sub-range fits the staging buffer, keys are read
+    # from global memory
+    A = wl.uniform_keys(43, 300000, 8)
+    B = wl.probe_keys(A, 44, 300000, 0.7, 8)
+    _check_rle(A, B, 50000, dev)
+
+
 def test_rle_zipf_heavy(dev):
     # C3's shape at 2^20: the top key holds ~10% of the keys (heavy bins)
     A = wl.zipf_keys(3, 1 << 20, 1.1)
