@@ -1035,7 +1035,7 @@ hg_status hg_intersect_rle(hg_ctx c, const hg_table_desc *descA, const void *d_o
   const size_t i_cnt = P.add((nch + 1) * 8), i_base = P.add((nch + 1) * 8), i_hcnt = P.add((nch + 1) * 4),
                i_hbase = P.add((nch + 1) * 4), i_pairs = P.add((nch + 1) * 8), i_small = P.add(256),
                i_heavy = P.add(heavy_cap * 4), i_status = P.add(scan_tiles(nch) * 8 + 64),
-               i_gmark = P.add(heavy_cap * 4), i_giant = P.add(rle_giant_bytes((int)descA->key_bytes)),
+               i_gmark = P.add(heavy_cap * 8), i_giant = P.add(rle_giant_bytes((int)descA->key_bytes)),
                i_pkey = P.add(rle_priv_records() * descA->key_bytes), i_pca = P.add(rle_priv_records() * 8),
                i_pcb = P.add(rle_priv_records() * 8), i_pn = P.add((size_t)rle_heavy_grid() * 4);
   HG_TRY(arena_ensure(c, 1, P.total()));

@@ -1200,6 +1200,13 @@ __global__ __launch_bounds__(BLOCK, 5) void k_rle_small(const OFF *__restrict__ 
 constexpr uint32_t HT_SLOTS = 4096;
 // records a k_rle_heavy workgroup keeps in its private region
 constexpr uint32_t RLE_PRIV_CAP = 1024;
+// heavy bins of at most RLE_HEAVY_SPLIT keys of A: 256-thread workgroups with
+// RLE_HEAVY_SMALL_TS-slot tables (several per CU); larger ones: 1024 threads
+// and HT_SLOTS slots.  Grids of each form; their private regions follow each
+// other.
+constexpr uint32_t RLE_HEAVY_SPLIT = 512;
+constexpr uint32_t RLE_HEAVY_SMALL_TS = 1024;
+constexpr uint32_t RLE_HEAVY_GRID_S = 2048;
 constexpr uint32_t RLE_HEAVY_GRID = 1024;
 
 __device__ __forceinline__ uint32_t lds_cas(uint32_t *p, uint32_t cmp, uint32_t v) { return atomicCAS(p, cmp, v); }
@@ -1269,27 +1276,45 @@ constexpr uint32_t GT_SLOTS = 4096;
 
 struct GiantCtl {
   uint32_t n, totalA, totalB, pad;
+  uint32_t nS, nL, pad2[2];  // k_rle_heavy's lists: bins of the small-table and HT_SLOTS forms
   uint32_t hidx[GIANT_CAP];
   uint32_t chA[GIANT_CAP + 1], chB[GIANT_CAP + 1];
   unsigned long long eA[GIANT_CAP], eB[GIANT_CAP];
 };
 
+// Appends v to list at a wave-aggregated position (one atomic per wave).
+__device__ __forceinline__ void wave_append(bool p, uint32_t *ctr, uint32_t *list, uint32_t v) {
+  const uint64_t m = __ballot(p);
+  if (m == 0) return;  // uniform over the wave
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t leader = (uint32_t)__ffsll((unsigned long long)m) - 1;
+  uint32_t base = 0;
+  if (lane == leader) base = atomicAdd(ctr, (uint32_t)__popcll(m));
+  base = __shfl(base, leader, 64);
+  if (p) list[base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = v;
+}
+
+// Sorts the heavy bins: giant ones (up to GIANT_CAP) to ctl->hidx, the rest
+// to the lists of k_rle_heavy's two forms (lists[0..), lists[heavy_cap..)).
 template <typename OFF>
 __global__ __launch_bounds__(256) void k_giant_mark(const OFF *__restrict__ offA, const OFF *__restrict__ offB,
                                                     const uint32_t *__restrict__ heavy, uint64_t nheavy,
-                                                    uint32_t *__restrict__ gmark, GiantCtl *__restrict__ ctl) {
+                                                    uint32_t *__restrict__ lists, uint64_t heavy_cap,
+                                                    GiantCtl *__restrict__ ctl) {
   for (uint64_t h = (uint64_t)blockIdx.x * 256 + threadIdx.x; h < nheavy; h += (uint64_t)gridDim.x * 256) {
-    const uint64_t b = heavy[h];
-    const uint64_t sz = (uint64_t)(offA[b + 1] - offA[b]) + (uint64_t)(offB[b + 1] - offB[b]);
-    uint32_t slot = ~0u;
+    const uint32_t b = heavy[h];
+    const uint64_t na = (uint64_t)(offA[b + 1] - offA[b]);
+    const uint64_t sz = na + (uint64_t)(offB[b + 1] - offB[b]);
+    bool giant = false;
     if (sz > GIANT_MIN) {
       const uint32_t s = atomicAdd(&ctl->n, 1u);
       if (s < GIANT_CAP) {
-        slot = s;
+        giant = true;
         ctl->hidx[s] = (uint32_t)h;
       }
     }
-    gmark[h] = slot;
+    wave_append(!giant && na <= RLE_HEAVY_SPLIT, &ctl->nS, lists, b);
+    wave_append(!giant && na > RLE_HEAVY_SPLIT, &ctl->nL, lists + heavy_cap, b);
   }
 }
 
@@ -1459,7 +1484,7 @@ __global__ __launch_bounds__(1024) void k_giant_emit(const GiantCtl *__restrict_
   }
 }
 
-template <typename K, typename OFF, int BLOCK>
+template <typename K, typename OFF, int BLOCK, uint32_t TS>
 __global__ __launch_bounds__(BLOCK) void k_rle_heavy(const OFF *__restrict__ offA, const K *__restrict__ keysA,
                                                      const OFF *__restrict__ offB, const K *__restrict__ keysB,
                                                      const uint32_t *__restrict__ heavy, uint64_t nheavy,
@@ -1468,26 +1493,33 @@ __global__ __launch_bounds__(BLOCK) void k_rle_heavy(const OFF *__restrict__ off
                                                      unsigned long long *__restrict__ cursor,
                                                      unsigned long long *__restrict__ pairs,
                                                      unsigned int *__restrict__ overflow,
-                                                     const uint32_t *__restrict__ gmark, K *__restrict__ pk,
+                                                     const uint32_t *__restrict__ list,
+                                                     const uint32_t *__restrict__ list_n, K *__restrict__ pk,
                                                      unsigned long long *__restrict__ pca,
                                                      unsigned long long *__restrict__ pcb,
-                                                     uint32_t *__restrict__ priv_n) {
+                                                     uint32_t *__restrict__ priv_n, uint32_t region0) {
   constexpr K EMPTY = (K)~(K)0;
-  __shared__ K s_key[HT_SLOTS];
-  __shared__ unsigned long long s_ca[HT_SLOTS], s_cb[HT_SLOTS];
+  __shared__ K s_key[TS];
+  // 32-bit counts: a bin with 2^32 or more keys on a side is refused (overflow flag)
+  __shared__ uint32_t s_ca[TS], s_cb[TS];
   __shared__ unsigned int s_fill, s_pn, s_priv;
   __shared__ unsigned long long s_eca, s_ecb, s_base, s_t64[BLOCK / 64 + 1];
   unsigned long long wpairs = 0;  // thread 0: the workgroup's pairs
   if (threadIdx.x == 0) s_pn = 0;
-  const uint64_t pbase = (uint64_t)blockIdx.x * RLE_PRIV_CAP;
-  for (uint64_t h = blockIdx.x; h < nheavy; h += gridDim.x) {
-    if (gmark[h] != ~0u) continue;  // a giant bin: k_giant_fold's
-    const uint64_t b = heavy[h];
+  const uint64_t pbase = (uint64_t)(region0 + blockIdx.x) * RLE_PRIV_CAP;
+  // this form's bins (k_giant_mark's list: giant bins are k_giant_fold's)
+  const uint32_t nlist = *list_n;
+  for (uint32_t h = blockIdx.x; h < nlist; h += gridDim.x) {
+    const uint64_t b = list[h];
     const uint64_t a0 = offA[b], a1 = offA[b + 1], x0 = offB[b], x1 = offB[b + 1];
+    if (a1 - a0 > 0xFFFFFFFFull || x1 - x0 > 0xFFFFFFFFull) {
+      if (threadIdx.x == 0) atomicOr(overflow, 1u);
+      continue;
+    }
     // a table of at least twice A's keys in the bin (its distinct values),
-    // 64 to HT_SLOTS slots: small bins clear and scan a small table
+    // 64 to TS slots: small bins clear and scan a small table
     uint32_t slots = 64;
-    while (slots < HT_SLOTS && (uint64_t)slots < 2 * (a1 - a0)) slots <<= 1;
+    while (slots < TS && (uint64_t)slots < 2 * (a1 - a0)) slots <<= 1;
     const uint32_t mask = slots - 1;
     for (uint32_t i = threadIdx.x; i < slots; i += BLOCK) {
       s_key[i] = EMPTY;
@@ -1525,7 +1557,7 @@ __global__ __launch_bounds__(BLOCK) void k_rle_heavy(const OFF *__restrict__ off
     for (uint32_t i = threadIdx.x; i < slots; i += BLOCK)
       if (s_cb[i]) {
         mine++;
-        mp += s_ca[i] * s_cb[i];
+        mp += (unsigned long long)s_ca[i] * s_cb[i];
       }
     const bool ehit = threadIdx.x == 0 && s_eca && s_ecb;
     if (ehit) {
@@ -1574,7 +1606,7 @@ __global__ __launch_bounds__(BLOCK) void k_rle_heavy(const OFF *__restrict__ off
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    priv_n[blockIdx.x] = s_pn;
+    priv_n[region0 + blockIdx.x] = s_pn;
     if (wpairs) atomicAdd(pairs, wpairs);
   }
 }
@@ -1609,8 +1641,9 @@ __global__ __launch_bounds__(256) void k_rle_compact(const uint32_t *__restrict_
 __global__ __launch_bounds__(1024) void k_rle_finish(const uint32_t *__restrict__ priv_n, uint32_t nblk,
                                                      unsigned long long *__restrict__ cursor) {
   __shared__ unsigned long long s_t64[1024 / 64 + 1];
-  const unsigned long long tot =
-      block_sum<1024, unsigned long long>(threadIdx.x < nblk ? (unsigned long long)priv_n[threadIdx.x] : 0ull, s_t64);
+  unsigned long long v = 0;
+  for (uint32_t i = threadIdx.x; i < nblk; i += 1024) v += priv_n[i];
+  const unsigned long long tot = block_sum<1024, unsigned long long>(v, s_t64);
   if (threadIdx.x == 0) *cursor += tot;
 }
 
@@ -2307,9 +2340,10 @@ static hipError_t launch_rle_t(const LaunchRle &r, hipStream_t st) {
   }
   if (r.pass == 2 && r.nheavy) {
     GiantCtl *ctl = (GiantCtl *)r.gctl;
-    hipLaunchKernelGGL(k_fill<uint32_t>, dim3(1), dim3(64), 0, st, (uint32_t *)&ctl->n, 0u, (uint64_t)1);
+    hipLaunchKernelGGL(k_fill<uint32_t>, dim3(1), dim3(64), 0, st, (uint32_t *)&ctl->n, 0u, (uint64_t)8);
     hipLaunchKernelGGL((k_giant_mark<OFF>), dim3((uint32_t)std::min<uint64_t>((r.nheavy + 255) / 256, 4096)), dim3(256), 0,
-                       st, (const OFF *)r.offA, (const OFF *)r.offB, (const uint32_t *)r.heavy, r.nheavy, r.gmark, ctl);
+                       st, (const OFF *)r.offA, (const OFF *)r.offB, (const uint32_t *)r.heavy, r.nheavy, r.gmark,
+                       r.heavy_cap, ctl);
     hipLaunchKernelGGL((k_giant_prefix<OFF>), dim3(1), dim3(64), 0, st, (const OFF *)r.offA, (const OFF *)r.offB,
                        (const uint32_t *)r.heavy, ctl);
     hipLaunchKernelGGL((k_giant_clear<K>), dim3(1024), dim3(256), 0, st, (const GiantCtl *)ctl, (K *)r.gkey, r.gca, r.gcb);
@@ -2320,10 +2354,17 @@ static hipError_t launch_rle_t(const LaunchRle &r, hipStream_t st) {
     hipLaunchKernelGGL((k_giant_emit<K>), dim3(GIANT_CAP), dim3(1024), 0, st, (const GiantCtl *)ctl, (const K *)r.gkey,
                        (const unsigned long long *)r.gca, (const unsigned long long *)r.gcb, (K *)r.rk, r.rca, r.rcb,
                        r.capacity, r.cursor, r.pairs);
-    const uint32_t g = (uint32_t)std::min<uint64_t>(r.nheavy, RLE_HEAVY_GRID);
-    hipLaunchKernelGGL((k_rle_heavy<K, OFF, 1024>), dim3(g), dim3(1024), 0, st, (const OFF *)r.offA, (const K *)r.keysA,
-                       (const OFF *)r.offB, (const K *)r.keysB, r.heavy, r.nheavy, (K *)r.rk, r.rca, r.rcb, r.capacity,
-                       r.cursor, r.pairs, r.overflow, (const uint32_t *)r.gmark, (K *)r.pkey, r.pca, r.pcb, r.priv_n);
+    const uint32_t gs = (uint32_t)std::min<uint64_t>(r.nheavy, RLE_HEAVY_GRID_S);
+    const uint32_t gb = (uint32_t)std::min<uint64_t>(r.nheavy, RLE_HEAVY_GRID);
+    hipLaunchKernelGGL((k_rle_heavy<K, OFF, 256, RLE_HEAVY_SMALL_TS>), dim3(gs), dim3(256), 0, st, (const OFF *)r.offA,
+                       (const K *)r.keysA, (const OFF *)r.offB, (const K *)r.keysB, r.heavy, r.nheavy, (K *)r.rk, r.rca,
+                       r.rcb, r.capacity, r.cursor, r.pairs, r.overflow, (const uint32_t *)r.gmark,
+                       (const uint32_t *)&ctl->nS, (K *)r.pkey, r.pca, r.pcb, r.priv_n, 0u);
+    hipLaunchKernelGGL((k_rle_heavy<K, OFF, 1024, HT_SLOTS>), dim3(gb), dim3(1024), 0, st, (const OFF *)r.offA,
+                       (const K *)r.keysA, (const OFF *)r.offB, (const K *)r.keysB, r.heavy, r.nheavy, (K *)r.rk, r.rca,
+                       r.rcb, r.capacity, r.cursor, r.pairs, r.overflow, (const uint32_t *)(r.gmark + r.heavy_cap),
+                       (const uint32_t *)&ctl->nL, (K *)r.pkey, r.pca, r.pcb, r.priv_n, gs);
+    const uint32_t g = gs + gb;
     hipLaunchKernelGGL((k_rle_compact<K>), dim3(g), dim3(256), 0, st, (const uint32_t *)r.priv_n, (const K *)r.pkey,
                        (const unsigned long long *)r.pca, (const unsigned long long *)r.pcb, (K *)r.rk, r.rca, r.rcb,
                        r.capacity, (const unsigned long long *)r.cursor);
@@ -2338,8 +2379,8 @@ hipError_t launch_rle(const LaunchRle &r, hipStream_t st) {
   return r.off_bytes == 4 ? launch_rle_t<uint64_t, uint32_t>(r, st) : launch_rle_t<uint64_t, uint64_t>(r, st);
 }
 size_t rle_giant_slots() { return (size_t)GIANT_CAP * GT_SLOTS; }
-size_t rle_priv_records() { return (size_t)RLE_HEAVY_GRID * RLE_PRIV_CAP; }
-uint32_t rle_heavy_grid() { return RLE_HEAVY_GRID; }
+size_t rle_priv_records() { return (size_t)(RLE_HEAVY_GRID_S + RLE_HEAVY_GRID) * RLE_PRIV_CAP; }
+uint32_t rle_heavy_grid() { return RLE_HEAVY_GRID_S + RLE_HEAVY_GRID; }
 size_t rle_giant_ctl_bytes() { return sizeof(GiantCtl); }
 size_t rle_giant_bytes(int key_bytes) { return sizeof(GiantCtl) + rle_giant_slots() * (key_bytes + 16) + 1024; }
 

@@ -104,7 +104,7 @@ struct LaunchRle {
   unsigned long long *cursor, *pairs;
   unsigned int *overflow;
   // giant heavy bins: per-heavy-bin slot (heavy_cap), control block, tables
-  uint32_t *gmark;
+  uint32_t *gmark;  // 2 * heavy_cap: the heavy bins of k_rle_heavy's two forms
   void *gctl;
   void *gkey;
   unsigned long long *gca, *gcb;
