@@ -1487,7 +1487,6 @@ __global__ __launch_bounds__(1024) void k_giant_emit(const GiantCtl *__restrict_
 template <typename K, typename OFF, int BLOCK, uint32_t TS>
 __global__ __launch_bounds__(BLOCK) void k_rle_heavy(const OFF *__restrict__ offA, const K *__restrict__ keysA,
                                                      const OFF *__restrict__ offB, const K *__restrict__ keysB,
-                                                     const uint32_t *__restrict__ heavy, uint64_t nheavy,
                                                      K *__restrict__ rk, unsigned long long *__restrict__ rca,
                                                      unsigned long long *__restrict__ rcb, uint64_t capacity,
                                                      unsigned long long *__restrict__ cursor,
@@ -2357,11 +2356,11 @@ static hipError_t launch_rle_t(const LaunchRle &r, hipStream_t st) {
     const uint32_t gs = (uint32_t)std::min<uint64_t>(r.nheavy, RLE_HEAVY_GRID_S);
     const uint32_t gb = (uint32_t)std::min<uint64_t>(r.nheavy, RLE_HEAVY_GRID);
     hipLaunchKernelGGL((k_rle_heavy<K, OFF, 256, RLE_HEAVY_SMALL_TS>), dim3(gs), dim3(256), 0, st, (const OFF *)r.offA,
-                       (const K *)r.keysA, (const OFF *)r.offB, (const K *)r.keysB, r.heavy, r.nheavy, (K *)r.rk, r.rca,
+                       (const K *)r.keysA, (const OFF *)r.offB, (const K *)r.keysB, (K *)r.rk, r.rca,
                        r.rcb, r.capacity, r.cursor, r.pairs, r.overflow, (const uint32_t *)r.gmark,
                        (const uint32_t *)&ctl->nS, (K *)r.pkey, r.pca, r.pcb, r.priv_n, 0u);
     hipLaunchKernelGGL((k_rle_heavy<K, OFF, 1024, HT_SLOTS>), dim3(gb), dim3(1024), 0, st, (const OFF *)r.offA,
-                       (const K *)r.keysA, (const OFF *)r.offB, (const K *)r.keysB, r.heavy, r.nheavy, (K *)r.rk, r.rca,
+                       (const K *)r.keysA, (const OFF *)r.offB, (const K *)r.keysB, (K *)r.rk, r.rca,
                        r.rcb, r.capacity, r.cursor, r.pairs, r.overflow, (const uint32_t *)(r.gmark + r.heavy_cap),
                        (const uint32_t *)&ctl->nL, (K *)r.pkey, r.pca, r.pcb, r.priv_n, gs);
     const uint32_t g = gs + gb;
