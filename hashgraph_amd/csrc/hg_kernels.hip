@@ -1890,9 +1890,25 @@ __global__ __launch_bounds__(BLOCK) void k_big_place(const K *__restrict__ part,
         lds_add_agg<false>(s_cnt, l, i < n && l < nb);
       }
       __syncthreads();
-      // reserve this chunk's places, bin by bin
-      for (uint32_t i = threadIdx.x; i < nb; i += BLOCK)
-        if (s_cnt[i]) s_cnt[i] = atomicAdd(&gcur[ctl->bin_base[slot] + b0 + i], s_cnt[i]);
+      // reserve this chunk's places, bin by bin (RU returning atomics of a
+      // thread in flight together)
+      {
+        constexpr int RU = 8;
+        uint32_t *gb = gcur + ctl->bin_base[slot] + b0;
+        for (uint32_t i0 = threadIdx.x; i0 < nb; i0 += BLOCK * RU) {
+          uint32_t c[RU], r[RU];
+#pragma unroll
+          for (int j = 0; j < RU; j++) {
+            const uint32_t i = i0 + j * BLOCK;
+            c[j] = i < nb ? s_cnt[i] : 0u;
+          }
+#pragma unroll
+          for (int j = 0; j < RU; j++) r[j] = c[j] ? atomicAdd(&gb[i0 + j * BLOCK], c[j]) : 0u;
+#pragma unroll
+          for (int j = 0; j < RU; j++)
+            if (c[j]) s_cnt[i0 + j * BLOCK] = r[j];
+        }
+      }
       __syncthreads();
       for (uint32_t i0 = 0; i0 < n; i0 += BLOCK) {
         const uint32_t i = i0 + threadIdx.x;
