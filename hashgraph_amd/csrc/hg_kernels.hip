@@ -902,7 +902,7 @@ __global__ __launch_bounds__(1024) void k_range_hist(const K *__restrict__ keys,
 // ---------------------------------------------------------------------------
 constexpr uint32_t RLE_SMALL = 64;
 constexpr uint32_t RLE_CHUNK = 1024;
-constexpr uint32_t RLE_KCAP = 1024;
+constexpr uint32_t RLE_KCAP = 768;
 constexpr uint32_t RLE_TINY = 8;  // bins above this on a side (up to RLE_SMALL) are walked by a wave
 
 // A medium bin (A's keys a0.., a of them; B's x0.., b of them; a, b <= 64)
@@ -987,7 +987,7 @@ __device__ __forceinline__ void rle_walk(const OFF *s_oa, const OFF *s_ob, uint3
 // bins at the chunk's heavy prefix and, when emit, writes the records at the
 // chunk's record prefix.  Pass 1 without emit only lists the heavy bins.
 template <typename K, typename OFF, int BLOCK>
-__global__ __launch_bounds__(BLOCK, 5) void k_rle_small(const OFF *__restrict__ offA, const K *__restrict__ keysA,
+__global__ __launch_bounds__(BLOCK, 6) void k_rle_small(const OFF *__restrict__ offA, const K *__restrict__ keysA,
                                                      const OFF *__restrict__ offB, const K *__restrict__ keysB, uint64_t M,
                                                      int pass, unsigned long long *__restrict__ chunk_cnt,
                                                      const unsigned long long *__restrict__ chunk_base,
