@@ -1204,8 +1204,8 @@ constexpr uint32_t RLE_PRIV_CAP = 1024;
 // RLE_HEAVY_SMALL_TS-slot tables (several per CU); larger ones: 1024 threads
 // and HT_SLOTS slots.  Grids of each form; their private regions follow each
 // other.
-constexpr uint32_t RLE_HEAVY_SPLIT = 512;
-constexpr uint32_t RLE_HEAVY_SMALL_TS = 1024;
+constexpr uint32_t RLE_HEAVY_SPLIT = 1024;
+constexpr uint32_t RLE_HEAVY_SMALL_TS = 2048;
 constexpr uint32_t RLE_HEAVY_GRID_S = 2048;
 constexpr uint32_t RLE_HEAVY_GRID = 1024;
 
