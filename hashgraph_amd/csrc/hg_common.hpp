@@ -91,6 +91,19 @@ struct FinItem {
   uint32_t table;   // index into the BinMap array
 };
 
+// Global-memory value table of one value-rich heavy bin of the run-length
+// join: slots [base, base + mask + 1) of the batch's table arrays.
+struct OvfTable {
+  uint64_t base;
+  uint64_t mask;
+};
+// A chunk of one side's keys of such a bin: keys[begin .. end), table tab.
+struct OvfChunk {
+  uint64_t begin, end;
+  uint32_t tab;
+  uint32_t pad;
+};
+
 // Learned CDF model (no counterpart in the reference).  Two FP32 linear
 // stages over integer anchors:
 //   feature = f(key - key_min)                     (root_kind 0), or

@@ -639,6 +639,99 @@ hg_status build_table(hg_ctx c, const hg_table_desc *desc, const void *d_keys, v
   return finish_two_level(c, cs, map, kb, ob, cv0.get<void>(i_part1), cv0.get<void>(i_part2), d_csr, d_off);
 }
 
+// Heavy bins of the run-length join with more distinct values than the LDS
+// and giant-bin tables hold: one global-memory table per bin of at least
+// twice its keys of A (4096 slots at least), laid out in batches of at most
+// kOvfBatchSlots slots (a larger bin takes a batch of its own).  Runs after
+// the rest of the join; its records follow the others.
+constexpr uint64_t kOvfBatchSlots = 1ull << 26;
+constexpr uint64_t kOvfChunk = 1ull << 16;
+
+hg_status rle_value_rich(hg_ctx c, const LaunchRle &r, const uint32_t *d_list, uint32_t n) {
+  const int kb = r.key_bytes;
+  // (a0, a1, x0, x1) of each listed bin, read back in pieces
+  std::vector<unsigned long long> sz((size_t)n * 4);
+  {
+    Plan P;
+    const size_t i_sz = P.add((size_t)n * 32);
+    HG_TRY(arena_ensure(c, 0, P.total()));
+    Carve cv;
+    cv.init(c, 0, P);
+    unsigned long long *d_sz = cv.get<unsigned long long>(i_sz);
+    HG_HIP(launch_ovf_sizes(d_list, n, r.offA, r.offB, r.off_bytes, d_sz, c->stream));
+    for (size_t i = 0; i < sz.size(); i += 1u << 17) {
+      const size_t m = std::min<size_t>(1u << 17, sz.size() - i);
+      HG_TRY(sync_read(c, sz.data() + i, d_sz + i, m * 8));
+    }
+  }
+  uint32_t i0 = 0;
+  while (i0 < n) {
+    // one batch: bins i0 .. i1
+    std::vector<OvfTable> tabs;
+    std::vector<OvfChunk> ca, cb;
+    uint64_t slots = 0;
+    uint32_t i1 = i0;
+    while (i1 < n) {
+      const uint64_t a0 = sz[4 * i1], a1 = sz[4 * i1 + 1], x0 = sz[4 * i1 + 2], x1 = sz[4 * i1 + 3];
+      uint64_t t = 4096;
+      while (t < 2 * (a1 - a0)) t <<= 1;
+      if (i1 > i0 && slots + t > kOvfBatchSlots) break;
+      OvfTable tb;
+      tb.base = slots;
+      tb.mask = t - 1;
+      const uint32_t ti = (uint32_t)tabs.size();
+      tabs.push_back(tb);
+      slots += t;
+      for (uint64_t k = a0; k < a1; k += kOvfChunk) ca.push_back({k, std::min(a1, k + kOvfChunk), ti, 0});
+      for (uint64_t k = x0; k < x1; k += kOvfChunk) cb.push_back({k, std::min(x1, k + kOvfChunk), ti, 0});
+      i1++;
+    }
+    Blob bl;
+    const size_t o_tabs = bl.put(tabs), o_ca = bl.put(ca), o_cb = bl.put(cb);
+    Plan P;
+    const size_t i_key = P.add(slots * kb), i_ca = P.add(slots * 4), i_cb = P.add(slots * 4),
+                 i_e = P.add(tabs.size() * 16), i_blob = P.add(bl.data.size());
+    HG_TRY(arena_ensure(c, 0, P.total()));
+    Carve cv;
+    cv.init(c, 0, P);
+    char *blob = cv.get<char>(i_blob);
+    HG_TRY(upload(c, blob, bl.data.data(), bl.data.size()));
+    LaunchRleOvf o{};
+    o.key_bytes = kb;
+    o.keysA = r.keysA;
+    o.keysB = r.keysB;
+    o.tabs = (const OvfTable *)(blob + o_tabs);
+    o.ntabs = (uint32_t)tabs.size();
+    o.chunksA = (const OvfChunk *)(blob + o_ca);
+    o.chunksB = (const OvfChunk *)(blob + o_cb);
+    o.nchunksA = (uint32_t)ca.size();
+    o.nchunksB = (uint32_t)cb.size();
+    o.nslots = slots;
+    o.tkey = cv.get<void>(i_key);
+    o.tca = cv.get<uint32_t>(i_ca);
+    o.tcb = cv.get<uint32_t>(i_cb);
+    o.eA = cv.get<unsigned long long>(i_e);
+    o.eB = o.eA + tabs.size();
+    o.rk = r.rk;
+    o.rca = r.rca;
+    o.rcb = r.rcb;
+    o.capacity = r.capacity;
+    o.cursor = r.cursor;
+    o.pairs = r.pairs;
+    if (kb == 4) HG_HIP(launch_fill_u32((uint32_t *)o.tkey, 0xFFFFFFFFu, slots, c->stream));
+    else HG_HIP(launch_fill_u64((unsigned long long *)o.tkey, ~0ull, slots, c->stream));
+    HG_HIP(launch_fill_u32(o.tca, 0u, slots, c->stream));
+    HG_HIP(launch_fill_u32(o.tcb, 0u, slots, c->stream));
+    HG_HIP(launch_fill_u64(o.eA, 0ull, 2 * tabs.size(), c->stream));
+    {
+      Prof pf(c, "rle_value_rich");
+      HG_HIP(launch_rle_ovf(o, c->stream));
+    }
+    i0 = i1;
+  }
+  return HG_OK;
+}
+
 uint64_t ref_binary_search(const std::vector<int64_t> &bins, uint64_t l, uint64_t r, uint64_t x) {
   // include/MultiHashGraphHostOperators.cuh:8-27 (recursive, size_t indices)
   for (;;) {
@@ -819,7 +912,7 @@ hg_status hg_rmi_train(hg_ctx c, const void *d_keys, uint64_t n_keys, uint32_t k
   Plan P;
   const size_t i_b0 = P.add(n_sample * key_bytes), i_b1 = P.add(n_sample * key_bytes);
   const size_t i_cnt = P.add(((size_t)256 * ntiles + 1) * 4);
-  const size_t i_st = P.add(scan_tiles((uint64_t)256 * ntiles) * 8), i_tk = P.add(64);
+  const size_t i_st = P.add(scan_status_bytes((uint64_t)256 * ntiles));
   const size_t i_part = P.add((size_t)nblk * 8 * sizeof(double));
   const size_t i_cand = P.add(64), i_lcnt = P.add((size_t)4 * n_leaves * 4);
   const size_t i_start = P.add(((size_t)n_leaves + 1) * 4);
@@ -836,8 +929,7 @@ hg_status hg_rmi_train(hg_ctx c, const void *d_keys, uint64_t n_keys, uint32_t k
   r.buf0 = cv.get<void>(i_b0);
   r.buf1 = cv.get<void>(i_b1);
   r.counts = cv.get<uint32_t>(i_cnt);
-  r.scan_status = cv.get<unsigned long long>(i_st);
-  r.scan_ticket = cv.get<unsigned int>(i_tk);
+  r.scan_status = cv.get<void>(i_st);
   r.partials = cv.get<double>(i_part);
   r.cand = cv.get<void>(i_cand);
   r.leafcnt = cv.get<uint32_t>(i_lcnt);
@@ -884,13 +976,11 @@ hg_status hg_exclusive_scan(hg_ctx c, const void *d_in, void *d_out, uint64_t n,
   DeviceGuard g(c->device);
   if (n == 0) return fill_offsets(c, d_out, (int)elem_bytes, 1, 0);
   Plan P;
-  const size_t i_status = P.add(scan_tiles(n) * 8);
-  const size_t i_ticket = P.add(64);
+  const size_t i_status = P.add(scan_status_bytes(n));
   HG_TRY(arena_ensure(c, 1, P.total()));
   Carve cv;
   cv.init(c, 1, P);
-  HG_HIP(launch_scan(d_in, d_out, n, (int)elem_bytes, cv.get<unsigned long long>(i_status), cv.get<unsigned int>(i_ticket),
-                     c->stream));
+  HG_HIP(launch_scan(d_in, d_out, n, (int)elem_bytes, cv.get<void>(i_status), c->stream));
   return HG_OK;
 }
 
@@ -909,31 +999,67 @@ hg_status hg_probe(hg_ctx c, const hg_table_desc *descA, const void *d_offA, con
   const BinMap map = make_map(M, descA->hash_kind, descA->d_rmi_model);
   const uint32_t W = choose_width(std::max<uint64_t>(n_probe, descA->n_keys), M, kb, probe_fill(kb));
   const uint64_t F = (M + W - 1) / W;
-  if (F > kFMax) return fail(HG_ERR_UNSUPPORTED, "probe geometry needs more than one partition level");
-  Level L;
-  SegDesc s = range_seg(0, n_probe, 0, M, W, map);
-  s.fin_off_base = 0;
-  s.fin_key_rel = 0;
-  s.table = 0;
-  L.segs.push_back(s);
-  L.maps.push_back(map);
-  L.auto_tile(n_probe);
-  L.prepare();
+  // Beyond kFMax buckets, B is partitioned twice: into super-buckets of k
+  // buckets, whose sizes are read back, then each super-bucket into its
+  // buckets (as the build's second level).
+  const bool two = F > kFMax;
+  const uint64_t k = two ? (F + kFMax - 1) / kFMax : 1;
+  if ((uint64_t)W * k >= (1ull << 32)) return fail(HG_ERR_UNSUPPORTED, "probe geometry too large");
   Plan P0;
   const size_t i_part = P0.add(n_probe * kb);
+  const size_t i_part2 = P0.add(two ? n_probe * kb : 1);
   const size_t i_small = P0.add(256);
-  const size_t i_mark = P0.add((size_t)L.ncols_total * 4);
+  const size_t i_mark = P0.add((size_t)(F + kFMax + 1) * 4);  // >= the buckets of the last level
   HG_TRY(arena_ensure(c, 0, P0.total()));
   Carve cv0;
   cv0.init(c, 0, P0);
-  LevelDev d;
-  HG_TRY(setup_level(c, L, d));
   void *part = cv0.get<void>(i_part);
   unsigned long long *small = cv0.get<unsigned long long>(i_small);
   unsigned int *hmark = cv0.get<unsigned int>(i_mark);
+  Level L;
+  LevelDev d;
+  if (!two) {
+    SegDesc s = range_seg(0, n_probe, 0, M, W, map);
+    s.fin_off_base = 0;
+    s.fin_key_rel = 0;
+    s.table = 0;
+    L.segs.push_back(s);
+    L.maps.push_back(map);
+    L.auto_tile(n_probe);
+    L.prepare();
+    HG_TRY(setup_level(c, L, d));
+    HG_TRY(run_level(c, L, d, kb, d_probe, part));
+  } else {
+    const uint64_t W1 = (uint64_t)W * k;
+    Level L1;
+    L1.segs.push_back(range_seg(0, n_probe, 0, M, (uint32_t)W1, map));
+    L1.maps.push_back(map);
+    L1.auto_tile(n_probe);
+    L1.prepare();
+    LevelDev d1;
+    HG_TRY(setup_level(c, L1, d1));
+    HG_TRY(run_level(c, L1, d1, kb, d_probe, part));
+    std::vector<uint32_t> cnt(L1.ncols_total), st(L1.ncols_total);
+    HG_TRY(sync_read(c, cnt.data(), d1.coltot, cnt.size() * 4));
+    HG_TRY(sync_read(c, st.data(), d1.bstart, st.size() * 4));
+    for (uint32_t g = 0; g < L1.ncols_total; g++) {
+      const uint64_t lo = (uint64_t)g * W1, nb = std::min<uint64_t>(W1, M - lo);
+      SegDesc s = range_seg(st[g], (uint64_t)st[g] + cnt[g], lo, nb, W, map);
+      s.fin_off_base = lo;
+      s.fin_key_rel = 0;
+      s.table = 0;
+      L.segs.push_back(s);
+    }
+    L.maps.push_back(map);
+    L.auto_tile(n_probe);
+    L.prepare();
+    HG_TRY(setup_level(c, L, d));
+    void *part2 = cv0.get<void>(i_part2);
+    HG_TRY(run_level(c, L, d, kb, part, part2));
+    part = part2;
+  }
   HG_HIP(launch_fill_u64(small, 0ull, 4, c->stream));
   HG_HIP(launch_fill_u32(hmark, 0u, L.ncols_total, c->stream));
-  HG_TRY(run_level(c, L, d, kb, d_probe, part));
   HG_HIP(launch_make_items(d.segs, (uint32_t)L.segs.size(), d.bstart, d.coltot, d.fin, L.ncols_total, c->stream));
   LaunchProbe pr{};
   pr.key_bytes = kb;
@@ -949,19 +1075,16 @@ hg_status hg_probe(hg_ctx c, const hg_table_desc *descA, const void *d_offA, con
   pr.out = d_out;
   pr.capacity = d_out ? capacity : 0;
   pr.cursor = small;
-  pr.flag = (unsigned int *)(small + 1);
   pr.gate = (unsigned int *)(small + 2);
   pr.hmark = hmark;
   {
     Prof pf(c, "probe");
     HG_HIP(launch_probe(pr, c->stream));
   }
-  unsigned long long res[2];
-  HG_TRY(sync_read(c, res, small, 16));
-  *h_count = res[0];
-  if (res[1] & 0xFFFFFFFFull)
-    return fail(HG_ERR_UNSUPPORTED, "a heavy bin of A holds too many distinct values; use hg_intersect_rle");
-  if (d_out && res[0] > capacity) return fail(HG_ERR_CAPACITY, "output capacity smaller than the match count");
+  unsigned long long cnt = 0;
+  HG_TRY(sync_read(c, &cnt, small, 8));
+  *h_count = cnt;
+  if (d_out && cnt > capacity) return fail(HG_ERR_CAPACITY, "output capacity smaller than the match count");
   return HG_OK;
 }
 
@@ -979,8 +1102,12 @@ hg_status hg_intersect(hg_ctx c, const hg_table_desc *descA, const void *d_offA,
     return fail(HG_ERR_INVALID_ARG, "null buffer");
   DeviceGuard g(c->device);
   *h_count = 0;
+  // bins above the nested loop's pair limit hold more than 128 keys of the
+  // two tables together, so at most (N_A + N_B) / 128 of them exist
+  const uint64_t hcap = std::min<uint64_t>(descA->n_bins, (descA->n_keys + descB->n_keys) / 128 + 1);
   Plan P;
   const size_t i_small = P.add(256);
+  const size_t i_hlist = P.add(hcap * 4);
   HG_TRY(arena_ensure(c, 1, P.total()));
   Carve cv;
   cv.init(c, 1, P);
@@ -998,7 +1125,9 @@ hg_status hg_intersect(hg_ctx c, const hg_table_desc *descA, const void *d_offA,
   j.out = d_out;
   j.capacity = d_out ? capacity : 0;
   j.cursor = small;
-  j.flag = (unsigned int *)(small + 1);
+  j.hlist = cv.get<uint32_t>(i_hlist);
+  j.hlist_n = (unsigned int *)(small + 1);
+  j.hlist_cap = (uint32_t)hcap;
   {
     Prof pf(c, "join");
     HG_HIP(launch_join(j, c->stream));
@@ -1006,8 +1135,7 @@ hg_status hg_intersect(hg_ctx c, const hg_table_desc *descA, const void *d_offA,
   unsigned long long res[2];
   HG_TRY(sync_read(c, res, small, 16));
   *h_count = res[0];
-  if (res[1] & 0xFFFFFFFFull)
-    return fail(HG_ERR_UNSUPPORTED, "a bin pair is too large for the nested join; use hg_intersect_rle");
+  if ((res[1] & 0xFFFFFFFFull) > hcap) return fail(HG_ERR_DEVICE, "heavy join bins exceed their bound");
   if (d_out && res[0] > capacity) return fail(HG_ERR_CAPACITY, "output capacity smaller than the match count");
   return HG_OK;
 }
@@ -1030,18 +1158,20 @@ hg_status hg_intersect_rle(hg_ctx c, const hg_table_desc *descA, const void *d_o
   *h_records = 0;
   *h_count = 0;
   const uint64_t M = descA->n_bins, nch = rle_chunks(M);
-  const uint64_t heavy_cap = std::min<uint64_t>(M, 1u << 22);
+  // a heavy bin holds more than 64 keys on a side, so at most
+  // N_A / 65 + N_B / 65 of them exist
+  const uint64_t heavy_cap = std::min<uint64_t>(M, descA->n_keys / 65 + descB->n_keys / 65 + 1);
   Plan P;
   const size_t i_cnt = P.add((nch + 1) * 8), i_base = P.add((nch + 1) * 8), i_hcnt = P.add((nch + 1) * 4),
                i_hbase = P.add((nch + 1) * 4), i_pairs = P.add((nch + 1) * 8), i_small = P.add(256),
-               i_heavy = P.add(heavy_cap * 4), i_status = P.add(scan_tiles(nch) * 8 + 64),
+               i_heavy = P.add(heavy_cap * 4), i_ovf = P.add(heavy_cap * 4), i_status = P.add(scan_status_bytes(nch)),
                i_gmark = P.add(heavy_cap * 8), i_giant = P.add(rle_giant_bytes((int)descA->key_bytes)),
                i_pkey = P.add(rle_priv_records() * descA->key_bytes), i_pca = P.add(rle_priv_records() * 8),
                i_pcb = P.add(rle_priv_records() * 8), i_pn = P.add((size_t)rle_heavy_grid() * 4);
   HG_TRY(arena_ensure(c, 1, P.total()));
   Carve cv;
   cv.init(c, 1, P);
-  unsigned long long *small = cv.get<unsigned long long>(i_small);  // [-, pairs, cursor, overflow]
+  unsigned long long *small = cv.get<unsigned long long>(i_small);  // [-, pairs, cursor, overflow, ovf_n]
   HG_HIP(launch_fill_u64(small, 0ull, 8, c->stream));
   LaunchRle r{};
   r.key_bytes = (int)descA->key_bytes;
@@ -1069,6 +1199,8 @@ hg_status hg_intersect_rle(hg_ctx c, const hg_table_desc *descA, const void *d_o
   r.pairs = small + 1;
   r.cursor = small + 2;
   r.overflow = (unsigned int *)(small + 3);
+  r.ovf_list = cv.get<uint32_t>(i_ovf);
+  r.ovf_n = (unsigned int *)(small + 4);
   {
     // giant-bin scratch: control block, then keys, counts of A, counts of B
     char *gb = cv.get<char>(i_giant);
@@ -1086,15 +1218,14 @@ hg_status hg_intersect_rle(hg_ctx c, const hg_table_desc *descA, const void *d_o
   }
   // record positions of the small bins and heavy-list positions: exclusive
   // scans of the chunk counts
-  unsigned long long *status = cv.get<unsigned long long>(i_status);
-  unsigned int *ticket = (unsigned int *)(status + scan_tiles(nch));
-  HG_HIP(launch_scan(r.chunk_cnt, (void *)r.chunk_base, nch, 8, status, ticket, c->stream));
-  HG_HIP(launch_scan(r.chunk_heavy, (void *)r.heavy_base, nch, 4, status, ticket, c->stream));
+  void *status = cv.get<void>(i_status);
+  HG_HIP(launch_scan(r.chunk_cnt, (void *)r.chunk_base, nch, 8, status, c->stream));
+  HG_HIP(launch_scan(r.chunk_heavy, (void *)r.heavy_base, nch, 4, status, c->stream));
   unsigned long long nsmall = 0;
   uint32_t nheavy = 0;
   HG_TRY(sync_read(c, &nsmall, r.chunk_base + nch, 8));
   HG_TRY(sync_read(c, &nheavy, r.heavy_base + nch, 4));
-  if (nheavy > heavy_cap) return fail(HG_ERR_UNSUPPORTED, "too many heavy bins");
+  if (nheavy > heavy_cap) return fail(HG_ERR_DEVICE, "heavy bins exceed their bound");
   // pass 1 lists the heavy bins and, with a capacity, writes the records
   r.pass = 1;
   {
@@ -1109,9 +1240,15 @@ hg_status hg_intersect_rle(hg_ctx c, const hg_table_desc *descA, const void *d_o
     Prof pf(c, "rle_heavy");
     HG_HIP(launch_rle(r, c->stream));
   }
-  unsigned long long res[4];
-  HG_TRY(sync_read(c, res, small, 32));
-  if (res[3] & 0xFFFFFFFFull) return fail(HG_ERR_UNSUPPORTED, "a heavy bin holds too many distinct values");
+  unsigned long long res[5];
+  HG_TRY(sync_read(c, res, small, 40));
+  if (res[3] & 0xFFFFFFFFull) return fail(HG_ERR_UNSUPPORTED, "a heavy bin holds 2^32 or more keys on a side");
+  const uint64_t novf = res[4] & 0xFFFFFFFFull;
+  if (novf > heavy_cap) return fail(HG_ERR_DEVICE, "value-rich bins exceed their bound");
+  if (novf) {
+    HG_TRY(rle_value_rich(c, r, r.ovf_list, (uint32_t)novf));
+    HG_TRY(sync_read(c, res, small, 40));
+  }
   *h_records = res[2];
   *h_count = res[1];
   if (res[2] > capacity && capacity) return fail(HG_ERR_CAPACITY, "record capacity smaller than the record count");
@@ -1327,7 +1464,7 @@ hg_status batch_launch_build(hg_ctx c, const hg_batch_desc *bd, BatchSide &S, co
 hg_status batch_launch_probe(hg_ctx c, const hg_batch_desc *bd, BatchSide &S, const void *d_probe,
                              const uint32_t *d_offsets, const void *d_csr, uint64_t *d_counts) {
   HG_HIP(launch_fill_u64((unsigned long long *)d_counts, 0ull, bd->n_tables, c->stream));
-  HG_HIP(launch_fill_u32(S.flag, 0u, 3, c->stream));  // overflow flag, probe gates
+  HG_HIP(launch_fill_u32(S.flag, 0u, 3, c->stream));  // [1], [2]: the probe gates
   HG_HIP(launch_fill_u32(S.hmark, 0u, S.L.ncols_total, c->stream));
   HG_TRY(run_level(c, S.L, S.d, (int)bd->key_bytes, d_probe, S.part));
   HG_HIP(launch_make_items(S.d.segs, (uint32_t)S.L.segs.size(), S.d.bstart, S.d.coltot, S.d.fin, S.L.ncols_total,
@@ -1346,7 +1483,6 @@ hg_status batch_launch_probe(hg_ctx c, const hg_batch_desc *bd, BatchSide &S, co
   pr.out = nullptr;
   pr.capacity = 0;
   pr.cursor = (unsigned long long *)d_counts;
-  pr.flag = S.flag;
   pr.gate = S.flag + 1;
   pr.hmark = S.hmark;
   Prof pf(c, "probe");
@@ -1439,9 +1575,6 @@ hg_status hg_batch_probe(hg_ctx c, const hg_batch_desc *bd, const uint32_t *d_of
   BatchSide S;
   HG_TRY(batch_prepare(c, bd, h_probe_off, BatchCfg(), S));
   HG_TRY(batch_launch_probe(c, bd, S, d_probe, d_offsets, d_csr_keys, d_counts));
-  unsigned int flag = 0;
-  HG_TRY(sync_read(c, &flag, S.flag, 4));
-  if (flag) return fail(HG_ERR_UNSUPPORTED, "a heavy bin of a batched table holds too many distinct values");
   if (h_total) {
     std::vector<uint64_t> cnt(bd->n_tables);
     for (uint32_t t0 = 0; t0 < bd->n_tables; t0 += 65536) {

@@ -17,9 +17,6 @@
 
 namespace hg {
 
-// A probe key walks at most this many keys of its bin, and a join bin at
-// most this many key pairs; larger bins (heavy duplicates) are reported to
-// the host, which answers them with the run-length join instead.
 // Bins longer than this are not walked by the probe kernels but answered from
 // a value table by k_probe_heavy (duplicates: a walk would cost
 // |A_bin| x |B_bin| comparisons).
@@ -28,7 +25,9 @@ constexpr uint64_t PROBE_BIN_LIMIT = 32;
 // between its smallest and largest key's leaves; wider ranges are read from
 // global memory).
 constexpr uint32_t RANGE_LEAVES = 512;
-constexpr uint64_t JOIN_PAIR_LIMIT = 1ull << 22;
+// A join bin pair with more key pairs than this is answered from a value
+// table (k_join_heavy) instead of the nested loop.
+constexpr uint64_t JOIN_PAIR_LIMIT = 4096;
 
 // Keys per thread that the final stage and the probe keep in VGPRs (u32
 // keys; u64 keys use 32).  1024 threads x KPT is the bucket capacity.
@@ -694,13 +693,15 @@ __global__ __launch_bounds__(BLOCK) void k_probe(const K *__restrict__ bpart, co
 // kernel simpleIntersect include/MultiHashGraphDeviceOperators.cuh:513-543):
 // every (a in A_b, x in B_b) with a == x counts and emits x.  One workgroup
 // per JB bins; A's and B's keys of the range are staged in LDS when they fit.
+// A bin pair above JOIN_PAIR_LIMIT key pairs is listed for k_join_heavy.
 // ---------------------------------------------------------------------------
 template <typename K, typename OFF, int BLOCK, int JB>
 __global__ __launch_bounds__(BLOCK) void k_join(const OFF *__restrict__ offA, const K *__restrict__ keysA,
                                                 const OFF *__restrict__ offB, const K *__restrict__ keysB, uint64_t M,
                                                 K *__restrict__ out, uint64_t capacity,
                                                 unsigned long long *__restrict__ cursor, uint32_t key_cap,
-                                                unsigned int *__restrict__ flag) {
+                                                uint32_t *__restrict__ hlist, unsigned int *__restrict__ hlist_n,
+                                                uint32_t hlist_cap) {
   extern __shared__ __align__(16) uint8_t smem[];
   __shared__ OFF s_oa[JB + 1], s_ob[JB + 1];
   __shared__ unsigned long long s_base;
@@ -725,7 +726,8 @@ __global__ __launch_bounds__(BLOCK) void k_join(const OFF *__restrict__ offA, co
   for (uint32_t l = threadIdx.x; l < n; l += BLOCK) {
     const uint64_t ea = (uint64_t)(s_oa[l + 1] - a0), eb = (uint64_t)(s_ob[l + 1] - x0);
     if ((ea - (uint64_t)(s_oa[l] - a0)) * (eb - (uint64_t)(s_ob[l] - x0)) > JOIN_PAIR_LIMIT) {
-      atomicOr(flag, 1u);
+      const uint32_t i = atomicAdd(hlist_n, 1u);  // k_join_heavy answers this bin
+      if (i < hlist_cap) hlist[i] = (uint32_t)(b0 + l);
       continue;
     }
     for (uint64_t ia = (uint64_t)(s_oa[l] - a0); ia < ea; ia++) {
@@ -765,12 +767,16 @@ __global__ __launch_bounds__(BLOCK) void k_join(const OFF *__restrict__ offA, co
 // ---------------------------------------------------------------------------
 // Single-pass exclusive scan with decoupled look-back (Merrill & Garland
 // 2016).  Tiles take ids from an atomic ticket so a tile only waits on tiles
-// that started before it.  Status word per tile: bits 62-63 flag (1 =
-// aggregate, 2 = inclusive prefix), bits 0-61 value.
+// that started before it.  Per tile: a status flag (0 none, 1 aggregate,
+// 2 inclusive prefix) and two 64-bit words, the aggregate and the inclusive
+// prefix, each written once before the flag that publishes it (release) and
+// read after the flag (acquire), so sums use all 64 bits.
 // ---------------------------------------------------------------------------
 template <typename T, int BLOCK, int IPT>
 __global__ __launch_bounds__(BLOCK) void k_scan_lookback(const T *__restrict__ in, T *__restrict__ out, uint64_t n,
-                                                         unsigned long long *__restrict__ status,
+                                                         unsigned int *__restrict__ flags,
+                                                         unsigned long long *__restrict__ agg_w,
+                                                         unsigned long long *__restrict__ inc_w,
                                                          unsigned int *__restrict__ ticket) {
   __shared__ unsigned int s_tile;
   __shared__ unsigned long long s_excl;
@@ -788,22 +794,25 @@ __global__ __launch_bounds__(BLOCK) void k_scan_lookback(const T *__restrict__ i
   }
   T agg;
   T ex = block_exclusive_sum<BLOCK, T>(sum, s_tmp, agg);
-  constexpr unsigned long long FA = 1ull << 62, FP = 2ull << 62, VM = (1ull << 62) - 1;
   if (threadIdx.x < 64) {
     unsigned long long prefix = 0;
     if (tile == 0) {
-      if (threadIdx.x == 0)
-        __hip_atomic_store(&status[0], FP | ((unsigned long long)agg & VM), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      if (threadIdx.x == 0) {
+        __hip_atomic_store(&inc_w[0], (unsigned long long)agg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&flags[0], 2u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      }
     } else {
-      if (threadIdx.x == 0)
-        __hip_atomic_store(&status[tile], FA | ((unsigned long long)agg & VM), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      if (threadIdx.x == 0) {
+        __hip_atomic_store(&agg_w[tile], (unsigned long long)agg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&flags[tile], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      }
       // wave-wide look-back: lane i inspects tile (p - i)
       int64_t p = (int64_t)tile - 1;
       const int lane = threadIdx.x;
       while (true) {
         const int64_t q = p - lane;
-        unsigned long long st = q >= 0 ? __hip_atomic_load(&status[q], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) : FP;
-        const unsigned long long flag = st >> 62;
+        const unsigned int flag =
+            q >= 0 ? __hip_atomic_load(&flags[q], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) : 2u;
         // lanes up to and including the first inclusive prefix must all be ready
         const unsigned long long inc_mask = __ballot(flag == 2);
         const unsigned long long zero_mask = __ballot(flag == 0);
@@ -813,15 +822,18 @@ __global__ __launch_bounds__(BLOCK) void k_scan_lookback(const T *__restrict__ i
           __builtin_amdgcn_s_sleep(1);
           continue;
         }
-        unsigned long long contrib = lane <= first_inc ? (st & VM) : 0ull;
+        unsigned long long contrib = 0;
+        if (lane <= first_inc && q >= 0)
+          contrib = flag == 2 ? __hip_atomic_load(&inc_w[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                              : __hip_atomic_load(&agg_w[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         contrib = wave_sum<unsigned long long>(contrib);
         prefix += contrib;
         if (first_inc < 64) break;
         p -= 64;
       }
       if (threadIdx.x == 0) {
-        __hip_atomic_store(&status[tile], FP | ((prefix + (unsigned long long)agg) & VM), __ATOMIC_RELEASE,
-                           __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&inc_w[tile], prefix + (unsigned long long)agg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&flags[tile], 2u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         s_excl = prefix;
       }
     }
@@ -1261,6 +1273,121 @@ __device__ __forceinline__ void vt_add_existing(const K *s_key, C *s_cnt, K v, u
 }
 
 // ---------------------------------------------------------------------------
+// Value table of the probe's and the join's long bins: HT_SLOTS values of A
+// with u32 copy counts in LDS; keys equal to the all-ones sentinel are
+// counted apart.  A bin of A is folded in rounds: before each chunk of BLOCK
+// keys a table more than half full is answered (the probe keys look their
+// values up) and cleared, so a chunk's inserts (at most BLOCK new values)
+// always find a free slot.  Copies of one value folded in different rounds
+// are answered in each round; their counts and emitted copies add up, so the
+// result equals one answer over the whole bin.
+// ---------------------------------------------------------------------------
+template <typename K, int BLOCK>
+__device__ __forceinline__ void vt_clear(K *s_key, uint32_t *s_cnt, uint32_t *s_fill, uint32_t *s_ecnt) {
+  for (uint32_t i = threadIdx.x; i < HT_SLOTS; i += BLOCK) {
+    s_key[i] = (K)~(K)0;
+    s_cnt[i] = 0;
+  }
+  if (threadIdx.x == 0) {
+    *s_fill = 0;
+    *s_ecnt = 0;
+  }
+  __syncthreads();
+}
+
+// Every key of q[0..n) looks its value up in the table: the matches are
+// added to *cursor and, with out, each key is written once per copy (one
+// reservation per workgroup and pass).  Ends with a barrier.
+template <typename K, int BLOCK>
+__device__ void vt_answer(const K *s_key, const uint32_t *s_cnt, uint32_t ecnt, const K *__restrict__ q, uint64_t n,
+                          K *__restrict__ out, uint64_t capacity, unsigned long long *cursor,
+                          unsigned long long *s_t64, unsigned long long *s_base) {
+  constexpr K EMPTY = (K)~(K)0;
+  constexpr int U = 8;  // keys per thread per pass (loads in flight together)
+  for (uint64_t c0 = 0; c0 < n; c0 += (uint64_t)BLOCK * U) {
+    K kk[U];
+    uint32_t mm[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const uint64_t i = c0 + (uint64_t)u * BLOCK + threadIdx.x;
+      kk[u] = i < n ? q[i] : (K)0;
+    }
+    unsigned long long mine = 0;
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const uint64_t i = c0 + (uint64_t)u * BLOCK + threadIdx.x;
+      const K k = kk[u];
+      uint32_t m = 0;
+      if (i < n) {
+        if (k == EMPTY) {
+          m = ecnt;
+        } else {
+          uint32_t slot = murmur4((uint32_t)k ^ (uint32_t)((uint64_t)k >> 32)) & (HT_SLOTS - 1);
+          for (uint32_t pr = 0; pr < HT_SLOTS; pr++, slot = (slot + 1) & (HT_SLOTS - 1)) {
+            const K cur = s_key[slot];
+            if (cur == EMPTY) break;
+            if (cur == k) {
+              m = s_cnt[slot];
+              break;
+            }
+          }
+        }
+      }
+      mm[u] = m;
+      mine += m;
+    }
+    if (out == nullptr) {
+      const unsigned long long t = block_sum<BLOCK, unsigned long long>(mine, s_t64);
+      if (threadIdx.x == 0 && t) atomicAdd(cursor, t);
+      continue;
+    }
+    unsigned long long tot;
+    const unsigned long long ex = block_exclusive_sum<BLOCK, unsigned long long>(mine, s_t64, tot);
+    if (threadIdx.x == 0) *s_base = tot ? atomicAdd(cursor, tot) : 0ull;
+    __syncthreads();
+    unsigned long long w = *s_base + ex;
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const unsigned long long m = mm[u];
+      if (m && w < capacity) {
+        const unsigned long long lim = min(m, (unsigned long long)capacity - w);
+        for (unsigned long long r = 0; r < lim; r++) out[w + r] = kk[u];
+      }
+      w += m;
+    }
+    __syncthreads();
+  }
+}
+
+// Folds keys[a0..a1) into the table in chunks of BLOCK keys, answering and
+// clearing it first whenever it is more than half full.  Must start after a
+// barrier; ends with one.  pending: values folded since the last answer.
+template <typename K, int BLOCK, typename ANSWER>
+__device__ __forceinline__ void vt_fold(const K *__restrict__ keys, uint64_t a0, uint64_t a1, K *s_key, uint32_t *s_cnt,
+                                        uint32_t *s_fill, uint32_t *s_ecnt, ANSWER &&answer, bool &pending) {
+  constexpr K EMPTY = (K)~(K)0;
+  for (uint64_t c0 = a0; c0 < a1; c0 += BLOCK) {
+    const bool full = *s_fill > HT_SLOTS / 2;  // uniform: read after a barrier
+    __syncthreads();                           // every thread has read it before it changes
+    if (full) {
+      answer();
+      vt_clear<K, BLOCK>(s_key, s_cnt, s_fill, s_ecnt);
+    }
+    const uint64_t q = c0 + threadIdx.x;
+    if (q < a1) {
+      const K v = keys[q];
+      const uint32_t c = wave_same_value(v);
+      if (c) {
+        if (v == EMPTY) atomicAdd(s_ecnt, c);
+        else vt_insert(s_key, s_cnt, s_fill, v, c);
+      }
+    }
+    __syncthreads();
+    pending = true;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Giant heavy bins of the run-length join (a hot key's bin: up to 10^8 keys
 // per side).  Up to GIANT_CAP of them are split into chunks of GIANT_CHUNK
 // keys over the whole grid: each chunk folds its keys into an LDS value table
@@ -1278,6 +1405,7 @@ struct GiantCtl {
   uint32_t n, totalA, totalB, pad;
   uint32_t nS, nL, pad2[2];  // k_rle_heavy's lists: bins of the small-table and HT_SLOTS forms
   uint32_t hidx[GIANT_CAP];
+  uint32_t bad[GIANT_CAP];   // the bin has more distinct values than the tables hold (k_rle_ovf_* redo it)
   uint32_t chA[GIANT_CAP + 1], chB[GIANT_CAP + 1];
   unsigned long long eA[GIANT_CAP], eB[GIANT_CAP];
 };
@@ -1332,6 +1460,7 @@ __global__ __launch_bounds__(64) void k_giant_prefix(const OFF *__restrict__ off
     b += (uint32_t)(((uint64_t)(offB[bin + 1] - offB[bin]) + GIANT_CHUNK - 1) / GIANT_CHUNK);
     ctl->eA[s] = 0;
     ctl->eB[s] = 0;
+    ctl->bad[s] = 0;
   }
   ctl->chA[n] = a;
   ctl->chB[n] = b;
@@ -1367,8 +1496,7 @@ __device__ __forceinline__ uint32_t giant_slot_of(const uint32_t *ch, uint32_t n
 template <typename K, typename OFF, int SIDE>
 __global__ __launch_bounds__(1024) void k_giant_fold(const OFF *__restrict__ off, const K *__restrict__ keys,
                                                      const uint32_t *__restrict__ heavy, GiantCtl *__restrict__ ctl,
-                                                     K *__restrict__ gkey, unsigned long long *__restrict__ gcnt,
-                                                     unsigned int *__restrict__ overflow) {
+                                                     K *__restrict__ gkey, unsigned long long *__restrict__ gcnt) {
   constexpr K EMPTY = (K)~(K)0;
   __shared__ K s_key[HT_SLOTS];
   __shared__ uint32_t s_cnt[HT_SLOTS];
@@ -1403,7 +1531,7 @@ __global__ __launch_bounds__(1024) void k_giant_fold(const OFF *__restrict__ off
     K *tk = gkey + (uint64_t)slot * GT_SLOTS;
     unsigned long long *tc = gcnt + (uint64_t)slot * GT_SLOTS;
     if (ovf) {
-      if (threadIdx.x == 0) atomicOr(overflow, 1u);
+      if (threadIdx.x == 0) atomicOr(&ctl->bad[slot], 1u);
     } else {
       for (uint32_t i = threadIdx.x; i < HT_SLOTS; i += 1024) {
         const K v = s_key[i];
@@ -1428,7 +1556,7 @@ __global__ __launch_bounds__(1024) void k_giant_fold(const OFF *__restrict__ off
             }
           }
         }
-        if (SIDE == 0 && !done) atomicOr(overflow, 1u);
+        if (SIDE == 0 && !done) atomicOr(&ctl->bad[slot], 1u);
       }
       if (threadIdx.x == 0 && s_e) atomicAdd(SIDE == 0 ? &ctl->eA[slot] : &ctl->eB[slot], (unsigned long long)s_e);
     }
@@ -1436,6 +1564,8 @@ __global__ __launch_bounds__(1024) void k_giant_fold(const OFF *__restrict__ off
   }
 }
 
+// Records of the giant bins; a bin whose tables overflowed is listed for
+// the global-table path instead (ovf_list).
 template <typename K>
 __global__ __launch_bounds__(1024) void k_giant_emit(const GiantCtl *__restrict__ ctl, const K *__restrict__ gkey,
                                                      const unsigned long long *__restrict__ gca,
@@ -1443,10 +1573,19 @@ __global__ __launch_bounds__(1024) void k_giant_emit(const GiantCtl *__restrict_
                                                      unsigned long long *__restrict__ rca,
                                                      unsigned long long *__restrict__ rcb, uint64_t capacity,
                                                      unsigned long long *__restrict__ cursor,
-                                                     unsigned long long *__restrict__ pairs) {
+                                                     unsigned long long *__restrict__ pairs,
+                                                     const uint32_t *__restrict__ heavy, uint32_t *__restrict__ ovf_list,
+                                                     unsigned int *__restrict__ ovf_n, uint64_t ovf_cap) {
   constexpr K EMPTY = (K)~(K)0;
   __shared__ unsigned long long s_t64[1024 / 64 + 1], s_base;
   for (uint32_t slot = blockIdx.x; slot < ctl->n; slot += gridDim.x) {
+    if (ctl->bad[slot]) {  // uniform
+      if (threadIdx.x == 0) {
+        const uint32_t i = atomicAdd(ovf_n, 1u);
+        if (i < ovf_cap) ovf_list[i] = heavy[ctl->hidx[slot]];
+      }
+      continue;
+    }
     const K *tk = gkey + (uint64_t)slot * GT_SLOTS;
     const unsigned long long *ta = gca + (uint64_t)slot * GT_SLOTS, *tb = gcb + (uint64_t)slot * GT_SLOTS;
     for (uint32_t i0 = 0; i0 <= GT_SLOTS; i0 += 1024) {
@@ -1496,7 +1635,9 @@ __global__ __launch_bounds__(BLOCK) void k_rle_heavy(const OFF *__restrict__ off
                                                      const uint32_t *__restrict__ list_n, K *__restrict__ pk,
                                                      unsigned long long *__restrict__ pca,
                                                      unsigned long long *__restrict__ pcb,
-                                                     uint32_t *__restrict__ priv_n, uint32_t region0) {
+                                                     uint32_t *__restrict__ priv_n, uint32_t region0,
+                                                     uint32_t *__restrict__ ovf_list, unsigned int *__restrict__ ovf_n,
+                                                     uint64_t ovf_cap) {
   constexpr K EMPTY = (K)~(K)0;
   __shared__ K s_key[TS];
   // 32-bit counts: a bin with 2^32 or more keys on a side is refused (overflow flag)
@@ -1540,7 +1681,10 @@ __global__ __launch_bounds__(BLOCK) void k_rle_heavy(const OFF *__restrict__ off
     }
     __syncthreads();
     if (s_fill > slots / 2) {  // too many distinct values for the table (only when slots == HT_SLOTS)
-      if (threadIdx.x == 0) atomicOr(overflow, 1u);
+      if (threadIdx.x == 0) {  // the global-table path takes the bin
+        const uint32_t i = atomicAdd(ovf_n, 1u);
+        if (i < ovf_cap) ovf_list[i] = (uint32_t)b;
+      }
       __syncthreads();
       continue;
     }
@@ -1607,6 +1751,127 @@ __global__ __launch_bounds__(BLOCK) void k_rle_heavy(const OFF *__restrict__ off
   if (threadIdx.x == 0) {
     priv_n[region0 + blockIdx.x] = s_pn;
     if (wpairs) atomicAdd(pairs, wpairs);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Heavy bins of the run-length join with more distinct values than an LDS or
+// giant-bin table holds (listed in ovf_list by k_rle_heavy / k_giant_emit).
+// The host lays them out in batches of global-memory tables, one per bin,
+// of at least twice the bin's keys of A (so every insert finds a slot):
+// k_ovf_fold<0> inserts A's keys (compare-and-swap on the key, 32-bit
+// counts), k_ovf_fold<1> adds B's keys to the values present, k_ovf_emit
+// writes a record per value found on both sides.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_ovf_sizes(const uint32_t *__restrict__ list, uint32_t n,
+                                                   const void *__restrict__ offA, const void *__restrict__ offB,
+                                                   int off_bytes, unsigned long long *__restrict__ out) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t b = list[i];
+  unsigned long long a0, a1, x0, x1;
+  if (off_bytes == 4) {
+    a0 = ((const uint32_t *)offA)[b];
+    a1 = ((const uint32_t *)offA)[b + 1];
+    x0 = ((const uint32_t *)offB)[b];
+    x1 = ((const uint32_t *)offB)[b + 1];
+  } else {
+    a0 = ((const unsigned long long *)offA)[b];
+    a1 = ((const unsigned long long *)offA)[b + 1];
+    x0 = ((const unsigned long long *)offB)[b];
+    x1 = ((const unsigned long long *)offB)[b + 1];
+  }
+  out[4 * (uint64_t)i + 0] = a0;
+  out[4 * (uint64_t)i + 1] = a1;
+  out[4 * (uint64_t)i + 2] = x0;
+  out[4 * (uint64_t)i + 3] = x1;
+}
+
+template <typename K, int SIDE>
+__global__ __launch_bounds__(1024) void k_ovf_fold(const K *__restrict__ keys, const OvfChunk *__restrict__ chunks,
+                                                   uint32_t nchunks, const OvfTable *__restrict__ tabs,
+                                                   K *__restrict__ tkey, uint32_t *__restrict__ tcnt,
+                                                   unsigned long long *__restrict__ ecnt) {
+  using U = typename std::conditional<sizeof(K) == 4, unsigned int, unsigned long long>::type;
+  constexpr K EMPTY = (K)~(K)0;
+  for (uint32_t g = blockIdx.x; g < nchunks; g += gridDim.x) {
+    const OvfChunk ch = chunks[g];
+    const OvfTable tb = tabs[ch.tab];
+    for (uint64_t i = ch.begin + threadIdx.x; i < ch.end; i += 1024) {
+      const K v = keys[i];
+      const uint32_t c = wave_same_value(v);
+      if (c == 0) continue;
+      if (v == EMPTY) {
+        atomicAdd(&ecnt[ch.tab], (unsigned long long)c);
+        continue;
+      }
+      uint64_t slot = murmur4((uint32_t)v ^ (uint32_t)((uint64_t)v >> 32)) & tb.mask;
+      for (uint64_t p = 0; p <= tb.mask; p++, slot = (slot + 1) & tb.mask) {
+        if (SIDE == 0) {
+          const K old = (K)atomicCAS((U *)&tkey[tb.base + slot], (U)EMPTY, (U)v);
+          if (old == EMPTY || old == v) {
+            atomicAdd(&tcnt[tb.base + slot], c);
+            break;
+          }
+        } else {
+          const K cur = tkey[tb.base + slot];
+          if (cur == EMPTY) break;
+          if (cur == v) {
+            atomicAdd(&tcnt[tb.base + slot], c);
+            break;
+          }
+        }
+      }
+    }
+  }
+}
+
+// One record per table slot holding a value present on both sides, then
+// (last workgroup-sized pass) the sentinel-valued keys of each bin.
+template <typename K>
+__global__ __launch_bounds__(1024) void k_ovf_emit(const K *__restrict__ tkey, const uint32_t *__restrict__ tca,
+                                                   const uint32_t *__restrict__ tcb, uint64_t nslots,
+                                                   const unsigned long long *__restrict__ eA,
+                                                   const unsigned long long *__restrict__ eB, uint32_t ntabs,
+                                                   K *__restrict__ rk, unsigned long long *__restrict__ rca,
+                                                   unsigned long long *__restrict__ rcb, uint64_t capacity,
+                                                   unsigned long long *__restrict__ cursor,
+                                                   unsigned long long *__restrict__ pairs) {
+  constexpr K EMPTY = (K)~(K)0;
+  __shared__ unsigned long long s_t64[1024 / 64 + 1], s_base;
+  const uint64_t total = nslots + ntabs;  // table slots, then one sentinel entry per table
+  for (uint64_t i0 = (uint64_t)blockIdx.x * 1024; i0 < total; i0 += (uint64_t)gridDim.x * 1024) {
+    const uint64_t i = i0 + threadIdx.x;
+    K v = EMPTY;
+    unsigned long long a = 0, b = 0;
+    if (i < nslots) {
+      v = tkey[i];
+      if (v != EMPTY) {
+        a = tca[i];
+        b = tcb[i];
+      }
+    } else if (i < total) {
+      a = eA[i - nslots];
+      b = eB[i - nslots];
+    }
+    const bool hit = a && b;
+    unsigned long long tot;
+    const unsigned long long ex = block_exclusive_sum<1024, unsigned long long>(hit ? 1ull : 0ull, s_t64, tot);
+    const unsigned long long pt = block_sum<1024, unsigned long long>(hit ? a * b : 0ull, s_t64);
+    if (threadIdx.x == 0) {
+      s_base = tot ? atomicAdd(cursor, tot) : 0ull;
+      if (pt) atomicAdd(pairs, pt);
+    }
+    __syncthreads();
+    if (hit) {
+      const unsigned long long w = s_base + ex;
+      if (w < capacity) {
+        rk[w] = v;
+        rca[w] = a;
+        rcb[w] = b;
+      }
+    }
+    __syncthreads();
   }
 }
 
@@ -2001,23 +2266,18 @@ __global__ __launch_bounds__(BLOCK) void k_probe_big(const K *__restrict__ bpart
 
 // Bins longer than PROBE_BIN_LIMIT (duplicates) of the items k_probe1, k_probe
 // and k_probe_big marked: one workgroup per marked item folds the bucket's
-// long bins of A, bin after bin, into one LDS table value -> copies in A
-// (slots claimed by compare-and-swap of the key against the all-ones
-// sentinel, sentinel keys counted apart; a value lives in one bin, so the
-// bins share the table).  When the table is half full it is answered: every
-// probe key of the bucket looks its value up (a value found is in a long
-// bin; the walking kernels skipped those) and the table is cleared.  A single
-// bin with more than HT_SLOTS/2 distinct values sets *flag (the host reports
-// the probe as unsupported).
+// long bins of A, bin after bin, into one LDS value table (vt_fold; a value
+// lives in one bin, so the bins share the table).  Each answer of the table
+// looks every probe key of the bucket up (a value found is in a long bin;
+// the walking kernels skipped those).  Bins with more distinct values than
+// the table holds are folded over several rounds.
 template <typename K, typename OFF, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_probe_heavy(const K *__restrict__ bpart, const FinItem *__restrict__ items,
-                                                       const BinMap *__restrict__ maps, const OFF *__restrict__ offA,
-                                                       const K *__restrict__ keysA, K *__restrict__ out,
-                                                       uint64_t capacity, unsigned long long *__restrict__ cursor,
+                                                       const OFF *__restrict__ offA, const K *__restrict__ keysA,
+                                                       K *__restrict__ out, uint64_t capacity,
+                                                       unsigned long long *__restrict__ cursor,
                                                        const unsigned int *__restrict__ hmark,
-                                                       unsigned int *__restrict__ flag,
                                                        const unsigned int *__restrict__ gate) {
-  constexpr K EMPTY = (K)~(K)0;
   __shared__ K s_key[HT_SLOTS];
   __shared__ uint32_t s_cnt[HT_SLOTS];
   __shared__ uint32_t s_hl[BLOCK];
@@ -2027,76 +2287,11 @@ __global__ __launch_bounds__(BLOCK) void k_probe_heavy(const K *__restrict__ bpa
   const FinItem it = items[blockIdx.x];
   const K *kp = bpart + it.key_start;
   const K *ka = keysA + it.key_rel;
-  auto clear = [&]() {
-    for (uint32_t i = threadIdx.x; i < HT_SLOTS; i += BLOCK) {
-      s_key[i] = EMPTY;
-      s_cnt[i] = 0;
-    }
-    if (threadIdx.x == 0) {
-      s_fill = 0;
-      s_ecnt = 0;
-    }
-    __syncthreads();
-  };
-  // every probe key of the bucket looks its value up in the table
   auto answer = [&]() {
-    constexpr int U = 8;  // keys per thread per pass (loads in flight together)
-    for (uint64_t c0 = 0; c0 < it.count; c0 += (uint64_t)BLOCK * U) {
-      K kk[U];
-      uint32_t mm[U];
-#pragma unroll
-      for (int u = 0; u < U; u++) {
-        const uint64_t i = c0 + (uint64_t)u * BLOCK + threadIdx.x;
-        kk[u] = i < it.count ? kp[i] : (K)0;
-      }
-      unsigned long long mine = 0;
-#pragma unroll
-      for (int u = 0; u < U; u++) {
-        const uint64_t i = c0 + (uint64_t)u * BLOCK + threadIdx.x;
-        const K k = kk[u];
-        uint32_t m = 0;
-        if (i < it.count) {
-          if (k == EMPTY) {
-            m = s_ecnt;
-          } else {
-            uint32_t slot = murmur4((uint32_t)k ^ (uint32_t)((uint64_t)k >> 32)) & (HT_SLOTS - 1);
-            for (uint32_t pr = 0; pr < HT_SLOTS; pr++, slot = (slot + 1) & (HT_SLOTS - 1)) {
-              const K cur = s_key[slot];
-              if (cur == EMPTY) break;
-              if (cur == k) {
-                m = s_cnt[slot];
-                break;
-              }
-            }
-          }
-        }
-        mm[u] = m;
-        mine += m;
-      }
-      if (out == nullptr) {
-        const unsigned long long t = block_sum<BLOCK, unsigned long long>(mine, s_t64);
-        if (threadIdx.x == 0 && t) atomicAdd(cursor + it.table, t);
-        continue;
-      }
-      unsigned long long tot;
-      const unsigned long long ex = block_exclusive_sum<BLOCK, unsigned long long>(mine, s_t64, tot);
-      if (threadIdx.x == 0) s_base = tot ? atomicAdd(cursor + it.table, tot) : 0ull;
-      __syncthreads();
-      unsigned long long w = s_base + ex;
-#pragma unroll
-      for (int u = 0; u < U; u++) {
-        const unsigned long long m = mm[u];
-        if (m && w < capacity) {
-          const unsigned long long lim = min(m, (unsigned long long)capacity - w);
-          for (unsigned long long r = 0; r < lim; r++) out[w + r] = kk[u];
-        }
-        w += m;
-      }
-      __syncthreads();
-    }
+    vt_answer<K, BLOCK>(s_key, s_cnt, s_ecnt, kp, it.count, out, capacity, cursor + it.table, s_t64, &s_base);
   };
-  clear();
-  bool pending = false;  // bins folded since the last answer (uniform)
+  vt_clear<K, BLOCK>(s_key, s_cnt, &s_fill, &s_ecnt);
+  bool pending = false;  // values folded since the last answer (uniform)
   for (uint32_t h0 = 0; h0 < it.nbins; h0 += BLOCK) {
     if (threadIdx.x == 0) s_nh = 0;
     __syncthreads();
@@ -2109,28 +2304,42 @@ __global__ __launch_bounds__(BLOCK) void k_probe_heavy(const K *__restrict__ bpa
     const uint32_t nh = s_nh;
     __syncthreads();
     for (uint32_t hi = 0; hi < nh; hi++) {
-      if (s_fill > HT_SLOTS / 2) {  // uniform (read after a barrier)
-        answer();
-        clear();
-        pending = false;
-      }
-      const uint32_t f0 = s_fill;
-      __syncthreads();  // every thread has read f0 before the fold adds to s_fill
       const uint32_t hb = s_hl[hi];
       const uint64_t a0 = (uint64_t)offA[it.off_pos + hb], a1 = (uint64_t)offA[it.off_pos + hb + 1];
-      for (uint64_t q = a0 + threadIdx.x; q < a1; q += BLOCK) {
-        const K v = ka[q];
-        const uint32_t c = wave_same_value(v);
-        if (c == 0) continue;
-        if (v == EMPTY) atomicAdd(&s_ecnt, c);
-        else vt_insert(s_key, s_cnt, &s_fill, v, c);
-      }
-      __syncthreads();
-      pending = true;
-      if (s_fill - f0 > HT_SLOTS / 2 && threadIdx.x == 0) atomicOr(flag, 1u);  // too many values in one bin
+      vt_fold<K, BLOCK>(ka, a0, a1, s_key, s_cnt, &s_fill, &s_ecnt, answer, pending);
     }
   }
   if (pending) answer();
+}
+
+// Join bins of two built tables above JOIN_PAIR_LIMIT key pairs, listed by
+// k_join: one workgroup per bin folds A's keys of the bin into the value
+// table (in rounds) and answers it with B's keys of the bin, so each B key x
+// is counted and emitted cA(x) times, as the nested loop would.
+template <typename K, typename OFF, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_join_heavy(const OFF *__restrict__ offA, const K *__restrict__ keysA,
+                                                      const OFF *__restrict__ offB, const K *__restrict__ keysB,
+                                                      K *__restrict__ out, uint64_t capacity,
+                                                      unsigned long long *__restrict__ cursor,
+                                                      const uint32_t *__restrict__ list,
+                                                      const unsigned int *__restrict__ list_n, uint32_t list_cap) {
+  __shared__ K s_key[HT_SLOTS];
+  __shared__ uint32_t s_cnt[HT_SLOTS];
+  __shared__ uint32_t s_fill, s_ecnt;
+  __shared__ unsigned long long s_t64[BLOCK / 64 + 1], s_base;
+  const uint32_t n = min(*list_n, list_cap);
+  for (uint32_t h = blockIdx.x; h < n; h += gridDim.x) {
+    const uint64_t b = list[h];
+    const uint64_t a0 = (uint64_t)offA[b], a1 = (uint64_t)offA[b + 1];
+    const uint64_t x0 = (uint64_t)offB[b], x1 = (uint64_t)offB[b + 1];
+    auto answer = [&]() {
+      vt_answer<K, BLOCK>(s_key, s_cnt, s_ecnt, keysB + x0, x1 - x0, out, capacity, cursor, s_t64, &s_base);
+    };
+    vt_clear<K, BLOCK>(s_key, s_cnt, &s_fill, &s_ecnt);
+    bool pending = false;
+    vt_fold<K, BLOCK>(keysA, a0, a1, s_key, s_cnt, &s_fill, &s_ecnt, answer, pending);
+    if (pending) answer();
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -2283,8 +2492,8 @@ static hipError_t launch_probe_t(const LaunchProbe &p, hipStream_t st) {
                      (const OFF *)p.offA, (const K *)p.keysA, (K *)p.out, p.capacity, p.cursor, small_cap, p.hmark,
                      p.gate);
   hipLaunchKernelGGL((k_probe_heavy<K, OFF, 1024>), dim3(p.nitems), dim3(1024), 0, st, (const K *)p.bpart, p.items,
-                     p.maps, (const OFF *)p.offA, (const K *)p.keysA, (K *)p.out, p.capacity, p.cursor,
-                     (const unsigned int *)p.hmark, p.flag, (const unsigned int *)p.gate);
+                     (const OFF *)p.offA, (const K *)p.keysA, (K *)p.out, p.capacity, p.cursor,
+                     (const unsigned int *)p.hmark, (const unsigned int *)p.gate);
   return hipGetLastError();
 }
 
@@ -2308,7 +2517,12 @@ static hipError_t launch_join_t(const LaunchJoin &j, hipStream_t st) {
   const size_t lds = 2 * (size_t)j.key_cap * sizeof(K);
   hipLaunchKernelGGL((k_join<K, OFF, 256, JB>), dim3((uint32_t)nblk), dim3(256), lds, st, (const OFF *)j.offA,
                      (const K *)j.keysA, (const OFF *)j.offB, (const K *)j.keysB, j.M, (K *)j.out, j.capacity, j.cursor,
-                     j.key_cap, j.flag);
+                     j.key_cap, j.hlist, j.hlist_n, j.hlist_cap);
+  // the heavy bins (a grid that returns at once when k_join listed none)
+  const uint32_t gh = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(j.hlist_cap, 1), 1024);
+  hipLaunchKernelGGL((k_join_heavy<K, OFF, 1024>), dim3(gh), dim3(1024), 0, st, (const OFF *)j.offA, (const K *)j.keysA,
+                     (const OFF *)j.offB, (const K *)j.keysB, (K *)j.out, j.capacity, j.cursor,
+                     (const uint32_t *)j.hlist, (const unsigned int *)j.hlist_n, j.hlist_cap);
   return hipGetLastError();
 }
 
@@ -2325,18 +2539,41 @@ uint32_t join_key_cap(int key_bytes, int off_bytes) {
   return (uint32_t)(budget / (2 * (size_t)key_bytes));
 }
 
-hipError_t launch_scan(const void *in, void *out, uint64_t n, int elem_bytes, unsigned long long *status,
-                       unsigned int *ticket, hipStream_t st) {
+// Status area of a scan over n elements: flags (u32 per tile), aggregates
+// and inclusive prefixes (u64 per tile each), the ticket.
+struct ScanStatus {
+  unsigned int *flags;
+  unsigned long long *agg, *inc;
+  unsigned int *ticket;
+};
+static ScanStatus scan_carve(void *status, uint64_t ntiles) {
+  char *p = (char *)status;
+  ScanStatus s;
+  const size_t fb = (ntiles * 4 + 255) & ~(size_t)255;
+  s.flags = (unsigned int *)p;
+  s.agg = (unsigned long long *)(p + fb);
+  s.inc = s.agg + ntiles;
+  s.ticket = (unsigned int *)(p + fb + ntiles * 16);
+  return s;
+}
+size_t scan_status_bytes(uint64_t n) {
+  const uint64_t nt = scan_tiles(n);
+  return ((nt * 4 + 255) & ~(size_t)255) + nt * 16 + 256;
+}
+
+hipError_t launch_scan(const void *in, void *out, uint64_t n, int elem_bytes, void *status, hipStream_t st) {
   constexpr int B = 256, IPT = 16;
-  const uint64_t ntiles = (n + B * IPT - 1) / (B * IPT);
-  hipLaunchKernelGGL((k_fill<unsigned long long>), dim3(256), dim3(256), 0, st, status, 0ull, ntiles);
-  hipLaunchKernelGGL((k_fill<unsigned int>), dim3(1), dim3(64), 0, st, ticket, 0u, (uint64_t)1);
+  const uint64_t ntiles = scan_tiles(n);
+  if (!ntiles) return hipSuccess;
+  const ScanStatus ss = scan_carve(status, ntiles);
+  hipLaunchKernelGGL((k_fill<unsigned int>), dim3(256), dim3(256), 0, st, ss.flags, 0u, ntiles);
+  hipLaunchKernelGGL((k_fill<unsigned int>), dim3(1), dim3(64), 0, st, ss.ticket, 0u, (uint64_t)1);
   if (elem_bytes == 4)
     hipLaunchKernelGGL((k_scan_lookback<uint32_t, B, IPT>), dim3((uint32_t)ntiles), dim3(B), 0, st, (const uint32_t *)in,
-                       (uint32_t *)out, n, status, ticket);
+                       (uint32_t *)out, n, ss.flags, ss.agg, ss.inc, ss.ticket);
   else
     hipLaunchKernelGGL((k_scan_lookback<uint64_t, B, IPT>), dim3((uint32_t)ntiles), dim3(B), 0, st, (const uint64_t *)in,
-                       (uint64_t *)out, n, status, ticket);
+                       (uint64_t *)out, n, ss.flags, ss.agg, ss.inc, ss.ticket);
   return hipGetLastError();
 }
 uint64_t scan_tiles(uint64_t n) { return (n + 256 * 16 - 1) / (256 * 16); }
@@ -2363,28 +2600,60 @@ static hipError_t launch_rle_t(const LaunchRle &r, hipStream_t st) {
                        (const uint32_t *)r.heavy, ctl);
     hipLaunchKernelGGL((k_giant_clear<K>), dim3(1024), dim3(256), 0, st, (const GiantCtl *)ctl, (K *)r.gkey, r.gca, r.gcb);
     hipLaunchKernelGGL((k_giant_fold<K, OFF, 0>), dim3(1024), dim3(1024), 0, st, (const OFF *)r.offA, (const K *)r.keysA,
-                       (const uint32_t *)r.heavy, ctl, (K *)r.gkey, r.gca, r.overflow);
+                       (const uint32_t *)r.heavy, ctl, (K *)r.gkey, r.gca);
     hipLaunchKernelGGL((k_giant_fold<K, OFF, 1>), dim3(1024), dim3(1024), 0, st, (const OFF *)r.offB, (const K *)r.keysB,
-                       (const uint32_t *)r.heavy, ctl, (K *)r.gkey, r.gcb, r.overflow);
+                       (const uint32_t *)r.heavy, ctl, (K *)r.gkey, r.gcb);
     hipLaunchKernelGGL((k_giant_emit<K>), dim3(GIANT_CAP), dim3(1024), 0, st, (const GiantCtl *)ctl, (const K *)r.gkey,
                        (const unsigned long long *)r.gca, (const unsigned long long *)r.gcb, (K *)r.rk, r.rca, r.rcb,
-                       r.capacity, r.cursor, r.pairs);
+                       r.capacity, r.cursor, r.pairs, (const uint32_t *)r.heavy, r.ovf_list, r.ovf_n, r.heavy_cap);
     const uint32_t gs = (uint32_t)std::min<uint64_t>(r.nheavy, RLE_HEAVY_GRID_S);
     const uint32_t gb = (uint32_t)std::min<uint64_t>(r.nheavy, RLE_HEAVY_GRID);
     hipLaunchKernelGGL((k_rle_heavy<K, OFF, 256, RLE_HEAVY_SMALL_TS>), dim3(gs), dim3(256), 0, st, (const OFF *)r.offA,
                        (const K *)r.keysA, (const OFF *)r.offB, (const K *)r.keysB, (K *)r.rk, r.rca,
                        r.rcb, r.capacity, r.cursor, r.pairs, r.overflow, (const uint32_t *)r.gmark,
-                       (const uint32_t *)&ctl->nS, (K *)r.pkey, r.pca, r.pcb, r.priv_n, 0u);
+                       (const uint32_t *)&ctl->nS, (K *)r.pkey, r.pca, r.pcb, r.priv_n, 0u, r.ovf_list, r.ovf_n,
+                       r.heavy_cap);
     hipLaunchKernelGGL((k_rle_heavy<K, OFF, 1024, HT_SLOTS>), dim3(gb), dim3(1024), 0, st, (const OFF *)r.offA,
                        (const K *)r.keysA, (const OFF *)r.offB, (const K *)r.keysB, (K *)r.rk, r.rca,
                        r.rcb, r.capacity, r.cursor, r.pairs, r.overflow, (const uint32_t *)(r.gmark + r.heavy_cap),
-                       (const uint32_t *)&ctl->nL, (K *)r.pkey, r.pca, r.pcb, r.priv_n, gs);
+                       (const uint32_t *)&ctl->nL, (K *)r.pkey, r.pca, r.pcb, r.priv_n, gs, r.ovf_list, r.ovf_n,
+                       r.heavy_cap);
     const uint32_t g = gs + gb;
     hipLaunchKernelGGL((k_rle_compact<K>), dim3(g), dim3(256), 0, st, (const uint32_t *)r.priv_n, (const K *)r.pkey,
                        (const unsigned long long *)r.pca, (const unsigned long long *)r.pcb, (K *)r.rk, r.rca, r.rcb,
                        r.capacity, (const unsigned long long *)r.cursor);
     hipLaunchKernelGGL(k_rle_finish, dim3(1), dim3(1024), 0, st, (const uint32_t *)r.priv_n, g, r.cursor);
   }
+  return hipGetLastError();
+}
+
+template <typename K>
+static hipError_t launch_rle_ovf_t(const LaunchRleOvf &o, hipStream_t st) {
+  if (o.nchunksA) {
+    const uint32_t g = (uint32_t)std::min<uint64_t>(o.nchunksA, 2048);
+    hipLaunchKernelGGL((k_ovf_fold<K, 0>), dim3(g), dim3(1024), 0, st, (const K *)o.keysA, o.chunksA, o.nchunksA, o.tabs,
+                       (K *)o.tkey, o.tca, o.eA);
+  }
+  if (o.nchunksB) {
+    const uint32_t g = (uint32_t)std::min<uint64_t>(o.nchunksB, 2048);
+    hipLaunchKernelGGL((k_ovf_fold<K, 1>), dim3(g), dim3(1024), 0, st, (const K *)o.keysB, o.chunksB, o.nchunksB, o.tabs,
+                       (K *)o.tkey, o.tcb, o.eB);
+  }
+  const uint32_t ge = (uint32_t)std::min<uint64_t>((o.nslots + o.ntabs + 1023) / 1024, 4096);
+  hipLaunchKernelGGL((k_ovf_emit<K>), dim3(ge), dim3(1024), 0, st, (const K *)o.tkey, (const uint32_t *)o.tca,
+                     (const uint32_t *)o.tcb, o.nslots, (const unsigned long long *)o.eA, (const unsigned long long *)o.eB,
+                     o.ntabs, (K *)o.rk, o.rca, o.rcb, o.capacity, o.cursor, o.pairs);
+  return hipGetLastError();
+}
+
+hipError_t launch_rle_ovf(const LaunchRleOvf &o, hipStream_t st) {
+  return o.key_bytes == 4 ? launch_rle_ovf_t<uint32_t>(o, st) : launch_rle_ovf_t<uint64_t>(o, st);
+}
+
+hipError_t launch_ovf_sizes(const uint32_t *list, uint32_t n, const void *offA, const void *offB, int off_bytes,
+                            unsigned long long *out, hipStream_t st) {
+  if (!n) return hipSuccess;
+  hipLaunchKernelGGL(k_ovf_sizes, dim3((n + 255) / 256), dim3(256), 0, st, list, n, offA, offB, off_bytes, out);
   return hipGetLastError();
 }
 

@@ -71,7 +71,6 @@ struct LaunchProbe {
   uint64_t capacity;
   unsigned long long *cursor;  // match counters, one per table (FinItem.table)
   unsigned int *hmark;         // nitems, zeroed: items with a bin beyond the walk limit
-  unsigned int *flag;          // a heavy bin with too many distinct values
   unsigned int *gate;          // 2 zeroed words: [0] k_probe1 left items, [1] a bin was marked heavy
 };
 
@@ -83,7 +82,11 @@ struct LaunchJoin {
   void *out;
   uint64_t capacity;
   unsigned long long *cursor;
-  unsigned int *flag;
+  // bins above the nested loop's pair limit: list (hlist_cap entries) and its
+  // zeroed counter
+  uint32_t *hlist;
+  unsigned int *hlist_n;
+  uint32_t hlist_cap;
 };
 
 struct LaunchRle {
@@ -112,6 +115,28 @@ struct LaunchRle {
   void *pkey;
   unsigned long long *pca, *pcb;
   uint32_t *priv_n;
+  // heavy bins with more distinct values than the LDS / giant tables hold:
+  // list (heavy_cap entries) and its zeroed counter
+  uint32_t *ovf_list;
+  unsigned int *ovf_n;
+};
+
+// One batch of the run-length join's global-table path (value-rich bins).
+struct LaunchRleOvf {
+  int key_bytes;
+  const void *keysA, *keysB;
+  const OvfChunk *chunksA, *chunksB;
+  uint32_t nchunksA, nchunksB;
+  const OvfTable *tabs;
+  uint32_t ntabs;
+  uint64_t nslots;           // table slots of the batch
+  void *tkey;                // nslots keys (cleared to all ones)
+  uint32_t *tca, *tcb;       // nslots counts each (zeroed)
+  unsigned long long *eA, *eB;  // ntabs sentinel-key counts each (zeroed)
+  void *rk;
+  unsigned long long *rca, *rcb;
+  uint64_t capacity;
+  unsigned long long *cursor, *pairs;
 };
 
 struct LaunchRmi {
@@ -121,8 +146,7 @@ struct LaunchRmi {
   uint64_t s0;               // mix64(seed)
   void *buf0, *buf1;         // n_sample keys each
   uint32_t *counts;          // 256 * sort tiles + 1
-  unsigned long long *scan_status;
-  unsigned int *scan_ticket;
+  void *scan_status;         // scan_status_bytes(256 * sort tiles)
   double *partials;          // 8 * nblk
   void *cand;                // 4 root candidates (16 bytes each)
   uint32_t *leafcnt;         // 4 * n_leaves
@@ -140,6 +164,10 @@ hipError_t launch_rmi_train(const LaunchRmi &r, hipStream_t st);
 uint32_t rmi_moment_blocks(uint64_t n_sample);
 uint32_t rmi_sort_tiles(uint64_t n_sample);
 hipError_t launch_rle(const LaunchRle &r, hipStream_t st);
+hipError_t launch_rle_ovf(const LaunchRleOvf &o, hipStream_t st);
+// (a0, a1, x0, x1) of each listed bin -> out (4 entries per bin)
+hipError_t launch_ovf_sizes(const uint32_t *list, uint32_t n, const void *offA, const void *offB, int off_bytes,
+                            unsigned long long *out, hipStream_t st);
 uint64_t rle_chunks(uint64_t M);
 // scratch bytes of the giant-bin path of the run-length join
 size_t rle_giant_bytes(int key_bytes);
@@ -153,8 +181,9 @@ hipError_t launch_colscan(const LaunchCol &lc, hipStream_t st);
 hipError_t launch_final(const LaunchFinal &lf, hipStream_t st);
 hipError_t launch_probe(const LaunchProbe &p, hipStream_t st);
 hipError_t launch_join(const LaunchJoin &j, hipStream_t st);
-hipError_t launch_scan(const void *in, void *out, uint64_t n, int elem_bytes, unsigned long long *status,
-                       unsigned int *ticket, hipStream_t st);
+// exclusive scan of n elements (total at out[n]); status: scan_status_bytes(n)
+hipError_t launch_scan(const void *in, void *out, uint64_t n, int elem_bytes, void *status, hipStream_t st);
+size_t scan_status_bytes(uint64_t n);
 hipError_t launch_fill_u64(unsigned long long *p, unsigned long long v, uint64_t n, hipStream_t st);
 hipError_t launch_fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t st);
 hipError_t launch_bins(const void *keys, uint64_t n, int key_bytes, const BinMap &mp, uint32_t *bins, hipStream_t st);

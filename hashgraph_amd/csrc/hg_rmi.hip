@@ -354,7 +354,7 @@ static hipError_t launch_rmi_train_t(const LaunchRmi &r, hipStream_t st) {
     const int shift = pass * 8;
     hipLaunchKernelGGL((k_sort_hist<K>), dim3(ntiles), dim3(SORT_BLOCK), 0, st, (const K *)a, ns, shift, r.counts,
                        ntiles);
-    hipError_t e = launch_scan(r.counts, r.counts, (uint64_t)256 * ntiles, 4, r.scan_status, r.scan_ticket, st);
+    hipError_t e = launch_scan(r.counts, r.counts, (uint64_t)256 * ntiles, 4, r.scan_status, st);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((k_sort_scatter<K>), dim3(ntiles), dim3(SORT_BLOCK), 0, st, (const K *)a, b, ns, shift,
                        (const uint32_t *)r.counts, ntiles);

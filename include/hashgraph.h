@@ -105,7 +105,11 @@ hg_status hg_build(hg_ctx ctx, const hg_table_desc *desc, const void *d_keys, vo
  * emits x.  Count = sum over values v of cA(v)*cB(v).  Both tables must
  * share key_bytes, hash_kind and n_bins.
  *  h_count    total matches (host, exact 64-bit)
- *  d_out      NULL for count only; else capacity keys, written in bin order
+ *  d_out      NULL for count only; else capacity keys: the same multiset as
+ *             the reference's list, grouped by bin ranges (the order inside
+ *             the list is unspecified, as bin contents are)
+ *  Every bin is answered, however heavy: a bin pair above 4096 key pairs is
+ *  joined through a value table of A's bin instead of the nested loop.
  *  Returns HG_ERR_CAPACITY (with *h_count set) when capacity < count.     */
 hg_status hg_intersect(hg_ctx ctx, const hg_table_desc *descA, const void *d_offA, const void *d_keysA,
                        const hg_table_desc *descB, const void *d_offB, const void *d_keysB, void *d_out,
@@ -115,7 +119,8 @@ hg_status hg_intersect(hg_ctx ctx, const hg_table_desc *descA, const void *d_off
  * Same result as building d_probe into a table B over A's bins and calling
  * hg_intersect(A, B) (same count, same multiset of emitted keys: each probe
  * key x is emitted cA(x) times), without materialising B's CSR.  Output order
- * is grouped by A's bins.                                                   */
+ * is grouped by buckets of A's bins.  Any table hg_build accepts can be
+ * probed (B is partitioned in two levels for tables beyond ~2^28 bins).    */
 hg_status hg_probe(hg_ctx ctx, const hg_table_desc *descA, const void *d_offA, const void *d_keysA,
                    const void *d_probe, uint64_t n_probe, void *d_out, uint64_t capacity, uint64_t *h_count);
 
@@ -123,8 +128,9 @@ hg_status hg_probe(hg_ctx ctx, const hg_table_desc *descA, const void *d_offA, c
  * Heavy duplicates make the emitted list of hg_intersect infeasible (a value
  * with 10^8 copies in both tables yields 10^16 pairs).  This form writes one
  * record per distinct value present in both tables: key (key_bytes),
- * count in A and count in B (uint64 each), bins in order.
- *  h_count    sum cA*cB (exact; saturates at 2^64-1)
+ * count in A and count in B (uint64 each).  The records of bins with at most
+ * 64 keys per side come first, in bin order; those of heavier bins follow.
+ *  h_count    sum cA*cB (exact below 2^64)
  *  h_records  number of records                                           */
 hg_status hg_intersect_rle(hg_ctx ctx, const hg_table_desc *descA, const void *d_offA, const void *d_keysA,
                            const hg_table_desc *descB, const void *d_offB, const void *d_keysB, void *d_rec_keys,
@@ -233,8 +239,9 @@ hg_status hg_batch_plan_destroy(hg_batch_plan plan);
 /* ---- primitives ----------------------------------------------------------
  * Exclusive prefix sum (single-pass decoupled look-back), replaces
  * cub::DeviceScan::ExclusiveSum at src/SingleHashGraph.cu:147-152,166-172 and
- * src/MultiHashGraph.cu:327-336,353-365,706-715.  elem_bytes 4 or 8;
- * d_out may alias d_in; d_out has n + 1 elements (the total goes last).   */
+ * src/MultiHashGraph.cu:327-336,353-365,706-715.  elem_bytes 4 or 8 (sums
+ * modulo 2^32 / 2^64, every bit kept); d_out may alias d_in; d_out has n + 1
+ * elements (the total goes last).                                         */
 hg_status hg_exclusive_scan(hg_ctx ctx, const void *d_in, void *d_out, uint64_t n, uint32_t elem_bytes);
 /* bins of n keys (murmur3, seed 0, low 32 bits, mod M) -> uint32. */
 hg_status hg_hash_bins(hg_ctx ctx, const void *d_keys, uint64_t n, uint32_t key_bytes, uint64_t n_bins,

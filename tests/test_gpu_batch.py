@@ -173,3 +173,34 @@ def test_batch_u64_tables(dev):
     exp = _expected_counts(keys, ko, probe, po)
     assert counts.cpu().numpy().tolist() == exp
     assert total == sum(exp)
+
+
+def test_plan_replays_value_rich_bins(dev):
+    """A batch whose tables have long bins with more distinct values than one
+    probe value table (a 5000-value table with 2 bins, an 8000-value table
+    with 1 bin): every plan replay's counts equal the oracle's."""
+    import torch
+    import hashgraph_amd as hg
+    sizes = [5000, 300, 8000, 2000]
+    parts = [wl.uniform_keys(200 + i, s, 4) for i, s in enumerate(sizes)]
+    keys = np.concatenate(parts)
+    ko = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
+    n_bins = np.array([2, 300, 1, 2048])
+    rng = np.random.default_rng(5)
+    probe = np.concatenate([np.concatenate([p[rng.integers(0, len(p), len(p) // 2)],
+                                            wl.uniform_keys(300 + i, len(p) - len(p) // 2, 4)])
+                            for i, p in enumerate(parts)])
+    po = ko.copy()
+    dk, dp = _to_dev(keys, dev), _to_dev(probe, dev)
+    exp = _expected_counts(keys, ko, probe, po)
+    off, csr, off_off = hg.batch_build(dk, ko, n_bins)
+    total, counts = hg.batch_probe(off, csr, ko, n_bins, off_off, dp, po)
+    assert counts.cpu().numpy().tolist() == exp
+    plan = hg.BatchPlan(dk, ko, n_bins, probe=dp, probe_off=po, autotune=False)
+    for r in range(3):
+        plan.run()
+        torch.cuda.synchronize()
+        assert plan.counts.cpu().numpy().tolist() == exp, f"replay {r}"
+        plan.counts.fill_(-1)
+        torch.cuda.synchronize()
+    plan.close()

@@ -155,3 +155,19 @@ def test_exclusive_scan(n, width, dev):
     exp = np.zeros(n + 1, dtype=dt)
     exp[1:] = np.cumsum(x, dtype=dt)
     assert np.array_equal(out, exp)
+
+
+def test_exclusive_scan_u64_beyond_2e62(dev):
+    """uint64 sums beyond 2^62 and 2^63 (and wrapping mod 2^64): the scan's
+    64-bit aggregate and prefix words carry them exactly."""
+    import torch
+    import hashgraph_amd as hg
+    n = 100000
+    x = (wl.stream_np(77, n) >> np.uint64(9)).astype(np.uint64)  # ~2^54 each: the sum passes 2^62, 2^63, 2^64
+    x[n // 2] = np.uint64(2**63 + 12345)
+    t = torch.from_numpy(x.view(np.int64).copy()).to(dev)
+    out = hg.exclusive_scan(t).cpu().numpy().view(np.uint64)
+    exp = np.zeros(n + 1, dtype=np.uint64)
+    exp[1:] = np.cumsum(x, dtype=np.uint64)
+    assert exp[-1] < exp[n // 2]  # wrapped at least once
+    assert np.array_equal(out, exp)

@@ -163,3 +163,28 @@ def test_rle_giant_u32(dev):
     A = vals[rng.integers(0, len(vals), 600000)]
     B = np.concatenate([vals[rng.integers(0, len(vals), 200000)], wl.uniform_keys(37, 1000, 4)])
     _check_rle(A, B, 3, dev)
+
+
+def test_rle_value_rich_heavy_bins_u64(dev):
+    """Two bins of ~6000 keys of A with ~5000 distinct values each: more
+    than the heavy-bin LDS table holds, so they take the global-table path;
+    records equal the oracle's, one per value."""
+    vals = wl.uniform_keys(51, 10000, 8)
+    A = np.concatenate([vals, vals[:2000], np.full(20, 2**64 - 1, dtype=np.uint64)])
+    B = np.concatenate([vals[::2], vals[:100], wl.uniform_keys(52, 3000, 8), np.full(7, 2**64 - 1, dtype=np.uint64)])
+    _check_rle(A, B, 2, dev)
+
+
+def test_rle_value_rich_giant_bin(dev):
+    """A giant bin (over 2^18 keys) with 120000 distinct values: its chunk
+    tables overflow, so the bin is redone in a global table; beside it a
+    giant single-value bin stays on the giant path."""
+    rng = np.random.default_rng(53)
+    vals = wl.uniform_keys(54, 120000, 4)
+    A = vals[rng.integers(0, len(vals), 400000)]
+    B = np.concatenate([vals[rng.integers(0, len(vals), 150000)], wl.uniform_keys(55, 20000, 4)])
+    _check_rle(A, B, 1, dev)
+    hot = np.uint32(77)
+    A2 = np.concatenate([A, np.full(300000, hot, dtype=np.uint32)])
+    B2 = np.concatenate([B, np.full(1000, hot, dtype=np.uint32)])
+    _check_rle(A2, B2, 3, dev)

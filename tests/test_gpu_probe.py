@@ -132,15 +132,45 @@ def test_many_long_bins_answered_in_groups(dev):
     _check_probe(A, B, 512, dev)
 
 
-def test_heavy_bin_with_many_values_is_refused(dev):
-    """One bin of 10000 distinct values: beyond both the walk limit and the
-    per-bin value table; the library reports it instead of running it."""
+def test_heavy_bin_with_many_values(dev):
+    """One bin (M = 1) of 10000 distinct values, some repeated: beyond the
+    walk limit and beyond one value table, so the bin is folded over several
+    table rounds; probe and join equal the oracle's bag join."""
+    base = wl.uniform_keys(1, 10000, 4)
+    A = np.concatenate([base, base[:700], np.full(40, 0xFFFFFFFF, dtype=np.uint32)])
+    B = np.concatenate([base[:1000], base[5000:5600], wl.uniform_keys(2, 800, 4),
+                        np.full(3, 0xFFFFFFFF, dtype=np.uint32)])
+    _check_probe(A, B, 1, dev)
+    _check_join(A, B, 1, dev)
+
+
+def test_value_rich_bins_u64(dev):
+    """Bins of ~6000 keys with ~5000 distinct values each (M = 4): every join
+    bin pair is above the nested loop's pair limit and every probe bin above
+    the walk limit; both take several table rounds per bin."""
+    vals = wl.uniform_keys(3, 20000, 8)
+    A = np.concatenate([vals, vals[:4000]])
+    B = np.concatenate([vals[::3], wl.uniform_keys(4, 5000, 8)])
+    _check_probe(A, B, 4, dev, ob=8)
+    _check_join(A, B, 4, dev, ob=8)
+
+
+def test_join_bin_above_2e22_pairs(dev):
+    """A hot value with 3000 copies in A and 2000 in B (6e6 pairs in one bin,
+    above 2^22): the join answers it from a value table; count equals the
+    bag count, and a smaller hot bin's emitted list equals the oracle's."""
     import hashgraph_amd as hg
-    A = wl.uniform_keys(1, 10000, 4)
-    off, csr = hg.build(_to_dev(A, dev), 1)
-    with pytest.raises(hg.HGError) as e:
-        hg.probe(off, csr, 1, _to_dev(A[:1000], dev), emit=False)
-    assert e.value.status == 6
+    rest = wl.uniform_keys(5, 30000, 4)
+    A = np.concatenate([np.full(3000, 987654321, dtype=np.uint32), rest])
+    B = np.concatenate([np.full(2000, 987654321, dtype=np.uint32), rest[:10000], wl.uniform_keys(6, 5000, 4)])
+    M = 1 << 14
+    offA, csrA = hg.build(_to_dev(A, dev), M)
+    offB, csrB = hg.build(_to_dev(B, dev), M)
+    cnt, _ = hg.intersect(offA, csrA, offB, csrB, M, emit=False)
+    assert cnt == o.bag_intersect_count(A, B)
+    A2 = np.concatenate([np.full(150, 42, dtype=np.uint32), rest[:2000]])
+    B2 = np.concatenate([np.full(90, 42, dtype=np.uint32), rest[:500]])
+    _check_join(A2, B2, 1 << 10, dev)
 
 
 def test_mismatched_tables_rejected(dev):
@@ -188,3 +218,33 @@ def test_full_size_2e28_probe_properties(dev):
     uo, co = torch.unique_consecutive(so, return_counts=True)
     assert torch.equal(uo, ub[hit])
     assert torch.equal(co.to(torch.int64), (ca[pos][hit].to(torch.int64) * cb[hit].to(torch.int64)))
+
+
+def test_two_level_probe_2e28_u64_properties(dev):
+    """2^28 uint64 keys probe a 2^28-bin table: more buckets than one
+    partition level holds, so B is partitioned in two levels.  The count
+    equals a device-side sort-merge count and the emitted multiset holds
+    cA(v) * cB(v) copies of each value."""
+    import torch
+    import hashgraph_amd as hg
+    n = M = 1 << 28
+    print("two-level probe 2^28 u64", flush=True)
+    A = wl.uniform_keys(11, n, 8, device=dev)
+    B = wl.probe_keys(A, 12, n, 0.25, 8, device=dev)
+    off, csr = hg.build(A, M)
+    cnt, out = hg.probe(off, csr, M, B)
+    del off, csr
+    sa = torch.sort(A)[0]
+    del A
+    sb = torch.sort(B)[0]
+    del B
+    ua, ca = torch.unique_consecutive(sa, return_counts=True)
+    ub, cb = torch.unique_consecutive(sb, return_counts=True)
+    del sa, sb
+    pos = torch.searchsorted(ua, ub).clamp(max=ua.numel() - 1)
+    hit = ua[pos] == ub
+    want = ca[pos][hit].to(torch.int64) * cb[hit].to(torch.int64)
+    assert cnt == int(want.sum())
+    uo, co = torch.unique_consecutive(torch.sort(out)[0], return_counts=True)
+    assert torch.equal(uo, ub[hit])
+    assert torch.equal(co.to(torch.int64), want)
