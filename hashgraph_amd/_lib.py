@@ -25,7 +25,7 @@ EXPORTS = [
     "hg_version", "hg_ctx_profile", "hg_ctx_profile_read", "hg_build", "hg_intersect", "hg_probe", "hg_intersect_rle", "hg_find_splits",
     "hg_build_partitioned", "hg_rmi_model_bytes", "hg_rmi_train", "hg_batch_build", "hg_batch_probe",
     "hg_exclusive_scan", "hg_hash_bins", "hg_table_bins", "hg_batch_plan_create", "hg_batch_plan_run",
-    "hg_batch_plan_query", "hg_batch_plan_destroy",
+    "hg_batch_plan_query", "hg_batch_plan_destroy", "hg_build_noalloc",
 ]
 
 
@@ -89,6 +89,7 @@ def load(path: str = LIB_PATH):
             "hg_ctx_profile": (i32, [vp, i32]),
             "hg_ctx_profile_read": (i32, [vp, P(KernelTime), u32, P(u32), i32]),
             "hg_build": (i32, [vp, P(TableDesc), vp, vp, vp]),
+            "hg_build_noalloc": (i32, [vp, P(TableDesc), vp, vp, vp]),
             "hg_intersect": (i32, [vp, P(TableDesc), vp, vp, P(TableDesc), vp, vp, vp, u64, P(u64)]),
             "hg_probe": (i32, [vp, P(TableDesc), vp, vp, vp, u64, vp, u64, P(u64)]),
             "hg_intersect_rle": (i32, [vp, P(TableDesc), vp, vp, P(TableDesc), vp, vp, vp, vp, vp, u64, P(u64),

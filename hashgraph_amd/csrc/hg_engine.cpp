@@ -6,6 +6,7 @@
 #include <string.h>
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -312,13 +313,14 @@ struct Level {
   // scan), two 1.31 + 0.16 ms, four 1.42 + 0.09 ms, ten 1.91 + 0.08 ms.  Two
   // rounds are used from 2^25 keys on (>= 512 tiles remain).  HG_TILE_ROUNDS
   // sets the round count (1..64) for tuning.
-  void auto_tile(uint64_t n) {
+  void auto_tile(uint64_t n) { tile_keys = tile_keys_for(n); }
+  static uint32_t tile_keys_for(uint64_t n) {
     uint64_t r = n >= (1ull << 25) ? 2 : 1;
     if (const char *e = getenv("HG_TILE_ROUNDS")) {
       const long v = atol(e);
       if (v >= 1 && v <= 64) r = (uint64_t)v;
     }
-    tile_keys = (uint32_t)(kTile * r);
+    return (uint32_t)(kTile * r);
   }
 
   void prepare() {
@@ -732,6 +734,33 @@ hg_status rle_value_rich(hg_ctx c, const LaunchRle &r, const uint32_t *d_list, u
   return HG_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Static scratch of hg_build_noalloc: the only device memory that entry point
+// uses beyond the caller's buffers (partitioned copy, level descriptors,
+// count matrix, final-stage items and oversized-bucket counters).  Each
+// device holds its own copy from module load to process end.  In host code
+// `scratch` is only a name for the symbol: its device address on each device
+// comes from hipGetSymbolAddress, taken once per device when a context is
+// created with that device current, and kept in g_scratch.
+// ---------------------------------------------------------------------------
+constexpr size_t kNoAllocBytes = 40ull << 20;
+static __device__ __align__(256) char scratch[kNoAllocBytes];
+constexpr int kMaxDevices = 64;
+char *g_scratch[kMaxDevices] = {};
+std::mutex g_scratch_mu;
+
+hg_status scratch_setup(int device) {
+  if (device < 0 || device >= kMaxDevices) return fail(HG_ERR_UNSUPPORTED, "device index beyond 63");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch[device]) return HG_OK;
+  void *p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p)
+    return fail(HG_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+  g_scratch[device] = (char *)p;
+  return HG_OK;
+}
+
 uint64_t ref_binary_search(const std::vector<int64_t> &bins, uint64_t l, uint64_t r, uint64_t x) {
   // include/MultiHashGraphHostOperators.cuh:8-27 (recursive, size_t indices)
   for (;;) {
@@ -777,6 +806,13 @@ hg_status hg_ctx_create(int device, hg_ctx *out) {
   if (hipHostMalloc((void **)&c->h_small, 1 << 20, hipHostMallocDefault) != hipSuccess) {
     hg_ctx_destroy(c);
     return fail(HG_ERR_DEVICE, "hipHostMalloc failed");
+  }
+  {
+    const hg_status st = scratch_setup(device);  // device is current (DeviceGuard)
+    if (st != HG_OK) {
+      hg_ctx_destroy(c);
+      return st;
+    }
   }
   static bool attrs_done[64] = {false};
   if (device < 64 && !attrs_done[device]) {
@@ -875,6 +911,118 @@ hg_status hg_build(hg_ctx c, const hg_table_desc *desc, const void *d_keys, void
   if (!d_offsets || (desc->n_keys && (!d_keys || !d_csr_keys))) return fail(HG_ERR_INVALID_ARG, "null buffer");
   DeviceGuard g(c->device);
   return build_table(c, desc, d_keys, d_offsets, d_csr_keys);
+}
+
+hg_status hg_build_noalloc(hg_ctx c, const hg_table_desc *desc, const void *d_keys, void *d_offsets, void *d_csr_keys) {
+  if (!c) return fail(HG_ERR_INVALID_ARG, "null ctx");
+  HG_TRY(check_desc(desc));
+  if (!d_offsets || (desc->n_keys && (!d_keys || !d_csr_keys))) return fail(HG_ERR_INVALID_ARG, "null buffer");
+  DeviceGuard g(c->device);
+  char *const base = c->device >= 0 && c->device < kMaxDevices ? g_scratch[c->device] : nullptr;
+  if (!base) return fail(HG_ERR_DEVICE, "hg_build_noalloc: no static scratch address for this device");
+  const uint64_t n = desc->n_keys, M = desc->n_bins;
+  const int kb = (int)desc->key_bytes, ob = (int)desc->offset_bytes;
+  const BinMap map = make_map(M, desc->hash_kind, desc->d_rmi_model);
+  if (n == 0) return fill_offsets(c, d_offsets, ob, M + 1, 0);
+  const uint32_t W = choose_width(n, M, kb, build_fill());
+  const uint64_t F = (M + W - 1) / W;
+  const uint32_t tile_keys = Level::tile_keys_for(n);
+  const uint64_t ntiles = (n + tile_keys - 1) / tile_keys;
+  const uint64_t nch = (ntiles + kChunk - 1) / kChunk, ncb = (F + 255) / 256;
+  const uint64_t big_cap = std::min<uint64_t>(M, 1ull << 20);
+  // scratch layout (256-byte aligned pieces), checked against the static size
+  size_t off = 0;
+  auto take = [&](uint64_t bytes) {
+    const size_t o = off;
+    off += (size_t)((bytes + 255) & ~255ull);
+    return o;
+  };
+  const size_t o_part = take(n * kb), o_tiles = take(ntiles * sizeof(TileDesc)), o_segs = take(sizeof(SegDesc)),
+               o_items = take(ncb * nch * sizeof(ColItem)), o_items2 = take(ncb * sizeof(ColItem)), o_psb = take(8),
+               o_nch = take(4), o_maps = take(sizeof(BinMap)), o_cmat = take(ntiles * F * 4),
+               o_psum = take(nch * F * 4), o_coltot = take(F * 4), o_bstart = take(F * 4),
+               o_fin = take(F * sizeof(FinItem)), o_bslot = take(F * 4), o_bctl = take(final_big_ctl_bytes()),
+               o_bcnt = take(big_cap * 4);
+  if (F > kFMax || off > kNoAllocBytes)
+    return fail(HG_ERR_UNSUPPORTED, "hg_build_noalloc: the table needs " + std::to_string(off) +
+                                        " bytes of scratch, the static area holds " + std::to_string(kNoAllocBytes) +
+                                        " (up to 2^22 keys of either width and 2^28 bins fit)");
+  SegDesc sd = range_seg(0, n, 0, M, W, map);
+  sd.fin_off_base = 0;
+  sd.fin_key_rel = 0;
+  sd.fin_end_pos = M;
+  sd.table = 0;
+  sd.tile_begin = 0;
+  sd.ntiles = (uint32_t)ntiles;
+  sd.row_base = 0;
+  sd.bucket_base = 0;
+  LevelDev d;
+  d.tiles = (TileDesc *)(base + o_tiles);
+  d.segs = (SegDesc *)(base + o_segs);
+  d.items = (ColItem *)(base + o_items);
+  d.items2 = (ColItem *)(base + o_items2);
+  d.psum_base = (uint64_t *)(base + o_psb);
+  d.nchunks = (uint32_t *)(base + o_nch);
+  d.maps = (BinMap *)(base + o_maps);
+  d.splits = nullptr;
+  d.cmat = (uint32_t *)(base + o_cmat);
+  d.psum = (uint32_t *)(base + o_psum);
+  d.coltot = (uint32_t *)(base + o_coltot);
+  d.bstart = (uint32_t *)(base + o_bstart);
+  d.fin = (FinItem *)(base + o_fin);
+  d.big_slot = (uint32_t *)(base + o_bslot);
+  d.big_ctl = base + o_bctl;
+  d.big_cnt = (uint32_t *)(base + o_bcnt);
+  void *part = base + o_part;
+  HG_HIP(launch_level_setup(sd, map, tile_keys, kChunk, d.tiles, d.segs, d.items, d.items2, d.psum_base, d.nchunks,
+                            d.maps, c->stream));
+  LaunchPart p{};
+  p.key_bytes = kb;
+  p.hk = (int)map.kind;
+  p.sk = 0;
+  p.ntiles = (uint32_t)ntiles;
+  p.max_cols = (uint32_t)F;
+  p.stage_cap = part_stage_cap((uint32_t)F, kb, 0);
+  if (p.stage_cap < 1024) return fail(HG_ERR_UNSUPPORTED, "partition level too wide for LDS");
+  p.keys = d_keys;
+  p.out = part;
+  p.tiles = d.tiles;
+  p.segs = d.segs;
+  p.cmat = d.cmat;
+  HG_HIP(launch_part_hist(p, c->stream));
+  LaunchCol lc{};
+  lc.nitems = (uint32_t)(ncb * nch);
+  lc.nitems2 = (uint32_t)ncb;
+  lc.nsegs = 1;
+  lc.cmat = d.cmat;
+  lc.items = d.items;
+  lc.items2 = d.items2;
+  lc.segs = d.segs;
+  lc.psum_base = d.psum_base;
+  lc.nchunks = d.nchunks;
+  lc.psum = d.psum;
+  lc.coltot = d.coltot;
+  lc.bstart = d.bstart;
+  HG_HIP(launch_colscan(lc, c->stream));
+  HG_HIP(launch_part_scatter(p, c->stream));
+  HG_HIP(launch_make_items(d.segs, 1, d.bstart, d.coltot, d.fin, (uint32_t)F, c->stream));
+  LaunchFinal f{};
+  f.key_bytes = kb;
+  f.off_bytes = ob;
+  f.hk = (int)map.kind;
+  f.nitems = (uint32_t)F;
+  f.lds_bytes = 128 * 1024;
+  f.part = part;
+  f.csr = d_csr_keys;
+  f.off = d_offsets;
+  f.items = d.fin;
+  f.maps = d.maps;
+  f.big_slot = d.big_slot;
+  f.big_ctl = d.big_ctl;
+  f.big_cnt = d.big_cnt;
+  f.big_cap = (uint32_t)big_cap;
+  HG_HIP(launch_final(f, c->stream));
+  return HG_OK;
 }
 
 hg_status hg_table_bins(hg_ctx c, const hg_table_desc *desc, const void *d_keys, uint64_t n, uint32_t *d_bins) {

@@ -1922,6 +1922,60 @@ __global__ __launch_bounds__(1024) void k_sum_add(const unsigned long long *__re
 }
 
 // ---------------------------------------------------------------------------
+// Descriptors of a one-segment partition level written on the device (the
+// no-allocation build uploads nothing from the host): tiles of tile_keys
+// keys, the column-scan items (column blocks of 256, chunks of `chunk`
+// rows) in the order the host's level layout uses, the segment and its map.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_level_setup(SegDesc sd, BinMap map, uint32_t tile_keys, uint32_t chunk,
+                                                     TileDesc *__restrict__ tiles, SegDesc *__restrict__ segs,
+                                                     ColItem *__restrict__ items, ColItem *__restrict__ items2,
+                                                     uint64_t *__restrict__ psum_base, uint32_t *__restrict__ nchunks,
+                                                     BinMap *__restrict__ maps) {
+  const uint32_t ntiles = sd.ntiles;
+  const uint32_t nch = (ntiles + chunk - 1) / chunk;
+  const uint32_t ncb = (sd.ncols + 255) / 256;
+  const uint32_t nitems = ncb * nch;
+  const uint32_t total = ntiles + nitems + ncb + 1;
+  for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < total; i += gridDim.x * 256) {
+    if (i < ntiles) {
+      TileDesc t;
+      t.key_start = sd.key_begin + (uint64_t)i * tile_keys;
+      t.count = (uint32_t)min((uint64_t)tile_keys, sd.key_end - t.key_start);
+      t.seg = 0;
+      tiles[i] = t;
+    } else if (i < ntiles + nitems) {
+      const uint32_t j = i - ntiles, cb = j / nch, ch = j % nch;
+      ColItem x{};
+      x.seg = 0;
+      x.chunk = ch;
+      x.row_lo = ch * chunk;
+      x.row_hi = min(ntiles, (ch + 1) * chunk);
+      x.col_lo = cb * 256;
+      items[j] = x;
+    } else if (i < ntiles + nitems + ncb) {
+      ColItem x{};
+      x.seg = 0;
+      x.col_lo = (i - ntiles - nitems) * 256;
+      items2[i - ntiles - nitems] = x;
+    } else {
+      segs[0] = sd;
+      maps[0] = map;
+      psum_base[0] = 0;
+      nchunks[0] = nch;
+    }
+  }
+}
+
+hipError_t launch_level_setup(const SegDesc &sd, const BinMap &map, uint32_t tile_keys, uint32_t chunk, TileDesc *tiles,
+                              SegDesc *segs, ColItem *items, ColItem *items2, uint64_t *psum_base, uint32_t *nchunks,
+                              BinMap *maps, hipStream_t st) {
+  hipLaunchKernelGGL(k_level_setup, dim3(64), dim3(256), 0, st, sd, map, tile_keys, chunk, tiles, segs, items, items2,
+                     psum_base, nchunks, maps);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // Work items of the final stage / probe from the partition pass results: one
 // item per column (bucket) of every segment.
 // ---------------------------------------------------------------------------

@@ -191,6 +191,10 @@ hipError_t launch_range_hist(const void *keys, uint64_t n, int key_bytes, const 
                              uint32_t nranges, unsigned long long *hist, unsigned int *bad, hipStream_t st);
 hipError_t set_kernel_attributes();
 hipError_t launch_gen_uniform(void *out, uint64_t n, int key_bytes, uint64_t s0, uint64_t range, hipStream_t st);
+// descriptors of a one-segment level (sd.ntiles tiles) written by a kernel
+hipError_t launch_level_setup(const SegDesc &sd, const BinMap &map, uint32_t tile_keys, uint32_t chunk, TileDesc *tiles,
+                              SegDesc *segs, ColItem *items, ColItem *items2, uint64_t *psum_base, uint32_t *nchunks,
+                              BinMap *maps, hipStream_t st);
 hipError_t launch_make_items(const SegDesc *segs, uint32_t nsegs, const uint32_t *bstart, const uint32_t *coltot,
                              FinItem *items, uint32_t ncols_total, hipStream_t st);
 

@@ -98,6 +98,19 @@ hg_status hg_ctx_profile_read(hg_ctx ctx, hg_kernel_time *out, uint32_t capacity
 hg_status hg_build(hg_ctx ctx, const hg_table_desc *desc, const void *d_keys, void *d_offsets,
                    void *d_csr_keys);
 
+/* ---- build without allocation ---------------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under hipGraph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the same result as hg_build, but every temporary lives
+ * in a static device area of 40 MiB that the library holds on each device
+ * from load to process end; the call allocates nothing (no device or pinned
+ * memory, no upload from the host).  Tables needing more scratch than the
+ * area holds are refused with HG_ERR_UNSUPPORTED: up to 2^22 keys of either
+ * width with up to 2^28 bins always fit.  All calls on one device share the
+ * area, so calls that overlap on one device race (order them on one stream). */
+hg_status hg_build_noalloc(hg_ctx ctx, const hg_table_desc *desc, const void *d_keys, void *d_offsets,
+                           void *d_csr_keys);
+
 /* ---- intersection of two built tables -----------------------------------
  * Replaces MultiHashGraph::intersect (src/MultiHashGraph.cu:631-740, kernel
  * simpleIntersect include/MultiHashGraphDeviceOperators.cuh:513-543): for
