@@ -122,6 +122,33 @@ def test_intersect_brute_force(kb):
     assert o.bag_intersect_count(A, B) == bc
 
 
+@pytest.mark.parametrize("kb", [4, 8])
+@pytest.mark.parametrize("threads", [1, 3, 8])
+def test_intersect_par_equals_serial_and_brute_force(kb, threads):
+    """The threaded join of the CPU baseline: same count as the brute force,
+    and the same list, in the same order, as the serial join (bins split
+    across threads, each thread writing at its prefix)."""
+    rng = np.random.default_rng(7 + threads)
+    dt = np.uint32 if kb == 4 else np.uint64
+    A = rng.integers(0, 400, size=3000).astype(dt)
+    B = rng.integers(100, 600, size=2500).astype(dt)
+    M = 101
+    oa, ka = o.build_lrb(A, M, 16000, threads)
+    ob, kb_ = o.build_lrb(B, M, 16000, threads)
+    cnt, lst = o.intersect(oa, ka, ob, kb_)
+    assert o.intersect_par(oa, ka, ob, kb_, threads) == cnt
+    out = np.empty(cnt, dtype=dt)
+    assert o.intersect_par(oa, ka, ob, kb_, threads, out) == cnt
+    assert np.array_equal(out, lst)
+    bc, bl = _brute_intersect(A, B)
+    assert cnt == bc
+    assert np.array_equal(np.sort(out), bl)
+    # a capacity below the count: the count is still exact, the list is cut
+    short = np.zeros(cnt // 2, dtype=dt)
+    assert o.intersect_par(oa, ka, ob, kb_, threads, short) == cnt
+    assert np.array_equal(short, lst[:cnt // 2])
+
+
 def test_intersect_sequential_closed_form():
     # the reference driver's inputs: A = 0..nA-1, B = 0..nB-1 -> {0..min-1}
     nA, nB = 5000, 3000
