@@ -25,7 +25,8 @@ EXPORTS = [
     "hg_version", "hg_ctx_profile", "hg_ctx_profile_read", "hg_build", "hg_intersect", "hg_probe", "hg_intersect_rle", "hg_find_splits",
     "hg_build_partitioned", "hg_rmi_model_bytes", "hg_rmi_train", "hg_batch_build", "hg_batch_probe",
     "hg_exclusive_scan", "hg_hash_bins", "hg_table_bins", "hg_batch_plan_create", "hg_batch_plan_run",
-    "hg_batch_plan_query", "hg_batch_plan_destroy", "hg_build_noalloc",
+    "hg_batch_plan_query", "hg_batch_plan_destroy", "hg_build_noalloc", "hg_build_indexed",
+    "hg_build_partitioned_indexed",
 ]
 
 
@@ -96,6 +97,8 @@ def load(path: str = LIB_PATH):
                                        P(u64)]),
             "hg_find_splits": (i32, [vp, P(TableDesc), vp, u64, u32, P(ctypes.c_int64)]),
             "hg_build_partitioned": (i32, [vp, P(TableDesc), vp, P(ctypes.c_int64), u32, vp, vp, P(u64)]),
+            "hg_build_indexed": (i32, [vp, P(TableDesc), vp, vp, vp, vp]),
+            "hg_build_partitioned_indexed": (i32, [vp, P(TableDesc), vp, P(ctypes.c_int64), u32, vp, vp, vp, P(u64)]),
             "hg_rmi_model_bytes": (ctypes.c_size_t, [u32]),
             "hg_rmi_train": (i32, [vp, vp, u64, u32, u64, u64, u64, u32, vp, P(u32), P(RmiReport)]),
             "hg_batch_build": (i32, [vp, P(BatchDesc), vp, vp, vp]),

@@ -30,7 +30,12 @@ CFLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", f"--off
           "-I", os.path.join(ROOT, "include"), "-I", CSRC]
 
 SOURCES = ["hg_kernels.hip", "hg_rmi.hip", "hg_engine.cpp", "SingleHashGraph.cpp", "MultiHashGraph.cpp"]
-DRIVERS = {"sing-hash": "drivers/sing_hash.cpp", "multi-hash": "drivers/multi_hash.cpp"}
+# driver programs: name -> (source, extra flags).  multi-hash-32 and
+# multi-hash-idx are the reference's B32 and INDEX_TRACK builds of the same
+# driver (include/MultiHashGraph.cuh:62-74 switches them at compile time).
+DRIVERS = {"sing-hash": ("drivers/sing_hash.cpp", []), "multi-hash": ("drivers/multi_hash.cpp", []),
+           "multi-hash-32": ("drivers/multi_hash.cpp", ["-DB32"]),
+           "multi-hash-idx": ("drivers/multi_hash.cpp", ["-DINDEX_TRACK"])}
 
 
 def _run(cmd):
@@ -59,12 +64,12 @@ def build(verbose: bool = False) -> str:
     tmp = LIB + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs)
     os.replace(tmp, LIB)
-    for name, src in DRIVERS.items():
+    for name, (src, extra) in DRIVERS.items():
         path = os.path.join(CSRC, src)
         if not os.path.exists(path):
             continue
-        _run([HIPCC] + CFLAGS + ["-x", "hip", path, "-o", os.path.join(BINDIR, name), "-L", LIBDIR,
-                                 "-lhashgraph_amd", "-Wl,-rpath,$ORIGIN/../lib"])
+        _run([HIPCC] + CFLAGS + extra + ["-x", "hip", path, "-o", os.path.join(BINDIR, name), "-L", LIBDIR,
+                                         "-lhashgraph_amd", "-Wl,-rpath,$ORIGIN/../lib"])
     if verbose:
         print("built", LIB)
     return LIB

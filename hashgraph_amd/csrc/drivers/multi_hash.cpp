@@ -1,5 +1,6 @@
 // multi-hash: the reference's multi-GPU driver (test/MultiHashGraphTest.cu)
-// for one GPU's share:
+// for one GPU's share (built three times: multi-hash with 64-bit keys,
+// multi-hash-32 with -DB32, multi-hash-idx with -DINDEX_TRACK):
 //   multi-hash countA maxkey binCount gpuCount lrbBins check|x countB build|x
 // gpuCount = logical partitions on this GPU.  Inputs are the reference's
 // sequential keys (A = 0..countA-1, B = 0..countB-1, test/MultiHashGraphTest.cu:81-128).
@@ -95,9 +96,10 @@ int main(int argc, char **argv) {
         B.buildSingle();
         std::vector<hkey_t> result;
         for (uint32_t t = 0; t < gpuCount; t++) {
-          std::vector<hkey_t> part(common[t]);
-          if (common[t]) hipMemcpy(part.data(), out[t], common[t] * sizeof(hkey_t), hipMemcpyDeviceToHost);
-          result.insert(result.end(), part.begin(), part.end());
+          std::vector<MultiHashGraph::keypair> part(common[t]);
+          if (common[t])
+            hipMemcpy(part.data(), out[t], common[t] * sizeof(MultiHashGraph::keypair), hipMemcpyDeviceToHost);
+          for (const auto &kp : part) result.push_back((hkey_t)kp.right);
         }
         std::sort(A.h_vals, A.h_vals + countSizeA);
         std::sort(B.h_vals, B.h_vals + countSizeB);

@@ -111,8 +111,9 @@ struct hg_ctx_s {
   // arena 0 holds an operation's large buffers (partitioned copies), arena 1
   // one partition level's descriptors and matrices; arena 1 may be regrown
   // between levels without touching arena 0.
-  char *arena[2] = {nullptr, nullptr};
-  size_t arena_cap[2] = {0, 0};
+  // arena 2 holds the records of the index-carrying builds
+  char *arena[3] = {nullptr, nullptr, nullptr};
+  size_t arena_cap[3] = {0, 0, 0};
   static constexpr int kSlots = 4;
   char *hslot[kSlots] = {nullptr, nullptr, nullptr, nullptr};
   size_t hslot_cap[kSlots] = {0, 0, 0, 0};
@@ -779,6 +780,12 @@ uint64_t ref_binary_search(const std::vector<int64_t> &bins, uint64_t l, uint64_
   }
 }
 
+hg_status check_partitioned(hg_ctx c, const hg_table_desc *desc, const void *d_keys, const int64_t *h_splits,
+                            uint32_t n_parts, void *d_offsets, void *d_csr_keys, uint64_t *h_part_offsets);
+hg_status index_records(hg_ctx c, uint64_t n, uint64_t *&rin, uint64_t *&rout);
+hg_status build_partitioned(hg_ctx c, const hg_table_desc *desc, const void *d_keys, const int64_t *h_splits,
+                            uint32_t n_parts, void *d_offsets, void *d_csr_keys, uint64_t *h_part_offsets);
+
 }  // namespace
 
 // ===========================================================================
@@ -829,7 +836,7 @@ hg_status hg_ctx_destroy(hg_ctx c) {
   if (!c) return HG_OK;
   DeviceGuard g(c->device);
   (void)hipStreamSynchronize(c->stream);
-  for (int a = 0; a < 2; a++)
+  for (int a = 0; a < 3; a++)
     if (c->arena[a]) (void)hipFree(c->arena[a]);
   for (int i = 0; i < hg_ctx_s::kSlots; i++) {
     if (c->hslot[i]) (void)hipHostFree(c->hslot[i]);
@@ -1447,6 +1454,58 @@ hg_status hg_find_splits(hg_ctx c, const hg_table_desc *desc, const void *d_keys
 
 hg_status hg_build_partitioned(hg_ctx c, const hg_table_desc *desc, const void *d_keys, const int64_t *h_splits,
                                uint32_t n_parts, void *d_offsets, void *d_csr_keys, uint64_t *h_part_offsets) {
+  HG_TRY(check_partitioned(c, desc, d_keys, h_splits, n_parts, d_offsets, d_csr_keys, h_part_offsets));
+  DeviceGuard g(c->device);
+  return build_partitioned(c, desc, d_keys, h_splits, n_parts, d_offsets, d_csr_keys, h_part_offsets);
+}
+
+hg_status hg_build_indexed(hg_ctx c, const hg_table_desc *desc, const void *d_keys, void *d_offsets, void *d_csr_keys,
+                           uint64_t *d_csr_index) {
+  if (!c) return fail(HG_ERR_INVALID_ARG, "null ctx");
+  HG_TRY(check_desc(desc));
+  if (!d_offsets || (desc->n_keys && (!d_keys || !d_csr_keys || !d_csr_index)))
+    return fail(HG_ERR_INVALID_ARG, "null buffer");
+  if (desc->hash_kind != HG_HASH_MURMUR3)
+    return fail(HG_ERR_UNSUPPORTED, "index-carrying builds take murmur3 tables (bins of the low 32 key bits)");
+  DeviceGuard g(c->device);
+  const uint64_t n = desc->n_keys;
+  if (n == 0) return fill_offsets(c, d_offsets, (int)desc->offset_bytes, desc->n_bins + 1, 0);
+  uint64_t *rin = nullptr, *rout = nullptr;
+  HG_TRY(index_records(c, n, rin, rout));
+  HG_HIP(launch_pack_index(d_keys, n, (int)desc->key_bytes, rin, c->stream));
+  hg_table_desc d8 = *desc;
+  d8.key_bytes = 8;
+  HG_TRY(build_table(c, &d8, rin, d_offsets, rout));
+  HG_HIP(launch_unpack_index(rout, n, d_keys, (int)desc->key_bytes, d_csr_keys, d_csr_index, c->stream));
+  return HG_OK;
+}
+
+hg_status hg_build_partitioned_indexed(hg_ctx c, const hg_table_desc *desc, const void *d_keys, const int64_t *h_splits,
+                                       uint32_t n_parts, void *d_offsets, void *d_csr_keys, uint64_t *d_csr_index,
+                                       uint64_t *h_part_offsets) {
+  HG_TRY(check_partitioned(c, desc, d_keys, h_splits, n_parts, d_offsets, d_csr_keys, h_part_offsets));
+  if (desc->n_keys && !d_csr_index) return fail(HG_ERR_INVALID_ARG, "null buffer");
+  if (desc->hash_kind != HG_HASH_MURMUR3)
+    return fail(HG_ERR_UNSUPPORTED, "index-carrying builds take murmur3 tables (bins of the low 32 key bits)");
+  DeviceGuard g(c->device);
+  const uint64_t n = desc->n_keys;
+  uint64_t *rin = nullptr, *rout = nullptr;
+  HG_TRY(index_records(c, std::max<uint64_t>(n, 1), rin, rout));
+  HG_HIP(launch_pack_index(d_keys, n, (int)desc->key_bytes, rin, c->stream));
+  hg_table_desc d8 = *desc;
+  d8.key_bytes = 8;
+  HG_TRY(build_partitioned(c, &d8, rin, h_splits, n_parts, d_offsets, rout, h_part_offsets));
+  HG_HIP(launch_unpack_index(rout, h_part_offsets[n_parts], d_keys, (int)desc->key_bytes, d_csr_keys, d_csr_index,
+                             c->stream));
+  return HG_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+hg_status check_partitioned(hg_ctx c, const hg_table_desc *desc, const void *d_keys, const int64_t *h_splits,
+                            uint32_t n_parts, void *d_offsets, void *d_csr_keys, uint64_t *h_part_offsets) {
   if (!c) return fail(HG_ERR_INVALID_ARG, "null ctx");
   HG_TRY(check_desc(desc));
   if (!h_splits || n_parts == 0 || n_parts > 4096 || !d_offsets || !h_part_offsets)
@@ -1457,7 +1516,24 @@ hg_status hg_build_partitioned(hg_ctx c, const hg_table_desc *desc, const void *
     if (h_splits[p + 1] < h_splits[p] || (uint64_t)h_splits[p + 1] > M)
       return fail(HG_ERR_INVALID_ARG, "splits must be non-decreasing within [0, M]");
   if (n && (!d_keys || !d_csr_keys)) return fail(HG_ERR_INVALID_ARG, "null buffer");
-  DeviceGuard g(c->device);
+  return HG_OK;
+}
+
+// The two record arrays (input, CSR) of an index-carrying build, in arena 2.
+hg_status index_records(hg_ctx c, uint64_t n, uint64_t *&rin, uint64_t *&rout) {
+  Plan P;
+  const size_t i_in = P.add(n * 8), i_out = P.add(n * 8);
+  HG_TRY(arena_ensure(c, 2, P.total()));
+  Carve cv;
+  cv.init(c, 2, P);
+  rin = cv.get<uint64_t>(i_in);
+  rout = cv.get<uint64_t>(i_out);
+  return HG_OK;
+}
+
+hg_status build_partitioned(hg_ctx c, const hg_table_desc *desc, const void *d_keys, const int64_t *h_splits,
+                            uint32_t n_parts, void *d_offsets, void *d_csr_keys, uint64_t *h_part_offsets) {
+  const uint64_t M = desc->n_bins, n = desc->n_keys;
   const int kb = (int)desc->key_bytes, ob = (int)desc->offset_bytes;
   const BinMap map = make_map(M, desc->hash_kind, desc->d_rmi_model);
   Plan P0;
@@ -1513,7 +1589,7 @@ hg_status hg_build_partitioned(hg_ctx c, const hg_table_desc *desc, const void *
   return finish_two_level(c, cs, map, kb, ob, part1, part2, d_csr_keys, d_offsets);
 }
 
-}  // extern "C"
+}  // namespace
 
 // ---------------------------------------------------------------------------
 // Batched tables (no counterpart in the reference).  Every table is a range

@@ -269,12 +269,31 @@ __global__ __launch_bounds__(BLOCK) void k_part_scatter(const K *__restrict__ ke
 #pragma unroll
     for (int j = 0; j < KPT; j++)
       kr[j] = buf_load<K>(rs, threadIdx.x * (uint32_t)sizeof(K), j * BLOCK * (uint32_t)sizeof(K));
+    if constexpr (sizeof(K) == 4) {
 #pragma unroll
-    for (int j = 0; j < KPT; j++) {
-      if (j < nv) {
-        const uint32_t b = bucket_of<K, HK, SK>(kr[j], sd, spl);
-        const uint32_t r = atomicAdd(&s_cnt[b], 1u);
-        pk[j] = (b << 16) | r;
+      for (int j = 0; j < KPT; j++) pk[j] = j < (int)nv ? bucket_of<K, HK, SK>(kr[j], sd, spl) : 0u;
+      // ranks in groups of RG atomics issued before their results are used
+      // (one LDS round trip per group instead of one per key)
+      constexpr int RG = 4;
+#pragma unroll
+      for (int g = 0; g < KPT; g += RG) {
+        uint32_t r[RG];
+#pragma unroll
+        for (int u = 0; u < RG; u++) {
+          r[u] = 0;
+          if (g + u < (int)nv) r[u] = atomicAdd(&s_cnt[pk[g + u]], 1u);
+        }
+#pragma unroll
+        for (int u = 0; u < RG; u++) pk[g + u] = (pk[g + u] << 16) | r[u];
+      }
+    } else {  // 64-bit keys fill the registers: one key at a time
+#pragma unroll
+      for (int j = 0; j < KPT; j++) {
+        if (j < (int)nv) {
+          const uint32_t b = bucket_of<K, HK, SK>(kr[j], sd, spl);
+          const uint32_t r = atomicAdd(&s_cnt[b], 1u);
+          pk[j] = (b << 16) | r;
+        }
       }
     }
     __syncthreads();
@@ -301,6 +320,7 @@ __global__ __launch_bounds__(BLOCK) void k_part_scatter(const K *__restrict__ ke
       }
       __syncthreads();
       const uint32_t n = min(stage_cap, cnt - r0);
+      // runs out (the LDS reads of a thread's staged keys are independent)
       for (uint32_t q = threadIdx.x; q < n; q += BLOCK) {
         const uint32_t b = s_bkt[q];
         if (b < nb) out[(uint32_t)(s_adj[b] + r0 + q)] = s_key[q];
@@ -2414,7 +2434,9 @@ template <typename F> static hipError_t dispatch_part(int hk, int sk, F &&f) {
   return dispatch_hk(hk, [&](auto h) { return sk == SK_SPLIT ? f(h, IC<SK_SPLIT>{}) : f(h, IC<SK_RANGE>{}); });
 }
 
-constexpr int PART_KPT = 32;  // tile = 1024 * 32 keys (the engine's tiling)
+// keys per thread of a scatter round (round = 1024 * 32 keys, the engine's
+// tiling unit)
+constexpr int SCATTER_KPT = 32;
 // The histogram walks a tile in rounds of 1024 * 16 keys: fewer registers,
 // so two workgroups share a CU.
 constexpr int HIST_KPT = 16;
@@ -2439,18 +2461,21 @@ static hipError_t launch_part_scatter_t(const LaunchPart &lp, hipStream_t st) {
   const size_t cnt_bytes = (size_t)((lp.max_cols + 3) & ~3u) * 4;
   const size_t lds = 2 * cnt_bytes + (size_t)lp.stage_cap * (sizeof(K) + 2);
   return dispatch_part(lp.hk, lp.sk, [&](auto h, auto b) {
-    hipLaunchKernelGGL((k_part_scatter<K, 1024, PART_KPT, decltype(h)::value, decltype(b)::value>), dim3(lp.ntiles),
-                       dim3(1024), lds, st, (const K *)lp.keys, lp.tiles, lp.segs, lp.cmat, (K *)lp.out, lp.stage_cap);
+    hipLaunchKernelGGL((k_part_scatter<K, 1024, SCATTER_KPT, decltype(h)::value, decltype(b)::value>),
+                       dim3(lp.ntiles), dim3(1024), lds, st, (const K *)lp.keys, lp.tiles, lp.segs, lp.cmat,
+                       (K *)lp.out, lp.stage_cap);
     return hipGetLastError();
   });
 }
 
+// Keys staged per LDS sub-round of the scatter: what fits beside the round
+// counters, the positions, the static scan scratch and (split kinds) the
+// splits in 160 KB of LDS, in multiples of 1024 (0 when fewer than 1024 fit).
 uint32_t part_stage_cap(uint32_t max_cols, int key_bytes, int sk) {
-  // counters, positions, static scan scratch and (split kinds) the splits
   const size_t used = 2 * (size_t)((max_cols + 3) & ~3u) * 4 + 256 + (sk != SK_RANGE ? (MAX_SPLITS + 1) * 8 : 0);
   const size_t avail = used < 160 * 1024 ? 160 * 1024 - used : 0;
   size_t cap = avail / (size_t)(key_bytes + 2) / 1024 * 1024;
-  cap = cap > 32768 ? 32768 : cap;
+  cap = cap > 1024u * SCATTER_KPT ? 1024u * SCATTER_KPT : cap;
   return (uint32_t)cap;
 }
 
@@ -2758,8 +2783,8 @@ hipError_t set_kernel_attributes() {
         constexpr int H = decltype(h)::value, S = decltype(b)::value;
         hipError_t r = set_lds(&k_part_hist<uint32_t, 1024, HIST_KPT, H, S>);
         if (r == hipSuccess) r = set_lds(&k_part_hist<uint64_t, 1024, HIST_KPT, H, S>);
-        if (r == hipSuccess) r = set_lds(&k_part_scatter<uint32_t, 1024, PART_KPT, H, S>);
-        if (r == hipSuccess) r = set_lds(&k_part_scatter<uint64_t, 1024, PART_KPT, H, S>);
+        if (r == hipSuccess) r = set_lds(&k_part_scatter<uint32_t, 1024, SCATTER_KPT, H, S>);
+        if (r == hipSuccess) r = set_lds(&k_part_scatter<uint64_t, 1024, SCATTER_KPT, H, S>);
         return r;
       }));
     }
@@ -2789,6 +2814,47 @@ hipError_t set_kernel_attributes() {
 #undef HG_SET
 #undef HG_SET_D
   return e;
+}
+
+// Records of the index-carrying builds: the key's input position above the
+// low 32 bits of the key (murmur3 bins depend on those bits only), and their
+// unpacking into CSR keys (gathered from the input by position) and positions.
+template <typename K>
+__global__ void k_pack_index(const K *__restrict__ keys, uint64_t n, uint64_t *__restrict__ rec) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    rec[i] = (i << 32) | (uint32_t)keys[i];
+}
+template <typename K>
+__global__ void k_unpack_index(const uint64_t *__restrict__ rec, uint64_t n, const K *__restrict__ keys,
+                               K *__restrict__ csr_keys, uint64_t *__restrict__ csr_index) {
+  for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < n; j += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t i = rec[j] >> 32;
+    csr_index[j] = i;
+    csr_keys[j] = keys[i];
+  }
+}
+
+hipError_t launch_pack_index(const void *keys, uint64_t n, int key_bytes, uint64_t *rec, hipStream_t st) {
+  if (!n) return hipSuccess;
+  const uint32_t grid = (uint32_t)std::min<uint64_t>((n + 255) / 256, 16384);
+  if (key_bytes == 4)
+    hipLaunchKernelGGL((k_pack_index<uint32_t>), dim3(grid), dim3(256), 0, st, (const uint32_t *)keys, n, rec);
+  else
+    hipLaunchKernelGGL((k_pack_index<uint64_t>), dim3(grid), dim3(256), 0, st, (const uint64_t *)keys, n, rec);
+  return hipGetLastError();
+}
+
+hipError_t launch_unpack_index(const uint64_t *rec, uint64_t n, const void *keys, int key_bytes, void *csr_keys,
+                               uint64_t *csr_index, hipStream_t st) {
+  if (!n) return hipSuccess;
+  const uint32_t grid = (uint32_t)std::min<uint64_t>((n + 255) / 256, 16384);
+  if (key_bytes == 4)
+    hipLaunchKernelGGL((k_unpack_index<uint32_t>), dim3(grid), dim3(256), 0, st, rec, n, (const uint32_t *)keys,
+                       (uint32_t *)csr_keys, csr_index);
+  else
+    hipLaunchKernelGGL((k_unpack_index<uint64_t>), dim3(grid), dim3(256), 0, st, rec, n, (const uint64_t *)keys,
+                       (uint64_t *)csr_keys, csr_index);
+  return hipGetLastError();
 }
 
 hipError_t launch_fill_u64(unsigned long long *p, unsigned long long v, uint64_t n, hipStream_t st) {

@@ -185,6 +185,11 @@ hipError_t launch_join(const LaunchJoin &j, hipStream_t st);
 hipError_t launch_scan(const void *in, void *out, uint64_t n, int elem_bytes, void *status, hipStream_t st);
 size_t scan_status_bytes(uint64_t n);
 hipError_t launch_fill_u64(unsigned long long *p, unsigned long long v, uint64_t n, hipStream_t st);
+// index-carrying builds: rec[i] = (i << 32) | low32(key i); unpack gathers
+// csr_keys[j] = keys[rec[j] >> 32] and csr_index[j] = rec[j] >> 32
+hipError_t launch_pack_index(const void *keys, uint64_t n, int key_bytes, uint64_t *rec, hipStream_t st);
+hipError_t launch_unpack_index(const uint64_t *rec, uint64_t n, const void *keys, int key_bytes, void *csr_keys,
+                               uint64_t *csr_index, hipStream_t st);
 hipError_t launch_fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t st);
 hipError_t launch_bins(const void *keys, uint64_t n, int key_bytes, const BinMap &mp, uint32_t *bins, hipStream_t st);
 hipError_t launch_range_hist(const void *keys, uint64_t n, int key_bytes, const BinMap &mp, uint32_t range,

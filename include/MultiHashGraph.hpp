@@ -8,9 +8,15 @@
 // fields (h_binSplits, h_dBinSplits: a second graph may take the first one's
 // splits as in test/MultiHashGraphTest.cu:298-299), h_vals, and the byte
 // offsets into h_dFinalKeys[tid] (h_offsetOff, h_edgesOff) through which
-// partition tid's CSR is read.  Types: hkey_t = int64_t, index_t = int64_t,
-// keys stored without the index payload (keyval_key, 8 bytes: the
-// reference's INDEX_TRACK mode is not built), offsets index_t.
+// partition tid's CSR is read.
+// Types follow the reference's compile-time switches
+// (include/MultiHashGraph.cuh:62-111):
+//   B32          hkey_t = HashKey = uint32_t (else int64_t); index_t = int64_t
+//   INDEX_TRACK  CSR entries are keyval_ind {key, ind} (ind = the key's
+//                position in the concatenated input), else keyval_key {key}
+// The reference's header defines INDEX_TRACK itself; here both switches are
+// opt-in compiler definitions (-DB32, -DINDEX_TRACK), so the default is the
+// 64-bit, index-free form of its headline experiments.  Offsets are index_t.
 // Ownership as in the reference: build() frees the caller's
 // h_dVals[i].d_keys / d_hash; intersect() hipMallocs h_dOutput[tid] (caller
 // frees).  Errors throw hashgraph::error (the reference prints and exits).
@@ -21,17 +27,25 @@
 #include <stdint.h>
 
 #include <mutex>
+#include <type_traits>
 
 #include "SingleHashGraph.hpp"  // hashgraph::error
 #include "hashgraph.h"
 
-class MultiHashGraph {
+namespace hashgraph {
+
+template <typename KeyT, bool IndexTrack>
+class BasicMultiHashGraph {
  public:
-  using hkey_t = int64_t;
+  using hkey_t = KeyT;
   using index_t = int64_t;
-  using HashKey = int64_t;
+  using HashKey = KeyT;
   struct keyval_key {
     hkey_t key;
+  };
+  struct keyval_ind {
+    hkey_t key;
+    index_t ind;
   };
   struct keypair {
     index_t right;
@@ -41,22 +55,24 @@ class MultiHashGraph {
     HashKey *d_hash;
     index_t len;
   };
-  using keyval = keyval_key;
+  using keyval = typename std::conditional<IndexTrack, keyval_ind, keyval_key>::type;
 
-  MultiHashGraph(inputData *h_dVals, index_t countSize, index_t maxkey, index_t tableSize, index_t binCount,
-                 index_t lrbBins, index_t gpuCount);
-  ~MultiHashGraph();
-  MultiHashGraph(const MultiHashGraph &) = delete;
-  MultiHashGraph &operator=(const MultiHashGraph &) = delete;
+  BasicMultiHashGraph(inputData *h_dVals, index_t countSize, index_t maxkey, index_t tableSize, index_t binCount,
+                      index_t lrbBins, index_t gpuCount);
+  ~BasicMultiHashGraph();
+  BasicMultiHashGraph(const BasicMultiHashGraph &) = delete;
+  BasicMultiHashGraph &operator=(const BasicMultiHashGraph &) = delete;
 
   void build(bool findSplits, index_t tid);
   void buildSingle();  // self-check against a plain build; mismatches go to stderr
-  static void intersect(MultiHashGraph &mgA, MultiHashGraph &mgB, index_t *h_Common, keypair **h_dOutput,
+  // output: B's key of every matching pair, as in the reference (keypair.right)
+  static void intersect(BasicMultiHashGraph &mgA, BasicMultiHashGraph &mgB, index_t *h_Common, keypair **h_dOutput,
                         index_t tid);
   void destroyMulti();
 
   // partition tid's CSR: offsets (index_t, R+1 entries, local bins) at
-  // h_dFinalKeys[tid] + h_offsetOff[tid], keys at h_dFinalKeys[tid] + h_edgesOff[tid]
+  // h_dFinalKeys[tid] + h_offsetOff[tid], keyval entries at
+  // h_dFinalKeys[tid] + h_edgesOff[tid]
   char **h_dFinalKeys = nullptr;
   index_t *h_hashOff = nullptr;
   index_t *h_counterOff = nullptr;
@@ -80,11 +96,26 @@ class MultiHashGraph {
   inputData *h_dVals = nullptr;
   index_t maxkey = 0, binCount = 0, lrbBins = 0;
   hg_ctx ctx_ = nullptr;
-  char *blob_ = nullptr;  // [ CSR keys (countSize) | offsets (tableSize + gpuCount) ]
+  char *blob_ = nullptr;  // [ CSR entries (countSize) | offsets (tableSize + gpuCount) ]
   hkey_t *d_keys_ = nullptr;
+  hkey_t *d_csrk_ = nullptr;     // CSR keys alone (the joins read them; == the entries without INDEX_TRACK)
+  uint64_t *d_csri_ = nullptr;   // CSR input positions (INDEX_TRACK)
   index_t *h_ownSplits = nullptr;
   index_t *h_partOff = nullptr;
   std::mutex mu_;
   bool built_ = false;
-  int round_calls_ = 0;
 };
+
+}  // namespace hashgraph
+
+#ifdef B32
+#define HASHGRAPH_KEY_T uint32_t
+#else
+#define HASHGRAPH_KEY_T int64_t
+#endif
+#ifdef INDEX_TRACK
+#define HASHGRAPH_INDEX_TRACK true
+#else
+#define HASHGRAPH_INDEX_TRACK false
+#endif
+using MultiHashGraph = hashgraph::BasicMultiHashGraph<HASHGRAPH_KEY_T, HASHGRAPH_INDEX_TRACK>;

@@ -173,6 +173,21 @@ hg_status hg_find_splits(hg_ctx ctx, const hg_table_desc *desc, const void *d_ke
 hg_status hg_build_partitioned(hg_ctx ctx, const hg_table_desc *desc, const void *d_keys, const int64_t *h_splits,
                                uint32_t n_parts, void *d_offsets, void *d_csr_keys, uint64_t *h_part_offsets);
 
+/* ---- builds that carry each key's input position (INDEX_TRACK) -------------
+ * Replace the reference's INDEX_TRACK mode, whose CSR entries are
+ * keyval_ind {key, ind} (include/MultiHashGraph.cuh:62,82-87,103) with
+ * ind = the key's position in the input (binHashValues,
+ * include/MultiHashGraphDeviceOperators.cuh:297-345).  Same results as
+ * hg_build / hg_build_partitioned, plus d_csr_index (uint64, one per CSR key):
+ * d_csr_keys[j] == d_keys[d_csr_index[j]].  Murmur3 tables only (their bins
+ * depend on the low 32 key bits, which the build carries beside the
+ * position); n_keys < 2^32.                                               */
+hg_status hg_build_indexed(hg_ctx ctx, const hg_table_desc *desc, const void *d_keys, void *d_offsets,
+                           void *d_csr_keys, uint64_t *d_csr_index);
+hg_status hg_build_partitioned_indexed(hg_ctx ctx, const hg_table_desc *desc, const void *d_keys,
+                                       const int64_t *h_splits, uint32_t n_parts, void *d_offsets, void *d_csr_keys,
+                                       uint64_t *d_csr_index, uint64_t *h_part_offsets);
+
 /* ---- learned hash (RMI) ---------------------------------------------------
  * No counterpart in the reference.  A two-stage piecewise-linear CDF model in
  * FP32: a linear root picks one of n_leaves leaves, each leaf's line maps the

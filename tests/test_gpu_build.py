@@ -171,3 +171,31 @@ def test_exclusive_scan_u64_beyond_2e62(dev):
     exp[1:] = np.cumsum(x, dtype=np.uint64)
     assert exp[-1] < exp[n // 2]  # wrapped at least once
     assert np.array_equal(out, exp)
+
+
+@pytest.mark.parametrize("kb", [4, 8])
+def test_build_indexed_carries_positions(kb, dev):
+    """hg_build_indexed (the reference's INDEX_TRACK entries {key, ind}): the
+    offsets equal hg_build's, every CSR key equals the input key at its
+    position, and the positions are a permutation of the input."""
+    import ctypes
+    import torch
+    import hashgraph_amd as hg
+    n, M = 300000, 1 << 17
+    keys = wl.uniform_keys(71, n, kb)
+    keys[:5000] = keys[5000:10000]  # duplicates
+    dk = _to_dev(keys, dev)
+    c = hg.context(0)
+    off = torch.empty(M + 1, dtype=torch.int32, device=dev)
+    csr = torch.empty_like(dk)
+    idx = torch.empty(n, dtype=torch.int64, device=dev)
+    d = hg._desc(n, M, kb)
+    st = c.lib.hg_build_indexed(c.bind_stream(), ctypes.byref(d), hg._p(dk), hg._p(off), hg._p(csr), hg._p(idx))
+    assert st == 0, hg._lib.load().hg_last_error()
+    ref_off, _ = hg.build(dk, M)
+    torch.cuda.synchronize()
+    assert torch.equal(off, ref_off)
+    assert torch.equal(csr, dk[idx])
+    assert torch.equal(torch.sort(idx)[0], torch.arange(n, device=dev))
+    eoff, _ = o.build_csr(keys, M)
+    assert np.array_equal(off.cpu().numpy().astype(np.uint64), eoff)

@@ -44,3 +44,20 @@ def test_multi_hash_intersect_check(gpus, dev):
     assert "intersect check ok" in r.stdout
     assert f"intersect count: {min(nA, nB)}" in r.stdout  # sequential keys: {0..min-1}
     assert "error" not in r.stderr.lower()
+
+
+@pytest.mark.parametrize("exe", ["multi-hash-32", "multi-hash-idx"])
+@pytest.mark.parametrize("gpus", [1, 16])
+def test_multi_hash_b32_and_index_modes(exe, gpus, dev):
+    """The reference's B32 (uint32 keys) and INDEX_TRACK ({key, ind} entries)
+    builds of the driver: buildSingle's checks (degrees, sorted lists and, with
+    INDEX_TRACK, every entry's position pointing at its key) and the CPU
+    intersect answer."""
+    nA, nB = 1 << 20, 1 << 19
+    r = _run([os.path.join(BIN, exe), str(nA), str(nA), "16000", str(gpus), "16000", "check", str(nB), "x"])
+    assert r.returncode == 0
+    assert "intersect check ok" in r.stdout
+    assert f"intersect count: {min(nA, nB)}" in r.stdout
+    assert "error" not in r.stderr.lower()
+    r = _run([os.path.join(BIN, exe), str(nA), str(nA), "16000", str(gpus), "16000", "check", str(nB), "build"])
+    assert r.returncode == 0 and "error" not in r.stderr.lower()
