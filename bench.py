@@ -1,16 +1,20 @@
 #!/usr/bin/env python3
 """HashGraph build + intersect throughput on one MI355X.
 
-One step = build the CSR of A (2^28 uint32 keys, 2^28 bins, murmur3) and
+Headline step = build the CSR of A (2^28 uint32 keys, 2^28 bins, murmur3) and
 intersect B (2^28 uint32 keys, 25% drawn from A) against it with the matched
 keys written out (hg_build + hg_probe).  Inputs are generated on the device
-before the timed region.  Prints one JSON line.
-
-The per-kernel roofline uses HIP events recorded by the library around each
-kernel on its stream (hg_ctx_profile) during the timed steps; the algorithmic
-bytes per launch of each kernel are defined in ALG_BYTES below.  The CPU
-baseline is the oracle's threaded port of the reference's LRB build and bin
-join on a smaller sample of the same workload.
+before the timed region.  Prints one JSON line: the headline figures at the
+top level, then one sub-result per other workload:
+  c2_u64   2^27 + 2^27 uint64 keys, 2^27 bins
+  c3       one GPU's share of MultiHashGraph: 2^30 uint64 Zipf(1.1) keys,
+           16 logical partitions, B = 2^29, run-length join
+  c5       4096 table/probe pairs (2^8..2^20 uint32 keys) in one captured plan
+  learned  the learned hash beside murmur3 on 2^27 skewed uint64 sets
+Each carries a `roofline` (the dominant kernel's algorithmic bytes over its
+average launch time, from HIP events the library records around each kernel
+on its stream, hg_ctx_profile) and a `cpu_baseline` (the oracle's threaded
+port of the reference algorithm on the host cores).
 """
 from __future__ import annotations
 
@@ -24,6 +28,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+METRIC = "Gkeys/s build & intersect at 2^28 u32 keys, %HBM peak; learned/murmur ratio"
+# host threads the CPU baselines use: the GPU box's CPU share for one GPU
+CPU_THREADS = min(16, os.cpu_count() or 1)
 
 CONFIGS = {
     # name: (nA, nB, M, key_bytes, probe fraction)
@@ -47,8 +54,8 @@ def alg_bytes(nA, nB, M, s, K):
     return build, inter
 
 
-def kernel_alg_bytes(name, nA, nB, M, s, K):
-    """Algorithmic bytes per launch of each kernel of a step.  part_hist and
+def c2_kernel_bytes(name, nA, nB, M, s, K):
+    """Algorithmic bytes per launch of each kernel of a C2 step.  part_hist and
     part_scatter run once for A (build) and once for B (probe); nA == nB in
     every configuration here."""
     n = nA
@@ -69,33 +76,208 @@ def load_traffic():
     return {}
 
 
-def cpu_baseline(cfg, threads):
+def roofline_from(kt, bytes_total, traffic=None, steps=1):
+    """Dominant kernel (largest summed time) of a profile {name: (launches,
+    ms)} and its achieved algorithmic bandwidth: bytes_total[name] (all of its
+    launches) over its summed time."""
+    known = {k: v for k, v in kt.items() if bytes_total.get(k)}
+    dom = max(kt.items(), key=lambda kv: kv[1][1])[0]
+    if dom not in known:  # a kernel without algorithmic bytes: the largest one with them
+        dom = max(known.items(), key=lambda kv: kv[1][1])[0] if known else dom
+    launches, ms = kt[dom]
+    b = bytes_total.get(dom)
+    achieved = b / (ms / 1e3) / 1e9 if b else None
+    return {
+        "kernel": dom, "bound": "hbm", "achieved": round(achieved, 1) if achieved else None, "peak": HBM_PEAK_GBS,
+        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None, "traffic": traffic,
+        "alg_bytes_per_launch": int(b / launches) if b else None, "avg_launch_ms": round(ms / launches, 5),
+        "launches_per_step": round(launches / steps, 2),
+    }
+
+
+def kernel_table(kt, steps, step_ms, bytes_total=None):
+    out = {}
+    for name, (launches, tot) in kt.items():
+        row = {"launches_per_step": round(launches / steps, 2), "avg_ms": round(tot / max(launches, 1), 5),
+               "share": round(tot / steps / step_ms, 4)}
+        b = (bytes_total or {}).get(name)
+        if b:
+            row["alg_GBps"] = round(b / (tot / 1e3) / 1e9, 1)
+        out[name] = row
+    return out
+
+
+# ---------------------------------------------------------------------------
+# CPU baselines: the oracle's port of the reference algorithm (LRB build with
+# per-thread coarse histograms and atomic fine counters, bin join) on the
+# host cores, on bounded samples of each workload.
+# ---------------------------------------------------------------------------
+def _cpu_join_c2(A, B, M, threads):
     import numpy as np
-    from hashgraph_amd import workloads as wl
     from oracle import oracle as o
-    nA, nB, M, kb, frac = cfg
-    scale = max(1, nA // (1 << 26))
-    n, m = nA // scale, M // scale
-    A = wl.uniform_keys(1, n, kb)
-    B = wl.probe_keys(A, 2, n, frac, kb)
-    # warm-up on a small slice (page in, thread pool)
-    o.build_lrb(A[: 1 << 20], 1 << 20, 16000, threads)
+    o.build_lrb(A[: 1 << 18], 1 << 18, 16000, threads)  # warm-up: page in, thread pool
     t0 = time.perf_counter()
-    oa, ka = o.build_lrb(A, m, 16000, threads)
-    ob, kbk = o.build_lrb(B, m, 16000, threads)
+    oa, ka = o.build_lrb(A, M, 16000, threads)
+    ob, kbk = o.build_lrb(B, M, 16000, threads)
     cnt = o.intersect_par(oa, ka, ob, kbk, threads)
     out = np.empty(max(cnt, 1), dtype=A.dtype)
     o.intersect_par(oa, ka, ob, kbk, threads, out)
-    t = time.perf_counter() - t0
+    return time.perf_counter() - t0, cnt
+
+
+def cpu_baseline_c2(A_dev, B_dev, M, threads, one_thread_keys):
+    """Full workload at `threads` threads, and a 1-thread figure on the first
+    one_thread_keys keys of each side (M scaled with them)."""
+    import numpy as np
+    kb = A_dev.element_size()
+    dt = np.uint32 if kb == 4 else np.uint64
+    A = A_dev.cpu().numpy().view(dt)
+    B = B_dev.cpu().numpy().view(dt)
+    n = len(A)
+    t, _ = _cpu_join_c2(A, B, M, threads)
+    m1 = max(1, M * one_thread_keys // n)
+    t1, _ = _cpu_join_c2(A[:one_thread_keys], B[:one_thread_keys], m1, 1)
+    w = "uint32" if kb == 4 else "uint64"
     return {
-        "value": round(2 * n / t / 1e9, 4),
-        "unit": "Gkeys/s",
-        "cores": threads,
-        "kind": "port",
-        "sample": f"A,B = 2^{n.bit_length() - 1} keys ({'uint32' if kb == 4 else 'uint64'}, same generators), "
-                  f"2^{m.bit_length() - 1} bins: LRB build of A and of B (16000 coarse bins) + bin join with "
-                  f"emitted list, {threads} threads, {t:.2f} s",
+        "value": round(2 * n / t / 1e9, 4), "unit": "Gkeys/s", "cores": threads, "kind": "port",
+        "sample": f"the full workload (A, B = {n} {w} keys, {M} bins): LRB build of A and of B (16000 coarse bins) + "
+                  f"bin join with emitted list, {threads} threads, {t:.2f} s",
+        "one_thread": {"value": round(2 * one_thread_keys / t1 / 1e9, 4), "unit": "Gkeys/s", "cores": 1,
+                       "sample": f"first {one_thread_keys} keys of A and of B, {m1} bins, 1 thread, {t1:.2f} s"},
     }
+
+
+def cpu_baseline_c3(threads, nA=1 << 24, nB=1 << 23):
+    """C3 sample: Zipf(1.1) uint64 keys (same generator, seeds 3 and 4), LRB
+    builds of A and B over nA bins, canonical bins and the run-length join."""
+    from hashgraph_amd import workloads as wl
+    from oracle import oracle as o
+    A = wl.zipf_keys(3, nA, 1.1)
+    B = wl.zipf_keys(4, nB, 1.1)
+    M = nA
+    t0 = time.perf_counter()
+    oa, ka = o.build_lrb(A, M, 16000, threads)
+    ob, kbk = o.build_lrb(B, M, 16000, threads)
+    o.canon(oa, ka)
+    o.canon(ob, kbk)
+    cnt, rk, _, _ = o.intersect_rle(oa, ka, ob, kbk)
+    t = time.perf_counter() - t0
+    return {"value": round((nA + nB) / t / 1e9, 4), "unit": "Gkeys/s", "cores": threads, "kind": "port",
+            "sample": f"A = {nA}, B = {nB} uint64 Zipf(1.1) keys (the workload's generator, seeds 3 and 4), {M} bins: "
+                      f"LRB builds, per-bin sort and run-length join ({len(rk)} records, count {cnt}), {threads} "
+                      f"threads, {t:.2f} s"}
+
+
+def cpu_baseline_c5(threads, budget_s=10.0, T=4096):
+    """C5 sample: the batch's table/probe pairs in order (same generator),
+    each built and joined on its own (the reference has no batched form),
+    until budget_s seconds of CPU work have passed."""
+    from hashgraph_amd import workloads as wl
+    from oracle import oracle as o
+    sizes = wl.batch_sizes(5, T, 8.0, 20.0)
+    total, nk, done, t_work = 0, 0, 0, 0.0
+    for t in range(T):
+        a, b = wl.batch_pair(5, t, int(sizes[t]), 1.2, 0.3)
+        m = max(len(a), 1)
+        t0 = time.perf_counter()
+        oa, ka = o.build_lrb(a, m, 16000, threads)
+        ob, kbk = o.build_lrb(b, m, 16000, threads)
+        total += o.intersect_par(oa, ka, ob, kbk, threads)
+        t_work += time.perf_counter() - t0
+        nk += len(a) + len(b)
+        done += 1
+        if t_work >= budget_s:
+            break
+    return {"value": round(nk / t_work / 1e9, 4), "unit": "Gkeys/s", "cores": threads, "kind": "port",
+            "sample": f"the first {done} of the {T} pairs, in order ({nk} keys over both sides): per pair LRB builds "
+                      f"of table and probe set + bin-join count ({total} matches; the Zipf tables' hot bins make "
+                      f"the reference's nested join quadratic), {threads} threads, {t_work:.2f} s"}
+
+
+# ---------------------------------------------------------------------------
+# GPU workloads
+# ---------------------------------------------------------------------------
+def run_c2(cfg_name, dev, steps, warmup, cpu=True, one_thread_keys=1 << 25):
+    import torch
+    import hashgraph_amd as hg
+    from hashgraph_amd import workloads as wl
+    nA, nB, M, kb, frac = CONFIGS[cfg_name]
+    s = kb
+    log(f"{cfg_name}: nA={nA} nB={nB} M={M} key_bytes={kb}")
+    A = wl.uniform_keys(1, nA, kb, device=dev)
+    B = wl.probe_keys(A, 2, nB, frac, kb, device=dev)
+    torch.cuda.synchronize()
+    off = torch.empty(M + 1, dtype=torch.int32, device=dev)
+    csr = torch.empty_like(A)
+    hg.build(A, M, out=(off, csr))
+    K, _ = hg.probe(off, csr, M, B, emit=False)
+    matches = torch.empty(max(K, 1), dtype=A.dtype, device=dev)
+    ctx = hg.context(0)
+
+    def step():
+        hg.build(A, M, out=(off, csr))
+        cnt, _ = hg.probe(off, csr, M, B, out=matches)
+        return cnt
+
+    for _ in range(warmup):
+        assert step() == K
+    # timed region: `steps` steps bracketed by synchronisation
+    torch.cuda.synchronize()
+    ctx.profile(True)
+    ctx.profile_read(reset=True)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        step()
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    kt = ctx.profile_read(reset=True)
+    ctx.profile(False)
+    # build-only and probe-only phases (separate loops, for the split figures)
+    e0.record()
+    for _ in range(steps):
+        hg.build(A, M, out=(off, csr))
+    e1.record()
+    torch.cuda.synchronize()
+    ms_build = e0.elapsed_time(e1) / steps
+    e0.record()
+    for _ in range(steps):
+        hg.probe(off, csr, M, B, out=matches)
+    e1.record()
+    torch.cuda.synchronize()
+    ms_probe = e0.elapsed_time(e1) / steps
+    b_bytes, i_bytes = alg_bytes(nA, nB, M, s, K)
+    per_launch = {k: c2_kernel_bytes(k, nA, nB, M, s, K) for k in kt}
+    bytes_total = {k: (v * kt[k][0] if v else None) for k, v in per_launch.items()}
+    traffic = None
+    roof = roofline_from(kt, bytes_total, steps=steps)
+    tr = load_traffic().get(cfg_name, {}).get(roof["kernel"])
+    roof["traffic"] = tr
+    step_gbs = (b_bytes + i_bytes) / (ms / 1e3) / 1e9
+    res = {
+        "value": round((nA + nB) / (ms / 1e3) / 1e9, 3), "unit": "Gkeys/s", "ms_per_step": round(ms, 4),
+        "dtype": "u32" if kb == 4 else "u64", "data": "synthetic (SplitMix64 uniform keys generated on the device)",
+        "config": {
+            "workload": f"build A + intersect B: A = {nA} {'uint32' if kb == 4 else 'uint64'} uniform keys, {M} bins, "
+                        f"murmur3; B = {nB} keys, {int(frac * 100)}% drawn from A; matched keys emitted",
+            "n_a": nA, "n_b": nB, "n_bins": M, "matches": K, "parallelism": "single GPU"},
+        "build_gkeys_s": round(nA / (ms_build / 1e3) / 1e9, 3),
+        "intersect_gkeys_s": round(nB / (ms_probe / 1e3) / 1e9, 3),
+        "ms_build": round(ms_build, 4), "ms_intersect": round(ms_probe, 4),
+        "step_alg_GBps": round(step_gbs, 1), "step_hbm_frac": round(step_gbs / HBM_PEAK_GBS, 4),
+        "roofline": roof, "kernels": kernel_table(kt, steps, ms, bytes_total),
+    }
+    del off, csr, matches, traffic
+    if cpu:
+        log(f"{cfg_name}: cpu baseline ({CPU_THREADS} threads, then 1 thread)")
+        res["cpu_baseline"] = cpu_baseline_c2(A, B, M, CPU_THREADS, min(one_thread_keys, nA))
+    else:
+        res["cpu_baseline"] = None
+    del A, B
+    torch.cuda.empty_cache()
+    return res
 
 
 def bin_stats(off):
@@ -166,13 +348,11 @@ def learned_block(dev, steps, n=1 << 27, frac=0.5):
                 row[tag], lists[tag] = _time_table(A, B, M, hk, mdl, steps, dev)
             except hg.HGError as e:
                 row[tag] = {"error": str(e)}
-        if "learned" not in lists or "murmur" not in lists:
-            res[name] = row
-            continue
-        row["identical_intersection"] = bool(lists["murmur"].numel() == lists["learned"].numel() and
-                                             torch.equal(lists["murmur"], lists["learned"]))
-        row["probe_ratio_learned_over_murmur"] = round(row["learned"]["probe_gkeys_s"] /
-                                                       row["murmur"]["probe_gkeys_s"], 4)
+        if "learned" in lists and "murmur" in lists:
+            row["identical_intersection"] = bool(lists["murmur"].numel() == lists["learned"].numel() and
+                                                 torch.equal(lists["murmur"], lists["learned"]))
+            row["probe_ratio_learned_over_murmur"] = round(row["learned"]["probe_gkeys_s"] /
+                                                           row["murmur"]["probe_gkeys_s"], 4)
         res[name] = row
         del A, B, model, lists
         torch.cuda.empty_cache()
@@ -181,7 +361,21 @@ def learned_block(dev, steps, n=1 << 27, frac=0.5):
             **res}
 
 
-def c3_run(dev, steps, warmup):
+def c3_kernel_bytes(nA, nB, M, P, s):
+    """Algorithmic bytes of a C3 step's kernels, summed over their launches.
+    Each side is partitioned twice (by split range, then by bucket inside each
+    partition), then built; the run-length join reads both sides' keys and
+    offsets in each of its two passes."""
+    n2 = 2 * (nA + nB)
+    return {
+        "part_hist": s * n2,
+        "part_scatter": 2 * s * n2,
+        "final": 2 * s * (nA + nB) + 4 * 2 * (M + P),
+        "rle_small": 2 * (s * (nA + nB) + 8 * (M + P)),
+    }
+
+
+def run_c3(dev, steps, warmup, cpu=True):
     """One GPU's share of MultiHashGraph: 2^30 uint64 Zipf(1.1) keys over 2^32
     ranks, 2^30 bins, 16 logical partitions (key-balanced splits over a
     16000-range histogram), per-partition CSR; B = 2^29 Zipf keys built with
@@ -195,6 +389,7 @@ def c3_run(dev, steps, warmup):
     A = wl.zipf_keys(3, nA, 1.1, device=dev)
     B = wl.zipf_keys(4, nB, 1.1, device=dev)
     torch.cuda.synchronize()
+    ctx = hg.context(0)
 
     def build_side(keys, splits=None):
         sp = hg.find_splits(keys, M, 16000, P) if splits is None else splits
@@ -230,6 +425,8 @@ def c3_run(dev, steps, warmup):
         del ta, tb
     torch.cuda.synchronize()
     tb_ms, ti_ms = 0.0, 0.0
+    ctx.profile(True)
+    ctx.profile_read(reset=True)
     for _ in range(steps):
         e0.record()
         ta = build_side(A)
@@ -243,19 +440,46 @@ def c3_run(dev, steps, warmup):
         torch.cuda.synchronize()
         ti_ms += e0.elapsed_time(e1)
         del ta, tb
+    kt = ctx.profile_read(reset=True)
+    ctx.profile(False)
     tb_ms /= steps
     ti_ms /= steps
-    return {"nA": nA, "nB": nB, "M": M, "P": P, "ms_build": tb_ms, "ms_intersect": ti_ms, "count": cnt,
-            "records": recs}
+    ms = tb_ms + ti_ms
+    s = 8
+    build_bytes = 4 * s * nA + 4 * (M + 1)
+    inter_bytes = 3 * s * nB + s * nA + 4 * (M + 1)  # + the run-length records
+    bt = c3_kernel_bytes(nA, nB, M, P, s)
+    res = {
+        "value": round((nA + nB) / (ms / 1e3) / 1e9, 3), "unit": "Gkeys/s", "ms_per_step": round(ms, 3),
+        "dtype": "u64", "data": "synthetic (Zipf(1.1) by rejection-inversion on the device)",
+        "config": {"workload": "one GPU's share of MultiHashGraph: A = 2^30 uint64 Zipf(1.1) keys over 2^32 ranks, "
+                               "2^30 bins, 16 logical partitions; B = 2^29 Zipf keys with A's splits; run-length join "
+                               "per partition", "parallelism": "single GPU"},
+        "build_gkeys_s": round(nA / tb_ms / 1e6, 3), "intersect_gkeys_s": round(nB / ti_ms / 1e6, 3),
+        "ms_build": round(tb_ms, 3), "ms_intersect": round(ti_ms, 3),
+        "build_hbm_frac": round(build_bytes / (tb_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+        "intersect_hbm_frac": round(inter_bytes / (ti_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+        "count": cnt, "rle_records": recs,
+        "roofline": roofline_from(kt, bt, steps=steps), "kernels": kernel_table(kt, steps, ms, bt),
+    }
+    del A, B
+    torch.cuda.empty_cache()
+    if cpu:
+        log(f"c3: cpu baseline ({CPU_THREADS} threads)")
+        res["cpu_baseline"] = cpu_baseline_c3(CPU_THREADS)
+    else:
+        res["cpu_baseline"] = None
+    return res
 
 
-def c5_run(dev, steps, warmup, T=4096):
+def run_c5(dev, steps, warmup, T=4096, cpu=True):
     """Batched form: T table/probe pairs, table sizes round(2^U[8,20]) uint32
     keys (even tables uniform, odd tables Zipf(1.2) ranks), probe sets of
     equal size with 30% drawn from their table, M_t = n_t; the learned hash
     per table where its sampled check wins.  The batch is one captured,
     autotuned plan (build + per-table probe counts); it is timed against T
     separate single-table build + probe calls."""
+    import numpy as np
     import torch
     import hashgraph_amd as hg
     from hashgraph_amd import workloads as wl
@@ -294,6 +518,21 @@ def c5_run(dev, steps, warmup, T=4096):
     torch.cuda.synchronize()
     ms_batch = e0.elapsed_time(e1) / steps
     total = int(plan.counts.sum().item())
+    info = plan.info()
+    plan.close()
+    # kernel breakdown of the same work, eager (the plan replays one graph)
+    ctx = hg.context(0)
+    off, csr, off_off = hg.batch_build(keys, ko, n_bins, hash_kinds=kinds, rmi_models=models, rmi_leaves=L)
+    hg.batch_probe(off, csr, ko, n_bins, off_off, probe, po, hash_kinds=kinds, rmi_models=models, rmi_leaves=L)
+    torch.cuda.synchronize()
+    ctx.profile(True)
+    ctx.profile_read(reset=True)
+    off, csr, off_off = hg.batch_build(keys, ko, n_bins, hash_kinds=kinds, rmi_models=models, rmi_leaves=L)
+    tot_e, _ = hg.batch_probe(off, csr, ko, n_bins, off_off, probe, po, hash_kinds=kinds, rmi_models=models,
+                              rmi_leaves=L)
+    kt = ctx.profile_read(reset=True)
+    ctx.profile(False)
+    del off, csr
     # the same work as separate single-table calls
     t_each = []
     cnt_each = 0
@@ -312,11 +551,42 @@ def c5_run(dev, steps, warmup, T=4096):
         cnt_each = c
     ms_each = min(t_each) * 1e3
     nk = int(ko[T]) + int(po[T])
-    info = plan.info()
-    plan.close()
-    return {"T": T, "keys": int(ko[T]), "probe_keys": int(po[T]), "ms_batch": ms_batch, "ms_separate": ms_each,
-            "count": total, "count_separate": cnt_each, "learned_tables": sum(k == 1 for k in kinds),
-            "plan": info, "train_s": t_train, "plan_s": t_plan, "nk": nk}
+    NA, NB, SM = int(ko[T]), int(po[T]), int(np.sum(n_bins))
+    s = 4
+    # small-table bytes (every table's keys + CSR fit 64 MiB): build reads the
+    # keys and writes CSR keys and offsets; the probe reads the probe keys,
+    # the table's keys and offsets, and writes one count per table
+    step_bytes = (2 * s * NA + 4 * (SM + T)) + (s * NB + s * NA + 4 * (SM + T) + 8 * T)
+    ach = step_bytes / (ms_batch / 1e3) / 1e9
+    bt = {"final": 2 * s * NA + 4 * (SM + T), "probe": s * NB + s * NA + 4 * (SM + T) + 8 * T,
+          "part_hist": s * (NA + NB), "part_scatter": 2 * s * (NA + NB)}
+    ms_eager = sum(v[1] for v in kt.values())
+    res = {
+        "value": round(nk / (ms_batch / 1e3) / 1e9, 3), "unit": "Gkeys/s", "ms_per_step": round(ms_batch, 3),
+        "dtype": "u32", "data": "synthetic (SplitMix64 / Zipf(1.2) on the device)",
+        "config": {"workload": f"{T} table/probe pairs, sizes round(2^U[8,20]) uint32 keys, half uniform, half "
+                               f"Zipf(1.2); probe sets of equal size, 30% drawn from their table; bins = table size; "
+                               f"one captured autotuned plan: build + per-table probe counts",
+                   "parallelism": "single GPU"},
+        "keys": NA, "probe_keys": NB, "count": total, "count_eager": tot_e, "count_separate_calls": cnt_each,
+        "learned_tables": sum(k == 1 for k in kinds), "plan": info, "ms_separate_calls": round(ms_each, 3),
+        "speedup_vs_separate_calls": round(ms_each / ms_batch, 2),
+        "separate_calls_gkeys_s": round(nk / (ms_each / 1e3) / 1e9, 3),
+        "train_s": round(t_train, 2), "plan_build_s": round(t_plan, 2),
+        "roofline": {"kernel": "plan replay (one graph: every kernel of the batch)", "bound": "hbm",
+                     "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "alg_bytes_per_launch": step_bytes,
+                     "avg_launch_ms": round(ms_batch, 4)},
+        "kernels_eager": kernel_table(kt, 1, ms_eager, bt),
+    }
+    del keys, probe, models
+    torch.cuda.empty_cache()
+    if cpu:
+        log(f"c5: cpu baseline ({CPU_THREADS} threads)")
+        res["cpu_baseline"] = cpu_baseline_c5(CPU_THREADS)
+    else:
+        res["cpu_baseline"] = None
+    return res
 
 
 def main():
@@ -324,10 +594,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="c2-u32", choices=sorted(CONFIGS) + ["c3", "c5"])
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--no-learned", action="store_true", help="skip the learned-hash comparison")
+    ap.add_argument("--config", default="all", choices=["all"] + sorted(CONFIGS) + ["c3", "c5", "learned"],
+                    help="all: the C2 u32 headline plus every other workload as sub-results")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run device copies of MIB MiB (calibrates the HBM byte counters)")
     args = ap.parse_args()
@@ -337,8 +606,6 @@ def main():
         sys.exit(2)
 
     import torch
-    import hashgraph_amd as hg
-    from hashgraph_amd import workloads as wl
 
     assert torch.cuda.is_available(), "bench.py needs a ROCm GPU"
     dev = torch.device("cuda", 0)
@@ -351,165 +618,30 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_read": n, "copy_bytes_written": n, "launches": max(args.steps, 1)}))
         return
-    if args.config == "c3":
-        r = c3_run(dev, args.steps, args.warmup)
-        ms = r["ms_build"] + r["ms_intersect"]
-        build_bytes = 36 * r["nA"]  # 4 s N + 4 (M + 1) with s = 8, N = M
-        inter_bytes = 24 * r["nB"] + 8 * r["nA"] + 4 * r["M"]
-        out = {
-            "metric": "Gkeys/s build & intersect at 2^28 u32 keys, %HBM peak; learned/murmur ratio",
-            "value": round((r["nA"] + r["nB"]) / (ms / 1e3) / 1e9, 3), "unit": "Gkeys/s", "n_gpus": 1,
-            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True,
-            "scaling": "weak", "vs_baseline": None, "dtype": "u64", "data": "synthetic (Zipf(1.1) on the device)",
-            "config": {"workload": "one GPU's share of MultiHashGraph: A = 2^30 uint64 Zipf(1.1) keys over 2^32 "
-                                   "ranks, 2^30 bins, 16 logical partitions; B = 2^29 Zipf keys with A's splits; "
-                                   "run-length join per partition", "parallelism": "single GPU"},
-            "build_gkeys_s": round(r["nA"] / r["ms_build"] / 1e6, 3),
-            "intersect_gkeys_s": round(r["nB"] / r["ms_intersect"] / 1e6, 3),
-            "ms_build": round(r["ms_build"], 3), "ms_intersect": round(r["ms_intersect"], 3),
-            "build_hbm_frac": round(build_bytes / (r["ms_build"] / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
-            "intersect_hbm_frac": round(inter_bytes / (r["ms_intersect"] / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
-            "count": r["count"], "rle_records": r["records"], "roofline": None, "cpu_baseline": None,
-        }
-        print(json.dumps(out))
+    cpu = not args.no_cpu
+    common = {"metric": METRIC, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": True,
+              "scaling": "weak", "vs_baseline": None}
+    if args.config in ("c3", "c5", "learned") or args.config in CONFIGS and args.config != "c2-u32":
+        if args.config == "c3":
+            r = run_c3(dev, args.steps, args.warmup, cpu)
+        elif args.config == "c5":
+            r = run_c5(dev, args.steps, args.warmup, cpu=cpu)
+        elif args.config == "learned":
+            r = {"learned": learned_block(dev, max(2, min(args.steps, 5)))}
+        else:
+            r = run_c2(args.config, dev, args.steps, args.warmup, cpu)
+        print(json.dumps({**common, **r}))
         return
-    if args.config == "c5":
-        r = c5_run(dev, args.steps, args.warmup)
-        out = {
-            "metric": "Gkeys/s build & intersect at 2^28 u32 keys, %HBM peak; learned/murmur ratio",
-            "value": round(r["nk"] / (r["ms_batch"] / 1e3) / 1e9, 3), "unit": "Gkeys/s", "n_gpus": 1,
-            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(r["ms_batch"], 3),
-            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32",
-            "data": "synthetic (SplitMix64 / Zipf(1.2) on the device)",
-            "config": {"workload": f"{r['T']} table/probe pairs, sizes round(2^U[8,20]) uint32 keys, half uniform, "
-                                   f"half Zipf(1.2); probe sets of equal size, 30% drawn from their table; bins = "
-                                   f"table size; one captured autotuned plan: build + per-table probe counts",
-                       "parallelism": "single GPU"},
-            "keys": r["keys"], "probe_keys": r["probe_keys"], "count": r["count"],
-            "count_separate_calls": r["count_separate"], "learned_tables": r["learned_tables"],
-            "plan": r["plan"], "ms_separate_calls": round(r["ms_separate"], 3),
-            "speedup_vs_separate_calls": round(r["ms_separate"] / r["ms_batch"], 2),
-            "separate_calls_gkeys_s": round(r["nk"] / (r["ms_separate"] / 1e3) / 1e9, 3),
-            "train_s": round(r["train_s"], 2), "plan_build_s": round(r["plan_s"], 2),
-            "roofline": None, "cpu_baseline": None,
-        }
-        print(json.dumps(out))
-        return
-    nA, nB, M, kb, frac = CONFIGS[args.config]
-    s = kb
-    log(f"config {args.config}: nA={nA} nB={nB} M={M} key_bytes={kb}")
-    A = wl.uniform_keys(1, nA, kb, device=dev)
-    B = wl.probe_keys(A, 2, nB, frac, kb, device=dev)
-    torch.cuda.synchronize()
-    off = torch.empty(M + 1, dtype=torch.int32, device=dev)
-    csr = torch.empty_like(A)
-    hg.build(A, M, out=(off, csr))
-    K, _ = hg.probe(off, csr, M, B, emit=False)
-    matches = torch.empty(max(K, 1), dtype=A.dtype, device=dev)
-    ctx = hg.context(0)
-
-    def step():
-        hg.build(A, M, out=(off, csr))
-        cnt, _ = hg.probe(off, csr, M, B, out=matches)
-        return cnt
-
-    for _ in range(args.warmup):
-        cnt = step()
-        assert cnt == K
-    # timed region: K steps bracketed by synchronisation
-    torch.cuda.synchronize()
-    ctx.profile(True)
-    ctx.profile_read(reset=True)
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    e0.record()
-    for _ in range(args.steps):
-        step()
-    e1.record()
-    torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1) / args.steps
-    kt = ctx.profile_read(reset=True)
-    ctx.profile(False)
-    # build-only and probe-only phases (separate loops, for the split figures)
-    torch.cuda.synchronize()
-    e0.record()
-    for _ in range(args.steps):
-        hg.build(A, M, out=(off, csr))
-    e1.record()
-    torch.cuda.synchronize()
-    ms_build = e0.elapsed_time(e1) / args.steps
-    e0.record()
-    for _ in range(args.steps):
-        hg.probe(off, csr, M, B, out=matches)
-    e1.record()
-    torch.cuda.synchronize()
-    ms_probe = e0.elapsed_time(e1) / args.steps
-
-    b_bytes, i_bytes = alg_bytes(nA, nB, M, s, K)
-    per_kernel = {}
-    for name, (launches, tot) in kt.items():
-        avg = tot / max(launches, 1)
-        kb_ = kernel_alg_bytes(name, nA, nB, M, s, K)
-        per_kernel[name] = {"launches_per_step": launches / args.steps, "avg_ms": round(avg, 5),
-                            "share": round(tot / args.steps / ms, 4)}
-        if kb_:
-            per_kernel[name]["alg_GBps"] = round(kb_ / (avg / 1e3) / 1e9, 1)
-    dom = max(kt.items(), key=lambda kv: kv[1][1])[0]
-    traffic = load_traffic().get(args.config, {}).get(dom)
-    dom_bytes = kernel_alg_bytes(dom, nA, nB, M, s, K)
-    dom_avg = kt[dom][1] / kt[dom][0]
-    achieved = dom_bytes / (dom_avg / 1e3) / 1e9 if dom_bytes else None
-    roofline = {
-        "kernel": dom,
-        "bound": "hbm",
-        "achieved": round(achieved, 1) if achieved else None,
-        "peak": HBM_PEAK_GBS,
-        "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
-        "traffic": traffic,
-        "alg_bytes_per_launch": dom_bytes,
-    }
-    step_gbs = (b_bytes + i_bytes) / (ms / 1e3) / 1e9
-    res = {
-        "metric": "Gkeys/s build & intersect at 2^28 u32 keys, %HBM peak; learned/murmur ratio",
-        "value": round((nA + nB) / (ms / 1e3) / 1e9, 3),
-        "unit": "Gkeys/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(ms, 4),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "u32" if kb == 4 else "u64",
-        "data": "synthetic (SplitMix64 uniform keys generated on the device)",
-        "config": {
-            "workload": f"build A + intersect B: A = {nA} {'uint32' if kb == 4 else 'uint64'} uniform keys, "
-                        f"{M} bins, murmur3; B = {nB} keys, {int(frac * 100)}% drawn from A; matched keys emitted",
-            "n_a": nA, "n_b": nB, "n_bins": M, "matches": K, "parallelism": "single GPU",
-        },
-        "build_gkeys_s": round(nA / (ms_build / 1e3) / 1e9, 3),
-        "intersect_gkeys_s": round(nB / (ms_probe / 1e3) / 1e9, 3),
-        "ms_build": round(ms_build, 4),
-        "ms_intersect": round(ms_probe, 4),
-        "step_alg_GBps": round(step_gbs, 1),
-        "step_hbm_frac": round(step_gbs / HBM_PEAK_GBS, 4),
-        "roofline": roofline,
-        "kernels": per_kernel,
-    }
-    del A, B, off, csr, matches
-    torch.cuda.empty_cache()
-    if not args.no_learned and args.config.startswith("c2"):
+    res = {**common, **run_c2("c2-u32", dev, args.steps, args.warmup, cpu)}
+    if args.config == "all":
+        sub_steps, sub_warm = max(1, min(args.steps, 5)), max(1, min(args.warmup, 2))
+        res["c2_u64"] = run_c2("c2-u64", dev, args.steps, args.warmup, cpu, one_thread_keys=1 << 24)
         log("learned-hash comparison")
         res["learned"] = learned_block(dev, max(2, min(args.steps, 5)))
         ratios = [res["learned"][k].get("probe_ratio_learned_over_murmur") for k in ("lognormal", "clusters")]
         res["learned_murmur_probe_ratio"] = {k: r for k, r in zip(("lognormal", "clusters"), ratios)}
-    if not args.no_cpu:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        log(f"cpu baseline ({threads} threads)")
-        res["cpu_baseline"] = cpu_baseline(CONFIGS[args.config], threads)
-    else:
-        res["cpu_baseline"] = None
+        res["c3"] = run_c3(dev, sub_steps, sub_warm, cpu)
+        res["c5"] = run_c5(dev, sub_steps, sub_warm, cpu=cpu)
     print(json.dumps(res))
 
 

@@ -344,6 +344,20 @@ def batch_sizes(seed: int, T: int, lo_bits: float = 8.0, hi_bits: float = 20.0):
     return np.rint(np.exp2(lo_bits + u * (hi_bits - lo_bits))).astype(np.int64)
 
 
+def batch_pair(seed: int, t: int, n: int, zipf_s: float = 1.2, frac: float = 0.3, device=None):
+    """Table t of a batch (n keys: uniform for even t, Zipf ranks' low 32 bits
+    for odd t) and its probe set (n keys, frac drawn from the table)."""
+    base = seed * 4096 + 2 * t + 1
+    if t % 2 == 0:
+        a = uniform_keys(base, n, 4, device)
+    elif device is None:
+        a = (zipf_keys(base + 1, n, zipf_s) & np.uint64(0xFFFFFFFF)).astype(np.uint32)
+    else:
+        z = zipf_keys(base + 1, n, zipf_s, device=device)
+        a = (z - ((z >> 31) << 32)).to(__import__("torch").int32)  # low 32 bits
+    return a, (probe_keys(a, base + 8192 * 4096, n, frac, 4, device) if n else a)
+
+
 def batch_tables(seed: int, T: int, lo_bits: float = 8.0, hi_bits: float = 20.0, zipf_s: float = 1.2,
                  frac: float = 0.3, device=None):
     """Keys and probe sets of a batch: (keys, key_off, probe, probe_off); numpy
@@ -351,16 +365,9 @@ def batch_tables(seed: int, T: int, lo_bits: float = 8.0, hi_bits: float = 20.0,
     sizes = batch_sizes(seed, T, lo_bits, hi_bits)
     keys, probes = [], []
     for t, n in enumerate(sizes.tolist()):
-        base = seed * 4096 + 2 * t + 1
-        if t % 2 == 0:
-            a = uniform_keys(base, n, 4, device)
-        elif device is None:
-            a = (zipf_keys(base + 1, n, zipf_s) & np.uint64(0xFFFFFFFF)).astype(np.uint32)
-        else:
-            z = zipf_keys(base + 1, n, zipf_s, device=device)
-            a = (z - ((z >> 31) << 32)).to(__import__("torch").int32)  # low 32 bits
+        a, b = batch_pair(seed, t, n, zipf_s, frac, device)
         keys.append(a)
-        probes.append(probe_keys(a, base + 8192 * 4096, n, frac, 4, device) if n else a)
+        probes.append(b)
     ko = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
     if device is None:
         return np.concatenate(keys), ko, np.concatenate(probes), ko.copy()
