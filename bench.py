@@ -316,9 +316,17 @@ def _time_table(A, B, M, hk, mdl, steps, dev):
     e1.record()
     torch.cuda.synchronize()
     ms_p = e0.elapsed_time(e1) / steps
+    # the probe's phases (partition of B, then the bucket kernels)
+    ctx = hg.context(dev.index if dev.index is not None else 0)
+    ctx.profile(True)
+    ctx.profile_read(reset=True)
+    hg.probe(off, csr, M, B, out=out, hash_kind=hk, rmi_model=mdl)
+    kt = ctx.profile_read(reset=True)
+    ctx.profile(False)
     st = bin_stats(off)
     st.update({"build_gkeys_s": round(n / ms_b / 1e6, 3), "probe_gkeys_s": round(B.numel() / ms_p / 1e6, 3),
-               "ms_build": round(ms_b, 4), "ms_probe": round(ms_p, 4), "matches": K})
+               "ms_build": round(ms_b, 4), "ms_probe": round(ms_p, 4), "matches": K,
+               "probe_phase_ms": {k: round(v[1], 4) for k, v in kt.items()}})
     return st, torch.sort(out[:K])[0]
 
 
@@ -349,6 +357,17 @@ def learned_block(dev, steps, n=1 << 27, frac=0.5):
                                                  torch.equal(lists["murmur"], lists["learned"]))
             row["probe_ratio_learned_over_murmur"] = round(row["learned"]["probe_gkeys_s"] /
                                                            row["murmur"]["probe_gkeys_s"], 4)
+            # what any hash could gain here: only the bucket kernels walk bins;
+            # even if all of their time scaled with the probe length and the
+            # learned evaluation cost nothing, murmur's probe would shrink by
+            # t_bucket * (1 - L_learned / L_murmur)
+            mu, le = row["murmur"], row["learned"]
+            t_bucket = mu["probe_phase_ms"].get("probe", 0.0)
+            saving = max(0.0, 1.0 - le["mean_probe_len"] / mu["mean_probe_len"])
+            row["probe_ratio_upper_bound"] = {
+                "value": round(mu["ms_probe"] / (mu["ms_probe"] - t_bucket * saving), 4),
+                "how": "murmur probe ms / (murmur probe ms - murmur bucket-kernel ms * (1 - learned/murmur mean "
+                       "probe length)): the partition of B does not depend on the probe length"}
         else:
             # the sampled check kept murmur3: the learned form's tables are murmur's
             row["learned"] = {"fallback": "murmur3 (the model lost the sampled check); figures as murmur"}

@@ -110,8 +110,10 @@ struct OvfChunk {
 //             the bits of f(key - key_min) + 1 read as an unsigned integer
 //             and converted to f32 (root_kind 1: a piecewise-linear log)
 //   leaf    = clamp(floor(root_slope * feature + root_icpt), 0, L-1)
-//   bin     = clamp(anchor[leaf] +- floor(slope[leaf] * f(|key - x0[leaf]|)),
-//                   0, M-1)        (+ for key >= x0, - below it)
+//   line    = anchor[leaf] +- floor(slope[leaf] * f(|key - x0[leaf]|))
+//                                  (+ for key >= x0, - below it)
+//   bin     = line when 0 <= line <= M-1, else murmur3(key) mod M (keys
+//             beyond the trained range are hashed, not piled into an edge bin)
 // f() is the round-to-nearest u64 -> f32 conversion, every product and sum is
 // rounded separately (no fused multiply-add), floor saturates at 2^32-256.
 // x0 (the leaf's first training key) keeps FP32 precision where it is

@@ -138,18 +138,20 @@ __device__ __forceinline__ uint32_t rmi_leaf(uint64_t key, const RmiHeader &h) {
   return rmi_leaf_at(rmi_feature(key, h.key_min, h.root_kind), h.root_slope, h.root_icpt, h.n_leaves);
 }
 
-// Bin of a key routed to leaf l (M bins).
+// Bin of a key routed to leaf l (M bins).  A key whose line value lies
+// outside [0, M-1] (beyond the range the model was trained on) takes
+// murmur3's bin instead of piling up in an edge bin.
 __device__ __forceinline__ uint32_t rmi_leaf_bin(uint64_t key, const RmiLeaf &l, uint32_t M) {
 #pragma clang fp contract(off)
   const bool up = key >= l.x0;
   const float d = up ? (float)(key - l.x0) : (float)(l.x0 - key);
   const float y = __fmul_rn(l.slope, d);
   // offset: 0 for NaN and y <= 0, trunc(y) up to 0xFFFFFF00 (the largest
-  // float below 2^32); the bin anchor +- offset clamped to [0, M-1] with
-  // saturating 32-bit arithmetic
+  // float below 2^32); the line value anchor +- offset
   const uint32_t off = min(cvt_u32_sat(y), 0xFFFFFF00u);
-  const uint32_t b = up ? __builtin_elementwise_add_sat(l.anchor, off) : __builtin_elementwise_sub_sat(l.anchor, off);
-  return min(b, M - 1);
+  const int64_t b = up ? (int64_t)l.anchor + off : (int64_t)l.anchor - off;
+  if (b >= 0 && b < (int64_t)M) return (uint32_t)b;
+  return murmur4((uint32_t)key) % M;
 }
 
 __device__ __forceinline__ uint32_t rmi_bin(uint64_t key, const void *model, uint32_t M) {

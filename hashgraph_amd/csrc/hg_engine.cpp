@@ -1232,6 +1232,8 @@ hg_status hg_probe(hg_ctx c, const hg_table_desc *descA, const void *d_offA, con
   pr.cursor = small;
   pr.gate = (unsigned int *)(small + 2);
   pr.hmark = hmark;
+  pr.big_slot = d.big_slot;
+  pr.big_ctl = d.big_ctl;
   {
     Prof pf(c, "probe");
     HG_HIP(launch_probe(pr, c->stream));
@@ -1709,6 +1711,8 @@ hg_status batch_launch_probe(hg_ctx c, const hg_batch_desc *bd, BatchSide &S, co
   pr.cursor = (unsigned long long *)d_counts;
   pr.gate = S.flag + 1;
   pr.hmark = S.hmark;
+  pr.big_slot = S.d.big_slot;
+  pr.big_ctl = S.d.big_ctl;
   Prof pf(c, "probe");
   HG_HIP(launch_probe(pr, c->stream));
   return HG_OK;

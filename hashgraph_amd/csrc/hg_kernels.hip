@@ -1379,32 +1379,52 @@ __device__ void vt_answer(const K *s_key, const uint32_t *s_cnt, uint32_t ecnt, 
   }
 }
 
-// Folds keys[a0..a1) into the table in chunks of BLOCK keys, answering and
-// clearing it first whenever it is more than half full.  Must start after a
-// barrier; ends with one.  pending: values folded since the last answer.
+// Folds the sequence key_at(0..n) into the table.  The keys are loaded
+// FOLD_U * BLOCK at a time (all loads in flight together) and inserted
+// BLOCK at a time; before each insert step a table more than half full is
+// answered and cleared, so a step's inserts (at most BLOCK new values) always
+// find a free slot.  Must start after a barrier; ends with one.  pending:
+// values folded since the last answer.
+constexpr int FOLD_U = 4;
+template <typename K, int BLOCK, typename AT, typename ANSWER>
+__device__ __forceinline__ void vt_fold_seq(uint64_t n, AT &&key_at, K *s_key, uint32_t *s_cnt, uint32_t *s_fill,
+                                            uint32_t *s_ecnt, ANSWER &&answer, bool &pending) {
+  constexpr K EMPTY = (K)~(K)0;
+  for (uint64_t c0 = 0; c0 < n; c0 += (uint64_t)BLOCK * FOLD_U) {
+    K v[FOLD_U];
+#pragma unroll
+    for (int u = 0; u < FOLD_U; u++) {
+      const uint64_t i = c0 + (uint64_t)u * BLOCK + threadIdx.x;
+      v[u] = i < n ? key_at(i) : EMPTY;
+    }
+#pragma unroll
+    for (int u = 0; u < FOLD_U; u++) {
+      if (c0 + (uint64_t)u * BLOCK >= n) break;  // uniform
+      const bool full = *s_fill > HT_SLOTS / 2;  // uniform: read after a barrier
+      __syncthreads();                           // every thread has read it before it changes
+      if (full) {
+        answer();
+        vt_clear<K, BLOCK>(s_key, s_cnt, s_fill, s_ecnt);
+      }
+      if (c0 + (uint64_t)u * BLOCK + threadIdx.x < n) {
+        const uint32_t c = wave_same_value(v[u]);
+        if (c) {
+          if (v[u] == EMPTY) atomicAdd(s_ecnt, c);
+          else vt_insert(s_key, s_cnt, s_fill, v[u], c);
+        }
+      }
+      __syncthreads();
+      pending = true;
+    }
+  }
+}
+
+// Folds keys[a0..a1) into the table (vt_fold_seq over one bin).
 template <typename K, int BLOCK, typename ANSWER>
 __device__ __forceinline__ void vt_fold(const K *__restrict__ keys, uint64_t a0, uint64_t a1, K *s_key, uint32_t *s_cnt,
                                         uint32_t *s_fill, uint32_t *s_ecnt, ANSWER &&answer, bool &pending) {
-  constexpr K EMPTY = (K)~(K)0;
-  for (uint64_t c0 = a0; c0 < a1; c0 += BLOCK) {
-    const bool full = *s_fill > HT_SLOTS / 2;  // uniform: read after a barrier
-    __syncthreads();                           // every thread has read it before it changes
-    if (full) {
-      answer();
-      vt_clear<K, BLOCK>(s_key, s_cnt, s_fill, s_ecnt);
-    }
-    const uint64_t q = c0 + threadIdx.x;
-    if (q < a1) {
-      const K v = keys[q];
-      const uint32_t c = wave_same_value(v);
-      if (c) {
-        if (v == EMPTY) atomicAdd(s_ecnt, c);
-        else vt_insert(s_key, s_cnt, s_fill, v, c);
-      }
-    }
-    __syncthreads();
-    pending = true;
-  }
+  const K *kp = keys + a0;
+  vt_fold_seq<K, BLOCK>(a1 - a0, [&](uint64_t i) { return kp[i]; }, s_key, s_cnt, s_fill, s_ecnt, answer, pending);
 }
 
 // ---------------------------------------------------------------------------
@@ -2045,14 +2065,15 @@ struct BigCtl {
   uint32_t bin_base[BIG_CAP + 1];
 };
 
-// Takes the oversized items in order while their bins fit big_cap counters
-// (and at most BIG_CAP of them); prefix sums give chunk and bin bases.
+// Takes the oversized items (count > small_cap) in order while their bins
+// fit big_cap counters (and at most BIG_CAP of them); prefix sums give chunk
+// (chunk_keys keys each) and bin bases.  *seen == 0: no oversized item.
 __global__ __launch_bounds__(1024) void k_big_list(const FinItem *__restrict__ items, uint32_t nitems, uint32_t small_cap,
-                                                   uint32_t big_cap, uint32_t *__restrict__ big_slot,
-                                                   BigCtl *__restrict__ ctl) {
+                                                   uint32_t big_cap, uint32_t chunk_keys, const unsigned int *seen,
+                                                   uint32_t *__restrict__ big_slot, BigCtl *__restrict__ ctl) {
   __shared__ uint32_t s_tmp[1024 / 64 + 1];
   __shared__ uint32_t s_slot, s_ch, s_bin, s_nacc, s_acc_ch, s_acc_bin;
-  if (ctl->seen == 0) {  // no oversized item: empty lists (k_final_big reads no big_slot)
+  if (*seen == 0) {  // no oversized item: empty lists (the big-item kernels read no big_slot)
     if (threadIdx.x == 0) {
       ctl->chunk_start[0] = 0;
       ctl->bin_base[0] = 0;
@@ -2069,7 +2090,7 @@ __global__ __launch_bounds__(1024) void k_big_list(const FinItem *__restrict__ i
     FinItem it{};
     if (i < nitems) it = items[i];
     const bool big = i < nitems && it.count > small_cap;
-    const uint32_t nch = big ? (uint32_t)((it.count + BIG_CHUNK - 1) / BIG_CHUNK) : 0u;
+    const uint32_t nch = big ? (uint32_t)((it.count + chunk_keys - 1) / chunk_keys) : 0u;
     const uint32_t nbn = big ? it.nbins : 0u;
     uint32_t t_slot, t_ch, t_bin;
     const uint32_t e_slot = block_exclusive_sum<1024, uint32_t>(big ? 1u : 0u, s_tmp, t_slot);
@@ -2266,57 +2287,71 @@ __global__ __launch_bounds__(BLOCK) void k_big_place(const K *__restrict__ part,
   }
 }
 
-// Probe of buckets too large for k_probe: one workgroup streams the
-// bucket's keys and walks A's bins in global memory.
+// Probe of buckets too large for k_probe: B's keys of the bucket are
+// streamed and each walks its bin of A in global memory.  probe_big_range
+// handles keys [k0, k1) of item `ii` (a workgroup-uniform range).
+constexpr uint32_t PROBE_BIG_CHUNK = 32768;  // keys of an oversized bucket per workgroup task
+constexpr uint32_t PROBE_BIG_GRID = 1024;
 template <typename K, typename OFF, int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_probe_big(const K *__restrict__ bpart, const FinItem *__restrict__ items,
-                                                     const BinMap *__restrict__ maps, const OFF *__restrict__ offA,
-                                                     const K *__restrict__ keysA, K *__restrict__ out, uint64_t capacity,
-                                                     unsigned long long *__restrict__ cursor, uint32_t small_cap,
-                                                     unsigned int *__restrict__ hmark,
-                                                     unsigned int *__restrict__ gate) {
-  __shared__ unsigned long long s_t64[BLOCK / 64 + 1];
-  __shared__ unsigned long long s_base;
-  if (gate[0] == 0) return;  // k_probe1 took every item
-  const FinItem it = items[blockIdx.x];
-  if (it.count <= small_cap) return;
-  const BinMap mp = maps[it.table];
+__device__ void probe_big_range(const K *__restrict__ bpart, const FinItem &it, uint32_t ii, const BinMap &mp,
+                                const OFF *__restrict__ offA, const K *__restrict__ keysA, K *__restrict__ out,
+                                uint64_t capacity, unsigned long long *__restrict__ cursor,
+                                unsigned int *__restrict__ hmark, unsigned int *__restrict__ gate, uint64_t k0,
+                                uint64_t k1, unsigned long long *s_t64, unsigned long long *s_base) {
   const K *kp = bpart + it.key_start;
   const K *ka = keysA + it.key_rel;
   // U keys per thread per pass: every load of a pass is in flight at once
   constexpr int U = 8;
-  for (uint64_t c0 = 0; c0 < it.count; c0 += (uint64_t)BLOCK * U) {
+  for (uint64_t c0 = k0; c0 < k1; c0 += (uint64_t)BLOCK * U) {
     K kk[U];
     uint64_t ss[U], ee[U];
     uint32_t mm[U];
 #pragma unroll
     for (int u = 0; u < U; u++) {
       const uint64_t i = c0 + (uint64_t)u * BLOCK + threadIdx.x;
-      kk[u] = i < it.count ? kp[i] : (K)0;
+      kk[u] = i < k1 ? kp[i] : (K)0;
     }
 #pragma unroll
     for (int u = 0; u < U; u++) {
       const uint64_t i = c0 + (uint64_t)u * BLOCK + threadIdx.x;
       ss[u] = ee[u] = 0;
-      if (i < it.count) {
+      if (i < k1) {
         const uint64_t l = bin_of<K>(kk[u], mp) - it.bin_lo;
         ss[u] = (uint64_t)offA[it.off_pos + l];
         ee[u] = (uint64_t)offA[it.off_pos + l + 1];
       }
     }
     unsigned long long mine = 0;
+    uint32_t maxlen = 0;
 #pragma unroll
     for (int u = 0; u < U; u++) {
       if (ee[u] - ss[u] > PROBE_BIN_LIMIT) {
-        hmark[blockIdx.x] = 1u;  // k_probe_heavy answers this bin
+        hmark[ii] = 1u;  // k_probe_heavy answers this bin
         gate[1] = 1u;
         ee[u] = ss[u];
       }
-      uint32_t m = 0;
-      for (uint64_t q = ss[u]; q < ee[u]; q++) m += ka[q] == kk[u];
-      mm[u] = m;
-      mine += m;
+      mm[u] = 0;
+      maxlen = max(maxlen, (uint32_t)(ee[u] - ss[u]));
     }
+    // the walks of the thread's U keys advance together, WG keys of every
+    // walk per step: all U * WG loads of a step are in flight at once
+    constexpr int WG = 4;
+    for (uint32_t g = 0; g < maxlen; g += WG) {
+      K v[U][WG];
+#pragma unroll
+      for (int u = 0; u < U; u++)
+#pragma unroll
+        for (int i = 0; i < WG; i++) {
+          const uint64_t q = ss[u] + g + i;
+          v[u][i] = q < ee[u] ? ka[q] : (K)~kk[u];
+        }
+#pragma unroll
+      for (int u = 0; u < U; u++)
+#pragma unroll
+        for (int i = 0; i < WG; i++) mm[u] += v[u][i] == kk[u];
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) mine += mm[u];
     if (out == nullptr) {
       const unsigned long long t = block_sum<BLOCK, unsigned long long>(mine, s_t64);
       if (threadIdx.x == 0 && t) atomicAdd(cursor + it.table, t);
@@ -2324,9 +2359,9 @@ __global__ __launch_bounds__(BLOCK) void k_probe_big(const K *__restrict__ bpart
     }
     unsigned long long tot;
     const unsigned long long ex = block_exclusive_sum<BLOCK, unsigned long long>(mine, s_t64, tot);
-    if (threadIdx.x == 0) s_base = tot ? atomicAdd(cursor + it.table, tot) : 0ull;
+    if (threadIdx.x == 0) *s_base = tot ? atomicAdd(cursor + it.table, tot) : 0ull;
     __syncthreads();
-    unsigned long long w = s_base + ex;
+    unsigned long long w = *s_base + ex;
 #pragma unroll
     for (int u = 0; u < U; u++)
       for (uint64_t q = ss[u]; q < ee[u] && mm[u]; q++)
@@ -2338,13 +2373,66 @@ __global__ __launch_bounds__(BLOCK) void k_probe_big(const K *__restrict__ bpart
   }
 }
 
+// Oversized buckets that k_big_list did not list (big_slot ~0: beyond
+// BIG_CAP of them): one workgroup per bucket.
+template <typename K, typename OFF, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_probe_big(const K *__restrict__ bpart, const FinItem *__restrict__ items,
+                                                     const BinMap *__restrict__ maps, const OFF *__restrict__ offA,
+                                                     const K *__restrict__ keysA, K *__restrict__ out, uint64_t capacity,
+                                                     unsigned long long *__restrict__ cursor, uint32_t small_cap,
+                                                     const uint32_t *__restrict__ big_slot,
+                                                     unsigned int *__restrict__ hmark,
+                                                     unsigned int *__restrict__ gate) {
+  __shared__ unsigned long long s_t64[BLOCK / 64 + 1];
+  __shared__ unsigned long long s_base;
+  if (gate[0] == 0) return;  // k_probe1 took every item
+  const FinItem it = items[blockIdx.x];
+  if (it.count <= small_cap || big_slot[blockIdx.x] != ~0u) return;
+  const BinMap mp = maps[it.table];
+  probe_big_range<K, OFF, BLOCK>(bpart, it, blockIdx.x, mp, offA, keysA, out, capacity, cursor, hmark, gate, 0,
+                                 it.count, s_t64, &s_base);
+}
+
+// The listed oversized buckets, cut into tasks of PROBE_BIG_CHUNK keys that
+// the grid shares (a bucket of a hot or out-of-distribution range of B keys
+// no longer runs on one workgroup).
+template <typename K, typename OFF, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_probe_big_chunks(const K *__restrict__ bpart, const FinItem *__restrict__ items,
+                                                            const BinMap *__restrict__ maps, const OFF *__restrict__ offA,
+                                                            const K *__restrict__ keysA, K *__restrict__ out,
+                                                            uint64_t capacity, unsigned long long *__restrict__ cursor,
+                                                            const BigCtl *__restrict__ ctl,
+                                                            unsigned int *__restrict__ hmark,
+                                                            unsigned int *__restrict__ gate) {
+  __shared__ unsigned long long s_t64[BLOCK / 64 + 1];
+  __shared__ unsigned long long s_base;
+  if (gate[0] == 0) return;  // k_probe1 took every item (the list was not written)
+  const uint32_t total = ctl->total_chunks;
+  for (uint32_t g = blockIdx.x; g < total; g += gridDim.x) {
+    const uint32_t slot = big_slot_of(ctl, g);
+    const uint32_t ii = ctl->item[slot];
+    const FinItem it = items[ii];
+    const BinMap mp = maps[it.table];
+    const uint64_t k0 = (uint64_t)(g - ctl->chunk_start[slot]) * PROBE_BIG_CHUNK;
+    const uint64_t k1 = min((uint64_t)it.count, k0 + PROBE_BIG_CHUNK);
+    probe_big_range<K, OFF, BLOCK>(bpart, it, ii, mp, offA, keysA, out, capacity, cursor, hmark, gate, k0, k1, s_t64,
+                                   &s_base);
+  }
+}
+
 // Bins longer than PROBE_BIN_LIMIT (duplicates) of the items k_probe1, k_probe
 // and k_probe_big marked: one workgroup per marked item folds the bucket's
-// long bins of A, bin after bin, into one LDS value table (vt_fold; a value
-// lives in one bin, so the bins share the table).  Each answer of the table
-// looks every probe key of the bucket up (a value found is in a long bin;
-// the walking kernels skipped those).  Bins with more distinct values than
-// the table holds are folded over several rounds.
+// long bins of A into one LDS value table (a value lives in one bin, so the
+// bins share the table).  The bucket's offsets are read HEAVY_UB bins per
+// thread at a time; the long bins found are listed in LDS (HEAVY_LIST at a
+// time, with the exclusive prefix of their lengths) and their keys are
+// folded as one concatenated sequence (vt_fold_seq: a key's bin is found by
+// a binary search of the prefix), so short long bins share fold steps.  Each
+// answer of the table looks every probe key of the bucket up (a value found
+// is in a long bin; the walking kernels skipped those).  Bins with more
+// distinct values than the table holds are folded over several rounds.
+constexpr int HEAVY_UB = 8;
+constexpr uint32_t HEAVY_LIST = 1024;
 template <typename K, typename OFF, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_probe_heavy(const K *__restrict__ bpart, const FinItem *__restrict__ items,
                                                        const OFF *__restrict__ offA, const K *__restrict__ keysA,
@@ -2352,35 +2440,67 @@ __global__ __launch_bounds__(BLOCK) void k_probe_heavy(const K *__restrict__ bpa
                                                        unsigned long long *__restrict__ cursor,
                                                        const unsigned int *__restrict__ hmark,
                                                        const unsigned int *__restrict__ gate) {
+  static_assert(HEAVY_LIST == BLOCK, "one listed bin per thread in the prefix step");
   __shared__ K s_key[HT_SLOTS];
   __shared__ uint32_t s_cnt[HT_SLOTS];
-  __shared__ uint32_t s_hl[BLOCK];
-  __shared__ uint32_t s_nh, s_fill, s_ecnt;
+  __shared__ uint64_t s_hs[HEAVY_LIST];      // first key (index into A's bucket) of each listed bin
+  __shared__ uint64_t s_hx[HEAVY_LIST + 1];  // exclusive prefix of the listed bins' lengths
+  __shared__ uint32_t s_fill, s_ecnt;
+  __shared__ uint32_t s_t32[BLOCK / 64 + 1];
   __shared__ unsigned long long s_t64[BLOCK / 64 + 1], s_base;
   if (gate[1] == 0 || !hmark[blockIdx.x]) return;
   const FinItem it = items[blockIdx.x];
   const K *kp = bpart + it.key_start;
   const K *ka = keysA + it.key_rel;
+  const OFF *of = offA + it.off_pos;  // of[0..nbins] are the bucket's offsets
   auto answer = [&]() {
     vt_answer<K, BLOCK>(s_key, s_cnt, s_ecnt, kp, it.count, out, capacity, cursor + it.table, s_t64, &s_base);
   };
   vt_clear<K, BLOCK>(s_key, s_cnt, &s_fill, &s_ecnt);
   bool pending = false;  // values folded since the last answer (uniform)
-  for (uint32_t h0 = 0; h0 < it.nbins; h0 += BLOCK) {
-    if (threadIdx.x == 0) s_nh = 0;
-    __syncthreads();
-    const uint32_t l = h0 + threadIdx.x;
-    if (l < it.nbins) {
-      const uint64_t s = (uint64_t)offA[it.off_pos + l], e = (uint64_t)offA[it.off_pos + l + 1];
-      if (e - s > PROBE_BIN_LIMIT) s_hl[atomicAdd(&s_nh, 1u)] = l;
-    }
-    __syncthreads();
-    const uint32_t nh = s_nh;
-    __syncthreads();
-    for (uint32_t hi = 0; hi < nh; hi++) {
-      const uint32_t hb = s_hl[hi];
-      const uint64_t a0 = (uint64_t)offA[it.off_pos + hb], a1 = (uint64_t)offA[it.off_pos + hb + 1];
-      vt_fold<K, BLOCK>(ka, a0, a1, s_key, s_cnt, &s_fill, &s_ecnt, answer, pending);
+  for (uint64_t w0 = 0; w0 < it.nbins; w0 += (uint64_t)BLOCK * HEAVY_UB) {
+    // this thread's bins: l0 .. l0 + HEAVY_UB - 1
+    const uint64_t l0 = w0 + (uint64_t)threadIdx.x * HEAVY_UB;
+    uint64_t o[HEAVY_UB + 1];
+#pragma unroll
+    for (int i = 0; i <= HEAVY_UB; i++) o[i] = l0 + i <= it.nbins ? (uint64_t)of[l0 + i] : 0ull;
+    uint32_t hm = 0;
+#pragma unroll
+    for (int i = 0; i < HEAVY_UB; i++)
+      if (l0 + i < it.nbins && o[i + 1] - o[i] > PROBE_BIN_LIMIT) hm |= 1u << i;
+    uint32_t nlong;
+    const uint32_t ex = block_exclusive_sum<BLOCK, uint32_t>((uint32_t)__popc(hm), s_t32, nlong);
+    for (uint32_t p0 = 0; p0 < nlong; p0 += HEAVY_LIST) {
+      const uint32_t nh = min(HEAVY_LIST, nlong - p0);
+      uint32_t pos = ex;
+#pragma unroll
+      for (int i = 0; i < HEAVY_UB; i++) {
+        if ((hm >> i) & 1u) {
+          if (pos >= p0 && pos < p0 + nh) {
+            s_hs[pos - p0] = o[i];
+            s_hx[pos - p0] = o[i + 1] - o[i];  // length; the prefix replaces it below
+          }
+          pos++;
+        }
+      }
+      __syncthreads();
+      const uint64_t len = threadIdx.x < nh ? s_hx[threadIdx.x] : 0ull;
+      unsigned long long tot;
+      const unsigned long long lx = block_exclusive_sum<BLOCK, unsigned long long>(len, s_t64, tot);
+      if (threadIdx.x < nh) s_hx[threadIdx.x] = lx;
+      if (threadIdx.x == 0) s_hx[nh] = tot;
+      __syncthreads();
+      // key i of the sequence: bin j with s_hx[j] <= i < s_hx[j + 1]
+      auto key_at = [&](uint64_t i) {
+        uint32_t lo = 0, hi = nh;  // s_hx[lo] <= i < s_hx[hi]
+        while (hi - lo > 1) {
+          const uint32_t mid = (lo + hi) >> 1;
+          if (s_hx[mid] <= i) lo = mid;
+          else hi = mid;
+        }
+        return ka[s_hs[lo] + (i - s_hx[lo])];
+      };
+      vt_fold_seq<K, BLOCK>(tot, key_at, s_key, s_cnt, &s_fill, &s_ecnt, answer, pending);
     }
   }
   if (pending) answer();
@@ -2519,8 +2639,8 @@ static hipError_t launch_final_t(const LaunchFinal &lf, hipStream_t st) {
     constexpr int KPT = sizeof(K) == 4 ? KPT_U32 : 32;
     const uint32_t small_cap = std::min(1024u * KPT, 65535u);
     BigCtl *ctl = (BigCtl *)lf.big_ctl;
-    hipLaunchKernelGGL(k_big_list, dim3(1), dim3(1024), 0, st, lf.items, lf.nitems, small_cap, lf.big_cap, lf.big_slot,
-                       ctl);
+    hipLaunchKernelGGL(k_big_list, dim3(1), dim3(1024), 0, st, lf.items, lf.nitems, small_cap, lf.big_cap, BIG_CHUNK,
+                       (const unsigned int *)&ctl->seen, lf.big_slot, ctl);
     hipLaunchKernelGGL(k_big_zero, dim3(256), dim3(256), 0, st, (const BigCtl *)ctl, lf.big_cnt);
     hipLaunchKernelGGL((k_big_count<K, 1024>), dim3(2048), dim3(1024), 0, st, (const K *)lf.part, lf.items, lf.maps,
                        (const BigCtl *)ctl, lf.big_cnt);
@@ -2567,9 +2687,17 @@ static hipError_t launch_probe_t(const LaunchProbe &p, hipStream_t st) {
   });
   if (e != hipSuccess) return e;
   const uint32_t small_cap = std::min(1024u * KPT, 65535u);
+  // oversized buckets: up to BIG_CAP of them listed and cut into tasks of
+  // PROBE_BIG_CHUNK keys for the whole grid, the rest one workgroup each
+  BigCtl *ctl = (BigCtl *)p.big_ctl;
+  hipLaunchKernelGGL(k_big_list, dim3(1), dim3(1024), 0, st, p.items, p.nitems, small_cap, 0xFFFFFFFFu, PROBE_BIG_CHUNK,
+                     (const unsigned int *)p.gate, p.big_slot, ctl);
   hipLaunchKernelGGL((k_probe_big<K, OFF, 1024>), dim3(p.nitems), dim3(1024), 0, st, (const K *)p.bpart, p.items, p.maps,
-                     (const OFF *)p.offA, (const K *)p.keysA, (K *)p.out, p.capacity, p.cursor, small_cap, p.hmark,
-                     p.gate);
+                     (const OFF *)p.offA, (const K *)p.keysA, (K *)p.out, p.capacity, p.cursor, small_cap,
+                     (const uint32_t *)p.big_slot, p.hmark, p.gate);
+  hipLaunchKernelGGL((k_probe_big_chunks<K, OFF, 1024>), dim3(PROBE_BIG_GRID), dim3(1024), 0, st, (const K *)p.bpart,
+                     p.items, p.maps, (const OFF *)p.offA, (const K *)p.keysA, (K *)p.out, p.capacity, p.cursor,
+                     (const BigCtl *)ctl, p.hmark, p.gate);
   hipLaunchKernelGGL((k_probe_heavy<K, OFF, 1024>), dim3(p.nitems), dim3(1024), 0, st, (const K *)p.bpart, p.items,
                      (const OFF *)p.offA, (const K *)p.keysA, (K *)p.out, p.capacity, p.cursor,
                      (const unsigned int *)p.hmark, (const unsigned int *)p.gate);

@@ -72,6 +72,8 @@ struct LaunchProbe {
   unsigned long long *cursor;  // match counters, one per table (FinItem.table)
   unsigned int *hmark;         // nitems, zeroed: items with a bin beyond the walk limit
   unsigned int *gate;          // 2 zeroed words: [0] k_probe1 left items, [1] a bin was marked heavy
+  uint32_t *big_slot;          // nitems words and a final_big_ctl_bytes() area: the list of the
+  void *big_ctl;               // oversized buckets (written by the probe itself)
 };
 
 struct LaunchJoin {

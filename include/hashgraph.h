@@ -192,7 +192,10 @@ hg_status hg_build_partitioned_indexed(hg_ctx ctx, const hg_table_desc *desc, co
  * No counterpart in the reference.  A two-stage piecewise-linear CDF model in
  * FP32: a linear root picks one of n_leaves leaves, each leaf's line maps the
  * key's distance from the leaf's first key to a bin; bases are monotone, so
- * the bin order follows the key order.  Layout: 32-byte header, then
+ * the bin order follows the key order.  A key whose line value falls outside
+ * [0, n_bins - 1] (beyond the range the model was trained on) takes murmur3's
+ * bin, so out-of-range keys spread over the table instead of filling an edge
+ * bin.  Layout: 32-byte header, then
  * n_leaves + 1 leaves of 16 bytes (the last is a sentinel).
  *
  * hg_rmi_train draws n_sample keys of d_keys (with replacement, SplitMix64

@@ -140,9 +140,11 @@ int hgo_build_csr(const void *keys, uint64_t n, int kb, uint64_t M, uint64_t *of
 /* root_kind, 4 pad bytes; then n_leaves leaves of {u64 x0, f32 slope, u32    */
 /* anchor}).  Every product and sum is rounded on its own                     */
 /* (-ffp-contract=off), u64 -> f32 conversions round to nearest even, float   */
-/* -> u32 truncates.  Pinned by closed forms on hand-built models, an exact   */
-/* integer restatement of the u64 -> f32 rounding in the tests, and the       */
-/* intersection being independent of the hash.                               */
+/* -> u32 truncates.  A key whose leaf line lands outside [0, M-1] takes     */
+/* murmur3's bin.  Pinned by closed forms on hand-built models, an exact      */
+/* integer restatement of the u64 -> f32 rounding in the tests, the murmur3   */
+/* vectors for the out-of-range keys, and the intersection being independent  */
+/* of the hash.                                                               */
 /* ------------------------------------------------------------------------ */
 typedef struct { uint64_t key_min; float root_slope, root_icpt; uint32_t n_leaves, n_bins, root_kind, pad; }
     hgo_rmi_header;
@@ -177,8 +179,8 @@ uint32_t hgo_rmi_bin(const void *model, uint64_t key, uint64_t M) {
   float y = l.slope * d;
   int64_t off = !(y > 0.0f) ? 0 : (y >= 4294967040.0f ? (int64_t)0xFFFFFF00u : (int64_t)(uint32_t)y);
   int64_t b = (int64_t)l.anchor + (up ? off : -off);
-  if (b < 0) b = 0;
-  if (b > (int64_t)M - 1) b = (int64_t)M - 1;
+  /* a line value outside [0, M-1] (a key beyond the trained range): murmur3 */
+  if (b < 0 || b > (int64_t)M - 1) return (uint32_t)(hgo_hash_key(key) % M);
   return (uint32_t)b;
 }
 

@@ -132,6 +132,31 @@ def test_many_long_bins_answered_in_groups(dev):
     _check_probe(A, B, 512, dev)
 
 
+def test_oversized_bucket_of_long_and_mid_bins(dev):
+    """One oversized bucket (M = 2^20, so a bucket spans 65535 bins) holding
+    1500 long bins of 40 keys among its first 8192 bins -- more long bins than
+    the heavy kernel lists at once -- and 600 bins of 20 copies further on,
+    which the oversized-bucket walk reads in groups (B's side of the bucket
+    is oversized too); the rest of the table is sparse.  Count and list
+    equal the oracle's bag join."""
+    M = 1 << 20
+    cand = wl.uniform_keys(61, 400000, 4)
+    bins = o.bins(cand, M)
+    _, first = np.unique(bins, return_index=True)  # one value per bin
+    cand, bins = cand[first], bins[first]
+    long_v = cand[bins < 8192][:1500]
+    mid_v = cand[(bins >= 8192) & (bins < 65535)][:600]
+    assert len(long_v) == 1500 and len(mid_v) == 600
+    rest = cand[bins >= 65535][:20000]
+    A = np.concatenate([np.repeat(long_v, 40), np.repeat(mid_v, 20), rest])
+    rng = np.random.default_rng(62)
+    rng.shuffle(A)
+    # B's part of the bucket (38000 keys) is beyond one probe workgroup's capacity
+    B = np.concatenate([long_v[rng.integers(0, 1500, 30000)], mid_v[rng.integers(0, 600, 8000)], rest[:5000],
+                        wl.uniform_keys(63, 4000, 4)])
+    _check_probe(A, B, M, dev)
+
+
 def test_heavy_bin_with_many_values(dev):
     """One bin (M = 1) of 10000 distinct values, some repeated: beyond the
     walk limit and beyond one value table, so the bin is folded over several

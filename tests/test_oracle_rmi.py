@@ -37,11 +37,14 @@ def test_exact_f32_known_values():
 
 
 def test_identity_leaf_is_truncation():
-    """One leaf, slope 1, anchor 0: bin = min(f32(key - x0), M-1) exactly."""
+    """One leaf, slope 1, anchor 0: bin = f32(key - x0) exactly while that is
+    a bin; keys past M - 1 are beyond the model and take murmur3's bin."""
     M = 1 << 20
     m = o.rmi_model(0, 0.0, 0.0, [(0, 1.0, 0)], M)
-    for k in [0, 1, 7, 1000, M - 2, M - 1, M, 1 << 22, (1 << 24) - 1]:
-        assert o.rmi_bin(m, k, M) == min(k, M - 1)
+    for k in [0, 1, 7, 1000, M - 2, M - 1]:
+        assert o.rmi_bin(m, k, M) == k
+    for k in [M, M + 1, 1 << 22, (1 << 24) - 1]:
+        assert o.rmi_bin(m, k, M) == o.hash_key(k) % M
 
 
 def test_power_of_two_slope():
@@ -70,8 +73,25 @@ def test_signed_extrapolation_below_x0():
     assert o.rmi_bin(m, 5011, 1 << 16) == 105
     assert o.rmi_bin(m, 4989, 1 << 16) == 95      # 100 - trunc(5.5)
     assert o.rmi_bin(m, 4800, 1 << 16) == 0       # 100 - 100
-    assert o.rmi_bin(m, 0, 1 << 16) == 0          # clamped
-    assert o.rmi_bin(m, 5000 + 10 ** 6, 1 << 16) == (1 << 16) - 1
+    assert o.rmi_bin(m, 4798, 1 << 16) == o.hash_key(4798) % (1 << 16)  # 100 - 101 < 0: murmur3
+    assert o.rmi_bin(m, 0, 1 << 16) == o.hash_key(0) % (1 << 16)  # line -2400: murmur3
+    assert o.rmi_bin(m, 5000 + 10 ** 6, 1 << 16) == o.hash_key(5000 + 10 ** 6) % (1 << 16)
+
+
+def test_out_of_range_keys_take_murmur3():
+    """The edges of the line: the last in-range value on each side is kept,
+    one past it is murmur3 (published vectors pin hash_key); the rule
+    depends on the key alone, so a table and its probe keys agree."""
+    M = 1000
+    m = o.rmi_model(0, 0.0, 0.0, [(10000, 1.0, 500)], M)
+    assert o.rmi_bin(m, 10000 + 499, M) == 999       # line 999: the last bin
+    assert o.rmi_bin(m, 10000 + 500, M) == o.hash_key(10500) % M
+    assert o.rmi_bin(m, 10000 - 500, M) == 0         # line 0
+    assert o.rmi_bin(m, 10000 - 501, M) == o.hash_key(9499) % M
+    # murmur3 of the four low bytes, seed 0 (published vector of 0x87654321)
+    assert o.hash_key(0x87654321) == 0xF55B516B
+    m2 = o.rmi_model(0, 0.0, 0.0, [(0, 1.0, 0)], 1 << 30)
+    assert o.rmi_bin(m2, (1 << 32) | 0x87654321, 1 << 30) == 0xF55B516B % (1 << 30)
 
 
 def test_key_min_anchor_of_root():
