@@ -61,3 +61,29 @@ def test_multi_hash_b32_and_index_modes(exe, gpus, dev):
     assert "error" not in r.stderr.lower()
     r = _run([os.path.join(BIN, exe), str(nA), str(nA), "16000", str(gpus), "16000", "check", str(nB), "build"])
     assert r.returncode == 0 and "error" not in r.stderr.lower()
+
+
+@pytest.mark.parametrize("name", ["c1_u32", "c2_u64"])
+def test_single_hash_graph_class_against_golden(name, dev, tmp_path):
+    """SingleHashGraph as a class (caller keys, build, CSR accessors,
+    intersect) against the golden fixtures: offsets identical, every bin the
+    same multiset of keys, intersection count and sorted key list identical
+    (keys are int64 in the class; the uint32 fixture's values widen)."""
+    import numpy as np
+    g = np.load(os.path.join(ROOT, "tests", "golden", name + ".npz"))
+    A, B, M = g["A"].astype(np.int64), g["B"].astype(np.int64), int(g["M"])
+    A.tofile(tmp_path / "A.bin")
+    B.tofile(tmp_path / "B.bin")
+    r = _run([os.path.join(BIN, "sing-hash"), "--files", str(tmp_path / "A.bin"), str(tmp_path / "B.bin"), str(M),
+              str(tmp_path)])
+    assert r.returncode == 0
+    assert f"intersect count: {int(g['count'])}" in r.stdout
+    off = np.fromfile(tmp_path / "off.bin", dtype=np.int32).astype(np.int64)
+    csr = np.fromfile(tmp_path / "csr.bin", dtype=np.int64)
+    out = np.fromfile(tmp_path / "out.bin", dtype=np.int64)
+    assert np.array_equal(off, g["offA"].astype(np.int64))
+    csr, out = csr.view(np.uint64), out.view(np.uint64)  # the fixtures' order is unsigned
+    for b in range(M):  # the fixture's bins are sorted (canonical order)
+        lo, hi = int(off[b]), int(off[b + 1])
+        assert np.array_equal(np.sort(csr[lo:hi]), g["csrA"][lo:hi].astype(np.uint64))
+    assert np.array_equal(np.sort(out), np.sort(g["matches"].astype(np.uint64)))
