@@ -467,7 +467,8 @@ def run_c3(dev, steps, warmup, cpu=True):
     s = 8
     build_bytes = 4 * s * nA + 4 * (M + 1)
     inter_bytes = 3 * s * nB + s * nA + 4 * (M + 1)  # + the run-length records
-    bt = c3_kernel_bytes(nA, nB, M, P, s)
+    # kt sums `steps` steps: the per-step bytes scaled to match
+    bt = {k: v * steps for k, v in c3_kernel_bytes(nA, nB, M, P, s).items()}
     res = {
         "value": round((nA + nB) / (ms / 1e3) / 1e9, 3), "unit": "Gkeys/s", "ms_per_step": round(ms, 3),
         "dtype": "u64", "data": "synthetic (Zipf(1.1) by rejection-inversion on the device)",
