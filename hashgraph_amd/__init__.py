@@ -17,7 +17,7 @@ from . import _lib
 from ._lib import HG_HASH_MURMUR3, HG_HASH_RMI, HGError, check
 
 __all__ = ["Context", "build", "probe", "intersect", "intersect_rle", "exclusive_scan", "hash_bins", "find_splits",
-           "build_partitioned", "rmi_train", "rmi_model_bytes", "batch_build", "batch_probe", "HGError",
+           "build_partitioned", "rmi_train", "rmi_model_bytes", "batch_build", "batch_probe", "batch_probe_emit", "HGError",
            "HG_HASH_MURMUR3", "HG_HASH_RMI", "version", "table_bins", "BatchPlan"]
 
 _ctx = {}
@@ -344,6 +344,31 @@ def batch_probe(offsets, csr_keys, key_off, n_bins, off_off, probe, probe_off, h
     check(c.lib.hg_batch_probe(c.bind_stream(), ctypes.byref(bd), _p(offsets), _p(csr_keys), _p(probe), po,
                                _p(counts), ctypes.byref(total)), "hg_batch_probe")
     return total.value, counts
+
+
+def batch_probe_emit(offsets, csr_keys, key_off, n_bins, off_off, probe, probe_off, out_off, out=None,
+                     hash_kinds=None, rmi_models=None, rmi_leaves=0):
+    """batch_probe with the matches written: table t's matched probe keys go to
+    out[out_off[t]:out_off[t+1]] (out allocated here when None).  Returns
+    (total, counts, out); raises HGError (HG_ERR_CAPACITY) when a table's
+    matches exceed its region."""
+    torch = _torch()
+    _check_dev(offsets, csr_keys, probe)
+    bd = _batch_desc(key_off, n_bins, off_off, _key_bytes(csr_keys), hash_kinds, rmi_models, rmi_leaves)
+    c = context(csr_keys.device.index)
+    T = len(n_bins)
+    oo = [int(x) for x in out_off]
+    if len(oo) != T + 1:
+        raise ValueError("out_off needs n_tables + 1 entries")
+    if out is None:
+        out = torch.empty(max(oo[-1], 1), dtype=csr_keys.dtype, device=csr_keys.device)
+    counts = torch.zeros(T, dtype=torch.int64, device=csr_keys.device)
+    po = (ctypes.c_uint64 * (T + 1))(*[int(x) for x in probe_off])
+    ho = (ctypes.c_uint64 * (T + 1))(*oo)
+    total = ctypes.c_uint64(0)
+    check(c.lib.hg_batch_probe_emit(c.bind_stream(), ctypes.byref(bd), _p(offsets), _p(csr_keys), _p(probe), po, ho,
+                                    _p(out), _p(counts), ctypes.byref(total)), "hg_batch_probe_emit")
+    return total.value, counts, out
 
 
 class BatchPlan:

@@ -86,6 +86,25 @@ double build_fill() {
   return 0.875;
 }
 
+// Super-buckets of a two-level partition (0: one level whenever the buckets
+// fit one pass).  HG_L1_FANOUT sets a first-level fan-out for tuning: a table
+// with more than twice that many buckets is then partitioned in two
+// low-fan-out passes.
+uint64_t l1_fanout() {
+  if (const char *e = getenv("HG_L1_FANOUT")) {
+    const long v = atol(e);
+    if (v >= 2 && v <= (long)kFMax) return (uint64_t)v;
+  }
+  return 0;
+}
+// Buckets per super-bucket of a table of F buckets (1: one level).
+uint64_t level1_group(uint64_t F) {
+  uint64_t k = F > kFMax ? (F + kFMax - 1) / kFMax : 1;
+  const uint64_t f1 = l1_fanout();
+  if (f1 && F > 2 * f1) k = std::max<uint64_t>(k, (F + f1 - 1) / f1);
+  return k;
+}
+
 // Bucket fill of the probe: a bucket's keys of A and its 16-bit offsets must
 // fit the one-pass probe's 156 KB of LDS (W bins at load factor 1 take 6 W
 // bytes for uint32 keys, 10 W for uint64: 0.72 and 0.43 of 32 K bins leave
@@ -584,7 +603,8 @@ hg_status build_table(hg_ctx c, const hg_table_desc *desc, const void *d_keys, v
   if (n == 0) return fill_offsets(c, d_off, ob, M + 1, 0);
   const uint32_t W = choose_width(n, M, kb, build_fill());
   const uint64_t F = (M + W - 1) / W;
-  if (F <= kFMax) {
+  const uint64_t k = level1_group(F);
+  if (k == 1) {
     Level L;
     SegDesc s = range_seg(0, n, 0, M, W, map);
     s.fin_off_base = 0;
@@ -608,7 +628,6 @@ hg_status build_table(hg_ctx c, const hg_table_desc *desc, const void *d_keys, v
   }
   // Two levels: coarse super-buckets of k*W bins, their sizes read back, then
   // every super-bucket is a segment of the second pass.
-  const uint64_t k = (F + kFMax - 1) / kFMax;
   const uint64_t W1 = (uint64_t)W * k;
   if (W1 >= (1ull << 32)) return fail(HG_ERR_UNSUPPORTED, "table geometry too large");
   Level L1;
@@ -1157,8 +1176,8 @@ hg_status hg_probe(hg_ctx c, const hg_table_desc *descA, const void *d_offA, con
   // Beyond kFMax buckets, B is partitioned twice: into super-buckets of k
   // buckets, whose sizes are read back, then each super-bucket into its
   // buckets (as the build's second level).
-  const bool two = F > kFMax;
-  const uint64_t k = two ? (F + kFMax - 1) / kFMax : 1;
+  const uint64_t k = level1_group(F);
+  const bool two = k > 1;
   if ((uint64_t)W * k >= (1ull << 32)) return fail(HG_ERR_UNSUPPORTED, "probe geometry too large");
   Plan P0;
   const size_t i_part = P0.add(n_probe * kb);
@@ -1688,7 +1707,8 @@ hg_status batch_launch_build(hg_ctx c, const hg_batch_desc *bd, BatchSide &S, co
 }
 
 hg_status batch_launch_probe(hg_ctx c, const hg_batch_desc *bd, BatchSide &S, const void *d_probe,
-                             const uint32_t *d_offsets, const void *d_csr, uint64_t *d_counts) {
+                             const uint32_t *d_offsets, const void *d_csr, uint64_t *d_counts, void *d_out = nullptr,
+                             const uint64_t *d_out_off = nullptr) {
   HG_HIP(launch_fill_u64((unsigned long long *)d_counts, 0ull, bd->n_tables, c->stream));
   HG_HIP(launch_fill_u32(S.flag, 0u, 3, c->stream));  // [1], [2]: the probe gates
   HG_HIP(launch_fill_u32(S.hmark, 0u, S.L.ncols_total, c->stream));
@@ -1706,7 +1726,8 @@ hg_status batch_launch_probe(hg_ctx c, const hg_batch_desc *bd, BatchSide &S, co
   pr.maps = S.d.maps;
   pr.offA = d_offsets;
   pr.keysA = d_csr;
-  pr.out = nullptr;
+  pr.out = d_out;  // with per-table regions d_out_off (null: counts only)
+  pr.out_off = d_out ? d_out_off : nullptr;
   pr.capacity = 0;
   pr.cursor = (unsigned long long *)d_counts;
   pr.gate = S.flag + 1;
@@ -1813,6 +1834,48 @@ hg_status hg_batch_probe(hg_ctx c, const hg_batch_desc *bd, const uint32_t *d_of
     for (uint64_t v : cnt) s += v;
     *h_total = s;
   }
+  return HG_OK;
+}
+
+hg_status hg_batch_probe_emit(hg_ctx c, const hg_batch_desc *bd, const uint32_t *d_offsets, const void *d_csr_keys,
+                              const void *d_probe, const uint64_t *h_probe_off, const uint64_t *h_out_off, void *d_out,
+                              uint64_t *d_counts, uint64_t *h_total) {
+  if (!c) return fail(HG_ERR_INVALID_ARG, "null ctx");
+  HG_TRY(batch_validate(bd));
+  if (!h_probe_off || !h_out_off || !d_counts || !d_offsets) return fail(HG_ERR_INVALID_ARG, "null argument");
+  const uint32_t T = bd->n_tables;
+  for (uint32_t t = 0; t < T; t++) {
+    if (h_probe_off[t + 1] < h_probe_off[t]) return fail(HG_ERR_INVALID_ARG, "probe offsets must not decrease");
+    if (h_out_off[t + 1] < h_out_off[t]) return fail(HG_ERR_INVALID_ARG, "output offsets must not decrease");
+  }
+  if (h_out_off[T] > h_out_off[0] && !d_out) return fail(HG_ERR_INVALID_ARG, "null output buffer");
+  DeviceGuard g(c->device);
+  BatchSide S;
+  HG_TRY(batch_prepare(c, bd, h_probe_off, BatchCfg(), S));
+  // the regions' offsets on the device (arena 2: unused by the batch's levels)
+  Plan P;
+  const size_t i_oo = P.add(((size_t)T + 1) * 8);
+  HG_TRY(arena_ensure(c, 2, P.total()));
+  Carve cv;
+  cv.init(c, 2, P);
+  uint64_t *d_oo = cv.get<uint64_t>(i_oo);
+  HG_TRY(upload(c, d_oo, h_out_off, ((size_t)T + 1) * 8));
+  // a dummy non-null buffer keeps the emitting kernels when every region is empty
+  void *out = d_out ? d_out : (void *)d_oo;
+  HG_TRY(batch_launch_probe(c, bd, S, d_probe, d_offsets, d_csr_keys, d_counts, out, d_oo));
+  std::vector<uint64_t> cnt(T);
+  for (uint32_t t0 = 0; t0 < T; t0 += 65536) {
+    const uint32_t k = std::min<uint32_t>(65536, T - t0);
+    HG_TRY(sync_read(c, cnt.data() + t0, d_counts + t0, (size_t)k * 8));
+  }
+  uint64_t total = 0;
+  bool short_region = false;
+  for (uint32_t t = 0; t < T; t++) {
+    total += cnt[t];
+    short_region |= cnt[t] > h_out_off[t + 1] - h_out_off[t];
+  }
+  if (h_total) *h_total = total;
+  if (short_region) return fail(HG_ERR_CAPACITY, "a table's matches exceed its output region");
   return HG_OK;
 }
 

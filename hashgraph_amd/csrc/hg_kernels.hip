@@ -516,12 +516,24 @@ __device__ __forceinline__ bool probe1_takes(uint32_t count, uint64_t na, uint32
          na * key_bytes + 16 + probe1_off_bytes(W) + PROBE1_LEAF_BYTES <= lds_bytes;
 }
 
+// Output region of table t's matches: the whole buffer (out_off null), or
+// out[out_off[t] .. out_off[t+1]) when a batch gives every table its own.
+template <typename K>
+__device__ __forceinline__ K *out_region(K *out, uint64_t capacity, const uint64_t *out_off, uint32_t t,
+                                         uint64_t &cap) {
+  cap = capacity;
+  if (out == nullptr || out_off == nullptr) return out;
+  cap = out_off[t + 1] - out_off[t];
+  return out + out_off[t];
+}
+
 template <typename K, typename OFF, int BLOCK, int KPT, int HK>
 __global__ __launch_bounds__(BLOCK) void k_probe1(const K *__restrict__ bpart, const FinItem *__restrict__ items,
                                                   const BinMap *__restrict__ maps, const OFF *__restrict__ offA,
                                                   const K *__restrict__ keysA, K *__restrict__ out, uint64_t capacity,
                                                   unsigned long long *__restrict__ cursor, uint32_t lds_bytes,
-                                                  unsigned int *__restrict__ hmark, unsigned int *__restrict__ gate) {
+                                                  unsigned int *__restrict__ hmark, unsigned int *__restrict__ gate,
+                                                  const uint64_t *__restrict__ out_off) {
   extern __shared__ __align__(16) uint8_t smem[];
   __shared__ uint32_t s_tmp[BLOCK / 64 + 1];
   __shared__ unsigned long long s_base;
@@ -593,12 +605,14 @@ __global__ __launch_bounds__(BLOCK) void k_probe1(const K *__restrict__ bpart, c
   if (threadIdx.x == 0) s_base = tot ? atomicAdd(cursor + it.table, (unsigned long long)tot) : 0ull;
   __syncthreads();
   if (out == nullptr || !mine) return;
+  uint64_t cap;
+  K *o = out_region(out, capacity, out_off, it.table, cap);
   unsigned long long w = s_base + ex;
 #pragma unroll
   for (int j = 0; j < KPT; j++) {
     const uint32_t m = (mc[j >> 1] >> ((j & 1) * 16)) & 0xFFFFu;
     for (uint32_t r = 0; r < m; r++, w++)
-      if (w < capacity) out[w] = kr[j];
+      if (w < cap) o[w] = kr[j];
   }
 }
 
@@ -619,7 +633,8 @@ __global__ __launch_bounds__(BLOCK) void k_probe(const K *__restrict__ bpart, co
                                                  const K *__restrict__ keysA, K *__restrict__ out, uint64_t capacity,
                                                  unsigned long long *__restrict__ cursor, uint32_t akey_cap,
                                                  unsigned int *__restrict__ hmark, uint32_t one_cnt,
-                                                 uint32_t one_lds, unsigned int *__restrict__ gate) {
+                                                 uint32_t one_lds, unsigned int *__restrict__ gate,
+                                                 const uint64_t *__restrict__ out_off) {
   extern __shared__ __align__(16) uint8_t smem[];
   __shared__ uint32_t s_tmp[BLOCK / 64 + 1];
   __shared__ uint32_t s_n;
@@ -699,12 +714,14 @@ __global__ __launch_bounds__(BLOCK) void k_probe(const K *__restrict__ bpart, co
   if (threadIdx.x == 0) s_base = tot ? atomicAdd(cursor + it.table, (unsigned long long)tot) : 0ull;
   __syncthreads();
   if (out == nullptr || !mine) return;
+  uint64_t cap;
+  K *o = out_region(out, capacity, out_off, it.table, cap);
   unsigned long long w = s_base + ex;
 #pragma unroll
   for (int j = 0; j < KPT; j++) {
     const uint32_t m = (mc[j >> 1] >> ((j & 1) * 16)) & 0xFFFFu;
     for (uint32_t r = 0; r < m; r++, w++)
-      if (w < capacity) out[w] = kr[j];
+      if (w < cap) o[w] = kr[j];
   }
 }
 
@@ -2297,9 +2314,12 @@ __device__ void probe_big_range(const K *__restrict__ bpart, const FinItem &it, 
                                 const OFF *__restrict__ offA, const K *__restrict__ keysA, K *__restrict__ out,
                                 uint64_t capacity, unsigned long long *__restrict__ cursor,
                                 unsigned int *__restrict__ hmark, unsigned int *__restrict__ gate, uint64_t k0,
-                                uint64_t k1, unsigned long long *s_t64, unsigned long long *s_base) {
+                                uint64_t k1, unsigned long long *s_t64, unsigned long long *s_base,
+                                const uint64_t *__restrict__ out_off) {
   const K *kp = bpart + it.key_start;
   const K *ka = keysA + it.key_rel;
+  uint64_t cap;
+  K *o = out_region(out, capacity, out_off, it.table, cap);
   // U keys per thread per pass: every load of a pass is in flight at once
   constexpr int U = 8;
   for (uint64_t c0 = k0; c0 < k1; c0 += (uint64_t)BLOCK * U) {
@@ -2366,7 +2386,7 @@ __device__ void probe_big_range(const K *__restrict__ bpart, const FinItem &it, 
     for (int u = 0; u < U; u++)
       for (uint64_t q = ss[u]; q < ee[u] && mm[u]; q++)
         if (ka[q] == kk[u]) {
-          if (w < capacity) out[w] = kk[u];
+          if (w < cap) o[w] = kk[u];
           w++;
         }
     __syncthreads();
@@ -2382,7 +2402,8 @@ __global__ __launch_bounds__(BLOCK) void k_probe_big(const K *__restrict__ bpart
                                                      unsigned long long *__restrict__ cursor, uint32_t small_cap,
                                                      const uint32_t *__restrict__ big_slot,
                                                      unsigned int *__restrict__ hmark,
-                                                     unsigned int *__restrict__ gate) {
+                                                     unsigned int *__restrict__ gate,
+                                                     const uint64_t *__restrict__ out_off) {
   __shared__ unsigned long long s_t64[BLOCK / 64 + 1];
   __shared__ unsigned long long s_base;
   if (gate[0] == 0) return;  // k_probe1 took every item
@@ -2390,7 +2411,7 @@ __global__ __launch_bounds__(BLOCK) void k_probe_big(const K *__restrict__ bpart
   if (it.count <= small_cap || big_slot[blockIdx.x] != ~0u) return;
   const BinMap mp = maps[it.table];
   probe_big_range<K, OFF, BLOCK>(bpart, it, blockIdx.x, mp, offA, keysA, out, capacity, cursor, hmark, gate, 0,
-                                 it.count, s_t64, &s_base);
+                                 it.count, s_t64, &s_base, out_off);
 }
 
 // The listed oversized buckets, cut into tasks of PROBE_BIG_CHUNK keys that
@@ -2403,7 +2424,8 @@ __global__ __launch_bounds__(BLOCK) void k_probe_big_chunks(const K *__restrict_
                                                             uint64_t capacity, unsigned long long *__restrict__ cursor,
                                                             const BigCtl *__restrict__ ctl,
                                                             unsigned int *__restrict__ hmark,
-                                                            unsigned int *__restrict__ gate) {
+                                                            unsigned int *__restrict__ gate,
+                                                            const uint64_t *__restrict__ out_off) {
   __shared__ unsigned long long s_t64[BLOCK / 64 + 1];
   __shared__ unsigned long long s_base;
   if (gate[0] == 0) return;  // k_probe1 took every item (the list was not written)
@@ -2416,7 +2438,7 @@ __global__ __launch_bounds__(BLOCK) void k_probe_big_chunks(const K *__restrict_
     const uint64_t k0 = (uint64_t)(g - ctl->chunk_start[slot]) * PROBE_BIG_CHUNK;
     const uint64_t k1 = min((uint64_t)it.count, k0 + PROBE_BIG_CHUNK);
     probe_big_range<K, OFF, BLOCK>(bpart, it, ii, mp, offA, keysA, out, capacity, cursor, hmark, gate, k0, k1, s_t64,
-                                   &s_base);
+                                   &s_base, out_off);
   }
 }
 
@@ -2439,7 +2461,8 @@ __global__ __launch_bounds__(BLOCK) void k_probe_heavy(const K *__restrict__ bpa
                                                        K *__restrict__ out, uint64_t capacity,
                                                        unsigned long long *__restrict__ cursor,
                                                        const unsigned int *__restrict__ hmark,
-                                                       const unsigned int *__restrict__ gate) {
+                                                       const unsigned int *__restrict__ gate,
+                                                       const uint64_t *__restrict__ out_off) {
   static_assert(HEAVY_LIST == BLOCK, "one listed bin per thread in the prefix step");
   __shared__ K s_key[HT_SLOTS];
   __shared__ uint32_t s_cnt[HT_SLOTS];
@@ -2453,8 +2476,10 @@ __global__ __launch_bounds__(BLOCK) void k_probe_heavy(const K *__restrict__ bpa
   const K *kp = bpart + it.key_start;
   const K *ka = keysA + it.key_rel;
   const OFF *of = offA + it.off_pos;  // of[0..nbins] are the bucket's offsets
+  uint64_t cap;
+  K *o = out_region(out, capacity, out_off, it.table, cap);
   auto answer = [&]() {
-    vt_answer<K, BLOCK>(s_key, s_cnt, s_ecnt, kp, it.count, out, capacity, cursor + it.table, s_t64, &s_base);
+    vt_answer<K, BLOCK>(s_key, s_cnt, s_ecnt, kp, it.count, o, cap, cursor + it.table, s_t64, &s_base);
   };
   vt_clear<K, BLOCK>(s_key, s_cnt, &s_fill, &s_ecnt);
   bool pending = false;  // values folded since the last answer (uniform)
@@ -2679,10 +2704,11 @@ static hipError_t launch_probe_t(const LaunchProbe &p, hipStream_t st) {
   const hipError_t e = dispatch_hk(p.hk, [&](auto h) {
     hipLaunchKernelGGL((k_probe1<K, OFF, 1024, KPT1, decltype(h)::value>), dim3(p.nitems), dim3(1024),
                        (size_t)PROBE1_LDS, st, (const K *)p.bpart, p.items, p.maps, (const OFF *)p.offA,
-                       (const K *)p.keysA, (K *)p.out, p.capacity, p.cursor, PROBE1_LDS, p.hmark, p.gate);
+                       (const K *)p.keysA, (K *)p.out, p.capacity, p.cursor, PROBE1_LDS, p.hmark, p.gate, p.out_off);
     hipLaunchKernelGGL((k_probe<K, OFF, 1024, KPT, SUB, decltype(h)::value>), dim3(p.nitems), dim3(1024), lds, st,
                        (const K *)p.bpart, p.items, p.maps, (const OFF *)p.offA, (const K *)p.keysA, (K *)p.out,
-                       p.capacity, p.cursor, p.akey_cap, p.hmark, (uint32_t)(1024 * KPT1), PROBE1_LDS, p.gate);
+                       p.capacity, p.cursor, p.akey_cap, p.hmark, (uint32_t)(1024 * KPT1), PROBE1_LDS, p.gate,
+                       p.out_off);
     return hipGetLastError();
   });
   if (e != hipSuccess) return e;
@@ -2694,13 +2720,13 @@ static hipError_t launch_probe_t(const LaunchProbe &p, hipStream_t st) {
                      (const unsigned int *)p.gate, p.big_slot, ctl);
   hipLaunchKernelGGL((k_probe_big<K, OFF, 1024>), dim3(p.nitems), dim3(1024), 0, st, (const K *)p.bpart, p.items, p.maps,
                      (const OFF *)p.offA, (const K *)p.keysA, (K *)p.out, p.capacity, p.cursor, small_cap,
-                     (const uint32_t *)p.big_slot, p.hmark, p.gate);
+                     (const uint32_t *)p.big_slot, p.hmark, p.gate, p.out_off);
   hipLaunchKernelGGL((k_probe_big_chunks<K, OFF, 1024>), dim3(PROBE_BIG_GRID), dim3(1024), 0, st, (const K *)p.bpart,
                      p.items, p.maps, (const OFF *)p.offA, (const K *)p.keysA, (K *)p.out, p.capacity, p.cursor,
-                     (const BigCtl *)ctl, p.hmark, p.gate);
+                     (const BigCtl *)ctl, p.hmark, p.gate, p.out_off);
   hipLaunchKernelGGL((k_probe_heavy<K, OFF, 1024>), dim3(p.nitems), dim3(1024), 0, st, (const K *)p.bpart, p.items,
                      (const OFF *)p.offA, (const K *)p.keysA, (K *)p.out, p.capacity, p.cursor,
-                     (const unsigned int *)p.hmark, (const unsigned int *)p.gate);
+                     (const unsigned int *)p.hmark, (const unsigned int *)p.gate, p.out_off);
   return hipGetLastError();
 }
 

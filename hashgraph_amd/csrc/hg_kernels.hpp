@@ -74,6 +74,8 @@ struct LaunchProbe {
   unsigned int *gate;          // 2 zeroed words: [0] k_probe1 left items, [1] a bin was marked heavy
   uint32_t *big_slot;          // nitems words and a final_big_ctl_bytes() area: the list of the
   void *big_ctl;               // oversized buckets (written by the probe itself)
+  const uint64_t *out_off;     // null: out[0 .. capacity) for every table; else table t's
+                               // matches go to out[out_off[t] .. out_off[t+1]) (device array)
 };
 
 struct LaunchJoin {

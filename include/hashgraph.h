@@ -242,6 +242,16 @@ hg_status hg_batch_build(hg_ctx ctx, const hg_batch_desc *bd, const void *d_keys
  * *h_total. */
 hg_status hg_batch_probe(hg_ctx ctx, const hg_batch_desc *bd, const uint32_t *d_offsets, const void *d_csr_keys,
                          const void *d_probe, const uint64_t *h_probe_off, uint64_t *d_counts, uint64_t *h_total);
+/* The same probe with the matches written (hg_probe's output, per table):
+ * table t's matched probe keys (each once per equal key of the table) go to
+ * d_out[h_out_off[t] .. h_out_off[t+1]) in no particular order.  d_counts
+ * and *h_total as above.  A table with more matches than its region gets the
+ * region filled and the call returns HG_ERR_CAPACITY (counts stay exact);
+ * regions from the exclusive prefix sum of a counting call always fit.
+ * h_out_off: host array of n_tables + 1 non-decreasing entries.            */
+hg_status hg_batch_probe_emit(hg_ctx ctx, const hg_batch_desc *bd, const uint32_t *d_offsets, const void *d_csr_keys,
+                              const void *d_probe, const uint64_t *h_probe_off, const uint64_t *h_out_off, void *d_out,
+                              uint64_t *d_counts, uint64_t *h_total);
 
 /* Plans: the batch's descriptors stay resident on the device and its launches
  * (build, then the probe counts when d_probe is given) are captured once into

@@ -204,3 +204,45 @@ def test_plan_replays_value_rich_bins(dev):
         plan.counts.fill_(-1)
         torch.cuda.synchronize()
     plan.close()
+
+
+def test_batch_probe_emit_lists(dev):
+    """The batch probe with its matches written to per-table regions: every
+    table's list equals the oracle's bin join (sorted), including Zipf tables
+    (long bins answered from the value table) and a table whose probe set
+    piles 40000 copies of one key into an oversized bucket (the bucket is cut
+    into tasks over the grid); regions shorter than the counts give
+    HG_ERR_CAPACITY with exact counts."""
+    import hashgraph_amd as hg
+    keys, ko, probe, po = _batch(17, 48, 4, 14)
+    hot = np.uint32(123456789)
+    ea = np.concatenate([np.full(20, hot, dtype=np.uint32), wl.uniform_keys(71, 20000, 4)])
+    eb = np.concatenate([np.full(40000, hot, dtype=np.uint32), ea[5000:9000], wl.uniform_keys(72, 3000, 4)])
+    keys = np.concatenate([keys, ea])
+    probe = np.concatenate([probe, eb])
+    ko = np.append(ko, ko[-1] + len(ea)).astype(np.uint64)
+    po = np.append(po, po[-1] + len(eb)).astype(np.uint64)
+    T = len(ko) - 1
+    n_bins = np.maximum(np.diff(ko), 1)
+    print(f"batch emit T={T} keys={len(keys)} probe={len(probe)}", flush=True)
+    dk, dp = _to_dev(keys, dev), _to_dev(probe, dev)
+    off, csr, off_off = hg.batch_build(dk, ko, n_bins)
+    total, counts = hg.batch_probe(off, csr, ko, n_bins, off_off, dp, po)
+    cnt = counts.cpu().numpy().astype(np.int64)
+    out_off = np.concatenate([[0], np.cumsum(cnt)])
+    total2, counts2, out = hg.batch_probe_emit(off, csr, ko, n_bins, off_off, dp, po, out_off)
+    assert total2 == total and np.array_equal(counts2.cpu().numpy(), cnt)
+    outn = out.cpu().numpy().view(np.uint32)
+    for t in range(T):
+        a, b, M = keys[ko[t]:ko[t + 1]], probe[po[t]:po[t + 1]], int(n_bins[t])
+        oa, ka = o.build_csr(a, M)
+        ob, kb = o.build_csr(b, M)
+        c, lst = o.intersect(oa, ka, ob, kb)
+        assert c == cnt[t], f"table {t} count"
+        assert np.array_equal(np.sort(outn[out_off[t]:out_off[t + 1]]), np.sort(lst)), f"table {t} list"
+    short = np.concatenate([[0], np.cumsum(cnt // 2)])
+    with pytest.raises(hg.HGError):
+        hg.batch_probe_emit(off, csr, ko, n_bins, off_off, dp, po, short)
+    # the counts of the refused call are still exact
+    _, counts3 = hg.batch_probe(off, csr, ko, n_bins, off_off, dp, po)
+    assert np.array_equal(counts3.cpu().numpy(), cnt)
