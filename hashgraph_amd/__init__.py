@@ -17,7 +17,7 @@ from . import _lib
 from ._lib import HG_HASH_MURMUR3, HG_HASH_RMI, HGError, check
 
 __all__ = ["Context", "build", "probe", "intersect", "intersect_rle", "exclusive_scan", "hash_bins", "find_splits",
-           "build_partitioned", "rmi_train", "rmi_model_bytes", "batch_build", "batch_probe", "batch_probe_emit", "HGError",
+           "build_partitioned", "rmi_train", "rmi_train_batch", "rmi_model_bytes", "batch_build", "batch_probe", "batch_probe_emit", "HGError",
            "HG_HASH_MURMUR3", "HG_HASH_RMI", "version", "table_bins", "BatchPlan"]
 
 _ctx = {}
@@ -278,6 +278,33 @@ def rmi_train(keys, n_bins: int, n_sample: int | None = None, seed: int = 7, n_l
     return model, bool(use.value), {
         "sample_bins": rep.sample_bins, "max_bin_rmi": rep.max_bin_rmi, "max_bin_murmur": rep.max_bin_murmur,
         "mean_probe_rmi": rep.mean_probe_rmi, "mean_probe_murmur": rep.mean_probe_murmur}
+
+
+def rmi_train_batch(keys, key_off, n_bins, n_sample, seeds, n_leaves: int = 4096, report: bool = False):
+    """One learned model per table (table t = keys[key_off[t]:key_off[t+1]]),
+    the models rmi_train would fit for (n_sample[t], seeds[t], n_bins[t]).
+    Returns (models uint8 tensor, model t at t * rmi_model_bytes(n_leaves),
+    list of use flags) or, with report=True, also the list of report dicts."""
+    torch = _torch()
+    _check_dev(keys)
+    c = context(keys.device.index)
+    T = len(n_bins)
+    mb = rmi_model_bytes(n_leaves)
+    models = torch.zeros(max(T * mb, 1), dtype=torch.uint8, device=keys.device)
+    ko = (ctypes.c_uint64 * (T + 1))(*[int(x) for x in key_off])
+    nb = (ctypes.c_uint64 * T)(*[int(x) for x in n_bins])
+    ns = (ctypes.c_uint64 * T)(*[int(x) for x in n_sample])
+    sd = (ctypes.c_uint64 * T)(*[int(x) for x in seeds])
+    use = (ctypes.c_uint32 * T)()
+    rep = (_lib.RmiReport * T)()
+    check(c.lib.hg_rmi_train_batch(c.bind_stream(), T, _key_bytes(keys), _p(keys), ko, nb, ns, sd, n_leaves,
+                                   _p(models), use, rep), "hg_rmi_train_batch")
+    flags = [bool(use[t]) for t in range(T)]
+    if not report:
+        return models, flags
+    reps = [{"sample_bins": r.sample_bins, "max_bin_rmi": r.max_bin_rmi, "max_bin_murmur": r.max_bin_murmur,
+             "mean_probe_rmi": r.mean_probe_rmi, "mean_probe_murmur": r.mean_probe_murmur} for r in rep]
+    return models, flags, reps
 
 
 def table_bins(keys, n_bins: int, hash_kind: int = HG_HASH_MURMUR3, rmi_model=None):

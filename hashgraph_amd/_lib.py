@@ -23,7 +23,7 @@ HG_HASH_RMI = 1
 EXPORTS = [
     "hg_ctx_create", "hg_ctx_destroy", "hg_ctx_set_stream", "hg_ctx_reserve", "hg_scratch_bytes", "hg_last_error",
     "hg_version", "hg_ctx_profile", "hg_ctx_profile_read", "hg_build", "hg_intersect", "hg_probe", "hg_intersect_rle", "hg_find_splits",
-    "hg_build_partitioned", "hg_rmi_model_bytes", "hg_rmi_train", "hg_batch_build", "hg_batch_probe", "hg_batch_probe_emit",
+    "hg_build_partitioned", "hg_rmi_model_bytes", "hg_rmi_train", "hg_rmi_train_batch", "hg_batch_build", "hg_batch_probe", "hg_batch_probe_emit",
     "hg_exclusive_scan", "hg_hash_bins", "hg_table_bins", "hg_batch_plan_create", "hg_batch_plan_run",
     "hg_batch_plan_query", "hg_batch_plan_destroy", "hg_build_noalloc", "hg_build_indexed",
     "hg_build_partitioned_indexed",
@@ -101,6 +101,8 @@ def load(path: str = LIB_PATH):
             "hg_build_partitioned_indexed": (i32, [vp, P(TableDesc), vp, P(ctypes.c_int64), u32, vp, vp, vp, P(u64)]),
             "hg_rmi_model_bytes": (ctypes.c_size_t, [u32]),
             "hg_rmi_train": (i32, [vp, vp, u64, u32, u64, u64, u64, u32, vp, P(u32), P(RmiReport)]),
+            "hg_rmi_train_batch": (i32, [vp, u32, u32, vp, P(u64), P(u64), P(u64), P(u64), u32, vp, P(u32),
+                                         P(RmiReport)]),
             "hg_batch_build": (i32, [vp, P(BatchDesc), vp, vp, vp]),
             "hg_batch_probe": (i32, [vp, P(BatchDesc), vp, vp, vp, P(u64), vp, P(u64)]),
             "hg_batch_probe_emit": (i32, [vp, P(BatchDesc), vp, vp, vp, P(u64), P(u64), vp, vp, P(u64)]),

@@ -1133,6 +1133,106 @@ hg_status hg_rmi_train(hg_ctx c, const void *d_keys, uint64_t n_keys, uint32_t k
   return HG_OK;
 }
 
+hg_status hg_rmi_train_batch(hg_ctx c, uint32_t n_tables, uint32_t key_bytes, const void *d_keys,
+                             const uint64_t *h_key_off, const uint64_t *h_n_bins, const uint64_t *h_n_sample,
+                             const uint64_t *h_seed, uint32_t n_leaves, void *d_models, uint32_t *h_use,
+                             hg_rmi_report *h_report) {
+  if (!c) return fail(HG_ERR_INVALID_ARG, "null ctx");
+  if (key_bytes != 4 && key_bytes != 8) return fail(HG_ERR_INVALID_ARG, "key_bytes must be 4 or 8");
+  if (n_leaves == 0 || n_leaves > 65536) return fail(HG_ERR_INVALID_ARG, "n_leaves must be in [1, 65536]");
+  if (!n_tables) return HG_OK;
+  if (!h_key_off || !h_n_bins || !h_n_sample || !h_seed || !d_models || !h_use || !d_keys)
+    return fail(HG_ERR_INVALID_ARG, "null argument");
+  for (uint32_t t = 0; t < n_tables; t++) {
+    if (h_key_off[t + 1] < h_key_off[t]) return fail(HG_ERR_INVALID_ARG, "key offsets must not decrease");
+    if (h_n_bins[t] == 0 || h_n_bins[t] >= (1ull << 32)) return fail(HG_ERR_INVALID_ARG, "n_bins must be in [1, 2^32)");
+    if (h_n_sample[t] == 0 || h_n_sample[t] >= (1ull << 31))
+      return fail(HG_ERR_INVALID_ARG, "n_sample must be in [1, 2^31)");
+  }
+  DeviceGuard g(c->device);
+  const size_t mb = hg_rmi_model_bytes(n_leaves);
+  const uint32_t kcap = RMI_ONE_MAX_KEY_BYTES / key_bytes;
+  std::vector<RmiBatchItem> items;
+  std::vector<uint32_t> big;  // tables for the multi-kernel trainer
+  uint32_t keys_cap = 16, cnt_cap = 4 * n_leaves;
+  for (uint32_t t = 0; t < n_tables; t++) {
+    h_use[t] = 0;
+    if (h_report) h_report[t] = hg_rmi_report{};
+    const uint64_t n = h_key_off[t + 1] - h_key_off[t], ns = h_n_sample[t];
+    if (n == 0) continue;  // no keys: murmur3 (the model is not written)
+    long double mp = (long double)h_n_bins[t] * (long double)ns / (long double)n;
+    uint64_t Mp = (uint64_t)(mp + 0.5L);
+    Mp = std::max<uint64_t>(1, std::min<uint64_t>(Mp, h_n_bins[t]));
+    uint64_t np2 = 16;
+    while (np2 < ns) np2 <<= 1;
+    if (np2 > kcap || Mp > RMI_ONE_MAX_CHECK_BINS || n_leaves > RMI_ONE_MAX_LEAVES) {
+      big.push_back(t);
+      continue;
+    }
+    RmiBatchItem it{};
+    it.key_begin = h_key_off[t];
+    it.n_keys = n;
+    it.s0 = splitmix_mix(h_seed[t]);
+    it.model_off = (uint64_t)t * mb;
+    it.map = make_map(h_n_bins[t], 0, nullptr);
+    it.ns = (uint32_t)ns;
+    it.np2 = (uint32_t)np2;
+    it.Mp = (uint32_t)Mp;
+    it.table = t;
+    items.push_back(it);
+    keys_cap = std::max<uint32_t>(keys_cap, (uint32_t)np2);
+    cnt_cap = std::max<uint32_t>(cnt_cap, (uint32_t)Mp);
+  }
+  if (!items.empty()) {
+    Blob bl;
+    const size_t o_items = bl.put(items);
+    Plan P;
+    const size_t i_blob = P.add(bl.data.size()), i_use = P.add((size_t)n_tables * 4),
+                 i_rep = P.add((size_t)n_tables * 32);
+    HG_TRY(arena_ensure(c, 2, P.total()));
+    Carve cv;
+    cv.init(c, 2, P);
+    char *blob = cv.get<char>(i_blob);
+    HG_TRY(upload(c, blob, bl.data.data(), bl.data.size()));
+    LaunchRmiBatch b{};
+    b.key_bytes = (int)key_bytes;
+    b.n_items = (uint32_t)items.size();
+    b.n_leaves = n_leaves;
+    b.keys_cap = keys_cap;
+    b.cnt_cap = cnt_cap;
+    b.keys = d_keys;
+    b.items = (const RmiBatchItem *)(blob + o_items);
+    b.models = d_models;
+    b.use = cv.get<uint32_t>(i_use);
+    b.report = cv.get<unsigned long long>(i_rep);
+    {
+      Prof pf(c, "rmi_train");
+      HG_HIP(launch_rmi_train_batch(b, c->stream));
+    }
+    std::vector<uint32_t> use(n_tables);
+    std::vector<unsigned long long> rep((size_t)n_tables * 4);
+    HG_TRY(sync_read(c, use.data(), b.use, (size_t)n_tables * 4));
+    if (h_report) HG_TRY(sync_read(c, rep.data(), b.report, (size_t)n_tables * 32));
+    for (const RmiBatchItem &it : items) {
+      h_use[it.table] = use[it.table];
+      if (h_report) {
+        hg_rmi_report &r = h_report[it.table];
+        r.sample_bins = it.Mp;
+        r.max_bin_rmi = rep[(size_t)it.table * 4];
+        r.max_bin_murmur = rep[(size_t)it.table * 4 + 2];
+        r.mean_probe_rmi = (double)rep[(size_t)it.table * 4 + 1] / (double)it.ns;
+        r.mean_probe_murmur = (double)rep[(size_t)it.table * 4 + 3] / (double)it.ns;
+      }
+    }
+  }
+  // samples beyond one workgroup's LDS: the multi-kernel trainer, table by table
+  for (uint32_t t : big)
+    HG_TRY(hg_rmi_train(c, (const char *)d_keys + h_key_off[t] * key_bytes, h_key_off[t + 1] - h_key_off[t], key_bytes,
+                        h_n_sample[t], h_seed[t], h_n_bins[t], n_leaves, (char *)d_models + (size_t)t * mb, &h_use[t],
+                        h_report ? &h_report[t] : nullptr));
+  return HG_OK;
+}
+
 hg_status hg_hash_bins(hg_ctx c, const void *d_keys, uint64_t n, uint32_t key_bytes, uint64_t n_bins, uint32_t *d_bins) {
   if (!c) return fail(HG_ERR_INVALID_ARG, "null ctx");
   if (key_bytes != 4 && key_bytes != 8) return fail(HG_ERR_INVALID_ARG, "key_bytes must be 4 or 8");

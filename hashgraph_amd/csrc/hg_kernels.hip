@@ -2965,6 +2965,7 @@ hipError_t set_kernel_attributes() {
   HG_SET(&k_join<uint64_t, uint64_t, 256, 4096>);
   HG_SET(&k_range_hist<uint32_t>);
   HG_SET(&k_range_hist<uint64_t>);
+  HG_SET_D(set_rmi_kernel_attributes());
 #undef HG_SET
 #undef HG_SET_D
   return e;

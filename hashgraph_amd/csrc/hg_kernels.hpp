@@ -165,6 +165,39 @@ struct LaunchRmi {
 };
 
 hipError_t launch_rmi_train(const LaunchRmi &r, hipStream_t st);
+
+// Batched training, one workgroup per table with everything in LDS: samples
+// up to RMI_ONE_MAX_KEY_BYTES of keys, up to RMI_ONE_MAX_LEAVES leaves and
+// RMI_ONE_MAX_CHECK_BINS bins in the sampled check.
+constexpr uint32_t RMI_ONE_MAX_KEY_BYTES = 64 * 1024;
+constexpr uint32_t RMI_ONE_MAX_LEAVES = 1024;
+constexpr uint32_t RMI_ONE_MAX_CHECK_BINS = 12288;
+constexpr uint32_t RMI_ONE_MAX_BLOCKS = 64;  // moment blocks of 256 keys
+struct RmiBatchItem {
+  uint64_t key_begin;  // the table's first key
+  uint64_t n_keys;
+  uint64_t s0;         // mix64(seed)
+  uint64_t model_off;  // byte offset of the table's model
+  BinMap map;          // murmur3 map of the table's bins
+  uint32_t ns;         // sample keys
+  uint32_t np2;        // ns rounded up to a power of two
+  uint32_t Mp;         // bins of the sampled check
+  uint32_t table;      // index of use[] and report[]
+};
+struct LaunchRmiBatch {
+  int key_bytes;
+  uint32_t n_items, n_leaves;
+  uint32_t keys_cap;   // max np2 of the items
+  uint32_t cnt_cap;    // max(4 * n_leaves, max Mp)
+  const void *keys;
+  const RmiBatchItem *items;
+  void *models;
+  uint32_t *use;                  // per table: 1 = the model passed the check
+  unsigned long long *report;     // per table: 4 figures, or null
+};
+size_t rmi_one_lds(uint32_t keys_cap, int key_bytes, uint32_t L, uint32_t cnt_cap);
+hipError_t launch_rmi_train_batch(const LaunchRmiBatch &b, hipStream_t st);
+hipError_t set_rmi_kernel_attributes();
 uint32_t rmi_moment_blocks(uint64_t n_sample);
 uint32_t rmi_sort_tiles(uint64_t n_sample);
 hipError_t launch_rle(const LaunchRle &r, hipStream_t st);

@@ -130,14 +130,17 @@ struct RootCand {
   uint32_t pad;
 };
 
-// Per block: sums of x, y, x^2, x*y for both features (8 doubles).
+// Sums of x, y, x^2, x*y for both features (8 doubles) of moment block blk
+// of nblk (elements blk*256 + t, stepping nblk*256), in v of every thread
+// (256 threads; fixed tree order).  Shared by the multi-kernel and the
+// one-workgroup trainers, so both give the same model bit for bit.
 template <typename K>
-__global__ __launch_bounds__(256) void k_rmi_root_moments(const K *__restrict__ sorted, uint64_t n, double yscale,
-                                                          double *__restrict__ part) {
-  __shared__ double s_red[8 * 256];
+__device__ __forceinline__ void root_moments_block(const K *sorted, uint64_t n, double yscale, uint64_t blk,
+                                                   uint64_t nblk, double (&v)[8], double *s_red) {
   const uint64_t kmin = (uint64_t)sorted[0];
-  double v[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) {
+#pragma unroll
+  for (int j = 0; j < 8; j++) v[j] = 0.0;
+  for (uint64_t i = blk * 256 + threadIdx.x; i < n; i += nblk * 256) {
     const double y = ((double)i + 0.5) * yscale;
 #pragma unroll
     for (uint32_t f = 0; f < 2; f++) {
@@ -149,6 +152,15 @@ __global__ __launch_bounds__(256) void k_rmi_root_moments(const K *__restrict__ 
     }
   }
   block_sum_d<8>(v, s_red);
+}
+
+// Per block: sums of x, y, x^2, x*y for both features (8 doubles).
+template <typename K>
+__global__ __launch_bounds__(256) void k_rmi_root_moments(const K *__restrict__ sorted, uint64_t n, double yscale,
+                                                          double *__restrict__ part) {
+  __shared__ double s_red[8 * 256];
+  double v[8];
+  root_moments_block<K>(sorted, n, yscale, blockIdx.x, gridDim.x, v, s_red);
   if (threadIdx.x < 8) part[(uint64_t)blockIdx.x * 8 + threadIdx.x] = v[threadIdx.x];
 }
 
@@ -163,11 +175,11 @@ __device__ __forceinline__ void lsq_line(double n, double sx, double sy, double 
   alpha = my - beta * mx;
 }
 
+// The four root candidates and the header (key range, sizes) from the moment
+// blocks' partial sums (256 threads; thread 0 writes cand and h).
 template <typename K>
-__global__ __launch_bounds__(256) void k_rmi_root_fit(const K *__restrict__ sorted, uint64_t n, const double *part,
-                                                      uint32_t nblk, uint32_t L, uint32_t M, RmiHeader *__restrict__ h,
-                                                      RootCand *__restrict__ cand) {
-  __shared__ double s_red[8 * 256];
+__device__ __forceinline__ void root_fit_from(const K *sorted, uint64_t n, const double *part, uint32_t nblk,
+                                              uint32_t L, uint32_t M, RmiHeader *h, RootCand *cand, double *s_red) {
   double v[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   for (uint32_t b = threadIdx.x; b < nblk; b += 256)
 #pragma unroll
@@ -191,6 +203,14 @@ __global__ __launch_bounds__(256) void k_rmi_root_fit(const K *__restrict__ sort
     hd.n_bins = M;
     *h = hd;
   }
+}
+
+template <typename K>
+__global__ __launch_bounds__(256) void k_rmi_root_fit(const K *__restrict__ sorted, uint64_t n, const double *part,
+                                                      uint32_t nblk, uint32_t L, uint32_t M, RmiHeader *__restrict__ h,
+                                                      RootCand *__restrict__ cand) {
+  __shared__ double s_red[8 * 256];
+  root_fit_from<K>(sorted, n, part, nblk, L, M, h, cand, s_red);
 }
 
 // Leaf loads of the sample under every candidate: cnt[c * L + leaf].
@@ -253,12 +273,8 @@ __global__ __launch_bounds__(256) void k_rmi_bounds(const K *__restrict__ sorted
 // than 8 keys or a single value) also takes the sample keys on either side,
 // so that its line follows its neighbourhood.
 template <typename K>
-__global__ __launch_bounds__(256) void k_rmi_leaf_fit(const K *__restrict__ sorted, uint64_t n,
-                                                      const uint32_t *__restrict__ start, uint32_t M,
-                                                      RmiLeaf *__restrict__ leaves) {
-  __shared__ double s_red[4 * 256];
-  const uint32_t l = blockIdx.x;
-  const uint32_t a = start[l], b = start[l + 1];
+__device__ __forceinline__ void leaf_fit_one(const K *sorted, uint64_t n, uint32_t a, uint32_t b, uint32_t M,
+                                             RmiLeaf *leaf, double *s_red) {
   const double yscale = (double)M / (double)n;
   const uint64_t x0 = (uint64_t)sorted[a < n ? a : n - 1];
   const bool sparse = b - a < 8 || (uint64_t)sorted[b - 1] == x0;
@@ -289,8 +305,17 @@ __global__ __launch_bounds__(256) void k_rmi_leaf_fit(const K *__restrict__ sort
     lf.x0 = x0;
     lf.slope = (float)beta;
     lf.anchor = (uint32_t)an;
-    leaves[l] = lf;
+    *leaf = lf;
   }
+}
+
+template <typename K>
+__global__ __launch_bounds__(256) void k_rmi_leaf_fit(const K *__restrict__ sorted, uint64_t n,
+                                                      const uint32_t *__restrict__ start, uint32_t M,
+                                                      RmiLeaf *__restrict__ leaves) {
+  __shared__ double s_red[4 * 256];
+  const uint32_t l = blockIdx.x;
+  leaf_fit_one<K>(sorted, n, start[l], start[l + 1], M, &leaves[l], s_red);
 }
 
 // Both hashes bin the sample; bins are scaled to Mp = M * n_sample / n_keys
@@ -336,6 +361,162 @@ __global__ __launch_bounds__(256) void k_rmi_check_reduce(const uint32_t *__rest
     atomicAdd(&out[3], sm);
   }
   (void)s_t;
+}
+
+// ---------------------------------------------------------------------------
+// Batched training: one 256-thread workgroup per table, the whole pipeline
+// in LDS (sample, bitonic sort, root moments and candidates, leaf loads and
+// pick, leaf bounds, leaf fits, sampled check).  The floating-point steps
+// are the multi-kernel trainer's own device functions in the same order
+// (moment blocks of 256 keys, fixed tree sums), so a table's model is the
+// one hg_rmi_train fits, bit for bit.
+// ---------------------------------------------------------------------------
+template <typename K>
+__global__ __launch_bounds__(256) void k_rmi_train_one(const K *__restrict__ keys,
+                                                       const RmiBatchItem *__restrict__ items, uint32_t L,
+                                                       uint32_t keys_cap, uint32_t cnt_cap, uint8_t *__restrict__ models,
+                                                       uint32_t *__restrict__ use,
+                                                       unsigned long long *__restrict__ report) {
+  extern __shared__ __align__(16) uint8_t smem[];
+  __shared__ double s_red[8 * 256];
+  __shared__ double s_part[RMI_ONE_MAX_BLOCKS * 8];
+  __shared__ RootCand s_cand[4];
+  __shared__ RmiHeader s_h;
+  __shared__ unsigned long long s_t[256 / 64 + 1];
+  __shared__ unsigned long long s_sc[4];
+  const RmiBatchItem it = items[blockIdx.x];
+  K *s_keys = (K *)smem;                                             // keys_cap
+  RmiLeaf *s_leaf = (RmiLeaf *)(smem + (size_t)keys_cap * sizeof(K));  // L
+  uint32_t *s_start = (uint32_t *)(s_leaf + L);                      // L + 1 (rounded to 4)
+  uint32_t *s_cnt = s_start + ((L + 4) & ~3u);                       // cnt_cap
+  const uint32_t ns = it.ns, M = it.map.M;
+  // 1. sample (as k_sample) padded to a power of two with the largest key
+  const K *kp = keys + it.key_begin;
+  for (uint32_t i = threadIdx.x; i < it.np2; i += 256) {
+    K v = (K)~(K)0;
+    if (i < ns) v = kp[__umul64hi(mix64(it.s0 + (uint64_t)(i + 1) * SPLITMIX_GAMMA), it.n_keys)];
+    s_keys[i] = v;
+  }
+  __syncthreads();
+  // 2. bitonic sort (the sorted sample is unique whatever the method)
+  for (uint32_t k = 2; k <= it.np2; k <<= 1)
+    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+      for (uint32_t i = threadIdx.x; i < it.np2; i += 256) {
+        const uint32_t p = i ^ j;
+        if (p > i) {
+          const K a = s_keys[i], b = s_keys[p];
+          if ((a > b) == ((i & k) == 0)) {
+            s_keys[i] = b;
+            s_keys[p] = a;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  // 3. root candidates (k_rmi_root_moments + k_rmi_root_fit)
+  const uint32_t nblk = (ns + 255) / 256;
+  const double yscale = (double)L / (double)ns;
+  for (uint32_t b = 0; b < nblk; b++) {
+    double v[8];
+    root_moments_block<K>(s_keys, ns, yscale, b, nblk, v, s_red);
+    if (threadIdx.x < 8) s_part[b * 8 + threadIdx.x] = v[threadIdx.x];
+  }
+  __syncthreads();
+  root_fit_from<K>(s_keys, ns, s_part, nblk, L, M, &s_h, s_cand, s_red);
+  __syncthreads();
+  // 4. leaf loads of the candidates and the pick (k_rmi_root_score + pick)
+  for (uint32_t i = threadIdx.x; i < 4 * L; i += 256) s_cnt[i] = 0;
+  __syncthreads();
+  {
+    const uint64_t kmin = (uint64_t)s_keys[0];
+    for (uint32_t i = threadIdx.x; i < ns; i += 256) {
+      const uint64_t k = (uint64_t)s_keys[i];
+      const float fr = rmi_feature(k, kmin, 0), fl = rmi_feature(k, kmin, 1);
+#pragma unroll
+      for (int j = 0; j < 4; j++)
+        atomicAdd(&s_cnt[(uint32_t)j * L + rmi_leaf_at(s_cand[j].kind ? fl : fr, s_cand[j].slope, s_cand[j].icpt, L)],
+                  1u);
+    }
+  }
+  __syncthreads();
+  for (int j = 0; j < 4; j++) {
+    unsigned long long q = 0;
+    for (uint32_t l = threadIdx.x; l < L; l += 256) {
+      const unsigned long long c = s_cnt[(uint32_t)j * L + l];
+      q += c * c;
+    }
+    q = block_sum<256, unsigned long long>(q, s_t);
+    if (threadIdx.x == 0) s_sc[j] = q;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int best = 0;
+    for (int j = 1; j < 4; j++)
+      if (s_sc[j] < s_sc[best]) best = j;
+    s_h.root_kind = s_cand[best].kind;
+    s_h.root_slope = s_cand[best].slope;
+    s_h.root_icpt = s_cand[best].icpt;
+  }
+  __syncthreads();
+  // 5. leaf bounds of the sorted sample (k_rmi_bounds)
+  {
+    const RmiHeader h = s_h;
+    for (uint32_t i = threadIdx.x; i <= ns; i += 256) {
+      const int64_t cur = i < ns ? (int64_t)rmi_leaf((uint64_t)s_keys[i], h) : (int64_t)h.n_leaves;
+      const int64_t prev = i > 0 ? (int64_t)rmi_leaf((uint64_t)s_keys[i - 1], h) : -1;
+      for (int64_t l = prev + 1; l <= cur; l++) s_start[l] = i;
+    }
+  }
+  __syncthreads();
+  // 6. leaf fits (k_rmi_leaf_fit, one leaf after the other)
+  for (uint32_t l = 0; l < L; l++) leaf_fit_one<K>(s_keys, ns, s_start[l], s_start[l + 1], M, &s_leaf[l], s_red);
+  __syncthreads();
+  // 7. the model to global memory
+  {
+    uint32_t *dst = (uint32_t *)(models + it.model_off);
+    const uint32_t *hw = (const uint32_t *)&s_h, *lw = (const uint32_t *)s_leaf;
+    const uint32_t hn = (uint32_t)(sizeof(RmiHeader) / 4), ln = L * (uint32_t)(sizeof(RmiLeaf) / 4);
+    for (uint32_t i = threadIdx.x; i < hn + ln; i += 256) dst[i] = i < hn ? hw[i] : lw[i - hn];
+  }
+  // 8. the sampled check (k_rmi_check + reduce): learned bins, then murmur3
+  unsigned long long res[4];
+  for (int pass = 0; pass < 2; pass++) {
+    for (uint32_t i = threadIdx.x; i < it.Mp; i += 256) s_cnt[i] = 0;
+    __syncthreads();
+    const RmiHeader h = s_h;
+    for (uint32_t i = threadIdx.x; i < ns; i += 256) {
+      const K k = s_keys[i];
+      const uint64_t b = pass == 0 ? (uint64_t)rmi_leaf_bin((uint64_t)k, s_leaf[rmi_leaf((uint64_t)k, h)], M)
+                                   : (uint64_t)bin_of<K, HK_MURMUR>(k, it.map);
+      atomicAdd(&s_cnt[b * it.Mp / M], 1u);
+    }
+    __syncthreads();
+    unsigned long long mx = 0, sq = 0;
+    for (uint32_t i = threadIdx.x; i < it.Mp; i += 256) {
+      const unsigned long long c = s_cnt[i];
+      mx = c > mx ? c : mx;
+      sq += c * c;
+    }
+    sq = block_sum<256, unsigned long long>(sq, s_t);
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) {
+      const unsigned long long x = __shfl_xor(mx, d, 64);
+      mx = x > mx ? x : mx;
+    }
+    if ((threadIdx.x & 63) == 0) s_t[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    mx = s_t[0];
+    for (int w = 1; w < 4; w++) mx = s_t[w] > mx ? s_t[w] : mx;
+    res[2 * pass] = mx;
+    res[2 * pass + 1] = sq;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    use[it.table] = (res[0] <= res[2] && res[1] <= res[3]) ? 1u : 0u;
+    if (report)
+      for (int j = 0; j < 4; j++) report[(uint64_t)it.table * 4 + j] = res[j];
+  }
+  (void)cnt_cap;
 }
 
 static uint32_t grid_for(uint64_t n, uint32_t cap) {
@@ -394,6 +575,38 @@ static hipError_t launch_rmi_train_t(const LaunchRmi &r, hipStream_t st) {
 
 hipError_t launch_rmi_train(const LaunchRmi &r, hipStream_t st) {
   return r.key_bytes == 4 ? launch_rmi_train_t<uint32_t>(r, st) : launch_rmi_train_t<uint64_t>(r, st);
+}
+
+size_t rmi_one_lds(uint32_t keys_cap, int key_bytes, uint32_t L, uint32_t cnt_cap) {
+  return (size_t)keys_cap * key_bytes + (size_t)L * sizeof(RmiLeaf) + (size_t)((L + 4) & ~3u) * 4 + (size_t)cnt_cap * 4;
+}
+
+hipError_t launch_rmi_train_batch(const LaunchRmiBatch &b, hipStream_t st) {
+  if (!b.n_items) return hipSuccess;
+  const size_t lds = rmi_one_lds(b.keys_cap, b.key_bytes, b.n_leaves, b.cnt_cap);
+  if (b.key_bytes == 4)
+    hipLaunchKernelGGL((k_rmi_train_one<uint32_t>), dim3(b.n_items), dim3(256), lds, st, (const uint32_t *)b.keys, b.items,
+                       b.n_leaves, b.keys_cap, b.cnt_cap, (uint8_t *)b.models, b.use, b.report);
+  else
+    hipLaunchKernelGGL((k_rmi_train_one<uint64_t>), dim3(b.n_items), dim3(256), lds, st, (const uint64_t *)b.keys, b.items,
+                       b.n_leaves, b.keys_cap, b.cnt_cap, (uint8_t *)b.models, b.use, b.report);
+  return hipGetLastError();
+}
+
+hipError_t set_rmi_kernel_attributes() {
+  for (const void *fn : {(const void *)&k_rmi_train_one<uint32_t>, (const void *)&k_rmi_train_one<uint64_t>}) {
+    hipFuncAttributes fa;
+    hipError_t e = hipFuncGetAttributes(&fa, fn);
+    if (e != hipSuccess) return e;
+    int dev = 0, lim = 0;
+    (void)hipGetDevice(&dev);
+    e = hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+    if (e != hipSuccess) return e;
+    const int room = lim - (int)fa.sharedSizeBytes;
+    if (room > 0) e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, room);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 uint32_t rmi_moment_blocks(uint64_t n_sample) { return grid_for(n_sample, 1024); }

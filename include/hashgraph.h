@@ -196,7 +196,7 @@ hg_status hg_build_partitioned_indexed(hg_ctx ctx, const hg_table_desc *desc, co
  * [0, n_bins - 1] (beyond the range the model was trained on) takes murmur3's
  * bin, so out-of-range keys spread over the table instead of filling an edge
  * bin.  Layout: 32-byte header, then
- * n_leaves + 1 leaves of 16 bytes (the last is a sentinel).
+ * n_leaves leaves of 16 bytes.
  *
  * hg_rmi_train draws n_sample keys of d_keys (with replacement, SplitMix64
  * stream `seed`), sorts them, fits root and leaves by least squares in FP64
@@ -217,6 +217,19 @@ size_t hg_rmi_model_bytes(uint32_t n_leaves);
 hg_status hg_rmi_train(hg_ctx ctx, const void *d_keys, uint64_t n_keys, uint32_t key_bytes, uint64_t n_sample,
                        uint64_t seed, uint64_t n_bins, uint32_t n_leaves, void *d_model, uint32_t *h_use_rmi,
                        hg_rmi_report *h_report);
+/* One model per table of a batch: table t's keys are
+ * d_keys[h_key_off[t] .. h_key_off[t+1]), its model (hg_rmi_model_bytes(n_leaves)
+ * bytes) goes to d_models + t * hg_rmi_model_bytes(n_leaves), and h_use[t] /
+ * h_report[t] (optional) are hg_rmi_train's outputs for (h_n_sample[t],
+ * h_seed[t], h_n_bins[t]).  The models equal hg_rmi_train's bit for bit.
+ * Tables whose sample fits one workgroup's LDS (up to 16384 uint32 or 8192
+ * uint64 keys, up to 1024 leaves, up to 12288 check bins) train together in
+ * one launch; larger ones table by table.  An empty table gets h_use[t] = 0
+ * and no model.                                                           */
+hg_status hg_rmi_train_batch(hg_ctx ctx, uint32_t n_tables, uint32_t key_bytes, const void *d_keys,
+                             const uint64_t *h_key_off, const uint64_t *h_n_bins, const uint64_t *h_n_sample,
+                             const uint64_t *h_seed, uint32_t n_leaves, void *d_models, uint32_t *h_use,
+                             hg_rmi_report *h_report);
 
 /* ---- batched form -----------------------------------------------------------
  * No counterpart in the reference.  Builds n_tables independent tables (and

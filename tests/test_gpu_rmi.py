@@ -200,3 +200,50 @@ def test_build_and_probe_learned(dev, kind, kb):
     cntm, outm = hg.probe(offm, csrm, M, dB)
     assert cnt == cntm == o.bag_intersect_count(A, B)
     assert np.array_equal(np.sort(out.cpu().numpy().view(A.dtype)), np.sort(outm.cpu().numpy().view(A.dtype)))
+
+
+@pytest.mark.parametrize("kb", [4, 8])
+def test_batch_training_equals_single_training(dev, kb):
+    """hg_rmi_train_batch (one workgroup per table, everything in LDS) fits
+    the models hg_rmi_train fits, byte for byte, with the same fallback
+    decisions and sampled figures: uniform, Zipf and clustered tables, tiny
+    tables, an empty table, a leaf count that leaves most leaves empty, and a
+    sample too large for one workgroup (trained by the multi-kernel path)."""
+    import torch
+    import hashgraph_amd as hg
+    rng = np.random.default_rng(90 + kb)
+    parts, ns = [], []
+    sizes = [1, 3, 0, 500, 5000, 40000, 200000, 1 << 20]
+    for i, n in enumerate(sizes):
+        if i % 3 == 0:
+            a = wl.uniform_keys(300 + i, n, kb)
+        elif i % 3 == 1:
+            a = (wl.zipf_keys(300 + i, n, 1.2) & np.uint64(0xFFFFFFFF if kb == 4 else (1 << 64) - 1)).astype(
+                np.uint32 if kb == 4 else np.uint64)
+        else:
+            c = rng.integers(0, 2 ** (8 * kb - 1), 8, dtype=np.uint64)
+            a = (c[rng.integers(0, 8, n)] + rng.integers(0, 1 << 20, n, dtype=np.uint64)).astype(
+                np.uint32 if kb == 4 else np.uint64)
+        parts.append(a)
+        ns.append(max(1, min(n, max(n // 100, 1024))) if n else 1)
+    ns[-1] = 20000  # beyond one workgroup for either key width
+    keys = np.concatenate(parts)
+    ko = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
+    n_bins = [max(n, 1) for n in sizes]
+    seeds = [7 + 13 * t for t in range(len(sizes))]
+    L = 256
+    dk = _to_dev(keys, dev)
+    print(f"rmi batch kb={kb} tables={len(sizes)}", flush=True)
+    models, use, reps = hg.rmi_train_batch(dk, ko, n_bins, ns, seeds, n_leaves=L, report=True)
+    mb = hg.rmi_model_bytes(L)
+    got = models.cpu().numpy()
+    for t, n in enumerate(sizes):
+        if n == 0:
+            assert not use[t]
+            continue
+        m, u, r = hg.rmi_train(dk[int(ko[t]):int(ko[t + 1])], n_bins[t], n_sample=ns[t], seed=seeds[t], n_leaves=L,
+                               report=True)
+        assert np.array_equal(got[t * mb:(t + 1) * mb], m.cpu().numpy()), f"table {t} model"
+        assert use[t] == u, f"table {t} decision"
+        assert reps[t] == r, f"table {t} report"
+    torch.cuda.synchronize()
