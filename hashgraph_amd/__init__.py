@@ -400,10 +400,13 @@ def batch_probe_emit(offsets, csr_keys, key_off, n_bins, off_off, probe, probe_o
 
 class BatchPlan:
     """A captured batch (hg_batch_plan): build of every table, then the
-    per-table probe counts when probe keys are given, replayed by run()."""
+    per-table probe counts when probe keys are given, replayed by run().
+    train=True trains every table's learned model inside the graph on each
+    replay (rmi_models: the models' buffer, rmi_leaves leaves each; the
+    sampled check picks learned or murmur3 per table, hash_kinds is ignored)."""
 
     def __init__(self, keys, key_off, n_bins, probe=None, probe_off=None, hash_kinds=None, rmi_models=None,
-                 rmi_leaves=0, autotune=True):
+                 rmi_leaves=0, autotune=True, train=False):
         torch = _torch()
         _check_dev(keys)
         self.off_off, run = [], 0
@@ -424,7 +427,9 @@ class BatchPlan:
         check(self._c.lib.hg_batch_plan_create(self._c.bind_stream(), ctypes.byref(self._bd), _p(keys),
                                                _p(self.offsets), _p(self.csr),
                                                _p(probe) if probe is not None else None, po,
-                                               _p(self.counts) if probe is not None else None, 1 if autotune else 0,
+                                               _p(self.counts) if probe is not None else None,
+                                               (_lib.HG_PLAN_AUTOTUNE if autotune else 0) |
+                                               (_lib.HG_PLAN_TRAIN if train else 0),
                                                ctypes.byref(self._plan)), "hg_batch_plan_create")
 
     def run(self):
@@ -434,7 +439,8 @@ class BatchPlan:
     def info(self):
         i = _lib.BatchPlanInfo()
         check(self._c.lib.hg_batch_plan_query(self._plan, ctypes.byref(i)), "hg_batch_plan_query")
-        return {"tile_keys": i.tile_keys, "fill": i.fill, "candidates": i.candidates, "tuned_ms": i.tuned_ms}
+        return {"tile_keys": i.tile_keys, "fill": i.fill, "candidates": i.candidates, "tuned_ms": i.tuned_ms,
+                "learned_tables": i.learned_tables}
 
     def close(self):
         if self._plan:

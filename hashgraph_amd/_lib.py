@@ -48,9 +48,13 @@ class RmiReport(ctypes.Structure):
                 ("mean_probe_murmur", ctypes.c_double)]
 
 
+HG_PLAN_AUTOTUNE = 1
+HG_PLAN_TRAIN = 2
+
+
 class BatchPlanInfo(ctypes.Structure):
     _fields_ = [("tile_keys", ctypes.c_uint32), ("fill", ctypes.c_float), ("candidates", ctypes.c_uint32),
-                ("tuned_ms", ctypes.c_float)]
+                ("tuned_ms", ctypes.c_float), ("learned_tables", ctypes.c_uint32)]
 
 
 class KernelTime(ctypes.Structure):

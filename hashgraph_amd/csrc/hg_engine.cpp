@@ -314,8 +314,10 @@ struct Level {
   uint32_t tile_keys = kTile;   // keys per partition tile
   uint32_t big_cap = 1;         // per-bin counters of the final stage's oversized buckets
 
+  bool any_kind = false;        // kinds change on the device (trained in a plan): general kernels
   // One kind shared by all segments selects a specialised kernel.
   int hash_kind() const {
+    if (any_kind) return KIND_MIXED;
     int k = -1;
     for (const BinMap &m : maps) k = (k < 0 || k == (int)m.kind) ? (int)m.kind : KIND_MIXED;
     for (const SegDesc &sd : segs) k = (k < 0 || k == (int)sd.map.kind) ? (int)sd.map.kind : KIND_MIXED;
@@ -1748,6 +1750,7 @@ BinMap batch_map(const hg_batch_desc *bd, uint32_t t) {
 struct BatchCfg {
   uint32_t tile_keys = kTile;
   double fill = 0.875;
+  bool train = false;  // models trained inside the plan: kinds set on the device
 };
 
 // One side of a batch: the level over all tables, its device descriptors and
@@ -1784,6 +1787,7 @@ hg_status batch_prepare(hg_ctx c, const hg_batch_desc *bd, const uint64_t *probe
     S.L.maps.push_back(map);
   }
   S.L.tile_keys = cfg.tile_keys;
+  S.L.any_kind = cfg.train;
   S.L.prepare();
   S.nkeys = src[bd->n_tables] - src[0];
   Plan P0;
@@ -1859,6 +1863,9 @@ struct hg_batch_plan_s {
   BatchSide sb, sp;
   BatchCfg cfg;
   hg_batch_plan_info info{};
+  // models trained inside the plan (HG_PLAN_TRAIN)
+  bool train = false;
+  LaunchRmiBatch rb{};
 };
 
 namespace {
@@ -1870,6 +1877,75 @@ void plan_drop_graph(hg_batch_plan p) {
   p->graph = nullptr;
 }
 
+// In-plan training: table t's model (at d_rmi_models + t * model bytes) is
+// fitted on max(n/100, min(n, 1024)) sampled keys with seed t, on every
+// replay; its bin maps point at the model and take the kind its sampled
+// check chooses.  Samples must fit the one-workgroup trainer.
+hg_status plan_setup_training(hg_batch_plan p) {
+  const hg_batch_desc &bd = p->bd;
+  const uint32_t T = bd.n_tables;
+  const uint32_t L = bd.rmi_leaves;
+  if (!bd.d_rmi_models || L == 0 || L > RMI_ONE_MAX_LEAVES)
+    return fail(HG_ERR_INVALID_ARG, "in-plan training needs d_rmi_models and rmi_leaves in [1, 1024]");
+  const int kb = (int)bd.key_bytes;
+  const size_t mb = hg_rmi_model_bytes(L);
+  std::vector<RmiBatchItem> items;
+  uint32_t keys_cap = 16, cnt_cap = 4 * L;
+  for (uint32_t t = 0; t < T; t++) {
+    const uint64_t n = p->key_off[t + 1] - p->key_off[t];
+    if (n == 0) continue;
+    const uint64_t ns = std::max<uint64_t>(n / 100, std::min<uint64_t>(n, 1024));
+    long double mp = (long double)p->n_bins[t] * (long double)ns / (long double)n;
+    uint64_t Mp = (uint64_t)(mp + 0.5L);
+    Mp = std::max<uint64_t>(1, std::min<uint64_t>(Mp, p->n_bins[t]));
+    uint64_t np2 = 16;
+    while (np2 < ns) np2 <<= 1;
+    if (np2 * kb > RMI_ONE_MAX_KEY_BYTES || Mp > RMI_ONE_MAX_CHECK_BINS)
+      return fail(HG_ERR_UNSUPPORTED, "in-plan training takes tables whose 1% sample fits one workgroup (up to "
+                                      "16384 uint32 or 8192 uint64 keys)");
+    RmiBatchItem it{};
+    it.key_begin = p->key_off[t];
+    it.n_keys = n;
+    it.s0 = splitmix_mix(t);
+    it.model_off = (uint64_t)t * mb;
+    it.map = make_map(p->n_bins[t], 0, nullptr);
+    it.ns = (uint32_t)ns;
+    it.np2 = (uint32_t)np2;
+    it.Mp = (uint32_t)Mp;
+    it.table = t;
+    items.push_back(it);
+    keys_cap = std::max<uint32_t>(keys_cap, (uint32_t)np2);
+    cnt_cap = std::max<uint32_t>(cnt_cap, (uint32_t)Mp);
+  }
+  // every table's maps point at its model slot; kinds follow the check
+  p->kinds.assign(T, HG_HASH_RMI);
+  p->bd.h_hash_kind = p->kinds.data();
+  Blob bl;
+  const size_t o_items = bl.put(items);
+  Plan P;
+  const size_t i_blob = P.add(std::max<size_t>(bl.data.size(), 16)), i_use = P.add((size_t)T * 4);
+  HG_TRY(arena_ensure(p->cb, 2, P.total()));
+  Carve cv;
+  cv.init(p->cb, 2, P);
+  char *blob = cv.get<char>(i_blob);
+  if (!bl.data.empty()) HG_TRY(upload(p->cb, blob, bl.data.data(), bl.data.size()));
+  uint32_t *use = cv.get<uint32_t>(i_use);
+  HG_HIP(launch_fill_u32(use, 0u, T, p->stream));  // empty tables stay murmur3
+  LaunchRmiBatch &b = p->rb;
+  b.key_bytes = kb;
+  b.n_items = (uint32_t)items.size();
+  b.n_leaves = L;
+  b.keys_cap = keys_cap;
+  b.cnt_cap = cnt_cap;
+  b.keys = p->d_keys;
+  b.items = (const RmiBatchItem *)(blob + o_items);
+  b.models = const_cast<void *>(bd.d_rmi_models);
+  b.use = use;
+  b.report = nullptr;
+  p->train = true;
+  return HG_OK;
+}
+
 // Lays out both sides for cfg, then captures their launches into a graph.
 hg_status plan_capture(hg_batch_plan p, const BatchCfg &cfg) {
   plan_drop_graph(p);
@@ -1877,7 +1953,20 @@ hg_status plan_capture(hg_batch_plan p, const BatchCfg &cfg) {
   if (p->d_probe) HG_TRY(batch_prepare(p->cp, &p->bd, p->probe_off.data(), cfg, p->sp));
   HG_HIP(hipStreamSynchronize(p->stream));  // uploads done before capture
   HG_HIP(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
-  hg_status st = batch_launch_build(p->cb, &p->bd, p->sb, p->d_keys, p->d_offsets, p->d_csr);
+  hg_status st = HG_OK;
+  if (p->train) {
+    // every table's model and its sampled check, then each side's bin maps
+    // take the kind the check chose
+    hipError_t e = launch_rmi_train_batch(p->rb, p->stream);
+    if (e == hipSuccess)
+      e = launch_set_kinds(p->rb.use, p->sb.d.segs, (uint32_t)p->sb.L.segs.size(), p->sb.d.maps,
+                           (uint32_t)p->sb.L.maps.size(), p->stream);
+    if (e == hipSuccess && p->d_probe)
+      e = launch_set_kinds(p->rb.use, p->sp.d.segs, (uint32_t)p->sp.L.segs.size(), p->sp.d.maps,
+                           (uint32_t)p->sp.L.maps.size(), p->stream);
+    if (e != hipSuccess) st = fail(HG_ERR_DEVICE, std::string("plan training launch: ") + hipGetErrorString(e));
+  }
+  if (st == HG_OK) st = batch_launch_build(p->cb, &p->bd, p->sb, p->d_keys, p->d_offsets, p->d_csr);
   if (st == HG_OK && p->d_probe)
     st = batch_launch_probe(p->cp, &p->bd, p->sp, p->d_probe, p->d_offsets, p->d_csr, p->d_counts);
   hipGraph_t g = nullptr;
@@ -2022,17 +2111,24 @@ hg_status hg_batch_plan_create(hg_ctx c, const hg_batch_desc *bd, const void *d_
   // inputs are produced on the caller's stream
   if (hipEventRecord(p->ev_in, c->stream) != hipSuccess || hipStreamWaitEvent(p->stream, p->ev_in, 0) != hipSuccess)
     return bail(fail(HG_ERR_DEVICE, "stream ordering failed"));
+  if (autotune & HG_PLAN_TRAIN) {
+    st = plan_setup_training(p);
+    if (st != HG_OK) return bail(st);
+  }
   std::vector<BatchCfg> cands;
-  if (autotune) {
+  if (autotune & HG_PLAN_AUTOTUNE) {
     for (double f : {0.875, 0.5})
       for (uint32_t tk : {kTile / 4, kTile, kTile * 4}) {
         BatchCfg x;
         x.fill = f;
         x.tile_keys = tk;
+        x.train = p->train;
         cands.push_back(x);
       }
   } else {
-    cands.push_back(BatchCfg());
+    BatchCfg x;
+    x.train = p->train;
+    cands.push_back(x);
   }
   BatchCfg best = cands[0];
   float best_ms = -1.0f;
@@ -2078,6 +2174,16 @@ hg_status hg_batch_plan_run(hg_batch_plan p) {
 hg_status hg_batch_plan_query(hg_batch_plan p, hg_batch_plan_info *info) {
   if (!p || !info) return fail(HG_ERR_INVALID_ARG, "null argument");
   *info = p->info;
+  info->learned_tables = 0;
+  if (p->train) {  // the last replay's decisions
+    DeviceGuard g(p->device);
+    std::vector<uint32_t> use(p->bd.n_tables);
+    HG_HIP(hipStreamSynchronize(p->stream));
+    HG_HIP(hipMemcpy(use.data(), p->rb.use, use.size() * 4, hipMemcpyDeviceToHost));
+    for (uint32_t u : use) info->learned_tables += u ? 1 : 0;
+  } else if (p->bd.h_hash_kind) {
+    for (uint32_t k : p->kinds) info->learned_tables += k == HG_HASH_RMI ? 1 : 0;
+  }
   return HG_OK;
 }
 

@@ -2971,6 +2971,23 @@ hipError_t set_kernel_attributes() {
   return e;
 }
 
+// Bin maps of a batch whose models were trained on the device: table t's
+// maps take the learned kind when use[t] is set, murmur3 otherwise.
+__global__ __launch_bounds__(256) void k_set_kinds(const uint32_t *__restrict__ use, SegDesc *__restrict__ segs,
+                                                   uint32_t nsegs, BinMap *__restrict__ maps, uint32_t nmaps) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i < nsegs) segs[i].map.kind = use[segs[i].table] ? 1u : 0u;
+  if (i < nmaps) maps[i].kind = use[i] ? 1u : 0u;
+}
+
+hipError_t launch_set_kinds(const uint32_t *use, SegDesc *segs, uint32_t nsegs, BinMap *maps, uint32_t nmaps,
+                            hipStream_t st) {
+  const uint32_t n = nsegs > nmaps ? nsegs : nmaps;
+  if (!n) return hipSuccess;
+  hipLaunchKernelGGL(k_set_kinds, dim3((n + 255) / 256), dim3(256), 0, st, use, segs, nsegs, maps, nmaps);
+  return hipGetLastError();
+}
+
 // Records of the index-carrying builds: the key's input position above the
 // low 32 bits of the key (murmur3 bins depend on those bits only), and their
 // unpacking into CSR keys (gathered from the input by position) and positions.

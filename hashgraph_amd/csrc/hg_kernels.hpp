@@ -198,6 +198,9 @@ struct LaunchRmiBatch {
 size_t rmi_one_lds(uint32_t keys_cap, int key_bytes, uint32_t L, uint32_t cnt_cap);
 hipError_t launch_rmi_train_batch(const LaunchRmiBatch &b, hipStream_t st);
 hipError_t set_rmi_kernel_attributes();
+// maps[t].kind and each segment's map kind from use[] (learned when set)
+hipError_t launch_set_kinds(const uint32_t *use, SegDesc *segs, uint32_t nsegs, BinMap *maps, uint32_t nmaps,
+                            hipStream_t st);
 uint32_t rmi_moment_blocks(uint64_t n_sample);
 uint32_t rmi_sort_tiles(uint64_t n_sample);
 hipError_t launch_rle(const LaunchRle &r, hipStream_t st);

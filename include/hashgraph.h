@@ -270,7 +270,7 @@ hg_status hg_batch_probe_emit(hg_ctx ctx, const hg_batch_desc *bd, const uint32_
  * (build, then the probe counts when d_probe is given) are captured once into
  * a hipGraph; hg_batch_plan_run replays it, ordered after the work already
  * queued on the context's stream and before work queued later.  With
- * autotune = 1 the plan times candidate tile sizes (8 K, 32 K, 128 K keys per
+ * autotune & HG_PLAN_AUTOTUNE the plan times candidate tile sizes (8 K, 32 K, 128 K keys per
  * partition workgroup) and bucket fills (bins per final-stage workgroup) by replaying
  * each candidate's graph, and keeps the fastest.  Buffers passed here must
  * stay valid for the plan's life.  A plan has its own stream and scratch;
@@ -281,7 +281,18 @@ typedef struct {
   float fill;           /* bucket fill target of the final stage */
   uint32_t candidates;  /* configurations timed */
   float tuned_ms;       /* replay time of the chosen one (-1 without tuning) */
+  uint32_t learned_tables; /* tables on their learned model (with HG_PLAN_TRAIN: after the last replay) */
 } hg_batch_plan_info;
+/* Flags of hg_batch_plan_create's `autotune` argument. */
+#define HG_PLAN_AUTOTUNE 1u
+/* Train every table's learned model inside the plan, on every replay (the
+ * plan then follows changing key contents): table t's model is fitted on
+ * max(n/100, min(n, 1024)) sampled keys with seed t into
+ * bd->d_rmi_models + t * hg_rmi_model_bytes(bd->rmi_leaves) (rmi_leaves in
+ * [1, 1024]), and the table uses it when its sampled check wins, murmur3
+ * otherwise (bd->h_hash_kind is ignored).  Samples must fit one workgroup
+ * (up to 16384 uint32 or 8192 uint64 keys): HG_ERR_UNSUPPORTED otherwise. */
+#define HG_PLAN_TRAIN 2u
 
 hg_status hg_batch_plan_create(hg_ctx ctx, const hg_batch_desc *bd, const void *d_keys, uint32_t *d_offsets,
                                void *d_csr_keys, const void *d_probe, const uint64_t *h_probe_off, uint64_t *d_counts,

@@ -246,3 +246,48 @@ def test_batch_probe_emit_lists(dev):
     # the counts of the refused call are still exact
     _, counts3 = hg.batch_probe(off, csr, ko, n_bins, off_off, dp, po)
     assert np.array_equal(counts3.cpu().numpy(), cnt)
+
+
+def test_plan_trains_models_in_graph(dev):
+    """A plan with HG_PLAN_TRAIN fits every table's model inside the graph on
+    each replay: after a replay the models equal the batch trainer's (seed t,
+    sample max(n/100, min(n, 1024))), each table's CSR equals the oracle's
+    under the kind its check chose, and the counts equal the bag joins; after
+    the keys change in place, the next replay follows them."""
+    import torch
+    import hashgraph_amd as hg
+    keys, ko, probe, po = _batch(19, 40, 4, 14)
+    n_bins = np.maximum(np.diff(ko), 1)
+    T = len(n_bins)
+    L = 256
+    mb = hg.rmi_model_bytes(L)
+    dk, dp = _to_dev(keys, dev), _to_dev(probe, dev)
+    models = torch.zeros(T * mb, dtype=torch.uint8, device=dev)
+    plan = hg.BatchPlan(dk, ko, n_bins, probe=dp, probe_off=po, rmi_models=models, rmi_leaves=L, autotune=False,
+                        train=True)
+    ns = [max(int(n) // 100, min(int(n), 1024)) for n in np.diff(ko)]
+    for rep in range(2):
+        if rep == 1:  # new key contents in the same buffers
+            keys = keys[::-1].copy() ^ np.uint32(0x5A5A5A5A)
+            probe = probe ^ np.uint32(0x5A5A5A5A)
+            dk.copy_(_to_dev(keys, dev))
+            dp.copy_(_to_dev(probe, dev))
+        plan.run()
+        torch.cuda.synchronize()
+        ref, use = hg.rmi_train_batch(dk, ko, n_bins, ns, list(range(T)), n_leaves=L)
+        got = models.cpu().numpy()
+        exp = ref.cpu().numpy()
+        for t in range(T):
+            if ko[t + 1] > ko[t]:
+                assert np.array_equal(got[t * mb:(t + 1) * mb], exp[t * mb:(t + 1) * mb]), f"replay {rep} table {t}"
+        assert plan.info()["learned_tables"] == sum(use)
+
+        def bins_fn(t, a, M):
+            if use[t]:
+                return o.bins_rmi(a, got[t * mb:(t + 1) * mb], M)
+            return o.bins(a, M)
+
+        _check_tables(keys, ko, n_bins, plan.off_off, plan.offsets, plan.csr, bins_fn)
+        assert plan.counts.cpu().numpy().tolist() == _expected_counts(keys, ko, probe, po)
+    print(f"learned on {sum(use)} of {T} tables", flush=True)
+    plan.close()
