@@ -807,6 +807,36 @@ hg_status index_records(hg_ctx c, uint64_t n, uint64_t *&rin, uint64_t *&rout);
 hg_status build_partitioned(hg_ctx c, const hg_table_desc *desc, const void *d_keys, const int64_t *h_splits,
                             uint32_t n_parts, void *d_offsets, void *d_csr_keys, uint64_t *h_part_offsets);
 
+// Launches of the batched trainer: items sorted by sample size and split at
+// 2048 keys, so that the many small samples run with small LDS (several
+// workgroups per CU) and only the large ones take a CU each.  The items must
+// be uploaded (in the returned order) to d_items.
+std::vector<LaunchRmiBatch> rmi_launches(std::vector<RmiBatchItem> &items, int kb, uint32_t L) {
+  std::stable_sort(items.begin(), items.end(),
+                   [](const RmiBatchItem &x, const RmiBatchItem &y) { return x.np2 < y.np2; });
+  std::vector<LaunchRmiBatch> out;
+  size_t i = 0;
+  while (i < items.size()) {
+    const bool small = items[i].np2 <= 2048;
+    size_t j = i;
+    LaunchRmiBatch b{};
+    b.key_bytes = kb;
+    b.n_leaves = L;
+    b.keys_cap = 16;
+    b.cnt_cap = 4 * L;
+    while (j < items.size() && (items[j].np2 <= 2048) == small) {
+      b.keys_cap = std::max<uint32_t>(b.keys_cap, items[j].np2);
+      b.cnt_cap = std::max<uint32_t>(b.cnt_cap, items[j].Mp);
+      j++;
+    }
+    b.n_items = (uint32_t)(j - i);
+    b.items = (const RmiBatchItem *)(uintptr_t)i;  // the item index until the upload
+    out.push_back(b);
+    i = j;
+  }
+  return out;
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -1156,7 +1186,6 @@ hg_status hg_rmi_train_batch(hg_ctx c, uint32_t n_tables, uint32_t key_bytes, co
   const uint32_t kcap = RMI_ONE_MAX_KEY_BYTES / key_bytes;
   std::vector<RmiBatchItem> items;
   std::vector<uint32_t> big;  // tables for the multi-kernel trainer
-  uint32_t keys_cap = 16, cnt_cap = 4 * n_leaves;
   for (uint32_t t = 0; t < n_tables; t++) {
     h_use[t] = 0;
     if (h_report) h_report[t] = hg_rmi_report{};
@@ -1182,10 +1211,9 @@ hg_status hg_rmi_train_batch(hg_ctx c, uint32_t n_tables, uint32_t key_bytes, co
     it.Mp = (uint32_t)Mp;
     it.table = t;
     items.push_back(it);
-    keys_cap = std::max<uint32_t>(keys_cap, (uint32_t)np2);
-    cnt_cap = std::max<uint32_t>(cnt_cap, (uint32_t)Mp);
   }
   if (!items.empty()) {
+    std::vector<LaunchRmiBatch> lb = rmi_launches(items, (int)key_bytes, n_leaves);
     Blob bl;
     const size_t o_items = bl.put(items);
     Plan P;
@@ -1196,25 +1224,21 @@ hg_status hg_rmi_train_batch(hg_ctx c, uint32_t n_tables, uint32_t key_bytes, co
     cv.init(c, 2, P);
     char *blob = cv.get<char>(i_blob);
     HG_TRY(upload(c, blob, bl.data.data(), bl.data.size()));
-    LaunchRmiBatch b{};
-    b.key_bytes = (int)key_bytes;
-    b.n_items = (uint32_t)items.size();
-    b.n_leaves = n_leaves;
-    b.keys_cap = keys_cap;
-    b.cnt_cap = cnt_cap;
-    b.keys = d_keys;
-    b.items = (const RmiBatchItem *)(blob + o_items);
-    b.models = d_models;
-    b.use = cv.get<uint32_t>(i_use);
-    b.report = cv.get<unsigned long long>(i_rep);
-    {
+    uint32_t *d_use = cv.get<uint32_t>(i_use);
+    unsigned long long *d_rep = cv.get<unsigned long long>(i_rep);
+    for (LaunchRmiBatch &b : lb) {
+      b.keys = d_keys;
+      b.items = (const RmiBatchItem *)(blob + o_items) + (uintptr_t)b.items;
+      b.models = d_models;
+      b.use = d_use;
+      b.report = d_rep;
       Prof pf(c, "rmi_train");
       HG_HIP(launch_rmi_train_batch(b, c->stream));
     }
     std::vector<uint32_t> use(n_tables);
     std::vector<unsigned long long> rep((size_t)n_tables * 4);
-    HG_TRY(sync_read(c, use.data(), b.use, (size_t)n_tables * 4));
-    if (h_report) HG_TRY(sync_read(c, rep.data(), b.report, (size_t)n_tables * 32));
+    HG_TRY(sync_read(c, use.data(), d_use, (size_t)n_tables * 4));
+    if (h_report) HG_TRY(sync_read(c, rep.data(), d_rep, (size_t)n_tables * 32));
     for (const RmiBatchItem &it : items) {
       h_use[it.table] = use[it.table];
       if (h_report) {
@@ -1865,7 +1889,8 @@ struct hg_batch_plan_s {
   hg_batch_plan_info info{};
   // models trained inside the plan (HG_PLAN_TRAIN)
   bool train = false;
-  LaunchRmiBatch rb{};
+  std::vector<LaunchRmiBatch> rb;
+  uint32_t *use = nullptr;
 };
 
 namespace {
@@ -1890,7 +1915,6 @@ hg_status plan_setup_training(hg_batch_plan p) {
   const int kb = (int)bd.key_bytes;
   const size_t mb = hg_rmi_model_bytes(L);
   std::vector<RmiBatchItem> items;
-  uint32_t keys_cap = 16, cnt_cap = 4 * L;
   for (uint32_t t = 0; t < T; t++) {
     const uint64_t n = p->key_off[t + 1] - p->key_off[t];
     if (n == 0) continue;
@@ -1914,9 +1938,8 @@ hg_status plan_setup_training(hg_batch_plan p) {
     it.Mp = (uint32_t)Mp;
     it.table = t;
     items.push_back(it);
-    keys_cap = std::max<uint32_t>(keys_cap, (uint32_t)np2);
-    cnt_cap = std::max<uint32_t>(cnt_cap, (uint32_t)Mp);
   }
+  p->rb = rmi_launches(items, kb, L);
   // every table's maps point at its model slot; kinds follow the check
   p->kinds.assign(T, HG_HASH_RMI);
   p->bd.h_hash_kind = p->kinds.data();
@@ -1929,19 +1952,15 @@ hg_status plan_setup_training(hg_batch_plan p) {
   cv.init(p->cb, 2, P);
   char *blob = cv.get<char>(i_blob);
   if (!bl.data.empty()) HG_TRY(upload(p->cb, blob, bl.data.data(), bl.data.size()));
-  uint32_t *use = cv.get<uint32_t>(i_use);
-  HG_HIP(launch_fill_u32(use, 0u, T, p->stream));  // empty tables stay murmur3
-  LaunchRmiBatch &b = p->rb;
-  b.key_bytes = kb;
-  b.n_items = (uint32_t)items.size();
-  b.n_leaves = L;
-  b.keys_cap = keys_cap;
-  b.cnt_cap = cnt_cap;
-  b.keys = p->d_keys;
-  b.items = (const RmiBatchItem *)(blob + o_items);
-  b.models = const_cast<void *>(bd.d_rmi_models);
-  b.use = use;
-  b.report = nullptr;
+  p->use = cv.get<uint32_t>(i_use);
+  HG_HIP(launch_fill_u32(p->use, 0u, T, p->stream));  // empty tables stay murmur3
+  for (LaunchRmiBatch &b : p->rb) {
+    b.keys = p->d_keys;
+    b.items = (const RmiBatchItem *)(blob + o_items) + (uintptr_t)b.items;
+    b.models = const_cast<void *>(bd.d_rmi_models);
+    b.use = p->use;
+    b.report = nullptr;
+  }
   p->train = true;
   return HG_OK;
 }
@@ -1957,12 +1976,14 @@ hg_status plan_capture(hg_batch_plan p, const BatchCfg &cfg) {
   if (p->train) {
     // every table's model and its sampled check, then each side's bin maps
     // take the kind the check chose
-    hipError_t e = launch_rmi_train_batch(p->rb, p->stream);
+    hipError_t e = hipSuccess;
+    for (const LaunchRmiBatch &b : p->rb)
+      if (e == hipSuccess) e = launch_rmi_train_batch(b, p->stream);
     if (e == hipSuccess)
-      e = launch_set_kinds(p->rb.use, p->sb.d.segs, (uint32_t)p->sb.L.segs.size(), p->sb.d.maps,
+      e = launch_set_kinds(p->use, p->sb.d.segs, (uint32_t)p->sb.L.segs.size(), p->sb.d.maps,
                            (uint32_t)p->sb.L.maps.size(), p->stream);
     if (e == hipSuccess && p->d_probe)
-      e = launch_set_kinds(p->rb.use, p->sp.d.segs, (uint32_t)p->sp.L.segs.size(), p->sp.d.maps,
+      e = launch_set_kinds(p->use, p->sp.d.segs, (uint32_t)p->sp.L.segs.size(), p->sp.d.maps,
                            (uint32_t)p->sp.L.maps.size(), p->stream);
     if (e != hipSuccess) st = fail(HG_ERR_DEVICE, std::string("plan training launch: ") + hipGetErrorString(e));
   }
@@ -2179,7 +2200,7 @@ hg_status hg_batch_plan_query(hg_batch_plan p, hg_batch_plan_info *info) {
     DeviceGuard g(p->device);
     std::vector<uint32_t> use(p->bd.n_tables);
     HG_HIP(hipStreamSynchronize(p->stream));
-    HG_HIP(hipMemcpy(use.data(), p->rb.use, use.size() * 4, hipMemcpyDeviceToHost));
+    HG_HIP(hipMemcpy(use.data(), p->use, use.size() * 4, hipMemcpyDeviceToHost));
     for (uint32_t u : use) info->learned_tables += u ? 1 : 0;
   } else if (p->bd.h_hash_kind) {
     for (uint32_t k : p->kinds) info->learned_tables += k == HG_HASH_RMI ? 1 : 0;

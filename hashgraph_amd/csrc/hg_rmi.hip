@@ -118,6 +118,30 @@ __device__ __forceinline__ void block_sum_d(double (&v)[NV], double *s_red /* 25
   __syncthreads();
 }
 
+// block_sum_d's 256-thread tree computed by one wave: lane i holds virtual
+// threads i + 64 q (q = 0..3) in a[q]; every level adds the same two values
+// as the shared-memory tree (st = 128 and 64 inside a lane, 32..1 across
+// lanes), so the total (returned in every lane) is block_sum_d's, bit for bit.
+template <int NV>
+__device__ __forceinline__ void wave_tree_256(double (&a)[4][NV]) {
+  const uint32_t lane = threadIdx.x & 63;
+#pragma unroll
+  for (int j = 0; j < NV; j++) {
+    a[0][j] += a[2][j];
+    a[1][j] += a[3][j];
+    a[0][j] += a[1][j];
+  }
+#pragma unroll
+  for (int st = 32; st >= 1; st >>= 1)
+#pragma unroll
+    for (int j = 0; j < NV; j++) {
+      const double o = __shfl_down(a[0][j], st, 64);
+      if (lane < (uint32_t)st) a[0][j] += o;
+    }
+#pragma unroll
+  for (int j = 0; j < NV; j++) a[0][j] = __shfl(a[0][j], 0, 64);
+}
+
 // Root candidates.  The root is linear in one of two features of the key
 // (raw distance from the smallest key, or the bits of its float: a
 // piecewise-linear log); for each feature it is fitted either by least
@@ -134,13 +158,15 @@ struct RootCand {
 // of nblk (elements blk*256 + t, stepping nblk*256), in v of every thread
 // (256 threads; fixed tree order).  Shared by the multi-kernel and the
 // one-workgroup trainers, so both give the same model bit for bit.
+// One (virtual) thread t's moment sums of block blk: elements blk*256 + t,
+// stepping nblk*256.
 template <typename K>
-__device__ __forceinline__ void root_moments_block(const K *sorted, uint64_t n, double yscale, uint64_t blk,
-                                                   uint64_t nblk, double (&v)[8], double *s_red) {
+__device__ __forceinline__ void root_moments_acc(const K *sorted, uint64_t n, double yscale, uint64_t blk,
+                                                 uint64_t nblk, uint32_t t, double (&v)[8]) {
   const uint64_t kmin = (uint64_t)sorted[0];
 #pragma unroll
   for (int j = 0; j < 8; j++) v[j] = 0.0;
-  for (uint64_t i = blk * 256 + threadIdx.x; i < n; i += nblk * 256) {
+  for (uint64_t i = blk * 256 + t; i < n; i += nblk * 256) {
     const double y = ((double)i + 0.5) * yscale;
 #pragma unroll
     for (uint32_t f = 0; f < 2; f++) {
@@ -151,7 +177,25 @@ __device__ __forceinline__ void root_moments_block(const K *sorted, uint64_t n, 
       v[4 * f + 3] += x * y;
     }
   }
+}
+
+template <typename K>
+__device__ __forceinline__ void root_moments_block(const K *sorted, uint64_t n, double yscale, uint64_t blk,
+                                                   uint64_t nblk, double (&v)[8], double *s_red) {
+  root_moments_acc<K>(sorted, n, yscale, blk, nblk, threadIdx.x, v);
   block_sum_d<8>(v, s_red);
+}
+
+// The same sums with one wave per block (wave_tree_256): every lane gets v.
+template <typename K>
+__device__ __forceinline__ void root_moments_block_wave(const K *sorted, uint64_t n, double yscale, uint64_t blk,
+                                                        uint64_t nblk, double (&v)[8]) {
+  double a[4][8];
+#pragma unroll
+  for (int q = 0; q < 4; q++) root_moments_acc<K>(sorted, n, yscale, blk, nblk, (threadIdx.x & 63) + 64 * q, a[q]);
+  wave_tree_256<8>(a);
+#pragma unroll
+  for (int j = 0; j < 8; j++) v[j] = a[0][j];
 }
 
 // Per block: sums of x, y, x^2, x*y for both features (8 doubles).
@@ -272,16 +316,24 @@ __global__ __launch_bounds__(256) void k_rmi_bounds(const K *__restrict__ sorted
 // on the signed distance from the leaf's first key.  A sparse leaf (fewer
 // than 8 keys or a single value) also takes the sample keys on either side,
 // so that its line follows its neighbourhood.
+// A leaf's fitting range: the sample positions [a, b) routed to it, widened
+// by one key on each side when the leaf is sparse (fewer than 8 keys or a
+// single value), and its first key x0.
 template <typename K>
-__device__ __forceinline__ void leaf_fit_one(const K *sorted, uint64_t n, uint32_t a, uint32_t b, uint32_t M,
-                                             RmiLeaf *leaf, double *s_red) {
-  const double yscale = (double)M / (double)n;
-  const uint64_t x0 = (uint64_t)sorted[a < n ? a : n - 1];
+__device__ __forceinline__ void leaf_range(const K *sorted, uint64_t n, uint32_t a, uint32_t b, uint64_t &x0,
+                                           uint32_t &lo, uint32_t &hi) {
+  x0 = (uint64_t)sorted[a < n ? a : n - 1];
   const bool sparse = b - a < 8 || (uint64_t)sorted[b - 1] == x0;
-  const uint32_t lo = sparse && a > 0 ? a - 1 : a;
-  const uint32_t hi = sparse && b < n ? b + 1 : b;
-  double v[4] = {0.0, 0.0, 0.0, 0.0};
-  for (uint32_t i = lo + threadIdx.x; i < hi; i += 256) {
+  lo = sparse && a > 0 ? a - 1 : a;
+  hi = sparse && b < n ? b + 1 : b;
+}
+
+// One (virtual) thread t's sums of x, y, x^2, x*y over the leaf's range.
+template <typename K>
+__device__ __forceinline__ void leaf_acc(const K *sorted, uint64_t x0, uint32_t lo, uint32_t hi, double yscale,
+                                         uint32_t t, double (&v)[4]) {
+  v[0] = v[1] = v[2] = v[3] = 0.0;
+  for (uint32_t i = lo + t; i < hi; i += 256) {
     const uint64_t k = (uint64_t)sorted[i];
     const double x = k >= x0 ? (double)(k - x0) : -(double)(x0 - k);
     const double y = ((double)i + 0.5) * yscale;
@@ -290,23 +342,55 @@ __device__ __forceinline__ void leaf_fit_one(const K *sorted, uint64_t n, uint32
     v[2] += x * x;
     v[3] += x * y;
   }
-  block_sum_d<4>(v, s_red);
-  if (threadIdx.x == 0) {
-    double alpha, beta;
-    if (hi > lo) {
-      lsq_line((double)(hi - lo), v[0], v[1], v[2], v[3], alpha, beta);
-    } else {
-      alpha = ((double)a + 0.5) * yscale;
-      beta = 0.0;
-    }
-    double an = floor(alpha + 0.5);
-    an = an < 0.0 ? 0.0 : (an > (double)(M - 1) ? (double)(M - 1) : an);
-    RmiLeaf lf;
-    lf.x0 = x0;
-    lf.slope = (float)beta;
-    lf.anchor = (uint32_t)an;
-    *leaf = lf;
+}
+
+// The leaf's line from its sums: least squares of the rank position on the
+// signed distance from x0, the anchor rounded and clamped to [0, M-1].
+__device__ __forceinline__ RmiLeaf leaf_line(const double (&v)[4], uint64_t n, uint32_t a, uint32_t lo, uint32_t hi,
+                                             uint64_t x0, uint32_t M) {
+  const double yscale = (double)M / (double)n;
+  double alpha, beta;
+  if (hi > lo) {
+    lsq_line((double)(hi - lo), v[0], v[1], v[2], v[3], alpha, beta);
+  } else {
+    alpha = ((double)a + 0.5) * yscale;
+    beta = 0.0;
   }
+  double an = floor(alpha + 0.5);
+  an = an < 0.0 ? 0.0 : (an > (double)(M - 1) ? (double)(M - 1) : an);
+  RmiLeaf lf;
+  lf.x0 = x0;
+  lf.slope = (float)beta;
+  lf.anchor = (uint32_t)an;
+  return lf;
+}
+
+template <typename K>
+__device__ __forceinline__ void leaf_fit_one(const K *sorted, uint64_t n, uint32_t a, uint32_t b, uint32_t M,
+                                             RmiLeaf *leaf, double *s_red) {
+  const double yscale = (double)M / (double)n;
+  uint64_t x0;
+  uint32_t lo, hi;
+  leaf_range<K>(sorted, n, a, b, x0, lo, hi);
+  double v[4];
+  leaf_acc<K>(sorted, x0, lo, hi, yscale, threadIdx.x, v);
+  block_sum_d<4>(v, s_red);
+  if (threadIdx.x == 0) *leaf = leaf_line(v, n, a, lo, hi, x0, M);
+}
+
+// The same fit by one wave (wave_tree_256); lane 0 writes the leaf.
+template <typename K>
+__device__ __forceinline__ void leaf_fit_one_wave(const K *sorted, uint64_t n, uint32_t a, uint32_t b, uint32_t M,
+                                                  RmiLeaf *leaf) {
+  const double yscale = (double)M / (double)n;
+  uint64_t x0;
+  uint32_t lo, hi;
+  leaf_range<K>(sorted, n, a, b, x0, lo, hi);
+  double acc[4][4];
+#pragma unroll
+  for (int q = 0; q < 4; q++) leaf_acc<K>(sorted, x0, lo, hi, yscale, (threadIdx.x & 63) + 64 * q, acc[q]);
+  wave_tree_256<4>(acc);
+  if ((threadIdx.x & 63) == 0) *leaf = leaf_line(acc[0], n, a, lo, hi, x0, M);
 }
 
 template <typename K>
@@ -416,10 +500,11 @@ __global__ __launch_bounds__(256) void k_rmi_train_one(const K *__restrict__ key
   // 3. root candidates (k_rmi_root_moments + k_rmi_root_fit)
   const uint32_t nblk = (ns + 255) / 256;
   const double yscale = (double)L / (double)ns;
-  for (uint32_t b = 0; b < nblk; b++) {
+  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (uint32_t b = wave; b < nblk; b += 4) {  // one wave per moment block
     double v[8];
-    root_moments_block<K>(s_keys, ns, yscale, b, nblk, v, s_red);
-    if (threadIdx.x < 8) s_part[b * 8 + threadIdx.x] = v[threadIdx.x];
+    root_moments_block_wave<K>(s_keys, ns, yscale, b, nblk, v);
+    if (lane < 8) s_part[b * 8 + lane] = v[lane];
   }
   __syncthreads();
   root_fit_from<K>(s_keys, ns, s_part, nblk, L, M, &s_h, s_cand, s_red);
@@ -468,8 +553,8 @@ __global__ __launch_bounds__(256) void k_rmi_train_one(const K *__restrict__ key
     }
   }
   __syncthreads();
-  // 6. leaf fits (k_rmi_leaf_fit, one leaf after the other)
-  for (uint32_t l = 0; l < L; l++) leaf_fit_one<K>(s_keys, ns, s_start[l], s_start[l + 1], M, &s_leaf[l], s_red);
+  // 6. leaf fits (k_rmi_leaf_fit's sums, one wave per leaf)
+  for (uint32_t l = wave; l < L; l += 4) leaf_fit_one_wave<K>(s_keys, ns, s_start[l], s_start[l + 1], M, &s_leaf[l]);
   __syncthreads();
   // 7. the model to global memory
   {
