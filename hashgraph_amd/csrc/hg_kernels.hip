@@ -49,6 +49,7 @@ constexpr size_t PROBE_LDS = HG_PROBE_LDS;
 constexpr uint32_t BIG_CAP = 16384;
 constexpr uint32_t BIG_CHUNK = 65536;
 constexpr uint32_t BIG_BINS = 32768;
+constexpr int BIG_UB = 8;  // keys per thread whose loads the oversized-bucket kernels keep in flight
 
 // Workgroup b of a partition pass runs on XCD b % 8 (round-robin dispatch).
 // Tiles are handed out so that each XCD walks a contiguous range of tiles:
@@ -1576,12 +1577,23 @@ __global__ __launch_bounds__(1024) void k_giant_fold(const OFF *__restrict__ off
       s_e = 0;
     }
     __syncthreads();
-    for (uint64_t i = c0 + threadIdx.x; i < c1; i += 1024) {
-      const K v = keys[i];
-      const uint32_t c = wave_same_value(v);
-      if (c == 0) continue;
-      if (v == EMPTY) atomicAdd(&s_e, c);
-      else vt_insert(s_key, s_cnt, &s_fill, v, c);
+    // BIG_UB keys per thread per step, their loads in flight together
+    for (uint64_t i0 = c0; i0 < c1; i0 += 1024 * BIG_UB) {
+      K vv[BIG_UB];
+#pragma unroll
+      for (int u = 0; u < BIG_UB; u++) {
+        const uint64_t i = i0 + (uint64_t)u * 1024 + threadIdx.x;
+        vv[u] = i < c1 ? keys[i] : EMPTY;
+      }
+#pragma unroll
+      for (int u = 0; u < BIG_UB; u++) {
+        if (i0 + (uint64_t)u * 1024 + threadIdx.x >= c1) continue;
+        const K v = vv[u];
+        const uint32_t c = wave_same_value(v);
+        if (c == 0) continue;
+        if (v == EMPTY) atomicAdd(&s_e, c);
+        else vt_insert(s_key, s_cnt, &s_fill, v, c);
+      }
     }
     __syncthreads();
     const bool ovf = s_fill > HT_SLOTS / 2;
@@ -2201,11 +2213,21 @@ __global__ __launch_bounds__(BLOCK) void k_big_count(const K *__restrict__ part,
       const uint32_t nb = min((uint32_t)BIG_BINS, it.nbins - b0);
       for (uint32_t i = threadIdx.x; i < nb; i += BLOCK) s_cnt[i] = 0;
       __syncthreads();
-      for (uint32_t i0 = 0; i0 < n; i0 += BLOCK) {
-        const uint32_t i = i0 + threadIdx.x;
-        uint32_t l = 0;
-        if (i < n) l = bin_of<K>(kp[i], mp) - it.bin_lo - b0;
-        lds_add_agg<false>(s_cnt, l, i < n && l < nb);
+      // BIG_UB keys per thread per step, their loads in flight together
+      for (uint32_t i0 = 0; i0 < n; i0 += BLOCK * BIG_UB) {
+        K kk[BIG_UB];
+#pragma unroll
+        for (int u = 0; u < BIG_UB; u++) {
+          const uint32_t i = i0 + u * BLOCK + threadIdx.x;
+          kk[u] = i < n ? kp[i] : (K)0;
+        }
+#pragma unroll
+        for (int u = 0; u < BIG_UB; u++) {
+          const uint32_t i = i0 + u * BLOCK + threadIdx.x;
+          uint32_t l = 0;
+          if (i < n) l = bin_of<K>(kk[u], mp) - it.bin_lo - b0;
+          lds_add_agg<false>(s_cnt, l, i < n && l < nb);
+        }
       }
       __syncthreads();
       for (uint32_t i = threadIdx.x; i < nb; i += BLOCK)
@@ -2260,11 +2282,20 @@ __global__ __launch_bounds__(BLOCK) void k_big_place(const K *__restrict__ part,
       const uint32_t nb = min((uint32_t)BIG_BINS, it.nbins - b0);
       for (uint32_t i = threadIdx.x; i < nb; i += BLOCK) s_cnt[i] = 0;
       __syncthreads();
-      for (uint32_t i0 = 0; i0 < n; i0 += BLOCK) {
-        const uint32_t i = i0 + threadIdx.x;
-        uint32_t l = 0;
-        if (i < n) l = bin_of<K>(kp[i], mp) - it.bin_lo - b0;
-        lds_add_agg<false>(s_cnt, l, i < n && l < nb);
+      for (uint32_t i0 = 0; i0 < n; i0 += BLOCK * BIG_UB) {
+        K kk[BIG_UB];
+#pragma unroll
+        for (int u = 0; u < BIG_UB; u++) {
+          const uint32_t i = i0 + u * BLOCK + threadIdx.x;
+          kk[u] = i < n ? kp[i] : (K)0;
+        }
+#pragma unroll
+        for (int u = 0; u < BIG_UB; u++) {
+          const uint32_t i = i0 + u * BLOCK + threadIdx.x;
+          uint32_t l = 0;
+          if (i < n) l = bin_of<K>(kk[u], mp) - it.bin_lo - b0;
+          lds_add_agg<false>(s_cnt, l, i < n && l < nb);
+        }
       }
       __syncthreads();
       // reserve this chunk's places, bin by bin (RU returning atomics of a
@@ -2287,17 +2318,22 @@ __global__ __launch_bounds__(BLOCK) void k_big_place(const K *__restrict__ part,
         }
       }
       __syncthreads();
-      for (uint32_t i0 = 0; i0 < n; i0 += BLOCK) {
-        const uint32_t i = i0 + threadIdx.x;
-        K k = 0;
-        uint32_t l = 0;
-        if (i < n) {
-          k = kp[i];
-          l = bin_of<K>(k, mp) - it.bin_lo - b0;
+      for (uint32_t i0 = 0; i0 < n; i0 += BLOCK * BIG_UB) {
+        K kk[BIG_UB];
+#pragma unroll
+        for (int u = 0; u < BIG_UB; u++) {
+          const uint32_t i = i0 + u * BLOCK + threadIdx.x;
+          kk[u] = i < n ? kp[i] : (K)0;
         }
-        const bool v = i < n && l < nb;
-        const uint32_t pos = lds_add_agg<true>(s_cnt, l, v);
-        if (v) dst[pos] = k;
+#pragma unroll
+        for (int u = 0; u < BIG_UB; u++) {
+          const uint32_t i = i0 + u * BLOCK + threadIdx.x;
+          uint32_t l = 0;
+          if (i < n) l = bin_of<K>(kk[u], mp) - it.bin_lo - b0;
+          const bool v = i < n && l < nb;
+          const uint32_t pos = lds_add_agg<true>(s_cnt, l, v);
+          if (v) dst[pos] = kk[u];
+        }
       }
       __syncthreads();
     }
