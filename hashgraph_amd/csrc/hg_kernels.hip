@@ -1258,6 +1258,7 @@ constexpr uint32_t RLE_HEAVY_SPLIT = 1024;
 constexpr uint32_t RLE_HEAVY_SMALL_TS = 2048;
 constexpr uint32_t RLE_HEAVY_GRID_S = 2048;
 constexpr uint32_t RLE_HEAVY_GRID = 1024;
+constexpr int HEAVY_RU = 4;  // keys per thread whose loads k_rle_heavy keeps in flight
 
 __device__ __forceinline__ uint32_t lds_cas(uint32_t *p, uint32_t cmp, uint32_t v) { return atomicCAS(p, cmp, v); }
 __device__ __forceinline__ uint64_t lds_cas(uint64_t *p, uint64_t cmp, uint64_t v) {
@@ -1741,12 +1742,23 @@ __global__ __launch_bounds__(BLOCK) void k_rle_heavy(const OFF *__restrict__ off
       s_ecb = 0;
     }
     __syncthreads();
-    for (uint64_t i = a0 + threadIdx.x; i < a1; i += BLOCK) {
-      const K v = keysA[i];
-      const uint32_t c = wave_same_value(v);
-      if (c == 0) continue;
-      if (v == EMPTY) atomicAdd(&s_eca, (unsigned long long)c);
-      else vt_insert(s_key, s_ca, &s_fill, v, c, mask);
+    // HEAVY_RU keys per thread per step, their loads in flight together
+    for (uint64_t i0 = a0; i0 < a1; i0 += (uint64_t)BLOCK * HEAVY_RU) {
+      K vv[HEAVY_RU];
+#pragma unroll
+      for (int u = 0; u < HEAVY_RU; u++) {
+        const uint64_t i = i0 + (uint64_t)u * BLOCK + threadIdx.x;
+        vv[u] = i < a1 ? keysA[i] : EMPTY;
+      }
+#pragma unroll
+      for (int u = 0; u < HEAVY_RU; u++) {
+        if (i0 + (uint64_t)u * BLOCK + threadIdx.x >= a1) continue;
+        const K v = vv[u];
+        const uint32_t c = wave_same_value(v);
+        if (c == 0) continue;
+        if (v == EMPTY) atomicAdd(&s_eca, (unsigned long long)c);
+        else vt_insert(s_key, s_ca, &s_fill, v, c, mask);
+      }
     }
     __syncthreads();
     if (s_fill > slots / 2) {  // too many distinct values for the table (only when slots == HT_SLOTS)
@@ -1757,12 +1769,22 @@ __global__ __launch_bounds__(BLOCK) void k_rle_heavy(const OFF *__restrict__ off
       __syncthreads();
       continue;
     }
-    for (uint64_t i = x0 + threadIdx.x; i < x1; i += BLOCK) {
-      const K v = keysB[i];
-      const uint32_t c = wave_same_value(v);
-      if (c == 0) continue;
-      if (v == EMPTY) atomicAdd(&s_ecb, (unsigned long long)c);
-      else vt_add_existing(s_key, s_cb, v, c, mask);
+    for (uint64_t i0 = x0; i0 < x1; i0 += (uint64_t)BLOCK * HEAVY_RU) {
+      K vv[HEAVY_RU];
+#pragma unroll
+      for (int u = 0; u < HEAVY_RU; u++) {
+        const uint64_t i = i0 + (uint64_t)u * BLOCK + threadIdx.x;
+        vv[u] = i < x1 ? keysB[i] : EMPTY;
+      }
+#pragma unroll
+      for (int u = 0; u < HEAVY_RU; u++) {
+        if (i0 + (uint64_t)u * BLOCK + threadIdx.x >= x1) continue;
+        const K v = vv[u];
+        const uint32_t c = wave_same_value(v);
+        if (c == 0) continue;
+        if (v == EMPTY) atomicAdd(&s_ecb, (unsigned long long)c);
+        else vt_add_existing(s_key, s_cb, v, c, mask);
+      }
     }
     __syncthreads();
     unsigned long long mine = 0, mp = 0;
