@@ -18,7 +18,8 @@ from ._lib import HG_HASH_MURMUR3, HG_HASH_RMI, HGError, check
 
 __all__ = ["Context", "build", "probe", "intersect", "intersect_rle", "exclusive_scan", "hash_bins", "find_splits",
            "build_partitioned", "rmi_train", "rmi_train_batch", "rmi_model_bytes", "batch_build", "batch_probe", "batch_probe_emit", "HGError",
-           "HG_HASH_MURMUR3", "HG_HASH_RMI", "version", "table_bins", "BatchPlan"]
+           "HG_HASH_MURMUR3", "HG_HASH_RMI", "version", "table_bins", "BatchPlan", "build_staged",
+           "build_partitioned_staged", "find_splits_staged", "intersect_staged"]
 
 _ctx = {}
 
@@ -251,6 +252,105 @@ def build_partitioned(keys, n_bins: int, splits, offset_bytes: int = 4, hash_kin
     check(c.lib.hg_build_partitioned(c.bind_stream(), ctypes.byref(d), _p(keys), sp, n_parts, _p(off), _p(csr), po),
           "hg_build_partitioned")
     return off, csr, list(po)
+
+
+# ---- staged tables: keys, offsets and CSR keys in host memory -------------
+
+def _hp(a):
+    """Host address of a contiguous numpy array or CPU tensor (pageable or pinned)."""
+    if a is None:
+        return ctypes.c_void_p(0)
+    if hasattr(a, "ctypes"):
+        if not a.flags.c_contiguous:
+            raise ValueError("host arrays must be contiguous")
+        return ctypes.c_void_p(a.ctypes.data)
+    if a.is_cuda or not a.is_contiguous():
+        raise ValueError("host tensors must be contiguous CPU tensors")
+    return ctypes.c_void_p(a.data_ptr())
+
+
+def _hbytes(a) -> int:
+    kb = a.dtype.itemsize if hasattr(a, "ctypes") else a.element_size()
+    if kb not in (4, 8):
+        raise TypeError("keys must be 32- or 64-bit integers")
+    return kb
+
+
+def _hempty(like, n, nbytes):
+    """Uninitialised host array of n elements of nbytes each, of like's kind."""
+    if hasattr(like, "ctypes"):
+        import numpy as np
+        return np.empty(n, dtype=np.uint32 if nbytes == 4 else np.uint64)
+    torch = _torch()
+    return torch.empty(n, dtype=torch.int32 if nbytes == 4 else torch.int64, pin_memory=like.is_pinned())
+
+
+def build_staged(keys, n_bins: int, offset_bytes: int = 4, hash_kind: int = HG_HASH_MURMUR3, rmi_model=None,
+                 device_budget: int = 0, device=None, out=None):
+    """hg_build over host-resident keys (numpy array or CPU tensor), for tables
+    larger than the device: returns host (offsets, csr_keys)."""
+    kb = _hbytes(keys)
+    n = len(keys)
+    c = context(device)
+    if out is None:
+        off, csr = _hempty(keys, n_bins + 1, offset_bytes), _hempty(keys, max(n, 1), kb)
+    else:
+        off, csr = out
+    d = _desc(n, n_bins, kb, offset_bytes, hash_kind, rmi_model)
+    check(c.lib.hg_build_staged(c.bind_stream(), ctypes.byref(d), _hp(keys), _hp(off), _hp(csr), device_budget),
+          "hg_build_staged")
+    return off, csr[:n]
+
+
+def build_partitioned_staged(keys, n_bins: int, splits, offset_bytes: int = 4, hash_kind: int = HG_HASH_MURMUR3,
+                             rmi_model=None, device_budget: int = 0, device=None):
+    """build_partitioned over host-resident keys: returns host (offsets, csr_keys, part_offsets)."""
+    kb = _hbytes(keys)
+    n = len(keys)
+    n_parts = len(splits) - 1
+    c = context(device)
+    off = _hempty(keys, int(splits[-1]) + n_parts, offset_bytes)
+    csr = _hempty(keys, max(n, 1), kb)
+    d = _desc(n, n_bins, kb, offset_bytes, hash_kind, rmi_model)
+    sp = (ctypes.c_int64 * (n_parts + 1))(*[int(x) for x in splits])
+    po = (ctypes.c_uint64 * (n_parts + 1))()
+    check(c.lib.hg_build_partitioned_staged(c.bind_stream(), ctypes.byref(d), _hp(keys), sp, n_parts, _hp(off),
+                                            _hp(csr), po, device_budget), "hg_build_partitioned_staged")
+    return off, csr[:n], list(po)
+
+
+def find_splits_staged(keys, n_bins: int, bin_count: int = 16000, n_parts: int = 16,
+                       hash_kind: int = HG_HASH_MURMUR3, rmi_model=None, device_budget: int = 0, device=None):
+    """find_splits over host-resident keys."""
+    c = context(device)
+    d = _desc(len(keys), n_bins, _hbytes(keys), 4, hash_kind, rmi_model)
+    sp = (ctypes.c_int64 * (n_parts + 1))()
+    check(c.lib.hg_find_splits_staged(c.bind_stream(), ctypes.byref(d), _hp(keys), bin_count, n_parts, sp,
+                                      device_budget), "hg_find_splits_staged")
+    return list(sp)
+
+
+def intersect_staged(offA, keysA, offB, keysB, n_bins: int, emit: bool = True, hash_kind: int = HG_HASH_MURMUR3,
+                     rmi_model=None, device_budget: int = 0, device=None, capacity: int | None = None):
+    """intersect over two host-resident tables: returns (count, host matched keys or None)."""
+    kb = _hbytes(keysA)
+    ob = _hbytes(offA)
+    c = context(device)
+    dA = _desc(len(keysA), n_bins, kb, ob, hash_kind, rmi_model)
+    dB = _desc(len(keysB), n_bins, kb, ob, hash_kind, rmi_model)
+    cnt = ctypes.c_uint64(0)
+    if capacity is None:
+        check(c.lib.hg_intersect_staged(c.bind_stream(), ctypes.byref(dA), _hp(offA), _hp(keysA), ctypes.byref(dB),
+                                        _hp(offB), _hp(keysB), None, 0, ctypes.byref(cnt), device_budget),
+              "hg_intersect_staged")
+        if not emit:
+            return cnt.value, None
+        capacity = cnt.value
+    out = _hempty(keysA, max(capacity, 1), kb)
+    check(c.lib.hg_intersect_staged(c.bind_stream(), ctypes.byref(dA), _hp(offA), _hp(keysA), ctypes.byref(dB),
+                                    _hp(offB), _hp(keysB), _hp(out), capacity, ctypes.byref(cnt), device_budget),
+          "hg_intersect_staged")
+    return cnt.value, out[: cnt.value]
 
 
 def rmi_model_bytes(n_leaves: int = 4096) -> int:

@@ -26,7 +26,8 @@ EXPORTS = [
     "hg_build_partitioned", "hg_rmi_model_bytes", "hg_rmi_train", "hg_rmi_train_batch", "hg_batch_build", "hg_batch_probe", "hg_batch_probe_emit",
     "hg_exclusive_scan", "hg_hash_bins", "hg_table_bins", "hg_batch_plan_create", "hg_batch_plan_run",
     "hg_batch_plan_query", "hg_batch_plan_destroy", "hg_build_noalloc", "hg_build_indexed",
-    "hg_build_partitioned_indexed",
+    "hg_build_partitioned_indexed", "hg_build_staged", "hg_build_partitioned_staged", "hg_find_splits_staged",
+    "hg_intersect_staged",
 ]
 
 
@@ -103,6 +104,12 @@ def load(path: str = LIB_PATH):
             "hg_build_partitioned": (i32, [vp, P(TableDesc), vp, P(ctypes.c_int64), u32, vp, vp, P(u64)]),
             "hg_build_indexed": (i32, [vp, P(TableDesc), vp, vp, vp, vp]),
             "hg_build_partitioned_indexed": (i32, [vp, P(TableDesc), vp, P(ctypes.c_int64), u32, vp, vp, vp, P(u64)]),
+            "hg_build_staged": (i32, [vp, P(TableDesc), vp, vp, vp, ctypes.c_size_t]),
+            "hg_build_partitioned_staged": (i32, [vp, P(TableDesc), vp, P(ctypes.c_int64), u32, vp, vp, P(u64),
+                                                  ctypes.c_size_t]),
+            "hg_find_splits_staged": (i32, [vp, P(TableDesc), vp, u64, u32, P(ctypes.c_int64), ctypes.c_size_t]),
+            "hg_intersect_staged": (i32, [vp, P(TableDesc), vp, vp, P(TableDesc), vp, vp, vp, u64, P(u64),
+                                          ctypes.c_size_t]),
             "hg_rmi_model_bytes": (ctypes.c_size_t, [u32]),
             "hg_rmi_train": (i32, [vp, vp, u64, u32, u64, u64, u64, u32, vp, P(u32), P(RmiReport)]),
             "hg_rmi_train_batch": (i32, [vp, u32, u32, vp, P(u64), P(u64), P(u64), P(u64), u32, vp, P(u32),

@@ -30,12 +30,16 @@ CFLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", f"--off
           "-I", os.path.join(ROOT, "include"), "-I", CSRC]
 
 SOURCES = ["hg_kernels.hip", "hg_rmi.hip", "hg_engine.cpp", "SingleHashGraph.cpp", "MultiHashGraph.cpp"]
-# driver programs: name -> (source, extra flags).  multi-hash-32 and
-# multi-hash-idx are the reference's B32 and INDEX_TRACK builds of the same
-# driver (include/MultiHashGraph.cuh:62-74 switches them at compile time).
+# driver programs: name -> (source, extra flags).  multi-hash-32,
+# multi-hash-idx and the -managed ones are the reference's B32, INDEX_TRACK
+# and MANAGED_MEM builds of the same driver (include/MultiHashGraph.cuh:62-74
+# switches them at compile time).
 DRIVERS = {"sing-hash": ("drivers/sing_hash.cpp", []), "multi-hash": ("drivers/multi_hash.cpp", []),
            "multi-hash-32": ("drivers/multi_hash.cpp", ["-DB32"]),
-           "multi-hash-idx": ("drivers/multi_hash.cpp", ["-DINDEX_TRACK"])}
+           "multi-hash-idx": ("drivers/multi_hash.cpp", ["-DINDEX_TRACK"]),
+           "multi-hash-managed": ("drivers/multi_hash.cpp", ["-DMANAGED_MEM"]),
+           "multi-hash-32-managed": ("drivers/multi_hash.cpp", ["-DB32", "-DMANAGED_MEM"]),
+           "multi-hash-idx-managed": ("drivers/multi_hash.cpp", ["-DINDEX_TRACK", "-DMANAGED_MEM"])}
 
 
 def _run(cmd):

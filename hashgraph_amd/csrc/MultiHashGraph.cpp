@@ -52,12 +52,28 @@ __global__ void k_widen(const uint32_t *__restrict__ in, uint64_t n, int64_t *__
 
 uint32_t grid_of(uint64_t n) { return (uint32_t)std::min<uint64_t>((n + 255) / 256, 16384); }
 
+// Storage of a table: device memory, or with MANAGED_MEM managed memory
+// prefetched to the device (src/MultiHashGraph.cu:229-247).  Where the device
+// has no demand paging, managed memory is host memory, and the prefetch a
+// hint that leaves it there; the staged calls stream it through HBM.
+template <bool MM, typename T>
+void table_alloc(T **p, size_t bytes, const char *what) {
+  if constexpr (MM) {
+    hip_ok(hipMallocManaged((void **)p, bytes), what);
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipMemPrefetchAsync(*p, bytes, dev, nullptr) != hipSuccess)
+      (void)hipGetLastError();
+  } else {
+    hip_ok(hipMalloc((void **)p, bytes), what);
+  }
+}
+
 }  // namespace
 
 // src/MultiHashGraph.cu:61-251: copies every input key to the host (h_vals)
 // and allocates the per-partition structures.
-template <typename K, bool IT>
-BasicMultiHashGraph<K, IT>::BasicMultiHashGraph(inputData *h_dVals_, index_t countSize_, index_t maxkey_,
+template <typename K, bool IT, bool MM>
+BasicMultiHashGraph<K, IT, MM>::BasicMultiHashGraph(inputData *h_dVals_, index_t countSize_, index_t maxkey_,
                                                 index_t tableSize_, index_t binCount_, index_t lrbBins_,
                                                 index_t gpuCount_) {
   if (countSize_ < 0 || tableSize_ <= 0 || binCount_ <= 0 || gpuCount_ <= 0 || !h_dVals_)
@@ -74,16 +90,16 @@ BasicMultiHashGraph<K, IT>::BasicMultiHashGraph(inputData *h_dVals_, index_t cou
   ok(hg_ctx_create(dev, &ctx_), "hg_ctx_create");
   const size_t nk = (size_t)std::max<index_t>(countSize, 1);
   h_vals = new hkey_t[nk]();
-  hip_ok(hipMalloc(&d_keys_, nk * sizeof(hkey_t)), "hipMalloc keys");
+  table_alloc<MM>(&d_keys_, nk * sizeof(hkey_t), "allocate keys");
   index_t at = 0;
   for (index_t i = 0; i < gpuCount; i++) {
     const index_t len = h_dVals[i].len;
     if (len < 0 || at + len > countSize)
       throw hashgraph::error(HG_ERR_INVALID_ARG, "MultiHashGraph: input lengths exceed countSize");
     if (len) {
-      hip_ok(hipMemcpy(d_keys_ + at, h_dVals[i].d_keys, (size_t)len * sizeof(hkey_t), hipMemcpyDeviceToDevice),
+      hip_ok(hipMemcpy(d_keys_ + at, h_dVals[i].d_keys, (size_t)len * sizeof(hkey_t), hipMemcpyDefault),
              "copy keys");
-      hip_ok(hipMemcpy(h_vals + at, h_dVals[i].d_keys, (size_t)len * sizeof(hkey_t), hipMemcpyDeviceToHost),
+      hip_ok(hipMemcpy(h_vals + at, h_dVals[i].d_keys, (size_t)len * sizeof(hkey_t), hipMemcpyDefault),
              "copy keys");
     }
     at += len;
@@ -104,17 +120,17 @@ BasicMultiHashGraph<K, IT>::BasicMultiHashGraph(inputData *h_dVals_, index_t cou
   h_edgesOff = new index_t[gpuCount]();
   h_lrbOff = new index_t[gpuCount]();
   const size_t bytes = nk * sizeof(keyval) + (size_t)(tableSize + gpuCount) * sizeof(index_t);
-  hip_ok(hipMalloc(&blob_, bytes), "hipMalloc CSR");
+  table_alloc<MM>(&blob_, bytes, "allocate CSR");
   if (IT) {
-    hip_ok(hipMalloc(&d_csrk_, nk * sizeof(hkey_t)), "hipMalloc CSR keys");
-    hip_ok(hipMalloc(&d_csri_, nk * sizeof(uint64_t)), "hipMalloc CSR positions");
+    table_alloc<MM>(&d_csrk_, nk * sizeof(hkey_t), "allocate CSR keys");
+    table_alloc<MM>(&d_csri_, nk * sizeof(uint64_t), "allocate CSR positions");
   } else {
     d_csrk_ = (hkey_t *)blob_;  // the entries are the keys
   }
 }
 
-template <typename K, bool IT>
-BasicMultiHashGraph<K, IT>::~BasicMultiHashGraph() {
+template <typename K, bool IT, bool MM>
+BasicMultiHashGraph<K, IT, MM>::~BasicMultiHashGraph() {
   if (ctx_) hg_ctx_destroy(ctx_);
   if (IT) {
     (void)hipFree(d_csrk_);
@@ -136,21 +152,25 @@ BasicMultiHashGraph<K, IT>::~BasicMultiHashGraph() {
 }
 
 // The reference's destroyMulti is compiled out (src/MultiHashGraph.cu:262-296).
-template <typename K, bool IT>
-void BasicMultiHashGraph<K, IT>::destroyMulti() {}
+template <typename K, bool IT, bool MM>
+void BasicMultiHashGraph<K, IT, MM>::destroyMulti() {}
 
 // src/MultiHashGraph.cu:408-629 for every partition of this GPU: splits
 // (countBinSizes) when findSplits, local partition, per-partition CSR; with
 // INDEX_TRACK the entries carry the key's input position.
-template <typename K, bool IT>
-void BasicMultiHashGraph<K, IT>::build(bool findSplits, index_t tid) {
+template <typename K, bool IT, bool MM>
+void BasicMultiHashGraph<K, IT, MM>::build(bool findSplits, index_t tid) {
   if (tid < 0 || tid >= gpuCount) throw hashgraph::error(HG_ERR_INVALID_ARG, "build: tid out of range");
   std::lock_guard<std::mutex> lk(mu_);
   if (tid != 0 && gpuCount > 1) return;  // partition 0's call builds them all
   const hg_table_desc d = desc_of<K>((uint64_t)countSize, (uint64_t)tableSize);
   if (findSplits) {
     std::vector<int64_t> sp(gpuCount + 1);
-    ok(hg_find_splits(ctx_, &d, d_keys_, (uint64_t)binCount, (uint32_t)gpuCount, sp.data()), "hg_find_splits");
+    if constexpr (MM)
+      ok(hg_find_splits_staged(ctx_, &d, d_keys_, (uint64_t)binCount, (uint32_t)gpuCount, sp.data(), stagedBudget),
+         "hg_find_splits_staged");
+    else
+      ok(hg_find_splits(ctx_, &d, d_keys_, (uint64_t)binCount, (uint32_t)gpuCount, sp.data()), "hg_find_splits");
     for (index_t i = 0; i <= gpuCount; i++) h_binSplits[i] = sp[i];
   }
   hip_ok(hipMemcpy(h_dBinSplits[0], h_binSplits, (size_t)(gpuCount + 1) * sizeof(index_t), hipMemcpyHostToDevice),
@@ -158,7 +178,31 @@ void BasicMultiHashGraph<K, IT>::build(bool findSplits, index_t tid) {
   std::vector<int64_t> sp(h_binSplits, h_binSplits + gpuCount + 1);
   std::vector<uint64_t> po(gpuCount + 1);
   index_t *off = (index_t *)(blob_ + (size_t)std::max<index_t>(countSize, 1) * sizeof(keyval));
-  if constexpr (IT) {
+  if constexpr (MM && IT) {
+    // records (input position << 32 | low 32 key bits) fall in their keys'
+    // bins; the entries are then filled from the host copy of the keys
+    const uint64_t n = (uint64_t)countSize;
+    if (n >= (1ull << 32)) throw hashgraph::error(HG_ERR_UNSUPPORTED, "INDEX_TRACK needs fewer than 2^32 keys");
+    std::vector<uint64_t> rec(std::max<uint64_t>(n, 1)), out(std::max<uint64_t>(n, 1));
+    for (uint64_t i = 0; i < n; i++) rec[i] = (i << 32) | (uint32_t)h_vals[i];
+    hg_table_desc d8 = d;
+    d8.key_bytes = 8;
+    ok(hg_build_partitioned_staged(ctx_, &d8, rec.data(), sp.data(), (uint32_t)gpuCount, off, out.data(), po.data(),
+                                   stagedBudget),
+       "hg_build_partitioned_staged");
+    keyval_ind *ent = (keyval_ind *)blob_;
+    for (uint64_t j = 0; j < po[gpuCount]; j++) {
+      const uint64_t i = out[j] >> 32;
+      d_csrk_[j] = h_vals[i];
+      d_csri_[j] = i;
+      ent[j].key = h_vals[i];
+      ent[j].ind = (index_t)i;
+    }
+  } else if constexpr (MM) {
+    ok(hg_build_partitioned_staged(ctx_, &d, d_keys_, sp.data(), (uint32_t)gpuCount, off, d_csrk_, po.data(),
+                                   stagedBudget),
+       "hg_build_partitioned_staged");
+  } else if constexpr (IT) {
     ok(hg_build_partitioned_indexed(ctx_, &d, d_keys_, sp.data(), (uint32_t)gpuCount, off, d_csrk_, d_csri_,
                                     po.data()),
        "hg_build_partitioned_indexed");
@@ -193,8 +237,8 @@ void BasicMultiHashGraph<K, IT>::build(bool findSplits, index_t tid) {
 
 // src/MultiHashGraph.cu:631-740 for partition tid: the matched B keys, as
 // keypair.right (int64).
-template <typename K, bool IT>
-void BasicMultiHashGraph<K, IT>::intersect(BasicMultiHashGraph &A, BasicMultiHashGraph &B, index_t *h_Common,
+template <typename K, bool IT, bool MM>
+void BasicMultiHashGraph<K, IT, MM>::intersect(BasicMultiHashGraph &A, BasicMultiHashGraph &B, index_t *h_Common,
                                            keypair **h_dOutput, index_t tid) {
   if (tid < 0 || tid >= A.gpuCount || tid >= B.gpuCount) throw hashgraph::error(HG_ERR_INVALID_ARG, "intersect: tid");
   const index_t R = A.h_binSplits[tid + 1] - A.h_binSplits[tid];
@@ -208,6 +252,25 @@ void BasicMultiHashGraph<K, IT>::intersect(BasicMultiHashGraph &A, BasicMultiHas
   const void *oa = A.h_dFinalKeys[tid] + A.h_offsetOff[tid], *ka = A.d_csrk_ + A.h_partOff[tid];
   const void *ob = B.h_dFinalKeys[tid] + B.h_offsetOff[tid], *kb = B.d_csrk_ + B.h_partOff[tid];
   uint64_t cnt = 0;
+  if constexpr (MM) {
+    // src/MultiHashGraph.cu:664-670: managed memory for the join's arrays;
+    // the tables stream through HBM slab by slab
+    ok(hg_intersect_staged(A.ctx_, &dA, oa, ka, &dB, ob, kb, nullptr, 0, &cnt, A.stagedBudget),
+       "hg_intersect_staged");
+    h_Common[tid] = (index_t)cnt;
+    if (!cnt) return;
+    table_alloc<MM>(&h_dOutput[tid], cnt * sizeof(keypair), "allocate output");
+    if (sizeof(K) == 8) {
+      ok(hg_intersect_staged(A.ctx_, &dA, oa, ka, &dB, ob, kb, h_dOutput[tid], cnt, &cnt, A.stagedBudget),
+         "hg_intersect_staged");
+      return;
+    }
+    std::vector<uint32_t> t(cnt);
+    ok(hg_intersect_staged(A.ctx_, &dA, oa, ka, &dB, ob, kb, t.data(), cnt, &cnt, A.stagedBudget),
+       "hg_intersect_staged");
+    for (uint64_t i = 0; i < cnt; i++) h_dOutput[tid][i].right = (index_t)t[i];
+    return;
+  }
   ok(hg_intersect(A.ctx_, &dA, oa, ka, &dB, ob, kb, nullptr, 0, &cnt), "hg_intersect");
   h_Common[tid] = (index_t)cnt;
   if (!cnt) return;
@@ -233,8 +296,8 @@ void BasicMultiHashGraph<K, IT>::intersect(BasicMultiHashGraph &A, BasicMultiHas
 // then, for every partition and local bin, the degree and the sorted key
 // list must equal those of global bin j + splits[i]; errors go to stderr.
 // With INDEX_TRACK every entry's position must also point at its key.
-template <typename K, bool IT>
-void BasicMultiHashGraph<K, IT>::buildSingle() {
+template <typename K, bool IT, bool MM>
+void BasicMultiHashGraph<K, IT, MM>::buildSingle() {
   if (!built_) throw hashgraph::error(HG_ERR_INVALID_ARG, "buildSingle: build() first");
   hkey_t *d_k = nullptr, *d_csr = nullptr;
   index_t *d_off = nullptr;
@@ -250,9 +313,9 @@ void BasicMultiHashGraph<K, IT>::buildSingle() {
   hip_ok(hipMemcpy(edges.data(), d_csr, edges.size() * sizeof(hkey_t), hipMemcpyDeviceToHost), "copy");
   std::vector<keyval> mk(countSize);
   std::vector<index_t> moff(tableSize + gpuCount);
-  hip_ok(hipMemcpy(mk.data(), blob_, mk.size() * sizeof(keyval), hipMemcpyDeviceToHost), "copy");
+  hip_ok(hipMemcpy(mk.data(), blob_, mk.size() * sizeof(keyval), hipMemcpyDefault), "copy");
   hip_ok(hipMemcpy(moff.data(), blob_ + (size_t)std::max<index_t>(countSize, 1) * sizeof(keyval),
-                   moff.size() * sizeof(index_t), hipMemcpyDeviceToHost),
+                   moff.size() * sizeof(index_t), hipMemcpyDefault),
          "copy");
   for (index_t i = 0; i < gpuCount; i++) {
     const index_t lo = h_binSplits[i], R = h_binSplits[i + 1] - lo;
@@ -280,9 +343,13 @@ void BasicMultiHashGraph<K, IT>::buildSingle() {
   (void)hipFree(d_off);
 }
 
-template class BasicMultiHashGraph<int64_t, false>;
-template class BasicMultiHashGraph<uint32_t, false>;
-template class BasicMultiHashGraph<int64_t, true>;
-template class BasicMultiHashGraph<uint32_t, true>;
+template class BasicMultiHashGraph<int64_t, false, false>;
+template class BasicMultiHashGraph<uint32_t, false, false>;
+template class BasicMultiHashGraph<int64_t, true, false>;
+template class BasicMultiHashGraph<uint32_t, true, false>;
+template class BasicMultiHashGraph<int64_t, false, true>;
+template class BasicMultiHashGraph<uint32_t, false, true>;
+template class BasicMultiHashGraph<int64_t, true, true>;
+template class BasicMultiHashGraph<uint32_t, true, true>;
 
 }  // namespace hashgraph

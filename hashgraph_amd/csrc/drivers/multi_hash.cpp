@@ -8,6 +8,9 @@
 // build B (sharing A's splits) + intersect, as the reference does
 // (test/MultiHashGraphTest.cu:288-338).  "check" runs buildSingle() and, for
 // intersect, the reference's CPU answer (sort both, count equal pairs).
+// Built with -DMANAGED_MEM (multi-hash-managed, -32-managed, -idx-managed) the
+// tables live in managed memory and stream through HBM; HG_STAGED_BUDGET
+// (bytes) then caps the device memory of each staged call.
 #include <hip/hip_runtime.h>
 #include <stdlib.h>
 #include <string.h>
@@ -33,6 +36,14 @@ static void generateInput(MultiHashGraph::inputData *h_dVals, index_t countSize,
     if (n) hipMemcpy(h_dVals[i].d_keys, h.data(), n * sizeof(hkey_t), hipMemcpyHostToDevice);
     h_dVals[i].len = n;
   }
+}
+
+static void set_budget(MultiHashGraph &g) {
+#ifdef MANAGED_MEM
+  if (const char *e = getenv("HG_STAGED_BUDGET")) g.stagedBudget = (size_t)strtoull(e, nullptr, 0);
+#else
+  (void)g;
+#endif
 }
 
 int main(int argc, char **argv) {
@@ -64,6 +75,7 @@ int main(int argc, char **argv) {
       std::vector<MultiHashGraph::inputData> in(gpuCount);
       generateInput(in.data(), countSizeA, gpuCount);
       MultiHashGraph mhg(in.data(), countSizeA, maxkey, tableSize, binCount, lrbBins, gpuCount);
+      set_budget(mhg);
       hipEventRecord(start);
       for (uint32_t t = 0; t < gpuCount; t++) mhg.build(true, t);
       hipEventRecord(stop);
@@ -77,6 +89,8 @@ int main(int argc, char **argv) {
       generateInput(inB.data(), countSizeB, gpuCount);
       MultiHashGraph A(inA.data(), countSizeA, maxkey, tableSize, binCount, lrbBins, gpuCount);
       MultiHashGraph B(inB.data(), countSizeB, maxkey, tableSize, binCount, lrbBins, gpuCount);
+      set_budget(A);
+      set_budget(B);
       std::vector<MultiHashGraph::keypair *> out(gpuCount, nullptr);
       std::vector<index_t> common(gpuCount, 0);
       hipEventRecord(start);
@@ -98,7 +112,7 @@ int main(int argc, char **argv) {
         for (uint32_t t = 0; t < gpuCount; t++) {
           std::vector<MultiHashGraph::keypair> part(common[t]);
           if (common[t])
-            hipMemcpy(part.data(), out[t], common[t] * sizeof(MultiHashGraph::keypair), hipMemcpyDeviceToHost);
+            hipMemcpy(part.data(), out[t], common[t] * sizeof(MultiHashGraph::keypair), hipMemcpyDefault);
           for (const auto &kp : part) result.push_back((hkey_t)kp.right);
         }
         std::sort(A.h_vals, A.h_vals + countSizeA);

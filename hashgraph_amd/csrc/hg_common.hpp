@@ -4,6 +4,10 @@
 
 namespace hg {
 
+// Most partitions a split-point segment may have (its splits are staged in
+// LDS by the partition kernels).
+constexpr uint32_t MAX_SPLITS = 4096;
+
 // Division of a 32-bit value by an invariant d (1 <= d < 2^32) with one
 // 32-bit multiply-high (Granlund & Montgomery, "Division by invariant
 // integers using multiplication", PLDI 1994), l = floor(log2 d):

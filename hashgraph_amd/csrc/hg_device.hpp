@@ -281,10 +281,6 @@ __device__ __forceinline__ uint32_t split_bucket(uint32_t bin, const int64_t *sp
   return p;
 }
 
-// Most partitions a split-point segment may have (its splits are staged in
-// LDS by the partition kernels).
-constexpr uint32_t MAX_SPLITS = 4096;
-
 // Bucket of a bin; spl: the segment's splits (an LDS copy, or sd.splits).
 template <int SK = SK_ANY>
 __device__ __forceinline__ uint32_t bucket_of_bin(uint32_t bin, const SegDesc &sd, const int64_t *spl) {

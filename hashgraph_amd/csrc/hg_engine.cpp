@@ -130,9 +130,10 @@ struct hg_ctx_s {
   // arena 0 holds an operation's large buffers (partitioned copies), arena 1
   // one partition level's descriptors and matrices; arena 1 may be regrown
   // between levels without touching arena 0.
-  // arena 2 holds the records of the index-carrying builds
-  char *arena[3] = {nullptr, nullptr, nullptr};
-  size_t arena_cap[3] = {0, 0, 0};
+  // arena 2 holds the records of the index-carrying builds, arena 3 the
+  // chunk and slab buffers of the staged (host-resident) tables
+  char *arena[4] = {nullptr, nullptr, nullptr, nullptr};
+  size_t arena_cap[4] = {0, 0, 0, 0};
   static constexpr int kSlots = 4;
   char *hslot[kSlots] = {nullptr, nullptr, nullptr, nullptr};
   size_t hslot_cap[kSlots] = {0, 0, 0, 0};
@@ -467,7 +468,9 @@ hg_status setup_level(hg_ctx c, Level &L, LevelDev &d) {
   return upload(c, blob, b1.data.data(), b1.data.size());
 }
 
-hg_status run_level(hg_ctx c, const Level &L, const LevelDev &d, int kb, const void *in, void *out) {
+// scatter = false stops after the column scan (bucket sizes only).
+hg_status run_level(hg_ctx c, const Level &L, const LevelDev &d, int kb, const void *in, void *out,
+                    bool scatter = true) {
   LaunchPart p{};
   p.key_bytes = kb;
   p.hk = L.hash_kind();
@@ -502,7 +505,7 @@ hg_status run_level(hg_ctx c, const Level &L, const LevelDev &d, int kb, const v
     Prof pf(c, "col_scan");
     HG_HIP(launch_colscan(lc, c->stream));
   }
-  if (p.ntiles) {
+  if (p.ntiles && scatter) {
     Prof pf(c, "part_scatter");
     HG_HIP(launch_part_scatter(p, c->stream));
   }
@@ -597,21 +600,21 @@ hg_status finish_two_level(hg_ctx c, const std::vector<Coarse> &cs, const BinMap
   return run_final(c, L2, d2, kb, ob, part2, csr, off);
 }
 
-// Driver of hg_build.
-hg_status build_table(hg_ctx c, const hg_table_desc *desc, const void *d_keys, void *d_off, void *d_csr) {
-  const uint64_t n = desc->n_keys, M = desc->n_bins;
-  const int kb = (int)desc->key_bytes, ob = (int)desc->offset_bytes;
-  const BinMap map = make_map(M, desc->hash_kind, desc->d_rmi_model);
-  if (n == 0) return fill_offsets(c, d_off, ob, M + 1, 0);
-  const uint32_t W = choose_width(n, M, kb, build_fill());
-  const uint64_t F = (M + W - 1) / W;
+// CSR of n keys whose bins all lie in [lo, lo + R): d_off[0..R] with values
+// 0..n (positions in d_csr) and d_csr.  hg_build is the range [0, M); a
+// staged build runs it once per slab of bins.
+hg_status build_range(hg_ctx c, const BinMap &map, int kb, int ob, uint64_t n, uint64_t lo, uint64_t R,
+                      const void *d_keys, void *d_off, void *d_csr) {
+  if (n == 0) return fill_offsets(c, d_off, ob, R + 1, 0);
+  const uint32_t W = choose_width(n, R, kb, build_fill());
+  const uint64_t F = (R + W - 1) / W;
   const uint64_t k = level1_group(F);
   if (k == 1) {
     Level L;
-    SegDesc s = range_seg(0, n, 0, M, W, map);
+    SegDesc s = range_seg(0, n, lo, R, W, map);
     s.fin_off_base = 0;
     s.fin_key_rel = 0;
-    s.fin_end_pos = M;
+    s.fin_end_pos = R;
     s.table = 0;
     L.segs.push_back(s);
     L.maps.push_back(map);
@@ -633,7 +636,7 @@ hg_status build_table(hg_ctx c, const hg_table_desc *desc, const void *d_keys, v
   const uint64_t W1 = (uint64_t)W * k;
   if (W1 >= (1ull << 32)) return fail(HG_ERR_UNSUPPORTED, "table geometry too large");
   Level L1;
-  L1.segs.push_back(range_seg(0, n, 0, M, (uint32_t)W1, map));
+  L1.segs.push_back(range_seg(0, n, lo, R, (uint32_t)W1, map));
   L1.maps.push_back(map);
   L1.auto_tile(n);
   L1.prepare();
@@ -653,14 +656,21 @@ hg_status build_table(hg_ctx c, const hg_table_desc *desc, const void *d_keys, v
     Coarse q;
     q.key_begin = st[g];
     q.count = cnt[g];
-    q.bin_lo = (uint64_t)g * W1;
-    q.nbins = std::min<uint64_t>(W1, M - q.bin_lo);
-    q.off_base = q.bin_lo;
+    q.off_base = (uint64_t)g * W1;
+    q.bin_lo = lo + q.off_base;
+    q.nbins = std::min<uint64_t>(W1, R - q.off_base);
     q.key_rel = 0;
-    q.end_pos = (g + 1 == L1.ncols_total) ? M : ~0ull;
+    q.end_pos = (g + 1 == L1.ncols_total) ? R : ~0ull;
     cs.push_back(q);
   }
   return finish_two_level(c, cs, map, kb, ob, cv0.get<void>(i_part1), cv0.get<void>(i_part2), d_csr, d_off);
+}
+
+// Driver of hg_build.
+hg_status build_table(hg_ctx c, const hg_table_desc *desc, const void *d_keys, void *d_off, void *d_csr) {
+  const BinMap map = make_map(desc->n_bins, desc->hash_kind, desc->d_rmi_model);
+  return build_range(c, map, (int)desc->key_bytes, (int)desc->offset_bytes, desc->n_keys, 0, desc->n_bins, d_keys,
+                     d_off, d_csr);
 }
 
 // Heavy bins of the run-length join with more distinct values than the LDS
@@ -806,6 +816,8 @@ hg_status check_partitioned(hg_ctx c, const hg_table_desc *desc, const void *d_k
 hg_status index_records(hg_ctx c, uint64_t n, uint64_t *&rin, uint64_t *&rout);
 hg_status build_partitioned(hg_ctx c, const hg_table_desc *desc, const void *d_keys, const int64_t *h_splits,
                             uint32_t n_parts, void *d_offsets, void *d_csr_keys, uint64_t *h_part_offsets);
+void splits_from_hist(const std::vector<unsigned long long> &h, uint64_t bin_count, uint64_t n, uint32_t n_parts,
+                      uint64_t range, uint64_t M, int64_t *h_splits);
 
 // Launches of the batched trainer: items sorted by sample size and split at
 // 2048 keys, so that the many small samples run with small LDS (several
@@ -887,7 +899,7 @@ hg_status hg_ctx_destroy(hg_ctx c) {
   if (!c) return HG_OK;
   DeviceGuard g(c->device);
   (void)hipStreamSynchronize(c->stream);
-  for (int a = 0; a < 3; a++)
+  for (int a = 0; a < 4; a++)
     if (c->arena[a]) (void)hipFree(c->arena[a]);
   for (int i = 0; i < hg_ctx_s::kSlots; i++) {
     if (c->hslot[i]) (void)hipHostFree(c->hslot[i]);
@@ -1582,20 +1594,7 @@ hg_status hg_find_splits(hg_ctx c, const hg_table_desc *desc, const void *d_keys
   std::vector<unsigned long long> h(bin_count + 1);
   HG_TRY(sync_read(c, h.data(), hist, (bin_count + 1) * 8));
   if (h[bin_count]) return fail(HG_ERR_INVALID_ARG, "bin range too small for bin_count (float rounding)");
-  // prefix as in include/MultiHashGraphHostOperators.cuh:55-59
-  std::vector<int64_t> ps(bin_count + 1);
-  ps[0] = 0;
-  for (uint64_t i = 1; i < bin_count; i++) ps[i] = ps[i - 1] + (int64_t)h[i - 1];
-  ps[bin_count] = (int64_t)n;
-  // avgKeyCount = ceil(countSize / (float)gpuCount)  (HostOperators.cuh:83)
-  const int64_t avg = (int64_t)std::ceil((float)n / (float)n_parts);
-  h_splits[0] = 0;
-  for (uint32_t t = 0; t < n_parts; t++) {
-    const uint64_t upper = (uint64_t)(avg * (int64_t)(t + 1));
-    const uint64_t idx = ref_binary_search(ps, 0, bin_count, upper);
-    const uint64_t minRange = idx * range;
-    h_splits[t + 1] = (int64_t)std::min<uint64_t>(minRange, M);
-  }
+  splits_from_hist(h, bin_count, n, n_parts, range, M, h_splits);
   return HG_OK;
 }
 
@@ -1737,6 +1736,438 @@ hg_status build_partitioned(hg_ctx c, const hg_table_desc *desc, const void *d_k
 }
 
 }  // namespace
+
+// ---------------------------------------------------------------------------
+// Staged tables: keys, offsets and CSR keys stay in host memory (pageable,
+// pinned, or managed memory that the device cannot hold) and pass through
+// HBM in chunks and bin-range slabs.  They replace the reference's MANAGED_MEM
+// mode (src/MultiHashGraph.cu:229-247, include/MultiHashGraphHostOperators.cuh:
+// 255-280, src/MultiHashGraph.cu:664-670), which puts a table in one
+// cudaMallocManaged blob and lets page faults move it: on a device without
+// demand paging, sequential DMA of whole slabs is how a table larger than
+// HBM is built.
+//   pass 1  each chunk of keys is uploaded and counted into pieces (bin
+//           ranges of at most wf bins inside one partition, <= MAX_SPLITS);
+//   pass 2  each chunk is uploaded again, partitioned by piece, and every
+//           slab's share is copied to the slab's final place in the caller's
+//           CSR keys (slab = consecutive pieces of one partition whose keys
+//           and bins fit the budget);
+//   pass 3  each slab is uploaded, built as a bin range in HBM, and its CSR
+//           keys and rebased offsets are copied back over its region.
+// Host-link traffic: 3 N s up, 2 N s down, offsets once down.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr size_t kStageMinBudget = 4ull << 20;
+constexpr uint64_t kStageMaxChunk = 1ull << 31;  // 32-bit key positions inside the kernels
+
+struct StageGeom {
+  uint64_t ks = 0;    // keys per chunk and per slab
+  uint64_t rmax = 0;  // bins per slab
+};
+
+// Device use of a build: the chunk or slab keys and the slab's CSR keys
+// (arena 3), one or two partitioned copies (arena 0), about 0.5 byte per key
+// of level matrices and per-bin counters of R bins (arena 1), the slab's
+// offsets (arena 3).  4/5 of the budget go to keys, 1/5 to bins.
+hg_status stage_geometry(hg_ctx c, int kb, int ob, size_t &budget, StageGeom &g) {
+  if (budget == 0) {
+    size_t fr = 0, tot = 0;
+    HG_HIP(hipMemGetInfo(&fr, &tot));
+    budget = fr / 2;
+  }
+  if (budget < kStageMinBudget) return fail(HG_ERR_INVALID_ARG, "device_budget below 4 MiB");
+  g.ks = std::min<uint64_t>((uint64_t)(0.8 * (double)budget / (4.0 * kb + 1.0)), kStageMaxChunk);
+  g.rmax = std::min<uint64_t>((uint64_t)(0.2 * (double)budget / (ob + 4.0)), (1ull << 32) - 1);
+  return HG_OK;
+}
+
+hg_status stage_copy(hg_ctx c, void *dst, const void *src, size_t bytes) {
+  if (bytes) HG_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, c->stream));
+  return HG_OK;
+}
+
+struct Pieces {
+  std::vector<int64_t> bounds;  // piece p holds bins [bounds[p], bounds[p+1])
+  std::vector<uint32_t> part;   // its partition
+};
+
+hg_status make_pieces(const int64_t *splits, uint32_t n_parts, uint64_t rmax, Pieces &pc) {
+  const uint64_t top = (uint64_t)splits[n_parts];
+  const uint64_t avail = MAX_SPLITS - n_parts;
+  const uint64_t wf = std::max<uint64_t>(1, (top + avail - 1) / avail);
+  if (wf > rmax) return fail(HG_ERR_UNSUPPORTED, "device_budget too small for the table's bin count");
+  for (uint32_t p = 0; p < n_parts; p++)
+    for (uint64_t b = (uint64_t)splits[p]; b < (uint64_t)splits[p + 1]; b += wf) {
+      pc.bounds.push_back((int64_t)b);
+      pc.part.push_back(p);
+    }
+  pc.bounds.push_back((int64_t)top);
+  return HG_OK;
+}
+
+// One kind-1 level over n keys: bucket p = piece p, the last column drops
+// bins at or above the last bound.
+void piece_level(Level &L, const BinMap &map, const Pieces &pc, uint64_t n) {
+  const uint32_t np = (uint32_t)pc.part.size();
+  SegDesc s{};
+  s.key_begin = 0;
+  s.key_end = n;
+  s.map = map;
+  s.div = make_div(1);
+  s.kind = 1;
+  s.nbuckets = np;
+  s.ncols = np + 1;
+  s.nsplits = np;
+  s.fin_end_pos = ~0ull;
+  L.segs.push_back(s);
+  L.maps.push_back(map);
+  L.splits = pc.bounds;
+  L.auto_tile(n);
+  L.prepare();
+}
+
+struct Slab {
+  uint32_t p0 = 0, p1 = 0, part = 0;
+  uint64_t lo = 0, R = 0, count = 0, pos = 0;
+};
+
+// splits over [0, M) of n_parts partitions; h_part_offsets gets the key
+// start of every partition (n_parts + 1 entries)
+hg_status staged_partitioned(hg_ctx c, const hg_table_desc *desc, const void *h_keys, const int64_t *splits,
+                             uint32_t n_parts, void *h_off, void *h_csr, uint64_t *h_part_offsets, size_t budget) {
+  const uint64_t n = desc->n_keys;
+  const int kb = (int)desc->key_bytes, ob = (int)desc->offset_bytes;
+  const BinMap map = make_map(desc->n_bins, desc->hash_kind, desc->d_rmi_model);
+  StageGeom g;
+  HG_TRY(stage_geometry(c, kb, ob, budget, g));
+  Pieces pc;
+  HG_TRY(make_pieces(splits, n_parts, g.rmax, pc));
+  const uint32_t np = (uint32_t)pc.part.size();
+  // arena 3: chunk / slab keys, slab CSR keys, slab offsets
+  Plan P3;
+  const size_t i_in = P3.add(std::max<uint64_t>(std::min(g.ks, n), 1) * kb);
+  const size_t i_out = P3.add(std::max<uint64_t>(std::min(g.ks, n), 1) * kb);
+  const size_t i_off = P3.add((std::min<uint64_t>(g.rmax, desc->n_bins) + 1) * ob);
+  HG_TRY(arena_ensure(c, 3, P3.total()));
+  Carve cv3;
+  cv3.init(c, 3, P3);
+  char *d_in = cv3.get<char>(i_in), *d_out = cv3.get<char>(i_out);
+  void *d_off = cv3.get<void>(i_off);
+  std::vector<uint32_t> cnt(np + 1), st(np + 1);
+  // pass 1: keys per piece
+  std::vector<uint64_t> T(np, 0);
+  for (uint64_t k0 = 0; k0 < n; k0 += g.ks) {
+    const uint64_t cn = std::min(g.ks, n - k0);
+    HG_TRY(stage_copy(c, d_in, (const char *)h_keys + k0 * kb, cn * kb));
+    Level L;
+    piece_level(L, map, pc, cn);
+    LevelDev d;
+    HG_TRY(setup_level(c, L, d));
+    HG_TRY(run_level(c, L, d, kb, d_in, nullptr, false));
+    HG_TRY(sync_read(c, cnt.data(), d.coltot, (np + 1) * 4));
+    for (uint32_t p = 0; p < np; p++) T[p] += cnt[p];
+  }
+  // slabs: consecutive pieces of one partition within ks keys and rmax bins
+  std::vector<Slab> slabs;
+  std::vector<uint64_t> po(n_parts + 1, 0);
+  uint64_t pos = 0;
+  for (uint32_t p = 0; p < np;) {
+    Slab sl;
+    sl.p0 = sl.p1 = p;
+    sl.part = pc.part[p];
+    sl.lo = (uint64_t)pc.bounds[p];
+    sl.pos = pos;
+    while (p < np && pc.part[p] == sl.part) {
+      const uint64_t pb = (uint64_t)(pc.bounds[p + 1] - pc.bounds[p]);
+      if (T[p] > g.ks)
+        return fail(HG_ERR_UNSUPPORTED, "bins [" + std::to_string(pc.bounds[p]) + ", " +
+                                            std::to_string(pc.bounds[p + 1]) + ") hold " + std::to_string(T[p]) +
+                                            " keys, more than a slab of this device_budget (" +
+                                            std::to_string(g.ks) + ")");
+      if (sl.p1 > sl.p0 && (sl.count + T[p] > g.ks || sl.R + pb > g.rmax)) break;
+      sl.count += T[p];
+      sl.R += pb;
+      sl.p1 = ++p;
+    }
+    pos += sl.count;
+    slabs.push_back(sl);
+  }
+  for (const Slab &sl : slabs) po[sl.part + 1] += sl.count;
+  for (uint32_t p = 0; p < n_parts; p++) po[p + 1] += po[p];
+  // pass 2: every chunk's share of a slab goes to the slab's final region
+  std::vector<uint64_t> cur(slabs.size(), 0);
+  for (uint64_t k0 = 0; k0 < n; k0 += g.ks) {
+    const uint64_t cn = std::min(g.ks, n - k0);
+    HG_TRY(stage_copy(c, d_in, (const char *)h_keys + k0 * kb, cn * kb));
+    Level L;
+    piece_level(L, map, pc, cn);
+    Plan P0;
+    const size_t i_part = P0.add(cn * kb);
+    HG_TRY(arena_ensure(c, 0, P0.total()));
+    Carve cv0;
+    cv0.init(c, 0, P0);
+    char *d_part = cv0.get<char>(i_part);
+    LevelDev d;
+    HG_TRY(setup_level(c, L, d));
+    HG_TRY(run_level(c, L, d, kb, d_in, d_part));
+    HG_TRY(sync_read(c, cnt.data(), d.coltot, (np + 1) * 4));
+    HG_TRY(sync_read(c, st.data(), d.bstart, (np + 1) * 4));
+    for (size_t s = 0; s < slabs.size(); s++) {
+      const Slab &sl = slabs[s];
+      const uint64_t a = st[sl.p0], e = (uint64_t)st[sl.p1 - 1] + cnt[sl.p1 - 1];
+      if (cur[s] + (e - a) > sl.count) return fail(HG_ERR_DEVICE, "staged build: keys changed between passes");
+      HG_TRY(stage_copy(c, (char *)h_csr + (sl.pos + cur[s]) * kb, d_part + a * kb, (e - a) * kb));
+      cur[s] += e - a;
+    }
+  }
+  for (size_t s = 0; s < slabs.size(); s++)
+    if (cur[s] != slabs[s].count) return fail(HG_ERR_DEVICE, "staged build: keys changed between passes");
+  // pass 3: build every slab in HBM and write its CSR back
+  for (const Slab &sl : slabs) {
+    HG_TRY(stage_copy(c, d_in, (const char *)h_csr + sl.pos * kb, sl.count * kb));
+    HG_TRY(build_range(c, map, kb, ob, sl.count, sl.lo, sl.R, d_in, d_off, d_out));
+    // offsets of a partitioned table count from the partition's first key
+    HG_HIP(launch_offsets_add(d_off, ob, sl.R + 1, sl.pos - po[sl.part], c->stream));
+    HG_TRY(stage_copy(c, (char *)h_csr + sl.pos * kb, d_out, sl.count * kb));
+    HG_TRY(stage_copy(c, (char *)h_off + (sl.lo + sl.part) * ob, d_off, (sl.R + 1) * ob));
+  }
+  HG_HIP(hipStreamSynchronize(c->stream));
+  // a partition of no bins has the single offset entry 0
+  for (uint32_t p = 0; p < n_parts; p++)
+    if (splits[p + 1] == splits[p]) {
+      char *q = (char *)h_off + ((uint64_t)splits[p] + p) * ob;
+      if (ob == 4) *(uint32_t *)q = 0;
+      else *(uint64_t *)q = 0;
+    }
+  for (uint32_t p = 0; p <= n_parts; p++) h_part_offsets[p] = po[p];
+  return HG_OK;
+}
+
+// include/MultiHashGraphHostOperators.cuh:55-59,83-88: key-balanced split
+// points from the bin_count-range histogram h (h[bin_count] = keys beyond)
+void splits_from_hist(const std::vector<unsigned long long> &h, uint64_t bin_count, uint64_t n, uint32_t n_parts,
+                      uint64_t range, uint64_t M, int64_t *h_splits) {
+  std::vector<int64_t> ps(bin_count + 1);
+  ps[0] = 0;
+  for (uint64_t i = 1; i < bin_count; i++) ps[i] = ps[i - 1] + (int64_t)h[i - 1];
+  ps[bin_count] = (int64_t)n;
+  const int64_t avg = (int64_t)std::ceil((float)n / (float)n_parts);
+  h_splits[0] = 0;
+  for (uint32_t t = 0; t < n_parts; t++) {
+    const uint64_t upper = (uint64_t)(avg * (int64_t)(t + 1));
+    const uint64_t idx = ref_binary_search(ps, 0, bin_count, upper);
+    h_splits[t + 1] = (int64_t)std::min<uint64_t>(idx * range, M);
+  }
+}
+
+uint64_t off_at(const void *off, int ob, uint64_t i) {
+  return ob == 4 ? (uint64_t)((const uint32_t *)off)[i] : ((const uint64_t *)off)[i];
+}
+
+// One slab of bins [lo, hi) of two host tables resident in HBM: offsets
+// rebased to 0, keys from the slab's first key.
+struct JoinSlab {
+  uint64_t lo = 0, hi = 0;
+  uint64_t a0 = 0, b0 = 0;  // first key of A and of B in the host tables
+  void *offA = nullptr, *offB = nullptr;
+  char *keysA = nullptr, *keysB = nullptr;
+};
+
+// Matches of bins [x, y) of the slab: counted, then emitted into d_stage and
+// copied to the host list when they fit the stage (else the range is halved).
+hg_status join_range(hg_ctx c, const hg_table_desc *descA, const void *h_offA, const void *h_offB, const JoinSlab &js,
+                     uint64_t x, uint64_t y, void *d_stage, uint64_t stage_cap, void *h_out, uint64_t capacity,
+                     uint64_t &written, uint64_t &total, bool &full) {
+  const int kb = (int)descA->key_bytes, ob = (int)descA->offset_bytes;
+  hg_table_desc dA = *descA, dB = *descA;
+  dA.n_bins = dB.n_bins = y - x;
+  dA.n_keys = off_at(h_offA, ob, y) - off_at(h_offA, ob, x);
+  dB.n_keys = off_at(h_offB, ob, y) - off_at(h_offB, ob, x);
+  const void *oa = (const char *)js.offA + (x - js.lo) * ob, *obp = (const char *)js.offB + (x - js.lo) * ob;
+  uint64_t cnt = 0;
+  HG_TRY(hg_intersect(c, &dA, oa, js.keysA, &dB, obp, js.keysB, nullptr, 0, &cnt));
+  total += cnt;
+  if (!h_out || full || cnt == 0) return HG_OK;
+  if (written + cnt > capacity) {
+    full = true;
+    return HG_OK;
+  }
+  if (cnt > stage_cap) {
+    if (y - x == 1)
+      return fail(HG_ERR_UNSUPPORTED, "bin " + std::to_string(x) + " has " + std::to_string(cnt) +
+                                          " matches, more than this device_budget stages (" +
+                                          std::to_string(stage_cap) + ")");
+    total -= cnt;
+    const uint64_t m = x + (y - x) / 2;
+    HG_TRY(join_range(c, descA, h_offA, h_offB, js, x, m, d_stage, stage_cap, h_out, capacity, written, total, full));
+    return join_range(c, descA, h_offA, h_offB, js, m, y, d_stage, stage_cap, h_out, capacity, written, total, full);
+  }
+  uint64_t got = 0;
+  HG_TRY(hg_intersect(c, &dA, oa, js.keysA, &dB, obp, js.keysB, d_stage, stage_cap, &got));
+  if (got != cnt) return fail(HG_ERR_DEVICE, "staged intersect: count changed between passes");
+  HG_TRY(stage_copy(c, (char *)h_out + written * kb, d_stage, cnt * kb));
+  HG_HIP(hipStreamSynchronize(c->stream));
+  written += cnt;
+  return HG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+hg_status hg_build_staged(hg_ctx c, const hg_table_desc *desc, const void *h_keys, void *h_offsets,
+                          void *h_csr_keys, size_t device_budget) {
+  if (!c) return fail(HG_ERR_INVALID_ARG, "null ctx");
+  HG_TRY(check_desc(desc, false));
+  if (!h_offsets || (desc->n_keys && (!h_keys || !h_csr_keys))) return fail(HG_ERR_INVALID_ARG, "null buffer");
+  DeviceGuard g(c->device);
+  const int64_t splits[2] = {0, (int64_t)desc->n_bins};
+  uint64_t po[2];
+  return staged_partitioned(c, desc, h_keys, splits, 1, h_offsets, h_csr_keys, po, device_budget);
+}
+
+hg_status hg_build_partitioned_staged(hg_ctx c, const hg_table_desc *desc, const void *h_keys,
+                                      const int64_t *h_splits, uint32_t n_parts, void *h_offsets, void *h_csr_keys,
+                                      uint64_t *h_part_offsets, size_t device_budget) {
+  if (!c) return fail(HG_ERR_INVALID_ARG, "null ctx");
+  HG_TRY(check_desc(desc, false));
+  if (!h_splits || n_parts == 0 || n_parts > MAX_SPLITS / 2 || !h_offsets || !h_part_offsets)
+    return fail(HG_ERR_INVALID_ARG, "bad partition arguments (n_parts must be in [1, 2048])");
+  if (h_splits[0] != 0) return fail(HG_ERR_INVALID_ARG, "splits[0] must be 0");
+  for (uint32_t p = 0; p < n_parts; p++)
+    if (h_splits[p + 1] < h_splits[p] || (uint64_t)h_splits[p + 1] > desc->n_bins)
+      return fail(HG_ERR_INVALID_ARG, "splits must be non-decreasing within [0, M]");
+  if (desc->n_keys && (!h_keys || !h_csr_keys)) return fail(HG_ERR_INVALID_ARG, "null buffer");
+  DeviceGuard g(c->device);
+  return staged_partitioned(c, desc, h_keys, h_splits, n_parts, h_offsets, h_csr_keys, h_part_offsets,
+                            device_budget);
+}
+
+hg_status hg_find_splits_staged(hg_ctx c, const hg_table_desc *desc, const void *h_keys, uint64_t bin_count,
+                                uint32_t n_parts, int64_t *h_splits, size_t device_budget) {
+  if (!c) return fail(HG_ERR_INVALID_ARG, "null ctx");
+  HG_TRY(check_desc(desc, false));
+  if (!h_splits || n_parts == 0 || bin_count == 0 || bin_count > 32768)
+    return fail(HG_ERR_INVALID_ARG, "bad split arguments (bin_count must be in [1, 32768])");
+  if (desc->n_keys && !h_keys) return fail(HG_ERR_INVALID_ARG, "null keys");
+  DeviceGuard g(c->device);
+  const uint64_t M = desc->n_bins, n = desc->n_keys;
+  const int kb = (int)desc->key_bytes;
+  const uint64_t range = (uint64_t)std::ceil((float)M / (float)bin_count);
+  if (range == 0) return fail(HG_ERR_INVALID_ARG, "bin range is zero");
+  StageGeom geo;
+  HG_TRY(stage_geometry(c, kb, (int)desc->offset_bytes, device_budget, geo));
+  Plan P3;
+  const size_t i_in = P3.add(std::max<uint64_t>(std::min(geo.ks, n), 1) * kb);
+  HG_TRY(arena_ensure(c, 3, P3.total()));
+  Carve cv3;
+  cv3.init(c, 3, P3);
+  char *d_in = cv3.get<char>(i_in);
+  Plan P;
+  const size_t i_hist = P.add(bin_count * 8 + 64);
+  HG_TRY(arena_ensure(c, 1, P.total()));
+  Carve cv;
+  cv.init(c, 1, P);
+  unsigned long long *hist = cv.get<unsigned long long>(i_hist);
+  unsigned int *bad = (unsigned int *)(hist + bin_count);
+  HG_HIP(launch_fill_u64(hist, 0ull, bin_count + 1, c->stream));
+  const BinMap map = make_map(M, desc->hash_kind, desc->d_rmi_model);
+  for (uint64_t k0 = 0; k0 < n; k0 += geo.ks) {
+    const uint64_t cn = std::min(geo.ks, n - k0);
+    HG_TRY(stage_copy(c, d_in, (const char *)h_keys + k0 * kb, cn * kb));
+    HG_HIP(launch_range_hist(d_in, cn, kb, map, (uint32_t)range, (uint32_t)bin_count, hist, bad, c->stream));
+  }
+  std::vector<unsigned long long> h(bin_count + 1);
+  HG_TRY(sync_read(c, h.data(), hist, (bin_count + 1) * 8));
+  if (h[bin_count]) return fail(HG_ERR_INVALID_ARG, "bin range too small for bin_count (float rounding)");
+  splits_from_hist(h, bin_count, n, n_parts, range, M, h_splits);
+  return HG_OK;
+}
+
+hg_status hg_intersect_staged(hg_ctx c, const hg_table_desc *descA, const void *h_offA, const void *h_keysA,
+                              const hg_table_desc *descB, const void *h_offB, const void *h_keysB, void *h_out,
+                              uint64_t capacity, uint64_t *h_count, size_t device_budget) {
+  if (!c) return fail(HG_ERR_INVALID_ARG, "null ctx");
+  HG_TRY(check_desc(descA, false));
+  HG_TRY(check_desc(descB, false));
+  if (!h_count) return fail(HG_ERR_INVALID_ARG, "null h_count");
+  if (descA->n_bins != descB->n_bins || descA->key_bytes != descB->key_bytes ||
+      descA->hash_kind != descB->hash_kind || descA->offset_bytes != descB->offset_bytes)
+    return fail(HG_ERR_MISMATCH, "tables do not share a bin space");
+  if (!h_offA || !h_offB || (descA->n_keys && !h_keysA) || (descB->n_keys && !h_keysB))
+    return fail(HG_ERR_INVALID_ARG, "null buffer");
+  DeviceGuard g(c->device);
+  *h_count = 0;
+  const uint64_t M = descA->n_bins;
+  const int kb = (int)descA->key_bytes, ob = (int)descA->offset_bytes;
+  if (off_at(h_offA, ob, M) != descA->n_keys || off_at(h_offB, ob, M) != descB->n_keys)
+    return fail(HG_ERR_INVALID_ARG, "offsets[n_bins] differs from n_keys");
+  if (device_budget == 0) {
+    size_t fr = 0, tot = 0;
+    HG_HIP(hipMemGetInfo(&fr, &tot));
+    device_budget = fr / 2;
+  }
+  if (device_budget < kStageMinBudget) return fail(HG_ERR_INVALID_ARG, "device_budget below 4 MiB");
+  // half of the budget for the slab's keys of A and B, 1/5 for its offsets,
+  // the rest for staged matches
+  const uint64_t kmax = std::min<uint64_t>((uint64_t)(0.5 * (double)device_budget / kb), kStageMaxChunk);
+  const uint64_t rmax = std::min<uint64_t>((uint64_t)(0.1 * (double)device_budget / ob), (1ull << 32) - 2);
+  const uint64_t stage_cap = (uint64_t)(0.25 * (double)device_budget / kb);
+  Plan P3;
+  const size_t i_ka = P3.add(kmax * kb), i_oa = P3.add((rmax + 1) * ob), i_ob = P3.add((rmax + 1) * ob),
+               i_st = P3.add(stage_cap * kb);
+  HG_TRY(arena_ensure(c, 3, P3.total()));
+  Carve cv3;
+  cv3.init(c, 3, P3);
+  JoinSlab js;
+  js.offA = cv3.get<void>(i_oa);
+  js.offB = cv3.get<void>(i_ob);
+  char *d_keys = cv3.get<char>(i_ka);
+  void *d_stage = cv3.get<void>(i_st);
+  auto keys_in = [&](uint64_t lo, uint64_t hi) {
+    return (off_at(h_offA, ob, hi) - off_at(h_offA, ob, lo)) + (off_at(h_offB, ob, hi) - off_at(h_offB, ob, lo));
+  };
+  uint64_t written = 0, total = 0;
+  bool full = false;
+  for (uint64_t lo = 0; lo < M;) {
+    // the largest hi with the slab's keys within kmax and bins within rmax
+    uint64_t top = std::min<uint64_t>(M, lo + rmax);
+    if (keys_in(lo, lo + 1) > kmax)
+      return fail(HG_ERR_UNSUPPORTED, "bin " + std::to_string(lo) + " holds " + std::to_string(keys_in(lo, lo + 1)) +
+                                          " keys of A and B, more than this device_budget stages (" +
+                                          std::to_string(kmax) + ")");
+    uint64_t good = lo + 1;
+    while (good < top) {
+      const uint64_t mid = good + (top - good + 1) / 2;
+      if (keys_in(lo, mid) <= kmax) good = mid;
+      else top = mid - 1;
+    }
+    const uint64_t hi = good;
+    js.lo = lo;
+    js.hi = hi;
+    js.a0 = off_at(h_offA, ob, lo);
+    js.b0 = off_at(h_offB, ob, lo);
+    const uint64_t na = off_at(h_offA, ob, hi) - js.a0, nb = off_at(h_offB, ob, hi) - js.b0;
+    js.keysA = d_keys;
+    js.keysB = d_keys + na * kb;
+    HG_TRY(stage_copy(c, js.keysA, (const char *)h_keysA + js.a0 * kb, na * kb));
+    HG_TRY(stage_copy(c, js.keysB, (const char *)h_keysB + js.b0 * kb, nb * kb));
+    HG_TRY(stage_copy(c, js.offA, (const char *)h_offA + lo * ob, (hi - lo + 1) * ob));
+    HG_TRY(stage_copy(c, js.offB, (const char *)h_offB + lo * ob, (hi - lo + 1) * ob));
+    HG_HIP(launch_offsets_add(js.offA, ob, hi - lo + 1, (uint64_t)0 - js.a0, c->stream));
+    HG_HIP(launch_offsets_add(js.offB, ob, hi - lo + 1, (uint64_t)0 - js.b0, c->stream));
+    HG_TRY(join_range(c, descA, h_offA, h_offB, js, lo, hi, d_stage, stage_cap, h_out, capacity, written, total,
+                      full));
+    lo = hi;
+  }
+  HG_HIP(hipStreamSynchronize(c->stream));
+  *h_count = total;
+  if (h_out && total > capacity) return fail(HG_ERR_CAPACITY, "output capacity smaller than the match count");
+  return HG_OK;
+}
+
+}  // extern "C"
 
 // ---------------------------------------------------------------------------
 // Batched tables (no counterpart in the reference).  Every table is a range

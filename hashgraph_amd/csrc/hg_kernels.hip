@@ -896,6 +896,12 @@ __global__ void k_fill(T *__restrict__ p, T v, uint64_t n) {
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) p[i] = v;
 }
 
+// p[i] += d (mod 2^bits of T): rebases a slab's offsets onto table positions
+template <typename T>
+__global__ void k_add(T *__restrict__ p, T d, uint64_t n) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) p[i] += d;
+}
+
 // SplitMix64 counter form x_i = mix64(s0 + (i+1)*gamma); key = x_i reduced
 // to [0, range) by the high half of x_i * range (range 0: no reduction;
 // 32-bit keys take the top 32 bits).
@@ -3093,6 +3099,15 @@ hipError_t launch_fill_u64(unsigned long long *p, unsigned long long v, uint64_t
 }
 hipError_t launch_fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t st) {
   hipLaunchKernelGGL((k_fill<uint32_t>), dim3(256), dim3(256), 0, st, p, v, n);
+  return hipGetLastError();
+}
+hipError_t launch_offsets_add(void *off, int off_bytes, uint64_t n, uint64_t delta, hipStream_t st) {
+  if (!n) return hipSuccess;
+  const uint32_t grid = (uint32_t)std::min<uint64_t>((n + 255) / 256, 4096);
+  if (off_bytes == 4)
+    hipLaunchKernelGGL((k_add<uint32_t>), dim3(grid), dim3(256), 0, st, (uint32_t *)off, (uint32_t)delta, n);
+  else
+    hipLaunchKernelGGL((k_add<uint64_t>), dim3(grid), dim3(256), 0, st, (uint64_t *)off, delta, n);
   return hipGetLastError();
 }
 

@@ -231,6 +231,8 @@ hipError_t launch_pack_index(const void *keys, uint64_t n, int key_bytes, uint64
 hipError_t launch_unpack_index(const uint64_t *rec, uint64_t n, const void *keys, int key_bytes, void *csr_keys,
                                uint64_t *csr_index, hipStream_t st);
 hipError_t launch_fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t st);
+// off[i] += delta modulo 2^(8 off_bytes), i < n
+hipError_t launch_offsets_add(void *off, int off_bytes, uint64_t n, uint64_t delta, hipStream_t st);
 hipError_t launch_bins(const void *keys, uint64_t n, int key_bytes, const BinMap &mp, uint32_t *bins, hipStream_t st);
 hipError_t launch_range_hist(const void *keys, uint64_t n, int key_bytes, const BinMap &mp, uint32_t range,
                              uint32_t nranges, unsigned long long *hist, unsigned int *bad, hipStream_t st);

@@ -17,6 +17,12 @@
 // The reference's header defines INDEX_TRACK itself; here both switches are
 // opt-in compiler definitions (-DB32, -DINDEX_TRACK), so the default is the
 // 64-bit, index-free form of its headline experiments.  Offsets are index_t.
+// MANAGED_MEM (include/MultiHashGraph.cuh:63, src/MultiHashGraph.cu:229-247)
+// puts the table's storage (the key copy, CSR entries and offsets, and the
+// intersect's output) in hipMallocManaged memory prefetched to the device, and
+// builds and joins through the staged calls of hashgraph.h, which stream it
+// through HBM in slabs of at most stagedBudget bytes of device memory: the
+// table may be larger than the GPU's memory.
 // Ownership as in the reference: build() frees the caller's
 // h_dVals[i].d_keys / d_hash; intersect() hipMallocs h_dOutput[tid] (caller
 // frees).  Errors throw hashgraph::error (the reference prints and exits).
@@ -34,7 +40,7 @@
 
 namespace hashgraph {
 
-template <typename KeyT, bool IndexTrack>
+template <typename KeyT, bool IndexTrack, bool Managed = false>
 class BasicMultiHashGraph {
  public:
   using hkey_t = KeyT;
@@ -89,6 +95,10 @@ class BasicMultiHashGraph {
 
   hkey_t *h_vals = nullptr;  // host copy of all keys (for correctness checks)
 
+  // MANAGED_MEM: device memory each staged call may take (0 = half of the
+  // free memory)
+  size_t stagedBudget = 0;
+
   // number of keys of partition tid after the last build
   index_t partitionKeys(index_t tid) const { return h_partOff[tid + 1] - h_partOff[tid]; }
 
@@ -118,4 +128,9 @@ class BasicMultiHashGraph {
 #else
 #define HASHGRAPH_INDEX_TRACK false
 #endif
-using MultiHashGraph = hashgraph::BasicMultiHashGraph<HASHGRAPH_KEY_T, HASHGRAPH_INDEX_TRACK>;
+#ifdef MANAGED_MEM
+#define HASHGRAPH_MANAGED true
+#else
+#define HASHGRAPH_MANAGED false
+#endif
+using MultiHashGraph = hashgraph::BasicMultiHashGraph<HASHGRAPH_KEY_T, HASHGRAPH_INDEX_TRACK, HASHGRAPH_MANAGED>;

@@ -188,6 +188,42 @@ hg_status hg_build_partitioned_indexed(hg_ctx ctx, const hg_table_desc *desc, co
                                        const int64_t *h_splits, uint32_t n_parts, void *d_offsets, void *d_csr_keys,
                                        uint64_t *d_csr_index, uint64_t *h_part_offsets);
 
+/* ---- staged tables: larger than HBM ----------------------------------------
+ * Replace the reference's MANAGED_MEM mode (src/MultiHashGraph.cu:229-247,
+ * include/MultiHashGraphHostOperators.cuh:255-280; intersect
+ * src/MultiHashGraph.cu:664-670), which keeps a table in one
+ * cudaMallocManaged blob and lets page faults move it.  Here keys, offsets,
+ * CSR keys and matched lists stay in host memory (pageable, pinned, or
+ * managed) and pass through HBM in chunks and bin-range slabs by sequential
+ * copies; results are those of hg_build / hg_build_partitioned /
+ * hg_find_splits / hg_intersect on the same inputs.
+ *  device_budget  bytes of device memory the call's buffers may take
+ *                 (0 = half of the free memory); at least 4 MiB.  Buffers
+ *                 stay in the context's scratch until hg_ctx_destroy.
+ *  n_keys may reach 2^32 and beyond (offset_bytes 8).  A run of bins that
+ *  holds more keys than a slab of the budget (about budget / (4 key_bytes + 1)
+ *  keys; for the intersect budget / (2 key_bytes) keys of A and B) is
+ *  refused with HG_ERR_UNSUPPORTED, as is a single bin with more matches than
+ *  budget / (4 key_bytes).  The call returns after its last copy.
+ * hg_build_staged       offsets (n_bins+1) and CSR keys in host memory; the
+ *                       CSR keys buffer also stages the keys between passes
+ * hg_build_partitioned_staged   hg_build_partitioned's layout; n_parts <= 2048
+ * hg_find_splits_staged         hg_find_splits over host keys
+ * hg_intersect_staged   two host tables of one bin space; h_out (host, may be
+ *                       NULL for count only) gets the matched keys, bin range
+ *                       after bin range; HG_ERR_CAPACITY (with *h_count set)
+ *                       when capacity < count, the list then incomplete.   */
+hg_status hg_build_staged(hg_ctx ctx, const hg_table_desc *desc, const void *h_keys, void *h_offsets,
+                          void *h_csr_keys, size_t device_budget);
+hg_status hg_build_partitioned_staged(hg_ctx ctx, const hg_table_desc *desc, const void *h_keys,
+                                      const int64_t *h_splits, uint32_t n_parts, void *h_offsets, void *h_csr_keys,
+                                      uint64_t *h_part_offsets, size_t device_budget);
+hg_status hg_find_splits_staged(hg_ctx ctx, const hg_table_desc *desc, const void *h_keys, uint64_t bin_count,
+                                uint32_t n_parts, int64_t *h_splits, size_t device_budget);
+hg_status hg_intersect_staged(hg_ctx ctx, const hg_table_desc *descA, const void *h_offA, const void *h_keysA,
+                              const hg_table_desc *descB, const void *h_offB, const void *h_keysB, void *h_out,
+                              uint64_t capacity, uint64_t *h_count, size_t device_budget);
+
 /* ---- learned hash (RMI) ---------------------------------------------------
  * No counterpart in the reference.  A two-stage piecewise-linear CDF model in
  * FP32: a linear root picks one of n_leaves leaves, each leaf's line maps the

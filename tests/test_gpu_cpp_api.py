@@ -63,6 +63,26 @@ def test_multi_hash_b32_and_index_modes(exe, gpus, dev):
     assert r.returncode == 0 and "error" not in r.stderr.lower()
 
 
+@pytest.mark.parametrize("exe", ["multi-hash-managed", "multi-hash-32-managed", "multi-hash-idx-managed"])
+@pytest.mark.parametrize("gpus", [1, 16])
+def test_multi_hash_managed_modes(exe, gpus, dev):
+    """The reference's MANAGED_MEM builds (src/MultiHashGraph.cu:229-247): the
+    tables live in managed memory and stream through a 4 MiB device budget
+    (HG_STAGED_BUDGET), so every build and join runs in many slabs; the same
+    buildSingle and CPU intersect checks as the device-memory builds."""
+    nA, nB = 1 << 20, 1 << 19
+    env = dict(os.environ, HG_STAGED_BUDGET=str(4 << 20))
+    for mode in ("x", "build"):
+        args = [os.path.join(BIN, exe), str(nA), str(nA), "16000", str(gpus), "16000", "check", str(nB), mode]
+        print("run", args, flush=True)
+        r = subprocess.run(args, capture_output=True, text=True, timeout=120, env=env)
+        print(r.stdout[-2000:], r.stderr[-2000:], flush=True)
+        assert r.returncode == 0 and "error" not in r.stderr.lower()
+        if mode == "x":
+            assert "intersect check ok" in r.stdout
+            assert f"intersect count: {min(nA, nB)}" in r.stdout
+
+
 @pytest.mark.parametrize("name", ["c1_u32", "c2_u64"])
 def test_single_hash_graph_class_against_golden(name, dev, tmp_path):
     """SingleHashGraph as a class (caller keys, build, CSR accessors,
