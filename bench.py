@@ -11,6 +11,8 @@ top level, then one sub-result per other workload:
            16 logical partitions, B = 2^29, run-length join
   c5       4096 table/probe pairs (2^8..2^20 uint32 keys) in one captured plan
   learned  the learned hash beside murmur3 on 2^27 skewed uint64 sets
+  staged   the C2 uint32 step with the tables in pinned host memory (the
+           managed-memory replacement), bound by the host link
 Each carries a `roofline` (the dominant kernel's algorithmic bytes over its
 average launch time, from HIP events the library records around each kernel
 on its stream, hg_ctx_profile) and a `cpu_baseline` (the oracle's threaded
@@ -628,12 +630,96 @@ def run_c5(dev, steps, warmup, T=4096, cpu=True):
     return res
 
 
+def run_staged(dev, steps, warmup, budget_mib=256):
+    """The C2 uint32 step with both tables in pinned host memory (the
+    MANAGED_MEM replacement): A and B built through a device budget of
+    budget_mib MiB (about an eighth of one table with its CSR), then the staged
+    join with its matched keys copied out.  Its bound is the host link, whose
+    pinned copy rates are measured here in both directions."""
+    import torch
+    import hashgraph_amd as hg
+    from hashgraph_amd import workloads as wl
+    nA, nB, M, kb, frac = CONFIGS["c2-u32"]
+    budget = budget_mib << 20
+    log(f"staged: nA={nA} nB={nB} M={M} in pinned host memory, device budget {budget_mib} MiB")
+    A = wl.uniform_keys(1, nA, kb, device=dev)
+    B = wl.probe_keys(A, 2, nB, frac, kb, device=dev)
+    off = torch.empty(M + 1, dtype=torch.int32, device=dev)
+    csr = torch.empty_like(A)
+    hg.build(A, M, out=(off, csr))
+    K, _ = hg.probe(off, csr, M, B, emit=False)
+    del off, csr
+    hA, hB = A.cpu().pin_memory(), B.cpu().pin_memory()
+    del A, B
+    torch.cuda.empty_cache()
+
+    def pin(n):
+        return torch.empty(n, dtype=torch.int32, pin_memory=True)
+
+    offA, csrA, offB, csrB, matches = pin(M + 1), pin(nA), pin(M + 1), pin(nB), pin(max(K, 1))
+    x = torch.empty(nA, dtype=torch.int32, device=dev)
+
+    def rate(f, reps=3):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            f()
+        torch.cuda.synchronize()
+        return reps * nA * kb / (time.perf_counter() - t0) / 1e9
+
+    h2d = rate(lambda: x.copy_(hA, non_blocking=True))
+    d2h = rate(lambda: csrA.copy_(x, non_blocking=True))
+    del x
+    torch.cuda.empty_cache()
+
+    def build(h, o, c):
+        hg.build_staged(h, M, device_budget=budget, device=0, out=(o, c))
+
+    def step():
+        build(hA, offA, csrA)
+        build(hB, offB, csrB)
+        cnt, _ = hg.intersect_staged(offA, csrA, offB, csrB, M, device_budget=budget, device=0, out=matches)
+        return cnt
+
+    for _ in range(warmup):
+        assert step() == K
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        build(hA, offA, csrA)
+    ms_build = (time.perf_counter() - t0) * 1e3 / steps
+    # host-link bytes: a staged build moves its keys up three times and down
+    # twice plus its offsets down; the join moves both tables up and the
+    # matches down
+    up_b, down_b = 3 * nA * kb, 2 * nA * kb + 4 * (M + 1)
+    up_j, down_j = (nA + nB) * kb + 2 * 4 * (M + 1), K * kb
+    bound_build = (up_b / h2d + down_b / d2h) / 1e6
+    bound_step = (2 * up_b / h2d + 2 * down_b / d2h + up_j / h2d + down_j / d2h) / 1e6
+    res = {
+        "value": round((nA + nB) / (ms / 1e3) / 1e9, 3), "unit": "Gkeys/s", "ms_per_step": round(ms, 2),
+        "config": {"workload": f"C2 uint32 step with the tables in pinned host memory: build A and B "
+                               f"({nA} keys each, {M} bins) through a {budget_mib} MiB device budget, staged "
+                               f"join with the matched keys copied out", "device_budget_bytes": budget,
+                   "matches": K},
+        "build_gkeys_s": round(nA / (ms_build / 1e3) / 1e9, 3), "ms_build": round(ms_build, 2),
+        "link_GBps": {"h2d_pinned": round(h2d, 1), "d2h_pinned": round(d2h, 1)},
+        "link_roofline": {"bound": "host link", "build_bound_ms": round(bound_build, 2),
+                          "build_frac": round(bound_build / ms_build, 3), "step_bound_ms": round(bound_step, 2),
+                          "step_frac": round(bound_step / ms, 3)},
+    }
+    del hA, hB, offA, csrA, offB, csrB, matches
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="all", choices=["all"] + sorted(CONFIGS) + ["c3", "c5", "learned"],
+    ap.add_argument("--config", default="all", choices=["all"] + sorted(CONFIGS) + ["c3", "c5", "learned", "staged"],
                     help="all: the C2 u32 headline plus every other workload as sub-results")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -660,8 +746,10 @@ def main():
     cpu = not args.no_cpu
     common = {"metric": METRIC, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": True,
               "scaling": "weak", "vs_baseline": None}
-    if args.config in ("c3", "c5", "learned") or args.config in CONFIGS and args.config != "c2-u32":
-        if args.config == "c3":
+    if args.config in ("c3", "c5", "learned", "staged") or args.config in CONFIGS and args.config != "c2-u32":
+        if args.config == "staged":
+            r = {"staged": run_staged(dev, max(1, min(args.steps, 3)), min(args.warmup, 1))}
+        elif args.config == "c3":
             r = run_c3(dev, args.steps, args.warmup, cpu)
         elif args.config == "c5":
             r = run_c5(dev, args.steps, args.warmup, cpu=cpu)
@@ -681,6 +769,8 @@ def main():
         res["learned_murmur_probe_ratio"] = {k: r for k, r in zip(("lognormal", "clusters"), ratios)}
         res["c3"] = run_c3(dev, sub_steps, sub_warm, cpu)
         res["c5"] = run_c5(dev, sub_steps, sub_warm, cpu=cpu)
+        log("staged tables (host memory)")
+        res["staged"] = run_staged(dev, max(1, min(args.steps, 3)), 1)
     print(json.dumps(res))
 
 

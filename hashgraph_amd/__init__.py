@@ -331,14 +331,17 @@ def find_splits_staged(keys, n_bins: int, bin_count: int = 16000, n_parts: int =
 
 
 def intersect_staged(offA, keysA, offB, keysB, n_bins: int, emit: bool = True, hash_kind: int = HG_HASH_MURMUR3,
-                     rmi_model=None, device_budget: int = 0, device=None, capacity: int | None = None):
-    """intersect over two host-resident tables: returns (count, host matched keys or None)."""
+                     rmi_model=None, device_budget: int = 0, device=None, capacity: int | None = None, out=None):
+    """intersect over two host-resident tables: returns (count, host matched keys or None).
+    `out` (a host array of `capacity` keys or more) receives the list."""
     kb = _hbytes(keysA)
     ob = _hbytes(offA)
     c = context(device)
     dA = _desc(len(keysA), n_bins, kb, ob, hash_kind, rmi_model)
     dB = _desc(len(keysB), n_bins, kb, ob, hash_kind, rmi_model)
     cnt = ctypes.c_uint64(0)
+    if out is not None and capacity is None:
+        capacity = len(out)
     if capacity is None:
         check(c.lib.hg_intersect_staged(c.bind_stream(), ctypes.byref(dA), _hp(offA), _hp(keysA), ctypes.byref(dB),
                                         _hp(offB), _hp(keysB), None, 0, ctypes.byref(cnt), device_budget),
@@ -346,7 +349,8 @@ def intersect_staged(offA, keysA, offB, keysB, n_bins: int, emit: bool = True, h
         if not emit:
             return cnt.value, None
         capacity = cnt.value
-    out = _hempty(keysA, max(capacity, 1), kb)
+    if out is None:
+        out = _hempty(keysA, max(capacity, 1), kb)
     check(c.lib.hg_intersect_staged(c.bind_stream(), ctypes.byref(dA), _hp(offA), _hp(keysA), ctypes.byref(dB),
                                     _hp(offB), _hp(keysB), _hp(out), capacity, ctypes.byref(cnt), device_budget),
           "hg_intersect_staged")

@@ -141,6 +141,9 @@ struct hg_ctx_s {
   bool hev_used[kSlots] = {false, false, false, false};
   int next_slot = 0;
   unsigned long long *h_small = nullptr;  // pinned read-back area
+  // staged tables: upload and copy-back streams and their events
+  hipStream_t st_in = nullptr, st_out = nullptr;
+  hipEvent_t st_ev[6] = {};
   // per-kernel timing (hg_ctx_profile)
   bool prof = false;
   std::vector<hipEvent_t> ev_pool;
@@ -906,6 +909,10 @@ hg_status hg_ctx_destroy(hg_ctx c) {
     if (c->hev[i]) (void)hipEventDestroy(c->hev[i]);
   }
   if (c->h_small) (void)hipHostFree(c->h_small);
+  if (c->st_in) (void)hipStreamDestroy(c->st_in);
+  if (c->st_out) (void)hipStreamDestroy(c->st_out);
+  for (hipEvent_t e : c->st_ev)
+    if (e) (void)hipEventDestroy(e);
   for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
   delete c;
   return HG_OK;
@@ -1754,36 +1761,55 @@ hg_status build_partitioned(hg_ctx c, const hg_table_desc *desc, const void *d_k
 //           and bins fit the budget);
 //   pass 3  each slab is uploaded, built as a bin range in HBM, and its CSR
 //           keys and rebased offsets are copied back over its region.
-// Host-link traffic: 3 N s up, 2 N s down, offsets once down.
+// Host-link traffic: 3 N s up, 2 N s down, offsets once down.  Uploads run on
+// one stream, copies back on another and kernels on the context's stream,
+// with two buffers of each kind, so that both link directions and the
+// kernels overlap.
 // ---------------------------------------------------------------------------
 namespace {
 
 constexpr size_t kStageMinBudget = 4ull << 20;
 constexpr uint64_t kStageMaxChunk = 1ull << 31;  // 32-bit key positions inside the kernels
 
+// Device memory per staged key and per staged bin, for the budget:
+//  chunk key (passes 1, 2): two upload and two partition buffers plus the
+//    piece level's matrices (~0.5 byte);
+//  slab key (pass 3): two upload and two CSR buffers, the build's two
+//    partitioned copies and its matrices (~1 byte);
+//  slab bin: two offsets buffers and the final stage's per-bin counters.
 struct StageGeom {
-  uint64_t ks = 0;    // keys per chunk and per slab
-  uint64_t rmax = 0;  // bins per slab
+  double cap = 0;  // bytes usable for buffers (0.9 of the budget)
+  double key_cost = 0, bin_cost = 0;
+  uint64_t chunk = 0;
+  bool fits(uint64_t keys, uint64_t bins) const {
+    return keys <= kStageMaxChunk && (double)keys * key_cost + (double)bins * bin_cost <= cap;
+  }
 };
 
-// Device use of a build: the chunk or slab keys and the slab's CSR keys
-// (arena 3), one or two partitioned copies (arena 0), about 0.5 byte per key
-// of level matrices and per-bin counters of R bins (arena 1), the slab's
-// offsets (arena 3).  4/5 of the budget go to keys, 1/5 to bins.
 hg_status stage_geometry(hg_ctx c, int kb, int ob, size_t &budget, StageGeom &g) {
+  (void)c;
   if (budget == 0) {
     size_t fr = 0, tot = 0;
     HG_HIP(hipMemGetInfo(&fr, &tot));
     budget = fr / 2;
   }
   if (budget < kStageMinBudget) return fail(HG_ERR_INVALID_ARG, "device_budget below 4 MiB");
-  g.ks = std::min<uint64_t>((uint64_t)(0.8 * (double)budget / (4.0 * kb + 1.0)), kStageMaxChunk);
-  g.rmax = std::min<uint64_t>((uint64_t)(0.2 * (double)budget / (ob + 4.0)), (1ull << 32) - 1);
+  g.cap = 0.9 * (double)budget;
+  g.key_cost = 6.0 * kb + 1.0;
+  g.bin_cost = 2.0 * ob + 4.0;
+  g.chunk = std::min<uint64_t>((uint64_t)(g.cap / (4.0 * kb + 0.5)), kStageMaxChunk);
   return HG_OK;
 }
 
-hg_status stage_copy(hg_ctx c, void *dst, const void *src, size_t bytes) {
-  if (bytes) HG_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, c->stream));
+// The upload stream, the copy-back stream and their events (created on
+// first use, default priority).
+struct StageStreams {
+  hipStream_t in = nullptr, out = nullptr;
+  hipEvent_t ev_in[2] = {}, ev_done[2] = {}, ev_out[2] = {};
+};
+
+hg_status stage_copy(hipStream_t s, void *dst, const void *src, size_t bytes) {
+  if (bytes) HG_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, s));
   return HG_OK;
 }
 
@@ -1792,11 +1818,12 @@ struct Pieces {
   std::vector<uint32_t> part;   // its partition
 };
 
-hg_status make_pieces(const int64_t *splits, uint32_t n_parts, uint64_t rmax, Pieces &pc) {
+hg_status make_pieces(const int64_t *splits, uint32_t n_parts, const StageGeom &g, Pieces &pc) {
   const uint64_t top = (uint64_t)splits[n_parts];
   const uint64_t avail = MAX_SPLITS - n_parts;
   const uint64_t wf = std::max<uint64_t>(1, (top + avail - 1) / avail);
-  if (wf > rmax) return fail(HG_ERR_UNSUPPORTED, "device_budget too small for the table's bin count");
+  if ((double)wf * g.bin_cost > 0.5 * g.cap)
+    return fail(HG_ERR_UNSUPPORTED, "device_budget too small for the table's bin count");
   for (uint32_t p = 0; p < n_parts; p++)
     for (uint64_t b = (uint64_t)splits[p]; b < (uint64_t)splits[p + 1]; b += wf) {
       pc.bounds.push_back((int64_t)b);
@@ -1832,6 +1859,33 @@ struct Slab {
   uint64_t lo = 0, R = 0, count = 0, pos = 0;
 };
 
+// Waits for every stream of a staged call on all exits, so that no copy is
+// left in flight over the context's buffers.
+struct StageDrain {
+  hg_ctx c;
+  const StageStreams &s;
+  ~StageDrain() {
+    (void)hipStreamSynchronize(s.in);
+    (void)hipStreamSynchronize(s.out);
+    (void)hipStreamSynchronize(c->stream);
+  }
+};
+
+hg_status stage_streams(hg_ctx c, StageStreams &s) {
+  if (!c->st_in) HG_HIP(hipStreamCreateWithFlags(&c->st_in, hipStreamNonBlocking));
+  if (!c->st_out) HG_HIP(hipStreamCreateWithFlags(&c->st_out, hipStreamNonBlocking));
+  for (hipEvent_t &e : c->st_ev)
+    if (!e) HG_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  s.in = c->st_in;
+  s.out = c->st_out;
+  for (int b = 0; b < 2; b++) {
+    s.ev_in[b] = c->st_ev[b];
+    s.ev_done[b] = c->st_ev[2 + b];
+    s.ev_out[b] = c->st_ev[4 + b];
+  }
+  return HG_OK;
+}
+
 // splits over [0, M) of n_parts partitions; h_part_offsets gets the key
 // start of every partition (n_parts + 1 entries)
 hg_status staged_partitioned(hg_ctx c, const hg_table_desc *desc, const void *h_keys, const int64_t *splits,
@@ -1842,36 +1896,51 @@ hg_status staged_partitioned(hg_ctx c, const hg_table_desc *desc, const void *h_
   StageGeom g;
   HG_TRY(stage_geometry(c, kb, ob, budget, g));
   Pieces pc;
-  HG_TRY(make_pieces(splits, n_parts, g.rmax, pc));
+  HG_TRY(make_pieces(splits, n_parts, g, pc));
   const uint32_t np = (uint32_t)pc.part.size();
-  // arena 3: chunk / slab keys, slab CSR keys, slab offsets
+  StageStreams sst;
+  HG_TRY(stage_streams(c, sst));
+  const StageStreams *ss = &sst;
+  StageDrain drain{c, sst};
+  const uint64_t C = std::max<uint64_t>(std::min(g.chunk, n), 1);
+  const uint64_t nch = (n + C - 1) / C;
+  // passes 1 and 2: arena 3 = two upload buffers and two partition buffers
   Plan P3;
-  const size_t i_in = P3.add(std::max<uint64_t>(std::min(g.ks, n), 1) * kb);
-  const size_t i_out = P3.add(std::max<uint64_t>(std::min(g.ks, n), 1) * kb);
-  const size_t i_off = P3.add((std::min<uint64_t>(g.rmax, desc->n_bins) + 1) * ob);
+  const size_t i_in0 = P3.add(C * kb), i_in1 = P3.add(C * kb), i_pt0 = P3.add(C * kb), i_pt1 = P3.add(C * kb);
   HG_TRY(arena_ensure(c, 3, P3.total()));
   Carve cv3;
   cv3.init(c, 3, P3);
-  char *d_in = cv3.get<char>(i_in), *d_out = cv3.get<char>(i_out);
-  void *d_off = cv3.get<void>(i_off);
+  char *d_in[2] = {cv3.get<char>(i_in0), cv3.get<char>(i_in1)};
+  char *d_pt[2] = {cv3.get<char>(i_pt0), cv3.get<char>(i_pt1)};
+  // chunk j goes to buffer j & 1 once chunk j - 2's kernels are done with it
+  auto upload = [&](uint64_t j) -> hg_status {
+    const int b = (int)(j & 1);
+    if (j >= 2) HG_HIP(hipStreamWaitEvent(ss->in, ss->ev_done[b], 0));
+    HG_TRY(stage_copy(ss->in, d_in[b], (const char *)h_keys + j * C * kb, std::min(C, n - j * C) * kb));
+    HG_HIP(hipEventRecord(ss->ev_in[b], ss->in));
+    return HG_OK;
+  };
   std::vector<uint32_t> cnt(np + 1), st(np + 1);
   // pass 1: keys per piece
   std::vector<uint64_t> T(np, 0);
-  for (uint64_t k0 = 0; k0 < n; k0 += g.ks) {
-    const uint64_t cn = std::min(g.ks, n - k0);
-    HG_TRY(stage_copy(c, d_in, (const char *)h_keys + k0 * kb, cn * kb));
+  if (nch) HG_TRY(upload(0));
+  for (uint64_t j = 0; j < nch; j++) {
+    const int b = (int)(j & 1);
+    if (j + 1 < nch) HG_TRY(upload(j + 1));
+    HG_HIP(hipStreamWaitEvent(c->stream, ss->ev_in[b], 0));
     Level L;
-    piece_level(L, map, pc, cn);
+    piece_level(L, map, pc, std::min(C, n - j * C));
     LevelDev d;
     HG_TRY(setup_level(c, L, d));
-    HG_TRY(run_level(c, L, d, kb, d_in, nullptr, false));
+    HG_TRY(run_level(c, L, d, kb, d_in[b], nullptr, false));
+    HG_HIP(hipEventRecord(ss->ev_done[b], c->stream));
     HG_TRY(sync_read(c, cnt.data(), d.coltot, (np + 1) * 4));
     for (uint32_t p = 0; p < np; p++) T[p] += cnt[p];
   }
-  // slabs: consecutive pieces of one partition within ks keys and rmax bins
+  // slabs: consecutive pieces of one partition within the budget
   std::vector<Slab> slabs;
   std::vector<uint64_t> po(n_parts + 1, 0);
-  uint64_t pos = 0;
+  uint64_t pos = 0, smax = 1, rmax = 1;
   for (uint32_t p = 0; p < np;) {
     Slab sl;
     sl.p0 = sl.p1 = p;
@@ -1880,58 +1949,86 @@ hg_status staged_partitioned(hg_ctx c, const hg_table_desc *desc, const void *h_
     sl.pos = pos;
     while (p < np && pc.part[p] == sl.part) {
       const uint64_t pb = (uint64_t)(pc.bounds[p + 1] - pc.bounds[p]);
-      if (T[p] > g.ks)
+      if (!g.fits(T[p], pb))
         return fail(HG_ERR_UNSUPPORTED, "bins [" + std::to_string(pc.bounds[p]) + ", " +
                                             std::to_string(pc.bounds[p + 1]) + ") hold " + std::to_string(T[p]) +
-                                            " keys, more than a slab of this device_budget (" +
-                                            std::to_string(g.ks) + ")");
-      if (sl.p1 > sl.p0 && (sl.count + T[p] > g.ks || sl.R + pb > g.rmax)) break;
+                                            " keys, more than a slab of this device_budget holds");
+      if (sl.p1 > sl.p0 && !g.fits(sl.count + T[p], sl.R + pb)) break;
       sl.count += T[p];
       sl.R += pb;
       sl.p1 = ++p;
     }
     pos += sl.count;
+    smax = std::max(smax, sl.count);
+    rmax = std::max(rmax, sl.R);
     slabs.push_back(sl);
   }
   for (const Slab &sl : slabs) po[sl.part + 1] += sl.count;
   for (uint32_t p = 0; p < n_parts; p++) po[p + 1] += po[p];
   // pass 2: every chunk's share of a slab goes to the slab's final region
   std::vector<uint64_t> cur(slabs.size(), 0);
-  for (uint64_t k0 = 0; k0 < n; k0 += g.ks) {
-    const uint64_t cn = std::min(g.ks, n - k0);
-    HG_TRY(stage_copy(c, d_in, (const char *)h_keys + k0 * kb, cn * kb));
+  if (nch) HG_TRY(upload(0));
+  for (uint64_t j = 0; j < nch; j++) {
+    const int b = (int)(j & 1);
+    if (j + 1 < nch) HG_TRY(upload(j + 1));
+    HG_HIP(hipStreamWaitEvent(c->stream, ss->ev_in[b], 0));
+    if (j >= 2) HG_HIP(hipStreamWaitEvent(c->stream, ss->ev_out[b], 0));  // chunk j - 2's copies are out
     Level L;
-    piece_level(L, map, pc, cn);
-    Plan P0;
-    const size_t i_part = P0.add(cn * kb);
-    HG_TRY(arena_ensure(c, 0, P0.total()));
-    Carve cv0;
-    cv0.init(c, 0, P0);
-    char *d_part = cv0.get<char>(i_part);
+    piece_level(L, map, pc, std::min(C, n - j * C));
     LevelDev d;
     HG_TRY(setup_level(c, L, d));
-    HG_TRY(run_level(c, L, d, kb, d_in, d_part));
+    HG_TRY(run_level(c, L, d, kb, d_in[b], d_pt[b]));
+    HG_HIP(hipEventRecord(ss->ev_done[b], c->stream));
     HG_TRY(sync_read(c, cnt.data(), d.coltot, (np + 1) * 4));
     HG_TRY(sync_read(c, st.data(), d.bstart, (np + 1) * 4));
+    HG_HIP(hipStreamWaitEvent(ss->out, ss->ev_done[b], 0));
     for (size_t s = 0; s < slabs.size(); s++) {
       const Slab &sl = slabs[s];
       const uint64_t a = st[sl.p0], e = (uint64_t)st[sl.p1 - 1] + cnt[sl.p1 - 1];
       if (cur[s] + (e - a) > sl.count) return fail(HG_ERR_DEVICE, "staged build: keys changed between passes");
-      HG_TRY(stage_copy(c, (char *)h_csr + (sl.pos + cur[s]) * kb, d_part + a * kb, (e - a) * kb));
+      HG_TRY(stage_copy(ss->out, (char *)h_csr + (sl.pos + cur[s]) * kb, d_pt[b] + a * kb, (e - a) * kb));
       cur[s] += e - a;
     }
+    HG_HIP(hipEventRecord(ss->ev_out[b], ss->out));
   }
   for (size_t s = 0; s < slabs.size(); s++)
     if (cur[s] != slabs[s].count) return fail(HG_ERR_DEVICE, "staged build: keys changed between passes");
-  // pass 3: build every slab in HBM and write its CSR back
-  for (const Slab &sl : slabs) {
-    HG_TRY(stage_copy(c, d_in, (const char *)h_csr + sl.pos * kb, sl.count * kb));
-    HG_TRY(build_range(c, map, kb, ob, sl.count, sl.lo, sl.R, d_in, d_off, d_out));
+  HG_HIP(hipStreamSynchronize(ss->out));
+  HG_HIP(hipStreamSynchronize(ss->in));
+  // pass 3: arena 3 = two upload, two CSR and two offsets buffers
+  Plan Q3;
+  const size_t j_in0 = Q3.add(smax * kb), j_in1 = Q3.add(smax * kb), j_out0 = Q3.add(smax * kb),
+               j_out1 = Q3.add(smax * kb), j_off0 = Q3.add((rmax + 1) * ob), j_off1 = Q3.add((rmax + 1) * ob);
+  HG_TRY(arena_ensure(c, 3, Q3.total()));
+  HG_TRY(arena_ensure(c, 0, 2 * smax * kb + 512));
+  cv3.init(c, 3, Q3);
+  char *s_in[2] = {cv3.get<char>(j_in0), cv3.get<char>(j_in1)};
+  char *s_out[2] = {cv3.get<char>(j_out0), cv3.get<char>(j_out1)};
+  void *s_off[2] = {cv3.get<void>(j_off0), cv3.get<void>(j_off1)};
+  auto upload_slab = [&](size_t s) -> hg_status {
+    const int b = (int)(s & 1);
+    if (s >= 2) HG_HIP(hipStreamWaitEvent(ss->in, ss->ev_done[b], 0));
+    HG_TRY(stage_copy(ss->in, s_in[b], (const char *)h_csr + slabs[s].pos * kb, slabs[s].count * kb));
+    HG_HIP(hipEventRecord(ss->ev_in[b], ss->in));
+    return HG_OK;
+  };
+  if (!slabs.empty()) HG_TRY(upload_slab(0));
+  for (size_t s = 0; s < slabs.size(); s++) {
+    const Slab &sl = slabs[s];
+    const int b = (int)(s & 1);
+    if (s + 1 < slabs.size()) HG_TRY(upload_slab(s + 1));
+    HG_HIP(hipStreamWaitEvent(c->stream, ss->ev_in[b], 0));
+    if (s >= 2) HG_HIP(hipStreamWaitEvent(c->stream, ss->ev_out[b], 0));  // slab s - 2's copies are out
+    HG_TRY(build_range(c, map, kb, ob, sl.count, sl.lo, sl.R, s_in[b], s_off[b], s_out[b]));
     // offsets of a partitioned table count from the partition's first key
-    HG_HIP(launch_offsets_add(d_off, ob, sl.R + 1, sl.pos - po[sl.part], c->stream));
-    HG_TRY(stage_copy(c, (char *)h_csr + sl.pos * kb, d_out, sl.count * kb));
-    HG_TRY(stage_copy(c, (char *)h_off + (sl.lo + sl.part) * ob, d_off, (sl.R + 1) * ob));
+    HG_HIP(launch_offsets_add(s_off[b], ob, sl.R + 1, sl.pos - po[sl.part], c->stream));
+    HG_HIP(hipEventRecord(ss->ev_done[b], c->stream));
+    HG_HIP(hipStreamWaitEvent(ss->out, ss->ev_done[b], 0));
+    HG_TRY(stage_copy(ss->out, (char *)h_csr + sl.pos * kb, s_out[b], sl.count * kb));
+    HG_TRY(stage_copy(ss->out, (char *)h_off + (sl.lo + sl.part) * ob, s_off[b], (sl.R + 1) * ob));
+    HG_HIP(hipEventRecord(ss->ev_out[b], ss->out));
   }
+  HG_HIP(hipStreamSynchronize(ss->out));
   HG_HIP(hipStreamSynchronize(c->stream));
   // a partition of no bins has the single offset entry 0
   for (uint32_t p = 0; p < n_parts; p++)
@@ -2006,7 +2103,7 @@ hg_status join_range(hg_ctx c, const hg_table_desc *descA, const void *h_offA, c
   uint64_t got = 0;
   HG_TRY(hg_intersect(c, &dA, oa, js.keysA, &dB, obp, js.keysB, d_stage, stage_cap, &got));
   if (got != cnt) return fail(HG_ERR_DEVICE, "staged intersect: count changed between passes");
-  HG_TRY(stage_copy(c, (char *)h_out + written * kb, d_stage, cnt * kb));
+  HG_TRY(stage_copy(c->stream, (char *)h_out + written * kb, d_stage, cnt * kb));
   HG_HIP(hipStreamSynchronize(c->stream));
   written += cnt;
   return HG_OK;
@@ -2059,7 +2156,7 @@ hg_status hg_find_splits_staged(hg_ctx c, const hg_table_desc *desc, const void 
   StageGeom geo;
   HG_TRY(stage_geometry(c, kb, (int)desc->offset_bytes, device_budget, geo));
   Plan P3;
-  const size_t i_in = P3.add(std::max<uint64_t>(std::min(geo.ks, n), 1) * kb);
+  const size_t i_in = P3.add(std::max<uint64_t>(std::min(geo.chunk, n), 1) * kb);
   HG_TRY(arena_ensure(c, 3, P3.total()));
   Carve cv3;
   cv3.init(c, 3, P3);
@@ -2073,9 +2170,9 @@ hg_status hg_find_splits_staged(hg_ctx c, const hg_table_desc *desc, const void 
   unsigned int *bad = (unsigned int *)(hist + bin_count);
   HG_HIP(launch_fill_u64(hist, 0ull, bin_count + 1, c->stream));
   const BinMap map = make_map(M, desc->hash_kind, desc->d_rmi_model);
-  for (uint64_t k0 = 0; k0 < n; k0 += geo.ks) {
-    const uint64_t cn = std::min(geo.ks, n - k0);
-    HG_TRY(stage_copy(c, d_in, (const char *)h_keys + k0 * kb, cn * kb));
+  for (uint64_t k0 = 0; k0 < n; k0 += geo.chunk) {
+    const uint64_t cn = std::min(geo.chunk, n - k0);
+    HG_TRY(stage_copy(c->stream, d_in, (const char *)h_keys + k0 * kb, cn * kb));
     HG_HIP(launch_range_hist(d_in, cn, kb, map, (uint32_t)range, (uint32_t)bin_count, hist, bad, c->stream));
   }
   std::vector<unsigned long long> h(bin_count + 1);
@@ -2151,10 +2248,10 @@ hg_status hg_intersect_staged(hg_ctx c, const hg_table_desc *descA, const void *
     const uint64_t na = off_at(h_offA, ob, hi) - js.a0, nb = off_at(h_offB, ob, hi) - js.b0;
     js.keysA = d_keys;
     js.keysB = d_keys + na * kb;
-    HG_TRY(stage_copy(c, js.keysA, (const char *)h_keysA + js.a0 * kb, na * kb));
-    HG_TRY(stage_copy(c, js.keysB, (const char *)h_keysB + js.b0 * kb, nb * kb));
-    HG_TRY(stage_copy(c, js.offA, (const char *)h_offA + lo * ob, (hi - lo + 1) * ob));
-    HG_TRY(stage_copy(c, js.offB, (const char *)h_offB + lo * ob, (hi - lo + 1) * ob));
+    HG_TRY(stage_copy(c->stream, js.keysA, (const char *)h_keysA + js.a0 * kb, na * kb));
+    HG_TRY(stage_copy(c->stream, js.keysB, (const char *)h_keysB + js.b0 * kb, nb * kb));
+    HG_TRY(stage_copy(c->stream, js.offA, (const char *)h_offA + lo * ob, (hi - lo + 1) * ob));
+    HG_TRY(stage_copy(c->stream, js.offB, (const char *)h_offB + lo * ob, (hi - lo + 1) * ob));
     HG_HIP(launch_offsets_add(js.offA, ob, hi - lo + 1, (uint64_t)0 - js.a0, c->stream));
     HG_HIP(launch_offsets_add(js.offB, ob, hi - lo + 1, (uint64_t)0 - js.b0, c->stream));
     HG_TRY(join_range(c, descA, h_offA, h_offB, js, lo, hi, d_stage, stage_cap, h_out, capacity, written, total,

@@ -47,7 +47,7 @@ def _check_build(keys, M, ob, budget, pinned=False):
     (1 << 20, 1 << 18, 8, 8, 4 * MiB, True),
     (300000, 7, 4, 4, 4 * MiB, False),
     (1 << 20, 3000017, 4, 8, 4 * MiB, True),
-    (77777, 1, 8, 4, 4 * MiB, False),
+    (77777, 1, 8, 4, 8 * MiB, False),
     (1 << 20, 1 << 20, 4, 4, 64 * MiB, False),
 ])
 def test_build_shapes(n, M, kb, ob, budget, pinned, dev):
