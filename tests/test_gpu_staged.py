@@ -218,3 +218,57 @@ def test_c2_shape_staged_equals_in_hbm_build(dev):
     del gc
     b = torch.sort((binid << 32) | ec).values
     assert torch.equal(a, b)
+
+
+def test_staged_build_of_2e32_keys(dev):
+    # 2^32 uint32 keys (16 GiB, pinned host memory; key i = i * 0x9E3779B1 mod
+    # 2^32, every 32-bit value once), 2^24 bins, 8-byte offsets, through an
+    # 8 GiB budget: more keys than one in-HBM call takes.
+    # Checked exactly on the device, chunk by chunk: offsets against the
+    # histogram of the input's bins; every CSR key hashes to the bin its
+    # position lies in; per-bin sums of keys and of squared keys (mod 2^64)
+    # equal the input's.
+    import torch
+    import hashgraph_amd as hg
+    n, M, G = 1 << 32, 1 << 24, 1 << 28
+    c = 0x9E3779B1
+
+    def keys_of(a, b):
+        # (i * c) mod 2^32 from 16-bit halves, inside int64 range
+        i = torch.arange(a, b, dtype=torch.int64, device=dev)
+        lo, hi = i & 0xFFFF, (i >> 16) & 0xFFFF
+        return (lo * c + (((hi * c) & 0xFFFF) << 16)) & 0xFFFFFFFF
+
+    def as_i32(k):
+        return (k - ((k >> 31) << 32)).to(torch.int32)
+
+    print("2^32 keys: generate", flush=True)
+    hk = torch.empty(n, dtype=torch.int32, pin_memory=True)
+    cnt, s1, s2 = (torch.zeros(M, dtype=torch.int64, device=dev) for _ in range(3))
+    for a in range(0, n, G):
+        b = min(n, a + G)
+        k = keys_of(a, b)
+        k32 = as_i32(k)
+        hk[a:b].copy_(k32)
+        bins = hg.hash_bins(k32, M).to(torch.int64)
+        cnt += torch.bincount(bins, minlength=M)
+        s1.index_add_(0, bins, k)
+        s2.index_add_(0, bins, k * k)
+    print("2^32 keys: staged build", flush=True)
+    off, csr = hg.build_staged(hk, M, offset_bytes=8, device_budget=8 << 30, device=0)
+    del hk
+    goff = off.to(dev)
+    assert int(goff[0]) == 0 and int(goff[-1]) == n
+    assert torch.equal(goff[1:] - goff[:-1], cnt)
+    print("2^32 keys: check the CSR", flush=True)
+    t1, t2 = (torch.zeros(M, dtype=torch.int64, device=dev) for _ in range(2))
+    for a in range(0, n, G):
+        b = min(n, a + G)
+        k32 = csr[a:b].to(dev)
+        bins = hg.hash_bins(k32, M).to(torch.int64)
+        pos = torch.arange(a, b, dtype=torch.int64, device=dev)
+        assert torch.equal(bins, torch.searchsorted(goff, pos, right=True) - 1)
+        k = k32.to(torch.int64) & 0xFFFFFFFF
+        t1.index_add_(0, bins, k)
+        t2.index_add_(0, bins, k * k)
+    assert torch.equal(t1, s1) and torch.equal(t2, s2)
