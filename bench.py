@@ -539,6 +539,24 @@ def run_c5(dev, steps, warmup, T=4096, cpu=True):
     total = int(plan.counts.sum().item())
     info = plan.info()
     plan.close()
+    # the same batch with murmur3 on every table: what the learned models
+    # cost or save in throughput (the counts must be the same)
+    plan_m = hg.BatchPlan(keys, ko, n_bins, probe=probe, probe_off=po, hash_kinds=[hg.HG_HASH_MURMUR3] * T,
+                          autotune=True)
+    for _ in range(warmup):
+        plan_m.run()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        plan_m.run()
+    e1.record()
+    torch.cuda.synchronize()
+    ms_murmur = e0.elapsed_time(e1) / steps
+    assert int(plan_m.counts.sum().item()) == total
+    murmur_only = {"ms_per_step": round(ms_murmur, 3), "info": plan_m.info()}
+    plan_m.close()
     # the same batch with every model trained inside the graph on each replay
     models_t = torch.zeros(T * mb, dtype=torch.uint8, device=dev)
     plan_t = hg.BatchPlan(keys, ko, n_bins, probe=probe, probe_off=po, rmi_models=models_t, rmi_leaves=L,
@@ -610,6 +628,8 @@ def run_c5(dev, steps, warmup, T=4096, cpu=True):
         "speedup_vs_separate_calls": round(ms_each / ms_batch, 2),
         "separate_calls_gkeys_s": round(nk / (ms_each / 1e3) / 1e9, 3),
         "train_ms": round(t_train * 1e3, 2), "plan_build_s": round(t_plan, 2),
+        "plan_murmur_only": {**murmur_only, "value": round(nk / (ms_murmur / 1e3) / 1e9, 3), "unit": "Gkeys/s",
+                             "note": "murmur3 on every table, same inputs and counts"},
         "plan_training_in_graph": {**trained, "value": round(nk / (trained["ms_per_step"] / 1e3) / 1e9, 3),
                                    "unit": "Gkeys/s",
                                    "note": "every model trained inside the graph on each replay (batched trainer, "
