@@ -385,14 +385,15 @@ def learned_block(dev, steps, n=1 << 27, frac=0.5):
 def c3_kernel_bytes(nA, nB, M, P, s):
     """Algorithmic bytes of a C3 step's kernels, summed over their launches.
     Each side is partitioned twice (by split range, then by bucket inside each
-    partition), then built; the run-length join reads both sides' keys and
-    offsets in each of its two passes."""
+    partition), then built; the run-length join needs one read of both sides'
+    keys and 4-byte offsets (its count and emit passes read them twice; the
+    measured traffic shows what they move)."""
     n2 = 2 * (nA + nB)
     return {
         "part_hist": s * n2,
         "part_scatter": 2 * s * n2,
         "final": 2 * s * (nA + nB) + 4 * 2 * (M + P),
-        "rle_small": 2 * (s * (nA + nB) + 8 * (M + P)),
+        "rle_small": s * (nA + nB) + 8 * (M + P),
     }
 
 
@@ -484,6 +485,7 @@ def run_c3(dev, steps, warmup, cpu=True):
         "count": cnt, "rle_records": recs,
         "roofline": roofline_from(kt, bt, steps=steps), "kernels": kernel_table(kt, steps, ms, bt),
     }
+    res["roofline"]["traffic"] = load_traffic().get("c3", {}).get(res["roofline"]["kernel"])
     del A, B
     torch.cuda.empty_cache()
     if cpu:
@@ -494,7 +496,7 @@ def run_c3(dev, steps, warmup, cpu=True):
     return res
 
 
-def run_c5(dev, steps, warmup, T=4096, cpu=True):
+def run_c5(dev, steps, warmup, T=4096, cpu=True, plan_only=False):
     """Batched form: T table/probe pairs, table sizes round(2^U[8,20]) uint32
     keys (even tables uniform, odd tables Zipf(1.2) ranks), probe sets of
     equal size with 30% drawn from their table, M_t = n_t; the learned hash
@@ -520,6 +522,17 @@ def run_c5(dev, steps, warmup, T=4096, cpu=True):
     t_train = time.perf_counter() - t0
     kinds = [hg.HG_HASH_RMI if u else hg.HG_HASH_MURMUR3 for u in flags]
     log(f"c5: trained {T} models in {t_train * 1e3:.1f} ms (batched trainer), learned on {sum(flags)} tables")
+    if plan_only:
+        # for counter runs: the plan's kernels alone, at the configuration the
+        # autotuner picks here (32 K-key tiles, fill 0.875, its default)
+        plan = hg.BatchPlan(keys, ko, n_bins, probe=probe, probe_off=po, hash_kinds=kinds, rmi_models=models,
+                            rmi_leaves=L, autotune=False)
+        for _ in range(warmup + steps):
+            plan.run()
+        torch.cuda.synchronize()
+        out = {"replays": warmup + steps, "info": plan.info(), "count": int(plan.counts.sum().item())}
+        plan.close()
+        return out
     t0 = time.perf_counter()
     plan = hg.BatchPlan(keys, ko, n_bins, probe=probe, probe_off=po, hash_kinds=kinds, rmi_models=models,
                         rmi_leaves=L, autotune=True)
@@ -636,7 +649,8 @@ def run_c5(dev, steps, warmup, T=4096, cpu=True):
                                            "one workgroup per table), then build + probe counts"},
         "roofline": {"kernel": "plan replay (one graph: every kernel of the batch)", "bound": "hbm",
                      "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "alg_bytes_per_launch": step_bytes,
+                     "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": load_traffic().get("c5", {}).get("plan replay"),
+                     "alg_bytes_per_launch": step_bytes,
                      "avg_launch_ms": round(ms_batch, 4)},
         "kernels_eager": kernel_table(kt, 1, ms_eager, bt),
     }
@@ -742,6 +756,8 @@ def main():
     ap.add_argument("--config", default="all", choices=["all"] + sorted(CONFIGS) + ["c3", "c5", "learned", "staged"],
                     help="all: the C2 u32 headline plus every other workload as sub-results")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
+    ap.add_argument("--plan-only", action="store_true",
+                    help="with --config c5: replay only the untuned plan warmup + steps times (counter runs)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run device copies of MIB MiB (calibrates the HBM byte counters)")
     args = ap.parse_args()
@@ -772,7 +788,7 @@ def main():
         elif args.config == "c3":
             r = run_c3(dev, args.steps, args.warmup, cpu)
         elif args.config == "c5":
-            r = run_c5(dev, args.steps, args.warmup, cpu=cpu)
+            r = run_c5(dev, args.steps, args.warmup, cpu=cpu, plan_only=args.plan_only)
         elif args.config == "learned":
             r = {"learned": learned_block(dev, max(2, min(args.steps, 5)))}
         else:
