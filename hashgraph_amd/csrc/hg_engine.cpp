@@ -1871,6 +1871,19 @@ struct StageDrain {
   }
 };
 
+// Device-only memory (hipMalloc) the host cannot read or write: the staged
+// calls read offsets and write empty partitions' entries from the host.
+// Pageable (unknown to HIP), pinned and managed memory pass.
+bool device_only(const void *p) {
+  if (!p) return false;
+  hipPointerAttribute_t a{};
+  if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return a.type == hipMemoryTypeDevice;
+}
+
 hg_status stage_streams(hg_ctx c, StageStreams &s) {
   if (!c->st_in) HG_HIP(hipStreamCreateWithFlags(&c->st_in, hipStreamNonBlocking));
   if (!c->st_out) HG_HIP(hipStreamCreateWithFlags(&c->st_out, hipStreamNonBlocking));
@@ -2071,11 +2084,21 @@ struct JoinSlab {
   char *keysA = nullptr, *keysB = nullptr;
 };
 
-// Matches of bins [x, y) of the slab: counted, then emitted into d_stage and
-// copied to the host list when they fit the stage (else the range is halved).
-hg_status join_range(hg_ctx c, const hg_table_desc *descA, const void *h_offA, const void *h_offB, const JoinSlab &js,
-                     uint64_t x, uint64_t y, void *d_stage, uint64_t stage_cap, void *h_out, uint64_t capacity,
-                     uint64_t &written, uint64_t &total, bool &full) {
+// The matches' staging buffers of a staged join: emissions alternate between
+// two, each copied back on the copy-back stream.
+struct JoinStage {
+  void *buf[2] = {nullptr, nullptr};
+  uint64_t cap = 0;
+  int next = 0;
+  uint32_t used = 0;  // emissions so far (a buffer is waited for from its second use on)
+};
+
+// Matches of bins [x, y) of the slab: counted, then emitted into a stage
+// buffer and copied to the host list when they fit it (else the range is
+// halved).
+hg_status join_range(hg_ctx c, const StageStreams &ss, const hg_table_desc *descA, const void *h_offA,
+                     const void *h_offB, const JoinSlab &js, uint64_t x, uint64_t y, JoinStage &stg, void *h_out,
+                     uint64_t capacity, uint64_t &written, uint64_t &total, bool &full) {
   const int kb = (int)descA->key_bytes, ob = (int)descA->offset_bytes;
   hg_table_desc dA = *descA, dB = *descA;
   dA.n_bins = dB.n_bins = y - x;
@@ -2090,21 +2113,26 @@ hg_status join_range(hg_ctx c, const hg_table_desc *descA, const void *h_offA, c
     full = true;
     return HG_OK;
   }
-  if (cnt > stage_cap) {
+  if (cnt > stg.cap) {
     if (y - x == 1)
       return fail(HG_ERR_UNSUPPORTED, "bin " + std::to_string(x) + " has " + std::to_string(cnt) +
                                           " matches, more than this device_budget stages (" +
-                                          std::to_string(stage_cap) + ")");
+                                          std::to_string(stg.cap) + ")");
     total -= cnt;
     const uint64_t m = x + (y - x) / 2;
-    HG_TRY(join_range(c, descA, h_offA, h_offB, js, x, m, d_stage, stage_cap, h_out, capacity, written, total, full));
-    return join_range(c, descA, h_offA, h_offB, js, m, y, d_stage, stage_cap, h_out, capacity, written, total, full);
+    HG_TRY(join_range(c, ss, descA, h_offA, h_offB, js, x, m, stg, h_out, capacity, written, total, full));
+    return join_range(c, ss, descA, h_offA, h_offB, js, m, y, stg, h_out, capacity, written, total, full);
   }
+  const int e = stg.next;
+  if (stg.used >= 2) HG_HIP(hipStreamWaitEvent(c->stream, ss.ev_out[e], 0));  // its last copy back is done
   uint64_t got = 0;
-  HG_TRY(hg_intersect(c, &dA, oa, js.keysA, &dB, obp, js.keysB, d_stage, stage_cap, &got));
+  HG_TRY(hg_intersect(c, &dA, oa, js.keysA, &dB, obp, js.keysB, stg.buf[e], stg.cap, &got));
   if (got != cnt) return fail(HG_ERR_DEVICE, "staged intersect: count changed between passes");
-  HG_TRY(stage_copy(c->stream, (char *)h_out + written * kb, d_stage, cnt * kb));
-  HG_HIP(hipStreamSynchronize(c->stream));
+  // hg_intersect has synchronised the context's stream: the matches are in place
+  HG_TRY(stage_copy(ss.out, (char *)h_out + written * kb, stg.buf[e], cnt * kb));
+  HG_HIP(hipEventRecord(ss.ev_out[e], ss.out));
+  stg.next = 1 - e;
+  stg.used++;
   written += cnt;
   return HG_OK;
 }
@@ -2119,6 +2147,7 @@ hg_status hg_build_staged(hg_ctx c, const hg_table_desc *desc, const void *h_key
   HG_TRY(check_desc(desc, false));
   if (!h_offsets || (desc->n_keys && (!h_keys || !h_csr_keys))) return fail(HG_ERR_INVALID_ARG, "null buffer");
   DeviceGuard g(c->device);
+  if (device_only(h_offsets)) return fail(HG_ERR_INVALID_ARG, "h_offsets must be host memory (pageable, pinned or managed)");
   const int64_t splits[2] = {0, (int64_t)desc->n_bins};
   uint64_t po[2];
   return staged_partitioned(c, desc, h_keys, splits, 1, h_offsets, h_csr_keys, po, device_budget);
@@ -2137,6 +2166,7 @@ hg_status hg_build_partitioned_staged(hg_ctx c, const hg_table_desc *desc, const
       return fail(HG_ERR_INVALID_ARG, "splits must be non-decreasing within [0, M]");
   if (desc->n_keys && (!h_keys || !h_csr_keys)) return fail(HG_ERR_INVALID_ARG, "null buffer");
   DeviceGuard g(c->device);
+  if (device_only(h_offsets)) return fail(HG_ERR_INVALID_ARG, "h_offsets must be host memory (pageable, pinned or managed)");
   return staged_partitioned(c, desc, h_keys, h_splits, n_parts, h_offsets, h_csr_keys, h_part_offsets,
                             device_budget);
 }
@@ -2196,6 +2226,8 @@ hg_status hg_intersect_staged(hg_ctx c, const hg_table_desc *descA, const void *
     return fail(HG_ERR_INVALID_ARG, "null buffer");
   DeviceGuard g(c->device);
   *h_count = 0;
+  if (device_only(h_offA) || device_only(h_offB))
+    return fail(HG_ERR_INVALID_ARG, "offsets must be host memory (pageable, pinned or managed)");
   const uint64_t M = descA->n_bins;
   const int kb = (int)descA->key_bytes, ob = (int)descA->offset_bytes;
   if (off_at(h_offA, ob, M) != descA->n_keys || off_at(h_offB, ob, M) != descB->n_keys)
@@ -2206,29 +2238,33 @@ hg_status hg_intersect_staged(hg_ctx c, const hg_table_desc *descA, const void *
     device_budget = fr / 2;
   }
   if (device_budget < kStageMinBudget) return fail(HG_ERR_INVALID_ARG, "device_budget below 4 MiB");
-  // half of the budget for the slab's keys of A and B, 1/5 for its offsets,
-  // the rest for staged matches
-  const uint64_t kmax = std::min<uint64_t>((uint64_t)(0.5 * (double)device_budget / kb), kStageMaxChunk);
-  const uint64_t rmax = std::min<uint64_t>((uint64_t)(0.1 * (double)device_budget / ob), (1ull << 32) - 2);
-  const uint64_t stage_cap = (uint64_t)(0.25 * (double)device_budget / kb);
+  // two slab buffers (each a quarter of the budget for keys of A and B and a
+  // twentieth for each table's offsets) and two stage buffers (an eighth each)
+  const uint64_t kmax = std::min<uint64_t>((uint64_t)(0.25 * (double)device_budget / kb), kStageMaxChunk);
+  const uint64_t rmax = std::min<uint64_t>((uint64_t)(0.05 * (double)device_budget / ob), (1ull << 32) - 2);
+  JoinStage stg;
+  stg.cap = (uint64_t)(0.125 * (double)device_budget / kb);
   Plan P3;
-  const size_t i_ka = P3.add(kmax * kb), i_oa = P3.add((rmax + 1) * ob), i_ob = P3.add((rmax + 1) * ob),
-               i_st = P3.add(stage_cap * kb);
+  size_t i_k[2], i_oa[2], i_ob[2], i_st[2];
+  for (int b = 0; b < 2; b++) {
+    i_k[b] = P3.add(kmax * kb);
+    i_oa[b] = P3.add((rmax + 1) * ob);
+    i_ob[b] = P3.add((rmax + 1) * ob);
+    i_st[b] = P3.add(stg.cap * kb);
+  }
+  StageStreams ss;
+  HG_TRY(stage_streams(c, ss));
+  StageDrain drain{c, ss};
   HG_TRY(arena_ensure(c, 3, P3.total()));
   Carve cv3;
   cv3.init(c, 3, P3);
-  JoinSlab js;
-  js.offA = cv3.get<void>(i_oa);
-  js.offB = cv3.get<void>(i_ob);
-  char *d_keys = cv3.get<char>(i_ka);
-  void *d_stage = cv3.get<void>(i_st);
+  for (int b = 0; b < 2; b++) stg.buf[b] = cv3.get<void>(i_st[b]);
   auto keys_in = [&](uint64_t lo, uint64_t hi) {
     return (off_at(h_offA, ob, hi) - off_at(h_offA, ob, lo)) + (off_at(h_offB, ob, hi) - off_at(h_offB, ob, lo));
   };
-  uint64_t written = 0, total = 0;
-  bool full = false;
+  // slabs: the largest hi with the keys within kmax and the bins within rmax
+  std::vector<JoinSlab> slabs;
   for (uint64_t lo = 0; lo < M;) {
-    // the largest hi with the slab's keys within kmax and bins within rmax
     uint64_t top = std::min<uint64_t>(M, lo + rmax);
     if (keys_in(lo, lo + 1) > kmax)
       return fail(HG_ERR_UNSUPPORTED, "bin " + std::to_string(lo) + " holds " + std::to_string(keys_in(lo, lo + 1)) +
@@ -2240,24 +2276,46 @@ hg_status hg_intersect_staged(hg_ctx c, const hg_table_desc *descA, const void *
       if (keys_in(lo, mid) <= kmax) good = mid;
       else top = mid - 1;
     }
-    const uint64_t hi = good;
+    JoinSlab js;
     js.lo = lo;
-    js.hi = hi;
+    js.hi = good;
     js.a0 = off_at(h_offA, ob, lo);
     js.b0 = off_at(h_offB, ob, lo);
-    const uint64_t na = off_at(h_offA, ob, hi) - js.a0, nb = off_at(h_offB, ob, hi) - js.b0;
-    js.keysA = d_keys;
-    js.keysB = d_keys + na * kb;
-    HG_TRY(stage_copy(c->stream, js.keysA, (const char *)h_keysA + js.a0 * kb, na * kb));
-    HG_TRY(stage_copy(c->stream, js.keysB, (const char *)h_keysB + js.b0 * kb, nb * kb));
-    HG_TRY(stage_copy(c->stream, js.offA, (const char *)h_offA + lo * ob, (hi - lo + 1) * ob));
-    HG_TRY(stage_copy(c->stream, js.offB, (const char *)h_offB + lo * ob, (hi - lo + 1) * ob));
-    HG_HIP(launch_offsets_add(js.offA, ob, hi - lo + 1, (uint64_t)0 - js.a0, c->stream));
-    HG_HIP(launch_offsets_add(js.offB, ob, hi - lo + 1, (uint64_t)0 - js.b0, c->stream));
-    HG_TRY(join_range(c, descA, h_offA, h_offB, js, lo, hi, d_stage, stage_cap, h_out, capacity, written, total,
-                      full));
-    lo = hi;
+    slabs.push_back(js);
+    lo = good;
   }
+  // slab s uploads into buffer s & 1 (its offsets rebased on the upload
+  // stream) once slab s - 2's joins are done with it
+  auto upload = [&](size_t s) -> hg_status {
+    const int b = (int)(s & 1);
+    JoinSlab &js = slabs[s];
+    const uint64_t na = off_at(h_offA, ob, js.hi) - js.a0, nb = off_at(h_offB, ob, js.hi) - js.b0;
+    js.keysA = cv3.get<char>(i_k[b]);
+    js.keysB = js.keysA + na * kb;
+    js.offA = cv3.get<void>(i_oa[b]);
+    js.offB = cv3.get<void>(i_ob[b]);
+    if (s >= 2) HG_HIP(hipStreamWaitEvent(ss.in, ss.ev_done[b], 0));
+    HG_TRY(stage_copy(ss.in, js.keysA, (const char *)h_keysA + js.a0 * kb, na * kb));
+    HG_TRY(stage_copy(ss.in, js.keysB, (const char *)h_keysB + js.b0 * kb, nb * kb));
+    HG_TRY(stage_copy(ss.in, js.offA, (const char *)h_offA + js.lo * ob, (js.hi - js.lo + 1) * ob));
+    HG_TRY(stage_copy(ss.in, js.offB, (const char *)h_offB + js.lo * ob, (js.hi - js.lo + 1) * ob));
+    HG_HIP(launch_offsets_add(js.offA, ob, js.hi - js.lo + 1, (uint64_t)0 - js.a0, ss.in));
+    HG_HIP(launch_offsets_add(js.offB, ob, js.hi - js.lo + 1, (uint64_t)0 - js.b0, ss.in));
+    HG_HIP(hipEventRecord(ss.ev_in[b], ss.in));
+    return HG_OK;
+  };
+  uint64_t written = 0, total = 0;
+  bool full = false;
+  if (!slabs.empty()) HG_TRY(upload(0));
+  for (size_t s = 0; s < slabs.size(); s++) {
+    const int b = (int)(s & 1);
+    if (s + 1 < slabs.size()) HG_TRY(upload(s + 1));
+    HG_HIP(hipStreamWaitEvent(c->stream, ss.ev_in[b], 0));
+    HG_TRY(join_range(c, ss, descA, h_offA, h_offB, slabs[s], slabs[s].lo, slabs[s].hi, stg, h_out, capacity,
+                      written, total, full));
+    HG_HIP(hipEventRecord(ss.ev_done[b], c->stream));
+  }
+  HG_HIP(hipStreamSynchronize(ss.out));
   HG_HIP(hipStreamSynchronize(c->stream));
   *h_count = total;
   if (h_out && total > capacity) return fail(HG_ERR_CAPACITY, "output capacity smaller than the match count");

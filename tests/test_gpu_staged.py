@@ -272,3 +272,25 @@ def test_staged_build_of_2e32_keys(dev):
         t1.index_add_(0, bins, k)
         t2.index_add_(0, bins, k * k)
     assert torch.equal(t1, s1) and torch.equal(t2, s2)
+
+
+def test_device_offsets_are_refused(dev):
+    # the staged calls read offsets (the join) and write empty partitions'
+    # entries (the builds) from the host: device memory there is an argument
+    # error, not a crash
+    import ctypes
+    import torch
+    import hashgraph_amd as hg
+    c = hg.context(0)
+    keys = wl.uniform_keys(3, 1000, 4)
+    off, csr = hg.build_staged(keys, 1000, device_budget=4 * MiB, device=0)
+    doff = torch.from_numpy(off.view(np.int32).copy()).to(dev)
+    d = hg._desc(len(keys), 1000, 4, 4)
+    cnt = ctypes.c_uint64(0)
+    st = c.lib.hg_intersect_staged(c.bind_stream(), ctypes.byref(d), ctypes.c_void_p(doff.data_ptr()), hg._hp(csr),
+                                   ctypes.byref(d), hg._hp(off), hg._hp(csr), None, 0, ctypes.byref(cnt), 4 * MiB)
+    assert st == 1
+    dout = torch.empty(1001, dtype=torch.int32, device=dev)
+    st = c.lib.hg_build_staged(c.bind_stream(), ctypes.byref(d), hg._hp(keys), ctypes.c_void_p(dout.data_ptr()),
+                               hg._hp(csr), 4 * MiB)
+    assert st == 1
