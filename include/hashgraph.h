@@ -200,11 +200,13 @@ hg_status hg_build_partitioned_indexed(hg_ctx ctx, const hg_table_desc *desc, co
  *  device_budget  bytes of device memory the call's buffers may take
  *                 (0 = half of the free memory); at least 4 MiB.  Buffers
  *                 stay in the context's scratch until hg_ctx_destroy.
- *  n_keys may reach 2^32 and beyond (offset_bytes 8).  A run of bins that
- *  holds more keys than a slab of the budget (about budget / (4 key_bytes + 1)
- *  keys; for the intersect budget / (2 key_bytes) keys of A and B) is
+ *  n_keys may reach 2^32 and beyond (offset_bytes 8).  A build slab of k keys
+ *  and r bins takes about k (6 key_bytes + 1) + r (2 offset_bytes + 4) bytes
+ *  of 0.9 of the budget; a join slab up to budget / (4 key_bytes) keys of A
+ *  and B together.  A run of bins with more keys than one slab holds is
  *  refused with HG_ERR_UNSUPPORTED, as is a single bin with more matches than
- *  budget / (4 key_bytes).  The call returns after its last copy.
+ *  budget / (8 key_bytes).  Offsets buffers must be host-accessible (device-
+ *  only memory is HG_ERR_INVALID_ARG).  The call returns after its last copy.
  * hg_build_staged       offsets (n_bins+1) and CSR keys in host memory; the
  *                       CSR keys buffer also stages the keys between passes
  * hg_build_partitioned_staged   hg_build_partitioned's layout; n_parts <= 2048
