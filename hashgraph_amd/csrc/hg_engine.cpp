@@ -682,6 +682,9 @@ hg_status build_table(hg_ctx c, const hg_table_desc *desc, const void *d_keys, v
 // kOvfBatchSlots slots (a larger bin takes a batch of its own).  Runs after
 // the rest of the join; its records follow the others.
 constexpr uint64_t kOvfBatchSlots = 1ull << 26;
+// Padded records of the run-length join's single pass (record + heavy-list
+// bytes) beyond which the two-pass form is used.
+constexpr uint64_t kRleOnePassBytes = 4ull << 30;
 constexpr uint64_t kOvfChunk = 1ull << 16;
 
 hg_status rle_value_rich(hg_ctx c, const LaunchRle &r, const uint32_t *d_list, uint32_t n) {
@@ -1488,7 +1491,9 @@ hg_status hg_intersect_rle(hg_ctx c, const hg_table_desc *descA, const void *d_o
                i_heavy = P.add(heavy_cap * 4), i_ovf = P.add(heavy_cap * 4), i_status = P.add(scan_status_bytes(nch)),
                i_gmark = P.add(heavy_cap * 8), i_giant = P.add(rle_giant_bytes((int)descA->key_bytes)),
                i_pkey = P.add(rle_priv_records() * descA->key_bytes), i_pca = P.add(rle_priv_records() * 8),
-               i_pcb = P.add(rle_priv_records() * 8), i_pn = P.add((size_t)rle_heavy_grid() * 4);
+               i_pcb = P.add(rle_priv_records() * 8), i_pn = P.add((size_t)rle_heavy_grid() * 4),
+               i_caprec = P.add((nch + 1) * 8), i_pbase = P.add((nch + 1) * 8), i_caph = P.add((nch + 1) * 4),
+               i_phbase = P.add((nch + 1) * 4);
   HG_TRY(arena_ensure(c, 1, P.total()));
   Carve cv;
   cv.init(c, 1, P);
@@ -1532,26 +1537,100 @@ hg_status hg_intersect_rle(hg_ctx c, const hg_table_desc *descA, const void *d_o
     r.gca = (unsigned long long *)(gb + ctl_bytes + (((size_t)rle_giant_slots() * descA->key_bytes + 255) & ~(size_t)255));
     r.gcb = r.gca + rle_giant_slots();
   }
-  r.pass = 0;
-  {
-    Prof pf(c, "rle_small");
-    HG_HIP(launch_rle(r, c->stream));
-  }
-  // record positions of the small bins and heavy-list positions: exclusive
-  // scans of the chunk counts
   void *status = cv.get<void>(i_status);
-  HG_HIP(launch_scan(r.chunk_cnt, (void *)r.chunk_base, nch, 8, status, c->stream));
-  HG_HIP(launch_scan(r.chunk_heavy, (void *)r.heavy_base, nch, 4, status, c->stream));
   unsigned long long nsmall = 0;
   uint32_t nheavy = 0;
-  HG_TRY(sync_read(c, &nsmall, r.chunk_base + nch, 8));
-  HG_TRY(sync_read(c, &nheavy, r.heavy_base + nch, 4));
-  if (nheavy > heavy_cap) return fail(HG_ERR_DEVICE, "heavy bins exceed their bound");
-  // pass 1 lists the heavy bins and, with a capacity, writes the records
-  r.pass = 1;
-  {
-    Prof pf(c, "rle_small");
-    HG_HIP(launch_rle(r, c->stream));
+  // With an output, one pass of the small-bin kernel counts and writes the
+  // records and heavy bins at padded places bounded from the offsets alone,
+  // and a gather packs them; without one (or when the padded records would
+  // take more than kRleOnePassBytes) a count pass sizes the output first.
+  bool one_pass = false;
+  unsigned long long *pbase = cv.get<unsigned long long>(i_pbase);
+  uint32_t *phbase = cv.get<uint32_t>(i_phbase);
+  unsigned long long s_rec = 0;
+  uint32_t s_heavy = 0;
+  if (capacity) {
+    HG_HIP(launch_rle_caps(d_offA, d_offB, r.off_bytes, M, cv.get<unsigned long long>(i_caprec),
+                           cv.get<uint32_t>(i_caph), c->stream));
+    HG_HIP(launch_scan(cv.get<unsigned long long>(i_caprec), (void *)pbase, nch, 8, status, c->stream));
+    HG_HIP(launch_scan(cv.get<uint32_t>(i_caph), (void *)phbase, nch, 4, status, c->stream));
+    HG_TRY(sync_read(c, &s_rec, pbase + nch, 8));
+    HG_TRY(sync_read(c, &s_heavy, phbase + nch, 4));
+    one_pass = (double)s_rec * (descA->key_bytes + 16) + (double)s_heavy * 4 <= (double)kRleOnePassBytes;
+  }
+  if (one_pass) {
+    Plan P0;
+    const size_t j_rk = P0.add(std::max<uint64_t>(s_rec, 1) * descA->key_bytes),
+                 j_ca = P0.add(std::max<uint64_t>(s_rec, 1) * 8), j_cb = P0.add(std::max<uint64_t>(s_rec, 1) * 8),
+                 j_h = P0.add(((uint64_t)s_heavy + 1) * 4);
+    HG_TRY(arena_ensure(c, 0, P0.total()));
+    Carve cv0;
+    cv0.init(c, 0, P0);
+    LaunchRle r3 = r;
+    r3.pass = 3;
+    r3.chunk_base = pbase;
+    r3.heavy_base = phbase;
+    r3.rk = cv0.get<void>(j_rk);
+    r3.rca = cv0.get<unsigned long long>(j_ca);
+    r3.rcb = cv0.get<unsigned long long>(j_cb);
+    r3.capacity = s_rec;
+    r3.heavy = cv0.get<uint32_t>(j_h);
+    r3.heavy_cap = s_heavy;
+    {
+      Prof pf(c, "rle_small");
+      HG_HIP(launch_rle(r3, c->stream));
+    }
+    HG_HIP(launch_scan(r.chunk_cnt, (void *)r.chunk_base, nch, 8, status, c->stream));
+    HG_HIP(launch_scan(r.chunk_heavy, (void *)r.heavy_base, nch, 4, status, c->stream));
+    LaunchRleGather gt{};
+    gt.key_bytes = r.key_bytes;
+    gt.nch = (uint32_t)nch;
+    gt.pbase = pbase;
+    gt.cbase = r.chunk_base;
+    gt.cnt = r.chunk_cnt;
+    gt.srk = r3.rk;
+    gt.srca = r3.rca;
+    gt.srcb = r3.rcb;
+    gt.rk = r.rk;
+    gt.rca = r.rca;
+    gt.rcb = r.rcb;
+    gt.capacity = capacity;
+    gt.phbase = phbase;
+    gt.hbase = r.heavy_base;
+    gt.hcnt = r.chunk_heavy;
+    gt.sheavy = r3.heavy;
+    gt.heavy = r.heavy;
+    gt.heavy_cap = heavy_cap;
+    gt.bad = (unsigned int *)(small + 5);
+    {
+      Prof pf(c, "rle_gather");
+      HG_HIP(launch_rle_gather(gt, c->stream));
+    }
+    HG_TRY(sync_read(c, &nsmall, r.chunk_base + nch, 8));
+    HG_TRY(sync_read(c, &nheavy, r.heavy_base + nch, 4));
+    unsigned int bad = 0;
+    HG_TRY(sync_read(c, &bad, small + 5, 4));
+    if (bad) return fail(HG_ERR_DEVICE, "a chunk of the run-length join exceeded its bound");
+    if (nheavy > heavy_cap) return fail(HG_ERR_DEVICE, "heavy bins exceed their bound");
+  } else {
+    r.pass = 0;
+    {
+      Prof pf(c, "rle_small");
+      HG_HIP(launch_rle(r, c->stream));
+    }
+    // record positions of the small bins and heavy-list positions: exclusive
+    // scans of the chunk counts
+    HG_HIP(launch_scan(r.chunk_cnt, (void *)r.chunk_base, nch, 8, status, c->stream));
+    HG_HIP(launch_scan(r.chunk_heavy, (void *)r.heavy_base, nch, 4, status, c->stream));
+    HG_TRY(sync_read(c, &nsmall, r.chunk_base + nch, 8));
+    HG_TRY(sync_read(c, &nheavy, r.heavy_base + nch, 4));
+    if (nheavy > heavy_cap) return fail(HG_ERR_DEVICE, "heavy bins exceed their bound");
+    // pass 1 lists the heavy bins and, with a capacity, writes the records
+    r.pass = 1;
+    {
+      Prof pf(c, "rle_small");
+      HG_HIP(launch_rle(r, c->stream));
+    }
   }
   // heavy bins append after the small bins' records
   HG_HIP(launch_fill_u64(small + 2, nsmall, 1, c->stream));

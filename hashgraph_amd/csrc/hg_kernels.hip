@@ -1060,7 +1060,7 @@ __global__ __launch_bounds__(BLOCK, 6) void k_rle_small(const OFF *__restrict__ 
   __shared__ K s_ka[RLE_KCAP], s_kb[RLE_KCAP];
   const uint64_t b0 = (uint64_t)blockIdx.x * RLE_CHUNK;
   const uint32_t nb = (uint32_t)min((uint64_t)RLE_CHUNK, M - b0);
-  const bool emit = pass == 0 || capacity != 0;
+  const bool emit = pass != 1 || capacity != 0;
   {
     // all offset loads of the thread in flight together (one round trip)
     OFF va[PER + 1], vb[PER + 1];
@@ -1108,8 +1108,9 @@ __global__ __launch_bounds__(BLOCK, 6) void k_rle_small(const OFF *__restrict__ 
   __shared__ uint32_t s_mtot;
   const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   constexpr uint32_t NW = BLOCK / 64;
-  // pass 0: the chunk's totals (records | heavy << 32, pairs); pass 1: the
-  // records and heavy bins written so far
+  // pass 0: the chunk's totals (records | heavy << 32, pairs); passes 1
+  // and 3: the records and heavy bins written so far (pass 3 also sums the
+  // pairs)
   unsigned long long acc = 0, accp = 0;
   const unsigned long long cbase = pass == 0 ? 0ull : chunk_base[blockIdx.x];
   const uint64_t hbase = pass == 0 ? 0ull : (uint64_t)heavy_base[blockIdx.x];
@@ -1183,6 +1184,7 @@ __global__ __launch_bounds__(BLOCK, 6) void k_rle_small(const OFF *__restrict__ 
       accp += uni64(block_sum<BLOCK, unsigned long long>(npair + mpair, s_t64));
       continue;
     }
+    if (pass == 3) accp += uni64(block_sum<BLOCK, unsigned long long>(npair, s_t64));  // the tiny bins' pairs
     // pass 1: the tiny bins' records at the sub-range's base plus the threads
     // before, then the medium bins' records after all of them
     const unsigned long long rbase = cbase + (acc & 0xFFFFFFFFull);
@@ -1221,6 +1223,7 @@ __global__ __launch_bounds__(BLOCK, 6) void k_rle_small(const OFF *__restrict__ 
     __syncthreads();
     const unsigned long long mb = uni64(s_mbase);
     acc += uni32(s_mtot);  // the medium bins' records
+    unsigned long long mp = 0;  // pass 3: the medium bins' pairs
     for (uint32_t k = wv; k < nmed; k += NW) {
       const uint32_t l = s_med[k];
       const uint64_t a0 = s_oa[l], x0 = s_ob[l];
@@ -1237,10 +1240,12 @@ __global__ __launch_bounds__(BLOCK, 6) void k_rle_small(const OFF *__restrict__ 
           rca[p] = ca;
           rcb[p] = cb;
         }
+        mp += (unsigned long long)ca * cb;
       }
     }
+    if (pass == 3) accp += uni64(block_sum<BLOCK, unsigned long long>(mp, s_t64));
   }
-  if (pass == 0 && threadIdx.x == 0) {
+  if ((pass == 0 || pass == 3) && threadIdx.x == 0) {
     chunk_cnt[blockIdx.x] = acc & 0xFFFFFFFFull;
     chunk_heavy[blockIdx.x] = (uint32_t)(acc >> 32);
     chunk_pairs[blockIdx.x] = accp;
@@ -2875,15 +2880,79 @@ hipError_t launch_scan(const void *in, void *out, uint64_t n, int elem_bytes, vo
 }
 uint64_t scan_tiles(uint64_t n) { return (n + 256 * 16 - 1) / (256 * 16); }
 
+// Bounds of every chunk of the run-length join from the offsets alone: its
+// records (values present on both sides) number at most min(A keys, B keys),
+// its heavy bins (more than RLE_SMALL keys on a side) at most
+// A keys / (RLE_SMALL + 1) + B keys / (RLE_SMALL + 1).
+template <typename OFF>
+__global__ void k_rle_caps(const OFF *__restrict__ offA, const OFF *__restrict__ offB, uint64_t M, uint32_t nch,
+                           unsigned long long *__restrict__ cap_rec, uint32_t *__restrict__ cap_h) {
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < nch; j += gridDim.x * blockDim.x) {
+    const uint64_t lo = (uint64_t)j * RLE_CHUNK, hi = min(lo + (uint64_t)RLE_CHUNK, M);
+    const uint64_t na = (uint64_t)offA[hi] - (uint64_t)offA[lo], nb = (uint64_t)offB[hi] - (uint64_t)offB[lo];
+    cap_rec[j] = min(na, nb);
+    cap_h[j] = (uint32_t)min(hi - lo, na / (RLE_SMALL + 1) + nb / (RLE_SMALL + 1));
+  }
+}
+
+// One workgroup per chunk (grid-stride): its records and heavy bins from the
+// padded places to the compact ones; records at or beyond capacity are
+// dropped, as when written directly.
+template <typename K>
+__global__ __launch_bounds__(256) void k_rle_gather(LaunchRleGather g) {
+  const K *srk = (const K *)g.srk;
+  K *rk = (K *)g.rk;
+  for (uint32_t j = blockIdx.x; j < g.nch; j += gridDim.x) {
+    const uint64_t s = g.pbase[j], n = g.cnt[j], d = g.cbase[j];
+    const uint32_t hs = g.phbase[j], hn = g.hcnt[j], hd = g.hbase[j];
+    if (n > g.pbase[j + 1] - s || hn > g.phbase[j + 1] - hs) {  // uniform
+      if (threadIdx.x == 0) atomicOr(g.bad, 1u);
+      continue;
+    }
+    for (uint64_t i = threadIdx.x; i < n; i += 256)
+      if (d + i < g.capacity) {
+        rk[d + i] = srk[s + i];
+        g.rca[d + i] = g.srca[s + i];
+        g.rcb[d + i] = g.srcb[s + i];
+      }
+    for (uint32_t i = threadIdx.x; i < hn; i += 256)
+      if ((uint64_t)hd + i < g.heavy_cap) g.heavy[hd + i] = g.sheavy[hs + i];
+  }
+}
+
+hipError_t launch_rle_caps(const void *offA, const void *offB, int off_bytes, uint64_t M,
+                           unsigned long long *cap_rec, uint32_t *cap_h, hipStream_t st) {
+  const uint32_t nch = (uint32_t)((M + RLE_CHUNK - 1) / RLE_CHUNK);
+  if (!nch) return hipSuccess;
+  const uint32_t grid = (uint32_t)std::min<uint64_t>((nch + 255) / 256, 4096);
+  if (off_bytes == 4)
+    hipLaunchKernelGGL(k_rle_caps<uint32_t>, dim3(grid), dim3(256), 0, st, (const uint32_t *)offA,
+                       (const uint32_t *)offB, M, nch, cap_rec, cap_h);
+  else
+    hipLaunchKernelGGL(k_rle_caps<uint64_t>, dim3(grid), dim3(256), 0, st, (const uint64_t *)offA,
+                       (const uint64_t *)offB, M, nch, cap_rec, cap_h);
+  return hipGetLastError();
+}
+
+hipError_t launch_rle_gather(const LaunchRleGather &g, hipStream_t st) {
+  if (!g.nch) return hipSuccess;
+  const uint32_t grid = std::min<uint32_t>(g.nch, 8192);
+  if (g.key_bytes == 4)
+    hipLaunchKernelGGL(k_rle_gather<uint32_t>, dim3(grid), dim3(256), 0, st, g);
+  else
+    hipLaunchKernelGGL(k_rle_gather<uint64_t>, dim3(grid), dim3(256), 0, st, g);
+  return hipGetLastError();
+}
+
 template <typename K, typename OFF>
 static hipError_t launch_rle_t(const LaunchRle &r, hipStream_t st) {
   const uint32_t nchunks = (uint32_t)((r.M + RLE_CHUNK - 1) / RLE_CHUNK);
-  if (r.pass <= 1 && nchunks) {
+  if ((r.pass <= 1 || r.pass == 3) && nchunks) {
     hipLaunchKernelGGL((k_rle_small<K, OFF, 256>), dim3(nchunks), dim3(256), 0, st, (const OFF *)r.offA,
                        (const K *)r.keysA, (const OFF *)r.offB, (const K *)r.keysB, r.M, r.pass, r.chunk_cnt,
                        r.chunk_base, r.chunk_heavy, r.heavy_base, r.chunk_pairs, (K *)r.rk, r.rca, r.rcb, r.capacity,
                        r.heavy, r.heavy_cap);
-    if (r.pass == 0)
+    if (r.pass == 0 || r.pass == 3)
       hipLaunchKernelGGL(k_sum_add, dim3(1), dim3(1024), 0, st, (const unsigned long long *)r.chunk_pairs,
                          (uint64_t)nchunks, r.pairs);
   }

@@ -93,9 +93,29 @@ struct LaunchJoin {
   uint32_t hlist_cap;
 };
 
+// Moves the run-length join's records and heavy bins of every chunk from
+// the padded places a single pass wrote them to (scans of their bounds) to
+// the compact places (scans of their counts).
+struct LaunchRleGather {
+  int key_bytes;
+  uint32_t nch;
+  const unsigned long long *pbase, *cbase, *cnt;  // padded base, compact base, records per chunk
+  const void *srk;
+  const unsigned long long *srca, *srcb;
+  void *rk;
+  unsigned long long *rca, *rcb;
+  uint64_t capacity;
+  const uint32_t *phbase, *hbase, *hcnt;  // the same for the heavy bins
+  const uint32_t *sheavy;
+  uint32_t *heavy;
+  uint64_t heavy_cap;
+  unsigned int *bad;  // set when a chunk exceeded its bound
+};
+
 struct LaunchRle {
   int key_bytes, off_bytes;
-  int pass;  // 0 count small bins, 1 emit small bins, 2 heavy bins
+  int pass;  // 0 count small bins, 1 emit small bins, 2 heavy bins,
+             // 3 count and emit small bins at padded bases (chunk_base, heavy_base)
   uint64_t M;
   const void *offA, *keysA, *offB, *keysB;
   unsigned long long *chunk_cnt;         // per chunk: records (pass 0)
@@ -204,6 +224,11 @@ hipError_t launch_set_kinds(const uint32_t *use, SegDesc *segs, uint32_t nsegs, 
 uint32_t rmi_moment_blocks(uint64_t n_sample);
 uint32_t rmi_sort_tiles(uint64_t n_sample);
 hipError_t launch_rle(const LaunchRle &r, hipStream_t st);
+// per chunk of the run-length join: records <= min(keys of A, keys of B),
+// heavy bins <= keys of A / 65 + keys of B / 65 (from the offsets alone)
+hipError_t launch_rle_caps(const void *offA, const void *offB, int off_bytes, uint64_t M,
+                           unsigned long long *cap_rec, uint32_t *cap_h, hipStream_t st);
+hipError_t launch_rle_gather(const LaunchRleGather &g, hipStream_t st);
 hipError_t launch_rle_ovf(const LaunchRleOvf &o, hipStream_t st);
 // (a0, a1, x0, x1) of each listed bin -> out (4 entries per bin)
 hipError_t launch_ovf_sizes(const uint32_t *list, uint32_t n, const void *offA, const void *offB, int off_bytes,
