@@ -31,7 +31,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 METRIC = "Gkeys/s build & intersect at 2^28 u32 keys, %HBM peak; learned/murmur ratio"
-# host threads the CPU baselines use: the GPU box's CPU share for one GPU
+# host threads the CPU baselines use: at most 16, one GPU's share of a node's cores
 CPU_THREADS = min(16, os.cpu_count() or 1)
 
 CONFIGS = {

@@ -3,8 +3,8 @@
  *
  * TEST INFRASTRUCTURE ONLY.  Nothing in the product library links, loads or
  * calls this file.  Only the tests, the smoke check and the benchmark's CPU
- * baseline leg use it, and only as the checker / the CPU port of the
- * reference.  It is plain C (gcc, optional OpenMP), integer arithmetic
+ * baseline leg use it, and only to check results / as the CPU port of
+ * the reference.  It is plain C (gcc, optional OpenMP), integer arithmetic
  * everywhere except where the reference itself uses float (split points,
  * LRB bucket width) and the learned-hash model, which restates the FP32
  * model evaluation of the product with -ffp-contract=off.

@@ -1,7 +1,7 @@
 """numpy front end of the CPU oracle (``hg_oracle.c``).
 
 TEST INFRASTRUCTURE ONLY: imported by ``tests/``, ``__graft_entry__.smoke()``
-and the CPU-baseline leg of ``bench.py``, always as the checker (or as the
+and the CPU-baseline leg of ``bench.py``, always to check results (or as the
 timed CPU port of the reference), never on the product path.
 
 Each wrapper names the reference file:line its C function restates.
