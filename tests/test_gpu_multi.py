@@ -188,3 +188,37 @@ def test_rle_value_rich_giant_bin(dev):
     A2 = np.concatenate([A, np.full(300000, hot, dtype=np.uint32)])
     B2 = np.concatenate([B, np.full(1000, hot, dtype=np.uint32)])
     _check_rle(A2, B2, 3, dev)
+
+
+def test_rle_short_capacity_reports_exact_counts(dev):
+    """A run-length join with one record less than it needs (the single-pass
+    form, which packs records after writing them at padded places): the call
+    returns HG_ERR_CAPACITY with the exact record and pair counts, and every
+    record it wrote is one of the true records."""
+    import ctypes
+    import torch
+    import hashgraph_amd as hg
+    M = 1 << 18
+    A = wl.zipf_keys(5, 1 << 18, 1.1)
+    B = wl.zipf_keys(6, 1 << 17, 1.1)
+    oa, ka = hg.build(_to_dev(A, dev), M)
+    ob, kb = hg.build(_to_dev(B, dev), M)
+    cnt, rk, ca, cb = hg.intersect_rle(oa, ka, ob, kb, M)
+    n = rk.numel()
+    assert n > 1 and cnt == o.bag_intersect_count(A, B)
+    c = hg.context(0)
+    dA = hg._desc(ka.numel(), M, 8, 4)
+    dB = hg._desc(kb.numel(), M, 8, 4)
+    cap = n - 1
+    rk2 = torch.zeros(cap, dtype=torch.int64, device=dev)
+    ca2 = torch.zeros(cap, dtype=torch.int64, device=dev)
+    cb2 = torch.zeros(cap, dtype=torch.int64, device=dev)
+    nrec, tot = ctypes.c_uint64(0), ctypes.c_uint64(0)
+    st = c.lib.hg_intersect_rle(c.bind_stream(), ctypes.byref(dA), hg._p(oa), hg._p(ka), ctypes.byref(dB), hg._p(ob),
+                                hg._p(kb), hg._p(rk2), hg._p(ca2), hg._p(cb2), cap, ctypes.byref(nrec),
+                                ctypes.byref(tot))
+    assert st == 4
+    assert nrec.value == n and tot.value == cnt
+    true = set(zip(rk.tolist(), ca.tolist(), cb.tolist()))
+    got = set(zip(rk2.tolist(), ca2.tolist(), cb2.tolist()))
+    assert got <= true and len(got) == cap  # every slot holds a distinct true record
