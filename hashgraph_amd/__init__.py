@@ -19,7 +19,8 @@ from ._lib import HG_HASH_MURMUR3, HG_HASH_RMI, HGError, check
 __all__ = ["Context", "build", "probe", "intersect", "intersect_rle", "exclusive_scan", "hash_bins", "find_splits",
            "build_partitioned", "rmi_train", "rmi_train_batch", "rmi_model_bytes", "batch_build", "batch_probe", "batch_probe_emit", "HGError",
            "HG_HASH_MURMUR3", "HG_HASH_RMI", "version", "table_bins", "BatchPlan", "build_staged",
-           "build_partitioned_staged", "find_splits_staged", "intersect_staged"]
+           "build_partitioned_staged", "find_splits_staged", "intersect_staged", "build_indexed",
+           "intersect_indexed"]
 
 _ctx = {}
 
@@ -198,6 +199,49 @@ def intersect_rle(offA, keysA, offB, keysB, n_bins: int, hash_kind: int = HG_HAS
           "hg_intersect_rle")
     n = nrec.value
     return cnt.value, rk[:n], ca[:n], cb[:n]
+
+
+def build_indexed(keys, n_bins: int, offset_bytes: int = 4):
+    """hg_build plus every CSR key's input position (the INDEX_TRACK entries).
+    Returns (offsets, csr_keys, csr_index int64)."""
+    torch = _torch()
+    _check_dev(keys)
+    c = context(keys.device.index)
+    off = torch.empty(n_bins + 1, dtype=torch.int32 if offset_bytes == 4 else torch.int64, device=keys.device)
+    csr = torch.empty_like(keys)
+    ind = torch.empty(max(keys.numel(), 1), dtype=torch.int64, device=keys.device)
+    d = _desc(keys.numel(), n_bins, _key_bytes(keys), offset_bytes)
+    check(c.lib.hg_build_indexed(c.bind_stream(), ctypes.byref(d), _p(keys), _p(off), _p(csr), _p(ind)),
+          "hg_build_indexed")
+    return off, csr, ind[: keys.numel()]
+
+
+def intersect_indexed(offA, keysA, indA, offB, keysB, indB, n_bins: int, emit: bool = True,
+                      capacity: int | None = None):
+    """Row-id join of two index-carrying tables: for every matching pair, A's
+    and B's input positions.  Returns (count, pos_a, pos_b), the positions
+    None for count only."""
+    torch = _torch()
+    _check_dev(offA, keysA, indA, offB, keysB, indB)
+    kb = _key_bytes(keysA)
+    ob = offA.element_size()
+    c = context(keysA.device.index)
+    dA = _desc(keysA.numel(), n_bins, kb, ob)
+    dB = _desc(keysB.numel(), n_bins, kb, ob)
+    cnt = ctypes.c_uint64(0)
+    if capacity is None:
+        check(c.lib.hg_intersect_indexed(c.bind_stream(), ctypes.byref(dA), _p(offA), _p(keysA), _p(indA),
+                                         ctypes.byref(dB), _p(offB), _p(keysB), _p(indB), None, None, 0,
+                                         ctypes.byref(cnt)), "hg_intersect_indexed")
+        if not emit:
+            return cnt.value, None, None
+        capacity = cnt.value
+    pa = torch.empty(max(capacity, 1), dtype=torch.int64, device=keysA.device)
+    pb = torch.empty(max(capacity, 1), dtype=torch.int64, device=keysA.device)
+    check(c.lib.hg_intersect_indexed(c.bind_stream(), ctypes.byref(dA), _p(offA), _p(keysA), _p(indA),
+                                     ctypes.byref(dB), _p(offB), _p(keysB), _p(indB), _p(pa), _p(pb), capacity,
+                                     ctypes.byref(cnt)), "hg_intersect_indexed")
+    return cnt.value, pa[: cnt.value], pb[: cnt.value]
 
 
 def exclusive_scan(x):

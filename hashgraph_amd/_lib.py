@@ -27,7 +27,7 @@ EXPORTS = [
     "hg_exclusive_scan", "hg_hash_bins", "hg_table_bins", "hg_batch_plan_create", "hg_batch_plan_run",
     "hg_batch_plan_query", "hg_batch_plan_destroy", "hg_build_noalloc", "hg_build_indexed",
     "hg_build_partitioned_indexed", "hg_build_staged", "hg_build_partitioned_staged", "hg_find_splits_staged",
-    "hg_intersect_staged",
+    "hg_intersect_staged", "hg_intersect_indexed",
 ]
 
 
@@ -103,6 +103,8 @@ def load(path: str = LIB_PATH):
             "hg_find_splits": (i32, [vp, P(TableDesc), vp, u64, u32, P(ctypes.c_int64)]),
             "hg_build_partitioned": (i32, [vp, P(TableDesc), vp, P(ctypes.c_int64), u32, vp, vp, P(u64)]),
             "hg_build_indexed": (i32, [vp, P(TableDesc), vp, vp, vp, vp]),
+            "hg_intersect_indexed": (i32, [vp, P(TableDesc), vp, vp, vp, P(TableDesc), vp, vp, vp, vp, vp, u64,
+                                           P(u64)]),
             "hg_build_partitioned_indexed": (i32, [vp, P(TableDesc), vp, P(ctypes.c_int64), u32, vp, vp, vp, P(u64)]),
             "hg_build_staged": (i32, [vp, P(TableDesc), vp, vp, vp, ctypes.c_size_t]),
             "hg_build_partitioned_staged": (i32, [vp, P(TableDesc), vp, P(ctypes.c_int64), u32, vp, vp, P(u64),

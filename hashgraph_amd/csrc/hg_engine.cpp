@@ -682,6 +682,9 @@ hg_status build_table(hg_ctx c, const hg_table_desc *desc, const void *d_keys, v
 // kOvfBatchSlots slots (a larger bin takes a batch of its own).  Runs after
 // the rest of the join; its records follow the others.
 constexpr uint64_t kOvfBatchSlots = 1ull << 26;
+// Key pairs of the row-id join's bins above 4096 pairs, compared one by one,
+// beyond which the call is refused (~2^36 compares take about a second).
+constexpr uint64_t kPairCompareLimit = 1ull << 36;
 // Padded records of the run-length join's single pass (record + heavy-list
 // bytes) beyond which the two-pass form is used.
 constexpr uint64_t kRleOnePassBytes = 4ull << 30;
@@ -1461,6 +1464,95 @@ hg_status hg_intersect(hg_ctx c, const hg_table_desc *descA, const void *d_offA,
   *h_count = res[0];
   if ((res[1] & 0xFFFFFFFFull) > hcap) return fail(HG_ERR_DEVICE, "heavy join bins exceed their bound");
   if (d_out && res[0] > capacity) return fail(HG_ERR_CAPACITY, "output capacity smaller than the match count");
+  return HG_OK;
+}
+
+hg_status hg_intersect_indexed(hg_ctx c, const hg_table_desc *descA, const void *d_offA, const void *d_keysA,
+                               const uint64_t *d_indA, const hg_table_desc *descB, const void *d_offB,
+                               const void *d_keysB, const uint64_t *d_indB, uint64_t *d_pos_a, uint64_t *d_pos_b,
+                               uint64_t capacity, uint64_t *h_count) {
+  if (!c) return fail(HG_ERR_INVALID_ARG, "null ctx");
+  if (!h_count) return fail(HG_ERR_INVALID_ARG, "null h_count");
+  if ((d_pos_a == nullptr) != (d_pos_b == nullptr)) return fail(HG_ERR_INVALID_ARG, "give both d_pos_a and d_pos_b");
+  if (d_pos_a && ((descA && descA->n_keys && !d_indA) || (descB && descB->n_keys && !d_indB)))
+    return fail(HG_ERR_INVALID_ARG, "null input positions");
+  // the count is the key join's
+  HG_TRY(hg_intersect(c, descA, d_offA, d_keysA, descB, d_offB, d_keysB, nullptr, 0, h_count));
+  if (!d_pos_a) return HG_OK;
+  const uint64_t count = *h_count;
+  if (count > capacity) return fail(HG_ERR_CAPACITY, "output capacity smaller than the match count");
+  if (count == 0) return HG_OK;
+  DeviceGuard g(c->device);
+  const uint64_t hcap = std::min<uint64_t>(descA->n_bins, (descA->n_keys + descB->n_keys) / 128 + 1);
+  Plan P;
+  const size_t i_small = P.add(256), i_hlist = P.add(hcap * 4), i_tiles = P.add((hcap + 1) * 8),
+               i_tbase = P.add((hcap + 1) * 8), i_status = P.add(scan_status_bytes(hcap));
+  HG_TRY(arena_ensure(c, 1, P.total()));
+  Carve cv;
+  cv.init(c, 1, P);
+  unsigned long long *small = cv.get<unsigned long long>(i_small);  // [cursor, listed bins, pairs]
+  HG_HIP(launch_fill_u64(small, 0ull, 4, c->stream));
+  LaunchJoin j{};
+  j.key_bytes = (int)descA->key_bytes;
+  j.off_bytes = (int)descA->offset_bytes;
+  j.M = descA->n_bins;
+  j.key_cap = join_key_cap(j.key_bytes, j.off_bytes);
+  j.offA = d_offA;
+  j.keysA = d_keysA;
+  j.offB = d_offB;
+  j.keysB = d_keysB;
+  j.out = nullptr;
+  j.capacity = capacity;
+  j.cursor = small;
+  j.hlist = cv.get<uint32_t>(i_hlist);
+  j.hlist_n = (unsigned int *)(small + 1);
+  j.hlist_cap = (uint32_t)hcap;
+  j.indA = d_indA;
+  j.indB = d_indB;
+  j.pa = d_pos_a;
+  j.pb = d_pos_b;
+  {
+    Prof pf(c, "join_pairs");
+    HG_HIP(launch_join(j, c->stream));
+  }
+  unsigned long long res[2];
+  HG_TRY(sync_read(c, res, small, 16));
+  const uint64_t nh = res[1] & 0xFFFFFFFFull;
+  if (nh > hcap) return fail(HG_ERR_DEVICE, "heavy join bins exceed their bound");
+  if (nh) {
+    unsigned long long *tiles = cv.get<unsigned long long>(i_tiles), *tbase = cv.get<unsigned long long>(i_tbase);
+    HG_HIP(launch_pair_tiles(d_offA, d_offB, j.off_bytes, j.hlist, (uint32_t)nh, tiles, small + 2, c->stream));
+    HG_HIP(launch_scan(tiles, (void *)tbase, nh, 8, cv.get<void>(i_status), c->stream));
+    unsigned long long pairs = 0, ntiles = 0;
+    HG_TRY(sync_read(c, &pairs, small + 2, 8));
+    HG_TRY(sync_read(c, &ntiles, tbase + nh, 8));
+    if (pairs > kPairCompareLimit)
+      return fail(HG_ERR_UNSUPPORTED, "bins with more than 4096 key pairs hold " + std::to_string(pairs) +
+                                          " key pairs together; the row-id join compares them one by one and "
+                                          "takes at most 2^36");
+    LaunchJoinPairs p{};
+    p.key_bytes = j.key_bytes;
+    p.off_bytes = j.off_bytes;
+    p.offA = d_offA;
+    p.keysA = d_keysA;
+    p.offB = d_offB;
+    p.keysB = d_keysB;
+    p.indA = d_indA;
+    p.indB = d_indB;
+    p.hlist = j.hlist;
+    p.nh = (uint32_t)nh;
+    p.tile_base = tbase;
+    p.ntiles = ntiles;
+    p.pa = d_pos_a;
+    p.pb = d_pos_b;
+    p.capacity = capacity;
+    p.cursor = small;
+    Prof pf(c, "join_pairs_heavy");
+    HG_HIP(launch_join_pairs_heavy(p, c->stream));
+  }
+  unsigned long long written = 0;
+  HG_TRY(sync_read(c, &written, small, 8));
+  if (written != count) return fail(HG_ERR_DEVICE, "row-id join wrote a different number of pairs than it counted");
   return HG_OK;
 }
 

@@ -739,7 +739,9 @@ __global__ __launch_bounds__(BLOCK) void k_join(const OFF *__restrict__ offA, co
                                                 K *__restrict__ out, uint64_t capacity,
                                                 unsigned long long *__restrict__ cursor, uint32_t key_cap,
                                                 uint32_t *__restrict__ hlist, unsigned int *__restrict__ hlist_n,
-                                                uint32_t hlist_cap) {
+                                                uint32_t hlist_cap, const uint64_t *__restrict__ indA,
+                                                const uint64_t *__restrict__ indB, uint64_t *__restrict__ pa,
+                                                uint64_t *__restrict__ pb) {
   extern __shared__ __align__(16) uint8_t smem[];
   __shared__ OFF s_oa[JB + 1], s_ob[JB + 1];
   __shared__ unsigned long long s_base;
@@ -773,7 +775,7 @@ __global__ __launch_bounds__(BLOCK) void k_join(const OFF *__restrict__ offA, co
       for (uint64_t ib = (uint64_t)(s_ob[l] - x0); ib < eb; ib++) mine += (b_lds ? s_b[ib] : keysB[x0 + ib]) == a;
     }
   }
-  if (out == nullptr) {
+  if (out == nullptr && pa == nullptr) {
     __shared__ unsigned long long s_t64[BLOCK / 64 + 1];
     unsigned long long t = block_sum<BLOCK, unsigned long long>(mine, s_t64);
     if (threadIdx.x == 0 && t) atomicAdd(cursor, t);
@@ -794,7 +796,14 @@ __global__ __launch_bounds__(BLOCK) void k_join(const OFF *__restrict__ offA, co
       for (uint64_t ib = (uint64_t)(s_ob[l] - x0); ib < eb; ib++) {
         const K x = b_lds ? s_b[ib] : keysB[x0 + ib];
         if (x == a) {
-          if (w < capacity) out[w] = x;
+          if (w < capacity) {
+            if (pa) {  // row-id join: the two entries' input positions
+              pa[w] = indA[a0 + ia];
+              pb[w] = indB[x0 + ib];
+            } else {
+              out[w] = x;
+            }
+          }
           w++;
         }
       }
@@ -2812,6 +2821,111 @@ uint32_t probe_akey_cap(int key_bytes, int off_bytes) {
   return (uint32_t)((PROBE_LDS - used) / key_bytes);
 }
 
+// Row-id join of the bins above JOIN_PAIR_LIMIT key pairs: every pair of
+// entries is compared, in tiles of PAIR_TILE x PAIR_TILE entries spread over
+// the grid.  k_pair_tiles gives every listed bin its tile count and adds its
+// key pairs (each capped at 2^37, so the sum of up to 2^26 bins cannot wrap)
+// to *pairs.
+constexpr uint32_t PAIR_TILE = 1024;
+template <typename OFF>
+__global__ void k_pair_tiles(const OFF *__restrict__ offA, const OFF *__restrict__ offB,
+                             const uint32_t *__restrict__ hlist, uint32_t n, unsigned long long *__restrict__ tiles,
+                             unsigned long long *__restrict__ pairs) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint32_t b = hlist[i];
+    const uint64_t a = (uint64_t)offA[b + 1] - (uint64_t)offA[b], x = (uint64_t)offB[b + 1] - (uint64_t)offB[b];
+    tiles[i] = ((a + PAIR_TILE - 1) / PAIR_TILE) * ((x + PAIR_TILE - 1) / PAIR_TILE);
+    const unsigned long long ax = (a >> 31) || (x >> 31) ? (1ull << 37) : (unsigned long long)(a * x);
+    atomicAdd(pairs, ax < (1ull << 37) ? ax : (1ull << 37));
+  }
+}
+
+// One workgroup per tile (grid-stride): the tile's entries of A and of B in
+// LDS, its matches counted, their places reserved with one atomic, then
+// written as (A's input position, B's input position).
+template <typename K, typename OFF, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_join_pairs_heavy(
+    const OFF *__restrict__ offA, const K *__restrict__ keysA, const uint64_t *__restrict__ indA,
+    const OFF *__restrict__ offB, const K *__restrict__ keysB, const uint64_t *__restrict__ indB,
+    const uint32_t *__restrict__ hlist, uint32_t nh, const unsigned long long *__restrict__ tile_base, uint64_t ntiles,
+    uint64_t *__restrict__ pa, uint64_t *__restrict__ pb, uint64_t capacity, unsigned long long *__restrict__ cursor) {
+  __shared__ K s_a[PAIR_TILE], s_b[PAIR_TILE];
+  __shared__ unsigned long long s_t64[BLOCK / 64 + 1];
+  __shared__ unsigned long long s_base;
+  for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    // the listed bin of tile t: tile_base[h] <= t < tile_base[h + 1]
+    uint32_t lo = 0, hi = nh;
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) / 2;
+      if (tile_base[mid] <= t) lo = mid;
+      else hi = mid;
+    }
+    const uint32_t bin = hlist[lo];
+    const uint64_t a0 = offA[bin], a1 = offA[bin + 1], x0 = offB[bin], x1 = offB[bin + 1];
+    const uint64_t nbt = (x1 - x0 + PAIR_TILE - 1) / PAIR_TILE;
+    const uint64_t lt = t - tile_base[lo];
+    const uint64_t ia0 = a0 + (lt / nbt) * PAIR_TILE, ix0 = x0 + (lt % nbt) * PAIR_TILE;
+    const uint32_t na = (uint32_t)min((uint64_t)PAIR_TILE, a1 - ia0), nb = (uint32_t)min((uint64_t)PAIR_TILE, x1 - ix0);
+    for (uint32_t q = threadIdx.x; q < na; q += BLOCK) s_a[q] = keysA[ia0 + q];
+    for (uint32_t q = threadIdx.x; q < nb; q += BLOCK) s_b[q] = keysB[ix0 + q];
+    __syncthreads();
+    unsigned long long c = 0;
+    for (uint32_t i = threadIdx.x; i < na; i += BLOCK) {
+      const K a = s_a[i];
+      for (uint32_t j = 0; j < nb; j++) c += s_b[j] == a;
+    }
+    unsigned long long tot;
+    const unsigned long long ex = block_exclusive_sum<BLOCK, unsigned long long>(c, s_t64, tot);
+    if (threadIdx.x == 0) s_base = tot ? atomicAdd(cursor, tot) : 0ull;
+    __syncthreads();
+    unsigned long long w = s_base + ex;
+    if (c)
+      for (uint32_t i = threadIdx.x; i < na; i += BLOCK) {
+        const K a = s_a[i];
+        for (uint32_t j = 0; j < nb; j++)
+          if (s_b[j] == a) {
+            if (w < capacity) {
+              pa[w] = indA[ia0 + i];
+              pb[w] = indB[ix0 + j];
+            }
+            w++;
+          }
+      }
+    __syncthreads();  // the next tile overwrites s_a, s_b
+  }
+}
+
+hipError_t launch_pair_tiles(const void *offA, const void *offB, int off_bytes, const uint32_t *hlist, uint32_t n,
+                             unsigned long long *tiles, unsigned long long *pairs, hipStream_t st) {
+  if (!n) return hipSuccess;
+  const uint32_t grid = (uint32_t)std::min<uint64_t>((n + 255) / 256, 4096);
+  if (off_bytes == 4)
+    hipLaunchKernelGGL(k_pair_tiles<uint32_t>, dim3(grid), dim3(256), 0, st, (const uint32_t *)offA,
+                       (const uint32_t *)offB, hlist, n, tiles, pairs);
+  else
+    hipLaunchKernelGGL(k_pair_tiles<uint64_t>, dim3(grid), dim3(256), 0, st, (const uint64_t *)offA,
+                       (const uint64_t *)offB, hlist, n, tiles, pairs);
+  return hipGetLastError();
+}
+
+template <typename K, typename OFF>
+static hipError_t launch_join_pairs_heavy_t(const LaunchJoinPairs &p, hipStream_t st) {
+  const uint32_t grid = (uint32_t)std::min<uint64_t>(p.ntiles, 8192);
+  hipLaunchKernelGGL((k_join_pairs_heavy<K, OFF, 256>), dim3(grid), dim3(256), 0, st, (const OFF *)p.offA,
+                     (const K *)p.keysA, p.indA, (const OFF *)p.offB, (const K *)p.keysB, p.indB, p.hlist, p.nh,
+                     p.tile_base, p.ntiles, p.pa, p.pb, p.capacity, p.cursor);
+  return hipGetLastError();
+}
+
+hipError_t launch_join_pairs_heavy(const LaunchJoinPairs &p, hipStream_t st) {
+  if (!p.ntiles) return hipSuccess;
+  if (p.key_bytes == 4)
+    return p.off_bytes == 4 ? launch_join_pairs_heavy_t<uint32_t, uint32_t>(p, st)
+                            : launch_join_pairs_heavy_t<uint32_t, uint64_t>(p, st);
+  return p.off_bytes == 4 ? launch_join_pairs_heavy_t<uint64_t, uint32_t>(p, st)
+                          : launch_join_pairs_heavy_t<uint64_t, uint64_t>(p, st);
+}
+
 template <typename K, typename OFF>
 static hipError_t launch_join_t(const LaunchJoin &j, hipStream_t st) {
   constexpr int JB = 4096;
@@ -2819,7 +2933,8 @@ static hipError_t launch_join_t(const LaunchJoin &j, hipStream_t st) {
   const size_t lds = 2 * (size_t)j.key_cap * sizeof(K);
   hipLaunchKernelGGL((k_join<K, OFF, 256, JB>), dim3((uint32_t)nblk), dim3(256), lds, st, (const OFF *)j.offA,
                      (const K *)j.keysA, (const OFF *)j.offB, (const K *)j.keysB, j.M, (K *)j.out, j.capacity, j.cursor,
-                     j.key_cap, j.hlist, j.hlist_n, j.hlist_cap);
+                     j.key_cap, j.hlist, j.hlist_n, j.hlist_cap, j.indA, j.indB, j.pa, j.pb);
+  if (j.pa) return hipGetLastError();  // the row-id join answers its listed bins itself
   // the heavy bins (a grid that returns at once when k_join listed none)
   const uint32_t gh = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(j.hlist_cap, 1), 1024);
   hipLaunchKernelGGL((k_join_heavy<K, OFF, 1024>), dim3(gh), dim3(1024), 0, st, (const OFF *)j.offA, (const K *)j.keysA,

@@ -91,6 +91,25 @@ struct LaunchJoin {
   uint32_t *hlist;
   unsigned int *hlist_n;
   uint32_t hlist_cap;
+  // row-id join (pa != null): the entries' input positions, and the pairs
+  // (A position, B position) written instead of keys; listed bins are left
+  // to launch_join_pairs_heavy
+  const uint64_t *indA = nullptr, *indB = nullptr;
+  uint64_t *pa = nullptr, *pb = nullptr;
+};
+
+// The listed bins of a row-id join, compared pair by pair in tiles.
+struct LaunchJoinPairs {
+  int key_bytes, off_bytes;
+  const void *offA, *keysA, *offB, *keysB;
+  const uint64_t *indA, *indB;
+  const uint32_t *hlist;
+  uint32_t nh;
+  const unsigned long long *tile_base;  // exclusive scan of the bins' tiles (nh + 1)
+  uint64_t ntiles;
+  uint64_t *pa, *pb;
+  uint64_t capacity;
+  unsigned long long *cursor;
 };
 
 // Moves the run-length join's records and heavy bins of every chunk from
@@ -246,6 +265,10 @@ hipError_t launch_colscan(const LaunchCol &lc, hipStream_t st);
 hipError_t launch_final(const LaunchFinal &lf, hipStream_t st);
 hipError_t launch_probe(const LaunchProbe &p, hipStream_t st);
 hipError_t launch_join(const LaunchJoin &j, hipStream_t st);
+// tiles (1024 x 1024 entries) and key pairs (summed, each capped at 2^37) of listed bins
+hipError_t launch_pair_tiles(const void *offA, const void *offB, int off_bytes, const uint32_t *hlist, uint32_t n,
+                             unsigned long long *tiles, unsigned long long *pairs, hipStream_t st);
+hipError_t launch_join_pairs_heavy(const LaunchJoinPairs &p, hipStream_t st);
 // exclusive scan of n elements (total at out[n]); status: scan_status_bytes(n)
 hipError_t launch_scan(const void *in, void *out, uint64_t n, int elem_bytes, void *status, hipStream_t st);
 size_t scan_status_bytes(uint64_t n);

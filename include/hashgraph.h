@@ -188,6 +188,27 @@ hg_status hg_build_partitioned_indexed(hg_ctx ctx, const hg_table_desc *desc, co
                                        const int64_t *h_splits, uint32_t n_parts, void *d_offsets, void *d_csr_keys,
                                        uint64_t *d_csr_index, uint64_t *h_part_offsets);
 
+/* ---- row-id join of two index-carrying tables (INDEX_TRACK) --------------
+ * The join of MultiHashGraph::intersect (src/MultiHashGraph.cu:631-740) over
+ * tables whose entries carry their input positions (keyval_ind,
+ * include/MultiHashGraph.cuh:82-87; built by hg_build_indexed): for every
+ * matching pair (a in A_bin, x in B_bin, a == x), the pair of their input
+ * positions (d_pos_a[w], d_pos_b[w]), the index-pair output the reference
+ * wrote for its simpleIntersect (include/MultiHashGraphDeviceOperators.cuh:
+ * 535-537, commented out there).  Same count as hg_intersect; the pairs form
+ * the same multiset whatever the bins' order, grouped by bin ranges for bins
+ * up to 4096 key pairs.
+ *  d_indA, d_indB    the tables' d_csr_index (uint64, one per CSR key)
+ *  d_pos_a, d_pos_b  NULL for count only; else capacity uint64 each
+ * Bins above 4096 key pairs are compared pair by pair in 1024 x 1024 tiles
+ * over the grid; more than 2^36 such pairs in all is HG_ERR_UNSUPPORTED
+ * (*h_count still set).  HG_ERR_CAPACITY (with *h_count set) when
+ * capacity < count.                                                       */
+hg_status hg_intersect_indexed(hg_ctx ctx, const hg_table_desc *descA, const void *d_offA, const void *d_keysA,
+                               const uint64_t *d_indA, const hg_table_desc *descB, const void *d_offB,
+                               const void *d_keysB, const uint64_t *d_indB, uint64_t *d_pos_a, uint64_t *d_pos_b,
+                               uint64_t capacity, uint64_t *h_count);
+
 /* ---- staged tables: larger than HBM ----------------------------------------
  * Replace the reference's MANAGED_MEM mode (src/MultiHashGraph.cu:229-247,
  * include/MultiHashGraphHostOperators.cuh:255-280; intersect

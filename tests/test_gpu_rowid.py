@@ -1,0 +1,98 @@
+"""GPU parity of the row-id join (hg_intersect_indexed): the index-pair output
+of MultiHashGraph::intersect over INDEX_TRACK tables (keyval_ind entries,
+include/MultiHashGraph.cuh:82-87; pair output
+include/MultiHashGraphDeviceOperators.cuh:535-537).  The pairs must equal,
+as a set, {(i, j) : A[i] == B[j]} over the input positions: a closed form that
+no hash or bin layout enters, computed here by sorting."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+from hashgraph_amd import workloads as wl
+
+
+def _to_dev(a, dev):
+    import torch
+    return torch.from_numpy(a.view(np.int32 if a.dtype == np.uint32 else np.int64).copy()).to(dev)
+
+
+def _expected_pairs(A, B):
+    """Every (i, j) with A[i] == B[j], packed as i << 32 | j, sorted."""
+    ia, ib = np.argsort(A, kind="stable"), np.argsort(B, kind="stable")
+    va, sa, na = np.unique(A[ia], return_index=True, return_counts=True)
+    vb, sb, nb = np.unique(B[ib], return_index=True, return_counts=True)
+    _, xa, xb = np.intersect1d(va, vb, assume_unique=True, return_indices=True)
+    a0, ac, b0, bc = sa[xa], na[xa], sb[xb], nb[xb]
+    per = ac.astype(np.int64) * bc
+    total = int(per.sum())
+    if total == 0:
+        return np.empty(0, dtype=np.uint64)
+    k = np.repeat(np.arange(len(per)), per)
+    local = np.arange(total, dtype=np.int64) - np.repeat(np.cumsum(per) - per, per)
+    i = ia[a0[k] + local // bc[k]].astype(np.uint64)
+    j = ib[b0[k] + local % bc[k]].astype(np.uint64)
+    return np.sort((i << np.uint64(32)) | j)
+
+
+def _check(A, B, M, dev, ob=4):
+    import hashgraph_amd as hg
+    print(f"row-id join nA={len(A)} nB={len(B)} M={M} kb={A.itemsize} ob={ob}", flush=True)
+    oa, ka, ia = hg.build_indexed(_to_dev(A, dev), M, offset_bytes=ob)
+    ob_, kb, ib = hg.build_indexed(_to_dev(B, dev), M, offset_bytes=ob)
+    cnt, pa, pb = hg.intersect_indexed(oa, ka, ia, ob_, kb, ib, M)
+    exp = _expected_pairs(A, B)
+    assert cnt == len(exp)
+    got = np.sort((pa.cpu().numpy().astype(np.uint64) << np.uint64(32)) | pb.cpu().numpy().astype(np.uint64))
+    assert np.array_equal(got, exp)
+    return cnt
+
+
+def test_uniform_u32_light_bins(dev):
+    A = wl.uniform_keys(1, 1 << 18, 4)
+    B = wl.probe_keys(A, 2, 1 << 18, 0.25, 4)
+    assert _check(A, B, 1 << 18, dev) > 0
+
+
+@pytest.mark.parametrize("M", [64, 4])
+def test_dense_duplicates_take_the_tiled_path(M, dev):
+    # 2^16 keys a side over 2^12 values (~16 copies each): every bin holds
+    # ~2^16 / M keys a side, far above 4096 key pairs; with M = 4 a bin spans
+    # 16 x 16 tiles of 1024 x 1024 entries
+    A = (wl.uniform_keys(5, 1 << 16, 4) % np.uint32(1 << 12)).astype(np.uint32)
+    B = (wl.uniform_keys(6, 1 << 16, 4) % np.uint32(1 << 12)).astype(np.uint32)
+    assert _check(A, B, M, dev) > (1 << 19)
+
+
+def test_u64_keys_mixed_bins_and_offsets_u64(dev):
+    # mostly unique keys plus one hot key (3000 copies a side): light bins and
+    # one tiled bin, 8-byte offsets
+    A = wl.uniform_keys(7, 200000, 8)
+    B = wl.probe_keys(A, 8, 150000, 0.3, 8)
+    A[:3000] = np.uint64(12345678901)
+    B[-3000:] = np.uint64(12345678901)
+    _check(A, B, 1 << 17, dev, ob=8)
+
+
+def test_short_capacity_and_compare_limit(dev):
+    import hashgraph_amd as hg
+    A = wl.uniform_keys(9, 1 << 16, 4)
+    B = wl.probe_keys(A, 10, 1 << 16, 0.5, 4)
+    oa, ka, ia = hg.build_indexed(_to_dev(A, dev), 1 << 16)
+    ob, kb, ib = hg.build_indexed(_to_dev(B, dev), 1 << 16)
+    cnt, _, _ = hg.intersect_indexed(oa, ka, ia, ob, kb, ib, 1 << 16, emit=False)
+    assert cnt > 1
+    with pytest.raises(hg.HGError) as e:
+        hg.intersect_indexed(oa, ka, ia, ob, kb, ib, 1 << 16, capacity=cnt - 1)
+    assert e.value.status == 4
+    # one bin of 2^19 keys a side: 2^38 key pairs to compare, refused before
+    # any tile runs; the count is still answered
+    A = wl.uniform_keys(11, 1 << 19, 4)
+    B = wl.probe_keys(A, 12, 1 << 19, 0.5, 4)
+    oa, ka, ia = hg.build_indexed(_to_dev(A, dev), 1)
+    ob, kb, ib = hg.build_indexed(_to_dev(B, dev), 1)
+    cnt, _, _ = hg.intersect_indexed(oa, ka, ia, ob, kb, ib, 1, emit=False)
+    assert cnt == len(_expected_pairs(A, B))
+    with pytest.raises(hg.HGError) as e:
+        hg.intersect_indexed(oa, ka, ia, ob, kb, ib, 1)
+    assert e.value.status == 6
