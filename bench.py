@@ -11,6 +11,8 @@ top level, then one sub-result per other workload:
            16 logical partitions, B = 2^29, run-length join
   c5       4096 table/probe pairs (2^8..2^20 uint32 keys) in one captured plan
   learned  the learned hash beside murmur3 on 2^27 skewed uint64 sets
+  rowid    the C2 uint32 shape built with input positions (INDEX_TRACK) and
+           the row-id join writing every matching pair's positions
   staged   the C2 uint32 step with the tables in pinned host memory (the
            managed-memory replacement), bound by the host link
 Each carries a `roofline` (the dominant kernel's algorithmic bytes over its
@@ -664,6 +666,58 @@ def run_c5(dev, steps, warmup, T=4096, cpu=True, plan_only=False):
     return res
 
 
+def run_rowid(dev, steps, warmup):
+    """Row-id join at the C2 uint32 shape: A and B built with every entry's
+    input position (the INDEX_TRACK entries: 8-byte records through the
+    partition), then the input positions of every matching pair written."""
+    import torch
+    import hashgraph_amd as hg
+    from hashgraph_amd import workloads as wl
+    nA, nB, M, kb, frac = CONFIGS["c2-u32"]
+    log(f"rowid: nA={nA} nB={nB} M={M} indexed builds + row-id join")
+    A = wl.uniform_keys(1, nA, kb, device=dev)
+    B = wl.probe_keys(A, 2, nB, frac, kb, device=dev)
+    ta = hg.build_indexed(A, M)
+    tb = hg.build_indexed(B, M)
+    K, _, _ = hg.intersect_indexed(*ta, *tb, M, emit=False)
+    ctx = hg.context(0)
+
+    def step():
+        a = hg.build_indexed(A, M)
+        b = hg.build_indexed(B, M)
+        return hg.intersect_indexed(*a, *b, M, capacity=K)[0]
+
+    for _ in range(warmup):
+        assert step() == K
+    torch.cuda.synchronize()
+    ctx.profile(True)
+    ctx.profile_read(reset=True)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        step()
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    kt = ctx.profile_read(reset=True)
+    ctx.profile(False)
+    # records of 8 bytes go through the partition; the join reads both
+    # tables' keys, positions and offsets and writes 16 bytes per pair
+    bt = {"part_hist": 8 * (nA + nB) * steps, "part_scatter": 2 * 8 * (nA + nB) * steps,
+          "final": (2 * 8 * (nA + nB) + 4 * 2 * (M + 1)) * steps,
+          "join_pairs": ((kb + 8) * (nA + nB) + 8 * (M + 1) + 16 * K) * steps}
+    res = {
+        "value": round((nA + nB) / (ms / 1e3) / 1e9, 3), "unit": "Gkeys/s", "ms_per_step": round(ms, 3),
+        "config": {"workload": f"indexed builds of A and B ({nA} uint32 keys each, {M} bins, 25% overlap) and the "
+                               f"row-id join writing the input positions of all {K} matching pairs"},
+        "matches": K, "roofline": roofline_from(kt, bt, steps=steps), "kernels": kernel_table(kt, steps, ms, bt),
+    }
+    del A, B, ta, tb
+    torch.cuda.empty_cache()
+    return res
+
+
 def run_staged(dev, steps, warmup, budget_mib=256):
     """The C2 uint32 step with both tables in pinned host memory (the
     MANAGED_MEM replacement): A and B built through a device budget of
@@ -753,7 +807,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="all", choices=["all"] + sorted(CONFIGS) + ["c3", "c5", "learned", "staged"],
+    ap.add_argument("--config", default="all",
+                    choices=["all"] + sorted(CONFIGS) + ["c3", "c5", "learned", "staged", "rowid"],
                     help="all: the C2 u32 headline plus every other workload as sub-results")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
     ap.add_argument("--plan-only", action="store_true",
@@ -782,8 +837,10 @@ def main():
     cpu = not args.no_cpu
     common = {"metric": METRIC, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": True,
               "scaling": "weak", "vs_baseline": None}
-    if args.config in ("c3", "c5", "learned", "staged") or args.config in CONFIGS and args.config != "c2-u32":
-        if args.config == "staged":
+    if args.config in ("c3", "c5", "learned", "staged", "rowid") or args.config in CONFIGS and args.config != "c2-u32":
+        if args.config == "rowid":
+            r = {"rowid": run_rowid(dev, max(1, min(args.steps, 5)), min(args.warmup, 1))}
+        elif args.config == "staged":
             r = {"staged": run_staged(dev, max(1, min(args.steps, 3)), min(args.warmup, 1))}
         elif args.config == "c3":
             r = run_c3(dev, args.steps, args.warmup, cpu)
@@ -805,6 +862,8 @@ def main():
         res["learned_murmur_probe_ratio"] = {k: r for k, r in zip(("lognormal", "clusters"), ratios)}
         res["c3"] = run_c3(dev, sub_steps, sub_warm, cpu)
         res["c5"] = run_c5(dev, sub_steps, sub_warm, cpu=cpu)
+        log("row-id join")
+        res["rowid"] = run_rowid(dev, sub_steps, sub_warm)
         log("staged tables (host memory)")
         res["staged"] = run_staged(dev, max(1, min(args.steps, 3)), 1)
     print(json.dumps(res))

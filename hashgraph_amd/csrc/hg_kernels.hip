@@ -28,6 +28,9 @@ constexpr uint32_t RANGE_LEAVES = 512;
 // A join bin pair with more key pairs than this is answered from a value
 // table (k_join_heavy) instead of the nested loop.
 constexpr uint64_t JOIN_PAIR_LIMIT = 4096;
+// bins per k_join workgroup: with their offsets and keys (78 KB of LDS) two
+// 1024-thread workgroups fit a CU
+constexpr int JOIN_BINS = 2048;
 
 // Keys per thread that the final stage and the probe keep in VGPRs (u32
 // keys; u64 keys use 32).  1024 threads x KPT is the bucket capacity.
@@ -2928,10 +2931,10 @@ hipError_t launch_join_pairs_heavy(const LaunchJoinPairs &p, hipStream_t st) {
 
 template <typename K, typename OFF>
 static hipError_t launch_join_t(const LaunchJoin &j, hipStream_t st) {
-  constexpr int JB = 4096;
+  constexpr int JB = JOIN_BINS;
   const uint64_t nblk = (j.M + JB - 1) / JB;
   const size_t lds = 2 * (size_t)j.key_cap * sizeof(K);
-  hipLaunchKernelGGL((k_join<K, OFF, 256, JB>), dim3((uint32_t)nblk), dim3(256), lds, st, (const OFF *)j.offA,
+  hipLaunchKernelGGL((k_join<K, OFF, 1024, JB>), dim3((uint32_t)nblk), dim3(1024), lds, st, (const OFF *)j.offA,
                      (const K *)j.keysA, (const OFF *)j.offB, (const K *)j.keysB, j.M, (K *)j.out, j.capacity, j.cursor,
                      j.key_cap, j.hlist, j.hlist_n, j.hlist_cap, j.indA, j.indB, j.pa, j.pb);
   if (j.pa) return hipGetLastError();  // the row-id join answers its listed bins itself
@@ -2952,7 +2955,7 @@ hipError_t launch_join(const LaunchJoin &j, hipStream_t st) {
 
 uint32_t join_key_cap(int key_bytes, int off_bytes) {
   // two key arrays next to the 2*(JB+1) offsets held in static LDS
-  const size_t budget = 150 * 1024 - 2 * (4096 + 1) * (size_t)off_bytes;
+  const size_t budget = 78 * 1024 - 2 * (JOIN_BINS + 1) * (size_t)off_bytes;
   return (uint32_t)(budget / (2 * (size_t)key_bytes));
 }
 
@@ -3207,10 +3210,10 @@ hipError_t set_kernel_attributes() {
       if (r == hipSuccess) r = set_lds(&k_probe<uint64_t, uint64_t, 1024, 32, probe_sub<uint64_t>(), H>);
       return r;
     }));
-  HG_SET(&k_join<uint32_t, uint32_t, 256, 4096>);
-  HG_SET(&k_join<uint32_t, uint64_t, 256, 4096>);
-  HG_SET(&k_join<uint64_t, uint32_t, 256, 4096>);
-  HG_SET(&k_join<uint64_t, uint64_t, 256, 4096>);
+  HG_SET(&k_join<uint32_t, uint32_t, 1024, JOIN_BINS>);
+  HG_SET(&k_join<uint32_t, uint64_t, 1024, JOIN_BINS>);
+  HG_SET(&k_join<uint64_t, uint32_t, 1024, JOIN_BINS>);
+  HG_SET(&k_join<uint64_t, uint64_t, 1024, JOIN_BINS>);
   HG_SET(&k_range_hist<uint32_t>);
   HG_SET(&k_range_hist<uint64_t>);
   HG_SET_D(set_rmi_kernel_attributes());
