@@ -1476,12 +1476,17 @@ hg_status hg_intersect_indexed(hg_ctx c, const hg_table_desc *descA, const void 
   if ((d_pos_a == nullptr) != (d_pos_b == nullptr)) return fail(HG_ERR_INVALID_ARG, "give both d_pos_a and d_pos_b");
   if (d_pos_a && ((descA && descA->n_keys && !d_indA) || (descB && descB->n_keys && !d_indB)))
     return fail(HG_ERR_INVALID_ARG, "null input positions");
-  // the count is the key join's
-  HG_TRY(hg_intersect(c, descA, d_offA, d_keysA, descB, d_offB, d_keysB, nullptr, 0, h_count));
-  if (!d_pos_a) return HG_OK;
-  const uint64_t count = *h_count;
-  if (count > capacity) return fail(HG_ERR_CAPACITY, "output capacity smaller than the match count");
-  if (count == 0) return HG_OK;
+  // count only: the key join's count.  With an output, the pair pass counts
+  // as it writes (places past capacity are dropped), and its cursor is the count.
+  if (!d_pos_a) return hg_intersect(c, descA, d_offA, d_keysA, descB, d_offB, d_keysB, nullptr, 0, h_count);
+  HG_TRY(check_desc(descA));
+  HG_TRY(check_desc(descB));
+  if (descA->n_bins != descB->n_bins || descA->key_bytes != descB->key_bytes || descA->hash_kind != descB->hash_kind ||
+      descA->offset_bytes != descB->offset_bytes)
+    return fail(HG_ERR_MISMATCH, "tables do not share a bin space");
+  if (!d_offA || !d_offB || (descA->n_keys && !d_keysA) || (descB->n_keys && !d_keysB))
+    return fail(HG_ERR_INVALID_ARG, "null buffer");
+  *h_count = 0;
   DeviceGuard g(c->device);
   const uint64_t hcap = std::min<uint64_t>(descA->n_bins, (descA->n_keys + descB->n_keys) / 128 + 1);
   Plan P;
@@ -1526,10 +1531,12 @@ hg_status hg_intersect_indexed(hg_ctx c, const hg_table_desc *descA, const void 
     unsigned long long pairs = 0, ntiles = 0;
     HG_TRY(sync_read(c, &pairs, small + 2, 8));
     HG_TRY(sync_read(c, &ntiles, tbase + nh, 8));
-    if (pairs > kPairCompareLimit)
+    if (pairs > kPairCompareLimit) {
+      HG_TRY(hg_intersect(c, descA, d_offA, d_keysA, descB, d_offB, d_keysB, nullptr, 0, h_count));
       return fail(HG_ERR_UNSUPPORTED, "bins with more than 4096 key pairs hold " + std::to_string(pairs) +
                                           " key pairs together; the row-id join compares them one by one and "
                                           "takes at most 2^36");
+    }
     LaunchJoinPairs p{};
     p.key_bytes = j.key_bytes;
     p.off_bytes = j.off_bytes;
@@ -1550,9 +1557,10 @@ hg_status hg_intersect_indexed(hg_ctx c, const hg_table_desc *descA, const void 
     Prof pf(c, "join_pairs_heavy");
     HG_HIP(launch_join_pairs_heavy(p, c->stream));
   }
-  unsigned long long written = 0;
-  HG_TRY(sync_read(c, &written, small, 8));
-  if (written != count) return fail(HG_ERR_DEVICE, "row-id join wrote a different number of pairs than it counted");
+  unsigned long long total = 0;
+  HG_TRY(sync_read(c, &total, small, 8));
+  *h_count = total;
+  if (total > capacity) return fail(HG_ERR_CAPACITY, "output capacity smaller than the match count");
   return HG_OK;
 }
 
