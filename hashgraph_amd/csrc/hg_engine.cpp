@@ -1501,7 +1501,7 @@ hg_status hg_intersect_indexed(hg_ctx c, const hg_table_desc *descA, const void 
   j.key_bytes = (int)descA->key_bytes;
   j.off_bytes = (int)descA->offset_bytes;
   j.M = descA->n_bins;
-  j.key_cap = join_key_cap(j.key_bytes, j.off_bytes);
+  j.key_cap = join_key_cap(j.key_bytes, j.off_bytes, true);
   j.offA = d_offA;
   j.keysA = d_keysA;
   j.offB = d_offB;
@@ -1804,10 +1804,14 @@ hg_status hg_build_indexed(hg_ctx c, const hg_table_desc *desc, const void *d_ke
   if (n == 0) return fill_offsets(c, d_offsets, (int)desc->offset_bytes, desc->n_bins + 1, 0);
   uint64_t *rin = nullptr, *rout = nullptr;
   HG_TRY(index_records(c, n, rin, rout));
-  HG_HIP(launch_pack_index(d_keys, n, (int)desc->key_bytes, rin, c->stream));
+  {
+    Prof pf(c, "pack_index");
+    HG_HIP(launch_pack_index(d_keys, n, (int)desc->key_bytes, rin, c->stream));
+  }
   hg_table_desc d8 = *desc;
   d8.key_bytes = 8;
   HG_TRY(build_table(c, &d8, rin, d_offsets, rout));
+  Prof pf(c, "unpack_index");
   HG_HIP(launch_unpack_index(rout, n, d_keys, (int)desc->key_bytes, d_csr_keys, d_csr_index, c->stream));
   return HG_OK;
 }
@@ -1823,10 +1827,14 @@ hg_status hg_build_partitioned_indexed(hg_ctx c, const hg_table_desc *desc, cons
   const uint64_t n = desc->n_keys;
   uint64_t *rin = nullptr, *rout = nullptr;
   HG_TRY(index_records(c, std::max<uint64_t>(n, 1), rin, rout));
-  HG_HIP(launch_pack_index(d_keys, n, (int)desc->key_bytes, rin, c->stream));
+  {
+    Prof pf(c, "pack_index");
+    HG_HIP(launch_pack_index(d_keys, n, (int)desc->key_bytes, rin, c->stream));
+  }
   hg_table_desc d8 = *desc;
   d8.key_bytes = 8;
   HG_TRY(build_partitioned(c, &d8, rin, h_splits, n_parts, d_offsets, rout, h_part_offsets));
+  Prof pf(c, "unpack_index");
   HG_HIP(launch_unpack_index(rout, h_part_offsets[n_parts], d_keys, (int)desc->key_bytes, d_csr_keys, d_csr_index,
                              c->stream));
   return HG_OK;

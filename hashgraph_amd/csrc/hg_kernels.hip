@@ -31,6 +31,9 @@ constexpr uint64_t JOIN_PAIR_LIMIT = 4096;
 // bins per k_join workgroup: with their offsets and keys (78 KB of LDS) two
 // 1024-thread workgroups fit a CU
 constexpr int JOIN_BINS = 2048;
+// matches a row-id join workgroup buffers in LDS (a pair of 16-bit LDS slots
+// each) to write them out contiguously; a range with more re-walks its bins
+constexpr uint32_t JOIN_EMIT_BUF = 1024;
 
 // Keys per thread that the final stage and the probe keep in VGPRs (u32
 // keys; u64 keys use 32).  1024 threads x KPT is the bucket capacity.
@@ -748,23 +751,76 @@ __global__ __launch_bounds__(BLOCK) void k_join(const OFF *__restrict__ offA, co
   extern __shared__ __align__(16) uint8_t smem[];
   __shared__ OFF s_oa[JB + 1], s_ob[JB + 1];
   __shared__ unsigned long long s_base;
-  K *s_a = (K *)smem;
+  // the row-id join stages the entries' input positions beside their keys, so
+  // that its emitting walk reads LDS only and its stores wait on no load
+  const uint32_t icap = pa ? key_cap : 0u;
+  uint64_t *s_ia = (uint64_t *)smem, *s_ib = s_ia + icap;
+  uint32_t *s_m = (uint32_t *)(s_ib + icap);
+  K *s_a = (K *)(s_m + (pa ? JOIN_EMIT_BUF : 0u));
   K *s_b = s_a + key_cap;
+  __shared__ uint32_t s_mn;
   const uint64_t b0 = (uint64_t)blockIdx.x * JB;
   const uint32_t n = (uint32_t)min((uint64_t)JB, M - b0);
-  for (uint32_t q = threadIdx.x; q <= n; q += BLOCK) {
-    s_oa[q] = offA[b0 + q];
-    s_ob[q] = offB[b0 + q];
+  if (threadIdx.x == 0) s_mn = 0;
+  // staging: every thread issues all its loads of a round before its LDS
+  // stores, so a workgroup keeps U * BLOCK loads per array in flight
+  constexpr uint32_t U = (JB + BLOCK) / BLOCK;
+  {
+    OFF va[U], vb[U];
+#pragma unroll
+    for (uint32_t u = 0; u < U; u++) {
+      const uint32_t q = u * BLOCK + threadIdx.x;
+      if (q <= n) {
+        va[u] = offA[b0 + q];
+        vb[u] = offB[b0 + q];
+      }
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < U; u++) {
+      const uint32_t q = u * BLOCK + threadIdx.x;
+      if (q <= n) {
+        s_oa[q] = va[u];
+        s_ob[q] = vb[u];
+      }
+    }
   }
   __syncthreads();
   const OFF a0 = s_oa[0], x0 = s_ob[0];
   const uint64_t na = (uint64_t)(s_oa[n] - a0), nbk = (uint64_t)(s_ob[n] - x0);
   const bool a_lds = na <= key_cap, b_lds = nbk <= key_cap;
-  if (a_lds)
-    for (uint32_t q = threadIdx.x; q < na; q += BLOCK) s_a[q] = keysA[a0 + q];
-  if (b_lds)
-    for (uint32_t q = threadIdx.x; q < nbk; q += BLOCK) s_b[q] = keysB[x0 + q];
+  const uint32_t sa = a_lds ? (uint32_t)na : 0u, sb = b_lds ? (uint32_t)nbk : 0u;
+  for (uint32_t r0 = 0; r0 < max(sa, sb); r0 += U * BLOCK) {
+    K ka[U], kx[U];
+    uint64_t ia_[U], ix_[U];
+#pragma unroll
+    for (uint32_t u = 0; u < U; u++) {
+      const uint32_t q = r0 + u * BLOCK + threadIdx.x;
+      if (q < sa) {
+        ka[u] = keysA[a0 + q];
+        if (pa) ia_[u] = indA[a0 + q];
+      }
+      if (q < sb) {
+        kx[u] = keysB[x0 + q];
+        if (pa) ix_[u] = indB[x0 + q];
+      }
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < U; u++) {
+      const uint32_t q = r0 + u * BLOCK + threadIdx.x;
+      if (q < sa) {
+        s_a[q] = ka[u];
+        if (pa) s_ia[q] = ia_[u];
+      }
+      if (q < sb) {
+        s_b[q] = kx[u];
+        if (pa) s_ib[q] = ix_[u];
+      }
+    }
+  }
   __syncthreads();
+  // row-id join with both sides in LDS: every match appends its two LDS slots
+  // (below key_cap < 2^16) to s_m during the counting walk
+  const bool buffered = pa != nullptr && a_lds && b_lds;
   uint64_t mine = 0;
   for (uint32_t l = threadIdx.x; l < n; l += BLOCK) {
     const uint64_t ea = (uint64_t)(s_oa[l + 1] - a0), eb = (uint64_t)(s_ob[l + 1] - x0);
@@ -773,9 +829,38 @@ __global__ __launch_bounds__(BLOCK) void k_join(const OFF *__restrict__ offA, co
       if (i < hlist_cap) hlist[i] = (uint32_t)(b0 + l);
       continue;
     }
+    if (buffered) {
+      for (uint32_t ia = (uint32_t)(s_oa[l] - a0); ia < (uint32_t)ea; ia++) {
+        const K a = s_a[ia];
+        for (uint32_t ib = (uint32_t)(s_ob[l] - x0); ib < (uint32_t)eb; ib++)
+          if (s_b[ib] == a) {
+            mine++;
+            const uint32_t slot = atomicAdd(&s_mn, 1u);
+            if (slot < JOIN_EMIT_BUF) s_m[slot] = (ia << 16) | ib;
+          }
+      }
+      continue;
+    }
     for (uint64_t ia = (uint64_t)(s_oa[l] - a0); ia < ea; ia++) {
       const K a = a_lds ? s_a[ia] : keysA[a0 + ia];
       for (uint64_t ib = (uint64_t)(s_ob[l] - x0); ib < eb; ib++) mine += (b_lds ? s_b[ib] : keysB[x0 + ib]) == a;
+    }
+  }
+  if (buffered) {
+    __syncthreads();
+    const uint32_t nm = s_mn;
+    if (nm <= JOIN_EMIT_BUF) {  // one reservation, then contiguous stores
+      if (threadIdx.x == 0) s_base = nm ? atomicAdd(cursor, (unsigned long long)nm) : 0ull;
+      __syncthreads();
+      for (uint32_t q = threadIdx.x; q < nm; q += BLOCK) {
+        const unsigned long long w = s_base + q;
+        if (w < capacity) {
+          const uint32_t v = s_m[q];
+          pa[w] = s_ia[v >> 16];
+          pb[w] = s_ib[v & 0xFFFFu];
+        }
+      }
+      return;
     }
   }
   if (out == nullptr && pa == nullptr) {
@@ -801,8 +886,8 @@ __global__ __launch_bounds__(BLOCK) void k_join(const OFF *__restrict__ offA, co
         if (x == a) {
           if (w < capacity) {
             if (pa) {  // row-id join: the two entries' input positions
-              pa[w] = indA[a0 + ia];
-              pb[w] = indB[x0 + ib];
+              pa[w] = a_lds ? s_ia[ia] : indA[a0 + ia];
+              pb[w] = b_lds ? s_ib[ib] : indB[x0 + ib];
             } else {
               out[w] = x;
             }
@@ -2933,7 +3018,7 @@ template <typename K, typename OFF>
 static hipError_t launch_join_t(const LaunchJoin &j, hipStream_t st) {
   constexpr int JB = JOIN_BINS;
   const uint64_t nblk = (j.M + JB - 1) / JB;
-  const size_t lds = 2 * (size_t)j.key_cap * sizeof(K);
+  const size_t lds = 2 * (size_t)j.key_cap * (sizeof(K) + (j.pa ? 8 : 0)) + (j.pa ? JOIN_EMIT_BUF * 4 : 0);
   hipLaunchKernelGGL((k_join<K, OFF, 1024, JB>), dim3((uint32_t)nblk), dim3(1024), lds, st, (const OFF *)j.offA,
                      (const K *)j.keysA, (const OFF *)j.offB, (const K *)j.keysB, j.M, (K *)j.out, j.capacity, j.cursor,
                      j.key_cap, j.hlist, j.hlist_n, j.hlist_cap, j.indA, j.indB, j.pa, j.pb);
@@ -2953,10 +3038,12 @@ hipError_t launch_join(const LaunchJoin &j, hipStream_t st) {
   return j.off_bytes == 4 ? launch_join_t<uint64_t, uint32_t>(j, st) : launch_join_t<uint64_t, uint64_t>(j, st);
 }
 
-uint32_t join_key_cap(int key_bytes, int off_bytes) {
-  // two key arrays next to the 2*(JB+1) offsets held in static LDS
-  const size_t budget = 78 * 1024 - 2 * (JOIN_BINS + 1) * (size_t)off_bytes;
-  return (uint32_t)(budget / (2 * (size_t)key_bytes));
+uint32_t join_key_cap(int key_bytes, int off_bytes, bool with_index) {
+  // two key arrays (for the row-id join with their 8-byte positions and the
+  // match buffer) next to the 2*(JB+1) offsets held in static LDS; the row-id
+  // join's cap stays below 2^16 (its buffered slots are 16-bit)
+  const size_t budget = 78 * 1024 - 2 * (JOIN_BINS + 1) * (size_t)off_bytes - (with_index ? JOIN_EMIT_BUF * 4 : 0);
+  return (uint32_t)(budget / (2 * ((size_t)key_bytes + (with_index ? 8 : 0))));
 }
 
 // Status area of a scan over n elements: flags (u32 per tile), aggregates
@@ -3241,7 +3328,8 @@ hipError_t launch_set_kinds(const uint32_t *use, SegDesc *segs, uint32_t nsegs, 
 
 // Records of the index-carrying builds: the key's input position above the
 // low 32 bits of the key (murmur3 bins depend on those bits only), and their
-// unpacking into CSR keys (gathered from the input by position) and positions.
+// unpacking into CSR keys (a 32-bit key from the record, a 64-bit one gathered
+// from the input by position) and positions.
 template <typename K>
 __global__ void k_pack_index(const K *__restrict__ keys, uint64_t n, uint64_t *__restrict__ rec) {
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
@@ -3251,9 +3339,13 @@ template <typename K>
 __global__ void k_unpack_index(const uint64_t *__restrict__ rec, uint64_t n, const K *__restrict__ keys,
                                K *__restrict__ csr_keys, uint64_t *__restrict__ csr_index) {
   for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < n; j += (uint64_t)gridDim.x * blockDim.x) {
-    const uint64_t i = rec[j] >> 32;
+    const uint64_t r = rec[j], i = r >> 32;
     csr_index[j] = i;
-    csr_keys[j] = keys[i];
+    // a 32-bit key is whole in the record's low half; a 64-bit one is read back
+    if constexpr (sizeof(K) == 4)
+      csr_keys[j] = (K)r;
+    else
+      csr_keys[j] = keys[i];
   }
 }
 

@@ -48,10 +48,14 @@ def _check(A, B, M, dev, ob=4):
     return cnt
 
 
-def test_uniform_u32_light_bins(dev):
+@pytest.mark.parametrize("frac", [0.25, 0.5, 1.0])
+def test_uniform_u32_light_bins(frac, dev):
+    # one key a bin on average: a workgroup's 2048 bins hold ~2048 * frac
+    # matches, so 0.5 puts ranges either side of the 1024 matches a workgroup
+    # buffers and 1.0 sends every range through the re-walk
     A = wl.uniform_keys(1, 1 << 18, 4)
-    B = wl.probe_keys(A, 2, 1 << 18, 0.25, 4)
-    assert _check(A, B, 1 << 18, dev) > 0
+    B = wl.probe_keys(A, 2, 1 << 18, frac, 4)
+    assert _check(A, B, 1 << 18, dev) > (1 << 18) * frac * 0.9
 
 
 @pytest.mark.parametrize("M", [64, 4])
