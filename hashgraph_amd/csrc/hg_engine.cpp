@@ -1444,7 +1444,7 @@ hg_status hg_intersect(hg_ctx c, const hg_table_desc *descA, const void *d_offA,
   j.key_bytes = (int)descA->key_bytes;
   j.off_bytes = (int)descA->offset_bytes;
   j.M = descA->n_bins;
-  j.key_cap = join_key_cap(j.key_bytes, j.off_bytes);
+  j.key_cap = join_key_cap(j.key_bytes, j.off_bytes, d_out != nullptr);
   j.offA = d_offA;
   j.keysA = d_keysA;
   j.offB = d_offB;
@@ -1501,7 +1501,7 @@ hg_status hg_intersect_indexed(hg_ctx c, const hg_table_desc *descA, const void 
   j.key_bytes = (int)descA->key_bytes;
   j.off_bytes = (int)descA->offset_bytes;
   j.M = descA->n_bins;
-  j.key_cap = join_key_cap(j.key_bytes, j.off_bytes, true);
+  j.key_cap = join_key_cap(j.key_bytes, j.off_bytes, true, true);
   j.offA = d_offA;
   j.keysA = d_keysA;
   j.offB = d_offB;

@@ -34,6 +34,9 @@ constexpr int JOIN_BINS = 2048;
 // matches a row-id join workgroup buffers in LDS (a pair of 16-bit LDS slots
 // each) to write them out contiguously; a range with more re-walks its bins
 constexpr uint32_t JOIN_EMIT_BUF = 1024;
+// what a k_join instance does with its matches: count them, write the matched
+// keys, or write the pairs of input positions (row-id join)
+constexpr int JOIN_COUNT = 0, JOIN_KEYS = 1, JOIN_PAIRS = 2;
 
 // Keys per thread that the final stage and the probe keep in VGPRs (u32
 // keys; u64 keys use 32).  1024 threads x KPT is the bucket capacity.
@@ -739,7 +742,7 @@ __global__ __launch_bounds__(BLOCK) void k_probe(const K *__restrict__ bpart, co
 // per JB bins; A's and B's keys of the range are staged in LDS when they fit.
 // A bin pair above JOIN_PAIR_LIMIT key pairs is listed for k_join_heavy.
 // ---------------------------------------------------------------------------
-template <typename K, typename OFF, int BLOCK, int JB>
+template <typename K, typename OFF, int BLOCK, int JB, int MODE>
 __global__ __launch_bounds__(BLOCK) void k_join(const OFF *__restrict__ offA, const K *__restrict__ keysA,
                                                 const OFF *__restrict__ offB, const K *__restrict__ keysB, uint64_t M,
                                                 K *__restrict__ out, uint64_t capacity,
@@ -751,12 +754,14 @@ __global__ __launch_bounds__(BLOCK) void k_join(const OFF *__restrict__ offA, co
   extern __shared__ __align__(16) uint8_t smem[];
   __shared__ OFF s_oa[JB + 1], s_ob[JB + 1];
   __shared__ unsigned long long s_base;
-  // the row-id join stages the entries' input positions beside their keys, so
-  // that its emitting walk reads LDS only and its stores wait on no load
-  const uint32_t icap = pa ? key_cap : 0u;
+  // the row-id join stages the entries' input positions beside their keys,
+  // and an emitting join buffers its matches (s_m), so that the output is
+  // written from LDS in one contiguous run per workgroup
+  constexpr bool emitting = MODE != JOIN_COUNT, pairs = MODE == JOIN_PAIRS;
+  const uint32_t icap = pairs ? key_cap : 0u;
   uint64_t *s_ia = (uint64_t *)smem, *s_ib = s_ia + icap;
   uint32_t *s_m = (uint32_t *)(s_ib + icap);
-  K *s_a = (K *)(s_m + (pa ? JOIN_EMIT_BUF : 0u));
+  K *s_a = (K *)(s_m + (emitting ? JOIN_EMIT_BUF : 0u));
   K *s_b = s_a + key_cap;
   __shared__ uint32_t s_mn;
   const uint64_t b0 = (uint64_t)blockIdx.x * JB;
@@ -797,11 +802,11 @@ __global__ __launch_bounds__(BLOCK) void k_join(const OFF *__restrict__ offA, co
       const uint32_t q = r0 + u * BLOCK + threadIdx.x;
       if (q < sa) {
         ka[u] = keysA[a0 + q];
-        if (pa) ia_[u] = indA[a0 + q];
+        if (pairs) ia_[u] = indA[a0 + q];
       }
       if (q < sb) {
         kx[u] = keysB[x0 + q];
-        if (pa) ix_[u] = indB[x0 + q];
+        if (pairs) ix_[u] = indB[x0 + q];
       }
     }
 #pragma unroll
@@ -809,18 +814,18 @@ __global__ __launch_bounds__(BLOCK) void k_join(const OFF *__restrict__ offA, co
       const uint32_t q = r0 + u * BLOCK + threadIdx.x;
       if (q < sa) {
         s_a[q] = ka[u];
-        if (pa) s_ia[q] = ia_[u];
+        if (pairs) s_ia[q] = ia_[u];
       }
       if (q < sb) {
         s_b[q] = kx[u];
-        if (pa) s_ib[q] = ix_[u];
+        if (pairs) s_ib[q] = ix_[u];
       }
     }
   }
   __syncthreads();
-  // row-id join with both sides in LDS: every match appends its two LDS slots
-  // (below key_cap < 2^16) to s_m during the counting walk
-  const bool buffered = pa != nullptr && a_lds && b_lds;
+  // an emitting join with both sides in LDS: every match appends its two LDS
+  // slots (below key_cap < 2^16) to s_m during the counting walk
+  const bool buffered = emitting && a_lds && b_lds;
   uint64_t mine = 0;
   for (uint32_t l = threadIdx.x; l < n; l += BLOCK) {
     const uint64_t ea = (uint64_t)(s_oa[l + 1] - a0), eb = (uint64_t)(s_ob[l + 1] - x0);
@@ -856,14 +861,18 @@ __global__ __launch_bounds__(BLOCK) void k_join(const OFF *__restrict__ offA, co
         const unsigned long long w = s_base + q;
         if (w < capacity) {
           const uint32_t v = s_m[q];
-          pa[w] = s_ia[v >> 16];
-          pb[w] = s_ib[v & 0xFFFFu];
+          if (pairs) {
+            pa[w] = s_ia[v >> 16];
+            pb[w] = s_ib[v & 0xFFFFu];
+          } else {
+            out[w] = s_b[v & 0xFFFFu];
+          }
         }
       }
       return;
     }
   }
-  if (out == nullptr && pa == nullptr) {
+  if (!emitting) {
     __shared__ unsigned long long s_t64[BLOCK / 64 + 1];
     unsigned long long t = block_sum<BLOCK, unsigned long long>(mine, s_t64);
     if (threadIdx.x == 0 && t) atomicAdd(cursor, t);
@@ -885,7 +894,7 @@ __global__ __launch_bounds__(BLOCK) void k_join(const OFF *__restrict__ offA, co
         const K x = b_lds ? s_b[ib] : keysB[x0 + ib];
         if (x == a) {
           if (w < capacity) {
-            if (pa) {  // row-id join: the two entries' input positions
+            if (pairs) {  // row-id join: the two entries' input positions
               pa[w] = a_lds ? s_ia[ia] : indA[a0 + ia];
               pb[w] = b_lds ? s_ib[ib] : indB[x0 + ib];
             } else {
@@ -3018,10 +3027,13 @@ template <typename K, typename OFF>
 static hipError_t launch_join_t(const LaunchJoin &j, hipStream_t st) {
   constexpr int JB = JOIN_BINS;
   const uint64_t nblk = (j.M + JB - 1) / JB;
-  const size_t lds = 2 * (size_t)j.key_cap * (sizeof(K) + (j.pa ? 8 : 0)) + (j.pa ? JOIN_EMIT_BUF * 4 : 0);
-  hipLaunchKernelGGL((k_join<K, OFF, 1024, JB>), dim3((uint32_t)nblk), dim3(1024), lds, st, (const OFF *)j.offA,
-                     (const K *)j.keysA, (const OFF *)j.offB, (const K *)j.keysB, j.M, (K *)j.out, j.capacity, j.cursor,
-                     j.key_cap, j.hlist, j.hlist_n, j.hlist_cap, j.indA, j.indB, j.pa, j.pb);
+  const size_t lds =
+      2 * (size_t)j.key_cap * (sizeof(K) + (j.pa ? 8 : 0)) + ((j.out || j.pa) ? JOIN_EMIT_BUF * 4 : 0);
+  auto kern = j.pa ? &k_join<K, OFF, 1024, JB, JOIN_PAIRS>
+                  : j.out ? &k_join<K, OFF, 1024, JB, JOIN_KEYS> : &k_join<K, OFF, 1024, JB, JOIN_COUNT>;
+  hipLaunchKernelGGL(kern, dim3((uint32_t)nblk), dim3(1024), lds, st, (const OFF *)j.offA, (const K *)j.keysA,
+                     (const OFF *)j.offB, (const K *)j.keysB, j.M, (K *)j.out, j.capacity, j.cursor, j.key_cap, j.hlist,
+                     j.hlist_n, j.hlist_cap, j.indA, j.indB, j.pa, j.pb);
   if (j.pa) return hipGetLastError();  // the row-id join answers its listed bins itself
   // the heavy bins (a grid that returns at once when k_join listed none)
   const uint32_t gh = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(j.hlist_cap, 1), 1024);
@@ -3038,11 +3050,12 @@ hipError_t launch_join(const LaunchJoin &j, hipStream_t st) {
   return j.off_bytes == 4 ? launch_join_t<uint64_t, uint32_t>(j, st) : launch_join_t<uint64_t, uint64_t>(j, st);
 }
 
-uint32_t join_key_cap(int key_bytes, int off_bytes, bool with_index) {
-  // two key arrays (for the row-id join with their 8-byte positions and the
-  // match buffer) next to the 2*(JB+1) offsets held in static LDS; the row-id
-  // join's cap stays below 2^16 (its buffered slots are 16-bit)
-  const size_t budget = 78 * 1024 - 2 * (JOIN_BINS + 1) * (size_t)off_bytes - (with_index ? JOIN_EMIT_BUF * 4 : 0);
+uint32_t join_key_cap(int key_bytes, int off_bytes, bool emit, bool with_index) {
+  // two key arrays (for the row-id join with their 8-byte positions; for an
+  // emitting join beside the match buffer) next to the 2*(JB+1) offsets held
+  // in static LDS; the cap stays below 2^16 (buffered slots are 16-bit)
+  const size_t budget =
+      78 * 1024 - 2 * (JOIN_BINS + 1) * (size_t)off_bytes - ((emit || with_index) ? JOIN_EMIT_BUF * 4 : 0);
   return (uint32_t)(budget / (2 * ((size_t)key_bytes + (with_index ? 8 : 0))));
 }
 
@@ -3265,6 +3278,16 @@ static hipError_t set_lds(F *fn) {
   return hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, room);
 }
 
+
+template <int MODE>
+static hipError_t join_set_lds() {
+  hipError_t r = set_lds(&k_join<uint32_t, uint32_t, 1024, JOIN_BINS, MODE>);
+  if (r == hipSuccess) r = set_lds(&k_join<uint32_t, uint64_t, 1024, JOIN_BINS, MODE>);
+  if (r == hipSuccess) r = set_lds(&k_join<uint64_t, uint32_t, 1024, JOIN_BINS, MODE>);
+  if (r == hipSuccess) r = set_lds(&k_join<uint64_t, uint64_t, 1024, JOIN_BINS, MODE>);
+  return r;
+}
+
 hipError_t set_kernel_attributes() {
   hipError_t e = hipSuccess;
 #define HG_SET(...) do { hipError_t e_ = set_lds(__VA_ARGS__); if (e_ != hipSuccess) e = e_; } while (0)
@@ -3297,10 +3320,9 @@ hipError_t set_kernel_attributes() {
       if (r == hipSuccess) r = set_lds(&k_probe<uint64_t, uint64_t, 1024, 32, probe_sub<uint64_t>(), H>);
       return r;
     }));
-  HG_SET(&k_join<uint32_t, uint32_t, 1024, JOIN_BINS>);
-  HG_SET(&k_join<uint32_t, uint64_t, 1024, JOIN_BINS>);
-  HG_SET(&k_join<uint64_t, uint32_t, 1024, JOIN_BINS>);
-  HG_SET(&k_join<uint64_t, uint64_t, 1024, JOIN_BINS>);
+  HG_SET_D(join_set_lds<JOIN_COUNT>());
+  HG_SET_D(join_set_lds<JOIN_KEYS>());
+  HG_SET_D(join_set_lds<JOIN_PAIRS>());
   HG_SET(&k_range_hist<uint32_t>);
   HG_SET(&k_range_hist<uint64_t>);
   HG_SET_D(set_rmi_kernel_attributes());

@@ -300,7 +300,7 @@ size_t final_big_ctl_bytes();
 // has max_cols columns (0: the columns do not leave room for staging)
 uint32_t part_stage_cap(uint32_t max_cols, int key_bytes, int sk);
 uint32_t probe_akey_cap(int key_bytes, int off_bytes);
-uint32_t join_key_cap(int key_bytes, int off_bytes, bool with_index = false);
+uint32_t join_key_cap(int key_bytes, int off_bytes, bool emit, bool with_index = false);
 uint64_t scan_tiles(uint64_t n);
 
 }  // namespace hg

@@ -72,6 +72,15 @@ def test_probe_shapes(nA, nB, M, kb, ob, dev):
     _check_join(A, B, M, dev, ob)
 
 
+@pytest.mark.parametrize("kb", [4, 8])
+def test_self_join_overflows_the_match_buffer(kb, dev):
+    # A joined with itself over one key a bin: ~2048 matches in every 2048-bin
+    # workgroup range, above the 1024 a workgroup buffers in LDS, so every
+    # range takes the counting re-walk
+    A = wl.uniform_keys(13, 1 << 18, kb)
+    _check_join(A, A.copy(), 1 << 18, dev)
+
+
 def test_duplicates_on_both_sides(dev):
     # bag product: a value with cA copies in A and cB in B contributes cA*cB
     vals = wl.uniform_keys(3, 3000, 4)
