@@ -702,12 +702,12 @@ def run_rowid(dev, steps, warmup):
     ms = e0.elapsed_time(e1) / steps
     kt = ctx.profile_read(reset=True)
     ctx.profile(False)
-    # records of 8 bytes go through the partition (packed from the keys,
-    # unpacked into keys and positions); the join reads both tables' keys,
-    # positions and offsets and writes 16 bytes per pair
-    bt = {"pack_index": (kb + 8) * (nA + nB) * steps, "unpack_index": (8 + kb + 8) * (nA + nB) * steps,
+    # records of 8 bytes (packed from the keys) go through the partition; the
+    # final stage stores each as its key and its position; the join reads both
+    # tables' keys, positions and offsets and writes 16 bytes per pair
+    bt = {"pack_index": (kb + 8) * (nA + nB) * steps,
           "part_hist": 8 * (nA + nB) * steps, "part_scatter": 2 * 8 * (nA + nB) * steps,
-          "final": (2 * 8 * (nA + nB) + 4 * 2 * (M + 1)) * steps,
+          "final": ((8 + kb + 8) * (nA + nB) + 4 * 2 * (M + 1)) * steps,
           "join_pairs": ((kb + 8) * (nA + nB) + 8 * (M + 1) + 16 * K) * steps}
     roof = roofline_from(kt, bt, steps=steps)
     roof["traffic"] = load_traffic().get("rowid", {}).get(roof["kernel"])

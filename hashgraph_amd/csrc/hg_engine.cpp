@@ -531,7 +531,8 @@ SegDesc range_seg(uint64_t kbeg, uint64_t kend, uint64_t bin_lo, uint64_t nbins,
 }
 
 // Final stage over the columns of a level.
-hg_status run_final(hg_ctx c, const Level &L, const LevelDev &d, int kb, int ob, const void *part, void *csr, void *off) {
+hg_status run_final(hg_ctx c, const Level &L, const LevelDev &d, int kb, int ob, const void *part, void *csr, void *off,
+                    uint64_t *csr_index = nullptr) {
   HG_HIP(launch_make_items(d.segs, (uint32_t)L.segs.size(), d.bstart, d.coltot, d.fin, L.ncols_total, c->stream));
   LaunchFinal f{};
   f.key_bytes = kb;
@@ -548,6 +549,7 @@ hg_status run_final(hg_ctx c, const Level &L, const LevelDev &d, int kb, int ob,
   f.big_ctl = d.big_ctl;
   f.big_cnt = d.big_cnt;
   f.big_cap = L.big_cap;
+  f.csr_index = csr_index;
   {
     Prof pf(c, "final");
     HG_HIP(launch_final(f, c->stream));
@@ -578,7 +580,7 @@ struct Coarse {
 };
 
 hg_status finish_two_level(hg_ctx c, const std::vector<Coarse> &cs, const BinMap &map, int kb, int ob, const void *part1,
-                           void *part2, void *csr, void *off) {
+                           void *part2, void *csr, void *off, uint64_t *csr_index = nullptr) {
   Level L2;
   for (const Coarse &q : cs) {
     if (q.nbins == 0) continue;
@@ -600,14 +602,14 @@ hg_status finish_two_level(hg_ctx c, const std::vector<Coarse> &cs, const BinMap
   LevelDev d2;
   HG_TRY(setup_level(c, L2, d2));
   HG_TRY(run_level(c, L2, d2, kb, part1, part2));
-  return run_final(c, L2, d2, kb, ob, part2, csr, off);
+  return run_final(c, L2, d2, kb, ob, part2, csr, off, csr_index);
 }
 
 // CSR of n keys whose bins all lie in [lo, lo + R): d_off[0..R] with values
 // 0..n (positions in d_csr) and d_csr.  hg_build is the range [0, M); a
 // staged build runs it once per slab of bins.
 hg_status build_range(hg_ctx c, const BinMap &map, int kb, int ob, uint64_t n, uint64_t lo, uint64_t R,
-                      const void *d_keys, void *d_off, void *d_csr) {
+                      const void *d_keys, void *d_off, void *d_csr, uint64_t *d_csr_index = nullptr) {
   if (n == 0) return fill_offsets(c, d_off, ob, R + 1, 0);
   const uint32_t W = choose_width(n, R, kb, build_fill());
   const uint64_t F = (R + W - 1) / W;
@@ -632,7 +634,7 @@ hg_status build_range(hg_ctx c, const BinMap &map, int kb, int ob, uint64_t n, u
     HG_TRY(setup_level(c, L, d));
     void *part = cv0.get<void>(i_part);
     HG_TRY(run_level(c, L, d, kb, d_keys, part));
-    return run_final(c, L, d, kb, ob, part, d_csr, d_off);
+    return run_final(c, L, d, kb, ob, part, d_csr, d_off, d_csr_index);
   }
   // Two levels: coarse super-buckets of k*W bins, their sizes read back, then
   // every super-bucket is a segment of the second pass.
@@ -666,7 +668,8 @@ hg_status build_range(hg_ctx c, const BinMap &map, int kb, int ob, uint64_t n, u
     q.end_pos = (g + 1 == L1.ncols_total) ? R : ~0ull;
     cs.push_back(q);
   }
-  return finish_two_level(c, cs, map, kb, ob, cv0.get<void>(i_part1), cv0.get<void>(i_part2), d_csr, d_off);
+  return finish_two_level(c, cs, map, kb, ob, cv0.get<void>(i_part1), cv0.get<void>(i_part2), d_csr, d_off,
+                          d_csr_index);
 }
 
 // Driver of hg_build.
@@ -822,9 +825,10 @@ uint64_t ref_binary_search(const std::vector<int64_t> &bins, uint64_t l, uint64_
 
 hg_status check_partitioned(hg_ctx c, const hg_table_desc *desc, const void *d_keys, const int64_t *h_splits,
                             uint32_t n_parts, void *d_offsets, void *d_csr_keys, uint64_t *h_part_offsets);
-hg_status index_records(hg_ctx c, uint64_t n, uint64_t *&rin, uint64_t *&rout);
+hg_status index_records(hg_ctx c, uint64_t n, uint64_t *&rin, uint64_t *&rout, bool with_out);
 hg_status build_partitioned(hg_ctx c, const hg_table_desc *desc, const void *d_keys, const int64_t *h_splits,
-                            uint32_t n_parts, void *d_offsets, void *d_csr_keys, uint64_t *h_part_offsets);
+                            uint32_t n_parts, void *d_offsets, void *d_csr_keys, uint64_t *h_part_offsets,
+                            uint64_t *d_csr_index = nullptr);
 void splits_from_hist(const std::vector<unsigned long long> &h, uint64_t bin_count, uint64_t n, uint32_t n_parts,
                       uint64_t range, uint64_t M, int64_t *h_splits);
 
@@ -1803,13 +1807,17 @@ hg_status hg_build_indexed(hg_ctx c, const hg_table_desc *desc, const void *d_ke
   const uint64_t n = desc->n_keys;
   if (n == 0) return fill_offsets(c, d_offsets, (int)desc->offset_bytes, desc->n_bins + 1, 0);
   uint64_t *rin = nullptr, *rout = nullptr;
-  HG_TRY(index_records(c, n, rin, rout));
+  HG_TRY(index_records(c, n, rin, rout, desc->key_bytes == 8));
   {
     Prof pf(c, "pack_index");
     HG_HIP(launch_pack_index(d_keys, n, (int)desc->key_bytes, rin, c->stream));
   }
   hg_table_desc d8 = *desc;
   d8.key_bytes = 8;
+  if (desc->key_bytes == 4) {  // the final stage stores each record as its key and position
+    const BinMap map = make_map(d8.n_bins, d8.hash_kind, d8.d_rmi_model);
+    return build_range(c, map, 8, (int)d8.offset_bytes, n, 0, d8.n_bins, rin, d_offsets, d_csr_keys, d_csr_index);
+  }
   HG_TRY(build_table(c, &d8, rin, d_offsets, rout));
   Prof pf(c, "unpack_index");
   HG_HIP(launch_unpack_index(rout, n, d_keys, (int)desc->key_bytes, d_csr_keys, d_csr_index, c->stream));
@@ -1826,13 +1834,15 @@ hg_status hg_build_partitioned_indexed(hg_ctx c, const hg_table_desc *desc, cons
   DeviceGuard g(c->device);
   const uint64_t n = desc->n_keys;
   uint64_t *rin = nullptr, *rout = nullptr;
-  HG_TRY(index_records(c, std::max<uint64_t>(n, 1), rin, rout));
+  HG_TRY(index_records(c, std::max<uint64_t>(n, 1), rin, rout, desc->key_bytes == 8));
   {
     Prof pf(c, "pack_index");
     HG_HIP(launch_pack_index(d_keys, n, (int)desc->key_bytes, rin, c->stream));
   }
   hg_table_desc d8 = *desc;
   d8.key_bytes = 8;
+  if (desc->key_bytes == 4)  // the final stage stores each record as its key and position
+    return build_partitioned(c, &d8, rin, h_splits, n_parts, d_offsets, d_csr_keys, h_part_offsets, d_csr_index);
   HG_TRY(build_partitioned(c, &d8, rin, h_splits, n_parts, d_offsets, rout, h_part_offsets));
   Prof pf(c, "unpack_index");
   HG_HIP(launch_unpack_index(rout, h_part_offsets[n_parts], d_keys, (int)desc->key_bytes, d_csr_keys, d_csr_index,
@@ -1860,19 +1870,22 @@ hg_status check_partitioned(hg_ctx c, const hg_table_desc *desc, const void *d_k
 }
 
 // The two record arrays (input, CSR) of an index-carrying build, in arena 2.
-hg_status index_records(hg_ctx c, uint64_t n, uint64_t *&rin, uint64_t *&rout) {
+// Record buffers of the index-carrying builds: the packed input and, when
+// the CSR is unpacked afterwards (64-bit keys), the built records.
+hg_status index_records(hg_ctx c, uint64_t n, uint64_t *&rin, uint64_t *&rout, bool with_out) {
   Plan P;
-  const size_t i_in = P.add(n * 8), i_out = P.add(n * 8);
+  const size_t i_in = P.add(n * 8), i_out = with_out ? P.add(n * 8) : 0;
   HG_TRY(arena_ensure(c, 2, P.total()));
   Carve cv;
   cv.init(c, 2, P);
   rin = cv.get<uint64_t>(i_in);
-  rout = cv.get<uint64_t>(i_out);
+  rout = with_out ? cv.get<uint64_t>(i_out) : nullptr;
   return HG_OK;
 }
 
 hg_status build_partitioned(hg_ctx c, const hg_table_desc *desc, const void *d_keys, const int64_t *h_splits,
-                            uint32_t n_parts, void *d_offsets, void *d_csr_keys, uint64_t *h_part_offsets) {
+                            uint32_t n_parts, void *d_offsets, void *d_csr_keys, uint64_t *h_part_offsets,
+                            uint64_t *d_csr_index) {
   const uint64_t M = desc->n_bins, n = desc->n_keys;
   const int kb = (int)desc->key_bytes, ob = (int)desc->offset_bytes;
   const BinMap map = make_map(M, desc->hash_kind, desc->d_rmi_model);
@@ -1926,7 +1939,7 @@ hg_status build_partitioned(hg_ctx c, const hg_table_desc *desc, const void *d_k
     kept += cnt[p];
   }
   h_part_offsets[n_parts] = kept;
-  return finish_two_level(c, cs, map, kb, ob, part1, part2, d_csr_keys, d_offsets);
+  return finish_two_level(c, cs, map, kb, ob, part1, part2, d_csr_keys, d_offsets, d_csr_index);
 }
 
 }  // namespace

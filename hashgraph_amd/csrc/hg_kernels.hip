@@ -344,6 +344,14 @@ __global__ __launch_bounds__(BLOCK) void k_part_scatter(const K *__restrict__ ke
 }
 
 
+// Final-stage store of an index-carrying build of 32-bit keys (SPLIT): the
+// 8-byte record (input position << 32 | key) goes to the CSR key array as its
+// low half and to the position array as its high half.
+__device__ __forceinline__ void split_store(uint32_t *keys, uint64_t *pos, uint64_t i, uint64_t rec) {
+  keys[i] = (uint32_t)rec;
+  pos[i] = rec >> 32;
+}
+
 // ---------------------------------------------------------------------------
 // Final stage: one workgroup per bucket (count <= BLOCK*KPT, count <= 65535,
 // nbins <= 65535).  Keys stay in VGPRs; the bucket-local bin (16 bit) is
@@ -351,10 +359,11 @@ __global__ __launch_bounds__(BLOCK) void k_part_scatter(const K *__restrict__ ke
 // word); their exclusive scan gives the offsets, a second atomic pass gives
 // every key's slot, then LDS is reused to stage the bin-ordered keys.
 // ---------------------------------------------------------------------------
-template <typename K, typename OFF, int BLOCK, int KPT, int HK>
+template <typename K, typename OFF, int BLOCK, int KPT, int HK, bool SPLIT = false>
 __global__ __launch_bounds__(BLOCK) void k_final(const K *__restrict__ part, K *__restrict__ csr, OFF *__restrict__ off,
                                                  const FinItem *__restrict__ items, const BinMap *__restrict__ maps,
-                                                 uint32_t lds_bytes, unsigned int *__restrict__ big_seen) {
+                                                 uint32_t lds_bytes, unsigned int *__restrict__ big_seen,
+                                                 uint64_t *__restrict__ csr_index) {
   extern __shared__ __align__(16) uint32_t smem32[];
   __shared__ uint32_t s_tmp[BLOCK / 64 + 1];
   const FinItem it = items[blockIdx.x];
@@ -455,7 +464,13 @@ __global__ __launch_bounds__(BLOCK) void k_final(const K *__restrict__ part, K *
     }
     __syncthreads();
     const uint32_t n = min(cap, cnt - r0);
-    for (uint32_t q = threadIdx.x; q < n; q += BLOCK) dst[r0 + q] = st[q];
+    if constexpr (SPLIT) {
+      uint32_t *dk = (uint32_t *)csr + it.key_start;
+      uint64_t *di = csr_index + it.key_start;
+      for (uint32_t q = threadIdx.x; q < n; q += BLOCK) split_store(dk, di, r0 + q, st[q]);
+    } else {
+      for (uint32_t q = threadIdx.x; q < n; q += BLOCK) dst[r0 + q] = st[q];
+    }
     __syncthreads();
   }
 }
@@ -465,11 +480,12 @@ __global__ __launch_bounds__(BLOCK) void k_final(const K *__restrict__ part, K *
 // workgroup streams the bucket: per sub-range of SUB bins, count in LDS,
 // scan, write offsets, stream again placing keys with LDS atomics.
 // ---------------------------------------------------------------------------
-template <typename K, typename OFF, int BLOCK, int SUB>
+template <typename K, typename OFF, int BLOCK, int SUB, bool SPLIT = false>
 __global__ __launch_bounds__(BLOCK) void k_final_big(const K *__restrict__ part, K *__restrict__ csr, OFF *__restrict__ off,
                                                      const FinItem *__restrict__ items, const BinMap *__restrict__ maps,
                                                      uint32_t small_cap, const uint32_t *__restrict__ big_slot,
-                                                     const unsigned int *__restrict__ big_seen) {
+                                                     const unsigned int *__restrict__ big_seen,
+                                                     uint64_t *__restrict__ csr_index) {
   __shared__ uint32_t s_cnt[SUB];
   __shared__ uint32_t s_tmp[BLOCK / 64 + 1];
   if (*big_seen == 0) return;  // no oversized item
@@ -497,7 +513,13 @@ __global__ __launch_bounds__(BLOCK) void k_final_big(const K *__restrict__ part,
     for (uint64_t i = threadIdx.x; i < cnt; i += BLOCK) {
       const K k = kp[i];
       const uint32_t l = bin_of<K>(k, mp) - it.bin_lo - s0;
-      if (l < n) dst[done + atomicAdd(&s_cnt[l], 1u)] = k;
+      if (l < n) {
+        const uint64_t w = done + atomicAdd(&s_cnt[l], 1u);
+        if constexpr (SPLIT)
+          split_store((uint32_t *)csr + it.key_start, csr_index + it.key_start, w, k);
+        else
+          dst[w] = k;
+      }
     }
     __syncthreads();
     done += tot;
@@ -2403,10 +2425,11 @@ __global__ __launch_bounds__(1024) void k_big_scan(const FinItem *__restrict__ i
   }
 }
 
-template <typename K, int BLOCK>
+template <typename K, int BLOCK, bool SPLIT = false>
 __global__ __launch_bounds__(BLOCK) void k_big_place(const K *__restrict__ part, K *__restrict__ csr,
                                                      const FinItem *__restrict__ items, const BinMap *__restrict__ maps,
-                                                     const BigCtl *__restrict__ ctl, uint32_t *__restrict__ gcur) {
+                                                     const BigCtl *__restrict__ ctl, uint32_t *__restrict__ gcur,
+                                                     uint64_t *__restrict__ csr_index) {
   __shared__ uint32_t s_cnt[BIG_BINS];
   const uint32_t total = ctl->total_chunks;
   for (uint32_t g = blockIdx.x; g < total; g += gridDim.x) {
@@ -2471,7 +2494,12 @@ __global__ __launch_bounds__(BLOCK) void k_big_place(const K *__restrict__ part,
           if (i < n) l = bin_of<K>(kk[u], mp) - it.bin_lo - b0;
           const bool v = i < n && l < nb;
           const uint32_t pos = lds_add_agg<true>(s_cnt, l, v);
-          if (v) dst[pos] = kk[u];
+          if (v) {
+            if constexpr (SPLIT)
+              split_store((uint32_t *)csr + it.key_start, csr_index + it.key_start, pos, kk[u]);
+            else
+              dst[pos] = kk[u];
+          }
         }
       }
       __syncthreads();
@@ -2822,15 +2850,18 @@ hipError_t launch_colscan(const LaunchCol &lc, hipStream_t st) {
   return hipGetLastError();
 }
 
-template <typename K, typename OFF>
+template <typename K, typename OFF, bool SPLIT>
 static hipError_t launch_final_t(const LaunchFinal &lf, hipStream_t st) {
   if (lf.nitems) hipLaunchKernelGGL(k_fill<uint32_t>, dim3(1), dim3(64), 0, st, &((BigCtl *)lf.big_ctl)->seen, 0u, (uint64_t)1);
   if (lf.nitems) {
     constexpr int KPT = sizeof(K) == 4 ? KPT_U32 : 32;
     const hipError_t e = dispatch_hk(lf.hk, [&](auto h) {
-      hipLaunchKernelGGL((k_final<K, OFF, 1024, KPT, decltype(h)::value>), dim3(lf.nitems), dim3(1024), lf.lds_bytes, st,
-                         (const K *)lf.part, (K *)lf.csr, (OFF *)lf.off, lf.items, lf.maps, lf.lds_bytes,
-                         &((BigCtl *)lf.big_ctl)->seen);
+      constexpr int H = decltype(h)::value;
+      // the split store exists for murmur3 tables only (launch_final checks)
+      auto kern = &k_final<K, OFF, 1024, KPT, H>;
+      if constexpr (SPLIT && H == HK_MURMUR) kern = &k_final<K, OFF, 1024, KPT, H, true>;
+      hipLaunchKernelGGL(kern, dim3(lf.nitems), dim3(1024), lf.lds_bytes, st, (const K *)lf.part, (K *)lf.csr,
+                         (OFF *)lf.off, lf.items, lf.maps, lf.lds_bytes, &((BigCtl *)lf.big_ctl)->seen, lf.csr_index);
       return hipGetLastError();
     });
     if (e != hipSuccess) return e;
@@ -2846,19 +2877,26 @@ static hipError_t launch_final_t(const LaunchFinal &lf, hipStream_t st) {
                        (const BigCtl *)ctl, lf.big_cnt);
     hipLaunchKernelGGL((k_big_scan<OFF>), dim3(256), dim3(1024), 0, st, lf.items, (const BigCtl *)ctl, lf.big_cnt,
                        (OFF *)lf.off);
-    hipLaunchKernelGGL((k_big_place<K, 1024>), dim3(2048), dim3(1024), 0, st, (const K *)lf.part, (K *)lf.csr, lf.items,
-                       lf.maps, (const BigCtl *)ctl, lf.big_cnt);
-    hipLaunchKernelGGL((k_final_big<K, OFF, 1024, 16384>), dim3(lf.nitems), dim3(1024), 0, st, (const K *)lf.part,
+    hipLaunchKernelGGL((k_big_place<K, 1024, SPLIT>), dim3(2048), dim3(1024), 0, st, (const K *)lf.part, (K *)lf.csr,
+                       lf.items, lf.maps, (const BigCtl *)ctl, lf.big_cnt, lf.csr_index);
+    hipLaunchKernelGGL((k_final_big<K, OFF, 1024, 16384, SPLIT>), dim3(lf.nitems), dim3(1024), 0, st, (const K *)lf.part,
                        (K *)lf.csr, (OFF *)lf.off, lf.items, lf.maps, small_cap, (const uint32_t *)lf.big_slot,
-                       (const unsigned int *)&ctl->seen);
+                       (const unsigned int *)&ctl->seen, lf.csr_index);
   }
   return hipGetLastError();
 }
 
 hipError_t launch_final(const LaunchFinal &lf, hipStream_t st) {
+  if (lf.csr_index) {  // index-carrying build of 32-bit keys: 8-byte murmur3 records split on store
+    if (lf.key_bytes != 8 || lf.hk != HK_MURMUR) return hipErrorInvalidValue;
+    return lf.off_bytes == 4 ? launch_final_t<uint64_t, uint32_t, true>(lf, st)
+                             : launch_final_t<uint64_t, uint64_t, true>(lf, st);
+  }
   if (lf.key_bytes == 4)
-    return lf.off_bytes == 4 ? launch_final_t<uint32_t, uint32_t>(lf, st) : launch_final_t<uint32_t, uint64_t>(lf, st);
-  return lf.off_bytes == 4 ? launch_final_t<uint64_t, uint32_t>(lf, st) : launch_final_t<uint64_t, uint64_t>(lf, st);
+    return lf.off_bytes == 4 ? launch_final_t<uint32_t, uint32_t, false>(lf, st)
+                             : launch_final_t<uint32_t, uint64_t, false>(lf, st);
+  return lf.off_bytes == 4 ? launch_final_t<uint64_t, uint32_t, false>(lf, st)
+                           : launch_final_t<uint64_t, uint64_t, false>(lf, st);
 }
 
 int final_kpt(int key_bytes) { return key_bytes == 4 ? KPT_U32 : 32; }
@@ -3310,6 +3348,10 @@ hipError_t set_kernel_attributes() {
       if (r == hipSuccess) r = set_lds(&k_final<uint32_t, uint64_t, 1024, KPT_U32, H>);
       if (r == hipSuccess) r = set_lds(&k_final<uint64_t, uint32_t, 1024, 32, H>);
       if (r == hipSuccess) r = set_lds(&k_final<uint64_t, uint64_t, 1024, 32, H>);
+      if constexpr (H == HK_MURMUR) {
+        if (r == hipSuccess) r = set_lds(&k_final<uint64_t, uint32_t, 1024, 32, H, true>);
+        if (r == hipSuccess) r = set_lds(&k_final<uint64_t, uint64_t, 1024, 32, H, true>);
+      }
       if (r == hipSuccess) r = set_lds(&k_probe1<uint32_t, uint32_t, 1024, probe1_kpt<uint32_t>(), H>);
       if (r == hipSuccess) r = set_lds(&k_probe1<uint32_t, uint64_t, 1024, probe1_kpt<uint32_t>(), H>);
       if (r == hipSuccess) r = set_lds(&k_probe1<uint64_t, uint32_t, 1024, probe1_kpt<uint64_t>(), H>);

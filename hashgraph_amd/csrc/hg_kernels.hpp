@@ -55,6 +55,10 @@ struct LaunchFinal {
   void *big_ctl;
   uint32_t *big_cnt;
   uint32_t big_cap;
+  // index-carrying build of 32-bit keys: part holds 8-byte records (input
+  // position << 32 | key); csr then receives the 32-bit keys and csr_index the
+  // positions (murmur3 tables only)
+  uint64_t *csr_index = nullptr;
 };
 
 struct LaunchProbe {
