@@ -702,11 +702,11 @@ def run_rowid(dev, steps, warmup):
     ms = e0.elapsed_time(e1) / steps
     kt = ctx.profile_read(reset=True)
     ctx.profile(False)
-    # records of 8 bytes (packed from the keys) go through the partition; the
-    # final stage stores each as its key and its position; the join reads both
-    # tables' keys, positions and offsets and writes 16 bytes per pair
-    bt = {"pack_index": (kb + 8) * (nA + nB) * steps,
-          "part_hist": 8 * (nA + nB) * steps, "part_scatter": 2 * 8 * (nA + nB) * steps,
+    # the partition counts the keys and scatters them as 8-byte records (input
+    # position << 32 | key); the final stage stores each record as its key and
+    # its position; the join reads both tables' keys, positions and offsets
+    # and writes 16 bytes per pair
+    bt = {"part_hist": kb * (nA + nB) * steps, "part_scatter": (kb + 8) * (nA + nB) * steps,
           "final": ((8 + kb + 8) * (nA + nB) + 4 * 2 * (M + 1)) * steps,
           "join_pairs": ((kb + 8) * (nA + nB) + 8 * (M + 1) + 16 * K) * steps}
     roof = roofline_from(kt, bt, steps=steps)

@@ -473,9 +473,10 @@ hg_status setup_level(hg_ctx c, Level &L, LevelDev &d) {
 
 // scatter = false stops after the column scan (bucket sizes only).
 hg_status run_level(hg_ctx c, const Level &L, const LevelDev &d, int kb, const void *in, void *out,
-                    bool scatter = true) {
+                    bool scatter = true, bool pack = false) {
   LaunchPart p{};
   p.key_bytes = kb;
+  p.pack = pack;
   p.hk = L.hash_kind();
   p.sk = L.bucket_kind();
   p.ntiles = (uint32_t)L.tiles.size();
@@ -609,7 +610,8 @@ hg_status finish_two_level(hg_ctx c, const std::vector<Coarse> &cs, const BinMap
 // 0..n (positions in d_csr) and d_csr.  hg_build is the range [0, M); a
 // staged build runs it once per slab of bins.
 hg_status build_range(hg_ctx c, const BinMap &map, int kb, int ob, uint64_t n, uint64_t lo, uint64_t R,
-                      const void *d_keys, void *d_off, void *d_csr, uint64_t *d_csr_index = nullptr) {
+                      const void *d_keys, void *d_off, void *d_csr, uint64_t *d_csr_index = nullptr,
+                      bool pack_in = false) {
   if (n == 0) return fill_offsets(c, d_off, ob, R + 1, 0);
   const uint32_t W = choose_width(n, R, kb, build_fill());
   const uint64_t F = (R + W - 1) / W;
@@ -633,7 +635,7 @@ hg_status build_range(hg_ctx c, const BinMap &map, int kb, int ob, uint64_t n, u
     LevelDev d;
     HG_TRY(setup_level(c, L, d));
     void *part = cv0.get<void>(i_part);
-    HG_TRY(run_level(c, L, d, kb, d_keys, part));
+    HG_TRY(run_level(c, L, d, kb, d_keys, part, true, pack_in));
     return run_final(c, L, d, kb, ob, part, d_csr, d_off, d_csr_index);
   }
   // Two levels: coarse super-buckets of k*W bins, their sizes read back, then
@@ -652,7 +654,7 @@ hg_status build_range(hg_ctx c, const BinMap &map, int kb, int ob, uint64_t n, u
   cv0.init(c, 0, P0);
   LevelDev d1;
   HG_TRY(setup_level(c, L1, d1));
-  HG_TRY(run_level(c, L1, d1, kb, d_keys, cv0.get<void>(i_part1)));
+  HG_TRY(run_level(c, L1, d1, kb, d_keys, cv0.get<void>(i_part1), true, pack_in));
   std::vector<uint32_t> cnt(L1.ncols_total), st(L1.ncols_total);
   HG_TRY(sync_read(c, cnt.data(), d1.coltot, cnt.size() * 4));
   HG_TRY(sync_read(c, st.data(), d1.bstart, st.size() * 4));
@@ -825,10 +827,10 @@ uint64_t ref_binary_search(const std::vector<int64_t> &bins, uint64_t l, uint64_
 
 hg_status check_partitioned(hg_ctx c, const hg_table_desc *desc, const void *d_keys, const int64_t *h_splits,
                             uint32_t n_parts, void *d_offsets, void *d_csr_keys, uint64_t *h_part_offsets);
-hg_status index_records(hg_ctx c, uint64_t n, uint64_t *&rin, uint64_t *&rout, bool with_out);
+hg_status index_records(hg_ctx c, uint64_t n, uint64_t *&rin, uint64_t *&rout);
 hg_status build_partitioned(hg_ctx c, const hg_table_desc *desc, const void *d_keys, const int64_t *h_splits,
                             uint32_t n_parts, void *d_offsets, void *d_csr_keys, uint64_t *h_part_offsets,
-                            uint64_t *d_csr_index = nullptr);
+                            uint64_t *d_csr_index = nullptr, bool pack_in = false);
 void splits_from_hist(const std::vector<unsigned long long> &h, uint64_t bin_count, uint64_t n, uint32_t n_parts,
                       uint64_t range, uint64_t M, int64_t *h_splits);
 
@@ -1806,17 +1808,20 @@ hg_status hg_build_indexed(hg_ctx c, const hg_table_desc *desc, const void *d_ke
   DeviceGuard g(c->device);
   const uint64_t n = desc->n_keys;
   if (n == 0) return fill_offsets(c, d_offsets, (int)desc->offset_bytes, desc->n_bins + 1, 0);
+  hg_table_desc d8 = *desc;
+  d8.key_bytes = 8;
+  if (desc->key_bytes == 4) {
+    // the first partition pass forms the records from the keys, the final
+    // stage stores each record as its key and its position
+    const BinMap map = make_map(d8.n_bins, d8.hash_kind, d8.d_rmi_model);
+    return build_range(c, map, 8, (int)d8.offset_bytes, n, 0, d8.n_bins, d_keys, d_offsets, d_csr_keys, d_csr_index,
+                       true);
+  }
   uint64_t *rin = nullptr, *rout = nullptr;
-  HG_TRY(index_records(c, n, rin, rout, desc->key_bytes == 8));
+  HG_TRY(index_records(c, n, rin, rout));
   {
     Prof pf(c, "pack_index");
     HG_HIP(launch_pack_index(d_keys, n, (int)desc->key_bytes, rin, c->stream));
-  }
-  hg_table_desc d8 = *desc;
-  d8.key_bytes = 8;
-  if (desc->key_bytes == 4) {  // the final stage stores each record as its key and position
-    const BinMap map = make_map(d8.n_bins, d8.hash_kind, d8.d_rmi_model);
-    return build_range(c, map, 8, (int)d8.offset_bytes, n, 0, d8.n_bins, rin, d_offsets, d_csr_keys, d_csr_index);
   }
   HG_TRY(build_table(c, &d8, rin, d_offsets, rout));
   Prof pf(c, "unpack_index");
@@ -1833,16 +1838,17 @@ hg_status hg_build_partitioned_indexed(hg_ctx c, const hg_table_desc *desc, cons
     return fail(HG_ERR_UNSUPPORTED, "index-carrying builds take murmur3 tables (bins of the low 32 key bits)");
   DeviceGuard g(c->device);
   const uint64_t n = desc->n_keys;
+  hg_table_desc d8 = *desc;
+  d8.key_bytes = 8;
+  if (desc->key_bytes == 4)  // records formed by the first pass, stored back as key and position
+    return build_partitioned(c, &d8, d_keys, h_splits, n_parts, d_offsets, d_csr_keys, h_part_offsets, d_csr_index,
+                             true);
   uint64_t *rin = nullptr, *rout = nullptr;
-  HG_TRY(index_records(c, std::max<uint64_t>(n, 1), rin, rout, desc->key_bytes == 8));
+  HG_TRY(index_records(c, std::max<uint64_t>(n, 1), rin, rout));
   {
     Prof pf(c, "pack_index");
     HG_HIP(launch_pack_index(d_keys, n, (int)desc->key_bytes, rin, c->stream));
   }
-  hg_table_desc d8 = *desc;
-  d8.key_bytes = 8;
-  if (desc->key_bytes == 4)  // the final stage stores each record as its key and position
-    return build_partitioned(c, &d8, rin, h_splits, n_parts, d_offsets, d_csr_keys, h_part_offsets, d_csr_index);
   HG_TRY(build_partitioned(c, &d8, rin, h_splits, n_parts, d_offsets, rout, h_part_offsets));
   Prof pf(c, "unpack_index");
   HG_HIP(launch_unpack_index(rout, h_part_offsets[n_parts], d_keys, (int)desc->key_bytes, d_csr_keys, d_csr_index,
@@ -1870,22 +1876,22 @@ hg_status check_partitioned(hg_ctx c, const hg_table_desc *desc, const void *d_k
 }
 
 // The two record arrays (input, CSR) of an index-carrying build, in arena 2.
-// Record buffers of the index-carrying builds: the packed input and, when
-// the CSR is unpacked afterwards (64-bit keys), the built records.
-hg_status index_records(hg_ctx c, uint64_t n, uint64_t *&rin, uint64_t *&rout, bool with_out) {
+// Record buffers of the index-carrying builds of 64-bit keys: the packed
+// input and the built records (unpacked afterwards).
+hg_status index_records(hg_ctx c, uint64_t n, uint64_t *&rin, uint64_t *&rout) {
   Plan P;
-  const size_t i_in = P.add(n * 8), i_out = with_out ? P.add(n * 8) : 0;
+  const size_t i_in = P.add(n * 8), i_out = P.add(n * 8);
   HG_TRY(arena_ensure(c, 2, P.total()));
   Carve cv;
   cv.init(c, 2, P);
   rin = cv.get<uint64_t>(i_in);
-  rout = with_out ? cv.get<uint64_t>(i_out) : nullptr;
+  rout = cv.get<uint64_t>(i_out);
   return HG_OK;
 }
 
 hg_status build_partitioned(hg_ctx c, const hg_table_desc *desc, const void *d_keys, const int64_t *h_splits,
                             uint32_t n_parts, void *d_offsets, void *d_csr_keys, uint64_t *h_part_offsets,
-                            uint64_t *d_csr_index) {
+                            uint64_t *d_csr_index, bool pack_in) {
   const uint64_t M = desc->n_bins, n = desc->n_keys;
   const int kb = (int)desc->key_bytes, ob = (int)desc->offset_bytes;
   const BinMap map = make_map(M, desc->hash_kind, desc->d_rmi_model);
@@ -1914,7 +1920,7 @@ hg_status build_partitioned(hg_ctx c, const hg_table_desc *desc, const void *d_k
   L1.prepare();
   LevelDev d1;
   HG_TRY(setup_level(c, L1, d1));
-  HG_TRY(run_level(c, L1, d1, kb, d_keys, part1));
+  HG_TRY(run_level(c, L1, d1, kb, d_keys, part1, true, pack_in));
   std::vector<uint32_t> cnt(L1.ncols_total), st(L1.ncols_total);
   HG_TRY(sync_read(c, cnt.data(), d1.coltot, cnt.size() * 4));
   HG_TRY(sync_read(c, st.data(), d1.bstart, st.size() * 4));

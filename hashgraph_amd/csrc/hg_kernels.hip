@@ -244,7 +244,7 @@ __global__ __launch_bounds__(256) void k_col_apply(uint32_t *__restrict__ cmat, 
 // (tile, bucket) run contiguously at its count-matrix position.
 // Requires tile count <= 65536 and ncols <= 65536.
 // ---------------------------------------------------------------------------
-template <typename K, int BLOCK, int KPT, int HK, int SK>
+template <typename K, int BLOCK, int KPT, int HK, int SK, bool PACK = false>
 __global__ __launch_bounds__(BLOCK) void k_part_scatter(const K *__restrict__ keys, const TileDesc *__restrict__ tiles,
                                                         const SegDesc *__restrict__ segs, const uint32_t *__restrict__ cpos,
                                                         K *__restrict__ out, uint32_t stage_cap) {
@@ -271,14 +271,24 @@ __global__ __launch_bounds__(BLOCK) void k_part_scatter(const K *__restrict__ ke
     const uint32_t cnt = min((uint32_t)(BLOCK * KPT), td.count - base);
     for (uint32_t i = threadIdx.x; i < ncols; i += BLOCK) s_cnt[i] = 0;
     __syncthreads();
-    const K *kp = keys + td.key_start + base;
     const uint32_t nv = cnt > threadIdx.x ? (cnt - threadIdx.x + BLOCK - 1) / BLOCK : 0;
-    const __amdgpu_buffer_rsrc_t rs = make_rsrc(kp, cnt * (uint32_t)sizeof(K));
     K kr[KPT];
     uint32_t pk[KPT];
+    if constexpr (PACK) {
+      // 32-bit input keys made into 8-byte records: input position << 32 | key
+      const uint32_t *kp = (const uint32_t *)keys + td.key_start + base;
+      const __amdgpu_buffer_rsrc_t rs = make_rsrc(kp, cnt * 4u);
+      const uint64_t i0 = td.key_start + base + threadIdx.x;
 #pragma unroll
-    for (int j = 0; j < KPT; j++)
-      kr[j] = buf_load<K>(rs, threadIdx.x * (uint32_t)sizeof(K), j * BLOCK * (uint32_t)sizeof(K));
+      for (int j = 0; j < KPT; j++)
+        kr[j] = ((i0 + (uint64_t)j * BLOCK) << 32) | buf_load<uint32_t>(rs, threadIdx.x * 4u, j * BLOCK * 4u);
+    } else {
+      const K *kp = keys + td.key_start + base;
+      const __amdgpu_buffer_rsrc_t rs = make_rsrc(kp, cnt * (uint32_t)sizeof(K));
+#pragma unroll
+      for (int j = 0; j < KPT; j++)
+        kr[j] = buf_load<K>(rs, threadIdx.x * (uint32_t)sizeof(K), j * BLOCK * (uint32_t)sizeof(K));
+    }
     if constexpr (sizeof(K) == 4) {
 #pragma unroll
       for (int j = 0; j < KPT; j++) pk[j] = j < (int)nv ? bucket_of<K, HK, SK>(kr[j], sd, spl) : 0u;
@@ -2809,8 +2819,14 @@ static hipError_t launch_part_scatter_t(const LaunchPart &lp, hipStream_t st) {
   const size_t cnt_bytes = (size_t)((lp.max_cols + 3) & ~3u) * 4;
   const size_t lds = 2 * cnt_bytes + (size_t)lp.stage_cap * (sizeof(K) + 2);
   return dispatch_part(lp.hk, lp.sk, [&](auto h, auto b) {
-    hipLaunchKernelGGL((k_part_scatter<K, 1024, SCATTER_KPT, decltype(h)::value, decltype(b)::value>),
-                       dim3(lp.ntiles), dim3(1024), lds, st, (const K *)lp.keys, lp.tiles, lp.segs, lp.cmat,
+    constexpr int H = decltype(h)::value, S = decltype(b)::value;
+    // the packing instance exists for murmur3 levels of 8-byte records only
+    // (launch_part_scatter checks)
+    auto kern = &k_part_scatter<K, 1024, SCATTER_KPT, H, S>;
+    if constexpr (sizeof(K) == 8 && H == HK_MURMUR) {
+      if (lp.pack) kern = &k_part_scatter<K, 1024, SCATTER_KPT, H, S, true>;
+    }
+    hipLaunchKernelGGL(kern, dim3(lp.ntiles), dim3(1024), lds, st, (const K *)lp.keys, lp.tiles, lp.segs, lp.cmat,
                        (K *)lp.out, lp.stage_cap);
     return hipGetLastError();
   });
@@ -2828,9 +2844,12 @@ uint32_t part_stage_cap(uint32_t max_cols, int key_bytes, int sk) {
 }
 
 hipError_t launch_part_hist(const LaunchPart &lp, hipStream_t st) {
-  return lp.key_bytes == 4 ? launch_part_hist_t<uint32_t>(lp, st) : launch_part_hist_t<uint64_t>(lp, st);
+  // packing levels count the 32-bit keys themselves (their records fall in
+  // the same buckets: murmur3 bins depend on the low 32 bits only)
+  return lp.key_bytes == 4 || lp.pack ? launch_part_hist_t<uint32_t>(lp, st) : launch_part_hist_t<uint64_t>(lp, st);
 }
 hipError_t launch_part_scatter(const LaunchPart &lp, hipStream_t st) {
+  if (lp.pack && (lp.key_bytes != 8 || lp.hk != HK_MURMUR)) return hipErrorInvalidValue;
   return lp.key_bytes == 4 ? launch_part_scatter_t<uint32_t>(lp, st) : launch_part_scatter_t<uint64_t>(lp, st);
 }
 
@@ -3338,6 +3357,9 @@ hipError_t set_kernel_attributes() {
         if (r == hipSuccess) r = set_lds(&k_part_hist<uint64_t, 1024, HIST_KPT, H, S>);
         if (r == hipSuccess) r = set_lds(&k_part_scatter<uint32_t, 1024, SCATTER_KPT, H, S>);
         if (r == hipSuccess) r = set_lds(&k_part_scatter<uint64_t, 1024, SCATTER_KPT, H, S>);
+        if constexpr (H == HK_MURMUR) {
+          if (r == hipSuccess) r = set_lds(&k_part_scatter<uint64_t, 1024, SCATTER_KPT, H, S, true>);
+        }
         return r;
       }));
     }

@@ -22,6 +22,9 @@ struct LaunchPart {
   const TileDesc *tiles;
   const SegDesc *segs;
   uint32_t *cmat;        // counts after hist, positions after the column scan
+  // keys are 32-bit and the level writes 8-byte records (input position << 32
+  // | key) of them (key_bytes 8, murmur3 only): index-carrying builds
+  bool pack = false;
 };
 
 struct LaunchCol {
