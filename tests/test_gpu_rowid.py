@@ -100,3 +100,39 @@ def test_short_capacity_and_compare_limit(dev):
     with pytest.raises(hg.HGError) as e:
         hg.intersect_indexed(oa, ka, ia, ob, kb, ib, 1)
     assert e.value.status == 6
+
+
+@pytest.mark.parametrize("kb,n", [(4, 1 << 28), (8, 1 << 26)])
+def test_large_shape_properties(kb, n, dev):
+    """The row-id join at its benchmark shape (2^28 + 2^28 uint32 keys, 2^28
+    bins, B 25% drawn from A; uint64 keys at 2^26), checked on the device
+    without hashing: every pair's two keys are equal, no pair repeats, and
+    the count equals the sum over values of cA(v) * cB(v) by sort-merge.
+    Together these fix the pair set to {(i, j) : A[i] == B[j]}."""
+    import torch
+    import hashgraph_amd as hg
+    M = n
+    print(f"row-id join kb={kb} n={n} M={M}", flush=True)
+    A = wl.uniform_keys(1, n, kb, device=dev)
+    B = wl.probe_keys(A, 2, n, 0.25, kb, device=dev)
+    ta = hg.build_indexed(A, M)
+    tb = hg.build_indexed(B, M)
+    cnt, pa, pb = hg.intersect_indexed(*ta, *tb, M)
+    del ta, tb
+    assert torch.equal(A[pa], B[pb]), "a pair joins different keys"
+    ks = torch.sort(pa * n + pb)[0]
+    assert not bool((ks[1:] == ks[:-1]).any()), "a pair repeats"
+    del ks, pa, pb
+    # uint32 keys as non-negative int64; uint64 keys keep their int64 view
+    # (equality is all the merge needs)
+    wa = A.to(torch.int64) & 0xFFFFFFFF if kb == 4 else A
+    wb = B.to(torch.int64) & 0xFFFFFFFF if kb == 4 else B
+    ua, ca = torch.unique(wa, return_counts=True)
+    ub, cb = torch.unique(wb, return_counts=True)
+    del wa, wb
+    pos = torch.searchsorted(ua, ub).clamp(max=ua.numel() - 1)
+    hit = ua[pos] == ub
+    expect = int((ca[pos][hit].to(torch.int64) * cb[hit].to(torch.int64)).sum())
+    assert cnt == expect and cnt > n // 5
+    del A, B, ua, ca, ub, cb, pos, hit
+    torch.cuda.empty_cache()
