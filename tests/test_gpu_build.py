@@ -199,3 +199,51 @@ def test_build_indexed_carries_positions(kb, dev):
     assert torch.equal(torch.sort(idx)[0], torch.arange(n, device=dev))
     eoff, _ = o.build_csr(keys, M)
     assert np.array_equal(off.cpu().numpy().astype(np.uint64), eoff)
+
+
+@pytest.mark.parametrize("kb", [4, 8])
+@pytest.mark.parametrize("cut", [False, True])
+def test_build_partitioned_indexed_carries_positions(kb, cut, dev):
+    """hg_build_partitioned_indexed: offsets and partition offsets equal
+    hg_build_partitioned's, every bin holds the same keys, every CSR key equals
+    the input key at its position, and the positions are exactly those of the
+    keys whose bins lie in the splits' range (all keys, or, with cut splits,
+    the keys of the first seven eighths of the bins)."""
+    import ctypes
+    import torch
+    import hashgraph_amd as hg
+    n, M = 300000, 1 << 17
+    keys = wl.uniform_keys(73, n, kb)
+    keys[:5000] = keys[5000:10000]  # duplicates
+    dk = _to_dev(keys, dev)
+    splits = [0, M // 2, 7 * M // 8] if cut else [0, M // 3, M // 2, M]
+    P = len(splits) - 1
+    print(f"partitioned indexed kb={kb} splits={splits}", flush=True)
+    ref_off, ref_csr, ref_po = hg.build_partitioned(dk, M, splits)
+    c = hg.context(0)
+    off = torch.empty(splits[-1] + P, dtype=torch.int32, device=dev)
+    csr = torch.empty_like(dk)
+    idx = torch.empty(n, dtype=torch.int64, device=dev)
+    d = hg._desc(n, M, kb)
+    sp = (ctypes.c_int64 * (P + 1))(*splits)
+    po = (ctypes.c_uint64 * (P + 1))()
+    st = c.lib.hg_build_partitioned_indexed(c.bind_stream(), ctypes.byref(d), hg._p(dk), sp, P, hg._p(off),
+                                            hg._p(csr), hg._p(idx), po)
+    assert st == 0, hg._lib.load().hg_last_error()
+    torch.cuda.synchronize()
+    kept = int(po[P])
+    assert list(po) == ref_po
+    assert torch.equal(off, ref_off)
+    assert torch.equal(csr[:kept], dk[idx[:kept]])
+    bins = hg.table_bins(dk, M).to(torch.int64) & 0xFFFFFFFF
+    inr = (bins >= splits[0]) & (bins < splits[-1])
+    assert torch.equal(torch.sort(idx[:kept])[0], torch.nonzero(inr).flatten())
+    goff = off.cpu().numpy()
+    gcsr, ecsr = csr.cpu().numpy().view(keys.dtype), ref_csr.cpu().numpy().view(keys.dtype)
+    for p in range(P):
+        loc = goff[splits[p] + p: splits[p + 1] + p + 1]
+        a = gcsr[ref_po[p]: ref_po[p + 1]].copy()
+        b = ecsr[ref_po[p]: ref_po[p + 1]].copy()
+        o.canon(loc, a)
+        o.canon(loc, b)
+        assert np.array_equal(a, b), f"partition {p} bins differ"
