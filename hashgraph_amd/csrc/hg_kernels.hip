@@ -31,9 +31,10 @@ constexpr uint64_t JOIN_PAIR_LIMIT = 4096;
 // bins per k_join workgroup: with their offsets and keys (78 KB of LDS) two
 // 1024-thread workgroups fit a CU
 constexpr int JOIN_BINS = 2048;
-// matches a row-id join workgroup buffers in LDS (a pair of 16-bit LDS slots
-// each) to write them out contiguously; a range with more re-walks its bins
-constexpr uint32_t JOIN_EMIT_BUF = 1024;
+// matches an emitting k_join workgroup buffers in LDS to write them out
+// contiguously (a range with more re-walks its bins): the row-id join keeps a
+// pair of 16-bit LDS slots per match, the key join B's slot only
+constexpr uint32_t JOIN_EMIT_PAIRS = 1024, JOIN_EMIT_KEYS = 3072;
 // what a k_join instance does with its matches: count them, write the matched
 // keys, or write the pairs of input positions (row-id join)
 constexpr int JOIN_COUNT = 0, JOIN_KEYS = 1, JOIN_PAIRS = 2;
@@ -790,10 +791,12 @@ __global__ __launch_bounds__(BLOCK) void k_join(const OFF *__restrict__ offA, co
   // and an emitting join buffers its matches (s_m), so that the output is
   // written from LDS in one contiguous run per workgroup
   constexpr bool emitting = MODE != JOIN_COUNT, pairs = MODE == JOIN_PAIRS;
+  constexpr uint32_t EBUF = pairs ? JOIN_EMIT_PAIRS : emitting ? JOIN_EMIT_KEYS : 0u;
   const uint32_t icap = pairs ? key_cap : 0u;
   uint64_t *s_ia = (uint64_t *)smem, *s_ib = s_ia + icap;
-  uint32_t *s_m = (uint32_t *)(s_ib + icap);
-  K *s_a = (K *)(s_m + (emitting ? JOIN_EMIT_BUF : 0u));
+  uint32_t *s_m = (uint32_t *)(s_ib + icap);  // pairs: (A slot << 16) | B slot
+  uint16_t *s_m16 = (uint16_t *)s_m;          // keys: B slot
+  K *s_a = (K *)((uint8_t *)s_m + EBUF * (pairs ? 4u : 2u));
   K *s_b = s_a + key_cap;
   __shared__ uint32_t s_mn;
   const uint64_t b0 = (uint64_t)blockIdx.x * JB;
@@ -873,7 +876,12 @@ __global__ __launch_bounds__(BLOCK) void k_join(const OFF *__restrict__ offA, co
           if (s_b[ib] == a) {
             mine++;
             const uint32_t slot = atomicAdd(&s_mn, 1u);
-            if (slot < JOIN_EMIT_BUF) s_m[slot] = (ia << 16) | ib;
+            if (slot < EBUF) {
+              if (pairs)
+                s_m[slot] = (ia << 16) | ib;
+              else
+                s_m16[slot] = (uint16_t)ib;
+            }
           }
       }
       continue;
@@ -886,18 +894,18 @@ __global__ __launch_bounds__(BLOCK) void k_join(const OFF *__restrict__ offA, co
   if (buffered) {
     __syncthreads();
     const uint32_t nm = s_mn;
-    if (nm <= JOIN_EMIT_BUF) {  // one reservation, then contiguous stores
+    if (nm <= EBUF) {  // one reservation, then contiguous stores
       if (threadIdx.x == 0) s_base = nm ? atomicAdd(cursor, (unsigned long long)nm) : 0ull;
       __syncthreads();
       for (uint32_t q = threadIdx.x; q < nm; q += BLOCK) {
         const unsigned long long w = s_base + q;
         if (w < capacity) {
-          const uint32_t v = s_m[q];
           if (pairs) {
+            const uint32_t v = s_m[q];
             pa[w] = s_ia[v >> 16];
             pb[w] = s_ib[v & 0xFFFFu];
           } else {
-            out[w] = s_b[v & 0xFFFFu];
+            out[w] = s_b[s_m16[q]];
           }
         }
       }
@@ -3084,8 +3092,8 @@ template <typename K, typename OFF>
 static hipError_t launch_join_t(const LaunchJoin &j, hipStream_t st) {
   constexpr int JB = JOIN_BINS;
   const uint64_t nblk = (j.M + JB - 1) / JB;
-  const size_t lds =
-      2 * (size_t)j.key_cap * (sizeof(K) + (j.pa ? 8 : 0)) + ((j.out || j.pa) ? JOIN_EMIT_BUF * 4 : 0);
+  const size_t lds = 2 * (size_t)j.key_cap * (sizeof(K) + (j.pa ? 8 : 0)) +
+                     (j.pa ? JOIN_EMIT_PAIRS * 4 : j.out ? JOIN_EMIT_KEYS * 2 : 0);
   auto kern = j.pa ? &k_join<K, OFF, 1024, JB, JOIN_PAIRS>
                   : j.out ? &k_join<K, OFF, 1024, JB, JOIN_KEYS> : &k_join<K, OFF, 1024, JB, JOIN_COUNT>;
   hipLaunchKernelGGL(kern, dim3((uint32_t)nblk), dim3(1024), lds, st, (const OFF *)j.offA, (const K *)j.keysA,
@@ -3111,8 +3119,8 @@ uint32_t join_key_cap(int key_bytes, int off_bytes, bool emit, bool with_index) 
   // two key arrays (for the row-id join with their 8-byte positions; for an
   // emitting join beside the match buffer) next to the 2*(JB+1) offsets held
   // in static LDS; the cap stays below 2^16 (buffered slots are 16-bit)
-  const size_t budget =
-      78 * 1024 - 2 * (JOIN_BINS + 1) * (size_t)off_bytes - ((emit || with_index) ? JOIN_EMIT_BUF * 4 : 0);
+  const size_t budget = 78 * 1024 - 2 * (JOIN_BINS + 1) * (size_t)off_bytes -
+                        (with_index ? JOIN_EMIT_PAIRS * 4 : emit ? JOIN_EMIT_KEYS * 2 : 0);
   return (uint32_t)(budget / (2 * ((size_t)key_bytes + (with_index ? 8 : 0))));
 }
 

@@ -73,12 +73,15 @@ def test_probe_shapes(nA, nB, M, kb, ob, dev):
 
 
 @pytest.mark.parametrize("kb", [4, 8])
-def test_self_join_overflows_the_match_buffer(kb, dev):
+@pytest.mark.parametrize("dup", [1, 2])
+def test_self_join_match_buffer(kb, dup, dev):
     # A joined with itself over one key a bin: ~2048 matches in every 2048-bin
-    # workgroup range, above the 1024 a workgroup buffers in LDS, so every
-    # range takes the counting re-walk
-    A = wl.uniform_keys(13, 1 << 18, kb)
-    _check_join(A, A.copy(), 1 << 18, dev)
+    # workgroup range, within the 3072 an emitting workgroup buffers in LDS;
+    # with every value twice on each side, ~4096, so every range takes the
+    # counting re-walk
+    vals = wl.uniform_keys(13, (1 << 18) // dup, kb)
+    A = np.repeat(vals, dup)
+    _check_join(A, A[::-1].copy(), 1 << 18, dev)
 
 
 def test_duplicates_on_both_sides(dev):
