@@ -1895,6 +1895,13 @@ hg_status build_partitioned(hg_ctx c, const hg_table_desc *desc, const void *d_k
   const uint64_t M = desc->n_bins, n = desc->n_keys;
   const int kb = (int)desc->key_bytes, ob = (int)desc->offset_bytes;
   const BinMap map = make_map(M, desc->hash_kind, desc->d_rmi_model);
+  if (n_parts == 1 && (uint64_t)h_splits[1] == M) {
+    // one partition over every bin keeps every key: the plain build, whose
+    // offsets are the partition's
+    h_part_offsets[0] = 0;
+    h_part_offsets[1] = n;
+    return build_range(c, map, kb, ob, n, 0, M, d_keys, d_offsets, d_csr_keys, d_csr_index, pack_in);
+  }
   Plan P0;
   const size_t i_part1 = P0.add(std::max<uint64_t>(n, 1) * kb), i_part2 = P0.add(std::max<uint64_t>(n, 1) * kb);
   HG_TRY(arena_ensure(c, 0, P0.total()));

@@ -202,13 +202,14 @@ def test_build_indexed_carries_positions(kb, dev):
 
 
 @pytest.mark.parametrize("kb", [4, 8])
-@pytest.mark.parametrize("cut", [False, True])
-def test_build_partitioned_indexed_carries_positions(kb, cut, dev):
+@pytest.mark.parametrize("shape", ["three", "cut", "one"])
+def test_build_partitioned_indexed_carries_positions(kb, shape, dev):
     """hg_build_partitioned_indexed: offsets and partition offsets equal
     hg_build_partitioned's, every bin holds the same keys, every CSR key equals
     the input key at its position, and the positions are exactly those of the
     keys whose bins lie in the splits' range (all keys, or, with cut splits,
-    the keys of the first seven eighths of the bins)."""
+    the keys of the first seven eighths of the bins; one partition over every
+    bin is built as a plain table)."""
     import ctypes
     import torch
     import hashgraph_amd as hg
@@ -216,7 +217,7 @@ def test_build_partitioned_indexed_carries_positions(kb, cut, dev):
     keys = wl.uniform_keys(73, n, kb)
     keys[:5000] = keys[5000:10000]  # duplicates
     dk = _to_dev(keys, dev)
-    splits = [0, M // 2, 7 * M // 8] if cut else [0, M // 3, M // 2, M]
+    splits = {"three": [0, M // 3, M // 2, M], "cut": [0, M // 2, 7 * M // 8], "one": [0, M]}[shape]
     P = len(splits) - 1
     print(f"partitioned indexed kb={kb} splits={splits}", flush=True)
     ref_off, ref_csr, ref_po = hg.build_partitioned(dk, M, splits)

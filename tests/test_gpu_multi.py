@@ -63,6 +63,13 @@ def test_partitioned_u32_few_parts(dev):
     _check_partitioned(keys, 250000, 3, dev)
 
 
+@pytest.mark.parametrize("kb", [4, 8])
+def test_partitioned_single_partition(kb, dev):
+    # one partition over every bin keeps every key: built as a plain table
+    keys = wl.uniform_keys(23, 300000, kb)
+    _check_partitioned(keys, 250000, 1, dev)
+
+
 def _rle_expected(A, B, M):
     oa, ka = o.build_csr(A, M)
     ob, kb = o.build_csr(B, M)
