@@ -771,9 +771,11 @@ __global__ __launch_bounds__(BLOCK) void k_probe(const K *__restrict__ bpart, co
 // ---------------------------------------------------------------------------
 // Join of two built tables over a range of bins (MultiHashGraph::intersect,
 // kernel simpleIntersect include/MultiHashGraphDeviceOperators.cuh:513-543):
-// every (a in A_b, x in B_b) with a == x counts and emits x.  One workgroup
-// per JB bins; A's and B's keys of the range are staged in LDS when they fit.
-// A bin pair above JOIN_PAIR_LIMIT key pairs is listed for k_join_heavy.
+// every (a in A_b, x in B_b) with a == x counts and emits x (MODE: count,
+// keys, or the pair of the entries' input positions for the row-id join).
+// One workgroup per JB bins; A's and B's keys of the range are staged in LDS
+// when they fit.  A bin pair above JOIN_PAIR_LIMIT key pairs is listed for
+// k_join_heavy (keys) or the tiled pair compare (row-id join).
 // ---------------------------------------------------------------------------
 template <typename K, typename OFF, int BLOCK, int JB, int MODE>
 __global__ __launch_bounds__(BLOCK) void k_join(const OFF *__restrict__ offA, const K *__restrict__ keysA,
