@@ -102,6 +102,35 @@ def test_short_capacity_and_compare_limit(dev):
     assert e.value.status == 6
 
 
+def test_short_capacity_with_tiled_bins_reports_exact_count(dev):
+    # bins above 4096 key pairs go through the tiled compare; a buffer one pair
+    # short still gets the exact count with HG_ERR_CAPACITY, and a buffer of
+    # the exact count the full pair set
+    import hashgraph_amd as hg
+    A = (wl.uniform_keys(15, 1 << 15, 4) % np.uint32(1 << 10)).astype(np.uint32)
+    B = (wl.uniform_keys(16, 1 << 15, 4) % np.uint32(1 << 10)).astype(np.uint32)
+    M = 16
+    oa, ka, ia = hg.build_indexed(_to_dev(A, dev), M)
+    ob, kb, ib = hg.build_indexed(_to_dev(B, dev), M)
+    exp = _expected_pairs(A, B)
+    import ctypes
+    import torch
+    c = hg.context(0)
+    dA, dB = hg._desc(ka.numel(), M, 4, 4), hg._desc(kb.numel(), M, 4, 4)
+    short = len(exp) - 1
+    pa = torch.empty(short, dtype=torch.int64, device=dev)
+    pb = torch.empty(short, dtype=torch.int64, device=dev)
+    got_cnt = ctypes.c_uint64(0)
+    st = c.lib.hg_intersect_indexed(c.bind_stream(), ctypes.byref(dA), hg._p(oa), hg._p(ka), hg._p(ia), ctypes.byref(dB),
+                                    hg._p(ob), hg._p(kb), hg._p(ib), hg._p(pa), hg._p(pb), short,
+                                    ctypes.byref(got_cnt))
+    assert st == 4 and got_cnt.value == len(exp)
+    cnt, pa, pb = hg.intersect_indexed(oa, ka, ia, ob, kb, ib, M, capacity=len(exp))
+    assert cnt == len(exp)
+    got = np.sort((pa.cpu().numpy().astype(np.uint64) << np.uint64(32)) | pb.cpu().numpy().astype(np.uint64))
+    assert np.array_equal(got, exp)
+
+
 @pytest.mark.parametrize("kb,n", [(4, 1 << 28), (8, 1 << 26)])
 def test_large_shape_properties(kb, n, dev):
     """The row-id join at its benchmark shape (2^28 + 2^28 uint32 keys, 2^28
