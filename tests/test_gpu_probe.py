@@ -285,3 +285,25 @@ def test_two_level_probe_2e28_u64_properties(dev):
     uo, co = torch.unique_consecutive(torch.sort(out)[0], return_counts=True)
     assert torch.equal(uo, ub[hit])
     assert torch.equal(co.to(torch.int64), want)
+
+
+def test_join_2e28_sequential_every_key_matches(dev):
+    """The key join at the reference experiment's shape (A = B = 0 .. 2^28-1
+    as uint32, 2^28 bins): every key matches once, so every 2048-bin range
+    emits about 2048 keys from its LDS match buffer.  Closed form: the count
+    is 2^28 and the sorted matched keys are 0 .. 2^28-1."""
+    import torch
+    import hashgraph_amd as hg
+    n = M = 1 << 28
+    print(f"sequential join n={n} M={M}", flush=True)
+    A = wl.sequential_keys(n, 4, device=dev)
+    offA, csrA = hg.build(A, M)
+    offB, csrB = hg.build(A.clone(), M)
+    cnt, _ = hg.intersect(offA, csrA, offB, csrB, M, emit=False)
+    assert cnt == n
+    cnt, out = hg.intersect(offA, csrA, offB, csrB, M)
+    assert cnt == n
+    del offA, csrA, offB, csrB
+    assert torch.equal(torch.sort(out)[0], torch.arange(n, dtype=torch.int32, device=dev))
+    del out, A
+    torch.cuda.empty_cache()
