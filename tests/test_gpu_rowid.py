@@ -78,6 +78,16 @@ def test_u64_keys_mixed_bins_and_offsets_u64(dev):
     _check(A, B, 1 << 17, dev, ob=8)
 
 
+@pytest.mark.parametrize("nA,nB,M", [(0, 1000, 512), (1000, 0, 512), (0, 0, 1), (300, 200, 1)])
+def test_empty_and_single_bin_tables(nA, nB, M, dev):
+    # an empty side joins to nothing; one bin holding both sides is one
+    # nested loop (300 x 200 key pairs, below the tiled path)
+    A = (wl.uniform_keys(17, nA, 4) % np.uint32(50)).astype(np.uint32)
+    B = (wl.uniform_keys(18, nB, 4) % np.uint32(50)).astype(np.uint32)
+    cnt = _check(A, B, M, dev)
+    assert (cnt == 0) == (nA == 0 or nB == 0)
+
+
 def test_short_capacity_and_compare_limit(dev):
     import hashgraph_amd as hg
     A = wl.uniform_keys(9, 1 << 16, 4)
